@@ -1,0 +1,192 @@
+"""Write the golden fixtures under tests/golden/ from the GENUINE reference.
+
+Runs only where the reference sources exist (the build container): it loads
+oracle/_ref/libsplref.so (built by `make -C oracle ref` from the Splotch
+sources) and records, for seeded synthetic inputs, what the genuine code
+returns:
+  * exptable<float32>(-20) op / expm1 on a sweep of arguments
+    (splotch/splotchutils.h:174-215);
+  * the camera matrix and shifts (splotch/splotch_host.cc:83-104);
+  * particle_project + particle_colorize outputs (splotch_host.cc:58-287);
+  * particle_sort permutations, all sort types (splotch_host.cc:289-321);
+  * render_new raw images (splotch_host.cc:330-697) in both modes, several tiles;
+  * one full host_rendering() call (splotch_host.cc:711-834).
+Fixtures are inputs + outputs only (numpy .npz, no pickles).
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import pyoracle as O  # noqa: E402
+from splotch_amd import synth  # noqa: E402
+
+
+def field_dict(f):
+    d = dict(x=f.x, y=f.y, z=f.z, hsml=f.hsml, inten=f.inten, col=f.col)
+    if f.type is not None:
+        d["type"] = f.type
+    if f.c1 is not None:
+        d["c1"] = f.c1
+        d["c2"] = f.c2
+    return d
+
+
+def view_dict(v):
+    return dict(campos=np.array(v.campos), lookat=np.array(v.lookat), sky=np.array(v.sky),
+                centerpos=np.array(v.center), res=np.array([v.xres, v.yres]),
+                fov=np.array([v.fov]))
+
+
+def proj_dict(p, prefix="p_"):
+    return {prefix + k: v for k, v in p.fields().items()}
+
+
+def save(name, **arrays):
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **arrays)
+    return path
+
+
+def params_for(v, extra=None):
+    d = {"xres": v.xres, "yres": v.yres, "fov": v.fov}
+    if extra:
+        d.update(extra)
+    return d
+
+
+def main():
+    if not O.ref_available():
+        raise SystemExit("genuine reference not built: make -C oracle ref")
+    manifest = {}
+    pal = synth.default_palette()
+
+    # --- exptable sweep
+    rng = np.random.default_rng(5)
+    args = np.concatenate([
+        -np.linspace(0, 25, 20001, dtype=np.float32),
+        rng.uniform(-21, 0.5, 20000).astype(np.float32),
+        np.array([0.0, -0.0, 1e-9, 3.0, -20.0, -19.99999, -1e-3, -6.17e-3, -6.18e-3, -1e9, 1e9,
+                  np.nan], np.float32)])
+    save("exptable", arg=args, op=O.exp_op(args, False, True), expm1=O.exp_op(args, True, True),
+         taylorlimit=np.array([O.ref().ref_taylor_limit()], np.float32))
+    manifest["exptable"] = "exptable<float32>(-20) op/expm1 over %d args" % args.size
+
+    # --- cameras
+    views = [synth.default_view(800), synth.default_view(160, 128),
+             synth.orbit_view(0.3, 0.4, xres=512), synth.orbit_view(2.0, -1.1, xres=256, yres=320),
+             synth.View((10.0, -40.0, 70.0), (55.0, 45.0, 40.0), (0.2, 0.1, 1.0), 300, 200, 45.0,
+                        (12.0, -38.0, 69.0))]
+    cam = {}
+    for i, v in enumerate(views):
+        m, d, f = O.ref_camera(v)
+        for k, a in view_dict(v).items():
+            cam["v%d_%s" % (i, k)] = a
+        cam["v%d_mat" % i] = m
+        cam["v%d_shift" % i] = d
+        cam["v%d_f" % i] = f
+    save("camera", nviews=np.array([len(views)]), **cam)
+    manifest["camera"] = "camera matrix/shifts for %d views" % len(views)
+
+    # --- plain additive, one type, non-square image
+    f = synth.plummer_field(20000, 11)
+    v = synth.default_view(160, 128)
+    pr = O.ref_project_colorize(f, v, [pal], params_for(v))
+    out = dict(**field_dict(f), **view_dict(v), **proj_dict(pr))
+    for T in (50, 16, 8, 64):
+        out["pic_T%d" % T] = O.ref_render(pr, v.xres, v.yres, True, 0.2, T)
+    save("additive_small", **out)
+    manifest["additive_small"] = "20k particles, 160x128, a_eq_e, T in 50/16/8/64"
+
+    # --- ordered (a_eq_e=false), sort_type 1, genuine std::sort order
+    f = synth.plummer_field(12000, 12, hsml_mu=math.log(0.5))
+    v = synth.default_view(96, 112)
+    pr = O.ref_project_colorize(f, v, [pal], params_for(v, {"brightness0": 0.5}))
+    perm = O.ref_sort_perm(pr, 1)
+    ps = pr.take(perm)
+    out = dict(**field_dict(f), **view_dict(v), **proj_dict(pr), perm_std=perm)
+    out["brightness"] = np.array([0.5], np.float32)
+    for T in (50, 16):
+        out["pic_T%d" % T] = O.ref_render(ps, v.xres, v.yres, False, 0.2, T)
+    # also the render of the stable order (the canonical order) by the genuine render_new
+    perm_st = O.sort_perm(pr, 1, stable=True)
+    out["perm_stable"] = perm_st
+    out["pic_stable_T50"] = O.ref_render(pr.take(perm_st), v.xres, v.yres, False, 0.2, 50)
+    out["pic_stable_T50_g05"] = O.ref_render(pr.take(perm_st), v.xres, v.yres, False, 0.5, 50)
+    save("ordered_small", **out)
+    manifest["ordered_small"] = "12k particles, 96x112, ordered, std::sort + stable orders"
+
+    # --- sort types 2,3,4 permutations (ties included: quantised keys)
+    q = synth.plummer_field(5000, 13)
+    pq = O.ref_project_colorize(q, synth.default_view(64), [pal], params_for(synth.default_view(64)))
+    pq.z = np.round(pq.z * 4) / 4  # force ties
+    pq.z = pq.z.astype(np.float32)
+    srt = {"z": pq.z, "e0": pq.e0, "r": pq.r}
+    for st in (1, 2, 3, 4):
+        srt["perm%d" % st] = O.ref_sort_perm(pq, st)
+    save("sort_types", **srt)
+    manifest["sort_types"] = "std::sort permutations, sort_type 1..4, with key ties"
+
+    # --- two types: different smooth / brightness, type 1 vector colour, edges
+    f0 = synth.plummer_field(6000, 14, ptype=0)
+    f1 = synth.plummer_field(4000, 15, hsml_fixed=0.05, ptype=1)
+    f = synth.concat(f0, f1)
+    rng = np.random.default_rng(16)
+    f.c1 = rng.uniform(0, 1, f.n).astype(np.float32)
+    f.c2 = rng.uniform(0, 1, f.n).astype(np.float32)
+    # a few huge / off-screen / behind-camera particles
+    f.hsml[:5] = np.array([40.0, 15.0, 3.0, 8.0, 25.0], np.float32)
+    f.z[5:10] = np.array([-150.0, -160.0, -149.0, -500.0, 1000.0], np.float32)
+    f.x[10:15] = np.array([-500.0, 600.0, 50.0, 50.0, -30.0], np.float32)
+    v = synth.default_view(200, 150)
+    extra = {"ptypes": 2, "smooth_factor0": 1.3, "smooth_factor1": 0.8, "brightness0": 0.7,
+             "brightness1": 2.0, "color_is_vector1": True, "minrad_pix": 1.5}
+    pal1 = (np.array([0.0, 0.5, 1.0]), np.array([[0, 0, .2], [0, .5, 1], [1, 1, 1]], np.float32))
+    pr = O.ref_project_colorize(f, v, [pal, pal1], params_for(v, extra), b_brightness=1.25)
+    out = dict(**field_dict(f), **view_dict(v), **proj_dict(pr))
+    out["pal1_x"], out["pal1_rgb"] = pal1
+    out["b_brightness"] = np.array([1.25], np.float32)
+    out["pic_T50"] = O.ref_render(pr, v.xres, v.yres, True, 0.2, 50)
+    out["pic_T24"] = O.ref_render(pr, v.xres, v.yres, True, 0.2, 24)
+    save("two_types", **out)
+    manifest["two_types"] = "2 types, vector colour, smooth/brightness/minrad, edge particles"
+
+    # --- orthographic projection, zmin/zmax culls
+    f = synth.plummer_field(8000, 17)
+    v = synth.default_view(128, 128)
+    extra = {"projection": False, "zmin": 120.0, "zmax": 260.0}
+    pr = O.ref_project_colorize(f, v, [pal], params_for(v, extra))
+    out = dict(**field_dict(f), **view_dict(v), **proj_dict(pr))
+    out["pic_T50"] = O.ref_render(pr, v.xres, v.yres, True, 0.2, 50)
+    save("ortho_zclip", **out)
+    manifest["ortho_zclip"] = "projection=false, zmin=120, zmax=260"
+
+    # --- full host_rendering() path, ordered mode, tile_size_cpu=32
+    f = synth.plummer_field(10000, 18, hsml_mu=math.log(0.4))
+    v = synth.orbit_view(1.0, 0.5, xres=120, yres=100)
+    params = params_for(v, {"a_eq_e": False, "tile_size_cpu": 32, "gray_absorption": 0.3,
+                            "sort_type": 1})
+    pic = O.ref_host_rendering(f, v, [pal], params)
+    save("host_rendering", **field_dict(f), **view_dict(v), pic=pic)
+    manifest["host_rendering"] = "host_rendering(): 10k, 120x100, ordered, T=32, g=0.3"
+
+    # palette used by every fixture (this project's own ramp)
+    np.savez_compressed(os.path.join(HERE, "palette.npz"), x=pal[0], rgb=pal[1])
+    with open(os.path.join(HERE, "MANIFEST.json"), "w") as fh:
+        json.dump(manifest, fh, indent=1, sort_keys=True)
+    print("wrote", len(manifest), "fixtures")
+
+
+if __name__ == "__main__":
+    main()
