@@ -1,0 +1,169 @@
+/*
+ * splotch_hip.h — C-ABI of the MI355X (gfx950) Splotch particle-splat renderer.
+ *
+ * Drop-in boundary for the render dispatch of Splotch (splotch/splotch.cc:248-264,
+ * server/server.cc:158-171): it replaces the CUDA backend entry points
+ *   cuda_rendercontext_init   cuda/cuda_splotch.h:33   (cuda/cuda_splotch.cc:26-64)
+ *   cuda_rendering            cuda/cuda_splotch.h:35   (cuda/cuda_splotch.cc:121-192)
+ * and computes what the host path host_rendering() computes
+ *   splotch/splotch_host.h:43 (splotch/splotch_host.cc:711-834).
+ *
+ * Conventions
+ *  - Plain pointers and sizes only; every function returns 0 on success and a
+ *    negative SPLHIP_E* code on error (message via splhip_last_error).
+ *  - Device pointers unless stated otherwise ("host" in the name / comment).
+ *  - Calls are ordered on the given hipStream_t (passed as void*; NULL = the
+ *    null stream).  A context is used by one host thread at a time.
+ *  - Images are x-major RGB float32: pic[(x*yres + y)*3 + c], the layout of
+ *    arr2<COLOUR> pic (cxxsupport/arr.h:474-488) that Splotch composites and
+ *    writes.  The raw image is what render_new leaves in pic
+ *    (splotch_host.cc:675-691); the tone map of splotch.cc:275-284 is applied
+ *    by the caller, as in the reference.
+ */
+#ifndef SPLOTCH_HIP_H
+#define SPLOTCH_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPLHIP_MAX_TYPES 8
+
+#define SPLHIP_OK 0
+#define SPLHIP_E_ARG (-1)       /* invalid argument */
+#define SPLHIP_E_HIP (-2)       /* HIP runtime error */
+#define SPLHIP_E_DEGENERATE (-3) /* degenerate camera matrix (kernel/transform.cxx:84) */
+#define SPLHIP_E_CAPACITY (-4)  /* a size exceeds what the library supports */
+#define SPLHIP_E_STATE (-5)     /* call out of order (e.g. backward before forward) */
+
+typedef struct splhip_ctx splhip_ctx;
+
+/* Per-frame camera: what splotch_host.cc:62-109 derives from the .par keys and
+   the scene's campos/lookat/sky/centerpos.  Filled by splhip_camera_setup. */
+typedef struct {
+  float mat[12];        /* TRANSFORM::Matrix().p (kernel/transform.h:36-40) */
+  double xshift, yshift;
+  float res2, ycorr, xfac, fovfct, dist;
+  float zminval, zmaxval, minrad_pix;
+  int32_t xres, yres, projection;
+} splhip_camera;
+
+/* Render parameters (the .par keys of splotch_host.cc:76-81,256-265,806-825). */
+typedef struct {
+  int32_t ptypes;                          /* ptypes (1) */
+  float smooth_fac[SPLHIP_MAX_TYPES];      /* smooth_factor{t} (1) */
+  float brightness[SPLHIP_MAX_TYPES];      /* brightness{t} * b_brightness (1) */
+  int32_t col_vector[SPLHIP_MAX_TYPES];    /* color_is_vector{t} (0) */
+  int32_t a_eq_e[SPLHIP_MAX_TYPES];        /* a_eq_e (1); per type (see splhip_render) */
+  int32_t sort_type;                       /* sort_type (1) */
+  float gray_absorption;                   /* gray_absorption (0.2) */
+  int32_t tile_size;                       /* tile_size_cpu (50): reference tile */
+  int32_t hip_region;                      /* hip_region: GPU work region, px (0 = auto) */
+  int32_t hip_batch;                       /* hip_batch: particles staged per batch (0 = auto) */
+} splhip_params;
+
+/* SoA particle fields (device pointers), post particle_normalize:
+   x,y,z world position; hsml smoothing length (particle_sim::r); inten (I);
+   col = colour scalar (e.r) or, for color_is_vector types, red with c1/c2 the
+   green/blue fields (may be NULL when no type is vector-coloured);
+   type: particle type per particle (NULL = all type 0). */
+typedef struct {
+  const float *x, *y, *z, *hsml, *inten, *col, *c1, *c2;
+  const uint8_t *type;
+  int64_t n;
+} splhip_particles;
+
+/* Reference particle_sim record (splotch/splotchutils.h:52-72), 36 bytes. */
+typedef struct {
+  float er, eg, eb, x, y, z, r, I;
+  uint16_t type;
+  uint8_t active;
+  uint8_t pad_;
+} splhip_particle_sim;
+
+/* --- context ------------------------------------------------------------- */
+
+/* Replaces cuda_rendercontext_init + cuda_renderer_init
+   (cuda/cuda_splotch.cc:26-118): binds a device and builds the exp tables
+   (splotch/splotchutils.h:174-215) on the host. */
+int splhip_create(int device, splhip_ctx **out);
+void splhip_destroy(splhip_ctx *ctx);
+const char *splhip_last_error(const splhip_ctx *ctx);
+
+/* --- host-side setup ------------------------------------------------------ */
+
+/* Camera setup of splotch_host.cc:62-109 (kernel/transform.cxx:28-219);
+   replaces cu_get_trans_params (cuda/cuda_utils.cu:47-118).  Host only. */
+int splhip_camera_setup(const double campos[3], const double lookat[3], const double sky[3],
+                        const double centerpos[3], int32_t xres, int32_t yres, float fov,
+                        float zmin, float zmax, int32_t projection, float minrad_pix,
+                        splhip_camera *out);
+
+/* Defaults of the .par keys (the find<T>(key, default) values). Host only. */
+int splhip_params_default(splhip_params *p);
+
+/* Reads the hot-path .par keys from `par_text` ("key=value" lines, '#'
+   comments; the paramfile format, cxxsupport/paramfile.cc) into p, cam_*
+   (camera keys) and b_brightness-free brightness.  Keys and defaults as read
+   by splotch_host.cc:62-109,256-265,806-825.  Host only.
+   scalars_out = {xres, yres, fov, zmin, zmax, projection, minrad_pix} (float). */
+int splhip_params_from_par(const char *par_text, float b_brightness, splhip_params *p,
+                           float scalars_out[7]);
+
+/* Palette for one type: n nodes x[i] (host double, ascending) and RGB (host
+   float, 3n).  Replaces setup_colormap / cu_init_colormap
+   (cuda/cuda_splotch.cc:195-234, cuda/cuda_utils.cu:259-271). */
+int splhip_set_palette(splhip_ctx *ctx, int32_t type, int32_t n, const double *x,
+                       const float *rgb);
+
+/* --- plain render --------------------------------------------------------- */
+
+/* Full hot path on device-resident SoA particles: project + colorize
+   (splotch_host.cc:58-287), depth sort for ordered types (:289-321),
+   binning and splat (render_new, :330-697).  Writes the raw image `pic`
+   (device, xres*yres*3 floats).  Particles are not modified.
+   Types with a_eq_e[t]=1 are splatted additively; types with a_eq_e[t]=0 are
+   composited in sort order.  When both kinds are present the image is
+   pic = P_ordered + P_additive, each rendered from zero as render_new does
+   for its mode (the reference's a_eq_e is one global flag). */
+int splhip_render(splhip_ctx *ctx, const splhip_particles *parts, const splhip_camera *cam,
+                  const splhip_params *prm, float *pic, void *stream);
+
+/* cuda_rendering semantics (cuda/cuda_splotch.cc:121-192): host AoS particle
+   records in, host image out; the host records are not modified. */
+int splhip_render_host(splhip_ctx *ctx, const splhip_particle_sim *host_particles, int64_t n,
+                       const splhip_camera *cam, const splhip_params *prm, float *host_pic,
+                       void *stream);
+
+/* --- parity hooks ----------------------------------------------------------- */
+
+/* particle_project + particle_colorize only.  rec_out: device, n*8 floats per
+   particle {x, y, r, e.r, e.g, e.b, z, I} (state after the reference loops);
+   active_out: device n bytes, 0 inactive, 1 active. */
+int splhip_project(splhip_ctx *ctx, const splhip_particles *parts, const splhip_camera *cam,
+                   const splhip_params *prm, float *rec_out, uint8_t *active_out,
+                   void *stream);
+
+/* Tile index lists of render_new's binning (splotch_host.cc:337-406) for the
+   particles of one mode (ordered=1: a_eq_e=0 types, in sort order; 0: a_eq_e=1
+   types, ascending index), at tile size prm->tile_size.  Tile id = iy*ncx+ix.
+   Host outputs: tile_off[ntiles+1] (int64) and idx[] = original particle
+   indices; *n_entries returns the total (call with idx=NULL to size).
+   order_out (host, optional): the render order (original indices). */
+int splhip_bins(splhip_ctx *ctx, const splhip_particles *parts, const splhip_camera *cam,
+                const splhip_params *prm, int32_t ordered, int64_t *tile_off,
+                uint32_t *idx, int64_t idx_cap, int64_t *n_entries, uint32_t *order_out,
+                int64_t *n_order, void *stream);
+
+/* Host-built tables (device copies are used by the kernels): tab[0..1023] =
+   tab1, tab[1024..2047] = tab2; consts = {expfac, taylorlimit, h2sigma, bfak,
+   rfac}.  Host only. */
+int splhip_exptable(float tab[2048], float consts[5]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPLOTCH_HIP_H */

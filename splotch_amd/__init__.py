@@ -1,0 +1,15 @@
+"""splotch_amd — MI355X (gfx950) particle-splat renderer with Splotch's render interface.
+
+The hot path (camera transform + perspective projection + palette colouring,
+depth sort, tile binning and Gaussian splat; splotch/splotch_host.cc:58-697)
+runs as hand-written HIP kernels in libsplotch_hip.so behind the C-ABI of
+include/splotch_hip.h.  This package holds the sources (csrc/), the in-tree
+build (build.py), the ctypes binding (capi.py) and the seeded synthetic
+workloads (synth.py).
+"""
+from . import synth  # noqa: F401
+from .capi import (Camera, Params, Renderer, SplhipError, camera_setup, exptable,  # noqa: F401
+                   params, params_from_par, to_device, to_particle_sim)
+
+__all__ = ["Camera", "Params", "Renderer", "SplhipError", "camera_setup", "exptable", "params",
+           "params_from_par", "synth", "to_device", "to_particle_sim"]

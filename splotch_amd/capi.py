@@ -1,0 +1,287 @@
+"""ctypes binding of the C-ABI in include/splotch_hip.h (libsplotch_hip.so).
+
+The Python layer mirrors the reference's render interface for this path
+(host_rendering / cuda_rendering, splotch/splotch_host.cc:711-834 and
+cuda/cuda_splotch.cc:121-192): a context bound to one device, palettes per
+particle type, a camera derived from campos/lookat/sky/centerpos and the .par
+keys, and a render call that writes the raw x-major RGB image.  Device memory
+and streams come from torch; the work is done by the HIP kernels.  If the
+library is missing, every entry point raises: there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsplotch_hip.so")
+MAX_TYPES = 8
+
+_lib = None
+
+
+class SplhipError(RuntimeError):
+    pass
+
+
+class Camera(C.Structure):
+    _fields_ = [("mat", C.c_float * 12), ("xshift", C.c_double), ("yshift", C.c_double),
+                ("res2", C.c_float), ("ycorr", C.c_float), ("xfac", C.c_float),
+                ("fovfct", C.c_float), ("dist", C.c_float), ("zminval", C.c_float),
+                ("zmaxval", C.c_float), ("minrad_pix", C.c_float), ("xres", C.c_int32),
+                ("yres", C.c_int32), ("projection", C.c_int32)]
+
+
+class Params(C.Structure):
+    _fields_ = [("ptypes", C.c_int32), ("smooth_fac", C.c_float * MAX_TYPES),
+                ("brightness", C.c_float * MAX_TYPES), ("col_vector", C.c_int32 * MAX_TYPES),
+                ("a_eq_e", C.c_int32 * MAX_TYPES), ("sort_type", C.c_int32),
+                ("gray_absorption", C.c_float), ("tile_size", C.c_int32),
+                ("hip_region", C.c_int32), ("hip_batch", C.c_int32)]
+
+
+class Particles(C.Structure):
+    _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("hsml", C.c_void_p),
+                ("inten", C.c_void_p), ("col", C.c_void_p), ("c1", C.c_void_p),
+                ("c2", C.c_void_p), ("type", C.c_void_p), ("n", C.c_int64)]
+
+
+# numpy dtype of the reference particle_sim record (splotch/splotchutils.h:52-72)
+PARTICLE_SIM = np.dtype([("er", "<f4"), ("eg", "<f4"), ("eb", "<f4"), ("x", "<f4"),
+                         ("y", "<f4"), ("z", "<f4"), ("r", "<f4"), ("I", "<f4"),
+                         ("type", "<u2"), ("active", "u1"), ("pad_", "u1")])
+
+EXPORTS = ["splhip_create", "splhip_destroy", "splhip_last_error", "splhip_camera_setup",
+           "splhip_params_default", "splhip_params_from_par", "splhip_set_palette",
+           "splhip_render", "splhip_render_host", "splhip_project", "splhip_bins",
+           "splhip_exptable"]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SplhipError("libsplotch_hip.so is not built (python -m splotch_amd.build)")
+        L = C.CDLL(LIB_PATH)
+        L.splhip_last_error.restype = C.c_char_p
+        L.splhip_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
+        L.splhip_destroy.argtypes = [C.c_void_p]
+        L.splhip_last_error.argtypes = [C.c_void_p]
+        L.splhip_set_palette.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
+                                         C.c_void_p]
+        L.splhip_render.argtypes = [C.c_void_p, C.POINTER(Particles), C.POINTER(Camera),
+                                    C.POINTER(Params), C.c_void_p, C.c_void_p]
+        L.splhip_render_host.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(Camera),
+                                         C.POINTER(Params), C.c_void_p, C.c_void_p]
+        L.splhip_project.argtypes = [C.c_void_p, C.POINTER(Particles), C.POINTER(Camera),
+                                     C.POINTER(Params), C.c_void_p, C.c_void_p, C.c_void_p]
+        L.splhip_bins.argtypes = [C.c_void_p, C.POINTER(Particles), C.POINTER(Camera),
+                                  C.POINTER(Params), C.c_int32, C.c_void_p, C.c_void_p,
+                                  C.c_int64, C.POINTER(C.c_int64), C.c_void_p,
+                                  C.POINTER(C.c_int64), C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _np(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _v3(t):
+    return np.ascontiguousarray(np.asarray(t, dtype=np.float64).reshape(3))
+
+
+def exptable():
+    tab = np.zeros(2048, np.float32)
+    k = np.zeros(5, np.float32)
+    rc = lib().splhip_exptable(_np(tab), _np(k))
+    if rc != 0:
+        raise SplhipError("splhip_exptable failed")
+    return tab, k
+
+
+def camera_setup(view, zmin=0.0, zmax=1e23, projection=True, minrad_pix=1.0) -> Camera:
+    """Camera of splotch_host.cc:62-109 for a synth.View-like object."""
+    cam = Camera()
+    rc = lib().splhip_camera_setup(_np(_v3(view.campos)), _np(_v3(view.lookat)),
+                                   _np(_v3(view.sky)), _np(_v3(view.center)),
+                                   C.c_int32(view.xres), C.c_int32(view.yres),
+                                   C.c_float(view.fov), C.c_float(zmin), C.c_float(zmax),
+                                   C.c_int32(int(projection)), C.c_float(minrad_pix),
+                                   C.byref(cam))
+    if rc == -3:
+        raise SplhipError("degenerate matrix in TRANSMAT::Inverse()")
+    if rc != 0:
+        raise SplhipError("splhip_camera_setup failed (%d)" % rc)
+    return cam
+
+
+def params(ptypes=1, smooth_fac=None, brightness=None, col_vector=None, a_eq_e=True,
+           sort_type=1, gray_absorption=0.2, tile_size=50, hip_region=0, hip_batch=0,
+           b_brightness=1.0) -> Params:
+    """Render parameters; brightness is brightness{t} (times b_brightness, as
+    splotch_host.cc:262-263 does, in float32)."""
+    p = Params()
+    lib().splhip_params_default(C.byref(p))
+    p.ptypes = ptypes
+    for t in range(MAX_TYPES):
+        if smooth_fac is not None and t < len(smooth_fac):
+            p.smooth_fac[t] = smooth_fac[t]
+        b = brightness[t] if brightness is not None and t < len(brightness) else 1.0
+        p.brightness[t] = float(np.float32(b) * np.float32(b_brightness))
+        if col_vector is not None and t < len(col_vector):
+            p.col_vector[t] = int(bool(col_vector[t]))
+        if isinstance(a_eq_e, (list, tuple)):
+            p.a_eq_e[t] = int(bool(a_eq_e[t])) if t < len(a_eq_e) else 1
+        else:
+            p.a_eq_e[t] = int(bool(a_eq_e))
+    p.sort_type = sort_type
+    p.gray_absorption = gray_absorption
+    p.tile_size = tile_size
+    p.hip_region = hip_region
+    p.hip_batch = hip_batch
+    return p
+
+
+def params_from_par(text: str, b_brightness: float = 1.0):
+    """Read the hot-path .par keys (see include/splotch_hip.h)."""
+    p = Params()
+    sc = np.zeros(7, np.float32)
+    rc = lib().splhip_params_from_par(text.encode(), C.c_float(b_brightness), C.byref(p),
+                                      _np(sc))
+    if rc != 0:
+        raise SplhipError("bad parameter text")
+    keys = ("xres", "yres", "fov", "zmin", "zmax", "projection", "minrad_pix")
+    scal = dict(zip(keys, sc.tolist()))
+    scal["xres"] = int(scal["xres"])
+    scal["yres"] = int(scal["yres"])
+    scal["projection"] = bool(scal["projection"])
+    return p, scal
+
+
+class Renderer:
+    """One context on one device (cuda_rendercontext_init + cuda_renderer_init)."""
+
+    def __init__(self, device: int = 0):
+        self._c = C.c_void_p()
+        rc = lib().splhip_create(C.c_int(device), C.byref(self._c))
+        if rc != 0:
+            raise SplhipError("splhip_create failed (%d)" % rc)
+        self.device = device
+
+    def close(self):
+        if self._c:
+            lib().splhip_destroy(self._c)
+            self._c = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = lib().splhip_last_error(self._c)
+            raise SplhipError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def set_palette(self, ptype: int, x, rgb):
+        x = np.ascontiguousarray(x, np.float64)
+        rgb = np.ascontiguousarray(rgb, np.float32).reshape(-1, 3)
+        self._check(lib().splhip_set_palette(self._c, ptype, len(x), _np(x), _np(rgb)),
+                    "set_palette")
+
+    @staticmethod
+    def particles(dev: dict) -> Particles:
+        """dev: dict of contiguous cuda tensors x,y,z,hsml,inten,col[,c1,c2,type]."""
+        P = Particles()
+        n = int(dev["x"].numel())
+        for k in ("x", "y", "z", "hsml", "inten", "col", "c1", "c2", "type"):
+            t = dev.get(k)
+            if t is not None:
+                assert t.is_contiguous() and t.numel() == n, k
+                setattr(P, k, t.data_ptr())
+        P.n = n
+        return P
+
+    @staticmethod
+    def _stream(stream):
+        if stream is None:
+            import torch
+            return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+        return C.c_void_p(stream)
+
+    def render(self, dev: dict, cam: Camera, prm: Params, pic, stream=None):
+        """pic: cuda float32 tensor of cam.xres*cam.yres*3 elements (x-major RGB)."""
+        assert pic.is_contiguous() and pic.numel() == cam.xres * cam.yres * 3
+        P = self.particles(dev)
+        self._check(lib().splhip_render(self._c, C.byref(P), C.byref(cam), C.byref(prm),
+                                        C.c_void_p(pic.data_ptr()), self._stream(stream)),
+                    "splhip_render")
+        return pic
+
+    def render_host(self, aos: np.ndarray, cam: Camera, prm: Params, stream=None):
+        """cuda_rendering semantics: host particle_sim records in, host image out."""
+        aos = np.ascontiguousarray(aos)
+        assert aos.dtype == PARTICLE_SIM
+        pic = np.zeros((cam.xres, cam.yres, 3), np.float32)
+        self._check(lib().splhip_render_host(self._c, _np(aos), C.c_int64(len(aos)),
+                                             C.byref(cam), C.byref(prm), _np(pic),
+                                             self._stream(stream)),
+                    "splhip_render_host")
+        return pic
+
+    def project(self, dev: dict, cam: Camera, prm: Params, stream=None):
+        import torch
+        P = self.particles(dev)
+        rec = torch.empty((max(P.n, 1), 8), dtype=torch.float32, device=dev["x"].device)
+        act = torch.empty((max(P.n, 1),), dtype=torch.uint8, device=dev["x"].device)
+        self._check(lib().splhip_project(self._c, C.byref(P), C.byref(cam), C.byref(prm),
+                                         C.c_void_p(rec.data_ptr()), C.c_void_p(act.data_ptr()),
+                                         self._stream(stream)), "splhip_project")
+        return rec[:P.n], act[:P.n]
+
+    def bins(self, dev: dict, cam: Camera, prm: Params, ordered: bool, stream=None):
+        """Reference-tile CSR (tile_off int64, idx = original indices) and render order."""
+        P = self.particles(dev)
+        T = prm.tile_size
+        ntiles = ((cam.xres + T - 1) // T) * ((cam.yres + T - 1) // T)
+        ne = C.c_int64()
+        no = C.c_int64()
+        self._check(lib().splhip_bins(self._c, C.byref(P), C.byref(cam), C.byref(prm),
+                                      int(ordered), None, None, 0, C.byref(ne), None,
+                                      C.byref(no), self._stream(stream)), "splhip_bins")
+        off = np.zeros(ntiles + 1, np.int64)
+        idx = np.zeros(max(ne.value, 1), np.uint32)
+        order = np.zeros(max(no.value, 1), np.uint32)
+        self._check(lib().splhip_bins(self._c, C.byref(P), C.byref(cam), C.byref(prm),
+                                      int(ordered), _np(off), _np(idx), C.c_int64(len(idx)),
+                                      C.byref(ne), _np(order), C.byref(no),
+                                      self._stream(stream)), "splhip_bins")
+        if no.value == 0:
+            off[:] = 0
+        return off, idx[:ne.value], order[:no.value]
+
+
+def to_device(field, device="cuda"):
+    """synth.Field -> dict of cuda tensors (SoA)."""
+    import torch
+    d = {}
+    for k, v in field.arrays().items():
+        if v is not None:
+            d[k] = torch.from_numpy(np.ascontiguousarray(v)).to(device)
+    return d
+
+
+def to_particle_sim(field) -> np.ndarray:
+    """synth.Field -> reference particle_sim records (e.r = colour scalar)."""
+    a = np.zeros(field.n, PARTICLE_SIM)
+    a["x"], a["y"], a["z"], a["r"], a["I"] = field.x, field.y, field.z, field.hsml, field.inten
+    a["er"] = field.col
+    if field.c1 is not None:
+        a["eg"], a["eb"] = field.c1, field.c2
+    if field.type is not None:
+        a["type"] = field.type
+    return a

@@ -1,0 +1,100 @@
+// Shared host/device definitions of the gfx950 Splotch renderer.
+//
+// Float semantics: every translation unit is compiled with -ffp-contract=off
+// and without fast-math, so each float/double operation below rounds exactly
+// as the reference's x86 code does (no fused multiply-adds, IEEE division and
+// square root).  Float->int conversions go through cvt_i32, which reproduces
+// x86 cvttss2si (NaN and out-of-range give INT_MIN); gfx950's v_cvt_i32_f32
+// saturates instead.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <climits>
+#include <cstdint>
+
+#include "../../include/splotch_hip.h"
+
+namespace splhip {
+
+constexpr int kMaxTypes = SPLHIP_MAX_TYPES;
+constexpr int kThreads = 256;  // 4 wave64 per workgroup
+
+// Projected particle in render order: 32 bytes, one dwordx4 pair per gather.
+struct alignas(16) Rec {
+  float x, y, r, e0, e1, e2, z, I;
+};
+
+// Constants of splotch_host.cc:44-49 and the exptable parameters of
+// splotchutils.h:174-215, all computed on the host.
+struct Consts {
+  float expfac, taylorlimit, h2sigma, bfak, rfac;
+};
+
+// int(f) with x86 cvttss2si semantics.
+__device__ __forceinline__ int cvt_i32(float f) {
+  return (f >= -2147483648.0f && f < 2147483648.0f) ? static_cast<int>(f) : INT_MIN;
+}
+
+// exptable<float32>::operator() (splotchutils.h:203-209); tab = tab1 | tab2.
+__device__ __forceinline__ float xexp_op(const float *tab, float arg, float expfac) {
+  const int iarg = cvt_i32(arg * expfac);
+  if (iarg & ~((1 << 20) - 1)) return (iarg < 0) ? 1.f : 0.f;
+  return tab[iarg >> 10] * tab[1024 + (iarg & 1023)];
+}
+
+// exptable<float32>::expm1 (splotchutils.h:210-214).
+__device__ __forceinline__ float xexp_expm1(const float *tab, float arg, float expfac,
+                                            float taylorlimit) {
+  if (fabsf(arg) < taylorlimit) return arg;
+  return xexp_op(tab, arg, expfac) - 1.f;
+}
+
+// Binning geometry: reference tiles of T px (tile id iy*ncx+ix) and, when
+// G > 0, work regions of G x G px aligned to each tile's origin
+// (region id = tile*ngm*ngm + gy*ngm + gx).
+struct BinGeom {
+  int T, xres, yres, ncx, ncy;
+  int G, ngm;
+  float rcell, cmid0, rfac;
+};
+
+// Visit the tiles (G == 0) or work regions (G > 0) a particle is binned to.
+// Tile rule: splotch_host.cc:367-383 (range + circle test, T = tile_size).
+// Region rule: the footprint box that render_new clips to the tile
+// (splotch_host.cc:462-472), split into G x G regions.
+template <class F>
+__device__ __forceinline__ void for_each_bin(const Rec &R, const BinGeom &g, F &&f) {
+  const float rfacr = g.rfac * R.r;
+  const int minx = max(0, cvt_i32(R.x - rfacr + 1) / g.T);
+  const int maxx = min(g.ncx - 1, cvt_i32(R.x + rfacr) / g.T);
+  const int miny = max(0, cvt_i32(R.y - rfacr + 1) / g.T);
+  const int maxy = min(g.ncy - 1, cvt_i32(R.y + rfacr) / g.T);
+  const float sumsq = (g.rcell + rfacr) * (g.rcell + rfacr);
+  for (int ix = minx; ix <= maxx; ++ix) {
+    const float cx = g.cmid0 + static_cast<float>(ix * g.T);
+    for (int iy = miny; iy <= maxy; ++iy) {
+      const float cy = g.cmid0 + static_cast<float>(iy * g.T);
+      const float rtot2 = (R.x - cx) * (R.x - cx) + (R.y - cy) * (R.y - cy);
+      if (!(rtot2 < sumsq)) continue;
+      const int tile = iy * g.ncx + ix;
+      if (g.G == 0) {
+        f(static_cast<uint32_t>(tile));
+        continue;
+      }
+      const int x0s = ix * g.T, y0s = iy * g.T;
+      const int w = min(x0s + g.T, g.xres) - x0s, h = min(y0s + g.T, g.yres) - y0s;
+      const float rf2 = R.r * g.rfac;
+      const float posx = R.x - static_cast<float>(x0s), posy = R.y - static_cast<float>(y0s);
+      const int bx0 = max(cvt_i32(posx - rf2 + 1), 0), bx1 = min(cvt_i32(posx + rf2 + 1), w);
+      const int by0 = max(cvt_i32(posy - rf2 + 1), 0), by1 = min(cvt_i32(posy + rf2 + 1), h);
+      if (bx0 >= bx1 || by0 >= by1) continue;
+      const int gx0 = bx0 / g.G, gx1 = (bx1 - 1) / g.G;
+      const int gy0 = by0 / g.G, gy1 = (by1 - 1) / g.G;
+      const uint32_t rbase = static_cast<uint32_t>(tile) * g.ngm * g.ngm;
+      for (int gy = gy0; gy <= gy1; ++gy)
+        for (int gx = gx0; gx <= gx1; ++gx) f(rbase + gy * g.ngm + gx);
+    }
+  }
+}
+
+}  // namespace splhip

@@ -1,0 +1,487 @@
+// gfx950 kernels of the Splotch particle-splat hot path.
+//
+//   k_project      particle_project + particle_colorize (splotch_host.cc:58-287)
+//                  over SoA streams; FP64 transform with the float32 matrix
+//                  exactly as TRANSFORM::TransPoint (kernel/transform.cxx:212-219).
+//   k_compact      stream compaction of one pass's particles (+ depth-sort keys).
+//   k_gather_count gather into render order + per-particle bin count.
+//   k_emit         (bin, particle) entries in particle order.
+//   k_bin_offsets  CSR offsets from the sorted bin keys (binary search).
+//   k_splat        one workgroup per work region: the region's particles are
+//                  staged in LDS in batches, per-particle column ranges and the
+//                  separable Gaussian factors are tabulated in LDS
+//                  (splotch_host.cc:596-607), and each lane owns one pixel of an
+//                  8x8 block and accumulates in registers in list order, i.e.
+//                  in the reference's per-pixel summation order.
+#include "kernels.hpp"
+
+#include <algorithm>
+
+#include "workspace.hpp"
+
+namespace splhip {
+namespace {
+
+__device__ __forceinline__ int64_t gtid() {
+  return static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+}
+
+__device__ __forceinline__ int64_t gstride() {
+  return static_cast<int64_t>(gridDim.x) * blockDim.x;
+}
+
+unsigned grid_for(int64_t n, int per_block = 256, int64_t cap = 1 << 20) {
+  int64_t g = (n + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return static_cast<unsigned>(g);
+}
+
+// ---------------------------------------------------------------- project
+
+__global__ __launch_bounds__(256) void k_project(ProjectArgs a) {
+  for (int64_t i = gtid(); i < a.n; i += gstride()) {
+    const int t = a.type ? a.type[i] : 0;
+    const int tt = t < kMaxTypes ? t : kMaxTypes - 1;
+    float px = a.x[i], py = a.y[i], pz = a.z[i];
+    float R = a.hsml[i], II = a.inten[i];
+    float e0 = a.col[i];
+    float e1 = a.c1 ? a.c1[i] : 0.f;
+    float e2 = a.c2 ? a.c2[i] : 0.f;
+    const float *m = a.cam.mat;
+    // TransPoint: double accumulation, left to right, no contraction.
+    const double dx = px, dy = py, dz = pz;
+    const double vx = dx * static_cast<double>(m[0]) + dy * static_cast<double>(m[1]) +
+                      dz * static_cast<double>(m[2]) + static_cast<double>(m[3]);
+    const double vy = dx * static_cast<double>(m[4]) + dy * static_cast<double>(m[5]) +
+                      dz * static_cast<double>(m[6]) + static_cast<double>(m[7]);
+    const double vz = dx * static_cast<double>(m[8]) + dy * static_cast<double>(m[9]) +
+                      dz * static_cast<double>(m[10]) + static_cast<double>(m[11]);
+    px = static_cast<float>(vx);
+    py = static_cast<float>(vy);
+    pz = static_cast<float>(vz);
+    uint8_t code = 0;
+    bool live = (t < a.ntypes) && !(-pz <= a.cam.zminval) && !(-pz >= a.cam.zmaxval);
+    if (live) {
+      float xfac2 = a.cam.xfac;
+      const float res2 = a.cam.res2;
+      if (!a.cam.projection) {
+        px = res2 + px * xfac2;
+        py = res2 + a.cam.ycorr + py * xfac2;
+      } else {
+        xfac2 = -res2 / (a.cam.fovfct * pz);
+        px = static_cast<float>(static_cast<double>(res2) + a.cam.xshift +
+                                static_cast<double>(px * xfac2));
+        py = static_cast<float>(static_cast<double>(res2) + a.cam.yshift +
+                                static_cast<double>(a.cam.ycorr) +
+                                static_cast<double>(py * xfac2));
+      }
+      II *= (0.5f * a.bfak) / R;
+      R *= 2;
+      R *= xfac2 * a.smooth_fac[tt];
+      const float mr = a.cam.minrad_pix;
+      const float rcorr = sqrtf(R * R + mr * mr) / R;
+      R *= rcorr;
+      II /= rcorr;
+      const float rfacr = a.rfac * R;
+      const int xres = a.cam.xres, yres = a.cam.yres;
+      int minx = cvt_i32(px - rfacr + 1);
+      int maxx = cvt_i32(px + rfacr + 1);
+      int miny = cvt_i32(py - rfacr + 1);
+      int maxy = cvt_i32(py + rfacr + 1);
+      bool act = !(minx >= xres);
+      minx = max(minx, 0);
+      act = act && !(maxx <= 0);
+      maxx = min(maxx, xres);
+      act = act && !(minx >= maxx);
+      act = act && !(miny >= yres);
+      miny = max(miny, 0);
+      act = act && !(maxy <= 0);
+      maxy = min(maxy, yres);
+      act = act && !(miny >= maxy);
+      if (act) {
+        // particle_colorize: palette lookup (linear_map.h:71-84, math_utils.h:186-202)
+        if (!a.col_vector[tt]) {
+          const double *X = a.pal_x + static_cast<int64_t>(tt) * a.pal_stride;
+          const float *Y = a.pal_rgb + static_cast<int64_t>(tt) * a.pal_stride * 3;
+          const int np = a.pal_n[tt];
+          const double xv = e0;
+          float cr, cg, cb;
+          if (np == 1) {
+            cr = Y[0]; cg = Y[1]; cb = Y[2];
+          } else if (xv >= X[np - 1]) {
+            cr = Y[3 * (np - 1)]; cg = Y[3 * (np - 1) + 1]; cb = Y[3 * (np - 1) + 2];
+          } else if (xv <= X[0]) {
+            cr = Y[0]; cg = Y[1]; cb = Y[2];
+          } else {
+            int lo = 0, len = np;
+            while (len > 0) {  // lower_bound
+              const int half = len >> 1;
+              if (X[lo + half] < xv) {
+                lo += half + 1;
+                len -= half + 1;
+              } else {
+                len = half;
+              }
+            }
+            int idx = lo;
+            if (idx > 0) --idx;
+            idx = min(np - 2, idx);
+            const double frac = (xv - X[idx]) / (X[idx + 1] - X[idx]);
+            const float *y0 = Y + 3 * idx, *y1 = Y + 3 * (idx + 1);
+            cr = static_cast<float>((1. - frac) * static_cast<double>(y0[0])) +
+                 static_cast<float>(frac * static_cast<double>(y1[0]));
+            cg = static_cast<float>((1. - frac) * static_cast<double>(y0[1])) +
+                 static_cast<float>(frac * static_cast<double>(y1[1]));
+            cb = static_cast<float>((1. - frac) * static_cast<double>(y0[2])) +
+                 static_cast<float>(frac * static_cast<double>(y1[2]));
+          }
+          e0 = cr; e1 = cg; e2 = cb;
+        }
+        const float fac = II * a.brightness[tt];
+        e0 *= fac; e1 *= fac; e2 *= fac;
+        code = static_cast<uint8_t>(a.pass_code[tt]);
+      }
+    }
+    Rec o;
+    o.x = px; o.y = py; o.r = R; o.e0 = e0; o.e1 = e1; o.e2 = e2; o.z = pz; o.I = II;
+    a.rec[i] = o;
+    a.act[i] = code;
+  }
+}
+
+// ---------------------------------------------------------------- compaction
+
+// Order-preserving float key (ascending); sort types of splotchutils.h:74-96.
+__device__ __forceinline__ uint32_t sort_key(const Rec &R, int sort_type) {
+  float f = (sort_type == 1) ? R.z : (sort_type == 4 ? R.r : R.e0);
+  if (f == 0.f) f = 0.f;  // -0 and +0 compare equal
+  uint32_t u = __float_as_uint(f);
+  u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  if (sort_type == 2 || sort_type == 4) u = ~u;  // descending
+  return u;
+}
+
+__global__ __launch_bounds__(256) void k_compact(const uint8_t *act, uint8_t code,
+                                                 const uint32_t *pos, int64_t n, const Rec *rec,
+                                                 int sort_type, uint32_t *keys,
+                                                 uint32_t *vals) {
+  for (int64_t i = gtid(); i < n; i += gstride()) {
+    if (act[i] != code) continue;
+    const uint32_t p = pos[i];
+    vals[p] = static_cast<uint32_t>(i);
+    if (keys) keys[p] = sort_key(rec[i], sort_type);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gather_count(const uint32_t *order, uint32_t m,
+                                                      const Rec *rec, Rec *recs, uint32_t *cnt,
+                                                      BinGeom g) {
+  for (int64_t s = gtid(); s < m; s += gstride()) {
+    const Rec R = rec[order[s]];
+    recs[s] = R;
+    uint32_t c = 0;
+    for_each_bin(R, g, [&](uint32_t) { ++c; });
+    cnt[s] = c;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_emit(const Rec *recs, uint32_t m, const uint32_t *off,
+                                              uint32_t e, BinGeom g, uint32_t *keys,
+                                              uint32_t *vals) {
+  for (int64_t s = gtid(); s < m; s += gstride()) {
+    const Rec R = recs[s];
+    uint32_t o = off[s];
+    const uint32_t end = (s + 1 < m) ? off[s + 1] : e;  // same enumeration as the count
+    const uint32_t sv = static_cast<uint32_t>(s);
+    for_each_bin(R, g, [&](uint32_t b) {
+      if (o < end) {
+        keys[o] = b;
+        vals[o] = sv;
+      }
+      ++o;
+    });
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bin_offsets(const uint32_t *keys, uint32_t e,
+                                                     uint32_t nbins, uint32_t *bin_off) {
+  for (int64_t r = gtid(); r <= nbins; r += gstride()) {
+    uint32_t lo = 0, hi = e;
+    while (lo < hi) {
+      const uint32_t mid = lo + ((hi - lo) >> 1);
+      if (keys[mid] < r) lo = mid + 1;
+      else hi = mid;
+    }
+    bin_off[r] = lo;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_map(const uint32_t *idx, const uint32_t *table,
+                                             uint32_t *out, int64_t n) {
+  for (int64_t i = gtid(); i < n; i += gstride()) out[i] = table[idx[i]];
+}
+
+__global__ __launch_bounds__(256) void k_fill(float *p, float v, int64_t n) {
+  for (int64_t i = gtid(); i < n; i += gstride()) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t *p, uint32_t v, int64_t n) {
+  for (int64_t i = gtid(); i < n; i += gstride()) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_aos_to_soa(const splhip_particle_sim *aos, int64_t n,
+                                                    float *soa, uint8_t *type) {
+  for (int64_t i = gtid(); i < n; i += gstride()) {
+    const splhip_particle_sim p = aos[i];
+    soa[0 * n + i] = p.x;
+    soa[1 * n + i] = p.y;
+    soa[2 * n + i] = p.z;
+    soa[3 * n + i] = p.r;
+    soa[4 * n + i] = p.I;
+    soa[5 * n + i] = p.er;
+    soa[6 * n + i] = p.eg;
+    soa[7 * n + i] = p.eb;
+    type[i] = static_cast<uint8_t>(p.type);
+  }
+}
+
+// ---------------------------------------------------------------- splat
+
+// Workgroup = one work region (G x G px, aligned to its reference tile's
+// origin) = (G/8)^2 pixel blocks of 8x8; wave w owns blocks w, w+4, ...
+// B particles are staged per batch.
+template <int G, int ADD, int B>
+__global__ __launch_bounds__(256) void k_splat(SplatArgs a) {
+  constexpr int NB = G / 8;
+  constexpr int MAXBLK = (NB * NB + 3) / 4;
+  __shared__ float s_tab[2048];
+  __shared__ float s_px[B], s_py[B], s_rad[B], s_stp[B];
+  __shared__ float s_a[3][B];
+  __shared__ float s_q[3][B];
+  __shared__ float s_maxa[B];
+  __shared__ uint32_t s_bx[B], s_by[B];  // footprint box, lo | hi << 16 (tile frame)
+  __shared__ uint32_t s_col[B][G];       // column y-range, lo | hi << 16
+  __shared__ float s_pre2[B][G];
+  __shared__ float s_pre1[B][G];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t rid = blockIdx.x;
+  const int per_tile = a.ngm * a.ngm;
+  const int tile = static_cast<int>(rid / per_tile);
+  const int rem = static_cast<int>(rid - static_cast<uint32_t>(tile) * per_tile);
+  const int gy = rem / a.ngm, gx = rem - gy * a.ngm;
+  const int cix = tile % a.ncx, ciy = tile / a.ncx;
+  const int x0s = cix * a.T, y0s = ciy * a.T;
+  const int w = min(x0s + a.T, a.xres) - x0s, h = min(y0s + a.T, a.yres) - y0s;
+  const int rx0 = gx * G, ry0 = gy * G;
+  if (rx0 >= w || ry0 >= h) return;
+  const int rw = min(G, w - rx0), rh = min(G, h - ry0);
+  const int nbx = (rw + 7) >> 3, nby = (rh + 7) >> 3, nblk = nbx * nby;
+
+  for (int i = tid; i < 2048; i += 256) s_tab[i] = a.exptab[i];
+
+  float acc[MAXBLK][3];
+#pragma unroll
+  for (int j = 0; j < MAXBLK; ++j) acc[j][0] = acc[j][1] = acc[j][2] = 0.f;
+
+  const uint32_t e0 = a.bin_off[rid], e1 = a.bin_off[rid + 1];
+  const float expfac = a.expfac;
+  for (uint32_t base = e0; base < e1; base += B) {
+    const int nb = static_cast<int>(min<uint32_t>(B, e1 - base));
+    __syncthreads();
+    if (tid < nb) {
+      const Rec R = a.recs[a.ent[base + tid]];
+      // splotch_host.cc:462-476 (tile frame)
+      const float rfacr = R.r * a.rfac;
+      const float posx = R.x - static_cast<float>(x0s), posy = R.y - static_cast<float>(y0s);
+      int bx0 = max(cvt_i32(posx - rfacr + 1), 0), bx1 = min(cvt_i32(posx + rfacr + 1), w);
+      int by0 = max(cvt_i32(posy - rfacr + 1), 0), by1 = min(cvt_i32(posy + rfacr + 1), h);
+      bx1 = max(bx1, bx0);
+      by1 = max(by1, by0);
+      const float sigma = a.h2sigma * R.r;
+      s_px[tid] = posx;
+      s_py[tid] = posy;
+      s_rad[tid] = rfacr * rfacr;
+      s_stp[tid] = -1.f / (sigma * sigma);
+      s_bx[tid] = static_cast<uint32_t>(bx0) | (static_cast<uint32_t>(bx1) << 16);
+      s_by[tid] = static_cast<uint32_t>(by0) | (static_cast<uint32_t>(by1) << 16);
+      s_a[0][tid] = -R.e0;
+      s_a[1][tid] = -R.e1;
+      s_a[2][tid] = -R.e2;
+      if (!ADD) {
+        const float g = a.grayabsorb;
+        s_q[0][tid] = R.e0 / (R.e0 + g);
+        s_q[1][tid] = R.e1 / (R.e1 + g);
+        s_q[2][tid] = R.e2 / (R.e2 + g);
+        s_maxa[tid] = fmaxf(fabsf(-R.e0), fmaxf(fabsf(-R.e1), fabsf(-R.e2)));
+      }
+    }
+    __syncthreads();
+    // Phase A: per particle, G column entries and G row entries.
+    for (int it = tid; it < nb * 2 * G; it += 256) {
+      const int k = it / (2 * G);
+      const int c = it - k * (2 * G);
+      const uint32_t bx = s_bx[k], by = s_by[k];
+      const float posx = s_px[k], posy = s_py[k], stp = s_stp[k];
+      if (c < G) {
+        const int x = rx0 + c;
+        uint32_t lohi = 0;
+        float pre2 = 0.f;
+        if (x >= static_cast<int>(bx & 0xffff) && x < static_cast<int>(bx >> 16)) {
+          const float dxsq = (static_cast<float>(x) - posx) * (static_cast<float>(x) - posx);
+          const float dy = sqrtf(s_rad[k] - dxsq);
+          const int lo = max(static_cast<int>(by & 0xffff), cvt_i32(posy - dy + 1));
+          const int hi = min(static_cast<int>(by >> 16), cvt_i32(posy + dy + 1));
+          if (hi > lo) lohi = static_cast<uint32_t>(lo) | (static_cast<uint32_t>(hi) << 16);
+          pre2 = xexp_op(s_tab, stp * dxsq, expfac);
+        }
+        s_col[k][c] = lohi;
+        s_pre2[k][c] = pre2;
+      } else {
+        const int y = ry0 + (c - G);
+        float pre1 = 0.f;
+        if (y >= static_cast<int>(by & 0xffff) && y < static_cast<int>(by >> 16)) {
+          const float dyf = static_cast<float>(y) - posy;
+          pre1 = xexp_op(s_tab, stp * dyf * dyf, expfac);
+        }
+        s_pre1[k][c - G] = pre1;
+      }
+    }
+    __syncthreads();
+    // Phase B: accumulate in list order.
+#pragma unroll
+    for (int j = 0; j < MAXBLK; ++j) {
+      const int blk = wave + 4 * j;
+      if (blk >= nblk) break;
+      const int bxo = rx0 + (blk % nbx) * 8, byo = ry0 + (blk / nbx) * 8;
+      const int lx = bxo + (lane >> 3), ly = byo + (lane & 7);
+      const int cx = lx - rx0, cy = ly - ry0;
+      float l0 = acc[j][0], l1 = acc[j][1], l2 = acc[j][2];
+      for (int k = 0; k < nb; ++k) {
+        const uint32_t bx = s_bx[k], by = s_by[k];
+        if (static_cast<int>(bx & 0xffff) >= bxo + 8 || static_cast<int>(bx >> 16) <= bxo ||
+            static_cast<int>(by & 0xffff) >= byo + 8 || static_cast<int>(by >> 16) <= byo)
+          continue;
+        const uint32_t lohi = s_col[k][cx];
+        if (ly < static_cast<int>(lohi & 0xffff) || ly >= static_cast<int>(lohi >> 16)) continue;
+        const float att = s_pre1[k][cy] * s_pre2[k][cx];
+        if (ADD) {
+          l0 += s_a[0][k] * att;
+          l1 += s_a[1][k] * att;
+          l2 += s_a[2][k] * att;
+        } else {
+          const float a0 = s_a[0][k], a1 = s_a[1][k], a2 = s_a[2][k];
+          const float q0 = s_q[0][k], q1 = s_q[1][k], q2 = s_q[2][k];
+          if ((s_maxa[k] * att) < a.taylorlimit) {
+            l0 += ((l0 - q0) * a0) * att;
+            l1 += ((l1 - q1) * a1) * att;
+            l2 += ((l2 - q2) * a2) * att;
+          } else {
+            l0 += xexp_expm1(s_tab, att * a0, expfac, a.taylorlimit) * (l0 - q0);
+            l1 += xexp_expm1(s_tab, att * a1, expfac, a.taylorlimit) * (l1 - q1);
+            l2 += xexp_expm1(s_tab, att * a2, expfac, a.taylorlimit) * (l2 - q2);
+          }
+        }
+      }
+      acc[j][0] = l0;
+      acc[j][1] = l1;
+      acc[j][2] = l2;
+    }
+  }
+  // write back (x-major RGB)
+#pragma unroll
+  for (int j = 0; j < MAXBLK; ++j) {
+    const int blk = wave + 4 * j;
+    if (blk >= nblk) break;
+    const int bxo = rx0 + (blk % nbx) * 8, byo = ry0 + (blk / nbx) * 8;
+    const int lx = bxo + (lane >> 3), ly = byo + (lane & 7);
+    if (lx < rx0 + rw && ly < ry0 + rh) {
+      const int64_t X = x0s + lx, Y = y0s + ly;
+      float *d = a.pic + (X * a.yres + Y) * 3;
+      if (a.accumulate) {
+        d[0] = d[0] + acc[j][0];
+        d[1] = d[1] + acc[j][1];
+        d[2] = d[2] + acc[j][2];
+      } else {
+        d[0] = acc[j][0];
+        d[1] = acc[j][1];
+        d[2] = acc[j][2];
+      }
+    }
+  }
+}
+
+}  // namespace
+
+void launch_project(const ProjectArgs &a, hipStream_t s) {
+  k_project<<<grid_for(a.n), 256, 0, s>>>(a);
+  hip_check(hipGetLastError(), "k_project launch");
+}
+
+void launch_compact(const uint8_t *act, uint8_t code, const uint32_t *pos, int64_t n,
+                    const Rec *rec, int sort_type, uint32_t *keys, uint32_t *vals,
+                    hipStream_t s) {
+  k_compact<<<grid_for(n), 256, 0, s>>>(act, code, pos, n, rec, sort_type, keys, vals);
+  hip_check(hipGetLastError(), "k_compact launch");
+}
+
+void launch_gather_count(const uint32_t *order, uint32_t m, const Rec *rec, Rec *recs,
+                         uint32_t *cnt, const BinGeom &g, hipStream_t s) {
+  k_gather_count<<<grid_for(m), 256, 0, s>>>(order, m, rec, recs, cnt, g);
+  hip_check(hipGetLastError(), "k_gather_count launch");
+}
+
+void launch_emit(const Rec *recs, uint32_t m, const uint32_t *off, uint32_t e, const BinGeom &g,
+                 uint32_t *keys, uint32_t *vals, hipStream_t s) {
+  k_emit<<<grid_for(m), 256, 0, s>>>(recs, m, off, e, g, keys, vals);
+  hip_check(hipGetLastError(), "k_emit launch");
+}
+
+void launch_bin_offsets(const uint32_t *sorted_keys, uint32_t e, uint32_t nbins,
+                        uint32_t *bin_off, hipStream_t s) {
+  k_bin_offsets<<<grid_for(static_cast<int64_t>(nbins) + 1), 256, 0, s>>>(sorted_keys, e, nbins,
+                                                                           bin_off);
+  hip_check(hipGetLastError(), "k_bin_offsets launch");
+}
+
+void launch_map(const uint32_t *idx, const uint32_t *table, uint32_t *out, int64_t n,
+                hipStream_t s) {
+  k_map<<<grid_for(n), 256, 0, s>>>(idx, table, out, n);
+  hip_check(hipGetLastError(), "k_map launch");
+}
+
+void launch_fill(float *p, float v, int64_t n, hipStream_t s) {
+  k_fill<<<grid_for(n), 256, 0, s>>>(p, v, n);
+  hip_check(hipGetLastError(), "k_fill launch");
+}
+
+void launch_fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t s) {
+  k_fill_u32<<<grid_for(n), 256, 0, s>>>(p, v, n);
+  hip_check(hipGetLastError(), "k_fill_u32 launch");
+}
+
+void launch_aos_to_soa(const splhip_particle_sim *aos, int64_t n, float *soa, uint8_t *type,
+                       hipStream_t s) {
+  k_aos_to_soa<<<grid_for(n), 256, 0, s>>>(aos, n, soa, type);
+  hip_check(hipGetLastError(), "k_aos_to_soa launch");
+}
+
+void launch_splat(int G, bool additive, uint32_t nbins, const SplatArgs &a, hipStream_t s) {
+  if (nbins == 0) return;
+  const dim3 grid(nbins), block(256);
+#define SPLAT_CASE(GG, BB)                                         \
+  if (G == GG) {                                                   \
+    if (additive) k_splat<GG, 1, BB><<<grid, block, 0, s>>>(a);    \
+    else k_splat<GG, 0, BB><<<grid, block, 0, s>>>(a);             \
+    hip_check(hipGetLastError(), "k_splat launch");                \
+    return;                                                        \
+  }
+  SPLAT_CASE(16, 64)
+  SPLAT_CASE(32, 64)
+  SPLAT_CASE(64, 32)
+#undef SPLAT_CASE
+  throw HipError{hipErrorInvalidValue, "unsupported work region size"};
+}
+
+}  // namespace splhip

@@ -1,0 +1,59 @@
+// Launch wrappers of the render kernels (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace splhip {
+
+struct ProjectArgs {
+  const float *x, *y, *z, *hsml, *inten, *col, *c1, *c2;
+  const uint8_t *type;
+  int64_t n;
+  splhip_camera cam;
+  float smooth_fac[kMaxTypes];
+  float brightness[kMaxTypes];
+  int32_t col_vector[kMaxTypes];
+  int32_t pass_code[kMaxTypes];  // 1: ordered pass, 2: additive pass, 0: not rendered
+  int32_t pal_n[kMaxTypes];
+  int32_t pal_stride;
+  int32_t ntypes;
+  const double *pal_x;
+  const float *pal_rgb;
+  float bfak, rfac;
+  Rec *rec;
+  uint8_t *act;
+};
+
+struct SplatArgs {
+  const uint32_t *bin_off;  // [nbins+1]
+  const uint32_t *ent;      // entries: slot in recs
+  const Rec *recs;
+  const float *exptab;      // 2048 floats
+  float *pic;
+  int T, xres, yres, ncx, ngm;
+  int accumulate;
+  float rfac, h2sigma, expfac, taylorlimit, grayabsorb;
+};
+
+void launch_project(const ProjectArgs &a, hipStream_t s);
+void launch_compact(const uint8_t *act, uint8_t code, const uint32_t *pos, int64_t n,
+                    const Rec *rec, int sort_type, uint32_t *keys, uint32_t *vals, hipStream_t s);
+void launch_gather_count(const uint32_t *order, uint32_t m, const Rec *rec, Rec *recs,
+                         uint32_t *cnt, const BinGeom &g, hipStream_t s);
+void launch_emit(const Rec *recs, uint32_t m, const uint32_t *off, uint32_t e, const BinGeom &g,
+                 uint32_t *keys, uint32_t *vals, hipStream_t s);
+void launch_bin_offsets(const uint32_t *sorted_keys, uint32_t e, uint32_t nbins,
+                        uint32_t *bin_off, hipStream_t s);
+void launch_map(const uint32_t *idx, const uint32_t *table, uint32_t *out, int64_t n,
+                hipStream_t s);
+void launch_fill(float *p, float v, int64_t n, hipStream_t s);
+void launch_fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t s);
+void launch_aos_to_soa(const splhip_particle_sim *aos, int64_t n, float *soa /*8 planes*/,
+                       uint8_t *type, hipStream_t s);
+// G: region size (16, 32 or 64 px); additive: a_eq_e mode.
+void launch_splat(int G, bool additive, uint32_t nbins, const SplatArgs &a, hipStream_t s);
+
+}  // namespace splhip

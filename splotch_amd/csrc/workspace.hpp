@@ -1,0 +1,68 @@
+// Device scratch owned by a context: named slots that grow on demand and are
+// reused across frames (no per-frame hipMalloc once the sizes are reached).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+namespace splhip {
+
+enum Slot : int {
+  S_REC, S_ACT, S_POS, S_K0, S_V0, S_K1, S_V1, S_RECS, S_CNT, S_OFF,
+  S_EK0, S_EV0, S_EK1, S_EV1, S_REGOFF, S_HIST, S_SCAN0, S_SCAN1, S_SCAN2, S_SCAN3,
+  S_TOTALS, S_SOA, S_AOS, S_PIC, S_MAP, S_AUX0, S_AUX1, S_AUX2, S_AUX3,
+  S_NSLOTS
+};
+
+struct HipError {
+  hipError_t code;
+  const char *what;
+};
+
+inline void hip_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) throw HipError{e, what};
+}
+
+class Workspace {
+ public:
+  ~Workspace() { release(); }
+  void *get(int slot, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    if (cap_[slot] < bytes) {
+      if (ptr_[slot]) {
+        hip_check(hipDeviceSynchronize(), "sync before workspace growth");
+        hip_check(hipFree(ptr_[slot]), "hipFree");
+        ptr_[slot] = nullptr;
+      }
+      size_t c = bytes + bytes / 8;  // headroom against small growth steps
+      c = (c + 4095) & ~size_t(4095);
+      hip_check(hipMalloc(&ptr_[slot], c), "hipMalloc workspace");
+      cap_[slot] = c;
+    }
+    return ptr_[slot];
+  }
+  template <class T>
+  T *get_as(int slot, size_t count) {
+    return static_cast<T *>(get(slot, count * sizeof(T)));
+  }
+  size_t total_bytes() const {
+    size_t s = 0;
+    for (int i = 0; i < S_NSLOTS; ++i) s += cap_[i];
+    return s;
+  }
+  void release() {
+    for (int i = 0; i < S_NSLOTS; ++i) {
+      if (ptr_[i]) (void)hipFree(ptr_[i]);
+      ptr_[i] = nullptr;
+      cap_[i] = 0;
+    }
+  }
+
+ private:
+  void *ptr_[S_NSLOTS] = {};
+  size_t cap_[S_NSLOTS] = {};
+};
+
+}  // namespace splhip

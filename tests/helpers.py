@@ -1,0 +1,45 @@
+"""Shared helpers for the parity tests (fixture decoding, comparisons)."""
+import numpy as np
+
+from splotch_amd import synth
+
+
+def view_from(f, prefix=""):
+    res = f[prefix + "res"]
+    return synth.View(tuple(f[prefix + "campos"]), tuple(f[prefix + "lookat"]),
+                      tuple(f[prefix + "sky"]), int(res[0]), int(res[1]),
+                      float(f[prefix + "fov"][0]), tuple(f[prefix + "centerpos"]))
+
+
+def field_from(f):
+    fl = synth.Field(f["x"], f["y"], f["z"], f["hsml"], f["inten"], f["col"],
+                     f["type"] if "type" in f else None)
+    if "c1" in f:
+        fl.c1 = f["c1"]
+        fl.c2 = f["c2"]
+    return fl
+
+
+def bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.uint32) if a.dtype == np.float32 else a
+
+
+def assert_bitexact(a, b, what=""):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    assert a.shape == b.shape, (what, a.shape, b.shape)
+    ba, bb = bits(a), bits(b)
+    bad = np.nonzero(ba.reshape(-1) != bb.reshape(-1))[0]
+    if bad.size:
+        i = bad[0]
+        raise AssertionError("%s: %d of %d differ; first at %d: %r vs %r"
+                             % (what, bad.size, ba.size, i, a.reshape(-1)[i], b.reshape(-1)[i]))
+
+
+def max_rel_err(a, b, floor_frac=1e-6):
+    """Max per-channel relative error |a-b| / max(|b|, floor_frac*max|b|)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = np.maximum(np.abs(b), floor_frac * max(np.abs(b).max(), 1e-30))
+    return float((np.abs(a - b) / den).max())

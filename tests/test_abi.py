@@ -1,0 +1,95 @@
+"""C-ABI library: loads without a GPU, exports every declared symbol, and its
+host-only functions (camera setup, exp tables, .par parsing) agree with the
+oracle bit for bit."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+import splotch_amd as S
+from helpers import assert_bitexact, view_from
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    syms = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", fn)).read()
+            syms |= set(re.findall(r"^\s*(?:int|void|const char \*)\s*\**\s*(splhip_\w+)\s*\(",
+                                   txt, re.M))
+    return syms
+
+
+def test_library_exports_declared_symbols():
+    L = S.capi.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 12
+    for s in sorted(syms):
+        assert hasattr(L, s), s
+    assert set(S.capi.EXPORTS) <= syms
+
+
+def test_exptable_matches_oracle():
+    tab, k = S.exptable()
+    t1, t2, expfac, tl = O.exptable()
+    assert_bitexact(tab[:1024], t1)
+    assert_bitexact(tab[1024:], t2)
+    assert k[0] == np.float32(expfac) and k[1] == np.float32(tl)
+    assert k[4] == np.float32(0.75)
+
+
+def test_camera_setup_matches_reference_fixture(golden):
+    f = golden("camera")
+    for i in range(int(f["nviews"][0])):
+        v = view_from(f, "v%d_" % i)
+        cam = S.camera_setup(v)
+        assert_bitexact(np.array(cam.mat[:], np.float32), f["v%d_mat" % i])
+        assert cam.xshift == f["v%d_shift" % i][0] and cam.yshift == f["v%d_shift" % i][1]
+        assert np.float32(cam.xfac) == f["v%d_f" % i][2]
+        oc = O.camera(v)
+        assert (cam.res2, cam.ycorr, cam.fovfct, cam.dist) == (oc.res2, oc.ycorr, oc.fovfct,
+                                                               oc.dist)
+
+
+def test_camera_degenerate_raises():
+    from splotch_amd import synth
+    with pytest.raises(S.SplhipError):
+        S.camera_setup(synth.View((50, 50, -150), (50, 50, 50), (0, 0, 1), 64, 64))
+
+
+def test_params_from_par_defaults_and_keys():
+    p, sc = S.params_from_par("")
+    assert sc == dict(xres=800, yres=800, fov=45.0, zmin=0.0, zmax=np.float32(1e23),
+                      projection=True, minrad_pix=1.0)
+    assert p.ptypes == 1 and p.sort_type == 1 and p.tile_size == 50
+    assert abs(p.gray_absorption - 0.2) < 1e-7 and p.a_eq_e[0] == 1
+    txt = """# comment
+    xres = 640
+    fov=30   # trailing comment
+    ptypes=2
+    brightness0=0.5
+    brightness1=2.0
+    smooth_factor1 = 0.8
+    color_is_vector1 = T
+    a_eq_e = false
+    a_eq_e1 = .true.
+    tile_size_cpu=32.0
+    gray_absorption=0.3
+    sort_type=2
+    projection=FALSE
+    """
+    p, sc = S.params_from_par(txt, b_brightness=1.5)
+    assert sc["xres"] == 640 and sc["yres"] == 640 and sc["fov"] == 30 and not sc["projection"]
+    assert p.ptypes == 2 and p.a_eq_e[0] == 0 and p.a_eq_e[1] == 1
+    assert p.brightness[0] == np.float32(np.float32(0.5) * np.float32(1.5))
+    assert p.brightness[1] == np.float32(3.0)
+    assert abs(p.smooth_fac[1] - 0.8) < 1e-7 and p.col_vector[1] == 1 and p.col_vector[0] == 0
+    assert p.tile_size == 32 and p.sort_type == 2
+    with pytest.raises(S.SplhipError):
+        S.params_from_par("projection=1")        # not a bool in the reference's parser
+    with pytest.raises(S.SplhipError):
+        S.params_from_par("xres=12abc")

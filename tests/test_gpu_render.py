@@ -1,0 +1,296 @@
+"""GPU parity of the plain render path, through the C-ABI.
+
+Bar: bit-exact against the genuine reference fixtures and the oracle for
+projected particles, tile lists and raw images (the per-pixel summation order
+of render_new is reproduced).  Ordered mode uses the stable (z, index) depth
+order; the genuine std::sort order differs only between equal keys, and the
+error against it is stated (max per-channel relative error, floor 1e-6*max).
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from helpers import assert_bitexact, field_from, max_rel_err, view_from
+from splotch_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def R():
+    import splotch_amd as S
+    assert torch.cuda.is_available(), "GPU test without a GPU"
+    r = S.Renderer(0)
+    yield r
+    torch.cuda.synchronize()
+    r.close()
+
+
+def palette(golden):
+    p = golden("palette")
+    return p["x"], p["rgb"]
+
+
+def render(R, field, view, pals, prm, **camkw):
+    import splotch_amd as S
+    for t, (x, rgb) in enumerate(pals):
+        R.set_palette(t, x, rgb)
+    cam = S.camera_setup(view, **camkw)
+    dev = S.to_device(field)
+    pic = torch.empty(view.xres * view.yres * 3, dtype=torch.float32, device="cuda")
+    R.render(dev, cam, prm, pic)
+    torch.cuda.synchronize()
+    return pic.cpu().numpy().reshape(view.xres, view.yres, 3)
+
+
+CASES = {
+    "additive_small": dict(),
+    "ordered_small": dict(brightness=None),
+    "ortho_zclip": dict(cam=dict(projection=False, zmin=120.0, zmax=260.0)),
+}
+
+
+@pytest.mark.parametrize("name", ["additive_small", "ordered_small", "ortho_zclip", "two_types"])
+def test_project_colorize_bitexact(R, golden, name):
+    import splotch_amd as S
+    log("case project", name)
+    f = golden(name)
+    pals = [palette(golden)]
+    camkw = {}
+    kw = {}
+    if name == "ordered_small":
+        kw["brightness"] = [float(f["brightness"][0])]
+    if name == "ortho_zclip":
+        camkw = dict(projection=False, zmin=120.0, zmax=260.0)
+    if name == "two_types":
+        pals.append((f["pal1_x"], f["pal1_rgb"]))
+        kw = dict(ptypes=2, smooth_fac=[1.3, 0.8], brightness=[0.7, 2.0], col_vector=[0, 1],
+                  b_brightness=float(f["b_brightness"][0]))
+        camkw = dict(minrad_pix=1.5)
+    for t, (x, rgb) in enumerate(pals):
+        R.set_palette(t, x, rgb)
+    v = view_from(f)
+    cam = S.camera_setup(v, **camkw)
+    prm = S.params(**kw)
+    field = field_from(f)
+    rec, act = R.project(S.to_device(field), cam, prm)
+    rec = rec.cpu().numpy()
+    act = (act.cpu().numpy() != 0).astype(np.uint8)
+    assert_bitexact(act, f["p_active"], "active")
+    for k, col in (("x", 0), ("y", 1), ("r", 2), ("z", 6), ("I", 7)):
+        assert_bitexact(rec[:, col], f["p_" + k], k)
+    for k, col in (("e0", 3), ("e1", 4), ("e2", 5)):
+        a = rec[:, col]
+        b = f["p_" + k]
+        m = act == 1
+        assert_bitexact(a[m], b[m], k)
+
+
+@pytest.mark.parametrize("T", [50, 16, 8, 64])
+def test_bins_bitexact(R, golden, T):
+    import splotch_amd as S
+    log("case bins T", T)
+    f = golden("additive_small")
+    R.set_palette(0, *palette(golden))
+    v = view_from(f)
+    prm = S.params(tile_size=T)
+    off, idx, order = R.bins(S.to_device(field_from(f)), S.camera_setup(v), prm, ordered=False)
+    p = O.Projected(len(f["p_x"]))
+    for k in ("x", "y", "z", "r", "I", "e0", "e1", "e2", "active"):
+        getattr(p, k)[:] = f["p_" + k]
+    ooff, oidx = O.bins(p, v.xres, v.yres, T)
+    assert_bitexact(off, ooff, "tile offsets")
+    assert_bitexact(idx, oidx, "tile lists")
+    assert_bitexact(order, np.nonzero(p.active)[0].astype(np.uint32), "order")
+
+
+def test_bins_ordered_mode(R, golden):
+    import splotch_amd as S
+    f = golden("ordered_small")
+    R.set_palette(0, *palette(golden))
+    v = view_from(f)
+    prm = S.params(a_eq_e=False, brightness=[float(f["brightness"][0])])
+    off, idx, order = R.bins(S.to_device(field_from(f)), S.camera_setup(v), prm, ordered=True)
+    p = O.Projected(len(f["p_x"]))
+    for k in ("x", "y", "z", "r", "I", "e0", "e1", "e2", "active"):
+        getattr(p, k)[:] = f["p_" + k]
+    perm = f["perm_stable"]
+    act_sorted = perm[p.active[perm] == 1]
+    assert_bitexact(order, act_sorted, "depth order")
+    ps = p.take(perm)
+    ooff, oidx = O.bins(ps, v.xres, v.yres, 50)
+    assert_bitexact(off, ooff, "tile offsets")
+    assert_bitexact(idx, perm[oidx], "tile lists (original indices)")
+
+
+@pytest.mark.parametrize("G", [16, 32, 64])
+@pytest.mark.parametrize("T", [50, 16, 8, 64])
+def test_render_additive_fixture(R, golden, T, G):
+    import splotch_amd as S
+    log("case additive T", T, "G", G)
+    f = golden("additive_small")
+    pic = render(R, field_from(f), view_from(f), [palette(golden)],
+                 S.params(tile_size=T, hip_region=G))
+    assert_bitexact(pic, f["pic_T%d" % T], "raw pic T=%d G=%d" % (T, G))
+
+
+@pytest.mark.parametrize("G", [16, 32, 64])
+def test_render_ordered_fixture(R, golden, G):
+    import splotch_amd as S
+    log("case ordered G", G)
+    f = golden("ordered_small")
+    v = view_from(f)
+    fld = field_from(f)
+    b = [float(f["brightness"][0])]
+    pic = render(R, fld, v, [palette(golden)], S.params(a_eq_e=False, brightness=b, hip_region=G))
+    assert_bitexact(pic, f["pic_stable_T50"], "ordered stable order")
+    pic = render(R, fld, v, [palette(golden)],
+                 S.params(a_eq_e=False, brightness=b, gray_absorption=0.5, hip_region=G))
+    assert_bitexact(pic, f["pic_stable_T50_g05"], "ordered g=0.5")
+    pic = render(R, fld, v, [palette(golden)],
+                 S.params(a_eq_e=False, brightness=b, tile_size=16, hip_region=G))
+    err = max_rel_err(pic, f["pic_T16"])
+    log("ordered T16 vs genuine std::sort order: max rel err %.3g" % err)
+    assert err < 1e-3
+
+
+def test_render_two_types_fixture(R, golden):
+    import splotch_amd as S
+    f = golden("two_types")
+    pals = [palette(golden), (f["pal1_x"], f["pal1_rgb"])]
+    kw = dict(ptypes=2, smooth_fac=[1.3, 0.8], brightness=[0.7, 2.0], col_vector=[0, 1],
+              b_brightness=float(f["b_brightness"][0]))
+    for T in (50, 24):
+        pic = render(R, field_from(f), view_from(f), pals, S.params(tile_size=T, **kw),
+                     minrad_pix=1.5)
+        assert_bitexact(pic, f["pic_T%d" % T], "two types T=%d" % T)
+
+
+def test_render_ortho_fixture(R, golden):
+    import splotch_amd as S
+    f = golden("ortho_zclip")
+    pic = render(R, field_from(f), view_from(f), [palette(golden)], S.params(),
+                 projection=False, zmin=120.0, zmax=260.0)
+    assert_bitexact(pic, f["pic_T50"], "orthographic")
+
+
+def test_render_host_records(R, golden):
+    """cuda_rendering semantics: host particle_sim records -> host image."""
+    import splotch_amd as S
+    f = golden("host_rendering")
+    v = view_from(f)
+    fld = field_from(f)
+    R.set_palette(0, *palette(golden))
+    aos = S.to_particle_sim(fld)
+    before = aos.copy()
+    prm = S.params(a_eq_e=False, tile_size=32, gray_absorption=0.3)
+    pic = R.render_host(aos, S.camera_setup(v), prm)
+    assert np.array_equal(aos.view(np.uint8), before.view(np.uint8)), "host records modified"
+    opic, _ = O.host_rendering(fld, v, [palette(golden)], a_eq_e=False, tile=32, grayabsorb=0.3,
+                               stable=True)
+    assert_bitexact(pic, opic, "host path vs oracle (stable order)")
+    err = max_rel_err(pic, f["pic"])
+    log("host_rendering fixture (std::sort order): max rel err %.3g" % err)
+    assert err < 1e-3
+
+
+@pytest.mark.parametrize("seed,n,res,aeq,T,G", [
+    (101, 200_000, (512, 384), True, 50, 32),
+    (102, 200_000, (384, 512), False, 50, 32),
+    (103, 100_000, (300, 300), True, 37, 16),
+    (104, 100_000, (256, 256), False, 64, 64),
+])
+def test_render_random_vs_oracle(R, seed, n, res, aeq, T, G):
+    import splotch_amd as S
+    log("case random", seed, n, res, aeq, T, G)
+    fld = synth.plummer_field(n, seed, hsml_mu=math.log(0.5))
+    v = synth.default_view(*res)
+    pal = synth.default_palette()
+    pic = render(R, fld, v, [pal], S.params(a_eq_e=aeq, tile_size=T, hip_region=G))
+    opic, _ = O.host_rendering(fld, v, [pal], a_eq_e=aeq, tile=T, stable=True)
+    assert_bitexact(pic, opic, "random field vs oracle")
+
+
+def test_sort_types_vs_oracle(R):
+    import splotch_amd as S
+    fld = synth.plummer_field(30_000, 105)
+    v = synth.default_view(160, 160)
+    pal = synth.default_palette()
+    for st in (0, 2, 3, 4):
+        log("case sort_type", st)
+        pic = render(R, fld, v, [pal], S.params(a_eq_e=False, sort_type=st))
+        opic, _ = O.host_rendering(fld, v, [pal], a_eq_e=False, sort_type=st, stable=True)
+        assert_bitexact(pic, opic, "sort_type %d" % st)
+
+
+def test_mixed_modes_combine_rule(R):
+    """gas ordered + stars additive: pic = P_ordered(gas) + P_additive(stars)."""
+    import splotch_amd as S
+    fld = synth.gas_and_stars(40_000, 40_000, 106, 107)
+    v = synth.default_view(256, 256)
+    pal = synth.default_palette()
+    prm = S.params(ptypes=2, a_eq_e=[0, 1], brightness=[0.5, 2.0])
+    pic = render(R, fld, v, [pal, pal], prm)
+    p = O.project_colorize(fld, v, [pal, pal], brightness=[0.5, 2.0])
+    gas = p.take(np.arange(p.n)); gas.active = (p.active * (p.type == 0)).astype(np.uint8)
+    stars = p.take(np.arange(p.n)); stars.active = (p.active * (p.type == 1)).astype(np.uint8)
+    perm = O.sort_perm(gas, 1, stable=True)
+    pg = O.render(gas.take(perm), v.xres, v.yres, False, 0.2, 50)
+    ps = O.render(stars, v.xres, v.yres, True, 0.2, 50)
+    assert_bitexact(pic, (pg + ps).astype(np.float32), "combine rule")
+
+
+def test_empty_and_culled(R):
+    import splotch_amd as S
+    v = synth.default_view(64, 48)
+    pal = synth.default_palette()
+    fld = synth.plummer_field(1000, 108)
+    fld.z[:] = -400.0  # all behind the camera (camera at z=-150 looking +z)
+    pic = render(R, fld, v, [pal], S.params())
+    assert not pic.any()
+    fld0 = synth.plummer_field(0, 109)
+    pic = render(R, fld0, v, [pal], S.params())
+    assert not pic.any()
+    one = synth.plummer_field(1, 110)
+    one.x[:] = 50; one.y[:] = 50; one.z[:] = 50; one.hsml[:] = 3.0
+    pic = render(R, one, v, [pal], S.params())
+    opic, _ = O.host_rendering(one, v, [pal])
+    assert_bitexact(pic, opic, "single particle")
+    assert pic.any()
+
+
+def test_deterministic(R):
+    import splotch_amd as S
+    fld = synth.plummer_field(150_000, 111)
+    v = synth.default_view(320, 240)
+    pal = synth.default_palette()
+    a = render(R, fld, v, [pal], S.params(a_eq_e=False))
+    b = render(R, fld, v, [pal], S.params(a_eq_e=False))
+    assert_bitexact(a, b, "repeat")
+
+
+def test_errors(R):
+    import splotch_amd as S
+    v = synth.default_view(64)
+    fld = synth.plummer_field(100, 112)
+    cam = S.camera_setup(v)
+    pic = torch.empty(64 * 64 * 3, dtype=torch.float32, device="cuda")
+    dev = S.to_device(fld)
+    with pytest.raises(S.SplhipError):
+        R.render(dev, cam, S.params(sort_type=9), pic)
+    with pytest.raises(S.SplhipError):
+        R.render(dev, cam, S.params(hip_region=24), pic)
+    fld.c1 = None
+    with pytest.raises(S.SplhipError):   # vector colour without c1/c2
+        R.set_palette(0, *synth.default_palette())
+        R.render(S.to_device(fld), cam, S.params(col_vector=[1]), pic)

@@ -1,0 +1,132 @@
+"""The oracle (oracle/splotch_oracle.cpp) against the genuine reference's fixtures.
+
+Every fixture under tests/golden/ was written by the genuine Splotch host code
+(tests/golden/make_golden.py); the oracle must reproduce each bit for bit.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from helpers import assert_bitexact, field_from, view_from
+from splotch_amd import synth
+
+
+def pal(golden):
+    p = golden("palette")
+    return (p["x"], p["rgb"])
+
+
+def test_exptable_fixture(golden):
+    f = golden("exptable")
+    assert_bitexact(O.exp_op(f["arg"], False), f["op"], "exptable op")
+    assert_bitexact(O.exp_op(f["arg"], True), f["expm1"], "exptable expm1")
+    _, _, _, tl = O.exptable()
+    assert np.float32(tl) == f["taylorlimit"][0]
+
+
+def test_camera_fixture(golden):
+    f = golden("camera")
+    for i in range(int(f["nviews"][0])):
+        v = view_from(f, "v%d_" % i)
+        cam = O.camera(v)
+        assert_bitexact(np.array(cam.mat[:], np.float32), f["v%d_mat" % i], "matrix %d" % i)
+        assert cam.xshift == f["v%d_shift" % i][0] and cam.yshift == f["v%d_shift" % i][1]
+        ff = f["v%d_f" % i]
+        assert (np.float32(cam.res2), np.float32(cam.ycorr), np.float32(cam.xfac),
+                np.float32(cam.fovfct), np.float32(cam.dist)) == tuple(ff)
+
+
+@pytest.mark.parametrize("name", ["additive_small", "ordered_small", "ortho_zclip"])
+def test_project_colorize_fixture(golden, name):
+    f = golden(name)
+    kw = {}
+    if name == "ordered_small":
+        kw["brightness"] = [float(f["brightness"][0])]
+    if name == "ortho_zclip":
+        kw.update(projection=False, zmin=120.0, zmax=260.0)
+    p = O.project_colorize(field_from(f), view_from(f), [pal(golden)], **kw)
+    for k, v in p.fields().items():
+        assert_bitexact(v, f["p_" + k], "%s %s" % (name, k))
+
+
+def test_project_colorize_two_types(golden):
+    f = golden("two_types")
+    pal1 = (f["pal1_x"], f["pal1_rgb"])
+    p = O.project_colorize(field_from(f), view_from(f), [pal(golden), pal1],
+                           smooth_fac=[1.3, 0.8], brightness=[0.7, 2.0], col_vector=[0, 1],
+                           b_brightness=float(f["b_brightness"][0]), minrad_pix=1.5)
+    for k, v in p.fields().items():
+        assert_bitexact(v, f["p_" + k], "two_types " + k)
+
+
+def _proj_from_fixture(f):
+    p = O.Projected(len(f["p_x"]))
+    for k in ("x", "y", "z", "r", "I", "e0", "e1", "e2", "active"):
+        getattr(p, k)[:] = f["p_" + k]
+    return p
+
+
+@pytest.mark.parametrize("T", [50, 16, 8, 64])
+def test_render_additive_fixture(golden, T):
+    f = golden("additive_small")
+    p = _proj_from_fixture(f)
+    v = view_from(f)
+    assert_bitexact(O.render(p, v.xres, v.yres, True, 0.2, T), f["pic_T%d" % T], "T=%d" % T)
+
+
+def test_render_two_types_fixture(golden):
+    f = golden("two_types")
+    p = _proj_from_fixture(f)
+    v = view_from(f)
+    for T in (50, 24):
+        assert_bitexact(O.render(p, v.xres, v.yres, True, 0.2, T), f["pic_T%d" % T])
+
+
+def test_sort_fixture(golden):
+    f = golden("ordered_small")
+    p = _proj_from_fixture(f)
+    assert_bitexact(O.sort_perm(p, 1, stable=False), f["perm_std"], "std::sort order")
+    assert_bitexact(O.sort_perm(p, 1, stable=True), f["perm_stable"], "stable order")
+    # stable order == numpy's stable lexsort on (z, index)
+    assert_bitexact(np.argsort(p.z, kind="stable").astype(np.uint32), f["perm_stable"])
+
+
+def test_sort_types_fixture(golden):
+    f = golden("sort_types")
+    p = O.Projected(len(f["z"]))
+    p.z[:] = f["z"]; p.e0[:] = f["e0"]; p.r[:] = f["r"]
+    for st in (1, 2, 3, 4):
+        assert_bitexact(O.sort_perm(p, st, stable=False), f["perm%d" % st], "sort_type %d" % st)
+    key = {1: p.z, 2: -p.e0, 3: p.e0, 4: -p.r}
+    for st, k in key.items():
+        assert_bitexact(O.sort_perm(p, st, stable=True),
+                        np.argsort(k, kind="stable").astype(np.uint32), "stable %d" % st)
+    with pytest.raises(ValueError):
+        O.sort_perm(p, 7)
+
+
+def test_render_ordered_fixture(golden):
+    f = golden("ordered_small")
+    p = _proj_from_fixture(f)
+    v = view_from(f)
+    for T in (50, 16):
+        ps = p.take(f["perm_std"])
+        assert_bitexact(O.render(ps, v.xres, v.yres, False, 0.2, T), f["pic_T%d" % T], "T%d" % T)
+    ps = p.take(f["perm_stable"])
+    assert_bitexact(O.render(ps, v.xres, v.yres, False, 0.2, 50), f["pic_stable_T50"])
+    assert_bitexact(O.render(ps, v.xres, v.yres, False, 0.5, 50), f["pic_stable_T50_g05"])
+
+
+def test_ortho_render_fixture(golden):
+    f = golden("ortho_zclip")
+    p = _proj_from_fixture(f)
+    v = view_from(f)
+    assert_bitexact(O.render(p, v.xres, v.yres, True, 0.2, 50), f["pic_T50"])
+
+
+def test_host_rendering_fixture(golden):
+    f = golden("host_rendering")
+    v = view_from(f)
+    pic, _ = O.host_rendering(field_from(f), v, [pal(golden)], a_eq_e=False, sort_type=1,
+                              grayabsorb=0.3, tile=32, stable=False)
+    assert_bitexact(pic, f["pic"], "host_rendering")
