@@ -157,6 +157,24 @@ int splhip_bins(splhip_ctx *ctx, const splhip_particles *parts, const splhip_cam
                 uint32_t *idx, int64_t idx_cap, int64_t *n_entries, uint32_t *order_out,
                 int64_t *n_order, void *stream);
 
+/* --- stage timing ------------------------------------------------------------ */
+
+/* The reference times its stages with the tstack timers "3D transform",
+   "Particle coloring", "Particle sorting", "Host Chunk preparation" and
+   "Host Rendering proper" (splotch_host.cc:737-833, cuda/cuda_render.cu:251-285).
+   With profiling on, each splhip_render records HIP events on its stream at
+   the stage boundaries; splhip_stage_times returns the last frame's GPU time
+   per stage in ms:
+     [0] project + colorize
+     [1] ordered pass: compaction + depth sort     [4] additive pass: compaction
+     [2] ordered pass: binning                     [5] additive pass: binning
+     [3] ordered pass: splat (one k_splat launch)  [6] additive pass: splat
+   counts (optional, 4 values): active particles and bin entries of the
+   ordered pass, then of the additive pass. */
+#define SPLHIP_NSTAGES 7
+int splhip_set_profiling(splhip_ctx *ctx, int32_t enable);
+int splhip_stage_times(splhip_ctx *ctx, float *ms, int64_t *counts);
+
 /* Host-built tables (device copies are used by the kernels): tab[0..1023] =
    tab1, tab[1024..2047] = tab2; consts = {expfac, taylorlimit, h2sigma, bfak,
    rfac}.  Host only. */

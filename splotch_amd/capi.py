@@ -56,7 +56,10 @@ PARTICLE_SIM = np.dtype([("er", "<f4"), ("eg", "<f4"), ("eb", "<f4"), ("x", "<f4
 EXPORTS = ["splhip_create", "splhip_destroy", "splhip_last_error", "splhip_camera_setup",
            "splhip_params_default", "splhip_params_from_par", "splhip_set_palette",
            "splhip_render", "splhip_render_host", "splhip_project", "splhip_bins",
-           "splhip_exptable"]
+           "splhip_exptable", "splhip_set_profiling", "splhip_stage_times"]
+
+STAGES = ("project", "ordered_sort", "ordered_bin", "ordered_splat", "additive_compact",
+          "additive_bin", "additive_splat")
 
 
 def lib():
@@ -81,6 +84,8 @@ def lib():
                                   C.POINTER(Params), C.c_int32, C.c_void_p, C.c_void_p,
                                   C.c_int64, C.POINTER(C.c_int64), C.c_void_p,
                                   C.POINTER(C.c_int64), C.c_void_p]
+        L.splhip_set_profiling.argtypes = [C.c_void_p, C.c_int32]
+        L.splhip_stage_times.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -186,6 +191,18 @@ class Renderer:
         if rc != 0:
             msg = lib().splhip_last_error(self._c)
             raise SplhipError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def set_profiling(self, on: bool = True):
+        self._check(lib().splhip_set_profiling(self._c, int(on)), "set_profiling")
+
+    def stage_times(self):
+        """Last frame's per-stage GPU ms (dict) and pass counts."""
+        ms = np.zeros(len(STAGES), np.float32)
+        cnt = np.zeros(4, np.int64)
+        self._check(lib().splhip_stage_times(self._c, _np(ms), _np(cnt)), "stage_times")
+        counts = dict(ordered_active=int(cnt[0]), ordered_entries=int(cnt[1]),
+                      additive_active=int(cnt[2]), additive_entries=int(cnt[3]))
+        return dict(zip(STAGES, ms.tolist())), counts
 
     def set_palette(self, ptype: int, x, rgb):
         x = np.ascontiguousarray(x, np.float64)

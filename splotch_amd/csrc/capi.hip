@@ -34,6 +34,13 @@ struct splhip_ctx {
   size_t pal_cap = 0;
   bool pal_dirty = true;
   uint32_t *h_tot = nullptr;  // pinned
+  // stage timing (splhip_set_profiling): begin/end events per stage
+  int prof = 0;
+  hipEvent_t ev[2 * SPLHIP_NSTAGES] = {};
+  bool used[SPLHIP_NSTAGES] = {};
+  float stage_ms[SPLHIP_NSTAGES] = {};
+  int64_t last_entries[2] = {};
+  int64_t last_active[2] = {};
 };
 
 namespace {
@@ -182,6 +189,26 @@ bool par_bool(const std::map<std::string, std::string> &d, const std::string &k,
 
 // ------------------------------------------------------------------ pipeline
 
+// Stage markers (HIP events on the render stream) when profiling is enabled.
+void mark(splhip_ctx *c, int stage, bool begin, hipStream_t s) {
+  if (!c->prof) return;
+  hipEvent_t &e = c->ev[2 * stage + (begin ? 0 : 1)];
+  if (!e) hip_check(hipEventCreate(&e), "hipEventCreate");
+  hip_check(hipEventRecord(e, s), "hipEventRecord");
+  if (!begin) c->used[stage] = true;
+}
+
+void collect_marks(splhip_ctx *c) {
+  if (!c->prof) return;
+  for (int k = 0; k < SPLHIP_NSTAGES; ++k) {
+    c->stage_ms[k] = 0.f;
+    if (!c->used[k]) continue;
+    hip_check(hipEventSynchronize(c->ev[2 * k + 1]), "hipEventSynchronize");
+    hip_check(hipEventElapsedTime(&c->stage_ms[k], c->ev[2 * k], c->ev[2 * k + 1]),
+              "hipEventElapsedTime");
+  }
+}
+
 uint32_t read_u32(splhip_ctx *c, const uint32_t *d, hipStream_t s) {
   hip_check(hipMemcpyAsync(c->h_tot, d, sizeof(uint32_t), hipMemcpyDeviceToHost, s),
             "read total");
@@ -320,6 +347,9 @@ void run_pass(splhip_ctx *c, int code, int64_t n, const Rec *rec, const uint8_t 
     uint32_t *v1 = ws.get_as<uint32_t>(S_V1, m);
     order = radix_sort_pairs(k0, v0, k1, v1, m, 0, 32, ws, s) ? v1 : v0;
   }
+  const int sbase = code == 1 ? 1 : 4;
+  mark(c, sbase, false, s);
+  mark(c, sbase + 1, true, s);
   Rec *recs = ws.get_as<Rec>(S_RECS, m);
   uint32_t *cnt = ws.get_as<uint32_t>(S_CNT, m);
   uint32_t *off = ws.get_as<uint32_t>(S_OFF, m);
@@ -337,6 +367,7 @@ void run_pass(splhip_ctx *c, int code, int64_t n, const Rec *rec, const uint8_t 
   const uint32_t *keys = r ? ek1 : ek0;
   uint32_t *bin_off = ws.get_as<uint32_t>(S_REGOFF, nbins + 1);
   launch_bin_offsets(keys, e, static_cast<uint32_t>(nbins), bin_off, s);
+  mark(c, sbase + 1, false, s);
   out.e = e;
   out.nbins = static_cast<uint32_t>(nbins);
   out.order = order;
@@ -359,14 +390,21 @@ int render_impl(splhip_ctx *c, const splhip_particles *P, const splhip_camera *c
     return fail(c, SPLHIP_E_ARG, "hip_region must be 16, 32 or 64");
   Rec *rec = c->ws.get_as<Rec>(S_REC, static_cast<size_t>(n));
   uint8_t *act = c->ws.get_as<uint8_t>(S_ACT, static_cast<size_t>(n));
+  for (int k = 0; k < SPLHIP_NSTAGES; ++k) c->used[k] = false;
+  c->last_entries[0] = c->last_entries[1] = c->last_active[0] = c->last_active[1] = 0;
+  mark(c, 0, true, s);
   run_project(c, P, cam, prm, rec, act, s);
+  mark(c, 0, false, s);
   const BinGeom g = make_geom(cam, prm, G, c->consts);
   bool wrote = false;
   for (int code = 1; code <= 2; ++code) {
     if (n == 0 || !pass_present(prm, code)) continue;
     PassOut po;
+    const int sbase = code == 1 ? 1 : 4;
+    mark(c, sbase, true, s);
     run_pass(c, code, n, rec, act, prm, g, s, po);
     if (po.m == 0) continue;
+    mark(c, sbase + 2, true, s);
     SplatArgs sa{};
     sa.bin_off = po.bin_off;
     sa.ent = po.ent;
@@ -385,10 +423,14 @@ int render_impl(splhip_ctx *c, const splhip_particles *P, const splhip_camera *c
     sa.taylorlimit = c->consts.taylorlimit;
     sa.grayabsorb = prm->gray_absorption;
     launch_splat(G, code == 2, po.nbins, sa, s);
+    mark(c, sbase + 2, false, s);
+    c->last_entries[code - 1] = po.e;
+    c->last_active[code - 1] = po.m;
     wrote = true;
   }
   if (!wrote)
     launch_fill(pic, 0.f, static_cast<int64_t>(cam->xres) * cam->yres * 3, s);
+  collect_marks(c);
   return 0;
 }
 
@@ -438,10 +480,30 @@ int splhip_create(int device, splhip_ctx **out) {
   return 0;
 }
 
+int splhip_set_profiling(splhip_ctx *c, int32_t enable) {
+  if (!c) return SPLHIP_E_ARG;
+  c->prof = enable ? 1 : 0;
+  return 0;
+}
+
+int splhip_stage_times(splhip_ctx *c, float *ms, int64_t *counts) {
+  if (!c || !ms) return SPLHIP_E_ARG;
+  for (int k = 0; k < SPLHIP_NSTAGES; ++k) ms[k] = c->used[k] ? c->stage_ms[k] : 0.f;
+  if (counts) {
+    counts[0] = c->last_active[0];
+    counts[1] = c->last_entries[0];
+    counts[2] = c->last_active[1];
+    counts[3] = c->last_entries[1];
+  }
+  return 0;
+}
+
 void splhip_destroy(splhip_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   (void)hipDeviceSynchronize();
+  for (auto &e : c->ev)
+    if (e) (void)hipEventDestroy(e);
   c->ws.release();
   if (c->d_exptab) (void)hipFree(c->d_exptab);
   if (c->d_palx) (void)hipFree(c->d_palx);
