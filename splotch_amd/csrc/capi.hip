@@ -266,7 +266,7 @@ int check_inputs(splhip_ctx *c, const splhip_particles *P, const splhip_camera *
 
 int region_size(const splhip_params *prm) {
   int G = prm->hip_region;
-  if (G == 0) G = 32;
+  if (G == 0) G = 8;
   return G;
 }
 
@@ -386,8 +386,8 @@ int render_impl(splhip_ctx *c, const splhip_particles *P, const splhip_camera *c
                 const splhip_params *prm, float *pic, hipStream_t s) {
   const int64_t n = P->n;
   const int G = region_size(prm);
-  if (G != 16 && G != 32 && G != 64)
-    return fail(c, SPLHIP_E_ARG, "hip_region must be 16, 32 or 64");
+  if (G != 8 && G != 16 && G != 32 && G != 64)
+    return fail(c, SPLHIP_E_ARG, "hip_region must be 8, 16, 32 or 64");
   Rec *rec = c->ws.get_as<Rec>(S_REC, static_cast<size_t>(n));
   uint8_t *act = c->ws.get_as<uint8_t>(S_ACT, static_cast<size_t>(n));
   for (int k = 0; k < SPLHIP_NSTAGES; ++k) c->used[k] = false;
@@ -422,7 +422,29 @@ int render_impl(splhip_ctx *c, const splhip_particles *P, const splhip_camera *c
     sa.expfac = c->consts.expfac;
     sa.taylorlimit = c->consts.taylorlimit;
     sa.grayabsorb = prm->gray_absorption;
-    launch_splat(G, code == 2, po.nbins, sa, s);
+    if (G == 8) {
+      // work items, heaviest unit first; long ordered lists split by channel
+      const uint32_t split = 4096;
+      const int ordered = code == 1 ? 1 : 0;
+      const int need_empty = wrote ? 0 : 1;
+      uint32_t *tot = c->ws.get_as<uint32_t>(S_TOTALS, 64);
+      uint32_t *icnt = c->ws.get_as<uint32_t>(S_AUX1, po.nbins);
+      uint32_t *ioff = c->ws.get_as<uint32_t>(S_AUX2, po.nbins);
+      launch_items_count(po.bin_off, po.nbins, sa, ordered, split, need_empty, icnt, s);
+      scan_u32(icnt, ioff, po.nbins, tot + 2, c->ws, s);
+      const uint32_t nitems = read_u32(c, tot + 2, s);
+      uint32_t *ik0 = c->ws.get_as<uint32_t>(S_K0, nitems);
+      uint32_t *iv0 = c->ws.get_as<uint32_t>(S_V0, nitems);
+      uint32_t *ik1 = c->ws.get_as<uint32_t>(S_K1, nitems);
+      uint32_t *iv1 = c->ws.get_as<uint32_t>(S_V1, nitems);
+      launch_items_emit(po.bin_off, po.nbins, sa, ordered, split, need_empty, ioff, ik0, iv0, s);
+      const uint32_t *items =
+          radix_sort_pairs(ik0, iv0, ik1, iv1, nitems, 0, 32, c->ws, s) ? iv1 : iv0;
+      mark(c, sbase + 2, true, s);
+      launch_splat8(code == 2, items, nitems, sa, s);
+    } else {
+      launch_splat(G, code == 2, po.nbins, sa, s);
+    }
     mark(c, sbase + 2, false, s);
     c->last_entries[code - 1] = po.e;
     c->last_active[code - 1] = po.m;

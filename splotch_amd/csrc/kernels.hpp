@@ -53,6 +53,16 @@ void launch_fill(float *p, float v, int64_t n, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t s);
 void launch_aos_to_soa(const splhip_particle_sim *aos, int64_t n, float *soa /*8 planes*/,
                        uint8_t *type, hipStream_t s);
+// Wave-per-8x8-block splat (splat8.hip): work items (unit << 2 | channel,
+// channel 3 = all), heaviest first.
+void launch_items_count(const uint32_t *bin_off, uint32_t nunits, const SplatArgs &a,
+                        int ordered, uint32_t split, int need_empty, uint32_t *cnt,
+                        hipStream_t s);
+void launch_items_emit(const uint32_t *bin_off, uint32_t nunits, const SplatArgs &a, int ordered,
+                       uint32_t split, int need_empty, const uint32_t *off, uint32_t *keys,
+                       uint32_t *vals, hipStream_t s);
+void launch_splat8(bool additive, const uint32_t *items, uint32_t nitems, const SplatArgs &a,
+                   hipStream_t s);
 // G: region size (16, 32 or 64 px); additive: a_eq_e mode.
 void launch_splat(int G, bool additive, uint32_t nbins, const SplatArgs &a, hipStream_t s);
 

@@ -1,0 +1,261 @@
+// Wave-per-block splat (8 x 8 pixel blocks) with heaviest-first scheduling.
+//
+// A work unit is one 8x8 pixel block of a reference tile (blocks aligned to
+// the tile origin); its particle list holds exactly the particles whose
+// footprint box (splotch_host.cc:462-472, tile frame) overlaps the block, in
+// render order.  One wave owns one unit: lane = pixel.  Per batch of 64
+// particles, lane l tabulates particle l's 8 column ranges + Gaussian column
+// factors and 8 row factors (splotch_host.cc:596-607) into the wave's LDS
+// slice; then the wave walks the batch in order and every lane updates its
+// pixel exactly as render_new does, so each pixel sees the reference's
+// summation / compositing order and rounding.
+//
+// Scheduling: units are issued heaviest first (work items sorted by list
+// length, descending).  In the ordered mode the per-pixel chain is serial, so a
+// long list is the critical path; units with at least `split` entries are
+// issued as three items, one per colour channel (the channels' chains are
+// independent), which cuts the serial work per lane to about a third.
+#include "kernels.hpp"
+
+#include "workspace.hpp"
+
+namespace splhip {
+namespace {
+
+__device__ __forceinline__ int64_t gtid() {
+  return static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+}
+__device__ __forceinline__ int64_t gstride() {
+  return static_cast<int64_t>(gridDim.x) * blockDim.x;
+}
+
+// exptable op with the x86 conversion semantics folded in: p = arg*expfac;
+// p >= 2^31 or NaN -> INT_MIN -> 1; p in [2^20, 2^31) -> 0; p <= -1 -> negative
+// index -> 1; otherwise the table product at trunc(p).
+__device__ __forceinline__ float xop(const float *tab, float arg, float expfac) {
+  const float p = arg * expfac;
+  if (p < 1048576.f) {
+    if (p > -1.f) {
+      const int i = static_cast<int>(p);
+      return tab[i >> 10] * tab[1024 + (i & 1023)];
+    }
+    return 1.f;
+  }
+  return (p < 2147483648.f) ? 0.f : 1.f;
+}
+
+__device__ __forceinline__ float xexpm1(const float *tab, float arg, float expfac, float tl) {
+  if (fabsf(arg) < tl) return arg;
+  return xop(tab, arg, expfac) - 1.f;
+}
+
+__device__ __forceinline__ float lane_f(float v, int k) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int kPad = 65;  // LDS row stride: conflict-free column writes and reads
+
+template <int ADD>
+__global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *items,
+                                                uint32_t nitems) {
+  __shared__ float s_tab[2048];
+  __shared__ uint32_t s_col[4][8][kPad];
+  __shared__ float s_pre2[4][8][kPad];
+  __shared__ float s_pre1[4][8][kPad];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int i = tid; i < 2048; i += 256) s_tab[i] = a.exptab[i];
+  __syncthreads();
+  const uint32_t it = blockIdx.x * 4u + wave;
+  if (it >= nitems) return;
+  const uint32_t item = items[it];
+  const uint32_t u = item >> 2;
+  const int chan = static_cast<int>(item & 3u);  // 3: all channels
+  const int per_tile = a.ngm * a.ngm;
+  const int tile = static_cast<int>(u / per_tile);
+  const int rem = static_cast<int>(u - static_cast<uint32_t>(tile) * per_tile);
+  const int gy = rem / a.ngm, gx = rem - gy * a.ngm;
+  const int x0s = (tile % a.ncx) * a.T, y0s = (tile / a.ncx) * a.T;
+  const int w = min(x0s + a.T, a.xres) - x0s, h = min(y0s + a.T, a.yres) - y0s;
+  const int bxo = gx * 8, byo = gy * 8;  // tile frame
+  const int cx = lane >> 3, cy = lane & 7;
+  const int lx = bxo + cx, ly = byo + cy;
+  uint32_t(*col)[kPad] = s_col[wave];
+  float(*pre2)[kPad] = s_pre2[wave];
+  float(*pre1)[kPad] = s_pre1[wave];
+  const float expfac = a.expfac, tl = a.taylorlimit;
+
+  float l0 = 0.f, l1 = 0.f, l2 = 0.f;
+  const uint32_t e0 = a.bin_off[u], e1 = a.bin_off[u + 1];
+  for (uint32_t base = e0; base < e1; base += 64) {
+    const int nb = static_cast<int>(min<uint32_t>(64u, e1 - base));
+    // Phase A: lane = particle (splotch_host.cc:462-476 + 596-607, tile frame)
+    float va0 = 0.f, va1 = 0.f, va2 = 0.f, vq0 = 0.f, vq1 = 0.f, vq2 = 0.f, vmax = 0.f;
+    if (lane < nb) {
+      const Rec R = a.recs[a.ent[base + lane]];
+      const float rfacr = R.r * a.rfac;
+      const float posx = R.x - static_cast<float>(x0s), posy = R.y - static_cast<float>(y0s);
+      const int bx0 = max(cvt_i32(posx - rfacr + 1), 0), bx1 = min(cvt_i32(posx + rfacr + 1), w);
+      const int by0 = max(cvt_i32(posy - rfacr + 1), 0), by1 = min(cvt_i32(posy + rfacr + 1), h);
+      const float radsq = rfacr * rfacr;
+      const float sigma = a.h2sigma * R.r;
+      const float stp = -1.f / (sigma * sigma);
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int x = bxo + c;
+        uint32_t lohi = 0;
+        float p2 = 0.f;
+        if (x >= bx0 && x < bx1) {
+          const float dxsq = (static_cast<float>(x) - posx) * (static_cast<float>(x) - posx);
+          const float dy = sqrtf(radsq - dxsq);
+          const int lo = max(by0, cvt_i32(posy - dy + 1));
+          const int hi = min(by1, cvt_i32(posy + dy + 1));
+          if (hi > lo) lohi = static_cast<uint32_t>(lo) | (static_cast<uint32_t>(hi) << 16);
+          p2 = xop(s_tab, stp * dxsq, expfac);
+        }
+        col[c][lane] = lohi;
+        pre2[c][lane] = p2;
+      }
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int y = byo + r;
+        float p1 = 0.f;
+        if (y >= by0 && y < by1) {
+          const float dyf = static_cast<float>(y) - posy;
+          p1 = xop(s_tab, stp * dyf * dyf, expfac);
+        }
+        pre1[r][lane] = p1;
+      }
+      va0 = -R.e0;
+      va1 = -R.e1;
+      va2 = -R.e2;
+      if (!ADD) {
+        const float g = a.grayabsorb;
+        vq0 = R.e0 / (R.e0 + g);
+        vq1 = R.e1 / (R.e1 + g);
+        vq2 = R.e2 / (R.e2 + g);
+        vmax = fmaxf(fabsf(va0), fmaxf(fabsf(va1), fabsf(va2)));
+      }
+    }
+    wave_sync();
+    // Phase B: lane = pixel, particles in list order.
+    for (int k = 0; k < nb; ++k) {
+      const uint32_t lohi = col[cx][k];
+      if (ly >= static_cast<int>(lohi & 0xffffu) && ly < static_cast<int>(lohi >> 16)) {
+        const float att = pre1[cy][k] * pre2[cx][k];
+        if (ADD) {
+          l0 += lane_f(va0, k) * att;
+          l1 += lane_f(va1, k) * att;
+          l2 += lane_f(va2, k) * att;
+        } else {
+          const float a0 = lane_f(va0, k), a1 = lane_f(va1, k), a2 = lane_f(va2, k);
+          const float q0 = lane_f(vq0, k), q1 = lane_f(vq1, k), q2 = lane_f(vq2, k);
+          if ((lane_f(vmax, k) * att) < tl) {
+            if (chan == 3 || chan == 0) l0 += ((l0 - q0) * a0) * att;
+            if (chan == 3 || chan == 1) l1 += ((l1 - q1) * a1) * att;
+            if (chan == 3 || chan == 2) l2 += ((l2 - q2) * a2) * att;
+          } else {
+            if (chan == 3 || chan == 0) l0 += xexpm1(s_tab, att * a0, expfac, tl) * (l0 - q0);
+            if (chan == 3 || chan == 1) l1 += xexpm1(s_tab, att * a1, expfac, tl) * (l1 - q1);
+            if (chan == 3 || chan == 2) l2 += xexpm1(s_tab, att * a2, expfac, tl) * (l2 - q2);
+          }
+        }
+      }
+    }
+    wave_sync();
+  }
+  if (lx < min(bxo + 8, w) && ly < min(byo + 8, h)) {
+    float *d = a.pic + (static_cast<int64_t>(x0s + lx) * a.yres + (y0s + ly)) * 3;
+    if (chan == 3 || chan == 0) d[0] = a.accumulate ? d[0] + l0 : l0;
+    if (chan == 3 || chan == 1) d[1] = a.accumulate ? d[1] + l1 : l1;
+    if (chan == 3 || chan == 2) d[2] = a.accumulate ? d[2] + l2 : l2;
+  }
+}
+
+// Work items per unit: none for units outside their tile; none for empty units
+// when accumulating; three (one per channel) for long ordered lists; else one.
+__device__ __forceinline__ uint32_t unit_items(const uint32_t *bin_off, uint32_t u, int T,
+                                               int xres, int yres, int ncx, int ngm,
+                                               int ordered, uint32_t split, int need_empty,
+                                               uint32_t &len) {
+  len = bin_off[u + 1] - bin_off[u];
+  const int per_tile = ngm * ngm;
+  const int tile = static_cast<int>(u / per_tile);
+  const int rem = static_cast<int>(u - static_cast<uint32_t>(tile) * per_tile);
+  const int gy = rem / ngm, gx = rem - gy * ngm;
+  const int x0s = (tile % ncx) * T, y0s = (tile / ncx) * T;
+  const int w = min(x0s + T, xres) - x0s, h = min(y0s + T, yres) - y0s;
+  if (gx * 8 >= w || gy * 8 >= h) return 0;
+  if (len == 0) return need_empty ? 1u : 0u;
+  return (ordered && len >= split) ? 3u : 1u;
+}
+
+__global__ __launch_bounds__(256) void k_items_count(const uint32_t *bin_off, uint32_t nunits,
+                                                     int T, int xres, int yres, int ncx, int ngm,
+                                                     int ordered, uint32_t split, int need_empty,
+                                                     uint32_t *cnt) {
+  for (int64_t u = gtid(); u < nunits; u += gstride()) {
+    uint32_t len;
+    cnt[u] = unit_items(bin_off, static_cast<uint32_t>(u), T, xres, yres, ncx, ngm, ordered,
+                        split, need_empty, len);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_items_emit(const uint32_t *bin_off, uint32_t nunits,
+                                                    int T, int xres, int yres, int ncx, int ngm,
+                                                    int ordered, uint32_t split, int need_empty,
+                                                    const uint32_t *off, uint32_t *keys,
+                                                    uint32_t *vals) {
+  for (int64_t u = gtid(); u < nunits; u += gstride()) {
+    uint32_t len;
+    const uint32_t k = unit_items(bin_off, static_cast<uint32_t>(u), T, xres, yres, ncx, ngm,
+                                  ordered, split, need_empty, len);
+    const uint32_t o = off[u];
+    for (uint32_t c = 0; c < k; ++c) {
+      keys[o + c] = ~len;  // ascending sort of ~len = heaviest first
+      vals[o + c] = (static_cast<uint32_t>(u) << 2) | (k == 3 ? c : 3u);
+    }
+  }
+}
+
+unsigned grid_for(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  if (g < 1) g = 1;
+  if (g > (1 << 20)) g = 1 << 20;
+  return static_cast<unsigned>(g);
+}
+
+}  // namespace
+
+void launch_items_count(const uint32_t *bin_off, uint32_t nunits, const SplatArgs &a,
+                        int ordered, uint32_t split, int need_empty, uint32_t *cnt,
+                        hipStream_t s) {
+  k_items_count<<<grid_for(nunits), 256, 0, s>>>(bin_off, nunits, a.T, a.xres, a.yres, a.ncx,
+                                                 a.ngm, ordered, split, need_empty, cnt);
+  hip_check(hipGetLastError(), "k_items_count launch");
+}
+
+void launch_items_emit(const uint32_t *bin_off, uint32_t nunits, const SplatArgs &a, int ordered,
+                       uint32_t split, int need_empty, const uint32_t *off, uint32_t *keys,
+                       uint32_t *vals, hipStream_t s) {
+  k_items_emit<<<grid_for(nunits), 256, 0, s>>>(bin_off, nunits, a.T, a.xres, a.yres, a.ncx,
+                                                a.ngm, ordered, split, need_empty, off, keys,
+                                                vals);
+  hip_check(hipGetLastError(), "k_items_emit launch");
+}
+
+void launch_splat8(bool additive, const uint32_t *items, uint32_t nitems, const SplatArgs &a,
+                   hipStream_t s) {
+  if (nitems == 0) return;
+  const unsigned grid = (nitems + 3) / 4;
+  if (additive) k_splat8<1><<<grid, 256, 0, s>>>(a, items, nitems);
+  else k_splat8<0><<<grid, 256, 0, s>>>(a, items, nitems);
+  hip_check(hipGetLastError(), "k_splat8 launch");
+}
+
+}  // namespace splhip

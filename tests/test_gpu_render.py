@@ -133,7 +133,7 @@ def test_bins_ordered_mode(R, golden):
     assert_bitexact(idx, perm[oidx], "tile lists (original indices)")
 
 
-@pytest.mark.parametrize("G", [16, 32, 64])
+@pytest.mark.parametrize("G", [8, 16, 32, 64])
 @pytest.mark.parametrize("T", [50, 16, 8, 64])
 def test_render_additive_fixture(R, golden, T, G):
     import splotch_amd as S
@@ -144,7 +144,7 @@ def test_render_additive_fixture(R, golden, T, G):
     assert_bitexact(pic, f["pic_T%d" % T], "raw pic T=%d G=%d" % (T, G))
 
 
-@pytest.mark.parametrize("G", [16, 32, 64])
+@pytest.mark.parametrize("G", [8, 16, 32, 64])
 def test_render_ordered_fixture(R, golden, G):
     import splotch_amd as S
     log("case ordered G", G)
@@ -277,6 +277,21 @@ def test_deterministic(R):
     a = render(R, fld, v, [pal], S.params(a_eq_e=False))
     b = render(R, fld, v, [pal], S.params(a_eq_e=False))
     assert_bitexact(a, b, "repeat")
+
+
+@pytest.mark.parametrize("aeq", [False, True])
+def test_dense_cluster_vs_oracle(R, aeq):
+    """A dense cluster puts >10k particles into single 8x8 blocks, so the
+    ordered pass issues them as per-channel work items."""
+    import splotch_amd as S
+    log("case dense cluster a_eq_e", aeq)
+    fld = synth.plummer_field(60_000, 113, halo_frac=1.0, n_halos=1, hsml_mu=math.log(0.8))
+    v = synth.default_view(160, 144)
+    pal = synth.default_palette()
+    for G in (8, 32):
+        pic = render(R, fld, v, [pal], S.params(a_eq_e=aeq, hip_region=G))
+        opic, _ = O.host_rendering(fld, v, [pal], a_eq_e=aeq, stable=True)
+        assert_bitexact(pic, opic, "dense cluster G=%d" % G)
 
 
 def test_errors(R):
