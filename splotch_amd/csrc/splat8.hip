@@ -49,6 +49,18 @@ __device__ __forceinline__ float xexpm1(const float *tab, float arg, float expfa
   return xop(tab, arg, expfac) - 1.f;
 }
 
+// Branch-free xexpm1 (same results as xop/xexpm1): the table is always read
+// at an in-range index so that several lookups can be in flight at once.
+__device__ __forceinline__ float xexpm1_bf(const float *tab, float arg, float expfac, float tl) {
+  const float p = arg * expfac;
+  const bool hi_ok = p < 1048576.f;
+  const bool ok = hi_ok && (p > -1.f);
+  const int i = static_cast<int>(ok ? p : 0.f);
+  const float prod = tab[i >> 10] * tab[1024 + (i & 1023)];
+  const float opv = ok ? prod : ((hi_ok || !(p < 2147483648.f)) ? 1.f : 0.f);
+  return (fabsf(arg) < tl) ? arg : opv - 1.f;
+}
+
 __device__ __forceinline__ float lane_f(float v, int k) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
@@ -143,26 +155,57 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
       }
     }
     wave_sync();
-    // Phase B: lane = pixel, particles in list order.
-    for (int k = 0; k < nb; ++k) {
-      const uint32_t lohi = col[cx][k];
-      if (ly >= static_cast<int>(lohi & 0xffffu) && ly < static_cast<int>(lohi >> 16)) {
-        const float att = pre1[cy][k] * pre2[cx][k];
-        if (ADD) {
-          l0 += lane_f(va0, k) * att;
-          l1 += lane_f(va1, k) * att;
-          l2 += lane_f(va2, k) * att;
-        } else {
-          const float a0 = lane_f(va0, k), a1 = lane_f(va1, k), a2 = lane_f(va2, k);
-          const float q0 = lane_f(vq0, k), q1 = lane_f(vq1, k), q2 = lane_f(vq2, k);
-          if ((lane_f(vmax, k) * att) < tl) {
-            if (chan == 3 || chan == 0) l0 += ((l0 - q0) * a0) * att;
-            if (chan == 3 || chan == 1) l1 += ((l1 - q1) * a1) * att;
-            if (chan == 3 || chan == 2) l2 += ((l2 - q2) * a2) * att;
-          } else {
-            if (chan == 3 || chan == 0) l0 += xexpm1(s_tab, att * a0, expfac, tl) * (l0 - q0);
-            if (chan == 3 || chan == 1) l1 += xexpm1(s_tab, att * a1, expfac, tl) * (l1 - q1);
-            if (chan == 3 || chan == 2) l2 += xexpm1(s_tab, att * a2, expfac, tl) * (l2 - q2);
+    // Phase B: lane = pixel, particles in list order, 4 per step: the table
+    // reads of a step are issued together and only the update chain is serial.
+    for (int k0 = 0; k0 < nb; k0 += 4) {
+      float att[4];
+      bool in[4];
+      int kk[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        kk[j] = min(k0 + j, nb - 1);
+        const uint32_t lohi = col[cx][kk[j]];
+        in[j] = (k0 + j < nb) && ly >= static_cast<int>(lohi & 0xffffu) &&
+                ly < static_cast<int>(lohi >> 16);
+        att[j] = pre1[cy][kk[j]] * pre2[cx][kk[j]];
+      }
+      if (ADD) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const float a0 = lane_f(va0, kk[j]), a1 = lane_f(va1, kk[j]), a2 = lane_f(va2, kk[j]);
+          if (chan == 3 || chan == 0) l0 = in[j] ? l0 + a0 * att[j] : l0;
+          if (chan == 3 || chan == 1) l1 = in[j] ? l1 + a1 * att[j] : l1;
+          if (chan == 3 || chan == 2) l2 = in[j] ? l2 + a2 * att[j] : l2;
+        }
+      } else {
+        float em0[4], em1[4], em2[4];
+        bool tay[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int k = kk[j];
+          tay[j] = (lane_f(vmax, k) * att[j]) < tl;
+          if (chan == 3 || chan == 0) em0[j] = xexpm1_bf(s_tab, att[j] * lane_f(va0, k), expfac, tl);
+          if (chan == 3 || chan == 1) em1[j] = xexpm1_bf(s_tab, att[j] * lane_f(va1, k), expfac, tl);
+          if (chan == 3 || chan == 2) em2[j] = xexpm1_bf(s_tab, att[j] * lane_f(va2, k), expfac, tl);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int k = kk[j];
+          const float at = att[j];
+          if (chan == 3 || chan == 0) {
+            const float a0 = lane_f(va0, k), q0 = lane_f(vq0, k);
+            const float t = tay[j] ? ((l0 - q0) * a0) * at : em0[j] * (l0 - q0);
+            l0 = in[j] ? l0 + t : l0;
+          }
+          if (chan == 3 || chan == 1) {
+            const float a1 = lane_f(va1, k), q1 = lane_f(vq1, k);
+            const float t = tay[j] ? ((l1 - q1) * a1) * at : em1[j] * (l1 - q1);
+            l1 = in[j] ? l1 + t : l1;
+          }
+          if (chan == 3 || chan == 2) {
+            const float a2 = lane_f(va2, k), q2 = lane_f(vq2, k);
+            const float t = tay[j] ? ((l2 - q2) * a2) * at : em2[j] * (l2 - q2);
+            l2 = in[j] ? l2 + t : l2;
           }
         }
       }
@@ -192,7 +235,7 @@ __device__ __forceinline__ uint32_t unit_items(const uint32_t *bin_off, uint32_t
   const int w = min(x0s + T, xres) - x0s, h = min(y0s + T, yres) - y0s;
   if (gx * 8 >= w || gy * 8 >= h) return 0;
   if (len == 0) return need_empty ? 1u : 0u;
-  return (ordered && len >= split) ? 3u : 1u;
+  return (len >= (ordered ? split : 2 * split)) ? 3u : 1u;
 }
 
 __global__ __launch_bounds__(256) void k_items_count(const uint32_t *bin_off, uint32_t nunits,
