@@ -60,8 +60,11 @@ typedef struct {
   int32_t sort_type;                       /* sort_type (1) */
   float gray_absorption;                   /* gray_absorption (0.2) */
   int32_t tile_size;                       /* tile_size_cpu (50): reference tile */
-  int32_t hip_region;                      /* hip_region: GPU work region, px (0 = auto) */
-  int32_t hip_batch;                       /* hip_batch: particles staged per batch (0 = auto) */
+  int32_t hip_region;                      /* hip_region: GPU work region, px: 8, 16, 32, 64 (0 = auto) */
+  int32_t hip_split;                       /* hip_split: list length from which an 8x8 block is
+                                              splatted as one work item per channel (0 = auto) */
+  int32_t hip_variant;                     /* hip_variant: splat inner loop, 0 = auto, 1 = branchy,
+                                              2 = 4-particle branch-free */
 } splhip_params;
 
 /* SoA particle fields (device pointers), post particle_normalize:

@@ -424,7 +424,7 @@ int render_impl(splhip_ctx *c, const splhip_particles *P, const splhip_camera *c
     sa.grayabsorb = prm->gray_absorption;
     if (G == 8) {
       // work items, heaviest unit first; long ordered lists split by channel
-      const uint32_t split = 4096;
+      const uint32_t split = prm->hip_split > 0 ? static_cast<uint32_t>(prm->hip_split) : 4096u;
       const int ordered = code == 1 ? 1 : 0;
       const int need_empty = wrote ? 0 : 1;
       uint32_t *tot = c->ws.get_as<uint32_t>(S_TOTALS, 64);
@@ -441,7 +441,7 @@ int render_impl(splhip_ctx *c, const splhip_particles *P, const splhip_camera *c
       const uint32_t *items =
           radix_sort_pairs(ik0, iv0, ik1, iv1, nitems, 0, 32, c->ws, s) ? iv1 : iv0;
       mark(c, sbase + 2, true, s);
-      launch_splat8(code == 2, items, nitems, sa, s);
+      launch_splat8(code == 2, prm->hip_variant == 2 ? 2 : 1, items, nitems, sa, s);
     } else {
       launch_splat(G, code == 2, po.nbins, sa, s);
     }
@@ -610,7 +610,8 @@ int splhip_params_default(splhip_params *p) {
   p->gray_absorption = 0.2f;
   p->tile_size = 50;
   p->hip_region = 0;
-  p->hip_batch = 0;
+  p->hip_split = 0;
+  p->hip_variant = 0;
   return 0;
 }
 
@@ -645,7 +646,8 @@ int splhip_params_from_par(const char *par_text, float b_brightness, splhip_para
     p->gray_absorption = par_num<float>(d, "gray_absorption", 0.2f);
     p->tile_size = static_cast<int>(par_num<float>(d, "tile_size_cpu", 50.f));
     p->hip_region = par_num<int>(d, "hip_region", 0);
-    p->hip_batch = par_num<int>(d, "hip_batch", 0);
+    p->hip_split = par_num<int>(d, "hip_split", 0);
+    p->hip_variant = par_num<int>(d, "hip_variant", 0);
   } catch (const ParError &) {
     return SPLHIP_E_ARG;
   }

@@ -61,8 +61,8 @@ void launch_items_count(const uint32_t *bin_off, uint32_t nunits, const SplatArg
 void launch_items_emit(const uint32_t *bin_off, uint32_t nunits, const SplatArgs &a, int ordered,
                        uint32_t split, int need_empty, const uint32_t *off, uint32_t *keys,
                        uint32_t *vals, hipStream_t s);
-void launch_splat8(bool additive, const uint32_t *items, uint32_t nitems, const SplatArgs &a,
-                   hipStream_t s);
+void launch_splat8(bool additive, int variant, const uint32_t *items, uint32_t nitems,
+                   const SplatArgs &a, hipStream_t s);
 // G: region size (16, 32 or 64 px); additive: a_eq_e mode.
 void launch_splat(int G, bool additive, uint32_t nbins, const SplatArgs &a, hipStream_t s);
 

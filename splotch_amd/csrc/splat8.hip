@@ -73,7 +73,7 @@ __device__ __forceinline__ void wave_sync() {
 
 constexpr int kPad = 65;  // LDS row stride: conflict-free column writes and reads
 
-template <int ADD>
+template <int ADD, int VAR>
 __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *items,
                                                 uint32_t nitems) {
   __shared__ float s_tab[2048];
@@ -155,8 +155,36 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
       }
     }
     wave_sync();
-    // Phase B: lane = pixel, particles in list order, 4 per step: the table
-    // reads of a step are issued together and only the update chain is serial.
+    // Phase B: lane = pixel, particles in list order.
+    if (VAR == 1) {
+      for (int k = 0; k < nb; ++k) {
+        const uint32_t lohi = col[cx][k];
+        if (ly >= static_cast<int>(lohi & 0xffffu) && ly < static_cast<int>(lohi >> 16)) {
+          const float att = pre1[cy][k] * pre2[cx][k];
+          if (ADD) {
+            if (chan == 3 || chan == 0) l0 += lane_f(va0, k) * att;
+            if (chan == 3 || chan == 1) l1 += lane_f(va1, k) * att;
+            if (chan == 3 || chan == 2) l2 += lane_f(va2, k) * att;
+          } else {
+            const float a0 = lane_f(va0, k), a1 = lane_f(va1, k), a2 = lane_f(va2, k);
+            const float q0 = lane_f(vq0, k), q1 = lane_f(vq1, k), q2 = lane_f(vq2, k);
+            if ((lane_f(vmax, k) * att) < tl) {
+              if (chan == 3 || chan == 0) l0 += ((l0 - q0) * a0) * att;
+              if (chan == 3 || chan == 1) l1 += ((l1 - q1) * a1) * att;
+              if (chan == 3 || chan == 2) l2 += ((l2 - q2) * a2) * att;
+            } else {
+              if (chan == 3 || chan == 0) l0 += xexpm1(s_tab, att * a0, expfac, tl) * (l0 - q0);
+              if (chan == 3 || chan == 1) l1 += xexpm1(s_tab, att * a1, expfac, tl) * (l1 - q1);
+              if (chan == 3 || chan == 2) l2 += xexpm1(s_tab, att * a2, expfac, tl) * (l2 - q2);
+            }
+          }
+        }
+      }
+      wave_sync();
+      continue;
+    }
+    // 4 particles per step: the table reads of a step are issued together and
+    // only the update chain is serial.
     for (int k0 = 0; k0 < nb; k0 += 4) {
       float att[4];
       bool in[4];
@@ -292,12 +320,17 @@ void launch_items_emit(const uint32_t *bin_off, uint32_t nunits, const SplatArgs
   hip_check(hipGetLastError(), "k_items_emit launch");
 }
 
-void launch_splat8(bool additive, const uint32_t *items, uint32_t nitems, const SplatArgs &a,
-                   hipStream_t s) {
+void launch_splat8(bool additive, int variant, const uint32_t *items, uint32_t nitems,
+                   const SplatArgs &a, hipStream_t s) {
   if (nitems == 0) return;
   const unsigned grid = (nitems + 3) / 4;
-  if (additive) k_splat8<1><<<grid, 256, 0, s>>>(a, items, nitems);
-  else k_splat8<0><<<grid, 256, 0, s>>>(a, items, nitems);
+  if (variant == 2) {
+    if (additive) k_splat8<1, 2><<<grid, 256, 0, s>>>(a, items, nitems);
+    else k_splat8<0, 2><<<grid, 256, 0, s>>>(a, items, nitems);
+  } else {
+    if (additive) k_splat8<1, 1><<<grid, 256, 0, s>>>(a, items, nitems);
+    else k_splat8<0, 1><<<grid, 256, 0, s>>>(a, items, nitems);
+  }
   hip_check(hipGetLastError(), "k_splat8 launch");
 }
 
