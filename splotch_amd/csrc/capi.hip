@@ -441,7 +441,7 @@ int render_impl(splhip_ctx *c, const splhip_particles *P, const splhip_camera *c
       const uint32_t *items =
           radix_sort_pairs(ik0, iv0, ik1, iv1, nitems, 0, 32, c->ws, s) ? iv1 : iv0;
       mark(c, sbase + 2, true, s);
-      launch_splat8(code == 2, prm->hip_variant == 2 ? 2 : 1, items, nitems, sa, s);
+      launch_splat8(code == 2, prm->hip_variant > 0 ? prm->hip_variant : 3, items, nitems, sa, s);
     } else {
       launch_splat(G, code == 2, po.nbins, sa, s);
     }

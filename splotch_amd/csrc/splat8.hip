@@ -73,6 +73,106 @@ __device__ __forceinline__ void wave_sync() {
 
 constexpr int kPad = 65;  // LDS row stride: conflict-free column writes and reads
 
+// exptable::expm1 (splotchutils.h:203-214) split in two for pipelining:
+// xexp_index gives the table index for p = arg*expfac (0 where the table does
+// not apply), and xexpm1_finish combines the two table values t1, t2 fetched
+// at that index with the out-of-range and Taylor cases.
+__device__ __forceinline__ float xexpm1_finish(float arg, float p, float t1, float t2, float tl) {
+  const bool hi_ok = p < 1048576.f;
+  const bool ok = hi_ok && (p > -1.f);
+  const float opv = ok ? t1 * t2 : ((hi_ok || !(p < 2147483648.f)) ? 1.f : 0.f);
+  return (fabsf(arg) < tl) ? arg : opv - 1.f;
+}
+
+__device__ __forceinline__ int xexp_index(float p) {
+  const bool ok = (p < 1048576.f) && (p > -1.f);
+  return static_cast<int>(ok ? p : 0.f);
+}
+
+// Software-pipelined walk of one batch for one channel class CH (0..2 one
+// channel, 3 all).  Iteration k updates particle k with table values fetched in
+// iteration k-1, computes particle k+1's attenuation and issues its table
+// reads, and loads particle k+2's column mask and Gaussian factors, so that the
+// only serial dependency per step is the pixel's own update.
+template <int ADD, int CH>
+__device__ __forceinline__ void walk_batch(int nb, int cx, int cy, const uint32_t (*cmask)[kPad],
+                                           const float (*pre2)[kPad], const float (*pre1)[kPad],
+                                           const float *tab, float va0, float va1, float va2,
+                                           float vq0, float vq1, float vq2, float vmax,
+                                           float expfac, float tl, float &l0, float &l1,
+                                           float &l2) {
+  constexpr bool C0 = (CH == 3 || CH == 0), C1 = (CH == 3 || CH == 1), C2 = (CH == 3 || CH == 2);
+  // stage inputs
+  uint32_t m_n = cmask[cx][0];                               // mask of k (current)
+  float att_n = pre1[cy][0] * pre2[cx][0];                    // att of k
+  uint32_t m_nn = 0;
+  float p1_nn = 0.f, p2_nn = 0.f;
+  if (nb > 1) {
+    m_nn = cmask[cx][1];
+    p1_nn = pre1[cy][1];
+    p2_nn = pre2[cx][1];
+  }
+  float arg0 = 0.f, arg1 = 0.f, arg2 = 0.f, pp0 = 0.f, pp1 = 0.f, pp2 = 0.f;
+  float t10 = 0.f, t20 = 0.f, t11 = 0.f, t21 = 0.f, t12 = 0.f, t22 = 0.f;
+  if (!ADD) {
+    const float e_ = expfac;
+    if (C0) { arg0 = att_n * lane_f(va0, 0); pp0 = arg0 * e_; const int i = xexp_index(pp0); t10 = tab[i >> 10]; t20 = tab[1024 + (i & 1023)]; }
+    if (C1) { arg1 = att_n * lane_f(va1, 0); pp1 = arg1 * e_; const int i = xexp_index(pp1); t11 = tab[i >> 10]; t21 = tab[1024 + (i & 1023)]; }
+    if (C2) { arg2 = att_n * lane_f(va2, 0); pp2 = arg2 * e_; const int i = xexp_index(pp2); t12 = tab[i >> 10]; t22 = tab[1024 + (i & 1023)]; }
+  }
+  for (int k = 0; k < nb; ++k) {
+    // current particle k: values computed / fetched in the previous step
+    const uint32_t m = m_n;
+    const float att = att_n;
+    const float carg0 = arg0, carg1 = arg1, carg2 = arg2;
+    const float cp0 = pp0, cp1 = pp1, cp2 = pp2;
+    const float c10 = t10, c20 = t20, c11 = t11, c21 = t21, c12 = t12, c22 = t22;
+    // particle k+1: attenuation from the prefetched factors, issue its table reads
+    if (k + 1 < nb) {
+      m_n = m_nn;
+      att_n = p1_nn * p2_nn;
+      if (!ADD) {
+        if (C0) { arg0 = att_n * lane_f(va0, k + 1); pp0 = arg0 * expfac; const int i = xexp_index(pp0); t10 = tab[i >> 10]; t20 = tab[1024 + (i & 1023)]; }
+        if (C1) { arg1 = att_n * lane_f(va1, k + 1); pp1 = arg1 * expfac; const int i = xexp_index(pp1); t11 = tab[i >> 10]; t21 = tab[1024 + (i & 1023)]; }
+        if (C2) { arg2 = att_n * lane_f(va2, k + 1); pp2 = arg2 * expfac; const int i = xexp_index(pp2); t12 = tab[i >> 10]; t22 = tab[1024 + (i & 1023)]; }
+      }
+      // particle k+2: prefetch column mask and factors
+      if (k + 2 < nb) {
+        m_nn = cmask[cx][k + 2];
+        p1_nn = pre1[cy][k + 2];
+        p2_nn = pre2[cx][k + 2];
+      }
+    }
+    // update of particle k (splotch_host.cc:615 / 648-658)
+    const bool in = (m >> cy) & 1u;
+    if (ADD) {
+      if (C0) l0 = in ? l0 + lane_f(va0, k) * att : l0;
+      if (C1) l1 = in ? l1 + lane_f(va1, k) * att : l1;
+      if (C2) l2 = in ? l2 + lane_f(va2, k) * att : l2;
+    } else {
+      const bool tay = (lane_f(vmax, k) * att) < tl;
+      if (C0) {
+        const float em = xexpm1_finish(carg0, cp0, c10, c20, tl);
+        const float d = l0 - lane_f(vq0, k);
+        const float t = tay ? (d * lane_f(va0, k)) * att : em * d;
+        l0 = in ? l0 + t : l0;
+      }
+      if (C1) {
+        const float em = xexpm1_finish(carg1, cp1, c11, c21, tl);
+        const float d = l1 - lane_f(vq1, k);
+        const float t = tay ? (d * lane_f(va1, k)) * att : em * d;
+        l1 = in ? l1 + t : l1;
+      }
+      if (C2) {
+        const float em = xexpm1_finish(carg2, cp2, c12, c22, tl);
+        const float d = l2 - lane_f(vq2, k);
+        const float t = tay ? (d * lane_f(va2, k)) * att : em * d;
+        l2 = in ? l2 + t : l2;
+      }
+    }
+  }
+}
+
 template <int ADD, int VAR>
 __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *items,
                                                 uint32_t nitems) {
@@ -127,7 +227,14 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
           const float dy = sqrtf(radsq - dxsq);
           const int lo = max(by0, cvt_i32(posy - dy + 1));
           const int hi = min(by1, cvt_i32(posy + dy + 1));
-          if (hi > lo) lohi = static_cast<uint32_t>(lo) | (static_cast<uint32_t>(hi) << 16);
+          if (hi > lo) {
+            if (VAR == 3) {  // bit r: block row byo + r lies in [lo, hi)
+              const int r0 = min(max(lo - byo, 0), 8), r1 = min(max(hi - byo, 0), 8);
+              lohi = (r1 > r0) ? (((1u << r1) - 1u) & ~((1u << r0) - 1u)) : 0u;
+            } else {
+              lohi = static_cast<uint32_t>(lo) | (static_cast<uint32_t>(hi) << 16);
+            }
+          }
           p2 = xop(s_tab, stp * dxsq, expfac);
         }
         col[c][lane] = lohi;
@@ -156,6 +263,16 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
     }
     wave_sync();
     // Phase B: lane = pixel, particles in list order.
+    if (VAR == 3) {
+      switch (chan) {
+        case 0: walk_batch<ADD, 0>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, expfac, tl, l0, l1, l2); break;
+        case 1: walk_batch<ADD, 1>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, expfac, tl, l0, l1, l2); break;
+        case 2: walk_batch<ADD, 2>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, expfac, tl, l0, l1, l2); break;
+        default: walk_batch<ADD, 3>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, expfac, tl, l0, l1, l2); break;
+      }
+      wave_sync();
+      continue;
+    }
     if (VAR == 1) {
       for (int k = 0; k < nb; ++k) {
         const uint32_t lohi = col[cx][k];
@@ -324,7 +441,10 @@ void launch_splat8(bool additive, int variant, const uint32_t *items, uint32_t n
                    const SplatArgs &a, hipStream_t s) {
   if (nitems == 0) return;
   const unsigned grid = (nitems + 3) / 4;
-  if (variant == 2) {
+  if (variant == 3) {
+    if (additive) k_splat8<1, 3><<<grid, 256, 0, s>>>(a, items, nitems);
+    else k_splat8<0, 3><<<grid, 256, 0, s>>>(a, items, nitems);
+  } else if (variant == 2) {
     if (additive) k_splat8<1, 2><<<grid, 256, 0, s>>>(a, items, nitems);
     else k_splat8<0, 2><<<grid, 256, 0, s>>>(a, items, nitems);
   } else {

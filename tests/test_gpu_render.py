@@ -288,10 +288,11 @@ def test_dense_cluster_vs_oracle(R, aeq):
     fld = synth.plummer_field(60_000, 113, halo_frac=1.0, n_halos=1, hsml_mu=math.log(0.8))
     v = synth.default_view(160, 144)
     pal = synth.default_palette()
-    for G in (8, 32):
-        pic = render(R, fld, v, [pal], S.params(a_eq_e=aeq, hip_region=G))
-        opic, _ = O.host_rendering(fld, v, [pal], a_eq_e=aeq, stable=True)
-        assert_bitexact(pic, opic, "dense cluster G=%d" % G)
+    opic, _ = O.host_rendering(fld, v, [pal], a_eq_e=aeq, stable=True)
+    for G, var, split in ((8, 3, 0), (8, 1, 0), (8, 2, 0), (8, 3, 64), (8, 1, 1 << 30), (32, 0, 0)):
+        pic = render(R, fld, v, [pal], S.params(a_eq_e=aeq, hip_region=G, hip_variant=var,
+                                                hip_split=split))
+        assert_bitexact(pic, opic, "dense cluster G=%d variant=%d split=%d" % (G, var, split))
 
 
 def test_errors(R):
