@@ -422,6 +422,7 @@ int render_impl(splhip_ctx *c, const splhip_particles *P, const splhip_camera *c
     sa.expfac = c->consts.expfac;
     sa.taylorlimit = c->consts.taylorlimit;
     sa.grayabsorb = prm->gray_absorption;
+    sa.prio_len = prm->hip_split > 0 ? static_cast<uint32_t>(prm->hip_split) : 4096u;
     if (G == 8) {
       // work items, heaviest unit first; long ordered lists split by channel
       const uint32_t split = prm->hip_split > 0 ? static_cast<uint32_t>(prm->hip_split) : 4096u;

@@ -36,6 +36,7 @@ struct SplatArgs {
   int T, xres, yres, ncx, ngm;
   int accumulate;
   float rfac, h2sigma, expfac, taylorlimit, grayabsorb;
+  uint32_t prio_len;  // splat8: lists at least this long run at raised wave priority
 };
 
 void launch_project(const ProjectArgs &a, hipStream_t s);

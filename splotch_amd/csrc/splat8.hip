@@ -61,6 +61,10 @@ __device__ __forceinline__ float xexpm1_bf(const float *tab, float arg, float ex
   return (fabsf(arg) < tl) ? arg : opv - 1.f;
 }
 
+__device__ __forceinline__ uint32_t uniform(uint32_t v) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
+}
+
 __device__ __forceinline__ float lane_f(float v, int k) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
@@ -185,7 +189,9 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
   __syncthreads();
   const uint32_t it = blockIdx.x * 4u + wave;
   if (it >= nitems) return;
-  const uint32_t item = items[it];
+  // wave-uniform values are moved to scalar registers so that loop bounds and
+  // channel selection compile to scalar branches
+  const uint32_t item = uniform(items[it]);
   const uint32_t u = item >> 2;
   const int chan = static_cast<int>(item & 3u);  // 3: all channels
   const int per_tile = a.ngm * a.ngm;
@@ -203,9 +209,11 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
   const float expfac = a.expfac, tl = a.taylorlimit;
 
   float l0 = 0.f, l1 = 0.f, l2 = 0.f;
-  const uint32_t e0 = a.bin_off[u], e1 = a.bin_off[u + 1];
+  const uint32_t e0 = uniform(a.bin_off[u]), e1 = uniform(a.bin_off[u + 1]);
+  // long lists are the critical path: let their waves issue ahead of the rest
+  if (e1 - e0 >= a.prio_len) __builtin_amdgcn_s_setprio(3);
   for (uint32_t base = e0; base < e1; base += 64) {
-    const int nb = static_cast<int>(min<uint32_t>(64u, e1 - base));
+    const int nb = static_cast<int>(uniform(min<uint32_t>(64u, e1 - base)));
     // Phase A: lane = particle (splotch_host.cc:462-476 + 596-607, tile frame)
     float va0 = 0.f, va1 = 0.f, va2 = 0.f, vq0 = 0.f, vq1 = 0.f, vq2 = 0.f, vmax = 0.f;
     if (lane < nb) {
