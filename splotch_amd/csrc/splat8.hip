@@ -93,6 +93,72 @@ __device__ __forceinline__ int xexp_index(float p) {
   return static_cast<int>(ok ? p : 0.f);
 }
 
+// Chunked walk of one batch for one channel class CH (0..2 one channel, 3 all):
+// for J particles at a time, phase 1 issues every LDS read of the chunk (column
+// masks, Gaussian factors, then the exp-table entries of each fragment) and
+// phase 2 applies the J updates in list order.  No loaded value is carried
+// across chunks, so the LDS latency is paid twice per chunk, not per particle.
+template <int ADD, int CH, int J>
+__device__ __forceinline__ void walk_chunks(int nb, int cx, int cy, const uint32_t (*cmask)[kPad],
+                                            const float (*pre2)[kPad], const float (*pre1)[kPad],
+                                            const float *tab, float va0, float va1, float va2,
+                                            float vq0, float vq1, float vq2, float vmax,
+                                            float expfac, float tl, float &l0, float &l1,
+                                            float &l2) {
+  constexpr bool C0 = (CH == 3 || CH == 0), C1 = (CH == 3 || CH == 1), C2 = (CH == 3 || CH == 2);
+  for (int k0 = 0; k0 < nb; k0 += J) {
+    float att[J], g0[J], g1[J], g2[J], p0[J], p1[J], p2[J];
+    float u0[J], w0[J], u1[J], w1[J], u2[J], w2[J];
+    uint32_t msk[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int k = (k0 + j < nb) ? k0 + j : k0;  // uniform; out-of-batch slots masked below
+      msk[j] = (k0 + j < nb) ? cmask[cx][k] : 0u;
+      att[j] = pre1[cy][k] * pre2[cx][k];
+    }
+    if (!ADD) {
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const int k = (k0 + j < nb) ? k0 + j : k0;
+        if (C0) { g0[j] = att[j] * lane_f(va0, k); p0[j] = g0[j] * expfac; const int i = xexp_index(p0[j]); u0[j] = tab[i >> 10]; w0[j] = tab[1024 + (i & 1023)]; }
+        if (C1) { g1[j] = att[j] * lane_f(va1, k); p1[j] = g1[j] * expfac; const int i = xexp_index(p1[j]); u1[j] = tab[i >> 10]; w1[j] = tab[1024 + (i & 1023)]; }
+        if (C2) { g2[j] = att[j] * lane_f(va2, k); p2[j] = g2[j] * expfac; const int i = xexp_index(p2[j]); u2[j] = tab[i >> 10]; w2[j] = tab[1024 + (i & 1023)]; }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int k = (k0 + j < nb) ? k0 + j : k0;
+      const bool in = (msk[j] >> cy) & 1u;
+      const float at = att[j];
+      if (ADD) {
+        if (C0) l0 = in ? l0 + lane_f(va0, k) * at : l0;
+        if (C1) l1 = in ? l1 + lane_f(va1, k) * at : l1;
+        if (C2) l2 = in ? l2 + lane_f(va2, k) * at : l2;
+      } else {
+        const bool tay = (lane_f(vmax, k) * at) < tl;
+        if (C0) {
+          const float em = xexpm1_finish(g0[j], p0[j], u0[j], w0[j], tl);
+          const float d = l0 - lane_f(vq0, k);
+          const float t = tay ? (d * lane_f(va0, k)) * at : em * d;
+          l0 = in ? l0 + t : l0;
+        }
+        if (C1) {
+          const float em = xexpm1_finish(g1[j], p1[j], u1[j], w1[j], tl);
+          const float d = l1 - lane_f(vq1, k);
+          const float t = tay ? (d * lane_f(va1, k)) * at : em * d;
+          l1 = in ? l1 + t : l1;
+        }
+        if (C2) {
+          const float em = xexpm1_finish(g2[j], p2[j], u2[j], w2[j], tl);
+          const float d = l2 - lane_f(vq2, k);
+          const float t = tay ? (d * lane_f(va2, k)) * at : em * d;
+          l2 = in ? l2 + t : l2;
+        }
+      }
+    }
+  }
+}
+
 // Software-pipelined walk of one batch for one channel class CH (0..2 one
 // channel, 3 all).  Iteration k updates particle k with table values fetched in
 // iteration k-1, computes particle k+1's attenuation and issues its table
@@ -236,7 +302,7 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
           const int lo = max(by0, cvt_i32(posy - dy + 1));
           const int hi = min(by1, cvt_i32(posy + dy + 1));
           if (hi > lo) {
-            if (VAR == 3) {  // bit r: block row byo + r lies in [lo, hi)
+            if (VAR >= 3) {  // bit r: block row byo + r lies in [lo, hi)
               const int r0 = min(max(lo - byo, 0), 8), r1 = min(max(hi - byo, 0), 8);
               lohi = (r1 > r0) ? (((1u << r1) - 1u) & ~((1u << r0) - 1u)) : 0u;
             } else {
@@ -271,6 +337,16 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
     }
     wave_sync();
     // Phase B: lane = pixel, particles in list order.
+    if (VAR == 4) {
+      switch (chan) {
+        case 0: walk_chunks<ADD, 0, 8>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, expfac, tl, l0, l1, l2); break;
+        case 1: walk_chunks<ADD, 1, 8>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, expfac, tl, l0, l1, l2); break;
+        case 2: walk_chunks<ADD, 2, 8>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, expfac, tl, l0, l1, l2); break;
+        default: walk_chunks<ADD, 3, (ADD ? 8 : 4)>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, expfac, tl, l0, l1, l2); break;
+      }
+      wave_sync();
+      continue;
+    }
     if (VAR == 3) {
       switch (chan) {
         case 0: walk_batch<ADD, 0>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, expfac, tl, l0, l1, l2); break;
@@ -449,7 +525,10 @@ void launch_splat8(bool additive, int variant, const uint32_t *items, uint32_t n
                    const SplatArgs &a, hipStream_t s) {
   if (nitems == 0) return;
   const unsigned grid = (nitems + 3) / 4;
-  if (variant == 3) {
+  if (variant == 4) {
+    if (additive) k_splat8<1, 4><<<grid, 256, 0, s>>>(a, items, nitems);
+    else k_splat8<0, 4><<<grid, 256, 0, s>>>(a, items, nitems);
+  } else if (variant == 3) {
     if (additive) k_splat8<1, 3><<<grid, 256, 0, s>>>(a, items, nitems);
     else k_splat8<0, 3><<<grid, 256, 0, s>>>(a, items, nitems);
   } else if (variant == 2) {
