@@ -178,6 +178,15 @@ int splhip_bins(splhip_ctx *ctx, const splhip_particles *parts, const splhip_cam
 int splhip_set_profiling(splhip_ctx *ctx, int32_t enable);
 int splhip_stage_times(splhip_ctx *ctx, float *ms, int64_t *counts);
 
+/* Work-item trace of the 8x8-block splat (no reference counterpart; the
+   reference's tstack timers stop at stage granularity).  With tracing on, each
+   render records per work item {start, end} (s_memrealtime, 100 MHz), the list
+   length | channel << 32 (channel 3 = all) and the hardware CU id;
+   splhip_trace_items copies the last frame's items of one pass (4 uint64 each;
+   call with out=NULL to get the count in *n). */
+int splhip_set_trace(splhip_ctx *ctx, int32_t enable);
+int splhip_trace_items(splhip_ctx *ctx, int32_t ordered, uint64_t *out, int64_t cap, int64_t *n);
+
 /* Host-built tables (device copies are used by the kernels): tab[0..1023] =
    tab1, tab[1024..2047] = tab2; consts = {expfac, taylorlimit, h2sigma, bfak,
    rfac}.  Host only. */

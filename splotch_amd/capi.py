@@ -57,7 +57,8 @@ PARTICLE_SIM = np.dtype([("er", "<f4"), ("eg", "<f4"), ("eb", "<f4"), ("x", "<f4
 EXPORTS = ["splhip_create", "splhip_destroy", "splhip_last_error", "splhip_camera_setup",
            "splhip_params_default", "splhip_params_from_par", "splhip_set_palette",
            "splhip_render", "splhip_render_host", "splhip_project", "splhip_bins",
-           "splhip_exptable", "splhip_set_profiling", "splhip_stage_times"]
+           "splhip_exptable", "splhip_set_profiling", "splhip_stage_times", "splhip_set_trace",
+           "splhip_trace_items"]
 
 STAGES = ("project", "ordered_sort", "ordered_bin", "ordered_splat", "additive_compact",
           "additive_bin", "additive_splat")
@@ -205,6 +206,20 @@ class Renderer:
         counts = dict(ordered_active=int(cnt[0]), ordered_entries=int(cnt[1]),
                       additive_active=int(cnt[2]), additive_entries=int(cnt[3]))
         return dict(zip(STAGES, ms.tolist())), counts
+
+    def set_trace(self, on: bool = True):
+        self._check(lib().splhip_set_trace(self._c, C.c_int32(int(on))), "set_trace")
+
+    def trace_items(self, ordered: bool = True):
+        """Last frame's splat work items: array (n, 4) of start, end (100 MHz
+        ticks), list length | channel << 32, CU id."""
+        n = C.c_int64()
+        L = lib()
+        L.splhip_trace_items(self._c, C.c_int32(int(ordered)), None, C.c_int64(0), C.byref(n))
+        out = np.zeros((max(n.value, 1), 4), np.uint64)
+        L.splhip_trace_items(self._c, C.c_int32(int(ordered)), _np(out), C.c_int64(n.value),
+                             C.byref(n))
+        return out[:n.value]
 
     def set_palette(self, ptype: int, x, rgb):
         x = np.ascontiguousarray(x, np.float64)

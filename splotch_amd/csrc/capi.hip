@@ -41,6 +41,9 @@ struct splhip_ctx {
   float stage_ms[SPLHIP_NSTAGES] = {};
   int64_t last_entries[2] = {};
   int64_t last_active[2] = {};
+  // work-item trace of the 8x8 splat (splhip_set_trace), per pass
+  int trace = 0;
+  std::vector<uint64_t> trace_items[2];
 };
 
 namespace {
@@ -441,8 +444,16 @@ int render_impl(splhip_ctx *c, const splhip_particles *P, const splhip_camera *c
       launch_items_emit(po.bin_off, po.nbins, sa, ordered, split, need_empty, ioff, ik0, iv0, s);
       const uint32_t *items =
           radix_sort_pairs(ik0, iv0, ik1, iv1, nitems, 0, 32, c->ws, s) ? iv1 : iv0;
+      if (c->trace) sa.trace = c->ws.get_as<uint64_t>(S_AUX3, static_cast<size_t>(nitems) * 4);
       mark(c, sbase + 2, true, s);
-      launch_splat8(code == 2, prm->hip_variant > 0 ? prm->hip_variant : 3, items, nitems, sa, s);
+      launch_splat8(code == 2, prm->hip_variant > 0 ? prm->hip_variant : 4, items, nitems, sa, s);
+      if (c->trace) {
+        std::vector<uint64_t> &tv = c->trace_items[code - 1];
+        tv.resize(static_cast<size_t>(nitems) * 4);
+        hip_check(hipMemcpyAsync(tv.data(), sa.trace, tv.size() * 8, hipMemcpyDeviceToHost, s),
+                  "D2H trace");
+        hip_check(hipStreamSynchronize(s), "sync trace");
+      }
     } else {
       launch_splat(G, code == 2, po.nbins, sa, s);
     }
@@ -506,6 +517,20 @@ int splhip_create(int device, splhip_ctx **out) {
 int splhip_set_profiling(splhip_ctx *c, int32_t enable) {
   if (!c) return SPLHIP_E_ARG;
   c->prof = enable ? 1 : 0;
+  return 0;
+}
+
+int splhip_set_trace(splhip_ctx *c, int32_t enable) {
+  if (!c) return SPLHIP_E_ARG;
+  c->trace = enable ? 1 : 0;
+  return 0;
+}
+
+int splhip_trace_items(splhip_ctx *c, int32_t ordered, uint64_t *out, int64_t cap, int64_t *n) {
+  if (!c || !n) return SPLHIP_E_ARG;
+  const std::vector<uint64_t> &tv = c->trace_items[ordered ? 0 : 1];
+  *n = static_cast<int64_t>(tv.size() / 4);
+  if (out) std::copy(tv.begin(), tv.begin() + std::min<size_t>(tv.size(), cap * 4), out);
   return 0;
 }
 

@@ -37,6 +37,7 @@ struct SplatArgs {
   int accumulate;
   float rfac, h2sigma, expfac, taylorlimit, grayabsorb;
   uint32_t prio_len;  // splat8: lists at least this long run at raised wave priority
+  uint64_t *trace;    // splat8: optional per-item {start, end, len | chan << 32, hw id}
 };
 
 void launch_project(const ProjectArgs &a, hipStream_t s);

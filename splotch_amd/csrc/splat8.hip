@@ -255,6 +255,7 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
   __syncthreads();
   const uint32_t it = blockIdx.x * 4u + wave;
   if (it >= nitems) return;
+  const uint64_t t_start = a.trace ? __builtin_amdgcn_s_memrealtime() : 0;
   // wave-uniform values are moved to scalar registers so that loop bounds and
   // channel selection compile to scalar branches
   const uint32_t item = uniform(items[it]);
@@ -440,6 +441,13 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
       }
     }
     wave_sync();
+  }
+  if (a.trace && lane == 0) {
+    uint64_t *tr = a.trace + 4 * static_cast<uint64_t>(it);
+    tr[0] = t_start;
+    tr[1] = __builtin_amdgcn_s_memrealtime();
+    tr[2] = static_cast<uint64_t>(e1 - e0) | (static_cast<uint64_t>(chan) << 32);
+    tr[3] = static_cast<uint64_t>(__smid());
   }
   if (lx < min(bxo + 8, w) && ly < min(byo + 8, h)) {
     float *d = a.pic + (static_cast<int64_t>(x0s + lx) * a.yres + (y0s + ly)) * 3;
