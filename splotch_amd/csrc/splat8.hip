@@ -98,6 +98,10 @@ __device__ __forceinline__ int xexp_index(float p) {
 // masks, Gaussian factors, then the exp-table entries of each fragment) and
 // phase 2 applies the J updates in list order.  No loaded value is carried
 // across chunks, so the LDS latency is paid twice per chunk, not per particle.
+// The code is branch-free: loads past the batch end read a valid slot and are
+// masked, and a pixel outside a footprint adds +0.  The accumulators start at
+// +0 and a sum of floats is -0 only if both terms are -0, so they never hold
+// -0 and l + 0 == l bit for bit: the result equals the reference's skip.
 template <int ADD, int CH, int J>
 __device__ __forceinline__ void walk_chunks(int nb, int cx, int cy, const uint32_t (*cmask)[kPad],
                                             const float (*pre2)[kPad], const float (*pre1)[kPad],
@@ -112,14 +116,15 @@ __device__ __forceinline__ void walk_chunks(int nb, int cx, int cy, const uint32
     uint32_t msk[J];
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-      const int k = (k0 + j < nb) ? k0 + j : k0;  // uniform; out-of-batch slots masked below
-      msk[j] = (k0 + j < nb) ? cmask[cx][k] : 0u;
+      const int k = min(k0 + j, nb - 1);  // uniform; out-of-batch slots masked below
+      const uint32_t mk = cmask[cx][k];
+      msk[j] = (k0 + j < nb) ? mk : 0u;
       att[j] = pre1[cy][k] * pre2[cx][k];
     }
     if (!ADD) {
 #pragma unroll
       for (int j = 0; j < J; ++j) {
-        const int k = (k0 + j < nb) ? k0 + j : k0;
+        const int k = min(k0 + j, nb - 1);
         if (C0) { g0[j] = att[j] * lane_f(va0, k); p0[j] = g0[j] * expfac; const int i = xexp_index(p0[j]); u0[j] = tab[i >> 10]; w0[j] = tab[1024 + (i & 1023)]; }
         if (C1) { g1[j] = att[j] * lane_f(va1, k); p1[j] = g1[j] * expfac; const int i = xexp_index(p1[j]); u1[j] = tab[i >> 10]; w1[j] = tab[1024 + (i & 1023)]; }
         if (C2) { g2[j] = att[j] * lane_f(va2, k); p2[j] = g2[j] * expfac; const int i = xexp_index(p2[j]); u2[j] = tab[i >> 10]; w2[j] = tab[1024 + (i & 1023)]; }
@@ -127,32 +132,32 @@ __device__ __forceinline__ void walk_chunks(int nb, int cx, int cy, const uint32
     }
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-      const int k = (k0 + j < nb) ? k0 + j : k0;
+      const int k = min(k0 + j, nb - 1);
       const bool in = (msk[j] >> cy) & 1u;
       const float at = att[j];
       if (ADD) {
-        if (C0) l0 = in ? l0 + lane_f(va0, k) * at : l0;
-        if (C1) l1 = in ? l1 + lane_f(va1, k) * at : l1;
-        if (C2) l2 = in ? l2 + lane_f(va2, k) * at : l2;
+        if (C0) { const float t = lane_f(va0, k) * at; l0 = l0 + (in ? t : 0.f); }
+        if (C1) { const float t = lane_f(va1, k) * at; l1 = l1 + (in ? t : 0.f); }
+        if (C2) { const float t = lane_f(va2, k) * at; l2 = l2 + (in ? t : 0.f); }
       } else {
         const bool tay = (lane_f(vmax, k) * at) < tl;
         if (C0) {
           const float em = xexpm1_finish(g0[j], p0[j], u0[j], w0[j], tl);
           const float d = l0 - lane_f(vq0, k);
-          const float t = tay ? (d * lane_f(va0, k)) * at : em * d;
-          l0 = in ? l0 + t : l0;
+          const float ta = (d * lane_f(va0, k)) * at, te = em * d;
+          l0 = l0 + (in ? (tay ? ta : te) : 0.f);
         }
         if (C1) {
           const float em = xexpm1_finish(g1[j], p1[j], u1[j], w1[j], tl);
           const float d = l1 - lane_f(vq1, k);
-          const float t = tay ? (d * lane_f(va1, k)) * at : em * d;
-          l1 = in ? l1 + t : l1;
+          const float ta = (d * lane_f(va1, k)) * at, te = em * d;
+          l1 = l1 + (in ? (tay ? ta : te) : 0.f);
         }
         if (C2) {
           const float em = xexpm1_finish(g2[j], p2[j], u2[j], w2[j], tl);
           const float d = l2 - lane_f(vq2, k);
-          const float t = tay ? (d * lane_f(va2, k)) * at : em * d;
-          l2 = in ? l2 + t : l2;
+          const float ta = (d * lane_f(va2, k)) * at, te = em * d;
+          l2 = l2 + (in ? (tay ? ta : te) : 0.f);
         }
       }
     }
