@@ -63,8 +63,12 @@ typedef struct {
   int32_t hip_region;                      /* hip_region: GPU work region, px: 8, 16, 32, 64 (0 = auto) */
   int32_t hip_split;                       /* hip_split: list length from which an 8x8 block is
                                               splatted as one work item per channel (0 = auto) */
-  int32_t hip_variant;                     /* hip_variant: splat inner loop, 0 = auto, 1 = branchy,
-                                              2 = 4-particle branch-free */
+  int32_t hip_variant;                     /* hip_variant: 8x8 splat inner loop, 0 = auto (4),
+                                              1 = per particle, 2 = 4-particle groups,
+                                              3 = software pipelined, 4 = chunked branch-free */
+  int32_t hip_heavy;                       /* hip_heavy: list length from which an 8x8 block is
+                                              splatted by a producer/consumer workgroup
+                                              (0 = auto, < 0 = never) */
 } splhip_params;
 
 /* SoA particle fields (device pointers), post particle_normalize:

@@ -44,6 +44,9 @@ struct splhip_ctx {
   // work-item trace of the 8x8 splat (splhip_set_trace), per pass
   int trace = 0;
   std::vector<uint64_t> trace_items[2];
+  // auxiliary stream (default priority) for the heavy-block kernel
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace {
@@ -210,6 +213,13 @@ void collect_marks(splhip_ctx *c) {
     hip_check(hipEventElapsedTime(&c->stage_ms[k], c->ev[2 * k], c->ev[2 * k + 1]),
               "hipEventElapsedTime");
   }
+}
+
+void ensure_aux(splhip_ctx *c) {
+  if (c->aux) return;
+  hip_check(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking), "hipStreamCreate");
+  hip_check(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "hipEventCreate");
+  hip_check(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming), "hipEventCreate");
 }
 
 uint32_t read_u32(splhip_ctx *c, const uint32_t *d, hipStream_t s) {
@@ -431,6 +441,9 @@ int render_impl(splhip_ctx *c, const splhip_particles *P, const splhip_camera *c
       const uint32_t split = prm->hip_split > 0 ? static_cast<uint32_t>(prm->hip_split) : 4096u;
       const int ordered = code == 1 ? 1 : 0;
       const int need_empty = wrote ? 0 : 1;
+      // lists at least this long get a producer/consumer workgroup (0 = auto, < 0 = never)
+      sa.heavy_len = prm->hip_heavy < 0 ? 0xffffffffu
+                     : prm->hip_heavy > 0 ? static_cast<uint32_t>(prm->hip_heavy) : 32768u;
       uint32_t *tot = c->ws.get_as<uint32_t>(S_TOTALS, 64);
       uint32_t *icnt = c->ws.get_as<uint32_t>(S_AUX1, po.nbins);
       uint32_t *ioff = c->ws.get_as<uint32_t>(S_AUX2, po.nbins);
@@ -442,11 +455,27 @@ int render_impl(splhip_ctx *c, const splhip_particles *P, const splhip_camera *c
       uint32_t *ik1 = c->ws.get_as<uint32_t>(S_K1, nitems);
       uint32_t *iv1 = c->ws.get_as<uint32_t>(S_V1, nitems);
       launch_items_emit(po.bin_off, po.nbins, sa, ordered, split, need_empty, ioff, ik0, iv0, s);
-      const uint32_t *items =
-          radix_sort_pairs(ik0, iv0, ik1, iv1, nitems, 0, 32, c->ws, s) ? iv1 : iv0;
+      const int rs = radix_sort_pairs(ik0, iv0, ik1, iv1, nitems, 0, 32, c->ws, s);
+      const uint32_t *items = rs ? iv1 : iv0;
+      launch_count_heavy(rs ? ik1 : ik0, nitems, sa.heavy_len, tot + 3, s);
+      const uint32_t nheavy = read_u32(c, tot + 3, s);
       if (c->trace) sa.trace = c->ws.get_as<uint64_t>(S_AUX3, static_cast<size_t>(nitems) * 4);
       mark(c, sbase + 2, true, s);
-      launch_splat8(code == 2, prm->hip_variant > 0 ? prm->hip_variant : 4, items, nitems, sa, s);
+      SplatArgs sn = sa;  // normal items follow the heavy ones
+      if (sn.trace) sn.trace += 4 * static_cast<size_t>(nheavy);
+      const int variant = prm->hip_variant > 0 ? prm->hip_variant : 4;
+      if (nheavy > 0) {
+        // heavy blocks run on a second stream, beside the normal items
+        ensure_aux(c);
+        hip_check(hipEventRecord(c->ev_fork, s), "record fork");
+        hip_check(hipStreamWaitEvent(c->aux, c->ev_fork, 0), "wait fork");
+        launch_splat_heavy(code == 2, items, nheavy, sa, c->aux);
+        hip_check(hipEventRecord(c->ev_join, c->aux), "record join");
+        launch_splat8(code == 2, variant, items + nheavy, nitems - nheavy, sn, s);
+        hip_check(hipStreamWaitEvent(s, c->ev_join, 0), "wait join");
+      } else {
+        launch_splat8(code == 2, variant, items, nitems, sn, s);
+      }
       if (c->trace) {
         std::vector<uint64_t> &tv = c->trace_items[code - 1];
         tv.resize(static_cast<size_t>(nitems) * 4);
@@ -552,6 +581,9 @@ void splhip_destroy(splhip_ctx *c) {
   (void)hipDeviceSynchronize();
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->aux) (void)hipStreamDestroy(c->aux);
   c->ws.release();
   if (c->d_exptab) (void)hipFree(c->d_exptab);
   if (c->d_palx) (void)hipFree(c->d_palx);
@@ -638,6 +670,7 @@ int splhip_params_default(splhip_params *p) {
   p->hip_region = 0;
   p->hip_split = 0;
   p->hip_variant = 0;
+  p->hip_heavy = 0;
   return 0;
 }
 
@@ -674,6 +707,7 @@ int splhip_params_from_par(const char *par_text, float b_brightness, splhip_para
     p->hip_region = par_num<int>(d, "hip_region", 0);
     p->hip_split = par_num<int>(d, "hip_split", 0);
     p->hip_variant = par_num<int>(d, "hip_variant", 0);
+    p->hip_heavy = par_num<int>(d, "hip_heavy", 0);
   } catch (const ParError &) {
     return SPLHIP_E_ARG;
   }

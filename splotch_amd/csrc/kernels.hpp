@@ -38,6 +38,7 @@ struct SplatArgs {
   float rfac, h2sigma, expfac, taylorlimit, grayabsorb;
   uint32_t prio_len;  // splat8: lists at least this long run at raised wave priority
   uint64_t *trace;    // splat8: optional per-item {start, end, len | chan << 32, hw id}
+  uint32_t heavy_len; // splat8: lists at least this long go to the producer/consumer kernel
 };
 
 void launch_project(const ProjectArgs &a, hipStream_t s);
@@ -65,6 +66,10 @@ void launch_items_emit(const uint32_t *bin_off, uint32_t nunits, const SplatArgs
                        uint32_t *vals, hipStream_t s);
 void launch_splat8(bool additive, int variant, const uint32_t *items, uint32_t nitems,
                    const SplatArgs &a, hipStream_t s);
+void launch_count_heavy(const uint32_t *sorted_keys, uint32_t nitems, uint32_t heavy,
+                        uint32_t *out, hipStream_t s);
+void launch_splat_heavy(bool additive, const uint32_t *items, uint32_t nheavy, const SplatArgs &a,
+                        hipStream_t s);
 // G: region size (16, 32 or 64 px); additive: a_eq_e mode.
 void launch_splat(int G, bool additive, uint32_t nbins, const SplatArgs &a, hipStream_t s);
 

@@ -462,12 +462,188 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
   }
 }
 
+// ---------------------------------------------------------------------------
+// Heavy blocks: producer / consumer workgroup (16 waves) per 8x8 block.
+//
+// A block whose list holds hundreds of thousands of particles is the critical
+// path of the frame: its pixels' update chains are serial.  Here the serial
+// part is all a consumer wave does: waves 0..2 own channel 0..2 of the block's
+// 64 pixels and, per particle, apply the reference update from values in LDS.
+// Waves 3..15 (832 lanes) produce those values one batch ahead: the particle
+// set-up two batches ahead (tile-frame position, box, Gaussian width,
+// emission / absorption, splotch_host.cc:462-476 + 591-632), then per
+// (particle, pixel) fragment the column range test, the two separable
+// Gaussian factors and, in ordered mode, exptable::expm1 of each channel
+// (splotch_host.cc:596-607, 640-658) - the same operations, in the same order,
+// as render_new, so every pixel's value is bit-identical.
+constexpr int kHB = 32;                // particles per batch
+constexpr int kHWaves = 16;
+constexpr int kHProd = (kHWaves - 3) * 64;
+
+struct HeavyPart {
+  float posx, posy, radsq, stp;
+  int bx0, bx1, by0, by1;
+};
+
+template <int ADD>
+__global__ __launch_bounds__(1024) void k_splat_heavy(SplatArgs a, const uint32_t *items,
+                                                      uint32_t nheavy) {
+  __shared__ float s_tab[2048];
+  __shared__ HeavyPart s_part[3][kHB];
+  __shared__ float4 s_cst[3][3][kHB];  // [slot][channel][k] = (a, q, maxa, 0)
+  __shared__ float s_at[2][kHB][64];   // attenuation, or -1 outside the footprint
+  __shared__ float s_em[ADD ? 1 : 2][3][kHB][64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const bool producer = wave >= 3;
+  const int ptid = tid - 192;
+  for (int i = tid; i < 2048; i += 1024) s_tab[i] = a.exptab[i];
+  const uint32_t it = blockIdx.x;
+  const uint64_t t_start = a.trace ? __builtin_amdgcn_s_memrealtime() : 0;
+  const uint32_t u = uniform(items[it]) >> 2;
+  const int per_tile = a.ngm * a.ngm;
+  const int tile = static_cast<int>(u / per_tile);
+  const int rem = static_cast<int>(u - static_cast<uint32_t>(tile) * per_tile);
+  const int gy = rem / a.ngm, gx = rem - gy * a.ngm;
+  const int x0s = (tile % a.ncx) * a.T, y0s = (tile / a.ncx) * a.T;
+  const int w = min(x0s + a.T, a.xres) - x0s, h = min(y0s + a.T, a.yres) - y0s;
+  const int bxo = gx * 8, byo = gy * 8;
+  const uint32_t e0 = uniform(a.bin_off[u]), e1 = uniform(a.bin_off[u + 1]);
+  const int n = static_cast<int>(e1 - e0);
+  const int nbat = (n + kHB - 1) / kHB;
+  const float expfac = a.expfac, tl = a.taylorlimit;
+  __builtin_amdgcn_s_setprio(3);
+
+  auto setup = [&](int b) {  // particle set-up of batch b into slot b % 3
+    const int slot = b % 3;
+    const int cnt = min(kHB, n - b * kHB);
+    if (ptid < cnt) {
+      const Rec R = a.recs[a.ent[e0 + b * kHB + ptid]];
+      const float rfacr = R.r * a.rfac;
+      const float posx = R.x - static_cast<float>(x0s), posy = R.y - static_cast<float>(y0s);
+      HeavyPart P;
+      P.posx = posx;
+      P.posy = posy;
+      P.bx0 = max(cvt_i32(posx - rfacr + 1), 0);
+      P.bx1 = min(cvt_i32(posx + rfacr + 1), w);
+      P.by0 = max(cvt_i32(posy - rfacr + 1), 0);
+      P.by1 = min(cvt_i32(posy + rfacr + 1), h);
+      P.radsq = rfacr * rfacr;
+      const float sigma = a.h2sigma * R.r;
+      P.stp = -1.f / (sigma * sigma);
+      s_part[slot][ptid] = P;
+      const float a0 = -R.e0, a1 = -R.e1, a2 = -R.e2;
+      float q0 = 0.f, q1 = 0.f, q2 = 0.f, mx = 0.f;
+      if (!ADD) {
+        const float g = a.grayabsorb;
+        q0 = R.e0 / (R.e0 + g);
+        q1 = R.e1 / (R.e1 + g);
+        q2 = R.e2 / (R.e2 + g);
+        mx = fmaxf(fabsf(a0), fmaxf(fabsf(a1), fabsf(a2)));
+      }
+      s_cst[slot][0][ptid] = make_float4(a0, q0, mx, 0.f);
+      s_cst[slot][1][ptid] = make_float4(a1, q1, mx, 0.f);
+      s_cst[slot][2][ptid] = make_float4(a2, q2, mx, 0.f);
+    }
+  };
+  auto fragments = [&](int b) {  // fragment values of batch b into buffer b & 1
+    const int slot = b % 3, buf = b & 1;
+    const int cnt = min(kHB, n - b * kHB);
+    for (int f = ptid; f < cnt * 64; f += kHProd) {
+      const int k = f >> 6, p = f & 63;
+      const HeavyPart P = s_part[slot][k];
+      const int x = bxo + (p >> 3), y = byo + (p & 7);
+      float att = -1.f;
+      if (x >= P.bx0 && x < P.bx1) {
+        const float dxsq = (static_cast<float>(x) - P.posx) * (static_cast<float>(x) - P.posx);
+        const float dy = sqrtf(P.radsq - dxsq);
+        const int lo = max(P.by0, cvt_i32(P.posy - dy + 1));
+        const int hi = min(P.by1, cvt_i32(P.posy + dy + 1));
+        if (y >= lo && y < hi) {
+          const float pre2 = xop(s_tab, P.stp * dxsq, expfac);
+          const float dyf = static_cast<float>(y) - P.posy;
+          const float pre1 = xop(s_tab, P.stp * dyf * dyf, expfac);
+          att = pre1 * pre2;
+        }
+      }
+      s_at[buf][k][p] = att;
+      if (!ADD) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const float ac = s_cst[slot][c][k].x;
+          s_em[ADD ? 0 : buf][c][k][p] = (att >= 0.f) ? xexpm1(s_tab, att * ac, expfac, tl) : 0.f;
+        }
+      }
+    }
+  };
+
+  if (producer) {
+    if (nbat > 0) setup(0);
+    if (nbat > 1) setup(1);
+  }
+  __syncthreads();
+  if (producer && nbat > 0) fragments(0);
+  __syncthreads();
+  float l = 0.f;  // consumer: channel `wave`, pixel `lane`
+  for (int b = 0; b < nbat; ++b) {
+    if (producer) {
+      if (b + 2 < nbat) setup(b + 2);
+      if (b + 1 < nbat) fragments(b + 1);
+    } else {
+      const int slot = b % 3, buf = b & 1, c = wave;
+      const int cnt = min(kHB, n - b * kHB);
+      const float *at_b = &s_at[buf][0][0];
+      const float *em_b = &s_em[ADD ? 0 : buf][ADD ? 0 : c][0][0];
+      const float4 *cst = s_cst[slot][c];
+      for (int k0 = 0; k0 < cnt; k0 += 8) {
+        float at[8], em[8];
+        float4 cs[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int k = min(k0 + j, cnt - 1);
+          const float v = at_b[k * 64 + lane];
+          at[j] = (k0 + j < cnt) ? v : -1.f;
+          if (!ADD) em[j] = em_b[k * 64 + lane];
+          cs[j] = cst[k];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const bool in = at[j] >= 0.f;
+          if (ADD) {
+            const float t = cs[j].x * at[j];
+            l = l + (in ? t : 0.f);
+          } else {
+            const bool tay = (cs[j].z * at[j]) < tl;
+            const float d = l - cs[j].y;
+            const float ta = (d * cs[j].x) * at[j], te = em[j] * d;
+            l = l + (in ? (tay ? ta : te) : 0.f);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (a.trace && tid == 0) {
+    uint64_t *tr = a.trace + 4 * static_cast<uint64_t>(it);
+    tr[0] = t_start;
+    tr[1] = __builtin_amdgcn_s_memrealtime();
+    tr[2] = static_cast<uint64_t>(n) | (uint64_t(3) << 32);
+    tr[3] = static_cast<uint64_t>(__smid());
+  }
+  if (!producer) {
+    const int lx = bxo + (lane >> 3), ly = byo + (lane & 7);
+    if (lx < min(bxo + 8, w) && ly < min(byo + 8, h)) {
+      float *d = a.pic + (static_cast<int64_t>(x0s + lx) * a.yres + (y0s + ly)) * 3 + wave;
+      *d = a.accumulate ? *d + l : l;
+    }
+  }
+}
+
 // Work items per unit: none for units outside their tile; none for empty units
 // when accumulating; three (one per channel) for long ordered lists; else one.
 __device__ __forceinline__ uint32_t unit_items(const uint32_t *bin_off, uint32_t u, int T,
                                                int xres, int yres, int ncx, int ngm,
-                                               int ordered, uint32_t split, int need_empty,
-                                               uint32_t &len) {
+                                               int ordered, uint32_t split, uint32_t heavy,
+                                               int need_empty, uint32_t &len) {
   len = bin_off[u + 1] - bin_off[u];
   const int per_tile = ngm * ngm;
   const int tile = static_cast<int>(u / per_tile);
@@ -477,35 +653,49 @@ __device__ __forceinline__ uint32_t unit_items(const uint32_t *bin_off, uint32_t
   const int w = min(x0s + T, xres) - x0s, h = min(y0s + T, yres) - y0s;
   if (gx * 8 >= w || gy * 8 >= h) return 0;
   if (len == 0) return need_empty ? 1u : 0u;
+  if (len >= heavy) return 1u;  // one producer/consumer workgroup for all channels
   return (len >= (ordered ? split : 2 * split)) ? 3u : 1u;
 }
 
 __global__ __launch_bounds__(256) void k_items_count(const uint32_t *bin_off, uint32_t nunits,
                                                      int T, int xres, int yres, int ncx, int ngm,
-                                                     int ordered, uint32_t split, int need_empty,
+                                                     int ordered, uint32_t split, uint32_t heavy, int need_empty,
                                                      uint32_t *cnt) {
   for (int64_t u = gtid(); u < nunits; u += gstride()) {
     uint32_t len;
     cnt[u] = unit_items(bin_off, static_cast<uint32_t>(u), T, xres, yres, ncx, ngm, ordered,
-                        split, need_empty, len);
+                        split, heavy, need_empty, len);
   }
 }
 
 __global__ __launch_bounds__(256) void k_items_emit(const uint32_t *bin_off, uint32_t nunits,
                                                     int T, int xres, int yres, int ncx, int ngm,
-                                                    int ordered, uint32_t split, int need_empty,
+                                                    int ordered, uint32_t split, uint32_t heavy, int need_empty,
                                                     const uint32_t *off, uint32_t *keys,
                                                     uint32_t *vals) {
   for (int64_t u = gtid(); u < nunits; u += gstride()) {
     uint32_t len;
     const uint32_t k = unit_items(bin_off, static_cast<uint32_t>(u), T, xres, yres, ncx, ngm,
-                                  ordered, split, need_empty, len);
+                                  ordered, split, heavy, need_empty, len);
     const uint32_t o = off[u];
     for (uint32_t c = 0; c < k; ++c) {
       keys[o + c] = ~len;  // ascending sort of ~len = heaviest first
       vals[o + c] = (static_cast<uint32_t>(u) << 2) | (k == 3 ? c : 3u);
     }
   }
+}
+
+// Number of heavy items at the front of the sorted item keys (keys = ~len,
+// ascending): the count of keys <= ~heavy.
+__global__ void k_count_heavy(const uint32_t *keys, uint32_t n, uint32_t heavy, uint32_t *out) {
+  const uint32_t thr = ~heavy;
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = lo + ((hi - lo) >> 1);
+    if (keys[mid] <= thr) lo = mid + 1;
+    else hi = mid;
+  }
+  *out = lo;
 }
 
 unsigned grid_for(int64_t n) {
@@ -521,7 +711,7 @@ void launch_items_count(const uint32_t *bin_off, uint32_t nunits, const SplatArg
                         int ordered, uint32_t split, int need_empty, uint32_t *cnt,
                         hipStream_t s) {
   k_items_count<<<grid_for(nunits), 256, 0, s>>>(bin_off, nunits, a.T, a.xres, a.yres, a.ncx,
-                                                 a.ngm, ordered, split, need_empty, cnt);
+                                                 a.ngm, ordered, split, a.heavy_len, need_empty, cnt);
   hip_check(hipGetLastError(), "k_items_count launch");
 }
 
@@ -529,9 +719,23 @@ void launch_items_emit(const uint32_t *bin_off, uint32_t nunits, const SplatArgs
                        uint32_t split, int need_empty, const uint32_t *off, uint32_t *keys,
                        uint32_t *vals, hipStream_t s) {
   k_items_emit<<<grid_for(nunits), 256, 0, s>>>(bin_off, nunits, a.T, a.xres, a.yres, a.ncx,
-                                                a.ngm, ordered, split, need_empty, off, keys,
+                                                a.ngm, ordered, split, a.heavy_len, need_empty, off, keys,
                                                 vals);
   hip_check(hipGetLastError(), "k_items_emit launch");
+}
+
+void launch_count_heavy(const uint32_t *sorted_keys, uint32_t nitems, uint32_t heavy,
+                        uint32_t *out, hipStream_t s) {
+  k_count_heavy<<<1, 1, 0, s>>>(sorted_keys, nitems, heavy, out);
+  hip_check(hipGetLastError(), "k_count_heavy launch");
+}
+
+void launch_splat_heavy(bool additive, const uint32_t *items, uint32_t nheavy, const SplatArgs &a,
+                        hipStream_t s) {
+  if (nheavy == 0) return;
+  if (additive) k_splat_heavy<1><<<nheavy, 1024, 0, s>>>(a, items, nheavy);
+  else k_splat_heavy<0><<<nheavy, 1024, 0, s>>>(a, items, nheavy);
+  hip_check(hipGetLastError(), "k_splat_heavy launch");
 }
 
 void launch_splat8(bool additive, int variant, const uint32_t *items, uint32_t nitems,

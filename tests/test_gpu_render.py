@@ -289,10 +289,26 @@ def test_dense_cluster_vs_oracle(R, aeq):
     v = synth.default_view(160, 144)
     pal = synth.default_palette()
     opic, _ = O.host_rendering(fld, v, [pal], a_eq_e=aeq, stable=True)
-    for G, var, split in ((8, 3, 0), (8, 1, 0), (8, 2, 0), (8, 3, 64), (8, 1, 1 << 30), (32, 0, 0)):
+    for G, var, split, heavy in ((8, 3, 0, 0), (8, 1, 0, -1), (8, 2, 0, -1), (8, 3, 64, -1),
+                                 (8, 1, 1 << 30, -1), (8, 4, 0, -1), (8, 4, 64, 200),
+                                 (8, 4, 0, 1), (32, 0, 0, 0)):
         pic = render(R, fld, v, [pal], S.params(a_eq_e=aeq, hip_region=G, hip_variant=var,
-                                                hip_split=split))
-        assert_bitexact(pic, opic, "dense cluster G=%d variant=%d split=%d" % (G, var, split))
+                                                hip_split=split, hip_heavy=heavy))
+        assert_bitexact(pic, opic, "dense cluster G=%d variant=%d split=%d heavy=%d"
+                        % (G, var, split, heavy))
+
+
+@pytest.mark.parametrize("aeq", [False, True])
+def test_heavy_blocks_on_fixtures(R, golden, aeq):
+    """Every non-empty block through the producer/consumer kernel."""
+    import splotch_amd as S
+    name = "additive_small" if aeq else "ordered_small"
+    f = golden(name)
+    kw = {} if aeq else dict(brightness=[float(f["brightness"][0])])
+    pic = render(R, field_from(f), view_from(f), [palette(golden)],
+                 S.params(a_eq_e=aeq, hip_heavy=1, **kw))
+    want = f["pic_T50"] if aeq else f["pic_stable_T50"]
+    assert_bitexact(pic, want, "heavy kernel " + name)
 
 
 def test_errors(R):
