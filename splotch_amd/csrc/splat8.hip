@@ -468,15 +468,20 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
 // A block whose list holds hundreds of thousands of particles is the critical
 // path of the frame: its pixels' update chains are serial.  Here the serial
 // part is all a consumer wave does: waves 0..2 own channel 0..2 of the block's
-// 64 pixels and, per particle, apply the reference update from values in LDS.
-// Waves 3..15 (832 lanes) produce those values one batch ahead: the particle
-// set-up two batches ahead (tile-frame position, box, Gaussian width,
-// emission / absorption, splotch_host.cc:462-476 + 591-632), then per
-// (particle, pixel) fragment the column range test, the two separable
-// Gaussian factors and, in ordered mode, exptable::expm1 of each channel
-// (splotch_host.cc:596-607, 640-658) - the same operations, in the same order,
-// as render_new, so every pixel's value is bit-identical.
-constexpr int kHB = 32;                // particles per batch
+// 64 pixels and, per particle, apply
+//     l += (l - q) * m1 * m2       (two roundings, left to right)
+// with multipliers prepared by the producers so that this reproduces
+// render_new's update bit for bit (splotch_host.cc:615, 648-658):
+//   Taylor branch  m1 = a,  m2 = att     ->  ((l - q) * a) * att
+//   expm1 branch   m1 = expm1(att * a), m2 = 1  ->  (l - q) * em   (x*1 == x)
+//   additive       l += a * m2, m2 = att
+//   outside        m2 = -1: no update.
+// Waves 3..15 (832 lanes) run a three-stage pipeline ahead of the consumers:
+// particle set-up three batches ahead (tile-frame position, box, Gaussian
+// width, emission / absorption; splotch_host.cc:462-476, 591-632), the
+// separable column / row tables two batches ahead (splotch_host.cc:596-607),
+// and the per-fragment multipliers one batch ahead.
+constexpr int kHB = 32;  // particles per batch
 constexpr int kHWaves = 16;
 constexpr int kHProd = (kHWaves - 3) * 64;
 
@@ -489,10 +494,13 @@ template <int ADD>
 __global__ __launch_bounds__(1024) void k_splat_heavy(SplatArgs a, const uint32_t *items,
                                                       uint32_t nheavy) {
   __shared__ float s_tab[2048];
-  __shared__ HeavyPart s_part[3][kHB];
-  __shared__ float4 s_cst[3][3][kHB];  // [slot][channel][k] = (a, q, maxa, 0)
-  __shared__ float s_at[2][kHB][64];   // attenuation, or -1 outside the footprint
-  __shared__ float s_em[ADD ? 1 : 2][3][kHB][64];
+  __shared__ HeavyPart s_part[4][kHB];
+  __shared__ float4 s_cst[4][3][kHB];  // [slot][channel][k] = (a, q, maxa, 0)
+  __shared__ uint32_t s_cm[2][kHB][8];  // column row-masks
+  __shared__ float s_p2[2][kHB][8];     // column factors
+  __shared__ float s_p1[2][kHB][8];     // row factors
+  __shared__ float s_m2[2][kHB][64];
+  __shared__ float s_m1[ADD ? 1 : 2][ADD ? 1 : 3][ADD ? 1 : kHB][64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const bool producer = wave >= 3;
   const int ptid = tid - 192;
@@ -513,10 +521,10 @@ __global__ __launch_bounds__(1024) void k_splat_heavy(SplatArgs a, const uint32_
   const float expfac = a.expfac, tl = a.taylorlimit;
   __builtin_amdgcn_s_setprio(3);
 
-  auto setup = [&](int b) {  // particle set-up of batch b into slot b % 3
-    const int slot = b % 3;
-    const int cnt = min(kHB, n - b * kHB);
-    if (ptid < cnt) {
+  auto count = [&](int b) { return min(kHB, n - b * kHB); };
+  auto setup = [&](int b) {  // particle b-batch set-up into slot b % 4
+    const int slot = b & 3;
+    if (ptid < count(b)) {
       const Rec R = a.recs[a.ent[e0 + b * kHB + ptid]];
       const float rfacr = R.r * a.rfac;
       const float posx = R.x - static_cast<float>(x0s), posy = R.y - static_cast<float>(y0s);
@@ -545,77 +553,91 @@ __global__ __launch_bounds__(1024) void k_splat_heavy(SplatArgs a, const uint32_
       s_cst[slot][2][ptid] = make_float4(a2, q2, mx, 0.f);
     }
   };
-  auto fragments = [&](int b) {  // fragment values of batch b into buffer b & 1
-    const int slot = b % 3, buf = b & 1;
-    const int cnt = min(kHB, n - b * kHB);
-    for (int f = ptid; f < cnt * 64; f += kHProd) {
-      const int k = f >> 6, p = f & 63;
+  auto tables = [&](int b) {  // column ranges / factors and row factors into b & 1
+    const int slot = b & 3, ts = b & 1;
+    if (ptid < count(b) * 16) {
+      const int k = ptid >> 4, j = ptid & 15;
       const HeavyPart P = s_part[slot][k];
-      const int x = bxo + (p >> 3), y = byo + (p & 7);
-      float att = -1.f;
-      if (x >= P.bx0 && x < P.bx1) {
-        const float dxsq = (static_cast<float>(x) - P.posx) * (static_cast<float>(x) - P.posx);
-        const float dy = sqrtf(P.radsq - dxsq);
-        const int lo = max(P.by0, cvt_i32(P.posy - dy + 1));
-        const int hi = min(P.by1, cvt_i32(P.posy + dy + 1));
-        if (y >= lo && y < hi) {
-          const float pre2 = xop(s_tab, P.stp * dxsq, expfac);
+      if (j < 8) {
+        const int x = bxo + j;
+        uint32_t m = 0;
+        float p2 = 0.f;
+        if (x >= P.bx0 && x < P.bx1) {
+          const float dxsq = (static_cast<float>(x) - P.posx) * (static_cast<float>(x) - P.posx);
+          const float dy = sqrtf(P.radsq - dxsq);
+          const int lo = max(P.by0, cvt_i32(P.posy - dy + 1));
+          const int hi = min(P.by1, cvt_i32(P.posy + dy + 1));
+          const int r0 = min(max(lo - byo, 0), 8), r1 = min(max(hi - byo, 0), 8);
+          if (hi > lo && r1 > r0) m = ((1u << r1) - 1u) & ~((1u << r0) - 1u);
+          p2 = xop(s_tab, P.stp * dxsq, expfac);
+        }
+        s_cm[ts][k][j] = m;
+        s_p2[ts][k][j] = p2;
+      } else {
+        const int y = byo + (j - 8);
+        float p1 = 0.f;
+        if (y >= P.by0 && y < P.by1) {
           const float dyf = static_cast<float>(y) - P.posy;
-          const float pre1 = xop(s_tab, P.stp * dyf * dyf, expfac);
-          att = pre1 * pre2;
+          p1 = xop(s_tab, P.stp * dyf * dyf, expfac);
         }
+        s_p1[ts][k][j - 8] = p1;
       }
-      s_at[buf][k][p] = att;
-      if (!ADD) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const float ac = s_cst[slot][c][k].x;
-          s_em[ADD ? 0 : buf][c][k][p] = (att >= 0.f) ? xexpm1(s_tab, att * ac, expfac, tl) : 0.f;
-        }
+    }
+  };
+  auto fragments = [&](int b) {  // per-fragment multipliers into b & 1
+    const int slot = b & 3, ts = b & 1;
+    const int cnt = count(b);
+    for (int f = ptid; f < cnt * 64; f += kHProd) {
+      const int k = f >> 6, p = f & 63, c = p >> 3, r = p & 7;
+      const bool in = (s_cm[ts][k][c] >> r) & 1u;
+      const float att = s_p1[ts][k][r] * s_p2[ts][k][c];
+      if (ADD) {
+        s_m2[ts][k][p] = in ? att : -1.f;
+      } else {
+        const float4 c0 = s_cst[slot][0][k], c1 = s_cst[slot][1][k], c2 = s_cst[slot][2][k];
+        const bool tay = (c0.z * att) < tl;
+        s_m2[ts][k][p] = in ? (tay ? att : 1.f) : -1.f;
+        const float e0v = xexpm1_bf(s_tab, att * c0.x, expfac, tl);
+        const float e1v = xexpm1_bf(s_tab, att * c1.x, expfac, tl);
+        const float e2v = xexpm1_bf(s_tab, att * c2.x, expfac, tl);
+        s_m1[ADD ? 0 : ts][0][k][p] = tay ? c0.x : e0v;
+        s_m1[ADD ? 0 : ts][1][k][p] = tay ? c1.x : e1v;
+        s_m1[ADD ? 0 : ts][2][k][p] = tay ? c2.x : e2v;
       }
     }
   };
 
-  if (producer) {
-    if (nbat > 0) setup(0);
-    if (nbat > 1) setup(1);
-  }
-  __syncthreads();
-  if (producer && nbat > 0) fragments(0);
-  __syncthreads();
   float l = 0.f;  // consumer: channel `wave`, pixel `lane`
-  for (int b = 0; b < nbat; ++b) {
+  for (int s = -3; s < nbat; ++s) {
     if (producer) {
-      if (b + 2 < nbat) setup(b + 2);
-      if (b + 1 < nbat) fragments(b + 1);
-    } else {
-      const int slot = b % 3, buf = b & 1, c = wave;
-      const int cnt = min(kHB, n - b * kHB);
-      const float *at_b = &s_at[buf][0][0];
-      const float *em_b = &s_em[ADD ? 0 : buf][ADD ? 0 : c][0][0];
+      if (s + 3 < nbat) setup(s + 3);
+      if (s + 2 >= 0 && s + 2 < nbat) tables(s + 2);
+      if (s + 1 >= 0 && s + 1 < nbat) fragments(s + 1);
+    } else if (s >= 0) {
+      const int slot = s & 3, fb = s & 1, c = wave;
+      const int cnt = count(s);
+      const float *m2b = &s_m2[fb][0][0];
+      const float *m1b = &s_m1[ADD ? 0 : fb][ADD ? 0 : c][0][0];
       const float4 *cst = s_cst[slot][c];
       for (int k0 = 0; k0 < cnt; k0 += 8) {
-        float at[8], em[8];
-        float4 cs[8];
+        float m1[8], m2[8], cv[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
           const int k = min(k0 + j, cnt - 1);
-          const float v = at_b[k * 64 + lane];
-          at[j] = (k0 + j < cnt) ? v : -1.f;
-          if (!ADD) em[j] = em_b[k * 64 + lane];
-          cs[j] = cst[k];
+          const float v = m2b[k * 64 + lane];
+          m2[j] = (k0 + j < cnt) ? v : -1.f;
+          if (!ADD) m1[j] = m1b[k * 64 + lane];
+          cv[j] = ADD ? cst[k].x : cst[k].y;  // a (additive) or q
         }
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-          const bool in = at[j] >= 0.f;
+          const bool in = m2[j] >= 0.f;
           if (ADD) {
-            const float t = cs[j].x * at[j];
+            const float t = cv[j] * m2[j];
             l = l + (in ? t : 0.f);
           } else {
-            const bool tay = (cs[j].z * at[j]) < tl;
-            const float d = l - cs[j].y;
-            const float ta = (d * cs[j].x) * at[j], te = em[j] * d;
-            l = l + (in ? (tay ? ta : te) : 0.f);
+            const float t = ((l - cv[j]) * m1[j]) * m2[j];
+            l = l + (in ? t : 0.f);
           }
         }
       }
