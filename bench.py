@@ -118,6 +118,50 @@ def cpu_baseline(field, view, pal, budget_s=20.0):
             "seconds": round(t, 3)}
 
 
+def bench_batched(S, R, torch, steps, warmup, n_jobs=64):
+    """Batched workload: n_jobs views (sizes, cameras and modes of
+    synth.batch_jobs) rendered as one batch, and the same views one
+    splhip_render call after another.  Aggregate Mparticles/s of each."""
+    from splotch_amd import synth
+    jobs = synth.batch_jobs(n_jobs)
+    views = []
+    for bj in jobs:
+        dev = S.to_device(bj.field)
+        cam = S.camera_setup(bj.view)
+        prm = S.params(a_eq_e=bj.additive)
+        pic = torch.empty(bj.view.xres * bj.view.yres * 3, dtype=torch.float32, device="cuda")
+        views.append((dev, cam, prm, pic))
+    ntot = sum(bj.field.n for bj in jobs)
+
+    def plain():
+        for dev, cam, prm, pic in views:
+            R.render(dev, cam, prm, pic)
+
+    def batched():
+        R.render_batched(views)
+
+    res = {}
+    for name, fn in (("plain", plain), ("batched", batched)):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        res[name] = (ms, ntot / (ms * 1e-3) / 1e6)
+    log("batched %d views, %d particles: plain %.2f ms, batched %.2f ms" %
+        (n_jobs, ntot, res["plain"][0], res["batched"][0]))
+    return {"views": n_jobs, "particles": ntot,
+            "sides": sorted({bj.view.xres for bj in jobs}),
+            "ms_per_batch": round(res["batched"][0], 3),
+            "mparticles_s": round(res["batched"][1], 3),
+            "plain_ms_per_batch": round(res["plain"][0], 3),
+            "plain_mparticles_s": round(res["plain"][1], 3),
+            "batched_vs_plain": round(res["batched"][1] / res["plain"][1], 3)}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +173,7 @@ def main():
     ap.add_argument("--region", type=int, default=0, help="hip_region (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched workload")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -223,6 +268,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
                      "bytes_per_launch": splat_bytes, "launch_ms": splat_ms},
     }
+    if not args.no_batched:
+        out["batched"] = bench_batched(S, R, torch, max(2, args.steps // 2), 1)
     if not args.no_cpu:
         del dev
         torch.cuda.empty_cache()

@@ -144,6 +144,27 @@ int splhip_render_host(splhip_ctx *ctx, const splhip_particle_sim *host_particle
                        const splhip_camera *cam, const splhip_params *prm, float *host_pic,
                        void *stream);
 
+/* --- batched render ----------------------------------------------------------- */
+
+/* One view of a batch: its particles (device SoA), camera, render keys and
+   output image (device, xres*yres*3 floats). */
+typedef struct {
+  splhip_particles parts;
+  splhip_camera cam;
+  splhip_params prm;
+  float *pic;
+} splhip_job;
+
+/* Many views in one pipeline (no reference counterpart as one call: it is the
+   reference's per-frame host_rendering / cuda_rendering (splotch.cc:248-264,
+   scenemaker.cc:599-673 frame loop) applied to njobs frames).  Every image is
+   bit-identical to splhip_render of that job alone.  All jobs share the
+   context's palettes; the hip_* tuning keys are taken from jobs[0].prm, and
+   batches use 8 px work regions.  Stages run once over all jobs (one launch
+   per stage, heaviest work items of all views first); the host waits three
+   times per batch for sizes. */
+int splhip_render_batched(splhip_ctx *ctx, int32_t njobs, const splhip_job *jobs, void *stream);
+
 /* --- parity hooks ----------------------------------------------------------- */
 
 /* particle_project + particle_colorize only.  rec_out: device, n*8 floats per

@@ -49,6 +49,11 @@ class Particles(C.Structure):
                 ("c2", C.c_void_p), ("type", C.c_void_p), ("n", C.c_int64)]
 
 
+class Job(C.Structure):
+    """splhip_job: one view of a batch."""
+    _fields_ = [("parts", Particles), ("cam", Camera), ("prm", Params), ("pic", C.c_void_p)]
+
+
 # numpy dtype of the reference particle_sim record (splotch/splotchutils.h:52-72)
 PARTICLE_SIM = np.dtype([("er", "<f4"), ("eg", "<f4"), ("eb", "<f4"), ("x", "<f4"),
                          ("y", "<f4"), ("z", "<f4"), ("r", "<f4"), ("I", "<f4"),
@@ -58,7 +63,7 @@ EXPORTS = ["splhip_create", "splhip_destroy", "splhip_last_error", "splhip_camer
            "splhip_params_default", "splhip_params_from_par", "splhip_set_palette",
            "splhip_render", "splhip_render_host", "splhip_project", "splhip_bins",
            "splhip_exptable", "splhip_set_profiling", "splhip_stage_times", "splhip_set_trace",
-           "splhip_trace_items"]
+           "splhip_trace_items", "splhip_render_batched"]
 
 STAGES = ("project", "ordered_sort", "ordered_bin", "ordered_splat", "additive_compact",
           "additive_bin", "additive_splat")
@@ -86,6 +91,7 @@ def lib():
                                   C.POINTER(Params), C.c_int32, C.c_void_p, C.c_void_p,
                                   C.c_int64, C.POINTER(C.c_int64), C.c_void_p,
                                   C.POINTER(C.c_int64), C.c_void_p]
+        L.splhip_render_batched.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
         L.splhip_set_profiling.argtypes = [C.c_void_p, C.c_int32]
         L.splhip_stage_times.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
@@ -257,6 +263,21 @@ class Renderer:
                                         C.c_void_p(pic.data_ptr()), self._stream(stream)),
                     "splhip_render")
         return pic
+
+    def render_batched(self, views, stream=None):
+        """views: list of (dev, cam, prm, pic) as for render(); all images in
+        one pipeline.  Returns the list of images."""
+        jobs = (Job * max(len(views), 1))()
+        for j, (dev, cam, prm, pic) in enumerate(views):
+            assert pic.is_contiguous() and pic.numel() == cam.xres * cam.yres * 3
+            jobs[j].parts = self.particles(dev)
+            jobs[j].cam = cam
+            jobs[j].prm = prm
+            jobs[j].pic = pic.data_ptr()
+        self._check(lib().splhip_render_batched(self._c, C.c_int32(len(views)),
+                                                C.cast(jobs, C.c_void_p),
+                                                self._stream(stream)), "splhip_render_batched")
+        return [v[3] for v in views]
 
     def render_host(self, aos: np.ndarray, cam: Camera, prm: Params, stream=None):
         """cuda_rendering semantics: host particle_sim records in, host image out."""

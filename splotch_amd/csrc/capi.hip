@@ -33,10 +33,20 @@ struct splhip_ctx {
   int pal_stride = 0;
   size_t pal_cap = 0;
   bool pal_dirty = true;
-  uint32_t *h_tot = nullptr;  // pinned
-  // stage timing (splhip_set_profiling): begin/end events per stage
+  uint32_t *h_tot = nullptr;  // pinned, device totals read back
+  char *h_stage = nullptr;    // pinned, table uploads and offset read-backs
+  size_t h_stage_cap = 0;
+  hipEvent_t ev_stage = nullptr;  // last copy that used h_stage
+  // stage timing (splhip_set_profiling): begin/end event pairs; a stage may
+  // consist of several intervals of the stream
   int prof = 0;
-  hipEvent_t ev[2 * SPLHIP_NSTAGES] = {};
+  struct Mark {
+    int stage;
+    hipEvent_t b, e;
+  };
+  std::vector<Mark> marks;
+  std::vector<hipEvent_t> ev_pool;
+  int nmarks = 0;
   bool used[SPLHIP_NSTAGES] = {};
   float stage_ms[SPLHIP_NSTAGES] = {};
   int64_t last_entries[2] = {};
@@ -44,6 +54,7 @@ struct splhip_ctx {
   // work-item trace of the 8x8 splat (splhip_set_trace), per pass
   int trace = 0;
   std::vector<uint64_t> trace_items[2];
+  std::vector<uint32_t> h_mo;
   // auxiliary stream (default priority) for the heavy-block kernel
   hipStream_t aux = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -198,20 +209,39 @@ bool par_bool(const std::map<std::string, std::string> &d, const std::string &k,
 // Stage markers (HIP events on the render stream) when profiling is enabled.
 void mark(splhip_ctx *c, int stage, bool begin, hipStream_t s) {
   if (!c->prof) return;
-  hipEvent_t &e = c->ev[2 * stage + (begin ? 0 : 1)];
-  if (!e) hip_check(hipEventCreate(&e), "hipEventCreate");
-  hip_check(hipEventRecord(e, s), "hipEventRecord");
-  if (!begin) c->used[stage] = true;
+  const size_t need = 2 * static_cast<size_t>(c->nmarks + 1);
+  while (c->ev_pool.size() < need) {
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    c->ev_pool.push_back(e);
+  }
+  if (begin) {
+    if (c->marks.size() < static_cast<size_t>(c->nmarks + 1)) c->marks.resize(c->nmarks + 1);
+    splhip_ctx::Mark &m = c->marks[c->nmarks];
+    m.stage = stage;
+    m.b = c->ev_pool[2 * c->nmarks];
+    m.e = c->ev_pool[2 * c->nmarks + 1];
+    hip_check(hipEventRecord(m.b, s), "hipEventRecord");
+    ++c->nmarks;
+  } else {
+    for (int k = c->nmarks - 1; k >= 0; --k)
+      if (c->marks[k].stage == stage) {
+        hip_check(hipEventRecord(c->marks[k].e, s), "hipEventRecord");
+        break;
+      }
+    c->used[stage] = true;
+  }
 }
 
 void collect_marks(splhip_ctx *c) {
   if (!c->prof) return;
-  for (int k = 0; k < SPLHIP_NSTAGES; ++k) {
-    c->stage_ms[k] = 0.f;
-    if (!c->used[k]) continue;
-    hip_check(hipEventSynchronize(c->ev[2 * k + 1]), "hipEventSynchronize");
-    hip_check(hipEventElapsedTime(&c->stage_ms[k], c->ev[2 * k], c->ev[2 * k + 1]),
-              "hipEventElapsedTime");
+  for (int k = 0; k < SPLHIP_NSTAGES; ++k) c->stage_ms[k] = 0.f;
+  for (int k = 0; k < c->nmarks; ++k) {
+    const splhip_ctx::Mark &m = c->marks[k];
+    float ms = 0.f;
+    hip_check(hipEventSynchronize(m.e), "hipEventSynchronize");
+    hip_check(hipEventElapsedTime(&ms, m.b, m.e), "hipEventElapsedTime");
+    c->stage_ms[m.stage] += ms;
   }
 }
 
@@ -222,12 +252,6 @@ void ensure_aux(splhip_ctx *c) {
   hip_check(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming), "hipEventCreate");
 }
 
-uint32_t read_u32(splhip_ctx *c, const uint32_t *d, hipStream_t s) {
-  hip_check(hipMemcpyAsync(c->h_tot, d, sizeof(uint32_t), hipMemcpyDeviceToHost, s),
-            "read total");
-  hip_check(hipStreamSynchronize(s), "sync total");
-  return c->h_tot[0];
-}
 
 void upload_palettes(splhip_ctx *c) {
   if (!c->pal_dirty) return;
@@ -300,199 +324,457 @@ BinGeom make_geom(const splhip_camera *cam, const splhip_params *prm, int G, con
   return g;
 }
 
-void run_project(splhip_ctx *c, const splhip_particles *P, const splhip_camera *cam,
-                 const splhip_params *prm, Rec *rec, uint8_t *act, hipStream_t s) {
-  upload_palettes(c);
-  ProjectArgs a{};
-  a.x = P->x; a.y = P->y; a.z = P->z; a.hsml = P->hsml; a.inten = P->inten; a.col = P->col;
-  a.c1 = P->c1; a.c2 = P->c2; a.type = P->type; a.n = P->n;
-  a.cam = *cam;
-  for (int t = 0; t < kMaxTypes; ++t) {
-    a.smooth_fac[t] = prm->smooth_fac[t];
-    a.brightness[t] = prm->brightness[t];
-    a.col_vector[t] = prm->col_vector[t];
-    a.pass_code[t] = prm->a_eq_e[t] ? 2 : 1;
-    a.pal_n[t] = static_cast<int32_t>(c->pal_x[t].size());
-  }
-  a.pal_stride = c->pal_stride;
-  a.ntypes = prm->ptypes;
-  a.pal_x = c->d_palx;
-  a.pal_rgb = c->d_palrgb;
-  a.bfak = c->consts.bfak;
-  a.rfac = c->consts.rfac;
-  a.rec = rec;
-  a.act = act;
-  if (P->n > 0) launch_project(a, s);
-}
-
-struct PassOut {
-  uint32_t m = 0, e = 0, nbins = 0;
-  const uint32_t *order = nullptr;  // render order -> original index
-  const Rec *recs = nullptr;
-  const uint32_t *ent = nullptr;
-  const uint32_t *bin_off = nullptr;
-};
-
 int ceil_log2(uint64_t v) {
   int b = 0;
   while ((uint64_t(1) << b) < v) ++b;
   return std::max(b, 1);
 }
 
-// One render pass (code 1: ordered types, 2: additive types): compaction,
-// optional depth sort, gather, binning into CSR.
-void run_pass(splhip_ctx *c, int code, int64_t n, const Rec *rec, const uint8_t *act,
-              const splhip_params *prm, const BinGeom &g, hipStream_t s, PassOut &out) {
+// ------------------------------------------------------------------ batch plan
+
+// One view as the entry points hand it over.
+struct JobIn {
+  const splhip_particles *P;
+  const splhip_camera *cam;
+  const splhip_params *prm;
+  float *pic;
+};
+
+// Host plan of a batch and the device copies of its tables.
+struct Plan {
+  int nj = 0, G = 8;
+  int64_t ntot = 0;
+  uint32_t nblocks = 0, nunits = 0;
+  bool present[2] = {false, false};  // some job has types of the pass
+  bool sort = false;                 // some job depth-sorts its ordered types
+  std::vector<ProjectArgs> pa;
+  std::vector<uint32_t> blk, pbase, ubase;
+  std::vector<BinGeom> geom;
+  std::vector<JobSplat> js;
+  // device tables (inside the S_JOBS slot)
+  const ProjectArgs *d_pa = nullptr;
+  const uint32_t *d_blk = nullptr, *d_pbase = nullptr, *d_ubase = nullptr;
+  const BinGeom *d_geom = nullptr;
+  JobSplat *d_js = nullptr;
+  // per pass
+  uint32_t m[2] = {0, 0}, e[2] = {0, 0}, nitems[2] = {0, 0}, nheavy[2] = {0, 0};
+  std::vector<uint32_t> mo[2];  // host copies of the job offsets in pass order
+  const uint32_t *order[2] = {nullptr, nullptr};
+  const Rec *recs[2] = {nullptr, nullptr};
+  const uint32_t *ent[2] = {nullptr, nullptr};
+  const uint32_t *bin_off[2] = {nullptr, nullptr};
+  const uint32_t *d_mo[2] = {nullptr, nullptr};
+};
+
+size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+// Host staging (pinned) of at least `bytes`, free for writing: waits for the
+// last copy that used it (marked by staged()).
+char *staging(splhip_ctx *c, size_t bytes) {
+  if (c->ev_stage) hip_check(hipEventSynchronize(c->ev_stage), "wait staging");
+  if (c->h_stage_cap < bytes) {
+    if (c->h_stage) hip_check(hipHostFree(c->h_stage), "hipHostFree");
+    c->h_stage = nullptr;
+    hip_check(hipHostMalloc(reinterpret_cast<void **>(&c->h_stage), bytes, hipHostMallocDefault),
+              "hipHostMalloc");
+    c->h_stage_cap = bytes;
+  }
+  return c->h_stage;
+}
+
+// Mark the end of the copies queued on s that use the staging buffer.
+void staged(splhip_ctx *c, hipStream_t s) {
+  if (!c->ev_stage)
+    hip_check(hipEventCreateWithFlags(&c->ev_stage, hipEventDisableTiming), "hipEventCreate");
+  hip_check(hipEventRecord(c->ev_stage, s), "hipEventRecord");
+}
+
+void make_plan(splhip_ctx *c, int nj, const JobIn *jobs, int G, Plan &pl, hipStream_t s) {
+  upload_palettes(c);
+  pl.nj = nj;
+  pl.G = G;
+  pl.pa.resize(nj);
+  pl.geom.resize(nj);
+  pl.js.resize(nj);
+  pl.blk.assign(nj + 1, 0);
+  pl.pbase.assign(nj + 1, 0);
+  pl.ubase.assign(nj + 1, 0);
+  uint64_t units = 0, blocks = 0;
+  int64_t ntot = 0;
+  for (int j = 0; j < nj; ++j) {
+    const splhip_particles *P = jobs[j].P;
+    const splhip_camera *cam = jobs[j].cam;
+    const splhip_params *prm = jobs[j].prm;
+    ProjectArgs &a = pl.pa[j];
+    a = ProjectArgs{};
+    a.x = P->x; a.y = P->y; a.z = P->z; a.hsml = P->hsml; a.inten = P->inten; a.col = P->col;
+    a.c1 = P->c1; a.c2 = P->c2; a.type = P->type; a.n = P->n;
+    a.poff = ntot;
+    a.cam = *cam;
+    for (int t = 0; t < kMaxTypes; ++t) {
+      a.smooth_fac[t] = prm->smooth_fac[t];
+      a.brightness[t] = prm->brightness[t];
+      a.col_vector[t] = prm->col_vector[t];
+      a.pass_code[t] = prm->a_eq_e[t] ? 2 : 1;
+      a.pal_n[t] = static_cast<int32_t>(c->pal_x[t].size());
+    }
+    for (int t = 0; t < prm->ptypes; ++t) pl.present[prm->a_eq_e[t] ? 1 : 0] = true;
+    a.pal_stride = c->pal_stride;
+    a.ntypes = prm->ptypes;
+    a.sort_type = prm->sort_type;
+    if (prm->sort_type != 0) pl.sort = true;
+    a.pal_x = c->d_palx;
+    a.pal_rgb = c->d_palrgb;
+    a.bfak = c->consts.bfak;
+    a.rfac = c->consts.rfac;
+    pl.pbase[j] = static_cast<uint32_t>(ntot);
+    pl.blk[j] = static_cast<uint32_t>(blocks);
+    pl.ubase[j] = static_cast<uint32_t>(units);
+    ntot += P->n;
+    blocks += (static_cast<uint64_t>(P->n) + kPerBlock - 1) / kPerBlock;
+    const BinGeom g = make_geom(cam, prm, G, c->consts);
+    pl.geom[j] = g;
+    units += static_cast<uint64_t>(g.ncx) * g.ncy * (G > 0 ? g.ngm * g.ngm : 1);
+    JobSplat &J = pl.js[j];
+    J = JobSplat{};
+    J.pic = jobs[j].pic;
+    J.T = g.T;
+    J.xres = cam->xres;
+    J.yres = cam->yres;
+    J.ncx = g.ncx;
+    J.ngm = g.ngm;
+    J.grayabsorb = prm->gray_absorption;
+  }
+  if (ntot >= (int64_t(1) << 32) - 1) throw HipError{hipErrorInvalidValue, "too many particles"};
+  if (units >= (uint64_t(1) << 30)) throw HipError{hipErrorInvalidValue, "too many bins"};
+  pl.ntot = ntot;
+  pl.nblocks = static_cast<uint32_t>(blocks);
+  pl.nunits = static_cast<uint32_t>(units);
+  pl.blk[nj] = pl.nblocks;
+  pl.pbase[nj] = static_cast<uint32_t>(ntot);
+  pl.ubase[nj] = pl.nunits;
+  // one upload: [ProjectArgs][blk][pbase][ubase][BinGeom][JobSplat]
+  const size_t o_pa = 0;
+  const size_t o_blk = align256(o_pa + nj * sizeof(ProjectArgs));
+  const size_t o_pb = align256(o_blk + (nj + 1) * 4);
+  const size_t o_ub = align256(o_pb + (nj + 1) * 4);
+  const size_t o_g = align256(o_ub + (nj + 1) * 4);
+  const size_t o_js = align256(o_g + nj * sizeof(BinGeom));
+  const size_t total = align256(o_js + nj * sizeof(JobSplat));
+  char *h = staging(c, total);
+  std::memcpy(h + o_pa, pl.pa.data(), nj * sizeof(ProjectArgs));
+  std::memcpy(h + o_blk, pl.blk.data(), (nj + 1) * 4);
+  std::memcpy(h + o_pb, pl.pbase.data(), (nj + 1) * 4);
+  std::memcpy(h + o_ub, pl.ubase.data(), (nj + 1) * 4);
+  std::memcpy(h + o_g, pl.geom.data(), nj * sizeof(BinGeom));
+  std::memcpy(h + o_js, pl.js.data(), nj * sizeof(JobSplat));
+  char *d = c->ws.get_as<char>(S_JOBS, total);
+  hip_check(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, s), "upload job tables");
+  staged(c, s);
+  pl.d_pa = reinterpret_cast<const ProjectArgs *>(d + o_pa);
+  pl.d_blk = reinterpret_cast<const uint32_t *>(d + o_blk);
+  pl.d_pbase = reinterpret_cast<const uint32_t *>(d + o_pb);
+  pl.d_ubase = reinterpret_cast<const uint32_t *>(d + o_ub);
+  pl.d_geom = reinterpret_cast<const BinGeom *>(d + o_g);
+  pl.d_js = reinterpret_cast<JobSplat *>(d + o_js);
+}
+
+// Totals slot layout (device u32): [0..1] pass sizes, [2..3] bin entries,
+// [4..5] work items, [6..7] heavy items.
+enum { T_M = 0, T_E = 2, T_ITEMS = 4, T_HEAVY = 6, T_N = 8 };
+
+// Project all jobs, then the compaction scan of each present pass and its job
+// offsets in pass order; one host sync reads the pass sizes and offsets.
+void stage_project_compact(splhip_ctx *c, Plan &pl, Rec *rec, uint8_t *act, uint32_t *tot,
+                           hipStream_t s) {
   Workspace &ws = c->ws;
-  uint32_t *tot = ws.get_as<uint32_t>(S_TOTALS, 64);
-  uint32_t *pos = ws.get_as<uint32_t>(S_POS, static_cast<size_t>(n));
-  scan_u8eq(act, static_cast<uint8_t>(code), pos, n, tot, ws, s);
-  const uint32_t m = read_u32(c, tot, s);
-  out.m = m;
-  if (m == 0) return;
-  const bool do_sort = (code == 1) && prm->sort_type != 0;
-  uint32_t *k0 = do_sort ? ws.get_as<uint32_t>(S_K0, m) : nullptr;
-  uint32_t *v0 = ws.get_as<uint32_t>(S_V0, m);
-  launch_compact(act, static_cast<uint8_t>(code), pos, n, rec, prm->sort_type, k0, v0, s);
+  mark(c, 0, true, s);
+  launch_project(pl.d_pa, pl.d_blk, pl.nj, pl.nblocks, rec, act, s);
+  mark(c, 0, false, s);
+  launch_fill_u32(tot, 0, T_N, s);
+  uint32_t *mo[2] = {nullptr, nullptr};
+  for (int p = 0; p < 2; ++p) {
+    if (!pl.present[p] || pl.ntot == 0) continue;
+    mark(c, p == 0 ? 1 : 4, true, s);
+    uint32_t *pos = ws.get_as<uint32_t>(pslot(p, P_POS), static_cast<size_t>(pl.ntot));
+    scan_u8eq(act, static_cast<uint8_t>(p + 1), pos, pl.ntot, tot + T_M + p, ws, s);
+    mo[p] = ws.get_as<uint32_t>(pslot(p, P_MO), pl.nj + 1);
+    launch_pass_offsets(pos, pl.d_pbase, pl.nj, pl.ntot, tot + T_M + p, mo[p], s);
+    mark(c, p == 0 ? 1 : 4, false, s);
+  }
+  // sync 1: pass sizes and per-job offsets
+  uint32_t *h = c->h_tot;
+  const size_t nmo = static_cast<size_t>(pl.nj) + 1;
+  std::vector<uint32_t> &hbv = c->h_mo;  // pageable: read after the sync below
+  hbv.resize(2 * nmo);
+  uint32_t *hb = hbv.data();
+  hip_check(hipMemcpyAsync(h, tot, T_N * 4, hipMemcpyDeviceToHost, s), "read totals");
+  for (int p = 0; p < 2; ++p)
+    if (mo[p])
+      hip_check(hipMemcpyAsync(hb + p * nmo, mo[p], nmo * 4, hipMemcpyDeviceToHost, s),
+                "read pass offsets");
+  hip_check(hipStreamSynchronize(s), "sync pass sizes");
+  for (int p = 0; p < 2; ++p) {
+    pl.m[p] = mo[p] ? h[T_M + p] : 0;
+    pl.d_mo[p] = mo[p];
+    if (mo[p]) pl.mo[p].assign(hb + p * nmo, hb + (p + 1) * nmo);
+    else pl.mo[p].assign(nmo, 0);
+  }
+}
+
+// Per-job pass flags from the pass sizes: the ordered pass writes the image
+// of every job that has ordered particles; the additive pass adds to it there
+// and writes it elsewhere (render_new renders each mode from zero).
+void upload_job_flags(splhip_ctx *c, Plan &pl, hipStream_t s) {
+  for (int j = 0; j < pl.nj; ++j) {
+    const bool w0 = pl.mo[0][j + 1] > pl.mo[0][j];
+    const bool w1 = pl.mo[1][j + 1] > pl.mo[1][j];
+    JobSplat &J = pl.js[j];
+    J.accumulate[0] = 0;
+    J.accumulate[1] = w0 ? 1 : 0;
+    J.need_empty[0] = w0 ? 1 : 0;
+    J.need_empty[1] = (w1 && !w0) ? 1 : 0;
+  }
+  const size_t bytes = pl.nj * sizeof(JobSplat);
+  char *h = staging(c, bytes);
+  std::memcpy(h, pl.js.data(), bytes);
+  hip_check(hipMemcpyAsync(pl.d_js, h, bytes, hipMemcpyHostToDevice, s), "upload job flags");
+  staged(c, s);
+}
+
+// Render order of pass p: compaction (job-major, index order), then for the
+// ordered pass the depth sort, stable, within each job.
+void stage_order(splhip_ctx *c, Plan &pl, int p, const Rec *rec, const uint8_t *act,
+                 hipStream_t s) {
+  Workspace &ws = c->ws;
+  const uint32_t m = pl.m[p];
+  const bool do_sort = (p == 0) && pl.sort;
+  mark(c, p == 0 ? 1 : 4, true, s);
+  uint32_t *k0 = do_sort ? ws.get_as<uint32_t>(pslot(p, P_K0), m) : nullptr;
+  uint32_t *v0 = ws.get_as<uint32_t>(pslot(p, P_V0), m);
+  const uint32_t *pos = ws.get_as<uint32_t>(pslot(p, P_POS), static_cast<size_t>(pl.ntot));
+  launch_compact(pl.d_pa, pl.d_blk, pl.nj, pl.nblocks, act, static_cast<uint8_t>(p + 1), pos, rec,
+                 k0, v0, s);
   const uint32_t *order = v0;
   if (do_sort) {
-    uint32_t *k1 = ws.get_as<uint32_t>(S_K1, m);
-    uint32_t *v1 = ws.get_as<uint32_t>(S_V1, m);
-    order = radix_sort_pairs(k0, v0, k1, v1, m, 0, 32, ws, s) ? v1 : v0;
+    uint32_t *k1 = ws.get_as<uint32_t>(pslot(p, P_K1), m);
+    uint32_t *v1 = ws.get_as<uint32_t>(pslot(p, P_V1), m);
+    const int r = radix_sort_pairs(k0, v0, k1, v1, m, 0, 32, ws, s);
+    uint32_t *kk = r ? k1 : k0, *vv = r ? v1 : v0;
+    uint32_t *ko = r ? k0 : k1, *vo = r ? v0 : v1;
+    order = vv;
+    if (pl.nj > 1) {  // regroup by job, stable: (job, key, index) order
+      launch_job_of_index(vv, m, pl.d_pbase, pl.nj, kk, s);
+      const int r2 = radix_sort_pairs(kk, vv, ko, vo, m, 0, ceil_log2(pl.nj), ws, s);
+      order = r2 ? vo : vv;
+    }
   }
-  const int sbase = code == 1 ? 1 : 4;
-  mark(c, sbase, false, s);
-  mark(c, sbase + 1, true, s);
-  Rec *recs = ws.get_as<Rec>(S_RECS, m);
-  uint32_t *cnt = ws.get_as<uint32_t>(S_CNT, m);
-  uint32_t *off = ws.get_as<uint32_t>(S_OFF, m);
-  launch_gather_count(order, m, rec, recs, cnt, g, s);
-  scan_u32(cnt, off, m, tot + 1, ws, s);
-  const uint32_t e = read_u32(c, tot + 1, s);
-  const uint64_t nbins = static_cast<uint64_t>(g.ncx) * g.ncy * g.ngm * g.ngm;
-  if (nbins >= (uint64_t(1) << 31)) throw HipError{hipErrorInvalidValue, "too many bins"};
-  uint32_t *ek0 = ws.get_as<uint32_t>(S_EK0, e);
-  uint32_t *ev0 = ws.get_as<uint32_t>(S_EV0, e);
-  uint32_t *ek1 = ws.get_as<uint32_t>(S_EK1, e);
-  uint32_t *ev1 = ws.get_as<uint32_t>(S_EV1, e);
-  launch_emit(recs, m, off, e, g, ek0, ev0, s);
-  const int r = radix_sort_pairs(ek0, ev0, ek1, ev1, e, 0, ceil_log2(nbins), ws, s);
-  const uint32_t *keys = r ? ek1 : ek0;
-  uint32_t *bin_off = ws.get_as<uint32_t>(S_REGOFF, nbins + 1);
-  launch_bin_offsets(keys, e, static_cast<uint32_t>(nbins), bin_off, s);
-  mark(c, sbase + 1, false, s);
-  out.e = e;
-  out.nbins = static_cast<uint32_t>(nbins);
-  out.order = order;
-  out.recs = recs;
-  out.ent = r ? ev1 : ev0;
-  out.bin_off = bin_off;
+  pl.order[p] = order;
+  mark(c, p == 0 ? 1 : 4, false, s);
 }
 
-bool pass_present(const splhip_params *prm, int code) {
-  for (int t = 0; t < prm->ptypes; ++t)
-    if ((prm->a_eq_e[t] ? 2 : 1) == code) return true;
-  return false;
+// Gather into render order and count each particle's units; the entry total
+// lands in tot[T_E + p].
+void stage_count(splhip_ctx *c, Plan &pl, int p, const Rec *rec, uint32_t *tot, hipStream_t s) {
+  Workspace &ws = c->ws;
+  const uint32_t m = pl.m[p];
+  mark(c, p == 0 ? 2 : 5, true, s);
+  Rec *recs = ws.get_as<Rec>(pslot(p, P_RECS), m);
+  uint32_t *cnt = ws.get_as<uint32_t>(pslot(p, P_CNT), m);
+  uint32_t *off = ws.get_as<uint32_t>(pslot(p, P_OFF), m);
+  launch_gather_count(pl.order[p], m, rec, recs, cnt, pl.d_geom, pl.d_ubase, pl.d_mo[p], pl.nj, s);
+  scan_u32(cnt, off, m, tot + T_E + p, ws, s);
+  pl.recs[p] = recs;
+  mark(c, p == 0 ? 2 : 5, false, s);
 }
 
-int render_impl(splhip_ctx *c, const splhip_particles *P, const splhip_camera *cam,
-                const splhip_params *prm, float *pic, hipStream_t s) {
-  const int64_t n = P->n;
-  const int G = region_size(prm);
+// Emit the (unit, particle) entries, sort them by unit (stable, so each list
+// keeps render order) and build the unit offsets.
+void stage_bins(splhip_ctx *c, Plan &pl, int p, hipStream_t s) {
+  Workspace &ws = c->ws;
+  const uint32_t m = pl.m[p], e = pl.e[p];
+  mark(c, p == 0 ? 2 : 5, true, s);
+  const uint32_t *off = ws.get_as<uint32_t>(pslot(p, P_OFF), m);
+  uint32_t *ek0 = ws.get_as<uint32_t>(pslot(p, P_EK0), e);
+  uint32_t *ev0 = ws.get_as<uint32_t>(pslot(p, P_EV0), e);
+  uint32_t *ek1 = ws.get_as<uint32_t>(pslot(p, P_EK1), e);
+  uint32_t *ev1 = ws.get_as<uint32_t>(pslot(p, P_EV1), e);
+  launch_emit(pl.recs[p], m, off, e, pl.d_geom, pl.d_ubase, pl.d_mo[p], pl.nj, ek0, ev0, s);
+  const int r = radix_sort_pairs(ek0, ev0, ek1, ev1, e, 0, ceil_log2(pl.nunits), ws, s);
+  uint32_t *bin_off = ws.get_as<uint32_t>(pslot(p, P_BINOFF), static_cast<size_t>(pl.nunits) + 1);
+  launch_bin_offsets(r ? ek1 : ek0, e, pl.nunits, bin_off, s);
+  pl.ent[p] = r ? ev1 : ev0;
+  pl.bin_off[p] = bin_off;
+  mark(c, p == 0 ? 2 : 5, false, s);
+}
+
+struct Tuning {
+  int variant;
+  uint32_t split, heavy;
+};
+
+Tuning tuning_of(const splhip_params *prm) {
+  Tuning t;
+  t.variant = prm->hip_variant > 0 ? prm->hip_variant : 4;
+  t.split = prm->hip_split > 0 ? static_cast<uint32_t>(prm->hip_split) : 4096u;
+  t.heavy = prm->hip_heavy < 0 ? 0xffffffffu
+            : prm->hip_heavy > 0 ? static_cast<uint32_t>(prm->hip_heavy) : 32768u;
+  return t;
+}
+
+SplatArgs splat_args(splhip_ctx *c, const Plan &pl, int p, const Tuning &tn) {
+  SplatArgs sa{};
+  sa.bin_off = pl.bin_off[p];
+  sa.ent = pl.ent[p];
+  sa.recs = pl.recs[p];
+  sa.exptab = c->d_exptab;
+  sa.jobs = pl.d_js;
+  sa.ubase = pl.d_ubase;
+  sa.nj = pl.nj;
+  sa.pass = p;
+  const JobSplat &J = pl.js[0];
+  sa.pic = J.pic;
+  sa.T = J.T;
+  sa.xres = J.xres;
+  sa.yres = J.yres;
+  sa.ncx = J.ncx;
+  sa.ngm = J.ngm;
+  sa.accumulate = J.accumulate[p];
+  sa.grayabsorb = J.grayabsorb;
+  sa.rfac = c->consts.rfac;
+  sa.h2sigma = c->consts.h2sigma;
+  sa.expfac = c->consts.expfac;
+  sa.taylorlimit = c->consts.taylorlimit;
+  sa.prio_len = tn.split;
+  sa.heavy_len = tn.heavy;
+  return sa;
+}
+
+// Work items of pass p: count per unit (heavy units counted too), scan.
+void stage_items_count(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, uint32_t *tot,
+                       hipStream_t s) {
+  Workspace &ws = c->ws;
+  mark(c, p == 0 ? 2 : 5, true, s);
+  const SplatArgs sa = splat_args(c, pl, p, tn);
+  uint32_t *icnt = ws.get_as<uint32_t>(pslot(p, P_ICNT), pl.nunits);
+  uint32_t *ioff = ws.get_as<uint32_t>(pslot(p, P_IOFF), pl.nunits);
+  launch_items_count(pl.bin_off[p], pl.nunits, sa, p == 0 ? 1 : 0, tn.split, icnt,
+                     tot + T_HEAVY + p, s);
+  scan_u32(icnt, ioff, pl.nunits, tot + T_ITEMS + p, ws, s);
+  mark(c, p == 0 ? 2 : 5, false, s);
+}
+
+// Splat of pass p: items heaviest first; heavy units on the auxiliary stream.
+void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s) {
+  Workspace &ws = c->ws;
+  const SplatArgs sa0 = splat_args(c, pl, p, tn);
+  mark(c, p == 0 ? 3 : 6, true, s);
+  if (pl.G != 8) {
+    launch_splat(pl.G, p == 1, pl.nunits, sa0, s);
+    mark(c, p == 0 ? 3 : 6, false, s);
+    return;
+  }
+  const uint32_t nitems = pl.nitems[p], nheavy = pl.nheavy[p];
+  const uint32_t *ioff = ws.get_as<uint32_t>(pslot(p, P_IOFF), pl.nunits);
+  uint32_t *ik0 = ws.get_as<uint32_t>(pslot(p, P_IK0), nitems);
+  uint32_t *iv0 = ws.get_as<uint32_t>(pslot(p, P_IV0), nitems);
+  uint32_t *ik1 = ws.get_as<uint32_t>(pslot(p, P_IK1), nitems);
+  uint32_t *iv1 = ws.get_as<uint32_t>(pslot(p, P_IV1), nitems);
+  launch_items_emit(pl.bin_off[p], pl.nunits, sa0, p == 0 ? 1 : 0, tn.split, ioff, ik0, iv0, s);
+  const int rs = radix_sort_pairs(ik0, iv0, ik1, iv1, nitems, 0, 32, ws, s);
+  const uint32_t *items = rs ? iv1 : iv0;
+  SplatArgs sa = sa0;
+  if (c->trace)
+    sa.trace = ws.get_as<uint64_t>(pslot(p, P_TRACE), static_cast<size_t>(nitems) * 4);
+  SplatArgs sn = sa;  // normal items follow the heavy ones
+  if (sn.trace) sn.trace += 4 * static_cast<size_t>(nheavy);
+  if (nheavy > 0) {
+    // heavy blocks run on a second stream, beside the normal items
+    ensure_aux(c);
+    hip_check(hipEventRecord(c->ev_fork, s), "record fork");
+    hip_check(hipStreamWaitEvent(c->aux, c->ev_fork, 0), "wait fork");
+    launch_splat_heavy(p == 1, items, nheavy, sa, c->aux);
+    hip_check(hipEventRecord(c->ev_join, c->aux), "record join");
+    // The normal items fill every CU; the heavy workgroups (the critical
+    // path) must get their slots first, so the normal launch waits briefly.
+    launch_delay(40, s);
+    launch_splat8(p == 1, tn.variant, items + nheavy, nitems - nheavy, sn, s);
+    hip_check(hipStreamWaitEvent(s, c->ev_join, 0), "wait join");
+  } else {
+    launch_splat8(p == 1, tn.variant, items, nitems, sn, s);
+  }
+  mark(c, p == 0 ? 3 : 6, false, s);
+  if (c->trace) {
+    std::vector<uint64_t> &tv = c->trace_items[p];
+    tv.resize(static_cast<size_t>(nitems) * 4);
+    if (nitems)
+      hip_check(hipMemcpyAsync(tv.data(), sa.trace, tv.size() * 8, hipMemcpyDeviceToHost, s),
+                "D2H trace");
+    hip_check(hipStreamSynchronize(s), "sync trace");
+  }
+}
+
+// Read n consecutive device totals into c->h_tot (one sync).
+void read_totals(splhip_ctx *c, const uint32_t *d, int n, hipStream_t s) {
+  hip_check(hipMemcpyAsync(c->h_tot, d, n * 4, hipMemcpyDeviceToHost, s), "read totals");
+  hip_check(hipStreamSynchronize(s), "sync totals");
+}
+
+// The frame pipeline over a batch of jobs; three host syncs per batch (pass
+// sizes, bin entries, work items).
+int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s) {
+  const int G = nj == 1 ? region_size(jobs[0].prm) : 8;
   if (G != 8 && G != 16 && G != 32 && G != 64)
     return fail(c, SPLHIP_E_ARG, "hip_region must be 8, 16, 32 or 64");
-  Rec *rec = c->ws.get_as<Rec>(S_REC, static_cast<size_t>(n));
-  uint8_t *act = c->ws.get_as<uint8_t>(S_ACT, static_cast<size_t>(n));
+  if (nj > 1 && jobs[0].prm->hip_region != 0 && jobs[0].prm->hip_region != 8)
+    return fail(c, SPLHIP_E_ARG, "batched renders use 8 px work regions (hip_region 0 or 8)");
   for (int k = 0; k < SPLHIP_NSTAGES; ++k) c->used[k] = false;
+  c->nmarks = 0;
   c->last_entries[0] = c->last_entries[1] = c->last_active[0] = c->last_active[1] = 0;
-  mark(c, 0, true, s);
-  run_project(c, P, cam, prm, rec, act, s);
-  mark(c, 0, false, s);
-  const BinGeom g = make_geom(cam, prm, G, c->consts);
-  bool wrote = false;
-  for (int code = 1; code <= 2; ++code) {
-    if (n == 0 || !pass_present(prm, code)) continue;
-    PassOut po;
-    const int sbase = code == 1 ? 1 : 4;
-    mark(c, sbase, true, s);
-    run_pass(c, code, n, rec, act, prm, g, s, po);
-    if (po.m == 0) continue;
-    mark(c, sbase + 2, true, s);
-    SplatArgs sa{};
-    sa.bin_off = po.bin_off;
-    sa.ent = po.ent;
-    sa.recs = po.recs;
-    sa.exptab = c->d_exptab;
-    sa.pic = pic;
-    sa.T = g.T;
-    sa.xres = cam->xres;
-    sa.yres = cam->yres;
-    sa.ncx = g.ncx;
-    sa.ngm = g.ngm;
-    sa.accumulate = wrote ? 1 : 0;
-    sa.rfac = c->consts.rfac;
-    sa.h2sigma = c->consts.h2sigma;
-    sa.expfac = c->consts.expfac;
-    sa.taylorlimit = c->consts.taylorlimit;
-    sa.grayabsorb = prm->gray_absorption;
-    sa.prio_len = prm->hip_split > 0 ? static_cast<uint32_t>(prm->hip_split) : 4096u;
-    if (G == 8) {
-      // work items, heaviest unit first; long ordered lists split by channel
-      const uint32_t split = prm->hip_split > 0 ? static_cast<uint32_t>(prm->hip_split) : 4096u;
-      const int ordered = code == 1 ? 1 : 0;
-      const int need_empty = wrote ? 0 : 1;
-      // lists at least this long get a producer/consumer workgroup (0 = auto, < 0 = never)
-      sa.heavy_len = prm->hip_heavy < 0 ? 0xffffffffu
-                     : prm->hip_heavy > 0 ? static_cast<uint32_t>(prm->hip_heavy) : 32768u;
-      uint32_t *tot = c->ws.get_as<uint32_t>(S_TOTALS, 64);
-      uint32_t *icnt = c->ws.get_as<uint32_t>(S_AUX1, po.nbins);
-      uint32_t *ioff = c->ws.get_as<uint32_t>(S_AUX2, po.nbins);
-      launch_items_count(po.bin_off, po.nbins, sa, ordered, split, need_empty, icnt, s);
-      scan_u32(icnt, ioff, po.nbins, tot + 2, c->ws, s);
-      const uint32_t nitems = read_u32(c, tot + 2, s);
-      uint32_t *ik0 = c->ws.get_as<uint32_t>(S_K0, nitems);
-      uint32_t *iv0 = c->ws.get_as<uint32_t>(S_V0, nitems);
-      uint32_t *ik1 = c->ws.get_as<uint32_t>(S_K1, nitems);
-      uint32_t *iv1 = c->ws.get_as<uint32_t>(S_V1, nitems);
-      launch_items_emit(po.bin_off, po.nbins, sa, ordered, split, need_empty, ioff, ik0, iv0, s);
-      const int rs = radix_sort_pairs(ik0, iv0, ik1, iv1, nitems, 0, 32, c->ws, s);
-      const uint32_t *items = rs ? iv1 : iv0;
-      launch_count_heavy(rs ? ik1 : ik0, nitems, sa.heavy_len, tot + 3, s);
-      const uint32_t nheavy = read_u32(c, tot + 3, s);
-      if (c->trace) sa.trace = c->ws.get_as<uint64_t>(S_AUX3, static_cast<size_t>(nitems) * 4);
-      mark(c, sbase + 2, true, s);
-      SplatArgs sn = sa;  // normal items follow the heavy ones
-      if (sn.trace) sn.trace += 4 * static_cast<size_t>(nheavy);
-      const int variant = prm->hip_variant > 0 ? prm->hip_variant : 4;
-      if (nheavy > 0) {
-        // heavy blocks run on a second stream, beside the normal items
-        ensure_aux(c);
-        hip_check(hipEventRecord(c->ev_fork, s), "record fork");
-        hip_check(hipStreamWaitEvent(c->aux, c->ev_fork, 0), "wait fork");
-        launch_splat_heavy(code == 2, items, nheavy, sa, c->aux);
-        hip_check(hipEventRecord(c->ev_join, c->aux), "record join");
-        launch_splat8(code == 2, variant, items + nheavy, nitems - nheavy, sn, s);
-        hip_check(hipStreamWaitEvent(s, c->ev_join, 0), "wait join");
-      } else {
-        launch_splat8(code == 2, variant, items, nitems, sn, s);
-      }
-      if (c->trace) {
-        std::vector<uint64_t> &tv = c->trace_items[code - 1];
-        tv.resize(static_cast<size_t>(nitems) * 4);
-        hip_check(hipMemcpyAsync(tv.data(), sa.trace, tv.size() * 8, hipMemcpyDeviceToHost, s),
-                  "D2H trace");
-        hip_check(hipStreamSynchronize(s), "sync trace");
-      }
-    } else {
-      launch_splat(G, code == 2, po.nbins, sa, s);
+  c->trace_items[0].clear();
+  c->trace_items[1].clear();
+  const Tuning tn = tuning_of(jobs[0].prm);
+  Plan pl;
+  make_plan(c, nj, jobs, G, pl, s);
+  Workspace &ws = c->ws;
+  Rec *rec = ws.get_as<Rec>(S_REC, static_cast<size_t>(std::max<int64_t>(pl.ntot, 1)));
+  uint8_t *act = ws.get_as<uint8_t>(S_ACT, static_cast<size_t>(std::max<int64_t>(pl.ntot, 1)));
+  uint32_t *tot = ws.get_as<uint32_t>(S_TOTALS, 64);
+  stage_project_compact(c, pl, rec, act, tot, s);
+  upload_job_flags(c, pl, s);
+  for (int p = 0; p < 2; ++p)
+    if (pl.m[p]) {
+      stage_order(c, pl, p, rec, act, s);
+      stage_count(c, pl, p, rec, tot, s);
     }
-    mark(c, sbase + 2, false, s);
-    c->last_entries[code - 1] = po.e;
-    c->last_active[code - 1] = po.m;
-    wrote = true;
+  if (pl.m[0] || pl.m[1]) {
+    read_totals(c, tot + T_E, 2, s);  // sync 2
+    for (int p = 0; p < 2; ++p) pl.e[p] = pl.m[p] ? c->h_tot[p] : 0;
+    for (int p = 0; p < 2; ++p)
+      if (pl.m[p]) {
+        stage_bins(c, pl, p, s);
+        if (G == 8) stage_items_count(c, pl, p, tn, tot, s);
+      }
+    if (G == 8) {
+      read_totals(c, tot + T_ITEMS, 4, s);  // sync 3
+      for (int p = 0; p < 2; ++p) {
+        pl.nitems[p] = pl.m[p] ? c->h_tot[p] : 0;
+        pl.nheavy[p] = pl.m[p] ? c->h_tot[2 + p] : 0;
+      }
+    }
+    for (int p = 0; p < 2; ++p)
+      if (pl.m[p]) stage_splat(c, pl, p, tn, s);
   }
-  if (!wrote)
-    launch_fill(pic, 0.f, static_cast<int64_t>(cam->xres) * cam->yres * 3, s);
+  // jobs with nothing to splat in either pass
+  for (int j = 0; j < nj; ++j) {
+    const bool w = pl.mo[0][j + 1] > pl.mo[0][j] || pl.mo[1][j + 1] > pl.mo[1][j];
+    if (!w)
+      launch_fill(jobs[j].pic, 0.f, static_cast<int64_t>(jobs[j].cam->xres) * jobs[j].cam->yres * 3, s);
+  }
+  for (int p = 0; p < 2; ++p) {
+    c->last_entries[p] = pl.e[p];
+    c->last_active[p] = pl.m[p];
+  }
   collect_marks(c);
   return 0;
 }
@@ -579,8 +861,9 @@ void splhip_destroy(splhip_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   (void)hipDeviceSynchronize();
-  for (auto &e : c->ev)
-    if (e) (void)hipEventDestroy(e);
+  for (auto &e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->ev_stage) (void)hipEventDestroy(c->ev_stage);
+  if (c->h_stage) (void)hipHostFree(c->h_stage);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   if (c->aux) (void)hipStreamDestroy(c->aux);
@@ -733,7 +1016,8 @@ int splhip_render(splhip_ctx *c, const splhip_particles *P, const splhip_camera 
   if (!pic) return fail(c, SPLHIP_E_ARG, "null image");
   try {
     hip_check(hipSetDevice(c->device), "hipSetDevice");
-    return render_impl(c, P, cam, prm, pic, static_cast<hipStream_t>(stream));
+    const JobIn job{P, cam, prm, pic};
+    return render_jobs(c, 1, &job, static_cast<hipStream_t>(stream));
   } catch (const HipError &e) {
     return hip_fail(c, e);
   }
@@ -763,7 +1047,8 @@ int splhip_render_host(splhip_ctx *c, const splhip_particle_sim *hp, int64_t n,
     P.type = typ;
     P.n = n;
     if (int rc = check_inputs(c, &P, cam, prm)) return rc;
-    if (int rc = render_impl(c, &P, cam, prm, pic, s)) return rc;
+    const JobIn job{&P, cam, prm, pic};
+    if (int rc = render_jobs(c, 1, &job, s)) return rc;
     hip_check(hipMemcpyAsync(host_pic, pic, npix * sizeof(float), hipMemcpyDeviceToHost, s),
               "D2H image");
     hip_check(hipStreamSynchronize(s), "sync");
@@ -780,7 +1065,11 @@ int splhip_project(splhip_ctx *c, const splhip_particles *P, const splhip_camera
   try {
     hip_check(hipSetDevice(c->device), "hipSetDevice");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    run_project(c, P, cam, prm, reinterpret_cast<Rec *>(rec_out), active_out, s);
+    const JobIn job{P, cam, prm, nullptr};
+    Plan pl;
+    make_plan(c, 1, &job, 0, pl, s);
+    launch_project(pl.d_pa, pl.d_blk, 1, pl.nblocks, reinterpret_cast<Rec *>(rec_out), active_out,
+                   s);
   } catch (const HipError &e) {
     return hip_fail(c, e);
   }
@@ -795,29 +1084,40 @@ int splhip_bins(splhip_ctx *c, const splhip_particles *P, const splhip_camera *c
   try {
     hip_check(hipSetDevice(c->device), "hipSetDevice");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const int64_t n = P->n;
-    Rec *rec = c->ws.get_as<Rec>(S_REC, static_cast<size_t>(n));
-    uint8_t *act = c->ws.get_as<uint8_t>(S_ACT, static_cast<size_t>(n));
-    run_project(c, P, cam, prm, rec, act, s);
-    const BinGeom g = make_geom(cam, prm, 0, c->consts);
-    const uint64_t ntiles = static_cast<uint64_t>(g.ncx) * g.ncy;
-    PassOut po;
-    if (n > 0) run_pass(c, ordered ? 1 : 2, n, rec, act, prm, g, s, po);
-    if (n_entries) *n_entries = po.e;
-    if (n_order) *n_order = po.m;
+    const int p = ordered ? 0 : 1;
+    const JobIn job{P, cam, prm, nullptr};
+    Plan pl;
+    make_plan(c, 1, &job, 0, pl, s);  // units = reference tiles
+    Workspace &ws = c->ws;
+    Rec *rec = ws.get_as<Rec>(S_REC, static_cast<size_t>(std::max<int64_t>(pl.ntot, 1)));
+    uint8_t *act = ws.get_as<uint8_t>(S_ACT, static_cast<size_t>(std::max<int64_t>(pl.ntot, 1)));
+    uint32_t *tot = ws.get_as<uint32_t>(S_TOTALS, 64);
+    c->nmarks = 0;
+    stage_project_compact(c, pl, rec, act, tot, s);
+    if (pl.m[p]) {
+      stage_order(c, pl, p, rec, act, s);
+      stage_count(c, pl, p, rec, tot, s);
+      read_totals(c, tot + T_E + p, 1, s);
+      pl.e[p] = c->h_tot[0];
+      stage_bins(c, pl, p, s);
+    }
+    const uint32_t m = pl.m[p], e = pl.e[p];
+    if (n_entries) *n_entries = e;
+    if (n_order) *n_order = m;
+    const uint64_t ntiles = pl.nunits;
     std::vector<uint32_t> off(ntiles + 1, 0);
-    if (po.m > 0)
-      hip_check(hipMemcpyAsync(off.data(), po.bin_off, (ntiles + 1) * 4, hipMemcpyDeviceToHost, s),
+    if (m > 0)
+      hip_check(hipMemcpyAsync(off.data(), pl.bin_off[p], (ntiles + 1) * 4, hipMemcpyDeviceToHost, s),
                 "D2H offsets");
-    if (order_out && po.m > 0)
-      hip_check(hipMemcpyAsync(order_out, po.order, static_cast<size_t>(po.m) * 4,
+    if (order_out && m > 0)
+      hip_check(hipMemcpyAsync(order_out, pl.order[p], static_cast<size_t>(m) * 4,
                                hipMemcpyDeviceToHost, s),
                 "D2H order");
-    if (idx && po.e > 0) {
-      if (idx_cap < po.e) return fail(c, SPLHIP_E_CAPACITY, "idx buffer too small");
-      uint32_t *mapped = c->ws.get_as<uint32_t>(S_MAP, po.e);
-      launch_map(po.ent, po.order, mapped, po.e, s);
-      hip_check(hipMemcpyAsync(idx, mapped, static_cast<size_t>(po.e) * 4, hipMemcpyDeviceToHost, s),
+    if (idx && e > 0) {
+      if (idx_cap < e) return fail(c, SPLHIP_E_CAPACITY, "idx buffer too small");
+      uint32_t *mapped = ws.get_as<uint32_t>(S_MAP, e);
+      launch_map(pl.ent[p], pl.order[p], mapped, e, s);
+      hip_check(hipMemcpyAsync(idx, mapped, static_cast<size_t>(e) * 4, hipMemcpyDeviceToHost, s),
                 "D2H entries");
     }
     hip_check(hipStreamSynchronize(s), "sync");
@@ -827,6 +1127,24 @@ int splhip_bins(splhip_ctx *c, const splhip_particles *P, const splhip_camera *c
     return hip_fail(c, e);
   }
   return 0;
+}
+
+int splhip_render_batched(splhip_ctx *c, int32_t njobs, const splhip_job *jobs, void *stream) {
+  if (!c || njobs < 0 || (njobs > 0 && !jobs)) return fail(c, SPLHIP_E_ARG, "null argument");
+  if (njobs == 0) return 0;
+  if (njobs > (1 << 20)) return fail(c, SPLHIP_E_CAPACITY, "too many jobs");
+  std::vector<JobIn> in(njobs);
+  for (int j = 0; j < njobs; ++j) {
+    if (int rc = check_inputs(c, &jobs[j].parts, &jobs[j].cam, &jobs[j].prm)) return rc;
+    if (!jobs[j].pic) return fail(c, SPLHIP_E_ARG, "null image");
+    in[j] = JobIn{&jobs[j].parts, &jobs[j].cam, &jobs[j].prm, jobs[j].pic};
+  }
+  try {
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    return render_jobs(c, njobs, in.data(), static_cast<hipStream_t>(stream));
+  } catch (const HipError &e) {
+    return hip_fail(c, e);
+  }
 }
 
 }  // extern "C"

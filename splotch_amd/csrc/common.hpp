@@ -58,6 +58,29 @@ struct BinGeom {
   float rcell, cmid0, rfac;
 };
 
+// Per-job splat description: one entry per view of a batch (a plain render is
+// a batch of one).  Units (8x8 blocks or tiles) of job j are numbered from
+// unit_base[j] (a separate table) in the job's own tile / region order.
+struct JobSplat {
+  float *pic;
+  int T, xres, yres, ncx, ngm;
+  int accumulate[2];  // per pass (0 ordered, 1 additive): add to what an earlier pass wrote
+  int need_empty[2];  // per pass: the pass writes every pixel of the job (empty units too)
+  float grayabsorb;
+};
+
+// Largest j with base[j] <= v, for v < base[nj] (base ascending, base[0] = 0;
+// empty jobs repeat their successor's base and are skipped).
+__device__ __forceinline__ int find_job(const uint32_t *base, int nj, uint32_t v) {
+  int lo = 0, hi = nj;  // invariant: base[lo] <= v < base[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (base[mid] <= v) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 // Visit the tiles (G == 0) or work regions (G > 0) a particle is binned to.
 // Tile rule: splotch_host.cc:367-383 (range + circle test, T = tile_size).
 // Region rule: the footprint box that render_new clips to the tile

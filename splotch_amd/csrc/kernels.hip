@@ -39,8 +39,17 @@ unsigned grid_for(int64_t n, int per_block = 256, int64_t cap = 1 << 20) {
 
 // ---------------------------------------------------------------- project
 
-__global__ __launch_bounds__(256) void k_project(ProjectArgs a) {
-  for (int64_t i = gtid(); i < a.n; i += gstride()) {
+// The job of a block is wave-uniform; with the tables marked restrict (the
+// kernel writes only rec/act) its fields load through the scalar cache.
+__global__ __launch_bounds__(256) void k_project(const ProjectArgs *__restrict__ jobs,
+                                                 const uint32_t *__restrict__ blk, int nj,
+                                                 Rec *__restrict__ rec,
+                                                 uint8_t *__restrict__ act) {
+  const int j = __builtin_amdgcn_readfirstlane(find_job(blk, nj, blockIdx.x));
+  const ProjectArgs &a = jobs[j];
+  const int64_t i0 = static_cast<int64_t>(blockIdx.x - blk[j]) * kPerBlock;
+  const int64_t iend = min(i0 + kPerBlock, a.n);
+  for (int64_t i = i0 + threadIdx.x; i < iend; i += blockDim.x) {
     const int t = a.type ? a.type[i] : 0;
     const int tt = t < kMaxTypes ? t : kMaxTypes - 1;
     float px = a.x[i], py = a.y[i], pz = a.z[i];
@@ -145,8 +154,8 @@ __global__ __launch_bounds__(256) void k_project(ProjectArgs a) {
     }
     Rec o;
     o.x = px; o.y = py; o.r = R; o.e0 = e0; o.e1 = e1; o.e2 = e2; o.z = pz; o.I = II;
-    a.rec[i] = o;
-    a.act[i] = code;
+    rec[a.poff + i] = o;
+    act[a.poff + i] = code;
   }
 }
 
@@ -154,6 +163,7 @@ __global__ __launch_bounds__(256) void k_project(ProjectArgs a) {
 
 // Order-preserving float key (ascending); sort types of splotchutils.h:74-96.
 __device__ __forceinline__ uint32_t sort_key(const Rec &R, int sort_type) {
+  if (sort_type == 0) return 0u;  // unsorted: the stable sort keeps index order
   float f = (sort_type == 1) ? R.z : (sort_type == 4 ? R.r : R.e0);
   if (f == 0.f) f = 0.f;  // -0 and +0 compare equal
   uint32_t u = __float_as_uint(f);
@@ -162,11 +172,20 @@ __device__ __forceinline__ uint32_t sort_key(const Rec &R, int sort_type) {
   return u;
 }
 
-__global__ __launch_bounds__(256) void k_compact(const uint8_t *act, uint8_t code,
-                                                 const uint32_t *pos, int64_t n, const Rec *rec,
-                                                 int sort_type, uint32_t *keys,
-                                                 uint32_t *vals) {
-  for (int64_t i = gtid(); i < n; i += gstride()) {
+__global__ __launch_bounds__(256) void k_compact(const ProjectArgs *__restrict__ jobs,
+                                                 const uint32_t *__restrict__ blk, int nj,
+                                                 const uint8_t *__restrict__ act, uint8_t code,
+                                                 const uint32_t *__restrict__ pos,
+                                                 const Rec *__restrict__ rec,
+                                                 uint32_t *__restrict__ keys,
+                                                 uint32_t *__restrict__ vals) {
+  const int j = __builtin_amdgcn_readfirstlane(find_job(blk, nj, blockIdx.x));
+  const int64_t n = jobs[j].n, poff = jobs[j].poff;
+  const int sort_type = jobs[j].sort_type;
+  const int64_t i0 = static_cast<int64_t>(blockIdx.x - blk[j]) * kPerBlock;
+  const int64_t iend = min(i0 + kPerBlock, n);
+  for (int64_t k = i0 + threadIdx.x; k < iend; k += blockDim.x) {
+    const int64_t i = poff + k;
     if (act[i] != code) continue;
     const uint32_t p = pos[i];
     vals[p] = static_cast<uint32_t>(i);
@@ -174,10 +193,26 @@ __global__ __launch_bounds__(256) void k_compact(const uint8_t *act, uint8_t cod
   }
 }
 
+__global__ __launch_bounds__(256) void k_pass_offsets(const uint32_t *pos, const uint32_t *pbase,
+                                                      int nj, int64_t n, const uint32_t *total,
+                                                      uint32_t *mo) {
+  for (int64_t j = gtid(); j <= nj; j += gstride())
+    mo[j] = (j < nj && pbase[j] < n) ? pos[pbase[j]] : *total;
+}
+
+__global__ __launch_bounds__(256) void k_job_of_index(const uint32_t *vals, uint32_t m,
+                                                      const uint32_t *pbase, int nj,
+                                                      uint32_t *keys) {
+  for (int64_t i = gtid(); i < m; i += gstride())
+    keys[i] = static_cast<uint32_t>(find_job(pbase, nj, vals[i]));
+}
+
 __global__ __launch_bounds__(256) void k_gather_count(const uint32_t *order, uint32_t m,
                                                       const Rec *rec, Rec *recs, uint32_t *cnt,
-                                                      BinGeom g) {
+                                                      const BinGeom *geoms,
+                                                      const uint32_t *mo, int nj) {
   for (int64_t s = gtid(); s < m; s += gstride()) {
+    const BinGeom g = geoms[find_job(mo, nj, static_cast<uint32_t>(s))];
     const Rec R = rec[order[s]];
     recs[s] = R;
     uint32_t c = 0;
@@ -187,16 +222,20 @@ __global__ __launch_bounds__(256) void k_gather_count(const uint32_t *order, uin
 }
 
 __global__ __launch_bounds__(256) void k_emit(const Rec *recs, uint32_t m, const uint32_t *off,
-                                              uint32_t e, BinGeom g, uint32_t *keys,
-                                              uint32_t *vals) {
+                                              uint32_t e, const BinGeom *geoms,
+                                              const uint32_t *ubase, const uint32_t *mo, int nj,
+                                              uint32_t *keys, uint32_t *vals) {
   for (int64_t s = gtid(); s < m; s += gstride()) {
+    const int j = find_job(mo, nj, static_cast<uint32_t>(s));
+    const BinGeom g = geoms[j];
+    const uint32_t ub = ubase[j];
     const Rec R = recs[s];
     uint32_t o = off[s];
     const uint32_t end = (s + 1 < m) ? off[s + 1] : e;  // same enumeration as the count
     const uint32_t sv = static_cast<uint32_t>(s);
     for_each_bin(R, g, [&](uint32_t b) {
       if (o < end) {
-        keys[o] = b;
+        keys[o] = ub + b;
         vals[o] = sv;
       }
       ++o;
@@ -224,6 +263,12 @@ __global__ __launch_bounds__(256) void k_map(const uint32_t *idx, const uint32_t
 
 __global__ __launch_bounds__(256) void k_fill(float *p, float v, int64_t n) {
   for (int64_t i = gtid(); i < n; i += gstride()) p[i] = v;
+}
+
+// Bounded wait of `ticks` s_memrealtime ticks (100 MHz): one wave, then exit.
+__global__ __launch_bounds__(64) void k_delay(uint32_t ticks) {
+  const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+  while (__builtin_amdgcn_s_memrealtime() - t0 < ticks) __builtin_amdgcn_s_sleep(8);
 }
 
 __global__ __launch_bounds__(256) void k_fill_u32(uint32_t *p, uint32_t v, int64_t n) {
@@ -414,27 +459,45 @@ __global__ __launch_bounds__(256) void k_splat(SplatArgs a) {
 
 }  // namespace
 
-void launch_project(const ProjectArgs &a, hipStream_t s) {
-  k_project<<<grid_for(a.n), 256, 0, s>>>(a);
+void launch_project(const ProjectArgs *jobs, const uint32_t *blk, int nj, uint32_t nblocks,
+                    Rec *rec, uint8_t *act, hipStream_t s) {
+  if (nblocks == 0) return;
+  k_project<<<nblocks, 256, 0, s>>>(jobs, blk, nj, rec, act);
   hip_check(hipGetLastError(), "k_project launch");
 }
 
-void launch_compact(const uint8_t *act, uint8_t code, const uint32_t *pos, int64_t n,
-                    const Rec *rec, int sort_type, uint32_t *keys, uint32_t *vals,
-                    hipStream_t s) {
-  k_compact<<<grid_for(n), 256, 0, s>>>(act, code, pos, n, rec, sort_type, keys, vals);
+void launch_compact(const ProjectArgs *jobs, const uint32_t *blk, int nj, uint32_t nblocks,
+                    const uint8_t *act, uint8_t code, const uint32_t *pos, const Rec *rec,
+                    uint32_t *keys, uint32_t *vals, hipStream_t s) {
+  if (nblocks == 0) return;
+  k_compact<<<nblocks, 256, 0, s>>>(jobs, blk, nj, act, code, pos, rec, keys, vals);
   hip_check(hipGetLastError(), "k_compact launch");
 }
 
+void launch_pass_offsets(const uint32_t *pos, const uint32_t *pbase, int nj, int64_t n,
+                         const uint32_t *total, uint32_t *mo, hipStream_t s) {
+  k_pass_offsets<<<grid_for(nj + 1), 256, 0, s>>>(pos, pbase, nj, n, total, mo);
+  hip_check(hipGetLastError(), "k_pass_offsets launch");
+}
+
+void launch_job_of_index(const uint32_t *vals, uint32_t m, const uint32_t *pbase, int nj,
+                         uint32_t *keys, hipStream_t s) {
+  k_job_of_index<<<grid_for(m), 256, 0, s>>>(vals, m, pbase, nj, keys);
+  hip_check(hipGetLastError(), "k_job_of_index launch");
+}
+
 void launch_gather_count(const uint32_t *order, uint32_t m, const Rec *rec, Rec *recs,
-                         uint32_t *cnt, const BinGeom &g, hipStream_t s) {
-  k_gather_count<<<grid_for(m), 256, 0, s>>>(order, m, rec, recs, cnt, g);
+                         uint32_t *cnt, const BinGeom *geoms, const uint32_t *ubase,
+                         const uint32_t *mo, int nj, hipStream_t s) {
+  (void)ubase;
+  k_gather_count<<<grid_for(m), 256, 0, s>>>(order, m, rec, recs, cnt, geoms, mo, nj);
   hip_check(hipGetLastError(), "k_gather_count launch");
 }
 
-void launch_emit(const Rec *recs, uint32_t m, const uint32_t *off, uint32_t e, const BinGeom &g,
+void launch_emit(const Rec *recs, uint32_t m, const uint32_t *off, uint32_t e,
+                 const BinGeom *geoms, const uint32_t *ubase, const uint32_t *mo, int nj,
                  uint32_t *keys, uint32_t *vals, hipStream_t s) {
-  k_emit<<<grid_for(m), 256, 0, s>>>(recs, m, off, e, g, keys, vals);
+  k_emit<<<grid_for(m), 256, 0, s>>>(recs, m, off, e, geoms, ubase, mo, nj, keys, vals);
   hip_check(hipGetLastError(), "k_emit launch");
 }
 
@@ -454,6 +517,11 @@ void launch_map(const uint32_t *idx, const uint32_t *table, uint32_t *out, int64
 void launch_fill(float *p, float v, int64_t n, hipStream_t s) {
   k_fill<<<grid_for(n), 256, 0, s>>>(p, v, n);
   hip_check(hipGetLastError(), "k_fill launch");
+}
+
+void launch_delay(uint32_t us, hipStream_t s) {
+  k_delay<<<1, 64, 0, s>>>(us * 100u);
+  hip_check(hipGetLastError(), "k_delay launch");
 }
 
 void launch_fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t s) {

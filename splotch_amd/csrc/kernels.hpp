@@ -1,4 +1,10 @@
-// Launch wrappers of the render kernels (kernels.hip).
+// Launch wrappers of the render kernels (kernels.hip, splat8.hip).
+//
+// Every stage works on a batch of jobs (views); a plain render is a batch of
+// one.  Particles of all jobs are concatenated (job j owns global indices
+// [pbase[j], pbase[j+1])), a pass's particles stay job-major through
+// compaction and the depth sort (job j owns pass positions [mo[j], mo[j+1])),
+// and the units (tiles or 8x8 blocks) of job j are numbered from ubase[j].
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -8,10 +14,12 @@
 
 namespace splhip {
 
+// Per-job projection inputs (device table, one entry per job).
 struct ProjectArgs {
   const float *x, *y, *z, *hsml, *inten, *col, *c1, *c2;
   const uint8_t *type;
   int64_t n;
+  int64_t poff;  // first global particle index of the job
   splhip_camera cam;
   float smooth_fac[kMaxTypes];
   float brightness[kMaxTypes];
@@ -20,18 +28,26 @@ struct ProjectArgs {
   int32_t pal_n[kMaxTypes];
   int32_t pal_stride;
   int32_t ntypes;
+  int32_t sort_type;
   const double *pal_x;
   const float *pal_rgb;
   float bfak, rfac;
-  Rec *rec;
-  uint8_t *act;
 };
 
+// Particle-parallel launches over jobs: jobs get whole blocks of kPerBlock
+// particles; blk[j] is the first block of job j (blk[nj] = grid size).
+constexpr int kPerBlock = 1024;
+
 struct SplatArgs {
-  const uint32_t *bin_off;  // [nbins+1]
+  const uint32_t *bin_off;  // [nunits+1]
   const uint32_t *ent;      // entries: slot in recs
   const Rec *recs;
   const float *exptab;      // 2048 floats
+  const JobSplat *jobs;     // per-job geometry, image and flags
+  const uint32_t *ubase;    // [nj+1] first unit of each job
+  int nj;
+  int pass;                 // 0 ordered, 1 additive
+  // single-job scalars (the G > 8 region kernel)
   float *pic;
   int T, xres, yres, ncx, ngm;
   int accumulate;
@@ -41,12 +57,24 @@ struct SplatArgs {
   uint32_t heavy_len; // splat8: lists at least this long go to the producer/consumer kernel
 };
 
-void launch_project(const ProjectArgs &a, hipStream_t s);
-void launch_compact(const uint8_t *act, uint8_t code, const uint32_t *pos, int64_t n,
-                    const Rec *rec, int sort_type, uint32_t *keys, uint32_t *vals, hipStream_t s);
+void launch_project(const ProjectArgs *jobs, const uint32_t *blk, int nj, uint32_t nblocks,
+                    Rec *rec, uint8_t *act, hipStream_t s);
+// Compaction of one pass (act == code) into pass order; keys (optional): the
+// depth-sort key of each job's sort_type.
+void launch_compact(const ProjectArgs *jobs, const uint32_t *blk, int nj, uint32_t nblocks,
+                    const uint8_t *act, uint8_t code, const uint32_t *pos, const Rec *rec,
+                    uint32_t *keys, uint32_t *vals, hipStream_t s);
+// mo[j] = pos[pbase[j]] (pass particles before job j), mo[nj] = *total.
+void launch_pass_offsets(const uint32_t *pos, const uint32_t *pbase, int nj, int64_t n,
+                         const uint32_t *total, uint32_t *mo, hipStream_t s);
+// keys[i] = job of global particle index vals[i].
+void launch_job_of_index(const uint32_t *vals, uint32_t m, const uint32_t *pbase, int nj,
+                         uint32_t *keys, hipStream_t s);
 void launch_gather_count(const uint32_t *order, uint32_t m, const Rec *rec, Rec *recs,
-                         uint32_t *cnt, const BinGeom &g, hipStream_t s);
-void launch_emit(const Rec *recs, uint32_t m, const uint32_t *off, uint32_t e, const BinGeom &g,
+                         uint32_t *cnt, const BinGeom *geoms, const uint32_t *ubase,
+                         const uint32_t *mo, int nj, hipStream_t s);
+void launch_emit(const Rec *recs, uint32_t m, const uint32_t *off, uint32_t e,
+                 const BinGeom *geoms, const uint32_t *ubase, const uint32_t *mo, int nj,
                  uint32_t *keys, uint32_t *vals, hipStream_t s);
 void launch_bin_offsets(const uint32_t *sorted_keys, uint32_t e, uint32_t nbins,
                         uint32_t *bin_off, hipStream_t s);
@@ -54,23 +82,24 @@ void launch_map(const uint32_t *idx, const uint32_t *table, uint32_t *out, int64
                 hipStream_t s);
 void launch_fill(float *p, float v, int64_t n, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t s);
+// A one-wave kernel that returns after `us` microseconds (bounded wait).
+void launch_delay(uint32_t us, hipStream_t s);
 void launch_aos_to_soa(const splhip_particle_sim *aos, int64_t n, float *soa /*8 planes*/,
                        uint8_t *type, hipStream_t s);
 // Wave-per-8x8-block splat (splat8.hip): work items (unit << 2 | channel,
-// channel 3 = all), heaviest first.
+// channel 3 = all), heaviest first.  heavy_cnt (device, optional): incremented
+// once per unit whose list reaches a.heavy_len.
 void launch_items_count(const uint32_t *bin_off, uint32_t nunits, const SplatArgs &a,
-                        int ordered, uint32_t split, int need_empty, uint32_t *cnt,
+                        int ordered, uint32_t split, uint32_t *cnt, uint32_t *heavy_cnt,
                         hipStream_t s);
 void launch_items_emit(const uint32_t *bin_off, uint32_t nunits, const SplatArgs &a, int ordered,
-                       uint32_t split, int need_empty, const uint32_t *off, uint32_t *keys,
-                       uint32_t *vals, hipStream_t s);
+                       uint32_t split, const uint32_t *off, uint32_t *keys, uint32_t *vals,
+                       hipStream_t s);
 void launch_splat8(bool additive, int variant, const uint32_t *items, uint32_t nitems,
                    const SplatArgs &a, hipStream_t s);
-void launch_count_heavy(const uint32_t *sorted_keys, uint32_t nitems, uint32_t heavy,
-                        uint32_t *out, hipStream_t s);
 void launch_splat_heavy(bool additive, const uint32_t *items, uint32_t nheavy, const SplatArgs &a,
                         hipStream_t s);
-// G: region size (16, 32 or 64 px); additive: a_eq_e mode.
+// G: region size (16, 32 or 64 px); additive: a_eq_e mode.  Single job.
 void launch_splat(int G, bool additive, uint32_t nbins, const SplatArgs &a, hipStream_t s);
 
 }  // namespace splhip

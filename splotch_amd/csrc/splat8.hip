@@ -77,6 +77,60 @@ __device__ __forceinline__ void wave_sync() {
 
 constexpr int kPad = 65;  // LDS row stride: conflict-free column writes and reads
 
+// Where global unit u (an 8x8 block of one job's reference tile) lies: the
+// job's image and flags, the tile origin and clipped size, the block's place
+// in the tile.  For a wave-uniform u the lookups compile to scalar loads.
+struct UnitGeom {
+  float *pic;
+  int yres, accumulate, need_empty;
+  float grayabsorb;
+  int x0s, y0s, w, h, gx, gy;
+};
+
+__device__ __forceinline__ UnitGeom unit_geom(const SplatArgs &a, uint32_t u) {
+  const int j = find_job(a.ubase, a.nj, u);
+  const JobSplat &J = a.jobs[j];
+  const uint32_t local = u - a.ubase[j];
+  const int ngm = J.ngm, T = J.T;
+  const int per_tile = ngm * ngm;
+  const int tile = static_cast<int>(local / per_tile);
+  const int rem = static_cast<int>(local - static_cast<uint32_t>(tile) * per_tile);
+  UnitGeom g;
+  g.gy = rem / ngm;
+  g.gx = rem - g.gy * ngm;
+  g.x0s = (tile % J.ncx) * T;
+  g.y0s = (tile / J.ncx) * T;
+  g.w = min(g.x0s + T, J.xres) - g.x0s;
+  g.h = min(g.y0s + T, J.yres) - g.y0s;
+  g.pic = J.pic;
+  g.yres = J.yres;
+  g.accumulate = J.accumulate[a.pass];
+  g.need_empty = J.need_empty[a.pass];
+  g.grayabsorb = J.grayabsorb;
+  return g;
+}
+
+// unit_geom for a wave-uniform unit, held in scalar registers.
+__device__ __forceinline__ UnitGeom unit_geom_uniform(const SplatArgs &a, uint32_t u) {
+  UnitGeom g = unit_geom(a, u);
+  auto ui = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
+  const uint64_t pp = reinterpret_cast<uint64_t>(g.pic);
+  const uint64_t lo = static_cast<uint32_t>(ui(static_cast<int>(pp & 0xffffffffu)));
+  const uint64_t hi = static_cast<uint32_t>(ui(static_cast<int>(pp >> 32)));
+  g.pic = reinterpret_cast<float *>(lo | (hi << 32));
+  g.yres = ui(g.yres);
+  g.accumulate = ui(g.accumulate);
+  g.need_empty = ui(g.need_empty);
+  g.grayabsorb = __int_as_float(ui(__float_as_int(g.grayabsorb)));
+  g.x0s = ui(g.x0s);
+  g.y0s = ui(g.y0s);
+  g.w = ui(g.w);
+  g.h = ui(g.h);
+  g.gx = ui(g.gx);
+  g.gy = ui(g.gy);
+  return g;
+}
+
 // exptable::expm1 (splotchutils.h:203-214) split in two for pipelining:
 // xexp_index gives the table index for p = arg*expfac (0 where the table does
 // not apply), and xexpm1_finish combines the two table values t1, t2 fetched
@@ -266,12 +320,8 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
   const uint32_t item = uniform(items[it]);
   const uint32_t u = item >> 2;
   const int chan = static_cast<int>(item & 3u);  // 3: all channels
-  const int per_tile = a.ngm * a.ngm;
-  const int tile = static_cast<int>(u / per_tile);
-  const int rem = static_cast<int>(u - static_cast<uint32_t>(tile) * per_tile);
-  const int gy = rem / a.ngm, gx = rem - gy * a.ngm;
-  const int x0s = (tile % a.ncx) * a.T, y0s = (tile / a.ncx) * a.T;
-  const int w = min(x0s + a.T, a.xres) - x0s, h = min(y0s + a.T, a.yres) - y0s;
+  const UnitGeom ug = unit_geom_uniform(a, u);
+  const int gy = ug.gy, gx = ug.gx, x0s = ug.x0s, y0s = ug.y0s, w = ug.w, h = ug.h;
   const int bxo = gx * 8, byo = gy * 8;  // tile frame
   const int cx = lane >> 3, cy = lane & 7;
   const int lx = bxo + cx, ly = byo + cy;
@@ -334,7 +384,7 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
       va1 = -R.e1;
       va2 = -R.e2;
       if (!ADD) {
-        const float g = a.grayabsorb;
+        const float g = ug.grayabsorb;
         vq0 = R.e0 / (R.e0 + g);
         vq1 = R.e1 / (R.e1 + g);
         vq2 = R.e2 / (R.e2 + g);
@@ -455,10 +505,10 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
     tr[3] = static_cast<uint64_t>(__smid());
   }
   if (lx < min(bxo + 8, w) && ly < min(byo + 8, h)) {
-    float *d = a.pic + (static_cast<int64_t>(x0s + lx) * a.yres + (y0s + ly)) * 3;
-    if (chan == 3 || chan == 0) d[0] = a.accumulate ? d[0] + l0 : l0;
-    if (chan == 3 || chan == 1) d[1] = a.accumulate ? d[1] + l1 : l1;
-    if (chan == 3 || chan == 2) d[2] = a.accumulate ? d[2] + l2 : l2;
+    float *d = ug.pic + (static_cast<int64_t>(x0s + lx) * ug.yres + (y0s + ly)) * 3;
+    if (chan == 3 || chan == 0) d[0] = ug.accumulate ? d[0] + l0 : l0;
+    if (chan == 3 || chan == 1) d[1] = ug.accumulate ? d[1] + l1 : l1;
+    if (chan == 3 || chan == 2) d[2] = ug.accumulate ? d[2] + l2 : l2;
   }
 }
 
@@ -508,12 +558,8 @@ __global__ __launch_bounds__(1024) void k_splat_heavy(SplatArgs a, const uint32_
   const uint32_t it = blockIdx.x;
   const uint64_t t_start = a.trace ? __builtin_amdgcn_s_memrealtime() : 0;
   const uint32_t u = uniform(items[it]) >> 2;
-  const int per_tile = a.ngm * a.ngm;
-  const int tile = static_cast<int>(u / per_tile);
-  const int rem = static_cast<int>(u - static_cast<uint32_t>(tile) * per_tile);
-  const int gy = rem / a.ngm, gx = rem - gy * a.ngm;
-  const int x0s = (tile % a.ncx) * a.T, y0s = (tile / a.ncx) * a.T;
-  const int w = min(x0s + a.T, a.xres) - x0s, h = min(y0s + a.T, a.yres) - y0s;
+  const UnitGeom ug = unit_geom_uniform(a, u);
+  const int gy = ug.gy, gx = ug.gx, x0s = ug.x0s, y0s = ug.y0s, w = ug.w, h = ug.h;
   const int bxo = gx * 8, byo = gy * 8;
   const uint32_t e0 = uniform(a.bin_off[u]), e1 = uniform(a.bin_off[u + 1]);
   const int n = static_cast<int>(e1 - e0);
@@ -542,7 +588,7 @@ __global__ __launch_bounds__(1024) void k_splat_heavy(SplatArgs a, const uint32_
       const float a0 = -R.e0, a1 = -R.e1, a2 = -R.e2;
       float q0 = 0.f, q1 = 0.f, q2 = 0.f, mx = 0.f;
       if (!ADD) {
-        const float g = a.grayabsorb;
+        const float g = ug.grayabsorb;
         q0 = R.e0 / (R.e0 + g);
         q1 = R.e1 / (R.e1 + g);
         q2 = R.e2 / (R.e2 + g);
@@ -654,70 +700,49 @@ __global__ __launch_bounds__(1024) void k_splat_heavy(SplatArgs a, const uint32_
   if (!producer) {
     const int lx = bxo + (lane >> 3), ly = byo + (lane & 7);
     if (lx < min(bxo + 8, w) && ly < min(byo + 8, h)) {
-      float *d = a.pic + (static_cast<int64_t>(x0s + lx) * a.yres + (y0s + ly)) * 3 + wave;
-      *d = a.accumulate ? *d + l : l;
+      float *d = ug.pic + (static_cast<int64_t>(x0s + lx) * ug.yres + (y0s + ly)) * 3 + wave;
+      *d = ug.accumulate ? *d + l : l;
     }
   }
 }
 
 // Work items per unit: none for units outside their tile; none for empty units
 // when accumulating; three (one per channel) for long ordered lists; else one.
-__device__ __forceinline__ uint32_t unit_items(const uint32_t *bin_off, uint32_t u, int T,
-                                               int xres, int yres, int ncx, int ngm,
-                                               int ordered, uint32_t split, uint32_t heavy,
-                                               int need_empty, uint32_t &len) {
-  len = bin_off[u + 1] - bin_off[u];
-  const int per_tile = ngm * ngm;
-  const int tile = static_cast<int>(u / per_tile);
-  const int rem = static_cast<int>(u - static_cast<uint32_t>(tile) * per_tile);
-  const int gy = rem / ngm, gx = rem - gy * ngm;
-  const int x0s = (tile % ncx) * T, y0s = (tile / ncx) * T;
-  const int w = min(x0s + T, xres) - x0s, h = min(y0s + T, yres) - y0s;
-  if (gx * 8 >= w || gy * 8 >= h) return 0;
-  if (len == 0) return need_empty ? 1u : 0u;
-  if (len >= heavy) return 1u;  // one producer/consumer workgroup for all channels
+// (need_empty: the job's pass writes its whole image, so empty blocks are
+// written too.)
+__device__ __forceinline__ uint32_t unit_items(const SplatArgs &a, uint32_t u, int ordered,
+                                               uint32_t split, uint32_t &len) {
+  len = a.bin_off[u + 1] - a.bin_off[u];
+  const UnitGeom g = unit_geom(a, u);
+  if (g.gx * 8 >= g.w || g.gy * 8 >= g.h) return 0;
+  if (len == 0) return g.need_empty ? 1u : 0u;
+  if (len >= a.heavy_len) return 1u;  // one producer/consumer workgroup for all channels
   return (len >= (ordered ? split : 2 * split)) ? 3u : 1u;
 }
 
-__global__ __launch_bounds__(256) void k_items_count(const uint32_t *bin_off, uint32_t nunits,
-                                                     int T, int xres, int yres, int ncx, int ngm,
-                                                     int ordered, uint32_t split, uint32_t heavy, int need_empty,
-                                                     uint32_t *cnt) {
+__global__ __launch_bounds__(256) void k_items_count(SplatArgs a, uint32_t nunits, int ordered,
+                                                     uint32_t split, uint32_t *cnt,
+                                                     uint32_t *heavy_cnt) {
   for (int64_t u = gtid(); u < nunits; u += gstride()) {
     uint32_t len;
-    cnt[u] = unit_items(bin_off, static_cast<uint32_t>(u), T, xres, yres, ncx, ngm, ordered,
-                        split, heavy, need_empty, len);
+    const uint32_t k = unit_items(a, static_cast<uint32_t>(u), ordered, split, len);
+    cnt[u] = k;
+    if (heavy_cnt && k == 1 && len >= a.heavy_len) atomicAdd(heavy_cnt, 1u);
   }
 }
 
-__global__ __launch_bounds__(256) void k_items_emit(const uint32_t *bin_off, uint32_t nunits,
-                                                    int T, int xres, int yres, int ncx, int ngm,
-                                                    int ordered, uint32_t split, uint32_t heavy, int need_empty,
-                                                    const uint32_t *off, uint32_t *keys,
-                                                    uint32_t *vals) {
+__global__ __launch_bounds__(256) void k_items_emit(SplatArgs a, uint32_t nunits, int ordered,
+                                                    uint32_t split, const uint32_t *off,
+                                                    uint32_t *keys, uint32_t *vals) {
   for (int64_t u = gtid(); u < nunits; u += gstride()) {
     uint32_t len;
-    const uint32_t k = unit_items(bin_off, static_cast<uint32_t>(u), T, xres, yres, ncx, ngm,
-                                  ordered, split, heavy, need_empty, len);
+    const uint32_t k = unit_items(a, static_cast<uint32_t>(u), ordered, split, len);
     const uint32_t o = off[u];
     for (uint32_t c = 0; c < k; ++c) {
       keys[o + c] = ~len;  // ascending sort of ~len = heaviest first
       vals[o + c] = (static_cast<uint32_t>(u) << 2) | (k == 3 ? c : 3u);
     }
   }
-}
-
-// Number of heavy items at the front of the sorted item keys (keys = ~len,
-// ascending): the count of keys <= ~heavy.
-__global__ void k_count_heavy(const uint32_t *keys, uint32_t n, uint32_t heavy, uint32_t *out) {
-  const uint32_t thr = ~heavy;
-  uint32_t lo = 0, hi = n;
-  while (lo < hi) {
-    const uint32_t mid = lo + ((hi - lo) >> 1);
-    if (keys[mid] <= thr) lo = mid + 1;
-    else hi = mid;
-  }
-  *out = lo;
 }
 
 unsigned grid_for(int64_t n) {
@@ -730,26 +755,23 @@ unsigned grid_for(int64_t n) {
 }  // namespace
 
 void launch_items_count(const uint32_t *bin_off, uint32_t nunits, const SplatArgs &a,
-                        int ordered, uint32_t split, int need_empty, uint32_t *cnt,
+                        int ordered, uint32_t split, uint32_t *cnt, uint32_t *heavy_cnt,
                         hipStream_t s) {
-  k_items_count<<<grid_for(nunits), 256, 0, s>>>(bin_off, nunits, a.T, a.xres, a.yres, a.ncx,
-                                                 a.ngm, ordered, split, a.heavy_len, need_empty, cnt);
+  if (nunits == 0) return;
+  SplatArgs b = a;
+  b.bin_off = bin_off;
+  k_items_count<<<grid_for(nunits), 256, 0, s>>>(b, nunits, ordered, split, cnt, heavy_cnt);
   hip_check(hipGetLastError(), "k_items_count launch");
 }
 
 void launch_items_emit(const uint32_t *bin_off, uint32_t nunits, const SplatArgs &a, int ordered,
-                       uint32_t split, int need_empty, const uint32_t *off, uint32_t *keys,
-                       uint32_t *vals, hipStream_t s) {
-  k_items_emit<<<grid_for(nunits), 256, 0, s>>>(bin_off, nunits, a.T, a.xres, a.yres, a.ncx,
-                                                a.ngm, ordered, split, a.heavy_len, need_empty, off, keys,
-                                                vals);
+                       uint32_t split, const uint32_t *off, uint32_t *keys, uint32_t *vals,
+                       hipStream_t s) {
+  if (nunits == 0) return;
+  SplatArgs b = a;
+  b.bin_off = bin_off;
+  k_items_emit<<<grid_for(nunits), 256, 0, s>>>(b, nunits, ordered, split, off, keys, vals);
   hip_check(hipGetLastError(), "k_items_emit launch");
-}
-
-void launch_count_heavy(const uint32_t *sorted_keys, uint32_t nitems, uint32_t heavy,
-                        uint32_t *out, hipStream_t s) {
-  k_count_heavy<<<1, 1, 0, s>>>(sorted_keys, nitems, heavy, out);
-  hip_check(hipGetLastError(), "k_count_heavy launch");
 }
 
 void launch_splat_heavy(bool additive, const uint32_t *items, uint32_t nheavy, const SplatArgs &a,

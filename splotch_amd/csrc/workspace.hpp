@@ -10,11 +10,22 @@
 namespace splhip {
 
 enum Slot : int {
-  S_REC, S_ACT, S_POS, S_K0, S_V0, S_K1, S_V1, S_RECS, S_CNT, S_OFF,
-  S_EK0, S_EV0, S_EK1, S_EV1, S_REGOFF, S_HIST, S_SCAN0, S_SCAN1, S_SCAN2, S_SCAN3,
-  S_TOTALS, S_SOA, S_AOS, S_PIC, S_MAP, S_AUX0, S_AUX1, S_AUX2, S_AUX3,
-  S_NSLOTS
+  S_REC, S_ACT, S_HIST, S_SCAN0, S_SCAN1, S_SCAN2, S_SCAN3,
+  S_TOTALS, S_SOA, S_AOS, S_PIC, S_MAP, S_AUX0, S_JOBS, S_GRAD0, S_GRAD1, S_GRAD2,
+  S_PASS  // first per-pass slot
 };
+
+// Slots of one render pass (both passes' data live at the same time).
+enum PassSlot : int {
+  P_POS, P_MO, P_K0, P_V0, P_K1, P_V1, P_RECS, P_CNT, P_OFF, P_EK0, P_EV0, P_EK1, P_EV1,
+  P_BINOFF, P_ICNT, P_IOFF, P_IK0, P_IV0, P_IK1, P_IV1, P_TRACE,
+  P_NSLOTS
+};
+
+constexpr int S_NSLOTS = S_PASS + 2 * P_NSLOTS;
+
+// Slot x of pass p (0: ordered, 1: additive).
+constexpr int pslot(int p, int x) { return S_PASS + p * P_NSLOTS + x; }
 
 struct HipError {
   hipError_t code;

@@ -178,3 +178,30 @@ def orbit_view(az: float, el: float, radius: float = 150.0, xres: int = 512,
          50.0 + radius * math.sin(el))
     return View(c, (50.0, 50.0, 50.0), (0.0, 0.0, 1.0), xres,
                 yres if yres is not None else xres, 30.0)
+
+
+@dataclass
+class BatchJob:
+    """One view of the batched workload: its particles, camera and mode."""
+    field: Field
+    view: View
+    additive: bool
+
+
+def batch_jobs(n_jobs: int = 64, seed: int = 1200, n_scale: float = 1.0,
+               n_min: float = 1e5, n_max: float = 4e6) -> list:
+    """Batched workload: job j has seed+j particles N_j = round(exp(U[ln n_min,
+    ln n_max]) * n_scale) from the plain workload's generator, a square image of
+    side 256/512/768/1024 and a camera orbiting the box at radius 150 (azimuth
+    U[0, 2pi), elevation asin(U[-0.98, 0.98]), sky +z); even jobs are
+    additive, odd jobs ordered (depth-sorted)."""
+    out = []
+    for j in range(n_jobs):
+        rng = np.random.default_rng(seed + j)
+        n = int(round(math.exp(rng.uniform(math.log(n_min), math.log(n_max))) * n_scale))
+        side = int(rng.choice([256, 512, 768, 1024]))
+        az = rng.uniform(0.0, 2.0 * math.pi)
+        el = math.asin(rng.uniform(-0.98, 0.98))
+        f = plummer_field(max(n, 1), seed=seed + j)
+        out.append(BatchJob(f, orbit_view(az, el, 150.0, side), j % 2 == 0))
+    return out

@@ -1,0 +1,151 @@
+"""GPU parity of the batched render (splhip_render_batched), through the C-ABI.
+
+Bar: every view of a batch is bit-identical to splhip_render of that view
+alone, and (on small views) to the oracle's host_rendering with the stable
+(z, index) depth order.  Batches mix additive and ordered views, image sizes,
+tile sizes, cameras, empty and fully culled views, and views with both
+particle kinds.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from helpers import assert_bitexact
+from splotch_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def R():
+    import splotch_amd as S
+    assert torch.cuda.is_available(), "GPU test without a GPU"
+    r = S.Renderer(0)
+    pal = synth.default_palette()
+    for t in range(2):
+        r.set_palette(t, *pal)
+    yield r
+    torch.cuda.synchronize()
+    r.close()
+
+
+def _views(jobs, **prm_kw):
+    import splotch_amd as S
+    out = []
+    for j, bj in enumerate(jobs):
+        dev = S.to_device(bj.field)
+        cam = S.camera_setup(bj.view)
+        types = bj.field.type
+        if types is not None and (types == 1).any():
+            prm = S.params(ptypes=2, a_eq_e=[0, 1], brightness=[0.5, 2.0], **prm_kw)
+        else:
+            prm = S.params(a_eq_e=bj.additive, **prm_kw)
+        pic = torch.full((bj.view.xres * bj.view.yres * 3,), float("nan"), dtype=torch.float32,
+                         device="cuda")
+        out.append((dev, cam, prm, pic))
+    return out
+
+
+def _plain(R, views):
+    imgs = []
+    for dev, cam, prm, pic in views:
+        p = torch.full_like(pic, float("nan"))
+        R.render(dev, cam, prm, p)
+        imgs.append(p)
+    torch.cuda.synchronize()
+    return [p.cpu().numpy() for p in imgs]
+
+
+def _small_batch(seed=2200, n_jobs=12):
+    jobs = synth.batch_jobs(n_jobs, seed=seed, n_min=2e3, n_max=6e4)
+    # shrink the images so the oracle stays fast
+    for bj in jobs:
+        bj.view.xres = bj.view.yres = bj.view.xres // 4
+    return jobs
+
+
+def test_batched_equals_plain(R):
+    jobs = _small_batch()
+    views = _views(jobs)
+    log("batched vs plain:", [(bj.field.n, bj.view.xres, bj.additive) for bj in jobs])
+    R.render_batched(views)
+    torch.cuda.synchronize()
+    got = [v[3].cpu().numpy() for v in views]
+    ref = _plain(R, views)
+    for j, (g, r) in enumerate(zip(got, ref)):
+        assert_bitexact(g, r, "job %d batched vs plain" % j)
+
+
+def test_batched_vs_oracle(R):
+    jobs = _small_batch(seed=2300, n_jobs=4)
+    views = _views(jobs)
+    R.render_batched(views)
+    torch.cuda.synchronize()
+    pal = synth.default_palette()
+    for j, (bj, v) in enumerate(zip(jobs, views)):
+        opic, _ = O.host_rendering(bj.field, bj.view, [pal], a_eq_e=bj.additive, stable=True)
+        got = v[3].cpu().numpy().reshape(bj.view.xres, bj.view.yres, 3)
+        assert_bitexact(got, opic, "job %d vs oracle" % j)
+
+
+def test_batched_mixed_edge_views(R):
+    """Tile sizes, both particle kinds in one view, an empty view, a view whose
+    particles are all behind the camera, and a view repeated twice."""
+    import splotch_amd as S
+    base = _small_batch(seed=2400, n_jobs=3)
+    gas = synth.plummer_field(8000, 2401, ptype=0)
+    stars = synth.plummer_field(8000, 2402, hsml_fixed=0.05, ptype=1)
+    mixed = synth.BatchJob(synth.concat(gas, stars), synth.orbit_view(0.3, 0.2, 150.0, 160), False)
+    empty = synth.BatchJob(synth.plummer_field(1, 2403), synth.orbit_view(1.0, 0.1, 150.0, 96),
+                           True)
+    empty.field = synth.Field(*[np.zeros(0, np.float32)] * 6)
+    behind = synth.BatchJob(synth.plummer_field(5000, 2404),
+                            synth.View((50.0, 50.0, 300.0), (50.0, 50.0, 400.0), (0.0, 1.0, 0.0),
+                                       128, 100, 30.0), False)
+    jobs = base + [mixed, empty, behind, base[1]]
+    views = _views(jobs)
+    # per-view tile sizes (a render key of each view)
+    for (dev, cam, prm, pic), T in zip(views, [50, 16, 37, 64, 50, 8, 50]):
+        prm.tile_size = T
+    log("edge batch sizes", [bj.field.n for bj in jobs])
+    R.render_batched(views)
+    torch.cuda.synchronize()
+    got = [v[3].cpu().numpy() for v in views]
+    ref = _plain(R, views)
+    for j, (g, r) in enumerate(zip(got, ref)):
+        assert_bitexact(g, r, "edge job %d batched vs plain" % j)
+    assert not np.any(got[4]), "empty view must be black"
+    assert not np.any(got[5]), "fully culled view must be black"
+    assert_bitexact(got[1], got[6], "repeated view")
+
+
+@pytest.mark.parametrize("variant,split,heavy", [(4, 0, 0), (1, 0, -1), (3, 64, 200), (4, 64, 1)])
+def test_batched_tuning_invariant(R, variant, split, heavy):
+    """The splat tuning keys change the GPU work decomposition only."""
+    jobs = _small_batch(seed=2500, n_jobs=6)
+    views0 = _views(jobs)
+    R.render_batched(views0)
+    views1 = _views(jobs, hip_variant=variant, hip_split=split, hip_heavy=heavy)
+    R.render_batched(views1)
+    torch.cuda.synchronize()
+    for j, (a, b) in enumerate(zip(views0, views1)):
+        assert_bitexact(b[3].cpu().numpy(), a[3].cpu().numpy(), "job %d tuned" % j)
+
+
+def test_batched_errors(R):
+    import splotch_amd as S
+    jobs = _small_batch(seed=2600, n_jobs=2)
+    views = _views(jobs, hip_region=16)
+    with pytest.raises(S.SplhipError):
+        R.render_batched(views)
+    R.render_batched([])  # no views: nothing to do
