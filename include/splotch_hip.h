@@ -165,6 +165,35 @@ typedef struct {
    times per batch for sizes. */
 int splhip_render_batched(splhip_ctx *ctx, int32_t njobs, const splhip_job *jobs, void *stream);
 
+/* --- differentiable form ------------------------------------------------------ */
+
+/* Gradient outputs (device, n floats each, original particle order).  c1, c2:
+   green / blue fields of color_is_vector types (may be NULL when no type is
+   vector-coloured).  screen (optional, n*6 floats): the gradient with respect
+   to the screen-space state {x, y, r, e.r, e.g, e.b} of each projected
+   particle. */
+typedef struct {
+  float *x, *y, *z, *hsml, *inten, *col, *c1, *c2;
+  float *screen;
+} splhip_grads;
+
+/* Backward of the last splhip_render on this context (no reference
+   counterpart: Splotch has no gradient path).  dL_dpic: device, xres*yres*3
+   floats, the gradient of a scalar loss with respect to that render's raw
+   image.  The particles, camera and params of that render must be unchanged
+   (the forward's bins and projected particles are reused; they stay valid
+   until the next call that renders or bins on this context).
+   The derivative is that of the smooth model on the forward's frozen
+   footprint: each (pixel, particle) pair that render visits contributes with
+   the true exponential, att = exp(stp*(y-sy)^2)*exp(stp*(x-sx)^2),
+   additive P += -e*att, ordered P = q + (P-q)*exp(-e*att) (the table-free
+   limit of splotch_host.cc:615,648-658); outputs d/d(x, y, z, hsml, I,
+   colour) through particle_project/particle_colorize (splotch_host.cc:111-287).
+   Requires 8 px work regions (hip_region 0 or 8) in that render.
+   Returns SPLHIP_E_STATE when there is no such forward. */
+int splhip_render_backward(splhip_ctx *ctx, const float *dL_dpic, const splhip_grads *out,
+                           void *stream);
+
 /* --- parity hooks ----------------------------------------------------------- */
 
 /* particle_project + particle_colorize only.  rec_out: device, n*8 floats per

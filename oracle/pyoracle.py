@@ -12,6 +12,7 @@ this module, as the checker / CPU baseline.  The product path never does.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -324,3 +325,150 @@ def adjoint64(p: Projected, s, G, xres, yres, a_eq_e=True, grayabsorb=0.2, tile=
                         _p(out["sy"]), _p(out["r"]), _p(out["e0"]), _p(out["e1"]),
                         _p(out["e2"]))
     return out
+
+
+# ------------------------------------------------------------------ FP64 projection and chain
+
+def _type_tables(field, palettes, smooth_fac, brightness, col_vector, b_brightness):
+    def pad(v, d, dt):
+        return np.asarray(list(v) + [d] * (8 - len(v)), dt) if v is not None else np.full(8, d, dt)
+    sf = pad(smooth_fac, 1.0, np.float32)
+    br = (pad(brightness, 1.0, np.float32) * np.float32(b_brightness)).astype(np.float32)
+    cv = pad(col_vector, 0, np.int32)
+    t = field.type.astype(np.int64) if field.type is not None else np.zeros(field.n, np.int64)
+    return sf, br, cv, np.minimum(t, 7)
+
+
+def _palette64(palettes, t, col):
+    """Palette colour (linear_map.h:71-84, interpolation in double) and its
+    slope d colour / d x at x = col, per particle; (n, 3) each."""
+    n = len(col)
+    rgb = np.zeros((n, 3))
+    slope = np.zeros((n, 3))
+    for tt, (xs, ys) in enumerate(palettes):
+        sel = t == tt
+        if not sel.any():
+            continue
+        xs = np.asarray(xs, np.float64)
+        ys = np.asarray(ys, np.float32).reshape(-1, 3).astype(np.float64)
+        xv = np.asarray(col[sel], np.float64)
+        if len(xs) == 1:
+            rgb[sel] = ys[0]
+            continue
+        idx = np.searchsorted(xs, xv, side="left") - 1  # lower_bound - 1
+        idx = np.clip(idx, 0, len(xs) - 2)
+        dxp = xs[idx + 1] - xs[idx]
+        frac = (xv - xs[idx]) / dxp
+        c = (1.0 - frac)[:, None] * ys[idx] + frac[:, None] * ys[idx + 1]
+        s = (ys[idx + 1] - ys[idx]) / dxp[:, None]
+        lo, hi = xv <= xs[0], xv >= xs[-1]
+        c[lo] = ys[0]
+        c[hi] = ys[-1]
+        s[lo | hi] = 0.0
+        rgb[sel] = c
+        slope[sel] = s
+    return rgb, slope
+
+
+def project64(field, view, palettes, smooth_fac=None, brightness=None, col_vector=None,
+              b_brightness=1.0, zmax=1e23, zmin=0.0, projection=True, minrad_pix=1.0):
+    """particle_project + particle_colorize (splotch_host.cc:111-178, 252-287)
+    evaluated in double for every particle (no culling): the screen-space state
+    {sx, sy, r, e0, e1, e2} of the differentiable form, original order.  The
+    camera (float32 matrix, double shifts) is the float renderer's."""
+    cam = camera(view, zmax, zmin, projection, minrad_pix)
+    m = np.array(cam.mat[:], np.float32).astype(np.float64)
+    sf, br, cv, t = _type_tables(field, palettes, smooth_fac, brightness, col_vector, b_brightness)
+    X, Y, Z = (np.asarray(a, np.float64) for a in (field.x, field.y, field.z))
+    vx = X * m[0] + Y * m[1] + Z * m[2] + m[3]
+    vy = X * m[4] + Y * m[5] + Z * m[6] + m[7]
+    vz = X * m[8] + Y * m[9] + Z * m[10] + m[11]
+    res2 = float(cam.res2)
+    if projection:
+        xfac2 = -res2 / (float(cam.fovfct) * vz)
+        sx = res2 + cam.xshift + vx * xfac2
+        sy = res2 + cam.yshift + float(cam.ycorr) + vy * xfac2
+    else:
+        xfac2 = np.full_like(vx, float(cam.xfac))
+        sx = res2 + vx * xfac2
+        sy = res2 + float(cam.ycorr) + vy * xfac2
+    hs = np.asarray(field.hsml, np.float64)
+    I = np.asarray(field.inten, np.float64)
+    bfak = 0.5 * math.pi ** (-5.0 / 6.0)
+    r2 = 2.0 * hs * xfac2 * sf[t].astype(np.float64)
+    mr = float(np.float32(minrad_pix))
+    r = np.sqrt(r2 * r2 + mr * mr)
+    Ie = I * (0.5 * float(np.float32(bfak))) / hs * r2 / r
+    vec = cv[t] != 0
+    col = np.asarray(field.col, np.float64)
+    rgb, _ = _palette64(palettes, t, np.asarray(field.col))
+    if vec.any():
+        rgb[vec, 0] = col[vec]
+        rgb[vec, 1] = np.asarray(field.c1, np.float64)[vec]
+        rgb[vec, 2] = np.asarray(field.c2, np.float64)[vec]
+    e = rgb * (Ie * br[t].astype(np.float64))[:, None]
+    return {"sx": sx, "sy": sy, "r": r, "e0": e[:, 0], "e1": e[:, 1], "e2": e[:, 2]}
+
+
+def chain64(field, view, palettes, g, smooth_fac=None, brightness=None, col_vector=None,
+            b_brightness=1.0, zmax=1e23, zmin=0.0, projection=True, minrad_pix=1.0):
+    """Chain rule of project64: world-space gradients (x, y, z, hsml, I, col
+    [, c1, c2]) from the screen-space gradient g (dict of sx, sy, r, e0..e2,
+    original order), in double."""
+    cam = camera(view, zmax, zmin, projection, minrad_pix)
+    m = np.array(cam.mat[:], np.float32).astype(np.float64)
+    sf, br, cv, t = _type_tables(field, palettes, smooth_fac, brightness, col_vector, b_brightness)
+    X, Y, Z = (np.asarray(a, np.float64) for a in (field.x, field.y, field.z))
+    vx = X * m[0] + Y * m[1] + Z * m[2] + m[3]
+    vy = X * m[4] + Y * m[5] + Z * m[6] + m[7]
+    vz = X * m[8] + Y * m[9] + Z * m[10] + m[11]
+    res2 = float(cam.res2)
+    xfac2 = -res2 / (float(cam.fovfct) * vz) if projection else np.full_like(vx, float(cam.xfac))
+    hs = np.asarray(field.hsml, np.float64)
+    I = np.asarray(field.inten, np.float64)
+    bfak = 0.5 * math.pi ** (-5.0 / 6.0)
+    k0 = 0.5 * float(np.float32(bfak))
+    sm = sf[t].astype(np.float64)
+    r2 = 2.0 * hs * xfac2 * sm
+    mr = float(np.float32(minrad_pix))
+    r = np.sqrt(r2 * r2 + mr * mr)
+    I1 = I * k0 / hs
+    Ie = I1 * r2 / r
+    b = br[t].astype(np.float64)
+    rgb, slope = _palette64(palettes, t, np.asarray(field.col))
+    vec = cv[t] != 0
+    if vec.any():
+        rgb[vec, 0] = np.asarray(field.col, np.float64)[vec]
+        rgb[vec, 1] = np.asarray(field.c1, np.float64)[vec]
+        rgb[vec, 2] = np.asarray(field.c2, np.float64)[vec]
+        slope[vec] = 0.0
+    ge = np.stack([g["e0"], g["e1"], g["e2"]], axis=1)
+    # e_c = rgb_c * Ie * b
+    gIe = (ge * rgb).sum(axis=1) * b
+    gr2 = g["r"] * (r2 / r) + gIe * (I1 * mr * mr / r ** 3)
+    out = {"hsml": gr2 * (2.0 * xfac2 * sm) + gIe * (-Ie / hs),
+           "inten": gIe * (k0 / hs) * (r2 / r),
+           "col": np.where(vec, ge[:, 0] * Ie * b, (ge * slope).sum(axis=1) * Ie * b)}
+    if vec.any():
+        out["c1"] = np.where(vec, ge[:, 1] * Ie * b, 0.0)
+        out["c2"] = np.where(vec, ge[:, 2] * Ie * b, 0.0)
+    gvx = g["sx"] * xfac2
+    gvy = g["sy"] * xfac2
+    if projection:
+        gvz = -(g["sx"] * vx + g["sy"] * vy) * xfac2 / vz - gr2 * r2 / vz
+    else:
+        gvz = np.zeros_like(gvx)
+    out["x"] = m[0] * gvx + m[4] * gvy + m[8] * gvz
+    out["y"] = m[1] * gvx + m[5] * gvy + m[9] * gvz
+    out["z"] = m[2] * gvx + m[6] * gvy + m[10] * gvz
+    return out
+
+
+def frozen_model(field, view, palettes, a_eq_e=True, sort_type=1, **kw):
+    """Float32 projected particles in render order (the frozen footprint) and
+    the render-order permutation of the original particles."""
+    p = project_colorize(field, view, palettes, **kw)
+    if a_eq_e:
+        return p, np.arange(p.n)
+    perm = sort_perm(p, sort_type, True)
+    return p.take(perm), perm

@@ -54,6 +54,13 @@ class Job(C.Structure):
     _fields_ = [("parts", Particles), ("cam", Camera), ("prm", Params), ("pic", C.c_void_p)]
 
 
+class Grads(C.Structure):
+    """splhip_grads: device gradient buffers."""
+    _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("hsml", C.c_void_p),
+                ("inten", C.c_void_p), ("col", C.c_void_p), ("c1", C.c_void_p),
+                ("c2", C.c_void_p), ("screen", C.c_void_p)]
+
+
 # numpy dtype of the reference particle_sim record (splotch/splotchutils.h:52-72)
 PARTICLE_SIM = np.dtype([("er", "<f4"), ("eg", "<f4"), ("eb", "<f4"), ("x", "<f4"),
                          ("y", "<f4"), ("z", "<f4"), ("r", "<f4"), ("I", "<f4"),
@@ -63,7 +70,7 @@ EXPORTS = ["splhip_create", "splhip_destroy", "splhip_last_error", "splhip_camer
            "splhip_params_default", "splhip_params_from_par", "splhip_set_palette",
            "splhip_render", "splhip_render_host", "splhip_project", "splhip_bins",
            "splhip_exptable", "splhip_set_profiling", "splhip_stage_times", "splhip_set_trace",
-           "splhip_trace_items", "splhip_render_batched"]
+           "splhip_trace_items", "splhip_render_batched", "splhip_render_backward"]
 
 STAGES = ("project", "ordered_sort", "ordered_bin", "ordered_splat", "additive_compact",
           "additive_bin", "additive_splat")
@@ -92,6 +99,8 @@ def lib():
                                   C.c_int64, C.POINTER(C.c_int64), C.c_void_p,
                                   C.POINTER(C.c_int64), C.c_void_p]
         L.splhip_render_batched.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
+        L.splhip_render_backward.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Grads),
+                                             C.c_void_p]
         L.splhip_set_profiling.argtypes = [C.c_void_p, C.c_int32]
         L.splhip_stage_times.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
@@ -278,6 +287,27 @@ class Renderer:
                                                 C.cast(jobs, C.c_void_p),
                                                 self._stream(stream)), "splhip_render_batched")
         return [v[3] for v in views]
+
+    def backward(self, grad_pic, n: int, vector: bool = False, screen: bool = False,
+                 stream=None):
+        """Gradient of the last render() with respect to the particle fields,
+        given grad_pic = dL/dpic (cuda float32, same layout as pic).  Returns a
+        dict of cuda tensors x, y, z, hsml, inten, col [, c1, c2] [, screen
+        (n, 6)]."""
+        import torch
+        assert grad_pic.is_contiguous() and grad_pic.dtype == torch.float32
+        dev = grad_pic.device
+        keys = ["x", "y", "z", "hsml", "inten", "col"] + (["c1", "c2"] if vector else [])
+        out = {k: torch.empty(max(n, 1), dtype=torch.float32, device=dev) for k in keys}
+        if screen:
+            out["screen"] = torch.empty((max(n, 1), 6), dtype=torch.float32, device=dev)
+        g = Grads()
+        for k, t in out.items():
+            setattr(g, k, t.data_ptr())
+        self._check(lib().splhip_render_backward(self._c, C.c_void_p(grad_pic.data_ptr()),
+                                                 C.byref(g), self._stream(stream)),
+                    "splhip_render_backward")
+        return {k: t[:n] for k, t in out.items()}
 
     def render_host(self, aos: np.ndarray, cam: Camera, prm: Params, stream=None):
         """cuda_rendering semantics: host particle_sim records in, host image out."""

@@ -20,7 +20,10 @@
 
 using namespace splhip;
 
+struct FwdState;  // the last single-view render, kept for splhip_render_backward
+
 struct splhip_ctx {
+  FwdState *fwd = nullptr;
   int device = 0;
   std::string err;
   Workspace ws;
@@ -364,6 +367,7 @@ struct Plan {
   const uint32_t *ent[2] = {nullptr, nullptr};
   const uint32_t *bin_off[2] = {nullptr, nullptr};
   const uint32_t *d_mo[2] = {nullptr, nullptr};
+  const uint32_t *items[2] = {nullptr, nullptr};  // sorted work items (8 px regions)
 };
 
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
@@ -683,6 +687,7 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
   launch_items_emit(pl.bin_off[p], pl.nunits, sa0, p == 0 ? 1 : 0, tn.split, ioff, ik0, iv0, s);
   const int rs = radix_sort_pairs(ik0, iv0, ik1, iv1, nitems, 0, 32, ws, s);
   const uint32_t *items = rs ? iv1 : iv0;
+  pl.items[p] = items;
   SplatArgs sa = sa0;
   if (c->trace)
     sa.trace = ws.get_as<uint64_t>(pslot(p, P_TRACE), static_cast<size_t>(nitems) * 4);
@@ -720,14 +725,32 @@ void read_totals(splhip_ctx *c, const uint32_t *d, int n, hipStream_t s) {
   hip_check(hipStreamSynchronize(s), "sync totals");
 }
 
+}  // namespace
+
+struct FwdState {
+  splhip_particles P;
+  splhip_camera cam;
+  splhip_params prm;
+  Plan pl;
+  Tuning tn;
+};
+
+namespace {
+
+void drop_fwd(splhip_ctx *c) {
+  delete c->fwd;
+  c->fwd = nullptr;
+}
+
 // The frame pipeline over a batch of jobs; three host syncs per batch (pass
 // sizes, bin entries, work items).
-int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s) {
+int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool keep_fwd) {
   const int G = nj == 1 ? region_size(jobs[0].prm) : 8;
   if (G != 8 && G != 16 && G != 32 && G != 64)
     return fail(c, SPLHIP_E_ARG, "hip_region must be 8, 16, 32 or 64");
   if (nj > 1 && jobs[0].prm->hip_region != 0 && jobs[0].prm->hip_region != 8)
     return fail(c, SPLHIP_E_ARG, "batched renders use 8 px work regions (hip_region 0 or 8)");
+  drop_fwd(c);
   for (int k = 0; k < SPLHIP_NSTAGES; ++k) c->used[k] = false;
   c->nmarks = 0;
   c->last_entries[0] = c->last_entries[1] = c->last_active[0] = c->last_active[1] = 0;
@@ -776,6 +799,42 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s) {
     c->last_active[p] = pl.m[p];
   }
   collect_marks(c);
+  if (keep_fwd && nj == 1 && G == 8) {
+    c->fwd = new FwdState{*jobs[0].P, *jobs[0].cam, *jobs[0].prm, pl, tn};
+  }
+  return 0;
+}
+
+// Gradient of the last single-view render (see splhip_render_backward).
+int backward_impl(splhip_ctx *c, const float *grad, const splhip_grads *out, hipStream_t s) {
+  FwdState &F = *c->fwd;
+  Plan &pl = F.pl;
+  const int64_t n = pl.ntot;
+  Workspace &ws = c->ws;
+  double *gscr = ws.get_as<double>(S_GRAD0, static_cast<size_t>(std::max<int64_t>(n, 1)) * 6);
+  launch_zero_f64(gscr, n * 6, s);
+  uint32_t *tot = ws.get_as<uint32_t>(S_TOTALS, 64);
+  for (int p = 0; p < 2; ++p) {
+    if (!pl.m[p]) continue;
+    // work items: list segments of at most kAdjSeg entries
+    const uint32_t nu = pl.nunits;
+    uint32_t *cnt = ws.get_as<uint32_t>(S_GRAD1, 2 * (static_cast<size_t>(nu) + 1));
+    uint32_t *soff = cnt + nu + 1;
+    launch_seg_count(pl.bin_off[p], nu, cnt, s);
+    scan_u32(cnt, soff, nu, soff + nu, ws, s);
+    hip_check(hipMemcpyAsync(tot, soff + nu, 4, hipMemcpyDeviceToDevice, s), "segment total");
+    read_totals(c, tot, 1, s);
+    const uint32_t nseg = c->h_tot[0];
+    if (nseg == 0) continue;
+    uint32_t *seg = ws.get_as<uint32_t>(S_GRAD3, 2 * static_cast<size_t>(nseg));
+    launch_seg_emit(pl.bin_off[p], nu, soff, seg, seg + nseg, s);
+    double *state = p == 0 ? ws.get_as<double>(S_GRAD4, static_cast<size_t>(nseg) * 3 * 64 * 3)
+                           : nullptr;
+    const SplatArgs sa = splat_args(c, pl, p, F.tn);
+    launch_adjoint(p == 1, seg, seg + nseg, nseg, soff, nu, sa, grad, pl.order[p], gscr, state, s);
+  }
+  const uint8_t *act = ws.get_as<uint8_t>(S_ACT, static_cast<size_t>(std::max<int64_t>(n, 1)));
+  launch_chain(pl.d_pa, n, act, gscr, *out, s);
   return 0;
 }
 
@@ -859,6 +918,7 @@ int splhip_stage_times(splhip_ctx *c, float *ms, int64_t *counts) {
 
 void splhip_destroy(splhip_ctx *c) {
   if (!c) return;
+  delete c->fwd;
   (void)hipSetDevice(c->device);
   (void)hipDeviceSynchronize();
   for (auto &e : c->ev_pool) (void)hipEventDestroy(e);
@@ -1017,7 +1077,7 @@ int splhip_render(splhip_ctx *c, const splhip_particles *P, const splhip_camera 
   try {
     hip_check(hipSetDevice(c->device), "hipSetDevice");
     const JobIn job{P, cam, prm, pic};
-    return render_jobs(c, 1, &job, static_cast<hipStream_t>(stream));
+    return render_jobs(c, 1, &job, static_cast<hipStream_t>(stream), true);
   } catch (const HipError &e) {
     return hip_fail(c, e);
   }
@@ -1048,7 +1108,7 @@ int splhip_render_host(splhip_ctx *c, const splhip_particle_sim *hp, int64_t n,
     P.n = n;
     if (int rc = check_inputs(c, &P, cam, prm)) return rc;
     const JobIn job{&P, cam, prm, pic};
-    if (int rc = render_jobs(c, 1, &job, s)) return rc;
+    if (int rc = render_jobs(c, 1, &job, s, false)) return rc;
     hip_check(hipMemcpyAsync(host_pic, pic, npix * sizeof(float), hipMemcpyDeviceToHost, s),
               "D2H image");
     hip_check(hipStreamSynchronize(s), "sync");
@@ -1065,6 +1125,7 @@ int splhip_project(splhip_ctx *c, const splhip_particles *P, const splhip_camera
   try {
     hip_check(hipSetDevice(c->device), "hipSetDevice");
     hipStream_t s = static_cast<hipStream_t>(stream);
+    drop_fwd(c);
     const JobIn job{P, cam, prm, nullptr};
     Plan pl;
     make_plan(c, 1, &job, 0, pl, s);
@@ -1085,6 +1146,7 @@ int splhip_bins(splhip_ctx *c, const splhip_particles *P, const splhip_camera *c
     hip_check(hipSetDevice(c->device), "hipSetDevice");
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int p = ordered ? 0 : 1;
+    drop_fwd(c);
     const JobIn job{P, cam, prm, nullptr};
     Plan pl;
     make_plan(c, 1, &job, 0, pl, s);  // units = reference tiles
@@ -1129,6 +1191,24 @@ int splhip_bins(splhip_ctx *c, const splhip_particles *P, const splhip_camera *c
   return 0;
 }
 
+int splhip_render_backward(splhip_ctx *c, const float *dL_dpic, const splhip_grads *out,
+                           void *stream) {
+  if (!c) return SPLHIP_E_ARG;
+  if (!c->fwd) return fail(c, SPLHIP_E_STATE, "no single-view render (8 px regions) to differentiate");
+  if (!dL_dpic || !out || !out->x || !out->y || !out->z || !out->hsml || !out->inten || !out->col)
+    return fail(c, SPLHIP_E_ARG, "null gradient buffer");
+  const splhip_params &prm = c->fwd->prm;
+  for (int t = 0; t < prm.ptypes; ++t)
+    if (prm.col_vector[t] && (!out->c1 || !out->c2))
+      return fail(c, SPLHIP_E_ARG, "vector colour type without c1/c2 gradient buffers");
+  try {
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    return backward_impl(c, dL_dpic, out, static_cast<hipStream_t>(stream));
+  } catch (const HipError &e) {
+    return hip_fail(c, e);
+  }
+}
+
 int splhip_render_batched(splhip_ctx *c, int32_t njobs, const splhip_job *jobs, void *stream) {
   if (!c || njobs < 0 || (njobs > 0 && !jobs)) return fail(c, SPLHIP_E_ARG, "null argument");
   if (njobs == 0) return 0;
@@ -1141,7 +1221,7 @@ int splhip_render_batched(splhip_ctx *c, int32_t njobs, const splhip_job *jobs, 
   }
   try {
     hip_check(hipSetDevice(c->device), "hipSetDevice");
-    return render_jobs(c, njobs, in.data(), static_cast<hipStream_t>(stream));
+    return render_jobs(c, njobs, in.data(), static_cast<hipStream_t>(stream), false);
   } catch (const HipError &e) {
     return hip_fail(c, e);
   }

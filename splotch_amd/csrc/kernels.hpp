@@ -99,6 +99,23 @@ void launch_splat8(bool additive, int variant, const uint32_t *items, uint32_t n
                    const SplatArgs &a, hipStream_t s);
 void launch_splat_heavy(bool additive, const uint32_t *items, uint32_t nheavy, const SplatArgs &a,
                         hipStream_t s);
+// Differentiable form (adjoint.hip): screen-space gradient accumulation over
+// the forward's work items (gscr: n x 6 doubles, zeroed by the caller), the
+// chain to the particle fields of job 0, and an FP64 fill.
+// Backward work items: list segments of at most kAdjSeg entries per unit
+// (cnt[u] segments; soff = exclusive scan of cnt).
+constexpr uint32_t kAdjSeg = 2048;
+void launch_seg_count(const uint32_t *bin_off, uint32_t nunits, uint32_t *cnt, hipStream_t s);
+void launch_seg_emit(const uint32_t *bin_off, uint32_t nunits, const uint32_t *soff,
+                     uint32_t *seg_unit, uint32_t *seg_first, hipStream_t s);
+// segstate: ordered mode only, nseg * 3 * 64 * 3 doubles.
+void launch_adjoint(bool additive, const uint32_t *seg_unit, const uint32_t *seg_first,
+                    uint32_t nseg, const uint32_t *soff, uint32_t nunits, const SplatArgs &a,
+                    const float *grad, const uint32_t *order, double *gscr, double *segstate,
+                    hipStream_t s);
+void launch_chain(const ProjectArgs *jobs, int64_t n, const uint8_t *act, const double *gscr,
+                  const splhip_grads &out, hipStream_t s);
+void launch_zero_f64(double *p, int64_t n, hipStream_t s);
 // G: region size (16, 32 or 64 px); additive: a_eq_e mode.  Single job.
 void launch_splat(int G, bool additive, uint32_t nbins, const SplatArgs &a, hipStream_t s);
 
