@@ -11,9 +11,18 @@ in HBM: project + colorize, depth sort, binning, splat of both passes.  The
 timed region is K frames bracketed by device synchronisation.
 
 The JSON line also carries:
-  roofline      for the dominant kernel (the ordered-pass splat), algorithmic
-                bytes per launch = 24 B per active particle (x, y, r, e) +
-                12 B per pixel written, over that launch's HIP-event duration;
+  roofline      for the dominant kernel (k_splat8 of the ordered pass, the
+                8x8-block splat of the normal work items), algorithmic bytes
+                per launch = 24 B per active ordered particle + 12 B per pixel,
+                over that launch's duration from HIP events the library
+                records around it on its own stream (average of the profiled
+                frames); traffic from rocprofv3 FETCH_SIZE + WRITE_SIZE when
+                --traffic-kib gives them (KiB per dispatch, already scaled);
+  batched       the batched workload (64 views, synth.batch_jobs) as one
+                splhip_render_batched call vs one splhip_render per view;
+  fwd_bwd       the differentiable workload: 2M particles, 512^2, additive,
+                and a 500k ordered variant; loss sum (pic - target)^2 against
+                the render of a perturbed copy; forward vs forward + backward;
   cpu_baseline  the genuine reference host renderer (oracle/_ref, built from
                 the Splotch sources) — or the oracle restatement if that
                 library is absent — timed on this host's cores on a bounded
@@ -162,6 +171,64 @@ def bench_batched(S, R, torch, steps, warmup, n_jobs=64):
             "batched_vs_plain": round(res["batched"][1] / res["plain"][1], 3)}
 
 
+def bench_backward(S, R, torch, steps, warmup):
+    """Differentiable workload: forward, and forward + backward of
+    L = sum (pic - target)^2, target = render of the same field with
+    x += N(0, 0.05) per axis and I *= U(0.8, 1.2)."""
+    import numpy as np
+    from splotch_amd import synth
+    out = {}
+    for name, n, seed, aeq in (("additive_2M", 2_000_000, 1300, True),
+                               ("ordered_500k", 500_000, 1301, False)):
+        fld = synth.plummer_field(n, seed)
+        rng = np.random.default_rng(1302)
+        tf = synth.Field(*(getattr(fld, k) + rng.normal(0, 0.05, n).astype(np.float32)
+                           for k in ("x", "y", "z")), fld.hsml,
+                         (fld.inten * rng.uniform(0.8, 1.2, n)).astype(np.float32), fld.col)
+        view = synth.default_view(512)
+        cam = S.camera_setup(view)
+        prm = S.params(a_eq_e=aeq)
+        dev = S.to_device(fld)
+        pic = torch.empty(512 * 512 * 3, dtype=torch.float32, device="cuda")
+        target = torch.empty_like(pic)
+        R.render(S.to_device(tf), cam, prm, target)
+
+        def fwd():
+            R.render(dev, cam, prm, pic)
+
+        def fwd_bwd():
+            R.render(dev, cam, prm, pic)
+            return R.backward((2.0 * (pic - target)).contiguous(), n)
+
+        res = {}
+        for key, fn in (("fwd", fwd), ("fwd_bwd", fwd_bwd)):
+            for _ in range(warmup):
+                fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                fn()
+            torch.cuda.synchronize()
+            res[key] = (time.perf_counter() - t0) * 1e3 / steps
+        log("fwd+bwd %s: fwd %.2f ms, fwd+bwd %.2f ms" % (name, res["fwd"], res["fwd_bwd"]))
+        out[name] = {"particles": n, "image": [512, 512], "additive": aeq,
+                     "ms_fwd": round(res["fwd"], 3), "ms_fwd_bwd": round(res["fwd_bwd"], 3),
+                     "fwd_bwd_over_fwd": round(res["fwd_bwd"] / res["fwd"], 3)}
+    return out
+
+
+def copy_calibration(torch, steps=20):
+    """A plain device copy of 1 GiB, for scaling rocprofv3 FETCH_SIZE /
+    WRITE_SIZE: each dispatch reads and writes exactly 1 GiB."""
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": 4 * n, "dispatches": steps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,6 +241,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched workload")
+    ap.add_argument("--no-backward", action="store_true", help="skip the fwd+bwd workload")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (profiler counter calibration)")
+    ap.add_argument("--traffic-kib", type=float, nargs=2, default=None,
+                    metavar=("FETCH", "WRITE"),
+                    help="scaled FETCH_SIZE and WRITE_SIZE (KiB) of one ordered k_splat8 "
+                         "dispatch, from rocprofv3, reported as roofline.traffic")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -184,6 +258,10 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.copy_calibration:
+        copy_calibration(torch)
+        return
 
     import splotch_amd as S
     from splotch_amd import synth
@@ -234,9 +312,21 @@ def main():
     torch.cuda.synchronize()
 
     W, H = view.xres, view.yres
-    splat_ms = stages["ordered_splat"]
+    splat_ms = stages["ordered_k_splat8"]
     splat_bytes = 24 * counts["ordered_active"] + 12 * W * H
     achieved = splat_bytes / (splat_ms * 1e-3) / 1e9
+    traffic, traffic_src = None, None
+    if args.traffic_kib:
+        traffic = int(round((args.traffic_kib[0] + args.traffic_kib[1]) * 1024))
+        traffic_src = "--traffic-kib"
+    else:  # the committed rocprofv3 summary of this workload, if present
+        prof = os.path.join(ROOT, "profiles", "pmc_traffic_gas_stars_16M_2048.md")
+        default = (args.n_gas, args.n_stars, args.res) == (8_000_000, 8_000_000, 2048)
+        if default and os.path.exists(prof):
+            for line in open(prof):
+                if line.startswith("traffic_bytes_per_dispatch k_splat8<0, 4>:"):
+                    traffic = int(line.split(":")[1])
+                    traffic_src = "profiles/pmc_traffic_gas_stars_16M_2048.md"
     frame_bytes = 25 * field.n + 12 * W * H
     log("frame %.3f ms (events %.3f ms); stages %s; counts %s" % (ms_frame, ev_ms, stages,
                                                                   counts))
@@ -257,19 +347,24 @@ def main():
         "config": {"workload": "gas_stars_16M_2048sq", "particles": field.n,
                    "gas_ordered": args.n_gas, "stars_additive": args.n_stars,
                    "image": [W, H], "tile_size": prm.tile_size,
-                   "hip_region": prm.hip_region or 32, "parallelism": "single GPU"},
+                   "hip_region": prm.hip_region or 8, "parallelism": "single GPU"},
         "ms_per_frame": round(ms_frame, 4),
         "ms_per_frame_events": round(ev_ms, 4),
         "stages_ms": stages,
         "counts": counts,
         "frame_hbm_fraction": round(frame_bytes / (ms_frame * 1e-3) / 1e9 / HBM_PEAK_GBS, 6),
-        "roofline": {"bound": "hbm", "kernel": "k_splat (ordered pass)",
+        "roofline": {"bound": "hbm", "kernel": "k_splat8<ordered> (normal work items)",
                      "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
-                     "bytes_per_launch": splat_bytes, "launch_ms": splat_ms},
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+                     "traffic_source": traffic_src,
+                     "bytes_per_launch": splat_bytes, "launch_ms": splat_ms,
+                     "note": "the splat is VALU/LDS-bound (per-pixel serial update "
+                             "chains); HBM is not its limit"},
     }
     if not args.no_batched:
         out["batched"] = bench_batched(S, R, torch, max(2, args.steps // 2), 1)
+    if not args.no_backward:
+        out["fwd_bwd"] = bench_backward(S, R, torch, max(3, args.steps // 2), 1)
     if not args.no_cpu:
         del dev
         torch.cuda.empty_cache()

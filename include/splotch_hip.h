@@ -219,16 +219,20 @@ int splhip_bins(splhip_ctx *ctx, const splhip_particles *parts, const splhip_cam
 /* The reference times its stages with the tstack timers "3D transform",
    "Particle coloring", "Particle sorting", "Host Chunk preparation" and
    "Host Rendering proper" (splotch_host.cc:737-833, cuda/cuda_render.cu:251-285).
-   With profiling on, each splhip_render records HIP events on its stream at
-   the stage boundaries; splhip_stage_times returns the last frame's GPU time
-   per stage in ms:
+   With profiling on, each render records HIP events at the stage boundaries
+   (a stage may span several intervals of the stream; their times are summed);
+   splhip_stage_times returns the last frame's GPU time per stage in ms:
      [0] project + colorize
      [1] ordered pass: compaction + depth sort     [4] additive pass: compaction
-     [2] ordered pass: binning                     [5] additive pass: binning
-     [3] ordered pass: splat (one k_splat launch)  [6] additive pass: splat
+     [2] ordered pass: binning + work items        [5] additive pass: binning + work items
+     [3] ordered pass: splat (all its kernels)     [6] additive pass: splat (all its kernels)
+   and single kernels, each bracketed by events on the stream it runs on:
+     [7] ordered k_splat8 launch (normal items)    [9] additive k_splat8 launch
+     [8] ordered k_splat_heavy launch (auxiliary   [10] additive k_splat_heavy launch
+         stream, concurrent with [7])
    counts (optional, 4 values): active particles and bin entries of the
    ordered pass, then of the additive pass. */
-#define SPLHIP_NSTAGES 7
+#define SPLHIP_NSTAGES 11
 int splhip_set_profiling(splhip_ctx *ctx, int32_t enable);
 int splhip_stage_times(splhip_ctx *ctx, float *ms, int64_t *counts);
 

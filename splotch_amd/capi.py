@@ -73,7 +73,8 @@ EXPORTS = ["splhip_create", "splhip_destroy", "splhip_last_error", "splhip_camer
            "splhip_trace_items", "splhip_render_batched", "splhip_render_backward"]
 
 STAGES = ("project", "ordered_sort", "ordered_bin", "ordered_splat", "additive_compact",
-          "additive_bin", "additive_splat")
+          "additive_bin", "additive_splat", "ordered_k_splat8", "ordered_k_splat_heavy",
+          "additive_k_splat8", "additive_k_splat_heavy")
 
 
 def lib():

@@ -698,15 +698,21 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
     ensure_aux(c);
     hip_check(hipEventRecord(c->ev_fork, s), "record fork");
     hip_check(hipStreamWaitEvent(c->aux, c->ev_fork, 0), "wait fork");
+    mark(c, p == 0 ? 8 : 10, true, c->aux);
     launch_splat_heavy(p == 1, items, nheavy, sa, c->aux);
+    mark(c, p == 0 ? 8 : 10, false, c->aux);
     hip_check(hipEventRecord(c->ev_join, c->aux), "record join");
     // The normal items fill every CU; the heavy workgroups (the critical
     // path) must get their slots first, so the normal launch waits briefly.
     launch_delay(40, s);
+    mark(c, p == 0 ? 7 : 9, true, s);
     launch_splat8(p == 1, tn.variant, items + nheavy, nitems - nheavy, sn, s);
+    mark(c, p == 0 ? 7 : 9, false, s);
     hip_check(hipStreamWaitEvent(s, c->ev_join, 0), "wait join");
   } else {
+    mark(c, p == 0 ? 7 : 9, true, s);
     launch_splat8(p == 1, tn.variant, items, nitems, sn, s);
+    mark(c, p == 0 ? 7 : 9, false, s);
   }
   mark(c, p == 0 ? 3 : 6, false, s);
   if (c->trace) {
