@@ -25,8 +25,10 @@
  *                     single-particle and two-particle closed forms
  *   orc_render64      FP64 twin, checked against orc_render within the exptable
  *                     bound and by additivity
- *   orc_adjoint64     FP64 adjoint, checked against FP64 central differences of
- *                     orc_render64_smooth
+ *   orc_adjoint64     FP64 adjoint; with pyoracle.chain64 (the FP64 chain rule
+ *                     through pyoracle.project64), checked against FP64 central
+ *                     differences (h = 1e-4) of orc_render64 after project64 on
+ *                     200 random particles, both modes (test_oracle_adjoint.py)
  */
 #include <cmath>
 #include <cstdint>
