@@ -67,8 +67,10 @@ typedef struct {
                                               1 = per particle, 2 = 4-particle groups,
                                               3 = software pipelined, 4 = chunked branch-free */
   int32_t hip_heavy;                       /* hip_heavy: list length from which an 8x8 block is
-                                              splatted by a producer/consumer workgroup
-                                              (0 = auto, < 0 = never) */
+                                              splatted by producer/consumer workgroups, one
+                                              per channel (0 = auto, < 0 = never) */
+  int32_t hip_heavy_max;                   /* hip_heavy_max: at most this many such workgroups
+                                              per pass, the heaviest (0 = auto: 256) */
 } splhip_params;
 
 /* SoA particle fields (device pointers), post particle_normalize:

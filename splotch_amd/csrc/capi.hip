@@ -614,7 +614,7 @@ void stage_bins(splhip_ctx *c, Plan &pl, int p, hipStream_t s) {
 
 struct Tuning {
   int variant;
-  uint32_t split, heavy;
+  uint32_t split, heavy, heavy_max;
 };
 
 Tuning tuning_of(const splhip_params *prm) {
@@ -623,6 +623,8 @@ Tuning tuning_of(const splhip_params *prm) {
   t.split = prm->hip_split > 0 ? static_cast<uint32_t>(prm->hip_split) : 4096u;
   t.heavy = prm->hip_heavy < 0 ? 0xffffffffu
             : prm->hip_heavy > 0 ? static_cast<uint32_t>(prm->hip_heavy) : 32768u;
+  // two 512-thread heavy workgroups fit a CU; more than two per CU queue
+  t.heavy_max = prm->hip_heavy_max > 0 ? static_cast<uint32_t>(prm->hip_heavy_max) : 256u;
   return t;
 }
 
@@ -788,7 +790,9 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
       read_totals(c, tot + T_ITEMS, 4, s);  // sync 3
       for (int p = 0; p < 2; ++p) {
         pl.nitems[p] = pl.m[p] ? c->h_tot[p] : 0;
-        pl.nheavy[p] = pl.m[p] ? c->h_tot[2 + p] : 0;
+        // the heaviest items (sorted first) up to the cap; a unit's channel
+        // items are independent, so the cut may fall between them
+        pl.nheavy[p] = pl.m[p] ? std::min(c->h_tot[2 + p], tn.heavy_max) : 0;
       }
     }
     for (int p = 0; p < 2; ++p)
@@ -1020,6 +1024,7 @@ int splhip_params_default(splhip_params *p) {
   p->hip_split = 0;
   p->hip_variant = 0;
   p->hip_heavy = 0;
+  p->hip_heavy_max = 0;
   return 0;
 }
 
@@ -1057,6 +1062,7 @@ int splhip_params_from_par(const char *par_text, float b_brightness, splhip_para
     p->hip_split = par_num<int>(d, "hip_split", 0);
     p->hip_variant = par_num<int>(d, "hip_variant", 0);
     p->hip_heavy = par_num<int>(d, "hip_heavy", 0);
+    p->hip_heavy_max = par_num<int>(d, "hip_heavy_max", 0);
   } catch (const ParError &) {
     return SPLHIP_E_ARG;
   }

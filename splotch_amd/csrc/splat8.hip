@@ -62,6 +62,10 @@ __device__ __forceinline__ float xexpm1_bf(const float *tab, float arg, float ex
   return (fabsf(arg) < tl) ? arg : opv - 1.f;
 }
 
+// Absorption range of the fast table path: att * a * expfac stays in [0, 2^20)
+// for att in [0, 1] (expfac = 2^20 / -20).
+constexpr float kFastA = -19.5f;
+
 // exptable::expm1 (splotchutils.h:203-214) split in two for pipelining:
 // xexp_index gives the table index for p = arg*expfac (0 where the table does
 // not apply), and xexpm1_finish combines the two table values t1, t2 fetched
@@ -92,10 +96,63 @@ __device__ __forceinline__ void walk_chunks(int nb, int cx, int cy, const uint32
                                             const float (*pre2)[kPad], const float (*pre1)[kPad],
                                             const float *tab, float va0, float va1, float va2,
                                             float vq0, float vq1, float vq2, float vmax,
-                                            float expfac, float tl, float &l0, float &l1,
-                                            float &l2) {
+                                            uint64_t fastmask, float expfac, float tl, float &l0,
+                                            float &l1, float &l2) {
   constexpr bool C0 = (CH == 3 || CH == 0), C1 = (CH == 3 || CH == 1), C2 = (CH == 3 || CH == 2);
   for (int k0 = 0; k0 < nb; k0 += J) {
+    if (!ADD) {
+      // Fast chunk: every particle of the chunk has all absorptions a_c in
+      // [kFastA, 0], so arg = att*a_c (att in [0, 1]) gives p = arg*expfac in
+      // [0, 2^20): the table index is trunc(p) with no range cases.  The
+      // update is written l += ((l - q) * m1) * m2 with
+      //   Taylor branch: m1 = a, m2 = att;  expm1 branch: m1 = em, m2 = 1
+      //   (x * 1 == x);  outside the footprint: m2 = 0 (l + +-0 == l, as l is
+      //   never -0),
+      // which is render_new's update bit for bit (splotch_host.cc:648-658).
+      const int cnt = min(J, nb - k0);
+      const uint64_t need = ((1ull << cnt) - 1ull) << k0;
+      if ((fastmask & need) == need) {
+        float m2[J], g0[J], g1[J], g2[J], u0[J], w0[J], u1[J], w1[J], u2[J], w2[J];
+        bool tay[J];
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+          const int k = min(k0 + j, nb - 1);  // uniform; out-of-batch slots masked
+          const uint32_t mk = cmask[cx][k];
+          const uint32_t mv = (k0 + j < nb) ? mk : 0u;
+          const bool in = (mv >> cy) & 1u;
+          const float at = pre1[cy][k] * pre2[cx][k];
+          tay[j] = (lane_f(vmax, k) * at) < tl;
+          m2[j] = in ? (tay[j] ? at : 1.f) : 0.f;
+          if (C0) { g0[j] = at * lane_f(va0, k); const int i = static_cast<int>(g0[j] * expfac); u0[j] = tab[i >> 10]; w0[j] = tab[1024 + (i & 1023)]; }
+          if (C1) { g1[j] = at * lane_f(va1, k); const int i = static_cast<int>(g1[j] * expfac); u1[j] = tab[i >> 10]; w1[j] = tab[1024 + (i & 1023)]; }
+          if (C2) { g2[j] = at * lane_f(va2, k); const int i = static_cast<int>(g2[j] * expfac); u2[j] = tab[i >> 10]; w2[j] = tab[1024 + (i & 1023)]; }
+        }
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+          const int k = min(k0 + j, nb - 1);
+          // particle values read into scalars before the selects that use them
+          if (C0) {
+            const float a = lane_f(va0, k), q = lane_f(vq0, k);
+            const float em = (fabsf(g0[j]) < tl) ? g0[j] : u0[j] * w0[j] - 1.f;
+            const float m1 = tay[j] ? a : em;
+            l0 = l0 + ((l0 - q) * m1) * m2[j];
+          }
+          if (C1) {
+            const float a = lane_f(va1, k), q = lane_f(vq1, k);
+            const float em = (fabsf(g1[j]) < tl) ? g1[j] : u1[j] * w1[j] - 1.f;
+            const float m1 = tay[j] ? a : em;
+            l1 = l1 + ((l1 - q) * m1) * m2[j];
+          }
+          if (C2) {
+            const float a = lane_f(va2, k), q = lane_f(vq2, k);
+            const float em = (fabsf(g2[j]) < tl) ? g2[j] : u2[j] * w2[j] - 1.f;
+            const float m1 = tay[j] ? a : em;
+            l2 = l2 + ((l2 - q) * m1) * m2[j];
+          }
+        }
+        continue;
+      }
+    }
     float att[J], g0[J], g1[J], g2[J], p0[J], p1[J], p2[J];
     float u0[J], w0[J], u1[J], w1[J], u2[J], w2[J];
     uint32_t msk[J];
@@ -263,12 +320,14 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
 
   float l0 = 0.f, l1 = 0.f, l2 = 0.f;
   const uint32_t e0 = uniform(a.bin_off[u]), e1 = uniform(a.bin_off[u + 1]);
-  // long lists are the critical path: let their waves issue ahead of the rest
-  if (e1 - e0 >= a.prio_len) __builtin_amdgcn_s_setprio(3);
+  // long lists are near the critical path: let their waves issue ahead of the
+  // rest (priority 2; the heavy-block workgroups, the critical path, run at 3)
+  if (e1 - e0 >= a.prio_len) __builtin_amdgcn_s_setprio(2);
   for (uint32_t base = e0; base < e1; base += 64) {
     const int nb = static_cast<int>(uniform(min<uint32_t>(64u, e1 - base)));
     // Phase A: lane = particle (splotch_host.cc:462-476 + 596-607, tile frame)
     float va0 = 0.f, va1 = 0.f, va2 = 0.f, vq0 = 0.f, vq1 = 0.f, vq2 = 0.f, vmax = 0.f;
+    bool vfast = false;
     if (lane < nb) {
       const Rec R = a.recs[a.ent[base + lane]];
       const float rfacr = R.r * a.rfac;
@@ -320,16 +379,19 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
         vq1 = R.e1 / (R.e1 + g);
         vq2 = R.e2 / (R.e2 + g);
         vmax = fmaxf(fabsf(va0), fmaxf(fabsf(va1), fabsf(va2)));
+        vfast = va0 >= kFastA && va0 <= 0.f && va1 >= kFastA && va1 <= 0.f && va2 >= kFastA &&
+                va2 <= 0.f;
       }
     }
     wave_sync();
+    const uint64_t fastmask = __ballot(vfast);
     // Phase B: lane = pixel, particles in list order.
     if (VAR == 4) {
       switch (chan) {
-        case 0: walk_chunks<ADD, 0, 8>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, expfac, tl, l0, l1, l2); break;
-        case 1: walk_chunks<ADD, 1, 8>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, expfac, tl, l0, l1, l2); break;
-        case 2: walk_chunks<ADD, 2, 8>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, expfac, tl, l0, l1, l2); break;
-        default: walk_chunks<ADD, 3, (ADD ? 8 : 4)>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, expfac, tl, l0, l1, l2); break;
+        case 0: walk_chunks<ADD, 0, 8>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, fastmask, expfac, tl, l0, l1, l2); break;
+        case 1: walk_chunks<ADD, 1, 8>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, fastmask, expfac, tl, l0, l1, l2); break;
+        case 2: walk_chunks<ADD, 2, 8>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, fastmask, expfac, tl, l0, l1, l2); break;
+        default: walk_chunks<ADD, 3, (ADD ? 8 : 4)>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, fastmask, expfac, tl, l0, l1, l2); break;
       }
       wave_sync();
       continue;
@@ -444,51 +506,67 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
 }
 
 // ---------------------------------------------------------------------------
-// Heavy blocks: producer / consumer workgroup (16 waves) per 8x8 block.
+// Heavy blocks: producer / consumer workgroups, one per (8x8 block, channel).
 //
 // A block whose list holds hundreds of thousands of particles is the critical
-// path of the frame: its pixels' update chains are serial.  Here the serial
-// part is all a consumer wave does: waves 0..2 own channel 0..2 of the block's
-// 64 pixels and, per particle, apply
-//     l += (l - q) * m1 * m2       (two roundings, left to right)
-// with multipliers prepared by the producers so that this reproduces
-// render_new's update bit for bit (splotch_host.cc:615, 648-658):
-//   Taylor branch  m1 = a,  m2 = att     ->  ((l - q) * a) * att
-//   expm1 branch   m1 = expm1(att * a), m2 = 1  ->  (l - q) * em   (x*1 == x)
-//   additive       l += a * m2, m2 = att
-//   outside        m2 = -1: no update.
-// Waves 3..15 (832 lanes) run a three-stage pipeline ahead of the consumers:
-// particle set-up three batches ahead (tile-frame position, box, Gaussian
-// width, emission / absorption; splotch_host.cc:462-476, 591-632), the
-// separable column / row tables two batches ahead (splotch_host.cc:596-607),
-// and the per-fragment multipliers one batch ahead.
-constexpr int kHB = 32;  // particles per batch
-constexpr int kHWaves = 16;
-constexpr int kHProd = (kHWaves - 3) * 64;
+// path of the frame: its pixels' update chains are serial.  Such a block is
+// issued as three work items, one per colour channel (the channels' chains are
+// independent), each a 512-thread workgroup.  Wave 0 is the consumer: lane =
+// pixel, and per particle it only applies
+//     l += ((l - q) * m1) * m2     (ordered)       l += t     (additive)
+// with operands that reproduce render_new's update bit for bit
+// (splotch_host.cc:615, 648-658):
+//   Taylor branch   m1 = a,  m2 = att           -> ((l - q) * a) * att
+//   expm1 branch    m1 = expm1(att * a), m2 = 1 -> (l - q) * em      (x*1 == x)
+//   additive        t = a * att
+//   outside         m1 = m2 = t = 0             -> l + (+-0) == l
+// (l starts at +0 and a float sum is -0 only if both terms are -0, so l is
+// never -0 and adding a zero leaves it unchanged, as the reference's skip.)
+// Waves 1..7 run a pipeline ahead of the consumer: particle set-up three
+// batches ahead (tile-frame position, box, Gaussian width, emission /
+// absorption; splotch_host.cc:462-476, 591-632), the separable column / row
+// tables two batches ahead (splotch_host.cc:596-607), and the per-fragment
+// operands one batch ahead.  Records reach the set-up through LDS in
+// super-batches of 256 that wave 1 loads about eight steps before their use,
+// so the gather latency stays off the critical path.
+constexpr int kHB = 32;        // particles per batch
+constexpr int kHThreads = 512;
+constexpr int kHProd = kHThreads - 64;  // producer lanes
+constexpr int kSB = 8;         // batches per record super-batch (kSB * kHB == 256)
 
 struct HeavyPart {
   float posx, posy, radsq, stp;
   int bx0, bx1, by0, by1;
 };
 
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <int ADD>
-__global__ __launch_bounds__(1024) void k_splat_heavy(SplatArgs a, const uint32_t *items,
-                                                      uint32_t nheavy) {
+__global__ __launch_bounds__(kHThreads) void k_splat_heavy(SplatArgs a, const uint32_t *items,
+                                                           uint32_t nheavy) {
   __shared__ float s_tab[2048];
-  __shared__ HeavyPart s_part[4][kHB];
-  __shared__ float4 s_cst[4][3][kHB];  // [slot][channel][k] = (a, q, maxa, 0)
+  __shared__ Rec s_rec[2][kSB * kHB];  // staged records, two super-batches
+  __shared__ HeavyPart s_part[2][64];  // 64: every lane of wave 1 stores
+  __shared__ float4 s_cst[4][64];      // [slot][k] = (a, q, maxa, fast)
   __shared__ uint32_t s_cm[2][kHB][8];  // column row-masks
   __shared__ float s_p2[2][kHB][8];     // column factors
   __shared__ float s_p1[2][kHB][8];     // row factors
-  __shared__ float s_m2[2][kHB][64];
-  __shared__ float s_m1[ADD ? 1 : 2][ADD ? 1 : 3][ADD ? 1 : kHB][64];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const bool producer = wave >= 3;
-  const int ptid = tid - 192;
-  for (int i = tid; i < 2048; i += 1024) s_tab[i] = a.exptab[i];
+  __shared__ float s_m2[ADD ? 1 : 2][ADD ? 1 : kHB][64];
+  __shared__ float s_m1[2][kHB][64];    // additive: t
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = static_cast<int>(uniform(static_cast<uint32_t>(tid >> 6)));
+  const bool producer = wave >= 1;
+  const int ptid = tid - 64;
+  for (int i = tid; i < 2048; i += kHThreads) s_tab[i] = a.exptab[i];
   const uint32_t it = blockIdx.x;
   const uint64_t t_start = a.trace ? __builtin_amdgcn_s_memrealtime() : 0;
-  const uint32_t u = uniform(items[it]) >> 2;
+  const uint32_t item = uniform(items[it]);
+  const uint32_t u = item >> 2;
+  const int ch = static_cast<int>(item & 3u);  // 0..2
   const UnitGeom ug = unit_geom_uniform(a, u);
   const int gy = ug.gy, gx = ug.gx, x0s = ug.x0s, y0s = ug.y0s, w = ug.w, h = ug.h;
   const int bxo = gx * 8, byo = gy * 8;
@@ -499,139 +577,221 @@ __global__ __launch_bounds__(1024) void k_splat_heavy(SplatArgs a, const uint32_
   __builtin_amdgcn_s_setprio(3);
 
   auto count = [&](int b) { return min(kHB, n - b * kHB); };
-  auto setup = [&](int b) {  // particle b-batch set-up into slot b % 4
-    const int slot = b & 3;
-    if (ptid < count(b)) {
-      const Rec R = a.recs[a.ent[e0 + b * kHB + ptid]];
-      const float rfacr = R.r * a.rfac;
-      const float posx = R.x - static_cast<float>(x0s), posy = R.y - static_cast<float>(y0s);
-      HeavyPart P;
-      P.posx = posx;
-      P.posy = posy;
-      P.bx0 = max(cvt_i32(posx - rfacr + 1), 0);
-      P.bx1 = min(cvt_i32(posx + rfacr + 1), w);
-      P.by0 = max(cvt_i32(posy - rfacr + 1), 0);
-      P.by1 = min(cvt_i32(posy + rfacr + 1), h);
-      P.radsq = rfacr * rfacr;
-      const float sigma = a.h2sigma * R.r;
-      P.stp = -1.f / (sigma * sigma);
-      s_part[slot][ptid] = P;
-      const float a0 = -R.e0, a1 = -R.e1, a2 = -R.e2;
-      float q0 = 0.f, q1 = 0.f, q2 = 0.f, mx = 0.f;
-      if (!ADD) {
-        const float g = ug.grayabsorb;
-        q0 = R.e0 / (R.e0 + g);
-        q1 = R.e1 / (R.e1 + g);
-        q2 = R.e2 / (R.e2 + g);
-        mx = fmaxf(fabsf(a0), fmaxf(fabsf(a1), fabsf(a2)));
-      }
-      s_cst[slot][0][ptid] = make_float4(a0, q0, mx, 0.f);
-      s_cst[slot][1][ptid] = make_float4(a1, q1, mx, 0.f);
-      s_cst[slot][2][ptid] = make_float4(a2, q2, mx, 0.f);
+  // particle set-up of batch b (wave 1).  Branch-free: lanes past the batch
+  // (and batches past the list) store clamped records into slots no stage of
+  // this step reads.
+  auto setup = [&](int b, const Rec &R) {
+    const float rfacr = R.r * a.rfac;
+    const float posx = R.x - static_cast<float>(x0s), posy = R.y - static_cast<float>(y0s);
+    HeavyPart P;
+    P.posx = posx;
+    P.posy = posy;
+    P.bx0 = max(cvt_i32(posx - rfacr + 1), 0);
+    P.bx1 = min(cvt_i32(posx + rfacr + 1), w);
+    P.by0 = max(cvt_i32(posy - rfacr + 1), 0);
+    P.by1 = min(cvt_i32(posy + rfacr + 1), h);
+    P.radsq = rfacr * rfacr;
+    const float sigma = a.h2sigma * R.r;
+    P.stp = -1.f / (sigma * sigma);
+    const float e = ch == 0 ? R.e0 : (ch == 1 ? R.e1 : R.e2);
+    const float av = -e;
+    float q = 0.f, mx = 0.f, fast = 0.f;
+    if (!ADD) {
+      q = e / (e + ug.grayabsorb);
+      mx = fmaxf(fabsf(-R.e0), fmaxf(fabsf(-R.e1), fabsf(-R.e2)));
+      fast = (av >= kFastA && av <= 0.f) ? 1.f : 0.f;
     }
+    s_part[b & 1][lane] = P;
+    s_cst[b & 3][lane] = make_float4(av, q, mx, fast);
   };
+  // Producer stages are unrolled over a lane's items with every LDS read
+  // issued before the arithmetic that uses it, so a step costs a few LDS
+  // round trips instead of one per item.
   auto tables = [&](int b) {  // column ranges / factors and row factors into b & 1
-    const int slot = b & 3, ts = b & 1;
-    if (ptid < count(b) * 16) {
-      const int k = ptid >> 4, j = ptid & 15;
-      const HeavyPart P = s_part[slot][k];
+    const int ts = b & 1, cnt = count(b);
+    const HeavyPart(&part)[64] = s_part[b & 1];
+    constexpr int NT = (kHB * 16 + kHProd - 1) / kHProd;
+    HeavyPart P[NT];
+    int kt[NT], jt[NT];
+    bool vt[NT];
+#pragma unroll
+    for (int i = 0; i < NT; ++i) {
+      const int t = ptid + i * kHProd;
+      vt[i] = t < cnt * 16;
+      const int tc = vt[i] ? t : 0;
+      kt[i] = tc >> 4;
+      jt[i] = tc & 15;
+      P[i] = part[kt[i]];
+    }
+    float arg[NT], dd[NT];
+    bool on[NT];
+    uint32_t mk[NT];
+#pragma unroll
+    for (int i = 0; i < NT; ++i) {
+      const int j = jt[i];
+      mk[i] = 0;
       if (j < 8) {
         const int x = bxo + j;
-        uint32_t m = 0;
-        float p2 = 0.f;
-        if (x >= P.bx0 && x < P.bx1) {
-          const float dxsq = (static_cast<float>(x) - P.posx) * (static_cast<float>(x) - P.posx);
-          const float dy = sqrtf(P.radsq - dxsq);
-          const int lo = max(P.by0, cvt_i32(P.posy - dy + 1));
-          const int hi = min(P.by1, cvt_i32(P.posy + dy + 1));
-          const int r0 = min(max(lo - byo, 0), 8), r1 = min(max(hi - byo, 0), 8);
-          if (hi > lo && r1 > r0) m = ((1u << r1) - 1u) & ~((1u << r0) - 1u);
-          p2 = xop(s_tab, P.stp * dxsq, expfac);
-        }
-        s_cm[ts][k][j] = m;
-        s_p2[ts][k][j] = p2;
+        on[i] = x >= P[i].bx0 && x < P[i].bx1;
+        const float dxsq = (static_cast<float>(x) - P[i].posx) * (static_cast<float>(x) - P[i].posx);
+        const float dy = sqrtf(P[i].radsq - dxsq);
+        const int lo = max(P[i].by0, cvt_i32(P[i].posy - dy + 1));
+        const int hi = min(P[i].by1, cvt_i32(P[i].posy + dy + 1));
+        const int r0 = min(max(lo - byo, 0), 8), r1 = min(max(hi - byo, 0), 8);
+        if (on[i] && hi > lo && r1 > r0) mk[i] = ((1u << r1) - 1u) & ~((1u << r0) - 1u);
+        arg[i] = P[i].stp * dxsq;
       } else {
         const int y = byo + (j - 8);
-        float p1 = 0.f;
-        if (y >= P.by0 && y < P.by1) {
-          const float dyf = static_cast<float>(y) - P.posy;
-          p1 = xop(s_tab, P.stp * dyf * dyf, expfac);
-        }
-        s_p1[ts][k][j - 8] = p1;
+        on[i] = y >= P[i].by0 && y < P[i].by1;
+        const float dyf = static_cast<float>(y) - P[i].posy;
+        arg[i] = P[i].stp * dyf * dyf;
+      }
+      dd[i] = on[i] ? xop(s_tab, arg[i], expfac) : 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < NT; ++i) {
+      if (!vt[i]) continue;
+      const int k = kt[i], j = jt[i];
+      if (j < 8) {
+        s_cm[ts][k][j] = mk[i];
+        s_p2[ts][k][j] = dd[i];
+      } else {
+        s_p1[ts][k][j - 8] = dd[i];
       }
     }
   };
-  auto fragments = [&](int b) {  // per-fragment multipliers into b & 1
-    const int slot = b & 3, ts = b & 1;
-    const int cnt = count(b);
-    for (int f = ptid; f < cnt * 64; f += kHProd) {
-      const int k = f >> 6, p = f & 63, c = p >> 3, r = p & 7;
-      const bool in = (s_cm[ts][k][c] >> r) & 1u;
-      const float att = s_p1[ts][k][r] * s_p2[ts][k][c];
+  auto fragments = [&](int b) {  // consumer operands of batch b, padded to kHB
+    const int ts = b & 1, cnt = count(b);
+    const float4 *cst = s_cst[b & 3];
+    constexpr int NF = (kHB * 64 + kHProd - 1) / kHProd;
+    uint32_t cm[NF];
+    float q1[NF], q2[NF];
+    float4 cv[NF];
+    int fk[NF], fp[NF];
+    bool fv[NF];
+#pragma unroll
+    for (int i = 0; i < NF; ++i) {
+      const int f = ptid + i * kHProd;
+      fv[i] = f < kHB * 64;
+      const int fc = fv[i] ? f : 0;
+      fk[i] = fc >> 6;
+      fp[i] = fc & 63;
+      const int kk = fk[i] < cnt ? fk[i] : 0;
+      const int c = fp[i] >> 3, r = fp[i] & 7;
+      cm[i] = s_cm[ts][kk][c];
+      q1[i] = s_p1[ts][kk][r];
+      q2[i] = s_p2[ts][kk][c];
+      cv[i] = cst[kk];
+    }
+    float att[NF], g[NF], pg[NF], u1[NF], u2[NF];
+    bool in[NF];
+#pragma unroll
+    for (int i = 0; i < NF; ++i) {
+      const int c = fp[i] >> 3, r = fp[i] & 7;
+      in[i] = fk[i] < cnt && ((cm[i] >> r) & 1u);
+      (void)c;
+      att[i] = q1[i] * q2[i];
+      if (ADD) continue;
+      g[i] = att[i] * cv[i].x;
+      pg[i] = g[i] * expfac;
+      const int ix = xexp_index(pg[i]);
+      u1[i] = s_tab[ix >> 10];
+      u2[i] = s_tab[1024 + (ix & 1023)];
+    }
+#pragma unroll
+    for (int i = 0; i < NF; ++i) {
+      if (!fv[i]) continue;
+      const int k = fk[i], p = fp[i];
       if (ADD) {
-        s_m2[ts][k][p] = in ? att : -1.f;
+        s_m1[ts][k][p] = in[i] ? cv[i].x * att[i] : 0.f;
       } else {
-        const float4 c0 = s_cst[slot][0][k], c1 = s_cst[slot][1][k], c2 = s_cst[slot][2][k];
-        const bool tay = (c0.z * att) < tl;
-        s_m2[ts][k][p] = in ? (tay ? att : 1.f) : -1.f;
-        const float e0v = xexpm1_bf(s_tab, att * c0.x, expfac, tl);
-        const float e1v = xexpm1_bf(s_tab, att * c1.x, expfac, tl);
-        const float e2v = xexpm1_bf(s_tab, att * c2.x, expfac, tl);
-        s_m1[ADD ? 0 : ts][0][k][p] = tay ? c0.x : e0v;
-        s_m1[ADD ? 0 : ts][1][k][p] = tay ? c1.x : e1v;
-        s_m1[ADD ? 0 : ts][2][k][p] = tay ? c2.x : e2v;
+        const bool tay = (cv[i].z * att[i]) < tl;
+        const float em = xexpm1_finish(g[i], pg[i], u1[i], u2[i], tl);
+        s_m2[ADD ? 0 : ts][k][p] = in[i] ? (tay ? att[i] : 1.f) : 0.f;
+        s_m1[ts][k][p] = in[i] ? (tay ? cv[i].x : em) : 0.f;
       }
     }
   };
 
-  float l = 0.f;  // consumer: channel `wave`, pixel `lane`
+  // Records through LDS: wave 1 holds the next super-batch's records and the
+  // one after that's list indices in registers (positions clamped to the list
+  // end), and stores a super-batch into its half of s_rec four steps before
+  // its first set-up.
+  auto entry = [&](int J, int r) {
+    return a.ent[e0 + min(J * (kSB * kHB) + r * 64 + lane, n - 1)];
+  };
+  Rec R[4];
+  uint32_t I[4];
+  if (wave == 1) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) I[r] = entry(0, r);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) R[r] = a.recs[I[r]];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) s_rec[0][r * 64 + lane] = R[r];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) I[r] = entry(1, r);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) R[r] = a.recs[I[r]];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) I[r] = entry(2, r);
+  }
+
+  float l = 0.f;  // consumer: channel ch, pixel lane
+  __syncthreads();  // s_tab, s_rec[0]
   for (int s = -3; s < nbat; ++s) {
     if (producer) {
-      if (s + 3 < nbat) setup(s + 3);
+      if (wave == 1) {
+        const int b = s + 3;
+        setup(b, s_rec[(b / kSB) & 1][(b % kSB) * kHB + (lane & (kHB - 1))]);
+        if ((s + 4) % kSB == 0 && s + 4 > 0) {  // super-batch J starts next step
+          const int J = (s + 4) / kSB;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) s_rec[J & 1][r * 64 + lane] = R[r];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) R[r] = a.recs[I[r]];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) I[r] = entry(J + 2, r);
+        }
+      }
       if (s + 2 >= 0 && s + 2 < nbat) tables(s + 2);
       if (s + 1 >= 0 && s + 1 < nbat) fragments(s + 1);
     } else if (s >= 0) {
-      const int slot = s & 3, fb = s & 1, c = wave;
-      const int cnt = count(s);
-      const float *m2b = &s_m2[fb][0][0];
-      const float *m1b = &s_m1[ADD ? 0 : fb][ADD ? 0 : c][0][0];
-      const float4 *cst = s_cst[slot][c];
-      for (int k0 = 0; k0 < cnt; k0 += 8) {
-        float m1[8], m2[8], cv[8];
+      const int fb = s & 1;
+      const float4 *cst = s_cst[s & 3];
+      const float *m1b = &s_m1[fb][0][0];
+      const float *m2b = &s_m2[ADD ? 0 : fb][0][0];
+#pragma unroll
+      for (int k0 = 0; k0 < kHB; k0 += 8) {
+        float m1[8], m2[8], cq[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-          const int k = min(k0 + j, cnt - 1);
-          const float v = m2b[k * 64 + lane];
-          m2[j] = (k0 + j < cnt) ? v : -1.f;
-          if (!ADD) m1[j] = m1b[k * 64 + lane];
-          cv[j] = ADD ? cst[k].x : cst[k].y;  // a (additive) or q
+          m1[j] = m1b[(k0 + j) * 64 + lane];
+          if (!ADD) {
+            m2[j] = m2b[(k0 + j) * 64 + lane];
+            cq[j] = cst[k0 + j].y;
+          }
         }
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-          const bool in = m2[j] >= 0.f;
-          if (ADD) {
-            const float t = cv[j] * m2[j];
-            l = l + (in ? t : 0.f);
-          } else {
-            const float t = ((l - cv[j]) * m1[j]) * m2[j];
-            l = l + (in ? t : 0.f);
-          }
+          if (ADD) l = l + m1[j];
+          else l = l + ((l - cq[j]) * m1[j]) * m2[j];
         }
       }
     }
-    __syncthreads();
+    lds_barrier();
   }
   if (a.trace && tid == 0) {
     uint64_t *tr = a.trace + 4 * static_cast<uint64_t>(it);
     tr[0] = t_start;
     tr[1] = __builtin_amdgcn_s_memrealtime();
-    tr[2] = static_cast<uint64_t>(n) | (uint64_t(3) << 32);
+    tr[2] = static_cast<uint64_t>(n) | (static_cast<uint64_t>(ch) << 32);
     tr[3] = static_cast<uint64_t>(__smid());
   }
   if (!producer) {
     const int lx = bxo + (lane >> 3), ly = byo + (lane & 7);
     if (lx < min(bxo + 8, w) && ly < min(byo + 8, h)) {
-      float *d = ug.pic + (static_cast<int64_t>(x0s + lx) * ug.yres + (y0s + ly)) * 3 + wave;
+      float *d = ug.pic + (static_cast<int64_t>(x0s + lx) * ug.yres + (y0s + ly)) * 3 + ch;
       *d = ug.accumulate ? *d + l : l;
     }
   }
@@ -647,7 +807,7 @@ __device__ __forceinline__ uint32_t unit_items(const SplatArgs &a, uint32_t u, i
   const UnitGeom g = unit_geom(a, u);
   if (g.gx * 8 >= g.w || g.gy * 8 >= g.h) return 0;
   if (len == 0) return g.need_empty ? 1u : 0u;
-  if (len >= a.heavy_len) return 1u;  // one producer/consumer workgroup for all channels
+  if (len >= a.heavy_len) return 3u;  // one producer/consumer workgroup per channel
   return (len >= (ordered ? split : 2 * split)) ? 3u : 1u;
 }
 
@@ -658,7 +818,7 @@ __global__ __launch_bounds__(256) void k_items_count(SplatArgs a, uint32_t nunit
     uint32_t len;
     const uint32_t k = unit_items(a, static_cast<uint32_t>(u), ordered, split, len);
     cnt[u] = k;
-    if (heavy_cnt && k == 1 && len >= a.heavy_len) atomicAdd(heavy_cnt, 1u);
+    if (heavy_cnt && k > 0 && len >= a.heavy_len) atomicAdd(heavy_cnt, k);
   }
 }
 
@@ -708,8 +868,8 @@ void launch_items_emit(const uint32_t *bin_off, uint32_t nunits, const SplatArgs
 void launch_splat_heavy(bool additive, const uint32_t *items, uint32_t nheavy, const SplatArgs &a,
                         hipStream_t s) {
   if (nheavy == 0) return;
-  if (additive) k_splat_heavy<1><<<nheavy, 1024, 0, s>>>(a, items, nheavy);
-  else k_splat_heavy<0><<<nheavy, 1024, 0, s>>>(a, items, nheavy);
+  if (additive) k_splat_heavy<1><<<nheavy, kHThreads, 0, s>>>(a, items, nheavy);
+  else k_splat_heavy<0><<<nheavy, kHThreads, 0, s>>>(a, items, nheavy);
   hip_check(hipGetLastError(), "k_splat_heavy launch");
 }
 
