@@ -71,6 +71,10 @@ typedef struct {
                                               per channel (0 = auto, < 0 = never) */
   int32_t hip_heavy_max;                   /* hip_heavy_max: at most this many such workgroups
                                               per pass, the heaviest (0 = auto: 256) */
+  int32_t hip_chunk_entries;               /* hip_chunk_entries: a single-view pass with more
+                                              (block, particle) entries is rendered in chunks
+                                              of consecutive render order, each continuing
+                                              the pixels' chains (0 = auto: 2^30) */
 } splhip_params;
 
 /* SoA particle fields (device pointers), post particle_normalize:

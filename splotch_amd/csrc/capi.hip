@@ -485,7 +485,7 @@ void make_plan(splhip_ctx *c, int nj, const JobIn *jobs, int G, Plan &pl, hipStr
 
 // Totals slot layout (device u32): [0..1] pass sizes, [2..3] bin entries,
 // [4..5] work items, [6..7] heavy items.
-enum { T_M = 0, T_E = 2, T_ITEMS = 4, T_HEAVY = 6, T_N = 8 };
+enum { T_M = 0, T_E = 2, T_ITEMS = 4, T_HEAVY = 6, T_E64 = 8, T_N = 12 };
 
 // Project all jobs, then the compaction scan of each present pass and its job
 // offsets in pass order; one host sync reads the pass sizes and offsets.
@@ -588,6 +588,7 @@ void stage_count(splhip_ctx *c, Plan &pl, int p, const Rec *rec, uint32_t *tot, 
   uint32_t *off = ws.get_as<uint32_t>(pslot(p, P_OFF), m);
   launch_gather_count(pl.order[p], m, rec, recs, cnt, pl.d_geom, pl.d_ubase, pl.d_mo[p], pl.nj, s);
   scan_u32(cnt, off, m, tot + T_E + p, ws, s);
+  launch_sum_u64(cnt, m, reinterpret_cast<uint64_t *>(tot + T_E64 + 2 * p), s);
   pl.recs[p] = recs;
   mark(c, p == 0 ? 2 : 5, false, s);
 }
@@ -615,6 +616,7 @@ void stage_bins(splhip_ctx *c, Plan &pl, int p, hipStream_t s) {
 struct Tuning {
   int variant;
   uint32_t split, heavy, heavy_max;
+  uint64_t chunk;  // most (block, particle) entries of one pass chunk
 };
 
 Tuning tuning_of(const splhip_params *prm) {
@@ -625,6 +627,8 @@ Tuning tuning_of(const splhip_params *prm) {
             : prm->hip_heavy > 0 ? static_cast<uint32_t>(prm->hip_heavy) : 32768u;
   // two 512-thread heavy workgroups fit a CU; more than two per CU queue
   t.heavy_max = prm->hip_heavy_max > 0 ? static_cast<uint32_t>(prm->hip_heavy_max) : 256u;
+  t.chunk = prm->hip_chunk_entries > 0 ? static_cast<uint64_t>(prm->hip_chunk_entries)
+                                       : (uint64_t(1) << 30);
   return t;
 }
 
@@ -750,6 +754,74 @@ void drop_fwd(splhip_ctx *c) {
   c->fwd = nullptr;
 }
 
+// A single-view pass whose bin entries exceed the chunk size: consecutive
+// ranges of its render order are binned and splatted one after another, each
+// continuing every pixel's chain from the value the previous range left in
+// the image (accumulate mode 2), which is the unchunked chain bit for bit.
+// An additive pass that adds to an ordered pass renders its chains into a
+// scratch image first and adds it at the end (pic = P_ordered + P_additive).
+void render_chunked_pass(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, uint32_t *tot,
+                         hipStream_t s) {
+  Workspace &ws = c->ws;
+  JobSplat &J = pl.js[0];
+  const JobSplat saved = J;
+  const uint32_t m = pl.m[p];
+  const Rec *recs = pl.recs[p];
+  uint32_t *cnt = ws.get_as<uint32_t>(pslot(p, P_CNT), m);
+  uint32_t *off = ws.get_as<uint32_t>(pslot(p, P_OFF), m);
+  const int64_t npix = static_cast<int64_t>(J.xres) * J.yres * 3;
+  const bool scratch = p == 1 && saved.accumulate[1] == 1;
+  float *target = scratch ? ws.get_as<float>(S_SCRATCH, static_cast<size_t>(npix)) : saved.pic;
+  uint64_t *e64 = reinterpret_cast<uint64_t *>(tot + T_E64 + 2 * p);
+  uint32_t s0 = 0;
+  for (int chunk = 0; s0 < m; ++chunk) {
+    uint32_t n = m - s0;
+    uint32_t e = 0;
+    for (;;) {  // shrink the range until its entries fit a chunk
+      scan_u32(cnt + s0, off, n, tot + T_E + p, ws, s);
+      launch_sum_u64(cnt + s0, n, e64, s);
+      read_totals(c, tot, T_N, s);
+      uint64_t ec;
+      std::memcpy(&ec, c->h_tot + T_E64 + 2 * p, 8);
+      e = c->h_tot[T_E + p];
+      if (ec <= tn.chunk || n == 1) break;
+      n = static_cast<uint32_t>(std::max<uint64_t>(1, static_cast<uint64_t>(n) * tn.chunk / ec * 15 / 16));
+    }
+    J.pic = target;
+    J.accumulate[p] = chunk == 0 ? (scratch ? 0 : saved.accumulate[p]) : 2;
+    J.need_empty[p] = chunk == 0 ? (scratch ? 1 : saved.need_empty[p]) : 0;
+    char *h = staging(c, sizeof(JobSplat));
+    std::memcpy(h, &J, sizeof(JobSplat));
+    hip_check(hipMemcpyAsync(pl.d_js, h, sizeof(JobSplat), hipMemcpyHostToDevice, s),
+              "upload chunk flags");
+    staged(c, s);
+    // this range as the pass: render positions are local to the range
+    pl.m[p] = n;
+    pl.e[p] = e;
+    pl.recs[p] = recs + s0;
+    if (e > 0) {
+      stage_bins(c, pl, p, s);
+      stage_items_count(c, pl, p, tn, tot, s);
+      read_totals(c, tot + T_ITEMS, 4, s);
+      pl.nitems[p] = c->h_tot[p];
+      pl.nheavy[p] = std::min(c->h_tot[2 + p], tn.heavy_max);
+      stage_splat(c, pl, p, tn, s);
+    } else if (chunk == 0 && J.need_empty[p]) {
+      launch_fill(target, 0.f, npix, s);  // no entries at all: the pass leaves zeros
+    }
+    s0 += n;
+  }
+  if (scratch) launch_add_images(saved.pic, target, npix, s);
+  J = saved;
+  char *h = staging(c, sizeof(JobSplat));
+  std::memcpy(h, &J, sizeof(JobSplat));
+  hip_check(hipMemcpyAsync(pl.d_js, h, sizeof(JobSplat), hipMemcpyHostToDevice, s),
+            "restore flags");
+  staged(c, s);
+  pl.m[p] = m;
+  pl.recs[p] = recs;
+}
+
 // The frame pipeline over a batch of jobs; three host syncs per batch (pass
 // sizes, bin entries, work items).
 int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool keep_fwd) {
@@ -778,9 +850,37 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
       stage_order(c, pl, p, rec, act, s);
       stage_count(c, pl, p, rec, tot, s);
     }
+  bool chunked[2] = {false, false};
   if (pl.m[0] || pl.m[1]) {
-    read_totals(c, tot + T_E, 2, s);  // sync 2
-    for (int p = 0; p < 2; ++p) pl.e[p] = pl.m[p] ? c->h_tot[p] : 0;
+    read_totals(c, tot + T_E, T_N - T_E, s);  // sync 2
+    uint64_t e64[2];
+    for (int p = 0; p < 2; ++p) {
+      std::memcpy(&e64[p], c->h_tot + (T_E64 - T_E) + 2 * p, 8);
+      pl.e[p] = pl.m[p] ? c->h_tot[p] : 0;
+      chunked[p] = pl.m[p] && e64[p] > tn.chunk;
+    }
+    if (chunked[0] || chunked[1]) {
+      if (nj != 1 || G != 8)
+        return fail(c, SPLHIP_E_CAPACITY,
+                    "a batched (or 16-64 px region) pass exceeds hip_chunk_entries bin entries; "
+                    "render its views one at a time");
+      for (int p = 0; p < 2; ++p) {
+        if (!pl.m[p]) continue;
+        if (chunked[p]) {
+          render_chunked_pass(c, pl, p, tn, tot, s);
+          c->last_entries[p] = static_cast<int64_t>(e64[p]);
+          continue;
+        }
+        stage_bins(c, pl, p, s);
+        stage_items_count(c, pl, p, tn, tot, s);
+        read_totals(c, tot + T_ITEMS, 4, s);
+        pl.nitems[p] = c->h_tot[p];
+        pl.nheavy[p] = std::min(c->h_tot[2 + p], tn.heavy_max);
+        stage_splat(c, pl, p, tn, s);
+      }
+    }
+  }
+  if ((pl.m[0] || pl.m[1]) && !chunked[0] && !chunked[1]) {
     for (int p = 0; p < 2; ++p)
       if (pl.m[p]) {
         stage_bins(c, pl, p, s);
@@ -805,11 +905,11 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
       launch_fill(jobs[j].pic, 0.f, static_cast<int64_t>(jobs[j].cam->xres) * jobs[j].cam->yres * 3, s);
   }
   for (int p = 0; p < 2; ++p) {
-    c->last_entries[p] = pl.e[p];
+    if (!chunked[p]) c->last_entries[p] = pl.e[p];
     c->last_active[p] = pl.m[p];
   }
   collect_marks(c);
-  if (keep_fwd && nj == 1 && G == 8) {
+  if (keep_fwd && nj == 1 && G == 8 && !chunked[0] && !chunked[1]) {
     c->fwd = new FwdState{*jobs[0].P, *jobs[0].cam, *jobs[0].prm, pl, tn};
   }
   return 0;
@@ -1025,6 +1125,7 @@ int splhip_params_default(splhip_params *p) {
   p->hip_variant = 0;
   p->hip_heavy = 0;
   p->hip_heavy_max = 0;
+  p->hip_chunk_entries = 0;
   return 0;
 }
 
@@ -1063,6 +1164,7 @@ int splhip_params_from_par(const char *par_text, float b_brightness, splhip_para
     p->hip_variant = par_num<int>(d, "hip_variant", 0);
     p->hip_heavy = par_num<int>(d, "hip_heavy", 0);
     p->hip_heavy_max = par_num<int>(d, "hip_heavy_max", 0);
+    p->hip_chunk_entries = par_num<int>(d, "hip_chunk_entries", 0);
   } catch (const ParError &) {
     return SPLHIP_E_ARG;
   }

@@ -64,7 +64,10 @@ struct BinGeom {
 struct JobSplat {
   float *pic;
   int T, xres, yres, ncx, ngm;
-  int accumulate[2];  // per pass (0 ordered, 1 additive): add to what an earlier pass wrote
+  // per pass (0 ordered, 1 additive): 0 the pass writes its chains' values,
+  // 1 it adds them to what an earlier pass wrote, 2 its chains continue from
+  // the image (a later chunk of the same pass)
+  int accumulate[2];
   int need_empty[2];  // per pass: the pass writes every pixel of the job (empty units too)
   float grayabsorb;
 };

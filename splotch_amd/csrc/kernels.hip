@@ -265,6 +265,25 @@ __global__ __launch_bounds__(256) void k_fill(float *p, float v, int64_t n) {
   for (int64_t i = gtid(); i < n; i += gstride()) p[i] = v;
 }
 
+// 64-bit sum of n u32 values (block partial sums added atomically).
+__global__ __launch_bounds__(256) void k_sum_u64(const uint32_t *v, int64_t n,
+                                                 unsigned long long *out) {
+  __shared__ unsigned long long part[256];
+  unsigned long long acc = 0;
+  for (int64_t i = gtid(); i < n; i += gstride()) acc += v[i];
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(out, part[0]);
+}
+
+__global__ __launch_bounds__(256) void k_add_images(float *pic, const float *add, int64_t n) {
+  for (int64_t i = gtid(); i < n; i += gstride()) pic[i] = pic[i] + add[i];
+}
+
 // Bounded wait of `ticks` s_memrealtime ticks (100 MHz): one wave, then exit.
 __global__ __launch_bounds__(64) void k_delay(uint32_t ticks) {
   const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
@@ -517,6 +536,19 @@ void launch_map(const uint32_t *idx, const uint32_t *table, uint32_t *out, int64
 void launch_fill(float *p, float v, int64_t n, hipStream_t s) {
   k_fill<<<grid_for(n), 256, 0, s>>>(p, v, n);
   hip_check(hipGetLastError(), "k_fill launch");
+}
+
+void launch_sum_u64(const uint32_t *v, int64_t n, uint64_t *out, hipStream_t s) {
+  k_fill_u32<<<1, 64, 0, s>>>(reinterpret_cast<uint32_t *>(out), 0u, 2);
+  if (n > 0)
+    k_sum_u64<<<grid_for(n, 256, 2048), 256, 0, s>>>(v, n,
+                                                     reinterpret_cast<unsigned long long *>(out));
+  hip_check(hipGetLastError(), "k_sum_u64 launch");
+}
+
+void launch_add_images(float *pic, const float *add, int64_t n, hipStream_t s) {
+  k_add_images<<<grid_for(n), 256, 0, s>>>(pic, add, n);
+  hip_check(hipGetLastError(), "k_add_images launch");
 }
 
 void launch_delay(uint32_t us, hipStream_t s) {

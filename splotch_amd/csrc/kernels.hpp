@@ -50,7 +50,7 @@ struct SplatArgs {
   // single-job scalars (the G > 8 region kernel)
   float *pic;
   int T, xres, yres, ncx, ngm;
-  int accumulate;
+  int accumulate;  // 0: write the chain's value, 1: add it, 2: continue from the image
   float rfac, h2sigma, expfac, taylorlimit, grayabsorb;
   uint32_t prio_len;  // splat8: lists at least this long run at raised wave priority
   uint64_t *trace;    // splat8: optional per-item {start, end, len | chan << 32, hw id}
@@ -82,6 +82,10 @@ void launch_map(const uint32_t *idx, const uint32_t *table, uint32_t *out, int64
                 hipStream_t s);
 void launch_fill(float *p, float v, int64_t n, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t s);
+// *out (device, 8 bytes) = sum of v[0..n) in 64 bits.
+void launch_sum_u64(const uint32_t *v, int64_t n, uint64_t *out, hipStream_t s);
+// pic[i] += add[i]
+void launch_add_images(float *pic, const float *add, int64_t n, hipStream_t s);
 // A one-wave kernel that returns after `us` microseconds (bounded wait).
 void launch_delay(uint32_t us, hipStream_t s);
 void launch_aos_to_soa(const splhip_particle_sim *aos, int64_t n, float *soa /*8 planes*/,

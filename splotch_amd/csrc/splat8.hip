@@ -319,6 +319,13 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
   const float expfac = a.expfac, tl = a.taylorlimit;
 
   float l0 = 0.f, l1 = 0.f, l2 = 0.f;
+  const bool pix_in = lx < min(bxo + 8, w) && ly < min(byo + 8, h);
+  if (ug.accumulate == 2 && pix_in) {  // a later chunk: continue the chain
+    const float *d = ug.pic + (static_cast<int64_t>(x0s + lx) * ug.yres + (y0s + ly)) * 3;
+    l0 = d[0];
+    l1 = d[1];
+    l2 = d[2];
+  }
   const uint32_t e0 = uniform(a.bin_off[u]), e1 = uniform(a.bin_off[u + 1]);
   // long lists are near the critical path: let their waves issue ahead of the
   // rest (priority 2; the heavy-block workgroups, the critical path, run at 3)
@@ -497,11 +504,12 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
     tr[2] = static_cast<uint64_t>(e1 - e0) | (static_cast<uint64_t>(chan) << 32);
     tr[3] = static_cast<uint64_t>(__smid());
   }
-  if (lx < min(bxo + 8, w) && ly < min(byo + 8, h)) {
+  if (pix_in) {
     float *d = ug.pic + (static_cast<int64_t>(x0s + lx) * ug.yres + (y0s + ly)) * 3;
-    if (chan == 3 || chan == 0) d[0] = ug.accumulate ? d[0] + l0 : l0;
-    if (chan == 3 || chan == 1) d[1] = ug.accumulate ? d[1] + l1 : l1;
-    if (chan == 3 || chan == 2) d[2] = ug.accumulate ? d[2] + l2 : l2;
+    const bool add = ug.accumulate == 1;
+    if (chan == 3 || chan == 0) d[0] = add ? d[0] + l0 : l0;
+    if (chan == 3 || chan == 1) d[1] = add ? d[1] + l1 : l1;
+    if (chan == 3 || chan == 2) d[2] = add ? d[2] + l2 : l2;
   }
 }
 
@@ -738,6 +746,10 @@ __global__ __launch_bounds__(kHThreads) void k_splat_heavy(SplatArgs a, const ui
   }
 
   float l = 0.f;  // consumer: channel ch, pixel lane
+  const int hlx = bxo + (lane >> 3), hly = byo + (lane & 7);
+  const bool hpix = hlx < min(bxo + 8, w) && hly < min(byo + 8, h);
+  float *hdst = ug.pic + (static_cast<int64_t>(x0s + hlx) * ug.yres + (y0s + hly)) * 3 + ch;
+  if (!producer && ug.accumulate == 2 && hpix) l = *hdst;  // a later chunk: continue
   __syncthreads();  // s_tab, s_rec[0]
   for (int s = -3; s < nbat; ++s) {
     if (producer) {
@@ -788,13 +800,7 @@ __global__ __launch_bounds__(kHThreads) void k_splat_heavy(SplatArgs a, const ui
     tr[2] = static_cast<uint64_t>(n) | (static_cast<uint64_t>(ch) << 32);
     tr[3] = static_cast<uint64_t>(__smid());
   }
-  if (!producer) {
-    const int lx = bxo + (lane >> 3), ly = byo + (lane & 7);
-    if (lx < min(bxo + 8, w) && ly < min(byo + 8, h)) {
-      float *d = ug.pic + (static_cast<int64_t>(x0s + lx) * ug.yres + (y0s + ly)) * 3 + ch;
-      *d = ug.accumulate ? *d + l : l;
-    }
-  }
+  if (!producer && hpix) *hdst = ug.accumulate == 1 ? *hdst + l : l;
 }
 
 // Work items per unit: none for units outside their tile; none for empty units

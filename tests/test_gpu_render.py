@@ -326,3 +326,27 @@ def test_errors(R):
     with pytest.raises(S.SplhipError):   # vector colour without c1/c2
         R.set_palette(0, *synth.default_palette())
         R.render(S.to_device(fld), cam, S.params(col_vector=[1]), pic)
+
+
+@pytest.mark.parametrize("chunk,heavy", [(20_000, 0), (3_000, 0), (20_000, 1)])
+def test_chunked_passes_bitexact(R, chunk, heavy):
+    """A pass with more bin entries than hip_chunk_entries is rendered as
+    consecutive ranges of its render order, each continuing the pixels'
+    chains from the image: the image equals the unchunked render bit for bit
+    (ordered, additive, and additive on top of ordered via a scratch image)."""
+    import splotch_amd as S
+    log("case chunked", chunk, heavy)
+    gas = synth.plummer_field(30_000, 1500, ptype=0, hsml_mu=math.log(0.8))
+    stars = synth.plummer_field(30_000, 1501, ptype=1, hsml_mu=math.log(0.4))
+    fld = synth.concat(gas, stars)
+    v = synth.default_view(192, 160)
+    pal = synth.default_palette()
+    for aeq in ([0, 1], [0, 0], [1, 1]):
+        base = S.params(ptypes=2, a_eq_e=aeq, brightness=[0.5, 2.0], hip_heavy=heavy)
+        want = render(R, fld, v, [pal, pal], base)
+        prm = S.params(ptypes=2, a_eq_e=aeq, brightness=[0.5, 2.0], hip_heavy=heavy,
+                       hip_chunk_entries=chunk)
+        got = render(R, fld, v, [pal, pal], prm)
+        _, counts = R.stage_times()
+        assert max(counts["ordered_entries"], counts["additive_entries"]) > chunk
+        assert_bitexact(got, want, "chunked %s" % (aeq,))
