@@ -217,6 +217,98 @@ def bench_backward(S, R, torch, steps, warmup):
     return out
 
 
+def bench_large_share(S, torch, args):
+    """200M particles (synth.large_share), 4096^2, ordered gas pass (halo half)
+    + additive pass (background half): plain frames, then 16 frames along an
+    orbit (azimuth k * 22.5 deg, elevation 0, radius 150, sky +z) rendered one
+    after another.  CPU baseline: the reference host renderer on a random 1/100
+    subsample, extrapolated by particle count (labelled)."""
+    import numpy as np
+    from splotch_amd import synth
+    t0 = time.time()
+    field = synth.large_share()
+    log("generated %d particles in %.1f s" % (field.n, time.time() - t0))
+    view = synth.default_view(4096)
+    pal = synth.default_palette()
+    dev = S.to_device(field, "cuda")
+    R = S.Renderer(0)
+    R.set_palette(0, *pal)
+    R.set_palette(1, *pal)
+    prm = S.params(ptypes=2, a_eq_e=[0, 1], brightness=[0.5, 2.0])
+    cam = S.camera_setup(view)
+    pic = torch.empty(view.xres * view.yres * 3, dtype=torch.float32, device="cuda")
+    for _ in range(args.warmup):
+        R.render(dev, cam, prm, pic)
+        torch.cuda.synchronize()
+        log("large share: warm-up frame done")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        R.render(dev, cam, prm, pic)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    log("large share: timed frames done")
+    R.set_profiling(True)
+    R.render(dev, cam, prm, pic)
+    stages, counts = R.stage_times()
+    R.set_profiling(False)
+    log("large share: %.1f ms/frame; stages %s; counts %s" % (ms, stages, counts))
+    orbit = []
+    for k in range(16):
+        v = synth.orbit_view(math.radians(22.5 * k), 0.0, 150.0, 4096)
+        orbit.append(S.camera_setup(v))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for c in orbit:
+        R.render(dev, c, prm, pic)
+    torch.cuda.synchronize()
+    ms_orbit = (time.perf_counter() - t0) * 1e3 / 16
+    out = {"metric": METRIC, "value": round(field.n / (ms * 1e-3) / 1e6, 3),
+           "unit": "Mparticles/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "large_share_200M_4096sq", "particles": field.n,
+                      "gas_ordered": int((field.type == 0).sum()),
+                      "additive": int((field.type == 1).sum()), "image": [4096, 4096],
+                      "tile_size": 50, "parallelism": "single GPU"},
+           "stages_ms": {k: round(v, 3) for k, v in stages.items()}, "counts": counts,
+           "orbit16_ms_per_frame": round(ms_orbit, 3),
+           "orbit16_mparticles_s": round(field.n / (ms_orbit * 1e-3) / 1e6, 3)}
+    R.close()
+    if not args.no_cpu:
+        del dev
+        torch.cuda.empty_cache()
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+        rng = np.random.default_rng(7)
+        idx = np.sort(rng.choice(field.n, size=field.n // 100, replace=False))
+        sub = synth.Field(field.x[idx], field.y[idx], field.z[idx], field.hsml[idx],
+                          field.inten[idx], field.col[idx], field.type[idx])
+        use_ref = O.ref_available()
+        t0 = time.perf_counter()
+        for ptype, aeq, br in ((0, False, 0.5), (1, True, 2.0)):
+            sel = sub.type == ptype
+            g = synth.Field(sub.x[sel], sub.y[sel], sub.z[sel], sub.hsml[sel], sub.inten[sel],
+                            sub.col[sel], np.zeros(int(sel.sum()), np.uint8))
+            if use_ref:
+                with quiet_stdout():
+                    O.ref_host_rendering(g, view, [pal], {"xres": 4096, "yres": 4096,
+                                                          "fov": view.fov, "a_eq_e": aeq,
+                                                          "brightness0": br})
+            else:
+                O.host_rendering(g, view, [pal], a_eq_e=aeq, brightness=[br], stable=False)
+        t = time.perf_counter() - t0
+        threads = int(O.lib().orc_num_threads())
+        cpu_mps = sub.n / t / 1e6
+        out["cpu_baseline"] = {
+            "value": round(cpu_mps, 4), "unit": "Mparticles/s", "cores": threads,
+            "kind": "reference" if use_ref else "port",
+            "sample": "EXTRAPOLATED: random 1/100 subsample (%d particles, both types) of the "
+                      "200M field rendered at 4096^2, %.1f s, rate taken as the full frame's"
+                      % (sub.n, t),
+            "speedup": round(out["value"] / cpu_mps, 2)}
+    print(json.dumps(out), flush=True)
+
+
 def copy_calibration(torch, steps=20):
     """A plain device copy of 1 GiB, for scaling rocprofv3 FETCH_SIZE /
     WRITE_SIZE: each dispatch reads and writes exactly 1 GiB."""
@@ -242,6 +334,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched workload")
     ap.add_argument("--no-backward", action="store_true", help="skip the fwd+bwd workload")
+    ap.add_argument("--workload", default="gas_stars",
+                    choices=["gas_stars", "large_share"],
+                    help="gas_stars: 16M at 2048^2 (default); large_share: 200M at 4096^2")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (profiler counter calibration)")
     ap.add_argument("--traffic-kib", type=float, nargs=2, default=None,
@@ -261,6 +356,10 @@ def main():
 
     if args.copy_calibration:
         copy_calibration(torch)
+        return
+    if args.workload == "large_share":
+        import splotch_amd as S
+        bench_large_share(S, torch, args)
         return
 
     import splotch_amd as S

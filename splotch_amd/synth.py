@@ -205,3 +205,17 @@ def batch_jobs(n_jobs: int = 64, seed: int = 1200, n_scale: float = 1.0,
         f = plummer_field(max(n, 1), seed=seed + j)
         out.append(BatchJob(f, orbit_view(az, el, 150.0, side), j % 2 == 0))
     return out
+
+
+def large_share(n: int = 200_000_000, seed: int = 1400) -> Field:
+    """One GPU's share of a large simulation: n particles, half in 5000 Plummer
+    halos and half uniform background, hsml log-normal (mu = ln 0.2, sigma =
+    1.2).  The halo half is the gas (type 0, depth-sorted emission-absorption)
+    and the background half the additive type 1."""
+    f = plummer_field(n, seed, halo_frac=0.5, n_halos=5000, hsml_mu=math.log(0.2),
+                      hsml_sigma=1.2)
+    n_halo = int(round(n * 0.5))
+    typ = np.ones(n, dtype=np.uint8)
+    typ[:n_halo] = 0
+    f.type = typ
+    return f
