@@ -149,8 +149,14 @@ def bench_batched(S, R, torch, steps, warmup, n_jobs=64):
     def batched():
         R.render_batched(views)
 
+    tuned_views = [(d, c, S.params(a_eq_e=bj.additive, hip_autotune=1), p)
+                   for (d, c, _, p), bj in zip(views, jobs)]
+
+    def batched_tuned():
+        R.render_batched(tuned_views)
+
     res = {}
-    for name, fn in (("plain", plain), ("batched", batched)):
+    for name, fn in (("plain", plain), ("batched", batched), ("batched_autotuned", batched_tuned)):
         for _ in range(warmup):
             fn()
         torch.cuda.synchronize()
@@ -160,15 +166,20 @@ def bench_batched(S, R, torch, steps, warmup, n_jobs=64):
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t0) * 1e3 / steps
         res[name] = (ms, ntot / (ms * 1e-3) / 1e6)
-    log("batched %d views, %d particles: plain %.2f ms, batched %.2f ms" %
-        (n_jobs, ntot, res["plain"][0], res["batched"][0]))
+        if name == "batched_autotuned":
+            chosen = R.tuning()
+    log("batched %d views, %d particles: plain %.2f ms, batched %.2f ms, autotuned %.2f ms (%s)" %
+        (n_jobs, ntot, res["plain"][0], res["batched"][0], res["batched_autotuned"][0], chosen))
     return {"views": n_jobs, "particles": ntot,
             "sides": sorted({bj.view.xres for bj in jobs}),
             "ms_per_batch": round(res["batched"][0], 3),
             "mparticles_s": round(res["batched"][1], 3),
             "plain_ms_per_batch": round(res["plain"][0], 3),
             "plain_mparticles_s": round(res["plain"][1], 3),
-            "batched_vs_plain": round(res["batched"][1] / res["plain"][1], 3)}
+            "batched_vs_plain": round(res["batched"][1] / res["plain"][1], 3),
+            "autotuned_ms_per_batch": round(res["batched_autotuned"][0], 3),
+            "autotuned_vs_plain": round(res["batched_autotuned"][1] / res["plain"][1], 3),
+            "autotuned_choice": chosen}
 
 
 def bench_backward(S, R, torch, steps, warmup):

@@ -75,6 +75,10 @@ typedef struct {
                                               (block, particle) entries is rendered in chunks
                                               of consecutive render order, each continuing
                                               the pixels' chains (0 = auto: 2^30) */
+  int32_t hip_autotune;                    /* hip_autotune: 1 = time the splat under a set of
+                                              (hip_variant, hip_split, hip_heavy_max) choices
+                                              on the first render of a batch shape and reuse
+                                              the fastest for that shape (0 = off) */
 } splhip_params;
 
 /* SoA particle fields (device pointers), post particle_normalize:
@@ -250,6 +254,11 @@ int splhip_stage_times(splhip_ctx *ctx, float *ms, int64_t *counts);
    call with out=NULL to get the count in *n). */
 int splhip_set_trace(splhip_ctx *ctx, int32_t enable);
 int splhip_trace_items(splhip_ctx *ctx, int32_t ordered, uint64_t *out, int64_t cap, int64_t *n);
+
+/* The splat tuning of the last render: {hip_variant, hip_split, heavy
+   workgroup cap (0 = none), 1 if the autotuner chose it}.  No reference
+   counterpart. */
+int splhip_last_tuning(splhip_ctx *ctx, int64_t out[4]);
 
 /* Host-built tables (device copies are used by the kernels): tab[0..1023] =
    tab1, tab[1024..2047] = tab2; consts = {expfac, taylorlimit, h2sigma, bfak,

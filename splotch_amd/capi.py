@@ -41,7 +41,8 @@ class Params(C.Structure):
                 ("gray_absorption", C.c_float), ("tile_size", C.c_int32),
                 ("hip_region", C.c_int32), ("hip_split", C.c_int32),
                 ("hip_variant", C.c_int32), ("hip_heavy", C.c_int32),
-                ("hip_heavy_max", C.c_int32), ("hip_chunk_entries", C.c_int32)]
+                ("hip_heavy_max", C.c_int32), ("hip_chunk_entries", C.c_int32),
+                ("hip_autotune", C.c_int32)]
 
 
 class Particles(C.Structure):
@@ -71,7 +72,8 @@ EXPORTS = ["splhip_create", "splhip_destroy", "splhip_last_error", "splhip_camer
            "splhip_params_default", "splhip_params_from_par", "splhip_set_palette",
            "splhip_render", "splhip_render_host", "splhip_project", "splhip_bins",
            "splhip_exptable", "splhip_set_profiling", "splhip_stage_times", "splhip_set_trace",
-           "splhip_trace_items", "splhip_render_batched", "splhip_render_backward"]
+           "splhip_trace_items", "splhip_render_batched", "splhip_render_backward",
+           "splhip_last_tuning"]
 
 STAGES = ("project", "ordered_sort", "ordered_bin", "ordered_splat", "additive_compact",
           "additive_bin", "additive_splat", "ordered_k_splat8", "ordered_k_splat_heavy",
@@ -144,7 +146,7 @@ def camera_setup(view, zmin=0.0, zmax=1e23, projection=True, minrad_pix=1.0) -> 
 
 def params(ptypes=1, smooth_fac=None, brightness=None, col_vector=None, a_eq_e=True,
            sort_type=1, gray_absorption=0.2, tile_size=50, hip_region=0, hip_split=0, hip_variant=0,
-           hip_heavy=0, hip_heavy_max=0, hip_chunk_entries=0,
+           hip_heavy=0, hip_heavy_max=0, hip_chunk_entries=0, hip_autotune=0,
            b_brightness=1.0) -> Params:
     """Render parameters; brightness is brightness{t} (times b_brightness, as
     splotch_host.cc:262-263 does, in float32)."""
@@ -171,6 +173,7 @@ def params(ptypes=1, smooth_fac=None, brightness=None, col_vector=None, a_eq_e=T
     p.hip_heavy = hip_heavy
     p.hip_heavy_max = hip_heavy_max
     p.hip_chunk_entries = hip_chunk_entries
+    p.hip_autotune = hip_autotune
     return p
 
 
@@ -227,6 +230,14 @@ class Renderer:
         counts = dict(ordered_active=int(cnt[0]), ordered_entries=int(cnt[1]),
                       additive_active=int(cnt[2]), additive_entries=int(cnt[3]))
         return dict(zip(STAGES, ms.tolist())), counts
+
+    def tuning(self):
+        """The splat tuning the last render used: dict variant, split,
+        heavy_max, and whether it came from the autotuner."""
+        v = np.zeros(4, np.int64)
+        self._check(lib().splhip_last_tuning(self._c, _np(v)), "splhip_last_tuning")
+        return {"variant": int(v[0]), "split": int(v[1]), "heavy_max": int(v[2]),
+                "autotuned": bool(v[3])}
 
     def set_trace(self, on: bool = True):
         self._check(lib().splhip_set_trace(self._c, C.c_int32(int(on))), "set_trace")

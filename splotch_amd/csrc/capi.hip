@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cctype>
 #include <cmath>
 #include <cstdio>
@@ -58,6 +59,9 @@ struct splhip_ctx {
   int trace = 0;
   std::vector<uint64_t> trace_items[2];
   std::vector<uint32_t> h_mo;
+  // autotuner: fastest splat tuning per batch shape; the last render's choice
+  std::map<uint64_t, std::array<uint32_t, 3>> tuned;
+  uint32_t last_tuning[4] = {0, 0, 0, 0};
   // auxiliary stream (default priority) for the heavy-block kernel
   hipStream_t aux = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -822,6 +826,80 @@ void render_chunked_pass(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, uint3
   pl.recs[p] = recs;
 }
 
+// Batch shape for the autotuner's cache: views, their particle counts, image
+// sizes, region size and pass mix (FNV-1a over the values).
+uint64_t batch_shape(const Plan &pl) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](uint64_t v) {
+    for (int i = 0; i < 8; ++i) {
+      h ^= (v >> (8 * i)) & 0xff;
+      h *= 1099511628211ull;
+    }
+  };
+  mix(static_cast<uint64_t>(pl.nj));
+  mix(static_cast<uint64_t>(pl.G));
+  for (int j = 0; j < pl.nj; ++j) {
+    mix(static_cast<uint64_t>(pl.pa[j].n));
+    mix((static_cast<uint64_t>(pl.js[j].xres) << 32) | static_cast<uint32_t>(pl.js[j].yres));
+    mix(static_cast<uint64_t>(pl.js[j].T));
+  }
+  mix((static_cast<uint64_t>(pl.m[0]) << 32) | pl.m[1]);
+  return h;
+}
+
+// Autotuner: times the splat of every pass that can be repeated (no view adds
+// to an earlier pass) under each candidate tuning and returns the fastest.
+// All candidates produce the same images; they differ only in how the GPU
+// work is decomposed: the 8x8 splat's inner loop (hip_variant 3: software
+// pipelined, 4: chunks of 4-8 particles), the list length from which a block
+// is split per channel, and how many heaviest blocks get producer/consumer
+// workgroups.
+Tuning autotune(splhip_ctx *c, Plan &pl, const Tuning &base, uint32_t *tot, hipStream_t s) {
+  bool repeatable[2];
+  for (int p = 0; p < 2; ++p) {
+    repeatable[p] = pl.m[p] > 0;
+    for (int j = 0; j < pl.nj; ++j)
+      if (pl.js[j].accumulate[p] != 0) repeatable[p] = false;
+  }
+  if (!repeatable[0] && !repeatable[1]) return base;
+  hipEvent_t e0, e1;
+  hip_check(hipEventCreate(&e0), "hipEventCreate");
+  hip_check(hipEventCreate(&e1), "hipEventCreate");
+  Tuning best = base;
+  float best_ms = 1e30f;
+  for (int variant : {4, 3})
+    for (uint32_t split : {2048u, 4096u, 8192u})
+      for (uint32_t hmax : {0u, 128u, 256u, 512u}) {
+        Tuning t = base;
+        t.variant = variant;
+        t.split = split;
+        t.heavy_max = hmax == 0 ? 1u : hmax;
+        if (hmax == 0) t.heavy = 0xffffffffu;
+        float ms = 0.f;
+        for (int p = 0; p < 2; ++p) {
+          if (!repeatable[p]) continue;
+          stage_items_count(c, pl, p, t, tot, s);
+          read_totals(c, tot + T_ITEMS, 4, s);
+          pl.nitems[p] = c->h_tot[p];
+          pl.nheavy[p] = std::min(c->h_tot[2 + p], t.heavy_max);
+          hip_check(hipEventRecord(e0, s), "hipEventRecord");
+          stage_splat(c, pl, p, t, s);
+          hip_check(hipEventRecord(e1, s), "hipEventRecord");
+          hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+          float x = 0.f;
+          hip_check(hipEventElapsedTime(&x, e0, e1), "hipEventElapsedTime");
+          ms += x;
+        }
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = t;
+        }
+      }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return best;
+}
+
 // The frame pipeline over a batch of jobs; three host syncs per batch (pass
 // sizes, bin entries, work items).
 int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool keep_fwd) {
@@ -836,7 +914,7 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
   c->last_entries[0] = c->last_entries[1] = c->last_active[0] = c->last_active[1] = 0;
   c->trace_items[0].clear();
   c->trace_items[1].clear();
-  const Tuning tn = tuning_of(jobs[0].prm);
+  Tuning tn = tuning_of(jobs[0].prm);
   Plan pl;
   make_plan(c, nj, jobs, G, pl, s);
   Workspace &ws = c->ws;
@@ -880,12 +958,27 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
       }
     }
   }
+  bool tuned = false;
   if ((pl.m[0] || pl.m[1]) && !chunked[0] && !chunked[1]) {
     for (int p = 0; p < 2; ++p)
-      if (pl.m[p]) {
-        stage_bins(c, pl, p, s);
-        if (G == 8) stage_items_count(c, pl, p, tn, tot, s);
+      if (pl.m[p]) stage_bins(c, pl, p, s);
+    if (G == 8 && jobs[0].prm->hip_autotune) {
+      const uint64_t key = batch_shape(pl);
+      auto it = c->tuned.find(key);
+      if (it == c->tuned.end()) {
+        const Tuning t = autotune(c, pl, tn, tot, s);
+        it = c->tuned.emplace(key, std::array<uint32_t, 3>{static_cast<uint32_t>(t.variant),
+                                                           t.split, t.heavy_max}).first;
+        if (t.heavy == 0xffffffffu) it->second[2] = 0;
       }
+      tn.variant = static_cast<int>(it->second[0]);
+      tn.split = it->second[1];
+      tn.heavy_max = it->second[2] == 0 ? 1u : it->second[2];
+      if (it->second[2] == 0) tn.heavy = 0xffffffffu;
+      tuned = true;
+    }
+    for (int p = 0; p < 2; ++p)
+      if (pl.m[p] && G == 8) stage_items_count(c, pl, p, tn, tot, s);
     if (G == 8) {
       read_totals(c, tot + T_ITEMS, 4, s);  // sync 3
       for (int p = 0; p < 2; ++p) {
@@ -909,6 +1002,10 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
     c->last_active[p] = pl.m[p];
   }
   collect_marks(c);
+  c->last_tuning[0] = static_cast<uint32_t>(tn.variant);
+  c->last_tuning[1] = tn.split;
+  c->last_tuning[2] = tn.heavy == 0xffffffffu ? 0u : tn.heavy_max;
+  c->last_tuning[3] = tuned ? 1u : 0u;
   if (keep_fwd && nj == 1 && G == 8 && !chunked[0] && !chunked[1]) {
     c->fwd = new FwdState{*jobs[0].P, *jobs[0].cam, *jobs[0].prm, pl, tn};
   }
@@ -997,6 +1094,12 @@ int splhip_create(int device, splhip_ctx **out) {
 int splhip_set_profiling(splhip_ctx *c, int32_t enable) {
   if (!c) return SPLHIP_E_ARG;
   c->prof = enable ? 1 : 0;
+  return 0;
+}
+
+int splhip_last_tuning(splhip_ctx *c, int64_t out[4]) {
+  if (!c || !out) return SPLHIP_E_ARG;
+  for (int k = 0; k < 4; ++k) out[k] = c->last_tuning[k];
   return 0;
 }
 
@@ -1126,6 +1229,7 @@ int splhip_params_default(splhip_params *p) {
   p->hip_heavy = 0;
   p->hip_heavy_max = 0;
   p->hip_chunk_entries = 0;
+  p->hip_autotune = 0;
   return 0;
 }
 
@@ -1165,6 +1269,7 @@ int splhip_params_from_par(const char *par_text, float b_brightness, splhip_para
     p->hip_heavy = par_num<int>(d, "hip_heavy", 0);
     p->hip_heavy_max = par_num<int>(d, "hip_heavy_max", 0);
     p->hip_chunk_entries = par_num<int>(d, "hip_chunk_entries", 0);
+    p->hip_autotune = par_num<int>(d, "hip_autotune", 0);
   } catch (const ParError &) {
     return SPLHIP_E_ARG;
   }

@@ -149,3 +149,24 @@ def test_batched_errors(R):
     with pytest.raises(S.SplhipError):
         R.render_batched(views)
     R.render_batched([])  # no views: nothing to do
+
+
+def test_batched_autotune(R):
+    """hip_autotune times the candidate splat tunings on the first render of a
+    batch shape, then reuses the choice; images are those of the default."""
+    jobs = _small_batch(seed=2700, n_jobs=6)
+    ref = _views(jobs)
+    R.render_batched(ref)
+    first = R.tuning()
+    assert not first["autotuned"]
+    for call in range(2):
+        views = _views(jobs, hip_autotune=1)
+        R.render_batched(views)
+        t = R.tuning()
+        log("autotune call %d chose" % call, t)
+        assert t["autotuned"]
+        assert t["variant"] in (3, 4) and t["split"] in (2048, 4096, 8192)
+        assert t["heavy_max"] in (0, 128, 256, 512)
+        torch.cuda.synchronize()
+        for j, (a, b) in enumerate(zip(views, ref)):
+            assert_bitexact(a[3].cpu().numpy(), b[3].cpu().numpy(), "autotuned job %d" % j)
