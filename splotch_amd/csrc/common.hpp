@@ -84,43 +84,60 @@ __device__ __forceinline__ int find_job(const uint32_t *base, int nj, uint32_t v
   return lo;
 }
 
-// Visit the tiles (G == 0) or work regions (G > 0) a particle is binned to.
+// Candidate tiles of a particle: [minx, maxx] x [miny, maxy] (empty when
+// min > max), and the squared distance bound of the circle test.
+struct BinRange {
+  int minx, maxx, miny, maxy;
+  float sumsq;
+};
+
+// A tile the particle is binned to, as the rectangle of its units:
+// unit(gx, gy) = base + gy * ngm + gx for gx0 <= gx <= gx1, gy0 <= gy <= gy1
+// (G == 0: the single unit base = tile).
+struct BinRect {
+  uint32_t base;
+  int gx0, gx1, gy0, gy1;
+};
+
 // Tile rule: splotch_host.cc:367-383 (range + circle test, T = tile_size).
-// Region rule: the footprint box that render_new clips to the tile
-// (splotch_host.cc:462-472), split into G x G regions.
-template <class F>
-__device__ __forceinline__ void for_each_bin(const Rec &R, const BinGeom &g, F &&f) {
+__device__ __forceinline__ BinRange bin_range(const Rec &R, const BinGeom &g) {
   const float rfacr = g.rfac * R.r;
-  const int minx = max(0, cvt_i32(R.x - rfacr + 1) / g.T);
-  const int maxx = min(g.ncx - 1, cvt_i32(R.x + rfacr) / g.T);
-  const int miny = max(0, cvt_i32(R.y - rfacr + 1) / g.T);
-  const int maxy = min(g.ncy - 1, cvt_i32(R.y + rfacr) / g.T);
-  const float sumsq = (g.rcell + rfacr) * (g.rcell + rfacr);
-  for (int ix = minx; ix <= maxx; ++ix) {
-    const float cx = g.cmid0 + static_cast<float>(ix * g.T);
-    for (int iy = miny; iy <= maxy; ++iy) {
-      const float cy = g.cmid0 + static_cast<float>(iy * g.T);
-      const float rtot2 = (R.x - cx) * (R.x - cx) + (R.y - cy) * (R.y - cy);
-      if (!(rtot2 < sumsq)) continue;
-      const int tile = iy * g.ncx + ix;
-      if (g.G == 0) {
-        f(static_cast<uint32_t>(tile));
-        continue;
-      }
-      const int x0s = ix * g.T, y0s = iy * g.T;
-      const int w = min(x0s + g.T, g.xres) - x0s, h = min(y0s + g.T, g.yres) - y0s;
-      const float rf2 = R.r * g.rfac;
-      const float posx = R.x - static_cast<float>(x0s), posy = R.y - static_cast<float>(y0s);
-      const int bx0 = max(cvt_i32(posx - rf2 + 1), 0), bx1 = min(cvt_i32(posx + rf2 + 1), w);
-      const int by0 = max(cvt_i32(posy - rf2 + 1), 0), by1 = min(cvt_i32(posy + rf2 + 1), h);
-      if (bx0 >= bx1 || by0 >= by1) continue;
-      const int gx0 = bx0 / g.G, gx1 = (bx1 - 1) / g.G;
-      const int gy0 = by0 / g.G, gy1 = (by1 - 1) / g.G;
-      const uint32_t rbase = static_cast<uint32_t>(tile) * g.ngm * g.ngm;
-      for (int gy = gy0; gy <= gy1; ++gy)
-        for (int gx = gx0; gx <= gx1; ++gx) f(rbase + gy * g.ngm + gx);
-    }
+  BinRange b;
+  b.minx = max(0, cvt_i32(R.x - rfacr + 1) / g.T);
+  b.maxx = min(g.ncx - 1, cvt_i32(R.x + rfacr) / g.T);
+  b.miny = max(0, cvt_i32(R.y - rfacr + 1) / g.T);
+  b.maxy = min(g.ncy - 1, cvt_i32(R.y + rfacr) / g.T);
+  b.sumsq = (g.rcell + rfacr) * (g.rcell + rfacr);
+  return b;
+}
+
+// Units of candidate tile (ix, iy); false when the particle is not binned
+// there.  Region rule: the footprint box that render_new clips to the tile
+// (splotch_host.cc:462-472), split into G x G regions.
+__device__ __forceinline__ bool bin_tile(const Rec &R, const BinGeom &g, const BinRange &b, int ix,
+                                         int iy, BinRect &rc) {
+  const float cx = g.cmid0 + static_cast<float>(ix * g.T);
+  const float cy = g.cmid0 + static_cast<float>(iy * g.T);
+  const float rtot2 = (R.x - cx) * (R.x - cx) + (R.y - cy) * (R.y - cy);
+  if (!(rtot2 < b.sumsq)) return false;
+  const int tile = iy * g.ncx + ix;
+  if (g.G == 0) {
+    rc = BinRect{static_cast<uint32_t>(tile), 0, 0, 0, 0};
+    return true;
   }
+  const int x0s = ix * g.T, y0s = iy * g.T;
+  const int w = min(x0s + g.T, g.xres) - x0s, h = min(y0s + g.T, g.yres) - y0s;
+  const float rf2 = R.r * g.rfac;
+  const float posx = R.x - static_cast<float>(x0s), posy = R.y - static_cast<float>(y0s);
+  const int bx0 = max(cvt_i32(posx - rf2 + 1), 0), bx1 = min(cvt_i32(posx + rf2 + 1), w);
+  const int by0 = max(cvt_i32(posy - rf2 + 1), 0), by1 = min(cvt_i32(posy + rf2 + 1), h);
+  if (bx0 >= bx1 || by0 >= by1) return false;
+  rc.base = static_cast<uint32_t>(tile) * g.ngm * g.ngm;
+  rc.gx0 = bx0 / g.G;
+  rc.gx1 = (bx1 - 1) / g.G;
+  rc.gy0 = by0 / g.G;
+  rc.gy1 = (by1 - 1) / g.G;
+  return true;
 }
 
 }  // namespace splhip

@@ -5,7 +5,8 @@
 //                  exactly as TRANSFORM::TransPoint (kernel/transform.cxx:212-219).
 //   k_compact      stream compaction of one pass's particles (+ depth-sort keys).
 //   k_gather_count gather into render order + per-particle bin count.
-//   k_emit         (bin, particle) entries in particle order.
+//   k_emit         (bin, particle) entries in particle order, balanced over
+//                  the wave in two levels (tiles, then units).
 //   k_bin_offsets  CSR offsets from the sorted bin keys (binary search).
 //   k_splat        one workgroup per work region: the region's particles are
 //                  staged in LDS in batches, per-particle column ranges and the
@@ -28,6 +29,12 @@ __device__ __forceinline__ int64_t gtid() {
 
 __device__ __forceinline__ int64_t gstride() {
   return static_cast<int64_t>(gridDim.x) * blockDim.x;
+}
+
+__device__ __forceinline__ void wave_sync_() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 unsigned grid_for(int64_t n, int per_block = 256, int64_t cap = 1 << 20) {
@@ -215,31 +222,148 @@ __global__ __launch_bounds__(256) void k_gather_count(const uint32_t *order, uin
     const BinGeom g = geoms[find_job(mo, nj, static_cast<uint32_t>(s))];
     const Rec R = rec[order[s]];
     recs[s] = R;
-    uint32_t c = 0;
-    for_each_bin(R, g, [&](uint32_t) { ++c; });
+    const BinRange b = bin_range(R, g);
+    uint32_t c = 0;  // units of the binned tiles
+    for (int ix = b.minx; ix <= b.maxx; ++ix)
+      for (int iy = b.miny; iy <= b.maxy; ++iy) {
+        BinRect rc;
+        if (bin_tile(R, g, b, ix, iy, rc))
+          c += static_cast<uint32_t>((rc.gx1 - rc.gx0 + 1) * (rc.gy1 - rc.gy0 + 1));
+      }
     cnt[s] = c;
   }
+}
+
+// Entries of 64 consecutive particles (one wave) are one contiguous range of
+// the output (off is their exclusive scan); a particle's entries go tile
+// column by column (ix outer, iy inner), each tile's units row by row.
+// Per-particle entry counts vary by orders of magnitude, so the work is
+// split in two balanced levels: each lane walks its particle's
+// candidate tiles (one per step) and queues every binned tile as an item (a
+// rectangle of units and its output position) in LDS; once 64 items are
+// queued, the wave writes the items' entries with one entry per lane.
+constexpr int kEmitQ = 128;  // a step queues at most 64 items; drained from 64 on
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t u = __shfl_up(v, d, 64);
+    if (lane >= d) v += u;
+  }
+  return v;
+}
+
+struct EmitQueue {
+  uint32_t pos[kEmitQ], key[kEmitQ], val[kEmitQ], nx_ngm[kEmitQ], n[kEmitQ], pre[kEmitQ];
+};
+
+// Write the entries of items [0, qn) of the queue: item k covers output
+// [pos, pos + n), entry i = (gy, gx) = (i / nx, i % nx) is unit key + gy*ngm + gx.
+__device__ __forceinline__ void emit_drain(EmitQueue &Q, int qn, int lane, uint32_t *keys,
+                                           uint32_t *vals) {
+  const uint32_t c0 = lane < qn ? Q.n[lane] : 0u, c1 = lane + 64 < qn ? Q.n[lane + 64] : 0u;
+  const uint32_t i0 = wave_incl_scan(c0, lane), i1 = wave_incl_scan(c1, lane);
+  const uint32_t t0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(i0), 63));
+  const uint32_t tot = t0 + static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(i1), 63));
+  Q.pre[lane] = i0 - c0;
+  Q.pre[lane + 64] = t0 + i1 - c1;
+  wave_sync_();
+  for (uint32_t t = 0; t < tot; t += 64) {
+    const uint32_t q = t + lane;
+    if (q < tot) {
+      int k = 0;  // last item with pre <= q (items have n > 0, so pre ascends strictly)
+#pragma unroll
+      for (int step = 64; step; step >>= 1)
+        if (k + step < qn && Q.pre[k + step] <= q) k += step;
+      const uint32_t i = q - Q.pre[k];
+      const uint32_t nn = Q.nx_ngm[k];
+      const int nx = static_cast<int>(nn & 0xffffu), ngm = static_cast<int>(nn >> 16);
+      int gy = static_cast<int>(static_cast<float>(i) * __builtin_amdgcn_rcpf(static_cast<float>(nx)));
+      int gx = static_cast<int>(i) - gy * nx;
+      if (gx < 0) {
+        --gy;
+        gx += nx;
+      } else if (gx >= nx) {
+        ++gy;
+        gx -= nx;
+      }
+      const uint32_t o = Q.pos[k] + i;
+      keys[o] = Q.key[k] + static_cast<uint32_t>(gy * ngm + gx);
+      vals[o] = Q.val[k];
+    }
+  }
+  wave_sync_();
 }
 
 __global__ __launch_bounds__(256) void k_emit(const Rec *recs, uint32_t m, const uint32_t *off,
                                               uint32_t e, const BinGeom *geoms,
                                               const uint32_t *ubase, const uint32_t *mo, int nj,
                                               uint32_t *keys, uint32_t *vals) {
-  for (int64_t s = gtid(); s < m; s += gstride()) {
-    const int j = find_job(mo, nj, static_cast<uint32_t>(s));
-    const BinGeom g = geoms[j];
-    const uint32_t ub = ubase[j];
-    const Rec R = recs[s];
-    uint32_t o = off[s];
-    const uint32_t end = (s + 1 < m) ? off[s + 1] : e;  // same enumeration as the count
-    const uint32_t sv = static_cast<uint32_t>(s);
-    for_each_bin(R, g, [&](uint32_t b) {
-      if (o < end) {
-        keys[o] = ub + b;
-        vals[o] = sv;
+  __shared__ EmitQueue s_q[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  EmitQueue &Q = s_q[wave];
+  const int64_t nwaves = static_cast<int64_t>(gridDim.x) * 4;
+  for (int64_t w = static_cast<int64_t>(blockIdx.x) * 4 + wave; w * 64 < m; w += nwaves) {
+    const int64_t s = w * 64 + lane;
+    const bool valid = s < m;
+    BinGeom g{};
+    Rec R{};
+    BinRange br{0, -1, 0, -1, 0.f};
+    uint32_t ub = 0, q = 0, end = 0;
+    if (valid) {
+      const int j = find_job(mo, nj, static_cast<uint32_t>(s));
+      g = geoms[j];
+      ub = ubase[j];
+      R = recs[s];
+      q = off[s];
+      end = (s + 1 < m) ? off[s + 1] : e;  // same enumeration as the count
+      br = bin_range(R, g);
+    }
+    bool more = br.minx <= br.maxx && br.miny <= br.maxy;
+    int ix = br.minx, iy = br.miny;
+    int qn = 0;  // uniform
+    while (__ballot(more)) {
+      bool have = false;
+      uint32_t ipos = 0, ikey = 0, inn = 0, in = 0;
+      if (more) {
+        BinRect rc;
+        if (bin_tile(R, g, br, ix, iy, rc)) {
+          const uint32_t nx = static_cast<uint32_t>(rc.gx1 - rc.gx0 + 1);
+          const uint32_t cnt = nx * static_cast<uint32_t>(rc.gy1 - rc.gy0 + 1);
+          in = min(cnt, end - q);
+          have = in > 0;
+          ipos = q;
+          ikey = ub + rc.base + static_cast<uint32_t>(rc.gy0 * g.ngm + rc.gx0);
+          inn = nx | (static_cast<uint32_t>(g.ngm) << 16);
+          q += in;
+        }
+        if (++iy > br.maxy) {
+          iy = br.miny;
+          more = ++ix <= br.maxx;
+        }
       }
-      ++o;
-    });
+      const uint64_t hb = __ballot(have);
+      if (have) {
+        const int slot = qn + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                                  static_cast<uint32_t>(hb >> 32),
+                                  __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(hb), 0u)));
+        Q.pos[slot] = ipos;
+        Q.key[slot] = ikey;
+        Q.val[slot] = static_cast<uint32_t>(s);
+        Q.nx_ngm[slot] = inn;
+        Q.n[slot] = in;
+      }
+      qn += __builtin_popcountll(hb);
+      if (qn >= 64) {
+        wave_sync_();
+        emit_drain(Q, qn, lane, keys, vals);
+        qn = 0;
+      }
+    }
+    if (qn) {
+      wave_sync_();
+      emit_drain(Q, qn, lane, keys, vals);
+    }
   }
 }
 
@@ -516,7 +640,7 @@ void launch_gather_count(const uint32_t *order, uint32_t m, const Rec *rec, Rec 
 void launch_emit(const Rec *recs, uint32_t m, const uint32_t *off, uint32_t e,
                  const BinGeom *geoms, const uint32_t *ubase, const uint32_t *mo, int nj,
                  uint32_t *keys, uint32_t *vals, hipStream_t s) {
-  k_emit<<<grid_for(m), 256, 0, s>>>(recs, m, off, e, geoms, ubase, mo, nj, keys, vals);
+  k_emit<<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(recs, m, off, e, geoms, ubase, mo, nj, keys, vals);
   hip_check(hipGetLastError(), "k_emit launch");
 }
 
