@@ -67,8 +67,8 @@ typedef struct {
                                               1 = per particle, 2 = 4-particle groups,
                                               3 = software pipelined, 4 = chunked branch-free */
   int32_t hip_heavy;                       /* hip_heavy: list length from which an 8x8 block is
-                                              splatted by producer/consumer workgroups, one
-                                              per channel (0 = auto, < 0 = never) */
+                                              splatted by a producer/consumer workgroup
+                                              (0 = auto: 32768, < 0 = never) */
   int32_t hip_heavy_max;                   /* hip_heavy_max: at most this many such workgroups
                                               per pass, the heaviest (0 = auto: 256) */
   int32_t hip_chunk_entries;               /* hip_chunk_entries: a single-view pass with more

@@ -672,6 +672,9 @@ void stage_items_count(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, uint32_
   const SplatArgs sa = splat_args(c, pl, p, tn);
   uint32_t *icnt = ws.get_as<uint32_t>(pslot(p, P_ICNT), pl.nunits);
   uint32_t *ioff = ws.get_as<uint32_t>(pslot(p, P_IOFF), pl.nunits);
+  // the count of this call's long-list items only (chunks and autotuning
+  // candidates count again)
+  launch_fill_u32(tot + T_HEAVY + p, 0u, 1, s);
   launch_items_count(pl.bin_off[p], pl.nunits, sa, p == 0 ? 1 : 0, tn.split, icnt,
                      tot + T_HEAVY + p, s);
   scan_u32(icnt, ioff, pl.nunits, tot + T_ITEMS + p, ws, s);
@@ -688,7 +691,7 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
     mark(c, p == 0 ? 3 : 6, false, s);
     return;
   }
-  const uint32_t nitems = pl.nitems[p], nheavy = pl.nheavy[p];
+  const uint32_t nitems = pl.nitems[p], nheavy = std::min(pl.nheavy[p], nitems);
   const uint32_t *ioff = ws.get_as<uint32_t>(pslot(p, P_IOFF), pl.nunits);
   uint32_t *ik0 = ws.get_as<uint32_t>(pslot(p, P_IK0), nitems);
   uint32_t *iv0 = ws.get_as<uint32_t>(pslot(p, P_IV0), nitems);
@@ -709,7 +712,7 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
     hip_check(hipEventRecord(c->ev_fork, s), "record fork");
     hip_check(hipStreamWaitEvent(c->aux, c->ev_fork, 0), "wait fork");
     mark(c, p == 0 ? 8 : 10, true, c->aux);
-    launch_splat_heavy(p == 1, items, nheavy, sa, c->aux);
+    launch_splat_pc(p == 1, items, nheavy, sa, c->aux);
     mark(c, p == 0 ? 8 : 10, false, c->aux);
     hip_check(hipEventRecord(c->ev_join, c->aux), "record join");
     // The normal items fill every CU; the heavy workgroups (the critical

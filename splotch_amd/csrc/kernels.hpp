@@ -101,8 +101,10 @@ void launch_items_emit(const uint32_t *bin_off, uint32_t nunits, const SplatArgs
                        hipStream_t s);
 void launch_splat8(bool additive, int variant, const uint32_t *items, uint32_t nitems,
                    const SplatArgs &a, hipStream_t s);
-void launch_splat_heavy(bool additive, const uint32_t *items, uint32_t nheavy, const SplatArgs &a,
-                        hipStream_t s);
+// Producer/consumer workgroups (splat8.hip), one per item: the first npc
+// items, the units whose lists reach a.heavy_len.
+void launch_splat_pc(bool additive, const uint32_t *items, uint32_t npc, const SplatArgs &a,
+                     hipStream_t s);
 // Differentiable form (adjoint.hip): screen-space gradient accumulation over
 // the forward's work items (gscr: n x 6 doubles, zeroed by the caller), the
 // chain to the particle fields of job 0, and an FP64 fill.

@@ -17,6 +17,8 @@
 // independent), which cuts the serial work per lane to about a third.
 #include "kernels.hpp"
 
+#include <type_traits>
+
 #include "device_util.hpp"
 #include "workspace.hpp"
 
@@ -328,7 +330,7 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
   }
   const uint32_t e0 = uniform(a.bin_off[u]), e1 = uniform(a.bin_off[u + 1]);
   // long lists are near the critical path: let their waves issue ahead of the
-  // rest (priority 2; the heavy-block workgroups, the critical path, run at 3)
+  // rest (priority 2; the producer/consumer workgroups, the critical path, run at 3)
   if (e1 - e0 >= a.prio_len) __builtin_amdgcn_s_setprio(2);
   for (uint32_t base = e0; base < e1; base += 64) {
     const int nb = static_cast<int>(uniform(min<uint32_t>(64u, e1 - base)));
@@ -514,14 +516,14 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
 }
 
 // ---------------------------------------------------------------------------
-// Heavy blocks: producer / consumer workgroups, one per (8x8 block, channel).
+// Long lists: producer / consumer workgroups, one per 8x8 block, all three
+// colour channels.
 //
-// A block whose list holds hundreds of thousands of particles is the critical
-// path of the frame: its pixels' update chains are serial.  Such a block is
-// issued as three work items, one per colour channel (the channels' chains are
-// independent), each a 512-thread workgroup.  Wave 0 is the consumer: lane =
-// pixel, and per particle it only applies
-//     l += ((l - q) * m1) * m2     (ordered)       l += t     (additive)
+// A block whose list holds thousands to hundreds of thousands of particles is
+// near the critical path of the frame: its pixels' update chains are serial.
+// Such a block gets a 512-thread workgroup.  Wave 0 is the consumer: lane =
+// pixel, and per particle it only applies, per channel c,
+//     l_c += ((l_c - q_c) * m1_c) * m2     (ordered)     l_c += t_c   (additive)
 // with operands that reproduce render_new's update bit for bit
 // (splotch_host.cc:615, 648-658):
 //   Taylor branch   m1 = a,  m2 = att           -> ((l - q) * a) * att
@@ -529,20 +531,35 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
 //   additive        t = a * att
 //   outside         m1 = m2 = t = 0             -> l + (+-0) == l
 // (l starts at +0 and a float sum is -0 only if both terms are -0, so l is
-// never -0 and adding a zero leaves it unchanged, as the reference's skip.)
+// never -0 and adding a zero leaves it unchanged, as the reference's skip.
+// For finite records l - q stays finite, so (l - q) * 0 is a zero.)
 // Waves 1..7 run a pipeline ahead of the consumer: particle set-up three
-// batches ahead (tile-frame position, box, Gaussian width, emission /
+// steps ahead (tile-frame position, box, Gaussian width, emission /
 // absorption; splotch_host.cc:462-476, 591-632), the separable column / row
-// tables two batches ahead (splotch_host.cc:596-607), and the per-fragment
-// operands one batch ahead.  Records reach the set-up through LDS in
-// super-batches of 256 that wave 1 loads about eight steps before their use,
-// so the gather latency stays off the critical path.
-constexpr int kHB = 32;        // particles per batch
-constexpr int kHThreads = 512;
-constexpr int kHProd = kHThreads - 64;  // producer lanes
-constexpr int kSB = 8;         // batches per record super-batch (kSB * kHB == 256)
+// tables two steps ahead (splotch_host.cc:596-607), and the operands one step
+// ahead, with lane = pixel and each producer wave on its own four particles
+// of the step: the producers do the table work of the lane-per-pixel kernel
+// once for all three channels, the consumer four operations per channel.
+// The consumer runs at a higher wave priority than the producers, which
+// share its SIMD.
+// Records reach the set-up through LDS in super-batches that wave 1 loads
+// about eight steps before their use, so the gather latency stays off the
+// critical path.
+constexpr int kPW = 7;                 // producer waves
+constexpr int kPB = 4;                 // particles per producer wave and step
+constexpr int kB = kPW * kPB;          // particles per step
+constexpr int kQ = 2;                  // operand stage: particles per group of LDS reads
+static_assert(kPB % kQ == 0, "groups tile a producer's particles");
+constexpr int kPThreads = 64 * (kPW + 1);
+constexpr int kPProd = 64 * kPW;       // producer lanes
+constexpr int kSB = 8;                 // steps per record super-batch
+constexpr int kSR = kSB * kB;          // records per super-batch
+constexpr int kSRL = (kSR + 63) / 64;  // records per wave-1 lane and super-batch
+static_assert(kSB >= 5, "a super-batch is stored four steps before its first use");
+constexpr int kSRP = kSRL * 64;        // staging row: whole wave stores (tail unused)
+static_assert(kB <= 32, "fast-particle mask is 32 bits");
 
-struct HeavyPart {
+struct PcPart {
   float posx, posy, radsq, stp;
   int bx0, bx1, by0, by1;
 };
@@ -554,44 +571,45 @@ __device__ __forceinline__ void lds_barrier() {
 }
 
 template <int ADD>
-__global__ __launch_bounds__(kHThreads) void k_splat_heavy(SplatArgs a, const uint32_t *items,
-                                                           uint32_t nheavy) {
+__global__ __launch_bounds__(kPThreads) void k_splat_pc(SplatArgs a, const uint32_t *items,
+                                                        uint32_t npc) {
   __shared__ float s_tab[2048];
-  __shared__ Rec s_rec[2][kSB * kHB];  // staged records, two super-batches
-  __shared__ HeavyPart s_part[2][64];  // 64: every lane of wave 1 stores
-  __shared__ float4 s_cst[4][64];      // [slot][k] = (a, q, maxa, fast)
-  __shared__ uint32_t s_cm[2][kHB][8];  // column row-masks
-  __shared__ float s_p2[2][kHB][8];     // column factors
-  __shared__ float s_p1[2][kHB][8];     // row factors
-  __shared__ float s_m2[ADD ? 1 : 2][ADD ? 1 : kHB][64];
-  __shared__ float s_m1[2][kHB][64];    // additive: t
+  __shared__ Rec s_rec[2][kSRP];        // staged records, two super-batches
+  __shared__ PcPart s_part[2][kB];
+  __shared__ float4 s_ca[4][kB];        // (a0, a1, a2, max |a|)
+  __shared__ float4 s_cq[4][kB];        // (q0, q1, q2, 0)
+  __shared__ uint32_t s_fast[4];        // bit k: particle k has every a_c in [kFastA, 0]
+  __shared__ uint32_t s_cm[2][kB][8];   // column row-masks
+  __shared__ float s_p2[2][kB][8];      // column factors
+  __shared__ float s_p1[2][kB][8];      // row factors
+  __shared__ float4 s_op[2][kB][64];    // consumer operands (m1_0, m1_1, m1_2, m2) / (t0, t1, t2, 0)
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = static_cast<int>(uniform(static_cast<uint32_t>(tid >> 6)));
   const bool producer = wave >= 1;
   const int ptid = tid - 64;
-  for (int i = tid; i < 2048; i += kHThreads) s_tab[i] = a.exptab[i];
+  for (int i = tid; i < 2048; i += kPThreads) s_tab[i] = a.exptab[i];
   const uint32_t it = blockIdx.x;
   const uint64_t t_start = a.trace ? __builtin_amdgcn_s_memrealtime() : 0;
-  const uint32_t item = uniform(items[it]);
-  const uint32_t u = item >> 2;
-  const int ch = static_cast<int>(item & 3u);  // 0..2
+  const uint32_t u = uniform(items[it]) >> 2;
   const UnitGeom ug = unit_geom_uniform(a, u);
   const int gy = ug.gy, gx = ug.gx, x0s = ug.x0s, y0s = ug.y0s, w = ug.w, h = ug.h;
   const int bxo = gx * 8, byo = gy * 8;
   const uint32_t e0 = uniform(a.bin_off[u]), e1 = uniform(a.bin_off[u + 1]);
   const int n = static_cast<int>(e1 - e0);
-  const int nbat = (n + kHB - 1) / kHB;
+  const int nbat = (n + kB - 1) / kB;
   const float expfac = a.expfac, tl = a.taylorlimit;
-  __builtin_amdgcn_s_setprio(3);
+  const int cx = lane >> 3, cy = lane & 7;  // producers and consumer: lane = pixel
+  if (producer) __builtin_amdgcn_s_setprio(2);
+  else __builtin_amdgcn_s_setprio(3);
 
-  auto count = [&](int b) { return min(kHB, n - b * kHB); };
-  // particle set-up of batch b (wave 1).  Branch-free: lanes past the batch
-  // (and batches past the list) store clamped records into slots no stage of
-  // this step reads.
+  auto count = [&](int b) { return min(kB, n - b * kB); };
+  // particle set-up of batch b (wave 1, lane = particle).  Lanes past the
+  // batch (and batches past the list) use clamped records, which the
+  // operand stage masks.
   auto setup = [&](int b, const Rec &R) {
     const float rfacr = R.r * a.rfac;
     const float posx = R.x - static_cast<float>(x0s), posy = R.y - static_cast<float>(y0s);
-    HeavyPart P;
+    PcPart P;
     P.posx = posx;
     P.posy = posy;
     P.bx0 = max(cvt_i32(posx - rfacr + 1), 0);
@@ -601,30 +619,38 @@ __global__ __launch_bounds__(kHThreads) void k_splat_heavy(SplatArgs a, const ui
     P.radsq = rfacr * rfacr;
     const float sigma = a.h2sigma * R.r;
     P.stp = -1.f / (sigma * sigma);
-    const float e = ch == 0 ? R.e0 : (ch == 1 ? R.e1 : R.e2);
-    const float av = -e;
-    float q = 0.f, mx = 0.f, fast = 0.f;
+    const float a0 = -R.e0, a1 = -R.e1, a2 = -R.e2;
+    float q0 = 0.f, q1 = 0.f, q2 = 0.f, mx = 0.f;
+    bool fast = false;
     if (!ADD) {
-      q = e / (e + ug.grayabsorb);
-      mx = fmaxf(fabsf(-R.e0), fmaxf(fabsf(-R.e1), fabsf(-R.e2)));
-      fast = (av >= kFastA && av <= 0.f) ? 1.f : 0.f;
+      const float g = ug.grayabsorb;
+      q0 = R.e0 / (R.e0 + g);
+      q1 = R.e1 / (R.e1 + g);
+      q2 = R.e2 / (R.e2 + g);
+      mx = fmaxf(fabsf(a0), fmaxf(fabsf(a1), fabsf(a2)));
+      fast = a0 >= kFastA && a0 <= 0.f && a1 >= kFastA && a1 <= 0.f && a2 >= kFastA && a2 <= 0.f;
     }
-    s_part[b & 1][lane] = P;
-    s_cst[b & 3][lane] = make_float4(av, q, mx, fast);
+    const uint64_t fb = __ballot(fast);
+    if (lane < kB) {
+      s_part[b & 1][lane] = P;
+      s_ca[b & 3][lane] = make_float4(a0, a1, a2, mx);
+      s_cq[b & 3][lane] = make_float4(q0, q1, q2, 0.f);
+    }
+    if (lane == 0) s_fast[b & 3] = static_cast<uint32_t>(fb) & ((1u << kB) - 1u);
   };
-  // Producer stages are unrolled over a lane's items with every LDS read
-  // issued before the arithmetic that uses it, so a step costs a few LDS
-  // round trips instead of one per item.
-  auto tables = [&](int b) {  // column ranges / factors and row factors into b & 1
+  // Column ranges / factors and row factors of batch b into slot b & 1: task
+  // (particle k, j) is column j (j < 8) or row j - 8.  Unrolled over a lane's
+  // tasks with every LDS read issued before the arithmetic that uses it.
+  auto tables = [&](int b) {
     const int ts = b & 1, cnt = count(b);
-    const HeavyPart(&part)[64] = s_part[b & 1];
-    constexpr int NT = (kHB * 16 + kHProd - 1) / kHProd;
-    HeavyPart P[NT];
+    const PcPart(&part)[kB] = s_part[b & 1];
+    constexpr int NT = (kB * 16 + kPProd - 1) / kPProd;
+    PcPart P[NT];
     int kt[NT], jt[NT];
     bool vt[NT];
 #pragma unroll
     for (int i = 0; i < NT; ++i) {
-      const int t = ptid + i * kHProd;
+      const int t = ptid + i * kPProd;
       vt[i] = t < cnt * 16;
       const int tc = vt[i] ? t : 0;
       kt[i] = tc >> 4;
@@ -668,56 +694,89 @@ __global__ __launch_bounds__(kHThreads) void k_splat_heavy(SplatArgs a, const ui
       }
     }
   };
-  auto fragments = [&](int b) {  // consumer operands of batch b, padded to kHB
-    const int ts = b & 1, cnt = count(b);
-    const float4 *cst = s_cst[b & 3];
-    constexpr int NF = (kHB * 64 + kHProd - 1) / kHProd;
-    uint32_t cm[NF];
-    float q1[NF], q2[NF];
-    float4 cv[NF];
-    int fk[NF], fp[NF];
-    bool fv[NF];
+  // Consumer operands of batch b: producer wave pw covers particles
+  // [pw * kPB, pw * kPB + kPB) of the batch, lane = pixel, in groups of kQ
+  // with every LDS read of a group issued first.  Particles past the batch
+  // get zero operands.
+  // FAST: every particle of the group has all a_c in [kFastA, 0], so the
+  // exp-table index is trunc(p) with p in [0, 2^20).
+  auto operands_group = [&](int b, int kb, auto fast_tag) {  // particles [kb, kb + kQ)
+    constexpr bool FAST = decltype(fast_tag)::value;
+    const int ts = b & 1, cs = b & 3, cnt = count(b);
+    uint32_t mk[kQ];
+    float p1[kQ], p2[kQ];
+    float4 ca[kQ];
 #pragma unroll
-    for (int i = 0; i < NF; ++i) {
-      const int f = ptid + i * kHProd;
-      fv[i] = f < kHB * 64;
-      const int fc = fv[i] ? f : 0;
-      fk[i] = fc >> 6;
-      fp[i] = fc & 63;
-      const int kk = fk[i] < cnt ? fk[i] : 0;
-      const int c = fp[i] >> 3, r = fp[i] & 7;
-      cm[i] = s_cm[ts][kk][c];
-      q1[i] = s_p1[ts][kk][r];
-      q2[i] = s_p2[ts][kk][c];
-      cv[i] = cst[kk];
+    for (int j = 0; j < kQ; ++j) {
+      const int k = kb + j;
+      mk[j] = s_cm[ts][k][cx];
+      p1[j] = s_p1[ts][k][cy];
+      p2[j] = s_p2[ts][k][cx];
+      ca[j] = s_ca[cs][k];
     }
-    float att[NF], g[NF], pg[NF], u1[NF], u2[NF];
-    bool in[NF];
+    float4 op[kQ];
+    if (ADD) {
 #pragma unroll
-    for (int i = 0; i < NF; ++i) {
-      const int c = fp[i] >> 3, r = fp[i] & 7;
-      in[i] = fk[i] < cnt && ((cm[i] >> r) & 1u);
-      (void)c;
-      att[i] = q1[i] * q2[i];
-      if (ADD) continue;
-      g[i] = att[i] * cv[i].x;
-      pg[i] = g[i] * expfac;
-      const int ix = xexp_index(pg[i]);
-      u1[i] = s_tab[ix >> 10];
-      u2[i] = s_tab[1024 + (ix & 1023)];
-    }
-#pragma unroll
-    for (int i = 0; i < NF; ++i) {
-      if (!fv[i]) continue;
-      const int k = fk[i], p = fp[i];
-      if (ADD) {
-        s_m1[ts][k][p] = in[i] ? cv[i].x * att[i] : 0.f;
-      } else {
-        const bool tay = (cv[i].z * att[i]) < tl;
-        const float em = xexpm1_finish(g[i], pg[i], u1[i], u2[i], tl);
-        s_m2[ADD ? 0 : ts][k][p] = in[i] ? (tay ? att[i] : 1.f) : 0.f;
-        s_m1[ts][k][p] = in[i] ? (tay ? cv[i].x : em) : 0.f;
+      for (int j = 0; j < kQ; ++j) {
+        const float at = p1[j] * p2[j];
+        const bool in = (kb + j < cnt) && ((mk[j] >> cy) & 1u);
+        op[j] = in ? make_float4(ca[j].x * at, ca[j].y * at, ca[j].z * at, 0.f)
+                   : make_float4(0.f, 0.f, 0.f, 0.f);
       }
+    } else {
+      float at[kQ], g0[kQ], g1[kQ], g2[kQ], pg0[kQ], pg1[kQ], pg2[kQ];
+      float u0[kQ], w0[kQ], u1[kQ], w1[kQ], u2[kQ], w2[kQ];
+#pragma unroll
+      for (int j = 0; j < kQ; ++j) {
+        at[j] = p1[j] * p2[j];
+        g0[j] = at[j] * ca[j].x;
+        g1[j] = at[j] * ca[j].y;
+        g2[j] = at[j] * ca[j].z;
+        pg0[j] = g0[j] * expfac;
+        pg1[j] = g1[j] * expfac;
+        pg2[j] = g2[j] * expfac;
+        const int i0 = FAST ? static_cast<int>(pg0[j]) : xexp_index(pg0[j]);
+        const int i1 = FAST ? static_cast<int>(pg1[j]) : xexp_index(pg1[j]);
+        const int i2 = FAST ? static_cast<int>(pg2[j]) : xexp_index(pg2[j]);
+        u0[j] = s_tab[i0 >> 10];
+        w0[j] = s_tab[1024 + (i0 & 1023)];
+        u1[j] = s_tab[i1 >> 10];
+        w1[j] = s_tab[1024 + (i1 & 1023)];
+        u2[j] = s_tab[i2 >> 10];
+        w2[j] = s_tab[1024 + (i2 & 1023)];
+      }
+#pragma unroll
+      for (int j = 0; j < kQ; ++j) {
+        const bool in = (kb + j < cnt) && ((mk[j] >> cy) & 1u);
+        const bool tay = (ca[j].w * at[j]) < tl;
+        float em0, em1, em2;
+        if (FAST) {
+          em0 = (fabsf(g0[j]) < tl) ? g0[j] : u0[j] * w0[j] - 1.f;
+          em1 = (fabsf(g1[j]) < tl) ? g1[j] : u1[j] * w1[j] - 1.f;
+          em2 = (fabsf(g2[j]) < tl) ? g2[j] : u2[j] * w2[j] - 1.f;
+        } else {
+          em0 = xexpm1_finish(g0[j], pg0[j], u0[j], w0[j], tl);
+          em1 = xexpm1_finish(g1[j], pg1[j], u1[j], w1[j], tl);
+          em2 = xexpm1_finish(g2[j], pg2[j], u2[j], w2[j], tl);
+        }
+        op[j] = in ? make_float4(tay ? ca[j].x : em0, tay ? ca[j].y : em1, tay ? ca[j].z : em2,
+                                 tay ? at[j] : 1.f)
+                   : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kQ; ++j) s_op[ts][kb + j][lane] = op[j];
+  };
+  auto operands = [&](int b, int pw) {
+    const uint32_t fb = uniform(s_fast[b & 3]);
+#pragma unroll
+    for (int g = 0; g < kPB; g += kQ) {
+      const int kb = pw * kPB + g;
+      const uint32_t want = ((1u << kQ) - 1u) << kb;
+      if (!ADD && (fb & want) == want)
+        operands_group(b, kb, std::true_type{});
+      else
+        operands_group(b, kb, std::false_type{});
     }
   };
 
@@ -725,69 +784,91 @@ __global__ __launch_bounds__(kHThreads) void k_splat_heavy(SplatArgs a, const ui
   // one after that's list indices in registers (positions clamped to the list
   // end), and stores a super-batch into its half of s_rec four steps before
   // its first set-up.
-  auto entry = [&](int J, int r) {
-    return a.ent[e0 + min(J * (kSB * kHB) + r * 64 + lane, n - 1)];
+  auto entry = [&](int J, int r) { return a.ent[e0 + min(J * kSR + r * 64 + lane, n - 1)]; };
+  // records as two 4-float halves in native vector registers (the arrays
+  // must stay in registers: struct copies would keep them in memory)
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  v4f RA[kSRL], RB[kSRL];
+  uint32_t I[kSRL];
+  auto load_recs = [&]() {
+#pragma unroll
+    for (int r = 0; r < kSRL; ++r) {
+      const v4f *src = reinterpret_cast<const v4f *>(a.recs + I[r]);
+      RA[r] = src[0];
+      RB[r] = src[1];
+    }
   };
-  Rec R[4];
-  uint32_t I[4];
+  auto store_recs = [&](int slot) {
+#pragma unroll
+    for (int r = 0; r < kSRL; ++r) {
+      v4f *dst = reinterpret_cast<v4f *>(&s_rec[slot][r * 64 + lane]);
+      dst[0] = RA[r];
+      dst[1] = RB[r];
+    }
+  };
   if (wave == 1) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) I[r] = entry(0, r);
+    for (int r = 0; r < kSRL; ++r) I[r] = entry(0, r);
+    load_recs();
+    store_recs(0);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) R[r] = a.recs[I[r]];
+    for (int r = 0; r < kSRL; ++r) I[r] = entry(1, r);
+    load_recs();
 #pragma unroll
-    for (int r = 0; r < 4; ++r) s_rec[0][r * 64 + lane] = R[r];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) I[r] = entry(1, r);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) R[r] = a.recs[I[r]];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) I[r] = entry(2, r);
+    for (int r = 0; r < kSRL; ++r) I[r] = entry(2, r);
   }
 
-  float l = 0.f;  // consumer: channel ch, pixel lane
-  const int hlx = bxo + (lane >> 3), hly = byo + (lane & 7);
+  float l0 = 0.f, l1 = 0.f, l2 = 0.f;  // consumer: pixel lane
+  const int hlx = bxo + cx, hly = byo + cy;
   const bool hpix = hlx < min(bxo + 8, w) && hly < min(byo + 8, h);
-  float *hdst = ug.pic + (static_cast<int64_t>(x0s + hlx) * ug.yres + (y0s + hly)) * 3 + ch;
-  if (!producer && ug.accumulate == 2 && hpix) l = *hdst;  // a later chunk: continue
+  float *hdst = ug.pic + (static_cast<int64_t>(x0s + hlx) * ug.yres + (y0s + hly)) * 3;
+  if (!producer && ug.accumulate == 2 && hpix) {  // a later chunk: continue the chains
+    l0 = hdst[0];
+    l1 = hdst[1];
+    l2 = hdst[2];
+  }
   __syncthreads();  // s_tab, s_rec[0]
   for (int s = -3; s < nbat; ++s) {
     if (producer) {
       if (wave == 1) {
         const int b = s + 3;
-        setup(b, s_rec[(b / kSB) & 1][(b % kSB) * kHB + (lane & (kHB - 1))]);
+        setup(b, s_rec[(b / kSB) & 1][(b % kSB) * kB + min(lane, kB - 1)]);
         if ((s + 4) % kSB == 0 && s + 4 > 0) {  // super-batch J starts next step
           const int J = (s + 4) / kSB;
+          store_recs(J & 1);
+          load_recs();
 #pragma unroll
-          for (int r = 0; r < 4; ++r) s_rec[J & 1][r * 64 + lane] = R[r];
-#pragma unroll
-          for (int r = 0; r < 4; ++r) R[r] = a.recs[I[r]];
-#pragma unroll
-          for (int r = 0; r < 4; ++r) I[r] = entry(J + 2, r);
+          for (int r = 0; r < kSRL; ++r) I[r] = entry(J + 2, r);
         }
       }
       if (s + 2 >= 0 && s + 2 < nbat) tables(s + 2);
-      if (s + 1 >= 0 && s + 1 < nbat) fragments(s + 1);
+      if (s + 1 >= 0 && s + 1 < nbat) operands(s + 1, wave - 1);
     } else if (s >= 0) {
-      const int fb = s & 1;
-      const float4 *cst = s_cst[s & 3];
-      const float *m1b = &s_m1[fb][0][0];
-      const float *m2b = &s_m2[ADD ? 0 : fb][0][0];
+      const float4(*op)[64] = s_op[s & 1];
+      const float4 *cq = s_cq[s & 3];
+      const int cnt = count(s);
+      constexpr int C = 4;  // particles per group of reads
+      static_assert(kB % C == 0, "groups tile the batch");
 #pragma unroll
-      for (int k0 = 0; k0 < kHB; k0 += 8) {
-        float m1[8], m2[8], cq[8];
+      for (int k0 = 0; k0 < kB; k0 += C) {
+        if (k0 >= cnt) break;
+        float4 m[C], q[C];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          m1[j] = m1b[(k0 + j) * 64 + lane];
-          if (!ADD) {
-            m2[j] = m2b[(k0 + j) * 64 + lane];
-            cq[j] = cst[k0 + j].y;
-          }
+        for (int j = 0; j < C; ++j) {
+          m[j] = op[k0 + j][lane];
+          if (!ADD) q[j] = cq[k0 + j];
         }
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          if (ADD) l = l + m1[j];
-          else l = l + ((l - cq[j]) * m1[j]) * m2[j];
+        for (int j = 0; j < C; ++j) {
+          if (ADD) {
+            l0 = l0 + m[j].x;
+            l1 = l1 + m[j].y;
+            l2 = l2 + m[j].z;
+          } else {
+            l0 = l0 + ((l0 - q[j].x) * m[j].x) * m[j].w;
+            l1 = l1 + ((l1 - q[j].y) * m[j].y) * m[j].w;
+            l2 = l2 + ((l2 - q[j].z) * m[j].z) * m[j].w;
+          }
         }
       }
     }
@@ -797,14 +878,20 @@ __global__ __launch_bounds__(kHThreads) void k_splat_heavy(SplatArgs a, const ui
     uint64_t *tr = a.trace + 4 * static_cast<uint64_t>(it);
     tr[0] = t_start;
     tr[1] = __builtin_amdgcn_s_memrealtime();
-    tr[2] = static_cast<uint64_t>(n) | (static_cast<uint64_t>(ch) << 32);
+    tr[2] = static_cast<uint64_t>(n) | (uint64_t(3) << 32);
     tr[3] = static_cast<uint64_t>(__smid());
   }
-  if (!producer && hpix) *hdst = ug.accumulate == 1 ? *hdst + l : l;
+  if (!producer && hpix) {
+    const bool add = ug.accumulate == 1;
+    hdst[0] = add ? hdst[0] + l0 : l0;
+    hdst[1] = add ? hdst[1] + l1 : l1;
+    hdst[2] = add ? hdst[2] + l2 : l2;
+  }
 }
 
 // Work items per unit: none for units outside their tile; none for empty units
-// when accumulating; three (one per channel) for long ordered lists; else one.
+// when accumulating; one producer/consumer item for lists of at least
+// heavy_len; otherwise three (one per channel) for long ordered lists, else one.
 // (need_empty: the job's pass writes its whole image, so empty blocks are
 // written too.)
 __device__ __forceinline__ uint32_t unit_items(const SplatArgs &a, uint32_t u, int ordered,
@@ -813,7 +900,7 @@ __device__ __forceinline__ uint32_t unit_items(const SplatArgs &a, uint32_t u, i
   const UnitGeom g = unit_geom(a, u);
   if (g.gx * 8 >= g.w || g.gy * 8 >= g.h) return 0;
   if (len == 0) return g.need_empty ? 1u : 0u;
-  if (len >= a.heavy_len) return 3u;  // one producer/consumer workgroup per channel
+  if (len >= a.heavy_len) return 1u;  // one producer/consumer workgroup
   return (len >= (ordered ? split : 2 * split)) ? 3u : 1u;
 }
 
@@ -871,12 +958,12 @@ void launch_items_emit(const uint32_t *bin_off, uint32_t nunits, const SplatArgs
   hip_check(hipGetLastError(), "k_items_emit launch");
 }
 
-void launch_splat_heavy(bool additive, const uint32_t *items, uint32_t nheavy, const SplatArgs &a,
-                        hipStream_t s) {
-  if (nheavy == 0) return;
-  if (additive) k_splat_heavy<1><<<nheavy, kHThreads, 0, s>>>(a, items, nheavy);
-  else k_splat_heavy<0><<<nheavy, kHThreads, 0, s>>>(a, items, nheavy);
-  hip_check(hipGetLastError(), "k_splat_heavy launch");
+void launch_splat_pc(bool additive, const uint32_t *items, uint32_t npc, const SplatArgs &a,
+                     hipStream_t s) {
+  if (npc == 0) return;
+  if (additive) k_splat_pc<1><<<npc, kPThreads, 0, s>>>(a, items, npc);
+  else k_splat_pc<0><<<npc, kPThreads, 0, s>>>(a, items, npc);
+  hip_check(hipGetLastError(), "k_splat_pc launch");
 }
 
 void launch_splat8(bool additive, int variant, const uint32_t *items, uint32_t nitems,

@@ -129,13 +129,15 @@ def test_batched_mixed_edge_views(R):
     assert_bitexact(got[1], got[6], "repeated view")
 
 
-@pytest.mark.parametrize("variant,split,heavy", [(4, 0, 0), (1, 0, -1), (3, 64, 200), (4, 64, 1)])
-def test_batched_tuning_invariant(R, variant, split, heavy):
+@pytest.mark.parametrize("variant,split,heavy,hmax", [(4, 0, 0, 0), (1, 0, -1, 0), (3, 64, 200, 0),
+                                                      (4, 64, 1, 0), (4, 0, 300, 1 << 30)])
+def test_batched_tuning_invariant(R, variant, split, heavy, hmax):
     """The splat tuning keys change the GPU work decomposition only."""
     jobs = _small_batch(seed=2500, n_jobs=6)
     views0 = _views(jobs)
     R.render_batched(views0)
-    views1 = _views(jobs, hip_variant=variant, hip_split=split, hip_heavy=heavy)
+    views1 = _views(jobs, hip_variant=variant, hip_split=split, hip_heavy=heavy,
+                    hip_heavy_max=hmax)
     R.render_batched(views1)
     torch.cuda.synchronize()
     for j, (a, b) in enumerate(zip(views0, views1)):

@@ -289,13 +289,15 @@ def test_dense_cluster_vs_oracle(R, aeq):
     v = synth.default_view(160, 144)
     pal = synth.default_palette()
     opic, _ = O.host_rendering(fld, v, [pal], a_eq_e=aeq, stable=True)
-    for G, var, split, heavy in ((8, 3, 0, 0), (8, 1, 0, -1), (8, 2, 0, -1), (8, 3, 64, -1),
-                                 (8, 1, 1 << 30, -1), (8, 4, 0, -1), (8, 4, 64, 200),
-                                 (8, 4, 0, 1), (32, 0, 0, 0)):
+    for G, var, split, heavy, hmax in ((8, 3, 0, 0, 0), (8, 1, 0, -1, 0), (8, 2, 0, -1, 0),
+                                       (8, 3, 64, -1, 0), (8, 1, 1 << 30, -1, 0), (8, 4, 0, -1, 0),
+                                       (8, 4, 64, 200, 0), (8, 4, 0, 1, 0), (8, 4, 0, 1, 1 << 30),
+                                       (8, 4, 64, 5000, 3), (32, 0, 0, 0, 0)):
         pic = render(R, fld, v, [pal], S.params(a_eq_e=aeq, hip_region=G, hip_variant=var,
-                                                hip_split=split, hip_heavy=heavy))
-        assert_bitexact(pic, opic, "dense cluster G=%d variant=%d split=%d heavy=%d"
-                        % (G, var, split, heavy))
+                                                hip_split=split, hip_heavy=heavy,
+                                                hip_heavy_max=hmax))
+        assert_bitexact(pic, opic, "dense cluster G=%d variant=%d split=%d heavy=%d max=%d"
+                        % (G, var, split, heavy, hmax))
 
 
 @pytest.mark.parametrize("aeq", [False, True])
@@ -306,7 +308,7 @@ def test_heavy_blocks_on_fixtures(R, golden, aeq):
     f = golden(name)
     kw = {} if aeq else dict(brightness=[float(f["brightness"][0])])
     pic = render(R, field_from(f), view_from(f), [palette(golden)],
-                 S.params(a_eq_e=aeq, hip_heavy=1, **kw))
+                 S.params(a_eq_e=aeq, hip_heavy=1, hip_heavy_max=1 << 30, **kw))
     want = f["pic_T50"] if aeq else f["pic_stable_T50"]
     assert_bitexact(pic, want, "heavy kernel " + name)
 
@@ -328,8 +330,9 @@ def test_errors(R):
         R.render(S.to_device(fld), cam, S.params(col_vector=[1]), pic)
 
 
-@pytest.mark.parametrize("chunk,heavy", [(20_000, 0), (3_000, 0), (20_000, 1)])
-def test_chunked_passes_bitexact(R, chunk, heavy):
+@pytest.mark.parametrize("chunk,heavy,hmax", [(20_000, 0, 0), (3_000, 0, 0), (20_000, 1, 0),
+                                              (20_000, 1, 1 << 30)])
+def test_chunked_passes_bitexact(R, chunk, heavy, hmax):
     """A pass with more bin entries than hip_chunk_entries is rendered as
     consecutive ranges of its render order, each continuing the pixels'
     chains from the image: the image equals the unchunked render bit for bit
@@ -342,10 +345,11 @@ def test_chunked_passes_bitexact(R, chunk, heavy):
     v = synth.default_view(192, 160)
     pal = synth.default_palette()
     for aeq in ([0, 1], [0, 0], [1, 1]):
-        base = S.params(ptypes=2, a_eq_e=aeq, brightness=[0.5, 2.0], hip_heavy=heavy)
+        base = S.params(ptypes=2, a_eq_e=aeq, brightness=[0.5, 2.0], hip_heavy=heavy,
+                        hip_heavy_max=hmax)
         want = render(R, fld, v, [pal, pal], base)
         prm = S.params(ptypes=2, a_eq_e=aeq, brightness=[0.5, 2.0], hip_heavy=heavy,
-                       hip_chunk_entries=chunk)
+                       hip_heavy_max=hmax, hip_chunk_entries=chunk)
         got = render(R, fld, v, [pal, pal], prm)
         _, counts = R.stage_times()
         assert max(counts["ordered_entries"], counts["additive_entries"]) > chunk
