@@ -548,7 +548,7 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
 constexpr int kPW = 7;                 // producer waves
 constexpr int kPB = 4;                 // particles per producer wave and step
 constexpr int kB = kPW * kPB;          // particles per step
-constexpr int kQ = 2;                  // operand stage: particles per group of LDS reads
+constexpr int kQ = 4;                  // operand stage: particles per group of LDS reads
 static_assert(kPB % kQ == 0, "groups tile a producer's particles");
 constexpr int kPThreads = 64 * (kPW + 1);
 constexpr int kPProd = 64 * kPW;       // producer lanes
