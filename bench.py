@@ -18,6 +18,11 @@ The JSON line also carries:
                 records around it on its own stream (average of the profiled
                 frames); traffic from rocprofv3 FETCH_SIZE + WRITE_SIZE when
                 --traffic-kib gives them (KiB per dispatch, already scaled);
+  valu          the limit the splat actually meets: VALU instructions of the
+                ordered splat per frame (rocprofv3 SQ_INSTS_VALU, committed
+                summary) over the ordered splat stage's live duration, against
+                the issue peak (256 CUs x 4 SIMDs x one wave64 instruction per
+                4 clocks at 2.4 GHz);
   batched       the batched workload (64 views, synth.batch_jobs) as one
                 splhip_render_batched call vs one splhip_render per view;
   fwd_bwd       the differentiable workload: 2M particles, 512^2, additive,
@@ -43,6 +48,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ms/frame & Mparticles/s (16M, 2048²); batched and fwd+bwd vs plain"
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK = 256 * 4 * 2.4e9 / 4  # wave64 VALU instructions per second (one per SIMD per 4 clocks)
 
 
 def log(*a):
@@ -437,6 +443,19 @@ def main():
                 if line.startswith("traffic_bytes_per_dispatch k_splat8<0, 4>:"):
                     traffic = int(line.split(":")[1])
                     traffic_src = "profiles/pmc_traffic_gas_stars_16M_2048.md"
+    valu = None
+    vprof = os.path.join(ROOT, "profiles", "pmc_valu_gas_stars_16M_2048.md")
+    if (args.n_gas, args.n_stars, args.res) == (8_000_000, 8_000_000, 2048) and \
+            os.path.exists(vprof):
+        for line in open(vprof):
+            if line.startswith("valu_instructions_per_frame ordered_splat:"):
+                ninst = int(line.split(":")[1])
+                rate = ninst / (stages["ordered_splat"] * 1e-3)
+                valu = {"kernels": "k_splat8<ordered> + k_splat_pc<ordered> (concurrent)",
+                        "instructions": ninst, "stage_ms": stages["ordered_splat"],
+                        "achieved": round(rate / 1e9, 2), "peak": VALU_PEAK / 1e9,
+                        "unit": "G wave64 instr/s", "frac": round(rate / VALU_PEAK, 4),
+                        "source": "profiles/pmc_valu_gas_stars_16M_2048.md"}
     frame_bytes = 25 * field.n + 12 * W * H
     log("frame %.3f ms (events %.3f ms); stages %s; counts %s" % (ms_frame, ev_ms, stages,
                                                                   counts))
@@ -468,8 +487,9 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
                      "traffic_source": traffic_src,
                      "bytes_per_launch": splat_bytes, "launch_ms": splat_ms,
-                     "note": "the splat is VALU/LDS-bound (per-pixel serial update "
-                             "chains); HBM is not its limit"},
+                     "note": "the splat is VALU-issue-bound (see valu); HBM is not its "
+                             "limit"},
+        "valu": valu,
     }
     if not args.no_batched:
         out["batched"] = bench_batched(S, R, torch, max(2, args.steps // 2), 1)
