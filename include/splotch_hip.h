@@ -62,7 +62,8 @@ typedef struct {
   int32_t tile_size;                       /* tile_size_cpu (50): reference tile */
   int32_t hip_region;                      /* hip_region: GPU work region, px: 8, 16, 32, 64 (0 = auto) */
   int32_t hip_split;                       /* hip_split: list length from which an 8x8 block is
-                                              splatted as one work item per channel (0 = auto) */
+                                              splatted as one work item per channel
+                                              (0 = auto: 32768) */
   int32_t hip_variant;                     /* hip_variant: 8x8 splat inner loop, 0 = auto (4),
                                               1 = per particle, 2 = 4-particle groups,
                                               3 = software pipelined, 4 = chunked branch-free */
@@ -70,7 +71,7 @@ typedef struct {
                                               splatted by a producer/consumer workgroup
                                               (0 = auto: 32768, < 0 = never) */
   int32_t hip_heavy_max;                   /* hip_heavy_max: at most this many such workgroups
-                                              per pass, the heaviest (0 = auto: 256) */
+                                              per pass, the heaviest (0 = auto: 192) */
   int32_t hip_chunk_entries;               /* hip_chunk_entries: a single-view pass with more
                                               (block, particle) entries is rendered in chunks
                                               of consecutive render order, each continuing

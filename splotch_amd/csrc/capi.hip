@@ -626,11 +626,14 @@ struct Tuning {
 Tuning tuning_of(const splhip_params *prm) {
   Tuning t;
   t.variant = prm->hip_variant > 0 ? prm->hip_variant : 4;
-  t.split = prm->hip_split > 0 ? static_cast<uint32_t>(prm->hip_split) : 4096u;
+  // lists from hip_heavy on go to producer/consumer workgroups (up to the cap)
+  // or stay whole: splitting by channel pays only without those workgroups
+  t.split = prm->hip_split > 0 ? static_cast<uint32_t>(prm->hip_split) : 32768u;
   t.heavy = prm->hip_heavy < 0 ? 0xffffffffu
             : prm->hip_heavy > 0 ? static_cast<uint32_t>(prm->hip_heavy) : 32768u;
-  // two 512-thread heavy workgroups fit a CU; more than two per CU queue
-  t.heavy_max = prm->hip_heavy_max > 0 ? static_cast<uint32_t>(prm->hip_heavy_max) : 256u;
+  // a producer/consumer workgroup occupies a CU: leave the splat8 items
+  // about a quarter of the CUs while the longest lists run
+  t.heavy_max = prm->hip_heavy_max > 0 ? static_cast<uint32_t>(prm->hip_heavy_max) : 192u;
   t.chunk = prm->hip_chunk_entries > 0 ? static_cast<uint64_t>(prm->hip_chunk_entries)
                                        : (uint64_t(1) << 30);
   return t;
@@ -871,8 +874,8 @@ Tuning autotune(splhip_ctx *c, Plan &pl, const Tuning &base, uint32_t *tot, hipS
   Tuning best = base;
   float best_ms = 1e30f;
   for (int variant : {4, 3})
-    for (uint32_t split : {2048u, 4096u, 8192u})
-      for (uint32_t hmax : {0u, 128u, 256u, 512u}) {
+    for (uint32_t split : {4096u, 8192u, 32768u})
+      for (uint32_t hmax : {0u, 96u, 192u, 256u}) {
         Tuning t = base;
         t.variant = variant;
         t.split = split;

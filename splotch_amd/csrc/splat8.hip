@@ -521,7 +521,7 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
 //
 // A block whose list holds thousands to hundreds of thousands of particles is
 // near the critical path of the frame: its pixels' update chains are serial.
-// Such a block gets a 512-thread workgroup.  Wave 0 is the consumer: lane =
+// Such a block gets a 768-thread workgroup.  Wave 0 is the consumer: lane =
 // pixel, and per particle it only applies, per channel c,
 //     l_c += ((l_c - q_c) * m1_c) * m2     (ordered)     l_c += t_c   (additive)
 // with operands that reproduce render_new's update bit for bit
@@ -533,7 +533,7 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
 // (l starts at +0 and a float sum is -0 only if both terms are -0, so l is
 // never -0 and adding a zero leaves it unchanged, as the reference's skip.
 // For finite records l - q stays finite, so (l - q) * 0 is a zero.)
-// Waves 1..7 run a pipeline ahead of the consumer: particle set-up three
+// Waves 1..11 run a pipeline ahead of the consumer: particle set-up three
 // steps ahead (tile-frame position, box, Gaussian width, emission /
 // absorption; splotch_host.cc:462-476, 591-632), the separable column / row
 // tables two steps ahead (splotch_host.cc:596-607), and the operands one step
@@ -543,21 +543,22 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
 // The consumer runs at a higher wave priority than the producers, which
 // share its SIMD.
 // Records reach the set-up through LDS in super-batches that wave 1 loads
-// about eight steps before their use, so the gather latency stays off the
-// critical path.
-constexpr int kPW = 7;                 // producer waves
+// about five steps before their use, so the gather latency stays off the
+// critical path.  Three waves per SIMD hide most of the LDS latency of the
+// stages; a workgroup occupies its CU (LDS and registers).
+constexpr int kPW = 11;                // producer waves (three waves per SIMD)
 constexpr int kPB = 4;                 // particles per producer wave and step
 constexpr int kB = kPW * kPB;          // particles per step
 constexpr int kQ = 4;                  // operand stage: particles per group of LDS reads
 static_assert(kPB % kQ == 0, "groups tile a producer's particles");
 constexpr int kPThreads = 64 * (kPW + 1);
 constexpr int kPProd = 64 * kPW;       // producer lanes
-constexpr int kSB = 8;                 // steps per record super-batch
+constexpr int kSB = 5;                 // steps per record super-batch
 constexpr int kSR = kSB * kB;          // records per super-batch
 constexpr int kSRL = (kSR + 63) / 64;  // records per wave-1 lane and super-batch
 static_assert(kSB >= 5, "a super-batch is stored four steps before its first use");
 constexpr int kSRP = kSRL * 64;        // staging row: whole wave stores (tail unused)
-static_assert(kB <= 32, "fast-particle mask is 32 bits");
+static_assert(kB <= 64, "fast-particle mask is 64 bits");
 
 struct PcPart {
   float posx, posy, radsq, stp;
@@ -578,7 +579,7 @@ __global__ __launch_bounds__(kPThreads) void k_splat_pc(SplatArgs a, const uint3
   __shared__ PcPart s_part[2][kB];
   __shared__ float4 s_ca[4][kB];        // (a0, a1, a2, max |a|)
   __shared__ float4 s_cq[4][kB];        // (q0, q1, q2, 0)
-  __shared__ uint32_t s_fast[4];        // bit k: particle k has every a_c in [kFastA, 0]
+  __shared__ uint64_t s_fast[4];        // bit k: particle k has every a_c in [kFastA, 0]
   __shared__ uint32_t s_cm[2][kB][8];   // column row-masks
   __shared__ float s_p2[2][kB][8];      // column factors
   __shared__ float s_p1[2][kB][8];      // row factors
@@ -636,7 +637,7 @@ __global__ __launch_bounds__(kPThreads) void k_splat_pc(SplatArgs a, const uint3
       s_ca[b & 3][lane] = make_float4(a0, a1, a2, mx);
       s_cq[b & 3][lane] = make_float4(q0, q1, q2, 0.f);
     }
-    if (lane == 0) s_fast[b & 3] = static_cast<uint32_t>(fb) & ((1u << kB) - 1u);
+    if (lane == 0) s_fast[b & 3] = fb;  // bits past the batch are not read
   };
   // Column ranges / factors and row factors of batch b into slot b & 1: task
   // (particle k, j) is column j (j < 8) or row j - 8.  Unrolled over a lane's
@@ -768,11 +769,13 @@ __global__ __launch_bounds__(kPThreads) void k_splat_pc(SplatArgs a, const uint3
     for (int j = 0; j < kQ; ++j) s_op[ts][kb + j][lane] = op[j];
   };
   auto operands = [&](int b, int pw) {
-    const uint32_t fb = uniform(s_fast[b & 3]);
+    const uint64_t f64 = s_fast[b & 3];
+    const uint64_t fb = (static_cast<uint64_t>(uniform(static_cast<uint32_t>(f64 >> 32))) << 32) |
+                        uniform(static_cast<uint32_t>(f64));
 #pragma unroll
     for (int g = 0; g < kPB; g += kQ) {
       const int kb = pw * kPB + g;
-      const uint32_t want = ((1u << kQ) - 1u) << kb;
+      const uint64_t want = ((uint64_t(1) << kQ) - 1u) << kb;
       if (!ADD && (fb & want) == want)
         operands_group(b, kb, std::true_type{});
       else
