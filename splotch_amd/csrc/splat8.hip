@@ -79,6 +79,18 @@ __device__ __forceinline__ float xexpm1_finish(float arg, float p, float t1, flo
   return (fabsf(arg) < tl) ? arg : opv - 1.f;
 }
 
+// Table entries of the fast path, where p = arg * expfac lies in [0, 2^20):
+// i4 = trunc(arg * 4 expfac) = 4 trunc(p) + (0..3) (scaling by 4 is exact), so
+// the byte offsets of tab1[trunc(p) >> 10] and tab2[trunc(p) & 1023] are two
+// masks of i4.
+__device__ __forceinline__ void xtab_fast(const float *tab, float arg, float expfac4, float &t1,
+                                          float &t2) {
+  const uint32_t i4 = static_cast<uint32_t>(static_cast<int>(arg * expfac4));
+  const char *b = reinterpret_cast<const char *>(tab);
+  t1 = *reinterpret_cast<const float *>(b + ((i4 >> 10) & 0xffcu));
+  t2 = *reinterpret_cast<const float *>(b + 4096 + (i4 & 0xffcu));
+}
+
 __device__ __forceinline__ int xexp_index(float p) {
   const bool ok = (p < 1048576.f) && (p > -1.f);
   return static_cast<int>(ok ? p : 0.f);
@@ -98,9 +110,11 @@ __device__ __forceinline__ void walk_chunks(int nb, int cx, int cy, const uint32
                                             const float (*pre2)[kPad], const float (*pre1)[kPad],
                                             const float *tab, float va0, float va1, float va2,
                                             float vq0, float vq1, float vq2, float vmax,
-                                            uint64_t fastmask, float expfac, float tl, float &l0,
-                                            float &l1, float &l2) {
+                                            const float4 (*cst)[2], uint64_t fastmask,
+                                            float expfac, float tl, float &l0, float &l1,
+                                            float &l2) {
   constexpr bool C0 = (CH == 3 || CH == 0), C1 = (CH == 3 || CH == 1), C2 = (CH == 3 || CH == 2);
+  const float expfac4 = 4.f * expfac;  // exact
   for (int k0 = 0; k0 < nb; k0 += J) {
     if (!ADD) {
       // Fast chunk: every particle of the chunk has all absorptions a_c in
@@ -111,45 +125,49 @@ __device__ __forceinline__ void walk_chunks(int nb, int cx, int cy, const uint32
       //   (x * 1 == x);  outside the footprint: m2 = 0 (l + +-0 == l, as l is
       //   never -0),
       // which is render_new's update bit for bit (splotch_host.cc:648-658).
+      // The particles' constants come from LDS (cst: (a0, a1, a2, max |a|),
+      // (q0, q1, q2, 0)) as vector operands.
       const int cnt = min(J, nb - k0);
       const uint64_t need = ((1ull << cnt) - 1ull) << k0;
       if ((fastmask & need) == need) {
         float m2[J], g0[J], g1[J], g2[J], u0[J], w0[J], u1[J], w1[J], u2[J], w2[J];
+        float4 ca[J], cq[J];
         bool tay[J];
 #pragma unroll
         for (int j = 0; j < J; ++j) {
           const int k = min(k0 + j, nb - 1);  // uniform; out-of-batch slots masked
-          const uint32_t mk = cmask[cx][k];
-          const uint32_t mv = (k0 + j < nb) ? mk : 0u;
-          const bool in = (mv >> cy) & 1u;
-          const float at = pre1[cy][k] * pre2[cx][k];
-          tay[j] = (lane_f(vmax, k) * at) < tl;
-          m2[j] = in ? (tay[j] ? at : 1.f) : 0.f;
-          if (C0) { g0[j] = at * lane_f(va0, k); const int i = static_cast<int>(g0[j] * expfac); u0[j] = tab[i >> 10]; w0[j] = tab[1024 + (i & 1023)]; }
-          if (C1) { g1[j] = at * lane_f(va1, k); const int i = static_cast<int>(g1[j] * expfac); u1[j] = tab[i >> 10]; w1[j] = tab[1024 + (i & 1023)]; }
-          if (C2) { g2[j] = at * lane_f(va2, k); const int i = static_cast<int>(g2[j] * expfac); u2[j] = tab[i >> 10]; w2[j] = tab[1024 + (i & 1023)]; }
+          ca[j] = cst[k][0];
+          cq[j] = cst[k][1];
         }
 #pragma unroll
         for (int j = 0; j < J; ++j) {
           const int k = min(k0 + j, nb - 1);
-          // particle values read into scalars before the selects that use them
+          const uint32_t mk = cmask[cx][k];
+          const uint32_t mv = (k0 + j < nb) ? mk : 0u;
+          const bool in = (mv >> cy) & 1u;
+          const float at = pre1[cy][k] * pre2[cx][k];
+          tay[j] = (ca[j].w * at) < tl;
+          m2[j] = in ? (tay[j] ? at : 1.f) : 0.f;
+          if (C0) { g0[j] = at * ca[j].x; xtab_fast(tab, g0[j], expfac4, u0[j], w0[j]); }
+          if (C1) { g1[j] = at * ca[j].y; xtab_fast(tab, g1[j], expfac4, u1[j], w1[j]); }
+          if (C2) { g2[j] = at * ca[j].z; xtab_fast(tab, g2[j], expfac4, u2[j], w2[j]); }
+        }
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
           if (C0) {
-            const float a = lane_f(va0, k), q = lane_f(vq0, k);
             const float em = (fabsf(g0[j]) < tl) ? g0[j] : u0[j] * w0[j] - 1.f;
-            const float m1 = tay[j] ? a : em;
-            l0 = l0 + ((l0 - q) * m1) * m2[j];
+            const float m1 = tay[j] ? ca[j].x : em;
+            l0 = l0 + ((l0 - cq[j].x) * m1) * m2[j];
           }
           if (C1) {
-            const float a = lane_f(va1, k), q = lane_f(vq1, k);
             const float em = (fabsf(g1[j]) < tl) ? g1[j] : u1[j] * w1[j] - 1.f;
-            const float m1 = tay[j] ? a : em;
-            l1 = l1 + ((l1 - q) * m1) * m2[j];
+            const float m1 = tay[j] ? ca[j].y : em;
+            l1 = l1 + ((l1 - cq[j].y) * m1) * m2[j];
           }
           if (C2) {
-            const float a = lane_f(va2, k), q = lane_f(vq2, k);
             const float em = (fabsf(g2[j]) < tl) ? g2[j] : u2[j] * w2[j] - 1.f;
-            const float m1 = tay[j] ? a : em;
-            l2 = l2 + ((l2 - q) * m1) * m2[j];
+            const float m1 = tay[j] ? ca[j].z : em;
+            l2 = l2 + ((l2 - cq[j].z) * m1) * m2[j];
           }
         }
         continue;
@@ -299,6 +317,7 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
   __shared__ uint32_t s_col[4][8][kPad];
   __shared__ float s_pre2[4][8][kPad];
   __shared__ float s_pre1[4][8][kPad];
+  __shared__ float4 s_cst[ADD ? 1 : 4][ADD ? 1 : 64][2];  // ordered: per-particle constants
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int i = tid; i < 2048; i += 256) s_tab[i] = a.exptab[i];
   __syncthreads();
@@ -318,6 +337,7 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
   uint32_t(*col)[kPad] = s_col[wave];
   float(*pre2)[kPad] = s_pre2[wave];
   float(*pre1)[kPad] = s_pre1[wave];
+  float4(*wcst)[2] = s_cst[ADD ? 0 : wave];
   const float expfac = a.expfac, tl = a.taylorlimit;
 
   float l0 = 0.f, l1 = 0.f, l2 = 0.f;
@@ -390,6 +410,8 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
         vmax = fmaxf(fabsf(va0), fmaxf(fabsf(va1), fabsf(va2)));
         vfast = va0 >= kFastA && va0 <= 0.f && va1 >= kFastA && va1 <= 0.f && va2 >= kFastA &&
                 va2 <= 0.f;
+        wcst[lane][0] = make_float4(va0, va1, va2, vmax);
+        wcst[lane][1] = make_float4(vq0, vq1, vq2, 0.f);
       }
     }
     wave_sync();
@@ -397,10 +419,10 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
     // Phase B: lane = pixel, particles in list order.
     if (VAR == 4) {
       switch (chan) {
-        case 0: walk_chunks<ADD, 0, 8>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, fastmask, expfac, tl, l0, l1, l2); break;
-        case 1: walk_chunks<ADD, 1, 8>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, fastmask, expfac, tl, l0, l1, l2); break;
-        case 2: walk_chunks<ADD, 2, 8>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, fastmask, expfac, tl, l0, l1, l2); break;
-        default: walk_chunks<ADD, 3, (ADD ? 8 : 4)>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, fastmask, expfac, tl, l0, l1, l2); break;
+        case 0: walk_chunks<ADD, 0, 8>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, wcst, fastmask, expfac, tl, l0, l1, l2); break;
+        case 1: walk_chunks<ADD, 1, 8>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, wcst, fastmask, expfac, tl, l0, l1, l2); break;
+        case 2: walk_chunks<ADD, 2, 8>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, wcst, fastmask, expfac, tl, l0, l1, l2); break;
+        default: walk_chunks<ADD, 3, (ADD ? 8 : 4)>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, wcst, fastmask, expfac, tl, l0, l1, l2); break;
       }
       wave_sync();
       continue;
@@ -599,6 +621,7 @@ __global__ __launch_bounds__(kPThreads) void k_splat_pc(SplatArgs a, const uint3
   const int n = static_cast<int>(e1 - e0);
   const int nbat = (n + kB - 1) / kB;
   const float expfac = a.expfac, tl = a.taylorlimit;
+  const float expfac4 = 4.f * expfac;  // exact
   const int cx = lane >> 3, cy = lane & 7;  // producers and consumer: lane = pixel
   if (producer) __builtin_amdgcn_s_setprio(2);
   else __builtin_amdgcn_s_setprio(3);
@@ -733,18 +756,22 @@ __global__ __launch_bounds__(kPThreads) void k_splat_pc(SplatArgs a, const uint3
         g0[j] = at[j] * ca[j].x;
         g1[j] = at[j] * ca[j].y;
         g2[j] = at[j] * ca[j].z;
-        pg0[j] = g0[j] * expfac;
-        pg1[j] = g1[j] * expfac;
-        pg2[j] = g2[j] * expfac;
-        const int i0 = FAST ? static_cast<int>(pg0[j]) : xexp_index(pg0[j]);
-        const int i1 = FAST ? static_cast<int>(pg1[j]) : xexp_index(pg1[j]);
-        const int i2 = FAST ? static_cast<int>(pg2[j]) : xexp_index(pg2[j]);
-        u0[j] = s_tab[i0 >> 10];
-        w0[j] = s_tab[1024 + (i0 & 1023)];
-        u1[j] = s_tab[i1 >> 10];
-        w1[j] = s_tab[1024 + (i1 & 1023)];
-        u2[j] = s_tab[i2 >> 10];
-        w2[j] = s_tab[1024 + (i2 & 1023)];
+        if (FAST) {
+          xtab_fast(s_tab, g0[j], expfac4, u0[j], w0[j]);
+          xtab_fast(s_tab, g1[j], expfac4, u1[j], w1[j]);
+          xtab_fast(s_tab, g2[j], expfac4, u2[j], w2[j]);
+        } else {
+          pg0[j] = g0[j] * expfac;
+          pg1[j] = g1[j] * expfac;
+          pg2[j] = g2[j] * expfac;
+          const int i0 = xexp_index(pg0[j]), i1 = xexp_index(pg1[j]), i2 = xexp_index(pg2[j]);
+          u0[j] = s_tab[i0 >> 10];
+          w0[j] = s_tab[1024 + (i0 & 1023)];
+          u1[j] = s_tab[i1 >> 10];
+          w1[j] = s_tab[1024 + (i1 & 1023)];
+          u2[j] = s_tab[i2 >> 10];
+          w2[j] = s_tab[1024 + (i2 & 1023)];
+        }
       }
 #pragma unroll
       for (int j = 0; j < kQ; ++j) {
