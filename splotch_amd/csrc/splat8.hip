@@ -114,6 +114,7 @@ __device__ __forceinline__ void walk_chunks(int nb, int cx, int cy, const uint32
                                             float expfac, float tl, float &l0, float &l1,
                                             float &l2) {
   constexpr bool C0 = (CH == 3 || CH == 0), C1 = (CH == 3 || CH == 1), C2 = (CH == 3 || CH == 2);
+  static_assert(64 % J == 0, "chunks tile a batch: slot k0 + j < 64");
   const float expfac4 = 4.f * expfac;  // exact
   for (int k0 = 0; k0 < nb; k0 += J) {
     if (!ADD) {
@@ -133,15 +134,17 @@ __device__ __forceinline__ void walk_chunks(int nb, int cx, int cy, const uint32
         float m2[J], g0[J], g1[J], g2[J], u0[J], w0[J], u1[J], w1[J], u2[J], w2[J];
         float4 ca[J], cq[J];
         bool tay[J];
+        // slots past the batch hold finite values of earlier batches (or
+        // zeros) and are masked; table offsets are masked into the tables
 #pragma unroll
         for (int j = 0; j < J; ++j) {
-          const int k = min(k0 + j, nb - 1);  // uniform; out-of-batch slots masked
+          const int k = k0 + j;
           ca[j] = cst[k][0];
           cq[j] = cst[k][1];
         }
 #pragma unroll
         for (int j = 0; j < J; ++j) {
-          const int k = min(k0 + j, nb - 1);
+          const int k = k0 + j;
           const uint32_t mk = cmask[cx][k];
           const uint32_t mv = (k0 + j < nb) ? mk : 0u;
           const bool in = (mv >> cy) & 1u;
@@ -338,6 +341,17 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
   float(*pre2)[kPad] = s_pre2[wave];
   float(*pre1)[kPad] = s_pre1[wave];
   float4(*wcst)[2] = s_cst[ADD ? 0 : wave];
+  if (!ADD) {
+    // the fast chunks read every slot of a chunk, also past a short batch
+    // (masked there): start from finite values
+    for (int i = lane; i < 8 * kPad; i += 64) {
+      (&col[0][0])[i] = 0u;
+      (&pre2[0][0])[i] = 0.f;
+      (&pre1[0][0])[i] = 0.f;
+    }
+    wcst[lane][0] = make_float4(0.f, 0.f, 0.f, 0.f);
+    wcst[lane][1] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
   const float expfac = a.expfac, tl = a.taylorlimit;
 
   float l0 = 0.f, l1 = 0.f, l2 = 0.f;
