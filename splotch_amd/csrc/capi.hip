@@ -65,6 +65,11 @@ struct splhip_ctx {
   // auxiliary stream (default priority) for the heavy-block kernel
   hipStream_t aux = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // second stream for the additive pass's splat, which runs beside the
+  // ordered pass's, with its own temporaries
+  hipStream_t aux2 = nullptr;
+  hipEvent_t ev_fork2 = nullptr, ev_join2 = nullptr;
+  Workspace ws2;
 };
 
 namespace {
@@ -257,6 +262,13 @@ void ensure_aux(splhip_ctx *c) {
   hip_check(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking), "hipStreamCreate");
   hip_check(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "hipEventCreate");
   hip_check(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming), "hipEventCreate");
+}
+
+void ensure_aux2(splhip_ctx *c) {
+  if (c->aux2) return;
+  hip_check(hipStreamCreateWithFlags(&c->aux2, hipStreamNonBlocking), "hipStreamCreate");
+  hip_check(hipEventCreateWithFlags(&c->ev_fork2, hipEventDisableTiming), "hipEventCreate");
+  hip_check(hipEventCreateWithFlags(&c->ev_join2, hipEventDisableTiming), "hipEventCreate");
 }
 
 
@@ -685,9 +697,14 @@ void stage_items_count(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, uint32_
 }
 
 // Splat of pass p: items heaviest first; heavy units on the auxiliary stream.
-void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s) {
+// wt: workspace of the temporaries (the item sort); jobs: a job table in
+// place of the plan's; fork: the long-list workgroups run on the auxiliary
+// stream beside splat8 (else on s, first).
+void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s,
+                 Workspace *wt = nullptr, JobSplat *jobs = nullptr, bool fork = true) {
   Workspace &ws = c->ws;
-  const SplatArgs sa0 = splat_args(c, pl, p, tn);
+  SplatArgs sa0 = splat_args(c, pl, p, tn);
+  if (jobs) sa0.jobs = jobs;
   mark(c, p == 0 ? 3 : 6, true, s);
   if (pl.G != 8) {
     launch_splat(pl.G, p == 1, pl.nunits, sa0, s);
@@ -701,7 +718,7 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
   uint32_t *ik1 = ws.get_as<uint32_t>(pslot(p, P_IK1), nitems);
   uint32_t *iv1 = ws.get_as<uint32_t>(pslot(p, P_IV1), nitems);
   launch_items_emit(pl.bin_off[p], pl.nunits, sa0, p == 0 ? 1 : 0, tn.split, ioff, ik0, iv0, s);
-  const int rs = radix_sort_pairs(ik0, iv0, ik1, iv1, nitems, 0, 32, ws, s);
+  const int rs = radix_sort_pairs(ik0, iv0, ik1, iv1, nitems, 0, 32, wt ? *wt : ws, s);
   const uint32_t *items = rs ? iv1 : iv0;
   pl.items[p] = items;
   SplatArgs sa = sa0;
@@ -709,7 +726,14 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
     sa.trace = ws.get_as<uint64_t>(pslot(p, P_TRACE), static_cast<size_t>(nitems) * 4);
   SplatArgs sn = sa;  // normal items follow the heavy ones
   if (sn.trace) sn.trace += 4 * static_cast<size_t>(nheavy);
-  if (nheavy > 0) {
+  if (nheavy > 0 && !fork) {
+    mark(c, p == 0 ? 8 : 10, true, s);
+    launch_splat_pc(p == 1, items, nheavy, sa, s);
+    mark(c, p == 0 ? 8 : 10, false, s);
+    mark(c, p == 0 ? 7 : 9, true, s);
+    launch_splat8(p == 1, tn.variant, items + nheavy, nitems - nheavy, sn, s);
+    mark(c, p == 0 ? 7 : 9, false, s);
+  } else if (nheavy > 0) {
     // heavy blocks run on a second stream, beside the normal items
     ensure_aux(c);
     hip_check(hipEventRecord(c->ev_fork, s), "record fork");
@@ -994,8 +1018,41 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
         pl.nheavy[p] = pl.m[p] ? std::min(c->h_tot[2 + p], tn.heavy_max) : 0;
       }
     }
-    for (int p = 0; p < 2; ++p)
-      if (pl.m[p]) stage_splat(c, pl, p, tn, s);
+    // A single view with both passes: the additive pass's splat renders into
+    // a scratch image on a second stream, beside the ordered splat, and is
+    // then added to the image: pic + l per pixel, as the pass's add mode
+    // (pixels it does not touch add +0, which leaves any value but -0 as it
+    // is; the ordered pass never writes -0).
+    const bool overlap = nj == 1 && G == 8 && pl.m[0] && pl.m[1] &&
+                         pl.js[0].accumulate[1] == 1;
+    if (overlap) {
+      const JobSplat &J0 = pl.js[0];
+      const int64_t npix = static_cast<int64_t>(J0.xres) * J0.yres * 3;
+      float *scratch = ws.get_as<float>(S_SCRATCH, static_cast<size_t>(npix));
+      JobSplat *d_alt = ws.get_as<JobSplat>(S_JOBS2, 1);
+      // same flags as the items were counted with, but written to a cleared
+      // scratch image (blocks without items stay zero)
+      JobSplat alt = J0;
+      alt.pic = scratch;
+      alt.accumulate[1] = 0;
+      char *h = staging(c, sizeof(JobSplat));
+      std::memcpy(h, &alt, sizeof(JobSplat));
+      hip_check(hipMemcpyAsync(d_alt, h, sizeof(JobSplat), hipMemcpyHostToDevice, s),
+                "upload scratch job");
+      staged(c, s);
+      ensure_aux2(c);
+      hip_check(hipEventRecord(c->ev_fork2, s), "record fork");
+      stage_splat(c, pl, 0, tn, s);
+      hip_check(hipStreamWaitEvent(c->aux2, c->ev_fork2, 0), "wait fork");
+      launch_fill(scratch, 0.f, npix, c->aux2);
+      stage_splat(c, pl, 1, tn, c->aux2, &c->ws2, d_alt, false);
+      hip_check(hipEventRecord(c->ev_join2, c->aux2), "record join");
+      hip_check(hipStreamWaitEvent(s, c->ev_join2, 0), "wait join");
+      launch_add_images(J0.pic, scratch, npix, s);
+    } else {
+      for (int p = 0; p < 2; ++p)
+        if (pl.m[p]) stage_splat(c, pl, p, tn, s);
+    }
   }
   // jobs with nothing to splat in either pass
   for (int j = 0; j < nj; ++j) {
@@ -1146,6 +1203,9 @@ void splhip_destroy(splhip_ctx *c) {
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   if (c->aux) (void)hipStreamDestroy(c->aux);
+  if (c->ev_fork2) (void)hipEventDestroy(c->ev_fork2);
+  if (c->ev_join2) (void)hipEventDestroy(c->ev_join2);
+  if (c->aux2) (void)hipStreamDestroy(c->aux2);
   c->ws.release();
   if (c->d_exptab) (void)hipFree(c->d_exptab);
   if (c->d_palx) (void)hipFree(c->d_palx);
