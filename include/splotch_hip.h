@@ -261,6 +261,13 @@ int splhip_trace_items(splhip_ctx *ctx, int32_t ordered, uint64_t *out, int64_t 
    counterpart. */
 int splhip_last_tuning(splhip_ctx *ctx, int64_t out[4]);
 
+/* How the last render binned its passes (no reference counterpart): out[p] =
+   the (16x16 region, particle) entries pass p (0 ordered, 1 additive) was
+   binned through, or 0 where its (8x8 block, particle) entries were sorted
+   directly.  Both give the same block lists; regions are used where they
+   merge at least a third of the entries. */
+int splhip_last_binning(splhip_ctx *ctx, int64_t out[2]);
+
 /* Host-built tables (device copies are used by the kernels): tab[0..1023] =
    tab1, tab[1024..2047] = tab2; consts = {expfac, taylorlimit, h2sigma, bfak,
    rfac}.  Host only. */

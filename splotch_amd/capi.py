@@ -73,7 +73,7 @@ EXPORTS = ["splhip_create", "splhip_destroy", "splhip_last_error", "splhip_camer
            "splhip_render", "splhip_render_host", "splhip_project", "splhip_bins",
            "splhip_exptable", "splhip_set_profiling", "splhip_stage_times", "splhip_set_trace",
            "splhip_trace_items", "splhip_render_batched", "splhip_render_backward",
-           "splhip_last_tuning"]
+           "splhip_last_tuning", "splhip_last_binning"]
 
 STAGES = ("project", "ordered_sort", "ordered_bin", "ordered_splat", "additive_compact",
           "additive_bin", "additive_splat", "ordered_k_splat8", "ordered_k_splat_heavy",
@@ -230,6 +230,13 @@ class Renderer:
         counts = dict(ordered_active=int(cnt[0]), ordered_entries=int(cnt[1]),
                       additive_active=int(cnt[2]), additive_entries=int(cnt[3]))
         return dict(zip(STAGES, ms.tolist())), counts
+
+    def binning(self):
+        """Region entries each pass (ordered, additive) of the last render was
+        binned through; 0 where it sorted its block entries directly."""
+        v = np.zeros(2, np.int64)
+        self._check(lib().splhip_last_binning(self._c, _np(v)), "splhip_last_binning")
+        return int(v[0]), int(v[1])
 
     def tuning(self):
         """The splat tuning the last render used: dict variant, split,

@@ -62,6 +62,7 @@ struct splhip_ctx {
   // autotuner: fastest splat tuning per batch shape; the last render's choice
   std::map<uint64_t, std::array<uint32_t, 3>> tuned;
   uint32_t last_tuning[4] = {0, 0, 0, 0};
+  int64_t last_regions[2] = {0, 0};  // region entries binned per pass (0: per block)
   // auxiliary stream (default priority) for the heavy-block kernel
   hipStream_t aux = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -364,19 +365,23 @@ struct Plan {
   int nj = 0, G = 8;
   int64_t ntot = 0;
   uint32_t nblocks = 0, nunits = 0;
+  uint32_t nregions = 0;             // G == 8: 16x16 regions (2x2 units)
   bool present[2] = {false, false};  // some job has types of the pass
   bool sort = false;                 // some job depth-sorts its ordered types
   std::vector<ProjectArgs> pa;
-  std::vector<uint32_t> blk, pbase, ubase;
+  std::vector<uint32_t> blk, pbase, ubase, rbase;
   std::vector<BinGeom> geom;
   std::vector<JobSplat> js;
   // device tables (inside the S_JOBS slot)
   const ProjectArgs *d_pa = nullptr;
-  const uint32_t *d_blk = nullptr, *d_pbase = nullptr, *d_ubase = nullptr;
+  const uint32_t *d_blk = nullptr, *d_pbase = nullptr, *d_ubase = nullptr, *d_rbase = nullptr;
   const BinGeom *d_geom = nullptr;
   JobSplat *d_js = nullptr;
   // per pass
   uint32_t m[2] = {0, 0}, e[2] = {0, 0}, nitems[2] = {0, 0}, nheavy[2] = {0, 0};
+  uint32_t e16[2] = {0, 0};         // (16x16 region, particle) entries
+  bool r16[2] = {false, false};     // the pass's region counts are scanned (e16 valid)
+  uint64_t used16[2] = {0, 0};      // region entries of the bins built through regions
   std::vector<uint32_t> mo[2];  // host copies of the job offsets in pass order
   const uint32_t *order[2] = {nullptr, nullptr};
   const Rec *recs[2] = {nullptr, nullptr};
@@ -419,7 +424,8 @@ void make_plan(splhip_ctx *c, int nj, const JobIn *jobs, int G, Plan &pl, hipStr
   pl.blk.assign(nj + 1, 0);
   pl.pbase.assign(nj + 1, 0);
   pl.ubase.assign(nj + 1, 0);
-  uint64_t units = 0, blocks = 0;
+  pl.rbase.assign(nj + 1, 0);
+  uint64_t units = 0, blocks = 0, regions = 0;
   int64_t ntot = 0;
   for (int j = 0; j < nj; ++j) {
     const splhip_particles *P = jobs[j].P;
@@ -450,11 +456,16 @@ void make_plan(splhip_ctx *c, int nj, const JobIn *jobs, int G, Plan &pl, hipStr
     pl.pbase[j] = static_cast<uint32_t>(ntot);
     pl.blk[j] = static_cast<uint32_t>(blocks);
     pl.ubase[j] = static_cast<uint32_t>(units);
+    pl.rbase[j] = static_cast<uint32_t>(regions);
     ntot += P->n;
     blocks += (static_cast<uint64_t>(P->n) + kPerBlock - 1) / kPerBlock;
     const BinGeom g = make_geom(cam, prm, G, c->consts);
     pl.geom[j] = g;
     units += static_cast<uint64_t>(g.ncx) * g.ncy * (G > 0 ? g.ngm * g.ngm : 1);
+    if (G == 8) {
+      const uint64_t n16 = (static_cast<uint64_t>(g.ngm) + 1) / 2;
+      regions += static_cast<uint64_t>(g.ncx) * g.ncy * n16 * n16;
+    }
     JobSplat &J = pl.js[j];
     J = JobSplat{};
     J.pic = jobs[j].pic;
@@ -473,12 +484,15 @@ void make_plan(splhip_ctx *c, int nj, const JobIn *jobs, int G, Plan &pl, hipStr
   pl.blk[nj] = pl.nblocks;
   pl.pbase[nj] = static_cast<uint32_t>(ntot);
   pl.ubase[nj] = pl.nunits;
-  // one upload: [ProjectArgs][blk][pbase][ubase][BinGeom][JobSplat]
+  pl.nregions = static_cast<uint32_t>(regions);
+  pl.rbase[nj] = pl.nregions;
+  // one upload: [ProjectArgs][blk][pbase][ubase][rbase][BinGeom][JobSplat]
   const size_t o_pa = 0;
   const size_t o_blk = align256(o_pa + nj * sizeof(ProjectArgs));
   const size_t o_pb = align256(o_blk + (nj + 1) * 4);
   const size_t o_ub = align256(o_pb + (nj + 1) * 4);
-  const size_t o_g = align256(o_ub + (nj + 1) * 4);
+  const size_t o_rb = align256(o_ub + (nj + 1) * 4);
+  const size_t o_g = align256(o_rb + (nj + 1) * 4);
   const size_t o_js = align256(o_g + nj * sizeof(BinGeom));
   const size_t total = align256(o_js + nj * sizeof(JobSplat));
   char *h = staging(c, total);
@@ -486,6 +500,7 @@ void make_plan(splhip_ctx *c, int nj, const JobIn *jobs, int G, Plan &pl, hipStr
   std::memcpy(h + o_blk, pl.blk.data(), (nj + 1) * 4);
   std::memcpy(h + o_pb, pl.pbase.data(), (nj + 1) * 4);
   std::memcpy(h + o_ub, pl.ubase.data(), (nj + 1) * 4);
+  std::memcpy(h + o_rb, pl.rbase.data(), (nj + 1) * 4);
   std::memcpy(h + o_g, pl.geom.data(), nj * sizeof(BinGeom));
   std::memcpy(h + o_js, pl.js.data(), nj * sizeof(JobSplat));
   char *d = c->ws.get_as<char>(S_JOBS, total);
@@ -495,13 +510,15 @@ void make_plan(splhip_ctx *c, int nj, const JobIn *jobs, int G, Plan &pl, hipStr
   pl.d_blk = reinterpret_cast<const uint32_t *>(d + o_blk);
   pl.d_pbase = reinterpret_cast<const uint32_t *>(d + o_pb);
   pl.d_ubase = reinterpret_cast<const uint32_t *>(d + o_ub);
+  pl.d_rbase = reinterpret_cast<const uint32_t *>(d + o_rb);
   pl.d_geom = reinterpret_cast<const BinGeom *>(d + o_g);
   pl.d_js = reinterpret_cast<JobSplat *>(d + o_js);
 }
 
 // Totals slot layout (device u32): [0..1] pass sizes, [2..3] bin entries,
-// [4..5] work items, [6..7] heavy items.
-enum { T_M = 0, T_E = 2, T_ITEMS = 4, T_HEAVY = 6, T_E64 = 8, T_N = 12 };
+// [4..5] work items, [6..7] heavy items, [8..11] bin entries as u64,
+// [12..13] 16x16-region entries.
+enum { T_M = 0, T_E = 2, T_ITEMS = 4, T_HEAVY = 6, T_E64 = 8, T_E16 = 12, T_N = 14 };
 
 // Project all jobs, then the compaction scan of each present pass and its job
 // offsets in pass order; one host sync reads the pass sizes and offsets.
@@ -594,33 +611,78 @@ void stage_order(splhip_ctx *c, Plan &pl, int p, const Rec *rec, const uint8_t *
 }
 
 // Gather into render order and count each particle's units; the entry total
-// lands in tot[T_E + p].
+// lands in tot[T_E + p].  With 8 px units and fewer than 2^28 particles
+// (entry values carry a 4-bit block mask above the particle index) the
+// entries can be binned through 16x16 regions: their counts are scanned too
+// (total in tot[T_E16 + p]).
 void stage_count(splhip_ctx *c, Plan &pl, int p, const Rec *rec, uint32_t *tot, hipStream_t s) {
   Workspace &ws = c->ws;
   const uint32_t m = pl.m[p];
   mark(c, p == 0 ? 2 : 5, true, s);
+  pl.r16[p] = pl.G == 8 && m < (1u << 28);
   Rec *recs = ws.get_as<Rec>(pslot(p, P_RECS), m);
   uint32_t *cnt = ws.get_as<uint32_t>(pslot(p, P_CNT), m);
   uint32_t *off = ws.get_as<uint32_t>(pslot(p, P_OFF), m);
-  launch_gather_count(pl.order[p], m, rec, recs, cnt, pl.d_geom, pl.d_ubase, pl.d_mo[p], pl.nj, s);
+  uint32_t *cnt16 = pl.r16[p] ? ws.get_as<uint32_t>(pslot(p, P_CNT16), m) : nullptr;
+  launch_gather_count(pl.order[p], m, rec, recs, cnt, cnt16, pl.d_geom, pl.d_mo[p], pl.nj, s);
   scan_u32(cnt, off, m, tot + T_E + p, ws, s);
   launch_sum_u64(cnt, m, reinterpret_cast<uint64_t *>(tot + T_E64 + 2 * p), s);
+  if (cnt16)
+    scan_u32(cnt16, ws.get_as<uint32_t>(pslot(p, P_OFF16), m), m, tot + T_E16 + p, ws, s);
   pl.recs[p] = recs;
   mark(c, p == 0 ? 2 : 5, false, s);
 }
 
 // Emit the (unit, particle) entries, sort them by unit (stable, so each list
-// keeps render order) and build the unit offsets.
+// keeps render order) and build the unit offsets.  Through 16x16 regions:
+// emit (region, particle | block mask) entries, sort them by region, and
+// split each region's list into its 8x8 blocks' lists (same lists, same
+// order: the stable sort keeps render order in both forms).
 void stage_bins(splhip_ctx *c, Plan &pl, int p, hipStream_t s) {
   Workspace &ws = c->ws;
   const uint32_t m = pl.m[p], e = pl.e[p];
   mark(c, p == 0 ? 2 : 5, true, s);
+  // regions pay off where they merge entries (large footprints); small
+  // footprints keep one entry per block either way
+  if (pl.r16[p] && 3ull * pl.e16[p] <= 2ull * e) {
+    const uint32_t e16 = pl.e16[p], nr = pl.nregions, nu = pl.nunits;
+    const uint32_t *off16 = ws.get_as<uint32_t>(pslot(p, P_OFF16), m);
+    uint32_t *rk0 = ws.get_as<uint32_t>(pslot(p, P_EK0), e16);
+    uint32_t *rv0 = ws.get_as<uint32_t>(pslot(p, P_EV0), e16);
+    uint32_t *rk1 = ws.get_as<uint32_t>(pslot(p, P_EK1), e16);
+    uint32_t *rv1 = ws.get_as<uint32_t>(pslot(p, P_EV1), e16);
+    launch_emit(1, pl.recs[p], m, off16, e16, pl.d_geom, pl.d_rbase, pl.d_mo[p], pl.nj, rk0, rv0,
+                s);
+    const int r = radix_sort_pairs(rk0, rv0, rk1, rv1, e16, 0, ceil_log2(nr), ws, s);
+    const uint32_t *rv = r ? rv1 : rv0;
+    uint32_t *roff = ws.get_as<uint32_t>(pslot(p, P_ROFF), static_cast<size_t>(nr) + 1);
+    launch_bin_offsets(r ? rk1 : rk0, e16, nr, roff, s);
+    const size_t nc4 = 4 * static_cast<size_t>(split_chunks(e16));
+    uint32_t *ccnt = ws.get_as<uint32_t>(pslot(p, P_CCNT), nc4 + 1);
+    uint32_t *cpre = ws.get_as<uint32_t>(pslot(p, P_CPRE), nc4 + 1);
+    uint32_t *sx = ws.get_as<uint32_t>(pslot(p, P_SX), 4 * static_cast<size_t>(nr) + 4);
+    uint32_t *ucnt = ws.get_as<uint32_t>(pslot(p, P_UCNT), nu);
+    uint32_t *bin_off = ws.get_as<uint32_t>(pslot(p, P_BINOFF), static_cast<size_t>(nu) + 1);
+    uint32_t *ent = ws.get_as<uint32_t>(pslot(p, P_ENT), e);
+    launch_split_chunks(rv, e16, ccnt, s);
+    scan_u32(ccnt, cpre, static_cast<int64_t>(nc4), cpre + nc4, ws, s);
+    launch_split_bounds(roff, rv, nr, cpre, e16, pl.d_rbase, pl.d_ubase, pl.d_geom, pl.nj, sx,
+                        ucnt, s);
+    scan_u32(ucnt, bin_off, nu, bin_off + nu, ws, s);
+    launch_split_base(nr, pl.d_rbase, pl.d_ubase, pl.d_geom, pl.nj, sx, bin_off, sx, s);
+    launch_split_scatter(r ? rk1 : rk0, rv, e16, cpre, sx, ent, s);
+    pl.ent[p] = ent;
+    pl.bin_off[p] = bin_off;
+    pl.used16[p] += e16;
+    mark(c, p == 0 ? 2 : 5, false, s);
+    return;
+  }
   const uint32_t *off = ws.get_as<uint32_t>(pslot(p, P_OFF), m);
   uint32_t *ek0 = ws.get_as<uint32_t>(pslot(p, P_EK0), e);
   uint32_t *ev0 = ws.get_as<uint32_t>(pslot(p, P_EV0), e);
   uint32_t *ek1 = ws.get_as<uint32_t>(pslot(p, P_EK1), e);
   uint32_t *ev1 = ws.get_as<uint32_t>(pslot(p, P_EV1), e);
-  launch_emit(pl.recs[p], m, off, e, pl.d_geom, pl.d_ubase, pl.d_mo[p], pl.nj, ek0, ev0, s);
+  launch_emit(0, pl.recs[p], m, off, e, pl.d_geom, pl.d_ubase, pl.d_mo[p], pl.nj, ek0, ev0, s);
   const int r = radix_sort_pairs(ek0, ev0, ek1, ev1, e, 0, ceil_log2(pl.nunits), ws, s);
   uint32_t *bin_off = ws.get_as<uint32_t>(pslot(p, P_BINOFF), static_cast<size_t>(pl.nunits) + 1);
   launch_bin_offsets(r ? ek1 : ek0, e, pl.nunits, bin_off, s);
@@ -803,6 +865,8 @@ void render_chunked_pass(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, uint3
   const Rec *recs = pl.recs[p];
   uint32_t *cnt = ws.get_as<uint32_t>(pslot(p, P_CNT), m);
   uint32_t *off = ws.get_as<uint32_t>(pslot(p, P_OFF), m);
+  uint32_t *cnt16 = pl.r16[p] ? ws.get_as<uint32_t>(pslot(p, P_CNT16), m) : nullptr;
+  uint32_t *off16 = pl.r16[p] ? ws.get_as<uint32_t>(pslot(p, P_OFF16), m) : nullptr;
   const int64_t npix = static_cast<int64_t>(J.xres) * J.yres * 3;
   const bool scratch = p == 1 && saved.accumulate[1] == 1;
   float *target = scratch ? ws.get_as<float>(S_SCRATCH, static_cast<size_t>(npix)) : saved.pic;
@@ -811,13 +875,16 @@ void render_chunked_pass(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, uint3
   for (int chunk = 0; s0 < m; ++chunk) {
     uint32_t n = m - s0;
     uint32_t e = 0;
+    uint32_t e16 = 0;
     for (;;) {  // shrink the range until its entries fit a chunk
       scan_u32(cnt + s0, off, n, tot + T_E + p, ws, s);
       launch_sum_u64(cnt + s0, n, e64, s);
+      if (pl.r16[p]) scan_u32(cnt16 + s0, off16, n, tot + T_E16 + p, ws, s);
       read_totals(c, tot, T_N, s);
       uint64_t ec;
       std::memcpy(&ec, c->h_tot + T_E64 + 2 * p, 8);
       e = c->h_tot[T_E + p];
+      e16 = c->h_tot[T_E16 + p];
       if (ec <= tn.chunk || n == 1) break;
       n = static_cast<uint32_t>(std::max<uint64_t>(1, static_cast<uint64_t>(n) * tn.chunk / ec * 15 / 16));
     }
@@ -832,6 +899,7 @@ void render_chunked_pass(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, uint3
     // this range as the pass: render positions are local to the range
     pl.m[p] = n;
     pl.e[p] = e;
+    pl.e16[p] = e16;
     pl.recs[p] = recs + s0;
     if (e > 0) {
       stage_bins(c, pl, p, s);
@@ -965,6 +1033,7 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
     for (int p = 0; p < 2; ++p) {
       std::memcpy(&e64[p], c->h_tot + (T_E64 - T_E) + 2 * p, 8);
       pl.e[p] = pl.m[p] ? c->h_tot[p] : 0;
+      pl.e16[p] = pl.m[p] ? c->h_tot[(T_E16 - T_E) + p] : 0;
       chunked[p] = pl.m[p] && e64[p] > tn.chunk;
     }
     if (chunked[0] || chunked[1]) {
@@ -1069,6 +1138,8 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
   c->last_tuning[1] = tn.split;
   c->last_tuning[2] = tn.heavy == 0xffffffffu ? 0u : tn.heavy_max;
   c->last_tuning[3] = tuned ? 1u : 0u;
+  c->last_regions[0] = static_cast<int64_t>(pl.used16[0]);
+  c->last_regions[1] = static_cast<int64_t>(pl.used16[1]);
   if (keep_fwd && nj == 1 && G == 8 && !chunked[0] && !chunked[1]) {
     c->fwd = new FwdState{*jobs[0].P, *jobs[0].cam, *jobs[0].prm, pl, tn};
   }
@@ -1163,6 +1234,13 @@ int splhip_set_profiling(splhip_ctx *c, int32_t enable) {
 int splhip_last_tuning(splhip_ctx *c, int64_t out[4]) {
   if (!c || !out) return SPLHIP_E_ARG;
   for (int k = 0; k < 4; ++k) out[k] = c->last_tuning[k];
+  return 0;
+}
+
+int splhip_last_binning(splhip_ctx *c, int64_t out[2]) {
+  if (!c || !out) return SPLHIP_E_ARG;
+  out[0] = c->last_regions[0];
+  out[1] = c->last_regions[1];
   return 0;
 }
 

@@ -20,6 +20,7 @@ enum Slot : int {
 enum PassSlot : int {
   P_POS, P_MO, P_K0, P_V0, P_K1, P_V1, P_RECS, P_CNT, P_OFF, P_EK0, P_EV0, P_EK1, P_EV1,
   P_BINOFF, P_ICNT, P_IOFF, P_IK0, P_IV0, P_IK1, P_IV1, P_TRACE,
+  P_CNT16, P_OFF16, P_ROFF, P_UCNT, P_ENT, P_CCNT, P_CPRE, P_SX,
   P_NSLOTS
 };
 

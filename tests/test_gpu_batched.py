@@ -80,6 +80,7 @@ def test_batched_equals_plain(R):
     log("batched vs plain:", [(bj.field.n, bj.view.xres, bj.additive) for bj in jobs])
     R.render_batched(views)
     torch.cuda.synchronize()
+    log("region entries of the batch's passes:", R.binning())
     got = [v[3].cpu().numpy() for v in views]
     ref = _plain(R, views)
     for j, (g, r) in enumerate(zip(got, ref)):
@@ -127,6 +128,33 @@ def test_batched_mixed_edge_views(R):
     assert not np.any(got[4]), "empty view must be black"
     assert not np.any(got[5]), "fully culled view must be black"
     assert_bitexact(got[1], got[6], "repeated view")
+
+
+def test_batched_region_binning(R):
+    """Views of large-footprint particles bin through 16x16 regions, numbered
+    per view from the view's first region: every image equals its plain
+    render and, for the first view, the oracle."""
+    tiles, res = [50, 37, 24, 50], [160, 144, 176, 120]
+    jobs = []
+    for k in range(4):
+        fld = synth.plummer_field(12_000, 2500 + k, halo_frac=1.0, n_halos=2,
+                                  hsml_mu=math.log(3.0))
+        jobs.append(synth.BatchJob(fld, synth.orbit_view(0.4 * k, 0.2, 150.0, res[k]), k % 2 == 1))
+    views = _views(jobs)
+    for (dev, cam, prm, pic), T in zip(views, tiles):
+        prm.tile_size = T
+    R.render_batched(views)
+    torch.cuda.synchronize()
+    regions = R.binning()
+    log("region entries of the batch's passes:", regions)
+    assert regions[0] > 0 and regions[1] > 0
+    got = [v[3].cpu().numpy() for v in views]
+    ref = _plain(R, views)
+    for j, (g, r) in enumerate(zip(got, ref)):
+        assert_bitexact(g, r, "region job %d batched vs plain" % j)
+    pal = synth.default_palette()
+    opic, _ = O.host_rendering(jobs[0].field, jobs[0].view, [pal], a_eq_e=False, stable=True)
+    assert_bitexact(got[0].reshape(opic.shape), opic, "region job 0 vs oracle")
 
 
 @pytest.mark.parametrize("variant,split,heavy,hmax", [(4, 0, 0, 0), (1, 0, -1, 0), (3, 64, 200, 0),

@@ -300,6 +300,36 @@ def test_dense_cluster_vs_oracle(R, aeq):
                         % (G, var, split, heavy, hmax))
 
 
+@pytest.mark.parametrize("T", [50, 37, 24, 16])
+def test_region_binning_vs_oracle(R, T):
+    """Large footprints are binned through 16x16 regions (2x2 blocks, split
+    into the blocks' lists afterwards), point-like ones per block; both give
+    the oracle's image bit for bit, with tile sizes whose 8x8 grid is odd (a
+    region with blocks past the tile edge) or even."""
+    import splotch_amd as S
+    log("case region binning T", T)
+    gas = synth.plummer_field(30_000, 1520 + T, ptype=0, halo_frac=1.0, n_halos=2,
+                              hsml_mu=math.log(3.0))  # radii of ~10-30 px
+    stars = synth.plummer_field(20_000, 1540 + T, ptype=1, hsml_mu=math.log(0.01))
+    fld = synth.concat(gas, stars)
+    v = synth.default_view(200, 168)
+    pal = synth.default_palette()
+    prm = S.params(ptypes=2, a_eq_e=[0, 1], brightness=[0.5, 2.0], tile_size=T)
+    pic = render(R, fld, v, [pal, pal], prm)
+    regions = R.binning()
+    _, counts = R.stage_times()
+    log("region entries", regions, "block entries", counts["ordered_entries"],
+        counts["additive_entries"])
+    assert regions[0] > 0 and 3 * regions[0] <= 2 * counts["ordered_entries"]
+    assert regions[1] == 0
+    p = O.project_colorize(fld, v, [pal, pal], brightness=[0.5, 2.0])
+    gas = p.take(np.arange(p.n)); gas.active = (p.active * (p.type == 0)).astype(np.uint8)
+    stars = p.take(np.arange(p.n)); stars.active = (p.active * (p.type == 1)).astype(np.uint8)
+    pg = O.render(gas.take(O.sort_perm(gas, 1, stable=True)), v.xres, v.yres, False, 0.2, T)
+    ps = O.render(stars, v.xres, v.yres, True, 0.2, T)
+    assert_bitexact(pic, (pg + ps).astype(np.float32), "region binning T=%d" % T)
+
+
 @pytest.mark.parametrize("aeq", [False, True])
 def test_heavy_blocks_on_fixtures(R, golden, aeq):
     """Every non-empty block through the producer/consumer kernel."""
