@@ -268,6 +268,44 @@ int splhip_last_tuning(splhip_ctx *ctx, int64_t out[4]);
    merge at least a third of the entries. */
 int splhip_last_binning(splhip_ctx *ctx, int64_t out[2]);
 
+/* --- post-processing and 8-bit output --------------------------------------- */
+
+/* The master's epilogue after the render (splotch.cc:272-310), which the
+   reference runs on the host over the float image:
+     a_eq_e      c = -xexp.expm1(c) per channel            (splotch.cc:275-284)
+     gamma, brightness, contrast (pic_gamma 1, pic_brightness 0, pic_contrast 1):
+                 c = float(contrast * pow(double(c), gamma) + brightness), only
+                 when one differs from its default   (colour_adjust, splotchutils.cc:230-250)
+     colorbar    one 10-row bar per type t < ptypes with col_vector[t] == 0, from
+                 the image's last row up, with the palette colour at
+                 ix / double(xres)                   (add_colorbar, splotchutils.cc:35-56;
+                 rows that would fall below row 0 are skipped)
+   then the 8-bit conversion of LS_Image::put_pixel, byte = max(0, min(255,
+   int(c * 256))) with x86 truncation (ls_image.h:65-75). */
+typedef struct {
+  int32_t a_eq_e;
+  int32_t colorbar;
+  int32_t ptypes;
+  int32_t col_vector[SPLHIP_MAX_TYPES];
+  double gamma, brightness, contrast;
+} splhip_post;
+
+/* The .par keys a_eq_e, pic_gamma, pic_brightness, pic_contrast, colorbar,
+   ptypes and color_is_vector{t}, with the reference's defaults. */
+int splhip_post_from_par(const char *par_text, splhip_post *out);
+
+/* One pass over the device image pic (x-major RGB float, xres * yres * 3;
+   3 xres yres < 2^31).  pic_out (device, may equal pic, may be NULL): the
+   post-processed float image.  rgb8 (device, may be NULL): 3 xres yres bytes,
+   layout 0 = x-major RGB (LS_Image's pixel[ix][iy]), layout 1 = the body of
+   the file LS_Image::write_TGA writes (ls_image.cc:178-202: rows iy
+   ascending, pixels ix ascending, B G R), to follow its 18-byte header
+   {0,0,2, 0,0,0,0,0, 0,0,0,0, xres%256, xres/256, yres%256, yres/256, 24, 32}.
+   Colorbars use the context's palettes (splhip_set_palette).  Stream-ordered. */
+int splhip_postprocess(splhip_ctx *ctx, const float *pic, int32_t xres, int32_t yres,
+                       const splhip_post *post, float *pic_out, uint8_t *rgb8, int32_t layout,
+                       void *stream);
+
 /* Host-built tables (device copies are used by the kernels): tab[0..1023] =
    tab1, tab[1024..2047] = tab2; consts = {expfac, taylorlimit, h2sigma, bfak,
    rfac}.  Host only. */

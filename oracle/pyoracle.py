@@ -303,6 +303,44 @@ def host_rendering(field, view, palettes, a_eq_e=True, sort_type=1, grayabsorb=0
     return render(p, view.xres, view.yres, a_eq_e, grayabsorb, tile), p
 
 
+def postprocess(pic, palettes=(), a_eq_e=True, gamma=1.0, brightness=0.0, contrast=1.0,
+                colorbar=False, col_vector=None):
+    """Oracle restatement of the post-processing and TGA output of
+    splotch.cc:272-310 (orc_postprocess).  pic: raw (xres, yres, 3) float32
+    image.  Returns (post-processed float image, TGA file bytes)."""
+    out = np.ascontiguousarray(pic, np.float32).copy()
+    xres, yres = out.shape[:2]
+    nt = len(palettes)
+    cv = np.zeros(8, np.int32) if col_vector is None else np.asarray(
+        list(col_vector) + [0] * (8 - len(col_vector)), np.int32)
+    if nt:
+        pn, stride, px, prgb = _pal_arrays(palettes)
+    else:
+        pn, stride, px, prgb = np.zeros(1, np.int32), 1, np.zeros(1), np.zeros(3, np.float32)
+    tga = np.zeros(18 + 3 * xres * yres, np.uint8)
+    lib().orc_postprocess(C.c_int(xres), C.c_int(yres), _p(out), C.c_int(int(a_eq_e)),
+                          C.c_double(gamma), C.c_double(brightness), C.c_double(contrast),
+                          C.c_int(int(colorbar)), C.c_int(nt), _p(cv), _p(pn), C.c_int(stride),
+                          _p(px), _p(prgb), _p(tga))
+    return out, tga
+
+
+def ref_postprocess(pic, palettes, params: dict, tga_path):
+    """The genuine post-processing (ref_postprocess): params are .par keys
+    (a_eq_e, pic_gamma, pic_brightness, pic_contrast, colorbar, ptypes,
+    color_is_vector{t}).  Returns (float image, TGA file bytes)."""
+    out = np.ascontiguousarray(pic, np.float32).copy()
+    xres, yres = out.shape[:2]
+    pn, stride, px, prgb = _pal_arrays(palettes)
+    tga = np.zeros(18 + 3 * xres * yres, np.uint8)
+    rc = ref().ref_postprocess(kv_text(params).encode(), C.c_int(xres), C.c_int(yres), _p(out),
+                               C.c_int(len(palettes)), _p(pn), C.c_int(stride), _p(px),
+                               _p(prgb), str(tga_path).encode(), _p(tga))
+    if rc != 0:
+        raise RuntimeError("reference post-processing failed")
+    return out, tga
+
+
 def render64(p: Projected, s, xres, yres, a_eq_e=True, grayabsorb=0.2, tile=50):
     """FP64 twin on the frozen float32 footprint; s: dict sx, sy, r, e0, e1, e2 (float64)."""
     pic = np.zeros((xres, yres, 3), np.float64)

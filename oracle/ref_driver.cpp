@@ -11,6 +11,8 @@
  *   host_funct::render_new         splotch/splotch_host.cc:330-697
  *   host_rendering                 splotch/splotch_host.cc:711-834
  *   exptable<float32>              splotch/splotchutils.h:174-215
+ *   colour_adjust, add_colorbar    splotch/splotchutils.cc:35-56,230-250
+ *   LS_Image::put_pixel/write_TGA  cxxsupport/ls_image.h:65-90,134-139, ls_image.cc:178-202
  * It is used to write the golden fixtures under tests/golden/ (in the build
  * container) and as the "reference" CPU baseline in bench.py.
  */
@@ -26,6 +28,11 @@
 #include "cxxsupport/paramfile.h"
 #include "cxxsupport/walltimer.h"
 #include "cxxsupport/lsconstants.h"
+#include "cxxsupport/ls_image.h"
+
+#include <cstdio>
+#include <fstream>
+#include <iterator>
 
 namespace host_funct {
 // Definitions in splotch_host.cc (the header declares different signatures).
@@ -233,6 +240,61 @@ int ref_host_rendering(const char *kv, int64_t n, const float *x, const float *y
         float *d = &pic[(size_t(ix) * yres + iy) * 3];
         d[0] = rc.pic[ix][iy].r; d[1] = rc.pic[ix][iy].g; d[2] = rc.pic[ix][iy].b;
       }
+  } catch (...) {
+    return -1;
+  }
+  return 0;
+}
+
+/* The master's post-processing of splotch.cc:272-284 and the image output of
+   splotch.cc:298-310, over a raw x-major RGB image (updated in place): for
+   a_eq_e (the .par key) -xexp.expm1 per channel (the three lines of
+   splotch.cc:275-284, restated here over the genuine exptable since splotch.cc
+   holds main()), then the genuine colour_adjust and, with the colorbar key,
+   add_colorbar; then the genuine LS_Image::put_pixel quantization and
+   write_TGA to tga_path.  tga_out (18 + 3 xres yres bytes) receives the file.
+   Returns 0/-1. */
+int ref_postprocess(const char *kv, int xres, int yres, float *pic, int ntypes,
+                    const int *pal_n, int pal_stride, const double *pal_x,
+                    const float *pal_rgb, const char *tga_path, uint8_t *tga_out) {
+  try {
+    paramfile params(parse_kv(kv), false);
+    render_context rc;
+    build_maps(ntypes, pal_n, pal_stride, pal_x, pal_rgb, rc.amap);
+    rc.pic.alloc(xres, yres);
+    rc.xres = xres;
+    rc.yres = yres;
+    for (int ix = 0; ix < xres; ++ix)
+      for (int iy = 0; iy < yres; ++iy) {
+        const float *d = &pic[(size_t(ix) * yres + iy) * 3];
+        rc.pic[ix][iy] = COLOUR(d[0], d[1], d[2]);
+      }
+    if (params.find<bool>("a_eq_e", true)) {
+      exptable<float32> xexp(-20.0);
+      for (int ix = 0; ix < xres; ++ix)
+        for (int iy = 0; iy < yres; ++iy) {
+          COLOUR &c = rc.pic[ix][iy];
+          c.r = -xexp.expm1(c.r);
+          c.g = -xexp.expm1(c.g);
+          c.b = -xexp.expm1(c.b);
+        }
+    }
+    colour_adjust(params, rc);
+    if (params.find<bool>("colorbar", false)) add_colorbar(params, rc.pic, rc.amap);
+    LS_Image img(xres, yres);
+    for (int ix = 0; ix < xres; ++ix)
+      for (int iy = 0; iy < yres; ++iy) {
+        const COLOUR &c = rc.pic[ix][iy];
+        img.put_pixel(ix, iy, Colour(c.r, c.g, c.b));
+        float *d = &pic[(size_t(ix) * yres + iy) * 3];
+        d[0] = c.r; d[1] = c.g; d[2] = c.b;
+      }
+    img.write_TGA(tga_path);
+    std::ifstream in(tga_path, std::ios::binary);
+    std::vector<char> bytes((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    if (bytes.size() != 18 + size_t(3) * xres * yres) return -1;
+    std::memcpy(tga_out, bytes.data(), bytes.size());
+    std::remove(tga_path);
   } catch (...) {
     return -1;
   }

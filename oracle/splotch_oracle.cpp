@@ -25,6 +25,10 @@
  *                     single-particle and two-particle closed forms
  *   orc_render64      FP64 twin, checked against orc_render within the exptable
  *                     bound and by additivity
+ *   orc_postprocess   genuine-build fixtures (exptable -expm1, colour_adjust,
+ *                     add_colorbar, LS_Image quantization and the TGA file, bit
+ *                     for bit); closed forms (bar rows = palette nodes, 8-bit
+ *                     code of k/256 is k, clamps, x86 conversion of NaN/inf)
  *   orc_adjoint64     FP64 adjoint; with pyoracle.chain64 (the FP64 chain rule
  *                     through pyoracle.project64), checked against FP64 central
  *                     differences (h = 1e-4) of orc_render64 after project64 on
@@ -74,6 +78,27 @@ struct ExpTable {
 const ExpTable &xexp() {
   static ExpTable t(-20.f);
   return t;
+}
+
+// linearMap<COLOUR>::getVal_const, cxxsupport/linear_map.h:71-84 with
+// interpol_helper, cxxsupport/math_utils.h:186-202: px (double, ascending)
+// and py (float RGB) hold np nodes.
+void pal_val(const double *px, const float *py, int np, double xv, float out[3]) {
+  if (np == 1) {
+    out[0] = py[0]; out[1] = py[1]; out[2] = py[2];
+  } else if (xv >= px[np - 1]) {
+    out[0] = py[3 * (np - 1)]; out[1] = py[3 * (np - 1) + 1]; out[2] = py[3 * (np - 1) + 2];
+  } else if (xv <= px[0]) {
+    out[0] = py[0]; out[1] = py[1]; out[2] = py[2];
+  } else {
+    int64_t idx = std::lower_bound(px, px + np, xv) - px;
+    if (idx > 0) --idx;
+    idx = std::min<int64_t>(np - 2, idx);
+    const double frac = (xv - px[idx]) / (px[idx + 1] - px[idx]);
+    const float *y0 = py + 3 * idx, *y1 = py + 3 * (idx + 1);
+    // (1.-frac)*y[index] + frac*y[index+1], each product rounded to float
+    for (int c = 0; c < 3; ++c) out[c] = float((1. - frac) * y0[c]) + float(frac * y1[c]);
+  }
 }
 
 struct V3 { double x, y, z; };
@@ -283,29 +308,10 @@ void orc_colorize(int64_t n, float *e0, float *e1, float *e2, const float *I,
     if (!active[m]) continue;
     int t = type ? type[m] : 0;
     if (!col_vector[t]) {
-      const double *px = pal_x + (int64_t)t * pal_stride;
-      const float *py = pal_rgb + (int64_t)t * pal_stride * 3;
-      int np = pal_n[t];
-      double xv = e0[m];
-      float cr, cg, cb;
-      if (np == 1) {
-        cr = py[0]; cg = py[1]; cb = py[2];
-      } else if (xv >= px[np - 1]) {
-        cr = py[3 * (np - 1)]; cg = py[3 * (np - 1) + 1]; cb = py[3 * (np - 1) + 2];
-      } else if (xv <= px[0]) {
-        cr = py[0]; cg = py[1]; cb = py[2];
-      } else {
-        int64_t idx = std::lower_bound(px, px + np, xv) - px;
-        if (idx > 0) --idx;
-        idx = std::min<int64_t>(np - 2, idx);
-        double frac = (xv - px[idx]) / (px[idx + 1] - px[idx]);
-        const float *y0 = py + 3 * idx, *y1 = py + 3 * (idx + 1);
-        // (1.-frac)*y[index] + frac*y[index+1], each product rounded to float
-        cr = float((1. - frac) * y0[0]) + float(frac * y1[0]);
-        cg = float((1. - frac) * y0[1]) + float(frac * y1[1]);
-        cb = float((1. - frac) * y0[2]) + float(frac * y1[2]);
-      }
-      e0[m] = cr; e1[m] = cg; e2[m] = cb;
+      float e[3];
+      pal_val(pal_x + (int64_t)t * pal_stride, pal_rgb + (int64_t)t * pal_stride * 3, pal_n[t],
+              double(e0[m]), e);
+      e0[m] = e[0]; e1[m] = e[1]; e2[m] = e[2];
     }
     float fac = I[m] * brightness[t];
     e0[m] *= fac; e1[m] *= fac; e2[m] *= fac;
@@ -531,6 +537,69 @@ void orc_render_bruteforce(int64_t n, const float *px, const float *py, const fl
       d[0] = acc[0]; d[1] = acc[1]; d[2] = acc[2];
     }
   }
+}
+
+/* The master's post-processing and image output, splotch.cc:272-310, over a
+   raw x-major RGB image pic[(ix*yres + iy)*3 + c] (updated in place):
+     a_eq_e:   c = -xexp.expm1(c)                      splotch.cc:275-284
+     colour_adjust (only if gamma != 1, brightness != 0 or contrast != 1):
+               c = float(contrast * pow(double(c), gamma) + brightness)
+                                                       splotchutils.cc:230-250
+     colorbar: for each type t < ptypes with col_vector[t] == 0 (bar k = the
+               k-th such type), rows iy = yres-1-10k-y, y < 10, take the
+               palette colour at ix/double(xres)  splotchutils.cc:35-56
+               (rows below 0 are skipped; the reference writes out of bounds)
+   then the Colour8 conversion of LS_Image::put_pixel, ls_image.h:65-75:
+     byte = max(0, min(255, int(c * 256))) with x86 truncation (NaN and values
+     outside the int range give INT_MIN, i.e. 0),
+   and the file of LS_Image::write_TGA, ls_image.cc:178-202: an 18-byte header
+   then rows iy = 0..yres-1 of pixels ix = 0..xres-1 as B, G, R.
+   tga: 18 + 3*xres*yres bytes (may be NULL). */
+static int x86_trunc(float v) {
+  if (!(v >= -2147483648.f && v < 2147483648.f)) return INT32_MIN;
+  return int(v);
+}
+
+void orc_postprocess(int xres, int yres, float *pic, int a_eq_e, double gamma, double brightness,
+                     double contrast, int colorbar, int ptypes, const int *col_vector,
+                     const int *pal_n, int pal_stride, const double *pal_x,
+                     const float *pal_rgb, uint8_t *tga) {
+  const int64_t npix = int64_t(xres) * yres;
+  const ExpTable &X = xexp();
+  if (a_eq_e)
+    for (int64_t i = 0; i < 3 * npix; ++i) pic[i] = -X.expm1(pic[i]);
+  if (gamma != 1.0 || brightness != 0.0 || contrast != 1.0)
+    for (int64_t i = 0; i < 3 * npix; ++i)
+      pic[i] = float(contrast * std::pow(double(pic[i]), gamma) + brightness);
+  if (colorbar) {
+    int offset = 0;
+    for (int t = 0; t < ptypes; ++t) {
+      if (col_vector[t]) continue;
+      for (int ix = 0; ix < xres; ++ix) {
+        float e[3];
+        pal_val(pal_x + int64_t(t) * pal_stride, pal_rgb + int64_t(t) * pal_stride * 3, pal_n[t],
+                ix / double(xres), e);
+        for (int y = 0; y < 10; ++y) {
+          const int iy = yres - offset - 1 - y;
+          if (iy < 0) continue;
+          float *d = pic + (int64_t(ix) * yres + iy) * 3;
+          d[0] = e[0]; d[1] = e[1]; d[2] = e[2];
+        }
+      }
+      offset += 10;
+    }
+  }
+  if (!tga) return;
+  const uint8_t hdr[18] = {0, 0, 2, 0, 0, 0, 0, 0, 0, 0, 0, 0, uint8_t(xres % 256),
+                           uint8_t(xres / 256), uint8_t(yres % 256), uint8_t(yres / 256), 24, 32};
+  std::memcpy(tga, hdr, 18);
+  for (int iy = 0; iy < yres; ++iy)
+    for (int ix = 0; ix < xres; ++ix) {
+      const float *d = pic + (int64_t(ix) * yres + iy) * 3;
+      uint8_t *o = tga + 18 + (int64_t(iy) * xres + ix) * 3;
+      for (int c = 0; c < 3; ++c)
+        o[2 - c] = uint8_t(std::max(0, std::min(255, x86_trunc(d[c] * 256))));
+    }
 }
 
 /* ---------------------------------------------------------------------------

@@ -8,8 +8,10 @@ build (build.py), the ctypes binding (capi.py) and the seeded synthetic
 workloads (synth.py).
 """
 from . import synth  # noqa: F401
-from .capi import (Camera, Params, Renderer, SplhipError, camera_setup, exptable,  # noqa: F401
-                   params, params_from_par, to_device, to_particle_sim)
+from .capi import (Camera, Params, Post, Renderer, SplhipError, camera_setup,  # noqa: F401
+                   exptable, params, params_from_par, post, post_from_par, tga_header,
+                   to_device, to_particle_sim)
 
-__all__ = ["Camera", "Params", "Renderer", "SplhipError", "camera_setup", "exptable", "params",
-           "params_from_par", "synth", "to_device", "to_particle_sim"]
+__all__ = ["Camera", "Params", "Post", "Renderer", "SplhipError", "camera_setup", "exptable",
+           "params", "params_from_par", "post", "post_from_par", "synth", "tga_header",
+           "to_device", "to_particle_sim"]

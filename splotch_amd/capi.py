@@ -45,6 +45,13 @@ class Params(C.Structure):
                 ("hip_autotune", C.c_int32)]
 
 
+class Post(C.Structure):
+    """splhip_post: the epilogue switches (splotch.cc:272-310)."""
+    _fields_ = [("a_eq_e", C.c_int32), ("colorbar", C.c_int32), ("ptypes", C.c_int32),
+                ("col_vector", C.c_int32 * MAX_TYPES), ("gamma", C.c_double),
+                ("brightness", C.c_double), ("contrast", C.c_double)]
+
+
 class Particles(C.Structure):
     _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("hsml", C.c_void_p),
                 ("inten", C.c_void_p), ("col", C.c_void_p), ("c1", C.c_void_p),
@@ -73,7 +80,8 @@ EXPORTS = ["splhip_create", "splhip_destroy", "splhip_last_error", "splhip_camer
            "splhip_render", "splhip_render_host", "splhip_project", "splhip_bins",
            "splhip_exptable", "splhip_set_profiling", "splhip_stage_times", "splhip_set_trace",
            "splhip_trace_items", "splhip_render_batched", "splhip_render_backward",
-           "splhip_last_tuning", "splhip_last_binning"]
+           "splhip_last_tuning", "splhip_last_binning", "splhip_post_from_par",
+           "splhip_postprocess"]
 
 STAGES = ("project", "ordered_sort", "ordered_bin", "ordered_splat", "additive_compact",
           "additive_bin", "additive_splat", "ordered_k_splat8", "ordered_k_splat_heavy",
@@ -193,6 +201,33 @@ def params_from_par(text: str, b_brightness: float = 1.0):
     return p, scal
 
 
+def post(a_eq_e=True, gamma=1.0, brightness=0.0, contrast=1.0, colorbar=False, ptypes=1,
+         col_vector=None) -> Post:
+    """splhip_post from keyword values (the .par defaults)."""
+    q = Post()
+    q.a_eq_e = int(bool(a_eq_e))
+    q.gamma, q.brightness, q.contrast = float(gamma), float(brightness), float(contrast)
+    q.colorbar = int(bool(colorbar))
+    q.ptypes = int(ptypes)
+    for t, v in enumerate(col_vector or []):
+        q.col_vector[t] = int(v)
+    return q
+
+
+def post_from_par(text: str) -> Post:
+    """The post-processing keys of a .par text (splhip_post_from_par)."""
+    q = Post()
+    if lib().splhip_post_from_par(text.encode(), C.byref(q)) != 0:
+        raise SplhipError("bad parameter text")
+    return q
+
+
+def tga_header(xres: int, yres: int) -> bytes:
+    """The 18-byte header LS_Image::write_TGA writes (ls_image.cc:183-186)."""
+    return bytes([0, 0, 2, 0, 0, 0, 0, 0, 0, 0, 0, 0, xres % 256, xres // 256, yres % 256,
+                  yres // 256, 24, 32])
+
+
 class Renderer:
     """One context on one device (cuda_rendercontext_init + cuda_renderer_init)."""
 
@@ -294,6 +329,22 @@ class Renderer:
                                         C.c_void_p(pic.data_ptr()), self._stream(stream)),
                     "splhip_render")
         return pic
+
+    def postprocess(self, pic, xres: int, yres: int, q: Post, out=None, rgb8=None,
+                    layout: int = 1, stream=None):
+        """The epilogue over a device image: out (float32 tensor, may be pic)
+        and/or rgb8 (uint8 tensor of 3*xres*yres; layout 1 = TGA body,
+        0 = x-major RGB).  Returns (out, rgb8)."""
+        n = xres * yres * 3
+        assert pic.is_contiguous() and pic.numel() == n
+        for t in (out, rgb8):
+            assert t is None or (t.is_contiguous() and t.numel() == n)
+        self._check(lib().splhip_postprocess(
+            self._c, C.c_void_p(pic.data_ptr()), C.c_int32(xres), C.c_int32(yres), C.byref(q),
+            C.c_void_p(out.data_ptr() if out is not None else None),
+            C.c_void_p(rgb8.data_ptr() if rgb8 is not None else None), C.c_int32(layout),
+            self._stream(stream)), "splhip_postprocess")
+        return out, rgb8
 
     def render_batched(self, views, stream=None):
         """views: list of (dev, cam, prm, pic) as for render(); all images in

@@ -1420,6 +1420,73 @@ int splhip_params_from_par(const char *par_text, float b_brightness, splhip_para
   return 0;
 }
 
+int splhip_post_from_par(const char *par_text, splhip_post *out) {
+  if (!out) return SPLHIP_E_ARG;
+  try {
+    const auto d = parse_par(par_text);
+    std::memset(out, 0, sizeof(*out));
+    // defaults: splotch.cc:186 (a_eq_e), :307 (colorbar), splotchutils.cc:232-234
+    out->a_eq_e = par_bool(d, "a_eq_e", true) ? 1 : 0;
+    out->gamma = par_num<double>(d, "pic_gamma", 1.0);
+    out->brightness = par_num<double>(d, "pic_brightness", 0.0);
+    out->contrast = par_num<double>(d, "pic_contrast", 1.0);
+    out->colorbar = par_bool(d, "colorbar", false) ? 1 : 0;
+    const int nt = par_num<int>(d, "ptypes", 1);
+    if (nt < 1 || nt > kMaxTypes) return SPLHIP_E_ARG;
+    out->ptypes = nt;
+    for (int t = 0; t < nt; ++t)
+      out->col_vector[t] = par_bool(d, "color_is_vector" + std::to_string(t), false) ? 1 : 0;
+  } catch (const ParError &) {
+    return SPLHIP_E_ARG;
+  }
+  return 0;
+}
+
+int splhip_postprocess(splhip_ctx *c, const float *pic, int32_t xres, int32_t yres,
+                       const splhip_post *post, float *pic_out, uint8_t *rgb8, int32_t layout,
+                       void *stream) {
+  if (!c || !pic || !post) return fail(c, SPLHIP_E_ARG, "null argument");
+  if (xres < 1 || yres < 1) return fail(c, SPLHIP_E_ARG, "bad image size");
+  if (static_cast<int64_t>(xres) * yres * 3 >= (int64_t(1) << 31))
+    return fail(c, SPLHIP_E_CAPACITY, "image too large for one post-processing pass");
+  if (layout != 0 && layout != 1) return fail(c, SPLHIP_E_ARG, "layout must be 0 or 1");
+  if (!pic_out && !rgb8) return fail(c, SPLHIP_E_ARG, "no output");
+  if (post->ptypes < 1 || post->ptypes > kMaxTypes) return fail(c, SPLHIP_E_ARG, "bad ptypes");
+  try {
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    PostArgs a{};
+    a.pic = pic;
+    a.out = pic_out;
+    a.rgb8 = rgb8;
+    a.xres = xres;
+    a.yres = yres;
+    a.a_eq_e = post->a_eq_e ? 1 : 0;
+    a.adjust = (post->gamma != 1.0 || post->brightness != 0.0 || post->contrast != 1.0) ? 1 : 0;
+    a.gamma = post->gamma;
+    a.brightness = post->brightness;
+    a.contrast = post->contrast;
+    if (post->colorbar)
+      for (int t = 0; t < post->ptypes; ++t) {
+        if (post->col_vector[t]) continue;
+        if (c->pal_x[t].empty()) return fail(c, SPLHIP_E_ARG, "colorbar of a type without palette");
+        a.bar_type[a.nbars++] = t;
+      }
+    if (a.nbars) upload_palettes(c);
+    for (int t = 0; t < kMaxTypes; ++t) a.pal_n[t] = static_cast<int32_t>(c->pal_x[t].size());
+    a.pal_stride = c->pal_stride;
+    a.pal_x = c->d_palx;
+    a.pal_rgb = c->d_palrgb;
+    a.exptab = c->d_exptab;
+    a.expfac = c->consts.expfac;
+    a.taylorlimit = c->consts.taylorlimit;
+    launch_post(a, layout, s);
+  } catch (const HipError &e) {
+    return hip_fail(c, e);
+  }
+  return 0;
+}
+
 int splhip_set_palette(splhip_ctx *c, int32_t type, int32_t n, const double *x,
                        const float *rgb) {
   if (!c) return SPLHIP_E_ARG;

@@ -35,6 +35,42 @@ __device__ __forceinline__ int cvt_i32(float f) {
   return (f >= -2147483648.0f && f < 2147483648.0f) ? static_cast<int>(f) : INT_MIN;
 }
 
+// linearMap<COLOUR>::getVal_const (linear_map.h:71-84) with interpol_helper
+// (math_utils.h:186-202): X (double, ascending) and Y (float RGB) hold np
+// nodes; the weights are double, each product is rounded to float.
+__device__ __forceinline__ void palette_val(const double *X, const float *Y, int np, double xv,
+                                            float &cr, float &cg, float &cb) {
+  if (np == 1) {
+    cr = Y[0]; cg = Y[1]; cb = Y[2];
+  } else if (xv >= X[np - 1]) {
+    cr = Y[3 * (np - 1)]; cg = Y[3 * (np - 1) + 1]; cb = Y[3 * (np - 1) + 2];
+  } else if (xv <= X[0]) {
+    cr = Y[0]; cg = Y[1]; cb = Y[2];
+  } else {
+    int lo = 0, len = np;
+    while (len > 0) {  // lower_bound
+      const int half = len >> 1;
+      if (X[lo + half] < xv) {
+        lo += half + 1;
+        len -= half + 1;
+      } else {
+        len = half;
+      }
+    }
+    int idx = lo;
+    if (idx > 0) --idx;
+    idx = min(np - 2, idx);
+    const double frac = (xv - X[idx]) / (X[idx + 1] - X[idx]);
+    const float *y0 = Y + 3 * idx, *y1 = Y + 3 * (idx + 1);
+    cr = static_cast<float>((1. - frac) * static_cast<double>(y0[0])) +
+         static_cast<float>(frac * static_cast<double>(y1[0]));
+    cg = static_cast<float>((1. - frac) * static_cast<double>(y0[1])) +
+         static_cast<float>(frac * static_cast<double>(y1[1]));
+    cb = static_cast<float>((1. - frac) * static_cast<double>(y0[2])) +
+         static_cast<float>(frac * static_cast<double>(y1[2]));
+  }
+}
+
 // exptable<float32>::operator() (splotchutils.h:203-209); tab = tab1 | tab2.
 __device__ __forceinline__ float xexp_op(const float *tab, float arg, float expfac) {
   const int iarg = cvt_i32(arg * expfac);

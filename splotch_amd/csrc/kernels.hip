@@ -120,38 +120,8 @@ __global__ __launch_bounds__(256) void k_project(const ProjectArgs *__restrict__
         if (!a.col_vector[tt]) {
           const double *X = a.pal_x + static_cast<int64_t>(tt) * a.pal_stride;
           const float *Y = a.pal_rgb + static_cast<int64_t>(tt) * a.pal_stride * 3;
-          const int np = a.pal_n[tt];
-          const double xv = e0;
           float cr, cg, cb;
-          if (np == 1) {
-            cr = Y[0]; cg = Y[1]; cb = Y[2];
-          } else if (xv >= X[np - 1]) {
-            cr = Y[3 * (np - 1)]; cg = Y[3 * (np - 1) + 1]; cb = Y[3 * (np - 1) + 2];
-          } else if (xv <= X[0]) {
-            cr = Y[0]; cg = Y[1]; cb = Y[2];
-          } else {
-            int lo = 0, len = np;
-            while (len > 0) {  // lower_bound
-              const int half = len >> 1;
-              if (X[lo + half] < xv) {
-                lo += half + 1;
-                len -= half + 1;
-              } else {
-                len = half;
-              }
-            }
-            int idx = lo;
-            if (idx > 0) --idx;
-            idx = min(np - 2, idx);
-            const double frac = (xv - X[idx]) / (X[idx + 1] - X[idx]);
-            const float *y0 = Y + 3 * idx, *y1 = Y + 3 * (idx + 1);
-            cr = static_cast<float>((1. - frac) * static_cast<double>(y0[0])) +
-                 static_cast<float>(frac * static_cast<double>(y1[0]));
-            cg = static_cast<float>((1. - frac) * static_cast<double>(y0[1])) +
-                 static_cast<float>(frac * static_cast<double>(y1[1]));
-            cb = static_cast<float>((1. - frac) * static_cast<double>(y0[2])) +
-                 static_cast<float>(frac * static_cast<double>(y1[2]));
-          }
+          palette_val(X, Y, a.pal_n[tt], static_cast<double>(e0), cr, cg, cb);
           e0 = cr; e1 = cg; e2 = cb;
         }
         const float fac = II * a.brightness[tt];

@@ -150,4 +150,25 @@ void launch_zero_f64(double *p, int64_t n, hipStream_t s);
 // G: region size (16, 32 or 64 px); additive: a_eq_e mode.  Single job.
 void launch_splat(int G, bool additive, uint32_t nbins, const SplatArgs &a, hipStream_t s);
 
+// Post-processing of a frame (post.hip): -expm1 when a_eq_e, colour_adjust
+// when adjust, nbars colorbars of types bar_type[], then Colour8 bytes.
+struct PostArgs {
+  const float *pic;  // x-major RGB image
+  float *out;        // post-processed float image (may be pic, or null)
+  uint8_t *rgb8;     // bytes (null: none)
+  int32_t xres, yres;
+  int32_t a_eq_e, adjust, nbars;
+  int32_t bar_type[kMaxTypes];
+  int32_t pal_n[kMaxTypes];
+  int32_t pal_stride;
+  double gamma, brightness, contrast;
+  const double *pal_x;
+  const float *pal_rgb;
+  const float *exptab;
+  float expfac, taylorlimit;
+};
+// layout 0: bytes in the image's x-major RGB order; 1: the TGA file body
+// (rows iy ascending, B G R).
+void launch_post(const PostArgs &a, int layout, hipStream_t s);
+
 }  // namespace splhip

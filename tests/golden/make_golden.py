@@ -10,10 +10,13 @@ returns:
   * particle_project + particle_colorize outputs (splotch_host.cc:58-287);
   * particle_sort permutations, all sort types (splotch_host.cc:289-321);
   * render_new raw images (splotch_host.cc:330-697) in both modes, several tiles;
-  * one full host_rendering() call (splotch_host.cc:711-834).
+  * one full host_rendering() call (splotch_host.cc:711-834);
+  * the master's post-processing and TGA output (splotch.cc:272-310):
+    -expm1, colour_adjust, add_colorbar, Colour8 quantization, write_TGA.
 Fixtures are inputs + outputs only (numpy .npz, no pickles).
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py                 # every fixture
+    python tests/golden/make_golden.py postprocess     # only postprocess.npz
 """
 from __future__ import annotations
 
@@ -181,12 +184,62 @@ def main():
     save("host_rendering", **field_dict(f), **view_dict(v), pic=pic)
     manifest["host_rendering"] = "host_rendering(): 10k, 120x100, ordered, T=32, g=0.3"
 
+    postprocess_fixture(manifest)
+
     # palette used by every fixture (this project's own ramp)
     np.savez_compressed(os.path.join(HERE, "palette.npz"), x=pal[0], rgb=pal[1])
+    write_manifest(manifest)
+
+
+def write_manifest(manifest):
     with open(os.path.join(HERE, "MANIFEST.json"), "w") as fh:
         json.dump(manifest, fh, indent=1, sort_keys=True)
     print("wrote", len(manifest), "fixtures")
 
 
+def postprocess_fixture(manifest):
+    """Raw images (genuine render_new outputs and an image of edge values)
+    through the genuine post-processing: case k stores pp{k}_in, pp{k}_kv (the
+    .par keys), pp{k}_out (float image after -expm1 / colour_adjust /
+    colorbar) and pp{k}_tga (the TGA file)."""
+    import tempfile
+    pal = synth.default_palette()
+    pal3 = (np.array([0.0, 0.3, 1.0]),
+            np.array([[0.1, 0.2, 0.3], [0.9, 0.5, 0.1], [1.0, 1.0, 0.8]], np.float32))
+    add = np.load(os.path.join(HERE, "additive_small.npz"))["pic_T50"]
+    ordp = np.load(os.path.join(HERE, "ordered_small.npz"))["pic_stable_T50"]
+    rng = np.random.default_rng(31)
+    edge = rng.normal(0.4, 0.6, (37, 53, 3)).astype(np.float32)
+    edge.ravel()[:12] = [np.nan, np.inf, -np.inf, 1e12, -1e12, -0.0, 0.0, 1.0, 255 / 256,
+                         256 / 256, 1 / 256, -1 / 256]
+    cases = [
+        (add, dict()),
+        (add, dict(pic_gamma=0.8, pic_brightness=0.02, pic_contrast=1.1, colorbar="true",
+                   ptypes=2)),
+        (ordp, dict(a_eq_e="false", pic_contrast=0.9, colorbar="true", ptypes=3,
+                    color_is_vector1="true")),
+        (edge, dict(a_eq_e="false", pic_gamma=0.7, pic_brightness=0.05, pic_contrast=1.3)),
+        (edge, dict(a_eq_e="true", colorbar="true", ptypes=1)),
+    ]
+    out = dict(pal3_x=pal3[0], pal3_rgb=pal3[1], ncases=np.array([len(cases)]))
+    with tempfile.TemporaryDirectory() as d:
+        for k, (pic, kv) in enumerate(cases):
+            pals = [pal, pal3, pal3][:int(kv.get("ptypes", 1))]
+            o, tga = O.ref_postprocess(pic, pals, kv, os.path.join(d, "pp.tga"))
+            out["pp%d_in" % k] = np.ascontiguousarray(pic, np.float32)
+            out["pp%d_kv" % k] = np.array(O.kv_text(kv))
+            out["pp%d_out" % k] = o
+            out["pp%d_tga" % k] = tga
+    save("postprocess", **out)
+    manifest["postprocess"] = ("post-processing + TGA output of %d raw images (defaults, "
+                               "gamma/brightness/contrast, colorbars, edge values)" % len(cases))
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["postprocess"]:
+        with open(os.path.join(HERE, "MANIFEST.json")) as fh:
+            man = json.load(fh)
+        postprocess_fixture(man)
+        write_manifest(man)
+    else:
+        main()

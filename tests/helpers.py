@@ -43,3 +43,31 @@ def max_rel_err(a, b, floor_frac=1e-6):
     b = np.asarray(b, np.float64)
     den = np.maximum(np.abs(b), floor_frac * max(np.abs(b).max(), 1e-30))
     return float((np.abs(a - b) / den).max())
+
+
+def parse_kv(text):
+    """'key=value' lines (a .par excerpt) -> dict of strings."""
+    return dict(line.split("=", 1) for line in str(text).splitlines() if "=" in line)
+
+
+def post_args(kv):
+    """The post-processing switches of a .par excerpt, as the oracle takes them
+    (defaults of splotch.cc:186,307 and splotchutils.cc:232-234)."""
+    truth = ("t", "y", "true", ".true.")  # stringToData<bool>, string_utils.cc:131-140
+    ptypes = int(kv.get("ptypes", 1))
+    return dict(a_eq_e=kv.get("a_eq_e", "true").lower() in truth,
+                gamma=float(kv.get("pic_gamma", 1.0)),
+                brightness=float(kv.get("pic_brightness", 0.0)),
+                contrast=float(kv.get("pic_contrast", 1.0)),
+                colorbar=kv.get("colorbar", "false").lower() in truth,
+                col_vector=[int(kv.get("color_is_vector%d" % t, "false").lower() in truth)
+                            for t in range(ptypes)])
+
+
+def pp_case(f, pal, k):
+    """Case k of the postprocess fixture: (raw image, kv dict, palettes,
+    expected float image, expected TGA bytes)."""
+    kv = parse_kv(f["pp%d_kv" % k])
+    pal3 = (f["pal3_x"], f["pal3_rgb"])
+    pals = [pal, pal3, pal3][:int(kv.get("ptypes", 1))]
+    return f["pp%d_in" % k], kv, pals, f["pp%d_out" % k], f["pp%d_tga" % k]

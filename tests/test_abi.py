@@ -93,3 +93,23 @@ def test_params_from_par_defaults_and_keys():
         S.params_from_par("projection=1")        # not a bool in the reference's parser
     with pytest.raises(S.SplhipError):
         S.params_from_par("xres=12abc")
+
+
+def test_post_from_par_defaults_and_keys(golden):
+    """The epilogue keys (splotch.cc:186,307; splotchutils.cc:232-234) read as
+    the oracle's tests read them from the fixture's .par excerpts."""
+    from helpers import parse_kv, post_args
+    q = S.post_from_par("")
+    assert (q.a_eq_e, q.colorbar, q.ptypes, q.gamma, q.brightness, q.contrast) == (
+        1, 0, 1, 1.0, 0.0, 1.0)
+    f = golden("postprocess")
+    for k in range(int(f["ncases"][0])):
+        text = str(f["pp%d_kv" % k])
+        q = S.post_from_par(text)
+        want = post_args(parse_kv(text))
+        assert bool(q.a_eq_e) == want["a_eq_e"] and bool(q.colorbar) == want["colorbar"]
+        assert (q.gamma, q.brightness, q.contrast) == (want["gamma"], want["brightness"],
+                                                       want["contrast"])
+        assert list(q.col_vector[:q.ptypes]) == want["col_vector"]
+    with pytest.raises(S.SplhipError):
+        S.post_from_par("colorbar=2")

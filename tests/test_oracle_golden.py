@@ -130,3 +130,38 @@ def test_host_rendering_fixture(golden):
     pic, _ = O.host_rendering(field_from(f), v, [pal(golden)], a_eq_e=False, sort_type=1,
                               grayabsorb=0.3, tile=32, stable=False)
     assert_bitexact(pic, f["pic"], "host_rendering")
+
+
+def test_postprocess_fixture(golden):
+    """-expm1, colour_adjust, colorbars, Colour8 quantization and the TGA file:
+    the oracle against the genuine code, bit for bit (NaN, inf and values
+    outside the int range included)."""
+    from helpers import post_args, pp_case
+    f = golden("postprocess")
+    for k in range(int(f["ncases"][0])):
+        pic, kv, pals, want, tga = pp_case(f, pal(golden), k)
+        got, gtga = O.postprocess(pic, pals, **post_args(kv))
+        assert_bitexact(got, want, "post-processed image, case %d" % k)
+        assert_bitexact(gtga, tga, "TGA file, case %d" % k)
+
+
+def test_postprocess_closed_forms(golden):
+    """Bar rows at palette nodes are the node colours; the 8-bit code of
+    k/256 is k (255 for 1.0 and above, 0 below 0, for NaN and for +-inf,
+    which the x86 conversion maps to INT_MIN)."""
+    x, rgb = pal(golden)
+    n = len(x)
+    assert np.allclose(x, np.arange(n) / (n - 1))
+    img = np.zeros((n - 1, 30, 3), np.float32)
+    out, tga = O.postprocess(img, [(x, rgb)], a_eq_e=False, colorbar=True)
+    for ix in range(n - 1):   # ix / double(xres) == x[ix] exactly
+        for iy in range(20, 30):
+            assert_bitexact(out[ix, iy], rgb[ix], "bar pixel %d,%d" % (ix, iy))
+    assert not np.any(out[:, :20])
+    vals = np.array([k / 256 for k in range(256)] + [1.0, 2.0, -1e-7, -3.0, np.nan, np.inf,
+                                                      -np.inf, 3e9], np.float32)
+    img = np.repeat(vals[:, None, None], 3, axis=2)
+    _, tga = O.postprocess(img, [], a_eq_e=False)
+    codes = tga[18::3]
+    want = list(range(256)) + [255, 255, 0, 0, 0, 0, 0, 0]
+    assert codes.tolist() == want
