@@ -28,6 +28,10 @@ The JSON line also carries:
   fwd_bwd       the differentiable workload: 2M particles, 512^2, additive,
                 and a 500k ordered variant; loss sum (pic - target)^2 against
                 the render of a perturbed copy; forward vs forward + backward;
+  epilogue      the post-processing of the frame (-expm1, 8-bit quantization,
+                TGA byte order; splhip_postprocess) as one kernel: algorithmic
+                bytes 12 B read + 3 B written per pixel over its event time,
+                and the host copy of the 8-bit frame vs the float frame;
   cpu_baseline  the genuine reference host renderer (oracle/_ref, built from
                 the Splotch sources) — or the oracle restatement if that
                 library is absent — timed on this host's cores on a bounded
@@ -232,6 +236,38 @@ def bench_backward(S, R, torch, steps, warmup):
                      "ms_fwd": round(res["fwd"], 3), "ms_fwd_bwd": round(res["fwd_bwd"], 3),
                      "fwd_bwd_over_fwd": round(res["fwd_bwd"] / res["fwd"], 3)}
     return out
+
+
+def bench_epilogue(S, R, torch, pic, W, H, reps=20):
+    q = S.post_from_par("")  # a_eq_e: -expm1, no adjustment, no colorbar
+    rgb8 = torch.empty(3 * W * H, dtype=torch.uint8, device="cuda")
+    for _ in range(2):
+        R.postprocess(pic, W, H, q, rgb8=rgb8, layout=1)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        R.postprocess(pic, W, H, q, rgb8=rgb8, layout=1)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nbytes = 15 * W * H
+
+    def d2h(t):
+        h = torch.empty(t.numel(), dtype=t.dtype, pin_memory=True)
+        h.copy_(t)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            h.copy_(t)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / 5
+
+    return {"kernel": "k_post_tga", "ms": round(ms, 5), "bytes": nbytes,
+            "achieved_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1), "peak_GBs": HBM_PEAK_GBS,
+            "frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "d2h_ms_rgb8": round(d2h(rgb8), 4), "d2h_ms_float": round(d2h(pic), 4)}
 
 
 def bench_large_share(S, torch, args):
@@ -491,6 +527,7 @@ def main():
                              "limit"},
         "valu": valu,
     }
+    out["epilogue"] = bench_epilogue(S, R, torch, pic, W, H)
     if not args.no_batched:
         out["batched"] = bench_batched(S, R, torch, max(2, args.steps // 2), 1)
     if not args.no_backward:

@@ -158,6 +158,7 @@ struct PostArgs {
   uint8_t *rgb8;     // bytes (null: none)
   int32_t xres, yres;
   int32_t a_eq_e, adjust, nbars;
+  int32_t vec_in, vec_out;  // float4 image columns / 32-bit byte rows (set by launch_post)
   int32_t bar_type[kMaxTypes];
   int32_t pal_n[kMaxTypes];
   int32_t pal_stride;

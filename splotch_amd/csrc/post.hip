@@ -15,7 +15,7 @@
 // x-major order (LS_Image's pixel[ix][iy]) or as the body of the TGA file
 // LS_Image::write_TGA writes (rows iy ascending, pixels ix ascending, B G R;
 // ls_image.cc:178-202), which is a transpose of the image: that form is
-// staged through LDS in 64 x 64 pixel tiles so both the float reads and the
+// staged through LDS in 64 x 16 pixel tiles so both the float reads and the
 // byte writes are contiguous.
 #include "kernels.hpp"
 
@@ -67,34 +67,86 @@ __global__ __launch_bounds__(256) void k_post_xmajor(PostArgs a) {
 }
 
 // TGA body through LDS: block (bx, by) owns pixels ix in [64 bx, 64 bx + 64),
-// iy in [64 by, 64 by + 64).
-constexpr int kPT = 64;
-constexpr int kRow = 3 * kPT + 4;  // bytes per LDS row (padded)
+// iy in [16 by, 16 by + 16).
+constexpr int kTX = 64, kTY = 16;
+constexpr int kRow = 3 * kTY + 4;  // LDS bytes per pixel column (padded)
 
 __global__ __launch_bounds__(256) void k_post_tga(PostArgs a) {
-  __shared__ uint8_t tile[kPT * kRow];
-  const int ix0 = blockIdx.x * kPT, iy0 = blockIdx.y * kPT;
-  const int nx = min(kPT, a.xres - ix0), ny = min(kPT, a.yres - iy0);
-  // read: per pixel column ix, the 3 ny floats of iy0.. are contiguous
-  for (int q = threadIdx.x; q < kPT * 3 * kPT; q += 256) {
-    const int r = q / (3 * kPT), w = q - r * (3 * kPT);
-    const int il = w / 3, c = w - 3 * il;
-    if (r < nx && il < ny) {
-      const int ix = ix0 + r, iy = iy0 + il;
-      const int64_t e = (static_cast<int64_t>(ix) * a.yres + iy) * 3 + c;
-      const float v = post_value(a, a.pic[e], ix, iy, c);
-      if (a.out) a.out[e] = v;
-      tile[r * kRow + w] = colour8(v);
+  __shared__ uint8_t tile[kTX * kRow];
+  const int ix0 = blockIdx.x * kTX, iy0 = blockIdx.y * kTY;
+  const int nx = min(kTX, a.xres - ix0), ny = min(kTY, a.yres - iy0);
+  // read: per pixel column ix, the 3 ny floats of iy0.. are contiguous; full
+  // columns of a 4-aligned image go as float4 (48 floats = 12 float4), all
+  // of a thread's loads issued before its stores (out may be pic)
+  if (ny == kTY && a.vec_in) {
+    constexpr int kPer = kTX * 12 / 256;  // float4 per thread
+    float4 v4[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int f = threadIdx.x + 256 * k, r = f / 12, w0 = 4 * (f - r * 12);
+      if (r < nx)
+        v4[k] = *reinterpret_cast<const float4 *>(
+            a.pic + (static_cast<int64_t>(ix0 + r) * a.yres + iy0) * 3 + w0);
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int f = threadIdx.x + 256 * k, r = f / 12, w0 = 4 * (f - r * 12);
+      if (r < nx) {
+        const int ix = ix0 + r;
+        float o[4] = {v4[k].x, v4[k].y, v4[k].z, v4[k].w};
+        int il = w0 / 3, c = w0 - 3 * il;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          o[q] = post_value(a, o[q], ix, iy0 + il, c);
+          tile[r * kRow + w0 + q] = colour8(o[q]);
+          if (++c == 3) {
+            c = 0;
+            ++il;
+          }
+        }
+        if (a.out)
+          *reinterpret_cast<float4 *>(a.out + (static_cast<int64_t>(ix) * a.yres + iy0) * 3 +
+                                      w0) = make_float4(o[0], o[1], o[2], o[3]);
+      }
+    }
+  } else {
+    for (int q = threadIdx.x; q < kTX * 3 * kTY; q += 256) {
+      const int r = q / (3 * kTY), w = q - r * (3 * kTY);
+      const int il = w / 3, c = w - 3 * il;
+      if (r < nx && il < ny) {
+        const int ix = ix0 + r, iy = iy0 + il;
+        const int64_t e = (static_cast<int64_t>(ix) * a.yres + iy) * 3 + c;
+        const float v = post_value(a, a.pic[e], ix, iy, c);
+        if (a.out) a.out[e] = v;
+        tile[r * kRow + w] = colour8(v);
+      }
     }
   }
   __syncthreads();
-  // write: per row iy, the 3 nx bytes of ix0.. are contiguous (B, G, R)
-  for (int q = threadIdx.x; q < kPT * 3 * kPT; q += 256) {
-    const int il = q / (3 * kPT), w = q - il * (3 * kPT);
-    const int r = w / 3, cb = w - 3 * r;
-    if (il < ny && r < nx) {
-      const int64_t o = (static_cast<int64_t>(iy0 + il) * a.xres + ix0) * 3 + w;
-      a.rgb8[o] = tile[r * kRow + il * 3 + (2 - cb)];
+  // write: per row iy, the 3 nx bytes of ix0.. are contiguous (B, G, R);
+  // full rows of a 4-aligned output go as 32-bit words (192 bytes = 48)
+  if (nx == kTX && a.vec_out) {
+    for (int f = threadIdx.x; f < kTY * 48; f += 256) {
+      const int il = f / 48, w0 = 4 * (f - il * 48);
+      if (il < ny) {
+        uint32_t word = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int w = w0 + k, r = w / 3, cb = w - 3 * r;
+          word |= static_cast<uint32_t>(tile[r * kRow + il * 3 + (2 - cb)]) << (8 * k);
+        }
+        const int64_t o = (static_cast<int64_t>(iy0 + il) * a.xres + ix0) * 3 + w0;
+        *reinterpret_cast<uint32_t *>(a.rgb8 + o) = word;
+      }
+    }
+  } else {
+    for (int q = threadIdx.x; q < kTY * 3 * kTX; q += 256) {
+      const int il = q / (3 * kTX), w = q - il * (3 * kTX);
+      const int r = w / 3, cb = w - 3 * r;
+      if (il < ny && r < nx) {
+        const int64_t o = (static_cast<int64_t>(iy0 + il) * a.xres + ix0) * 3 + w;
+        a.rgb8[o] = tile[r * kRow + il * 3 + (2 - cb)];
+      }
     }
   }
 }
@@ -103,8 +155,14 @@ __global__ __launch_bounds__(256) void k_post_tga(PostArgs a) {
 
 void launch_post(const PostArgs &a, int layout, hipStream_t s) {
   if (layout == 1 && a.rgb8) {
-    const dim3 grid((a.xres + kPT - 1) / kPT, (a.yres + kPT - 1) / kPT);
-    k_post_tga<<<grid, 256, 0, s>>>(a);
+    const dim3 grid((a.xres + kTX - 1) / kTX, (a.yres + kTY - 1) / kTY);
+    PostArgs b = a;
+    // float4 columns need every column start (ix yres + iy0) 3 on a 16-byte
+    // boundary; 32-bit rows need (iy xres + ix0) 3 on a 4-byte boundary
+    b.vec_in = a.yres % 4 == 0 && reinterpret_cast<uintptr_t>(a.pic) % 16 == 0 &&
+               reinterpret_cast<uintptr_t>(a.out) % 16 == 0;
+    b.vec_out = a.xres % 4 == 0 && reinterpret_cast<uintptr_t>(a.rgb8) % 4 == 0;
+    k_post_tga<<<grid, 256, 0, s>>>(b);
     hip_check(hipGetLastError(), "k_post_tga launch");
   } else {
     const int64_t n = static_cast<int64_t>(a.xres) * a.yres * 3;
