@@ -306,6 +306,57 @@ int splhip_postprocess(splhip_ctx *ctx, const float *pic, int32_t xres, int32_t 
                        const splhip_post *post, float *pic_out, uint8_t *rgb8, int32_t layout,
                        void *stream);
 
+/* --- particle normalization --------------------------------------------------- */
+
+/* sceneMaker::particle_normalize (splotch/scenemaker.cc:40-296, host branch),
+   which the scene maker runs on the particles before each render: per type t,
+   intensity_log{t} replaces I by log10(I) (I <= 0 becomes -38 and stays out of
+   the range), color_log{t} does the same to the colour (the second and third
+   components of a vector colour are logged unchecked), color_asinh{t} applies
+   asinh otherwise; then the type's data ranges (min / max over the values,
+   NaN never enters) are replaced by intensity_min{t} ... color_max{t} where
+   given, and Normalizer::normAndClamp maps each value v to
+   (max(min, min(max, v)) - min) / (max - min), or 1 when max == min
+   (splotch/splotchutils.h:138-171).  The intensity is clamped too unless the
+   reference is built with NO_I_NORM (norm_intensity = 0). */
+typedef struct {
+  int32_t ptypes;
+  int32_t intensity_log[SPLHIP_MAX_TYPES], color_log[SPLHIP_MAX_TYPES];
+  int32_t color_asinh[SPLHIP_MAX_TYPES], col_vector[SPLHIP_MAX_TYPES];
+  int32_t has_intensity_min[SPLHIP_MAX_TYPES], has_intensity_max[SPLHIP_MAX_TYPES];
+  int32_t has_color_min[SPLHIP_MAX_TYPES], has_color_max[SPLHIP_MAX_TYPES];
+  float intensity_min[SPLHIP_MAX_TYPES], intensity_max[SPLHIP_MAX_TYPES];
+  float color_min[SPLHIP_MAX_TYPES], color_max[SPLHIP_MAX_TYPES];
+  int32_t norm_intensity;
+} splhip_norm;
+
+/* Per type, the ranges particle_normalize finds and applies (it records the
+   colour ones as the parameters range_min{t}, range_max{t}, clamp_min{t} and
+   clamp_max{t}).  A type without particles keeps 1e37 / -1e37. */
+typedef struct {
+  float range_min, range_max;          /* colour values of the data */
+  float clamp_min, clamp_max;          /* colour range applied */
+  float intensity_min, intensity_max;  /* intensity range applied */
+  float size_min, size_max;            /* smoothing lengths of the data */
+} splhip_norm_ranges;
+
+/* The keys above from .par text (ptypes, intensity_log{t}, color_log{t},
+   color_asinh{t}, color_is_vector{t}, intensity_min{t}, intensity_max{t},
+   color_min{t}, color_max{t}); norm_intensity = 1. */
+int splhip_norm_from_par(const char *par_text, splhip_norm *out);
+
+/* Normalizes n particles' device fields in place: inten, col (the scalar
+   colour or first component), c1 and c2 (second and third components; needed
+   when a type has a vector colour, else may be NULL); hsml and type (NULL:
+   all type 0) are read.  Particles of a type >= ptypes are left as they are.
+   ranges (host, ptypes entries, may be NULL): when given, the call waits for
+   the stream.  log10 and asinh are evaluated in double and rounded to float:
+   they may differ from glibc's log10f / the reference's double log by one
+   float ulp (log10f is not correctly rounded; about 1.4% of arguments). */
+int splhip_normalize(splhip_ctx *ctx, int64_t n, float *inten, float *col, float *c1, float *c2,
+                     const float *hsml, const uint8_t *type, const splhip_norm *norm,
+                     splhip_norm_ranges *ranges, void *stream);
+
 /* Host-built tables (device copies are used by the kernels): tab[0..1023] =
    tab1, tab[1024..2047] = tab2; consts = {expfac, taylorlimit, h2sigma, bfak,
    rfac}.  Host only. */

@@ -341,6 +341,59 @@ def ref_postprocess(pic, palettes, params: dict, tga_path):
     return out, tga
 
 
+NORM_FLAGS = ("intensity_log", "color_log", "color_asinh", "color_is_vector")
+NORM_OVR = ("intensity_min", "intensity_max", "color_min", "color_max")
+
+
+def norm_tables(kv: dict, ntypes: int):
+    """Per-type flags / overrides of the normalization keys of a .par dict."""
+    truth = ("t", "y", "true", ".true.")
+    flags = np.zeros((ntypes, 4), np.int32)
+    ovr_set = np.zeros((ntypes, 4), np.int32)
+    ovr = np.zeros((ntypes, 4), np.float32)
+    for t in range(ntypes):
+        for k, key in enumerate(NORM_FLAGS):
+            flags[t, k] = str(kv.get("%s%d" % (key, t), "false")).lower() in truth
+        for k, key in enumerate(NORM_OVR):
+            if "%s%d" % (key, t) in kv:
+                ovr_set[t, k] = 1
+                ovr[t, k] = np.float32(float(kv["%s%d" % (key, t)]))
+    return flags, ovr_set, ovr
+
+
+def normalize(fields: dict, kv: dict, ntypes: int, norm_int=True):
+    """Oracle restatement of sceneMaker::particle_normalize (orc_normalize).
+    fields: I, r, er, eg, eb (float32), type (uint8).  Returns (new fields,
+    ranges float32[ntypes, 6], size ranges float32[ntypes, 2])."""
+    f = {k: np.ascontiguousarray(fields[k], np.float32).copy() for k in ("I", "r", "er", "eg", "eb")}
+    typ = np.ascontiguousarray(fields["type"], np.uint8)
+    flags, ovr_set, ovr = norm_tables(kv, ntypes)
+    out = np.zeros((ntypes, 6), np.float32)
+    size = np.zeros((ntypes, 2), np.float32)
+    lib().orc_normalize(C.c_int64(len(typ)), _p(f["I"]), _p(f["r"]), _p(f["er"]), _p(f["eg"]),
+                        _p(f["eb"]), _p(typ), C.c_int(ntypes), _p(flags), _p(ovr_set), _p(ovr),
+                        C.c_int(int(norm_int)), _p(out), _p(size))
+    f["type"] = typ
+    return f, out, size
+
+
+def ref_normalize(fields: dict, kv: dict, ntypes: int):
+    """The genuine sceneMaker::particle_normalize (ref_normalize).  Returns
+    (new fields, float64[ntypes, 4] of the range_min, range_max, clamp_min,
+    clamp_max parameters it sets)."""
+    f = {k: np.ascontiguousarray(fields[k], np.float32).copy() for k in ("I", "r", "er", "eg", "eb")}
+    typ = np.ascontiguousarray(fields["type"], np.uint8)
+    ranges = np.zeros((ntypes, 4), np.float64)
+    kv = dict(kv, ptypes=ntypes)
+    rc = ref().ref_normalize(kv_text(kv).encode(), C.c_int64(len(typ)), _p(f["I"]), _p(f["r"]),
+                             _p(f["er"]), _p(f["eg"]), _p(f["eb"]), _p(typ), C.c_int(ntypes),
+                             _p(ranges))
+    if rc != 0:
+        raise RuntimeError("reference particle_normalize failed")
+    f["type"] = typ
+    return f, ranges
+
+
 def render64(p: Projected, s, xres, yres, a_eq_e=True, grayabsorb=0.2, tile=50):
     """FP64 twin on the frozen float32 footprint; s: dict sx, sy, r, e0, e1, e2 (float64)."""
     pic = np.zeros((xres, yres, 3), np.float64)

@@ -12,6 +12,7 @@
  *   host_rendering                 splotch/splotch_host.cc:711-834
  *   exptable<float32>              splotch/splotchutils.h:174-215
  *   colour_adjust, add_colorbar    splotch/splotchutils.cc:35-56,230-250
+ *   sceneMaker::particle_normalize splotch/scenemaker.cc:40-296
  *   LS_Image::put_pixel/write_TGA  cxxsupport/ls_image.h:65-90,134-139, ls_image.cc:178-202
  * It is used to write the golden fixtures under tests/golden/ (in the build
  * container) and as the "reference" CPU baseline in bench.py.
@@ -33,6 +34,19 @@
 #include <cstdio>
 #include <fstream>
 #include <iterator>
+#include <sstream>
+#include <iostream>
+#include <algorithm>
+#include <cmath>
+#include <math.h>
+
+// particle_normalize is a private member of sceneMaker.  This translation unit
+// (test infrastructure only) opens the class's access so it can call the
+// genuine function; the reference objects are compiled from their sources
+// unchanged, and access does not change the class layout.
+#define private public
+#include "splotch/scenemaker.h"
+#undef private
 
 namespace host_funct {
 // Definitions in splotch_host.cc (the header declares different signatures).
@@ -295,6 +309,45 @@ int ref_postprocess(const char *kv, int xres, int yres, float *pic, int ntypes,
     if (bytes.size() != 18 + size_t(3) * xres * yres) return -1;
     std::memcpy(tga_out, bytes.data(), bytes.size());
     std::remove(tga_path);
+  } catch (...) {
+    return -1;
+  }
+  return 0;
+}
+
+/* The genuine sceneMaker::particle_normalize (host branch: logs, asinh,
+   per-type min/max, clamping) over n particles given as SoA (updated in
+   place: I, er, eg, eb; r is read).  kv: .par keys (ptypes, intensity_log{t},
+   color_log{t}, color_asinh{t}, color_is_vector{t}, intensity_min{t} ...).
+   ranges[t*4 + k] = the parameters the function sets: range_min{t},
+   range_max{t}, clamp_min{t}, clamp_max{t}.  Returns 0/-1. */
+int ref_normalize(const char *kv, int64_t n, float *I, const float *r, float *er, float *eg,
+                  float *eb, const uint8_t *type, int ntypes, double *ranges) {
+  try {
+    auto m = parse_kv(kv);
+    if (!m.count("outfile")) m["outfile"] = "normalize_probe";
+    paramfile params(m, false);
+    sceneMaker sm(params);
+    std::vector<particle_sim> p(n);
+    for (int64_t i = 0; i < n; ++i) {
+      p[i].e = COLOUR(er[i], eg[i], eb[i]);
+      p[i].x = p[i].y = p[i].z = 0.f;
+      p[i].r = r[i];
+      p[i].I = I[i];
+      p[i].type = type ? type[i] : 0;
+      p[i].active = true;
+    }
+    sm.particle_normalize(p, false);
+    for (int64_t i = 0; i < n; ++i) {
+      I[i] = p[i].I; er[i] = p[i].e.r; eg[i] = p[i].e.g; eb[i] = p[i].e.b;
+    }
+    for (int t = 0; t < ntypes; ++t) {
+      const std::string ts = dataToString(t);
+      ranges[4 * t + 0] = sm.params.find<double>("range_min" + ts);
+      ranges[4 * t + 1] = sm.params.find<double>("range_max" + ts);
+      ranges[4 * t + 2] = sm.params.find<double>("clamp_min" + ts);
+      ranges[4 * t + 3] = sm.params.find<double>("clamp_max" + ts);
+    }
   } catch (...) {
     return -1;
   }

@@ -29,6 +29,9 @@
  *                     add_colorbar, LS_Image quantization and the TGA file, bit
  *                     for bit); closed forms (bar rows = palette nodes, 8-bit
  *                     code of k/256 is k, clamps, x86 conversion of NaN/inf)
+ *   orc_normalize     genuine-build fixtures (sceneMaker::particle_normalize,
+ *                     bit for bit); closed forms (min -> 0, max -> 1, one value
+ *                     -> 1, log10 of powers of ten, -38 for non-positive logs)
  *   orc_adjoint64     FP64 adjoint; with pyoracle.chain64 (the FP64 chain rule
  *                     through pyoracle.project64), checked against FP64 central
  *                     differences (h = 1e-4) of orc_render64 after project64 on
@@ -600,6 +603,95 @@ void orc_postprocess(int xres, int yres, float *pic, int a_eq_e, double gamma, d
       for (int c = 0; c < 3; ++c)
         o[2 - c] = uint8_t(std::max(0, std::min(255, x86_trunc(d[c] * 256))));
     }
+}
+
+/* sceneMaker::particle_normalize, host branch (splotch/scenemaker.cc:40-296),
+   over SoA fields updated in place (I, er, eg, eb; r read), with Normalizer
+   (splotch/splotchutils.h:138-171) semantics: collect keeps min/max through
+   std::min/std::max (NaN never enters), normAndClamp maps
+   (max(minv, min(maxv, v)) - minv) / (maxv - minv), or 1 when maxv == minv.
+   Per type t: flags[t*4 + 0..3] = intensity_log, color_log, color_asinh,
+   color_is_vector; ovr_set[t*4 + k] / ovr[t*4 + k] = intensity_min,
+   intensity_max, color_min, color_max given in the parameters.  norm_int: 0
+   for a build with NO_I_NORM.  out[t*6 + 0..5] = colour range of the data,
+   colour range used, intensity range used (before normAndClamp); size[t*2]
+   = the smoothing-length range.  Single-threaded: the reference's OpenMP
+   reduction order only decides which of +0 / -0 survives a tie. */
+static double my_asinh(double v) { return std::log(v + std::sqrt(1. + v * v)); }
+
+void orc_normalize(int64_t n, float *I, const float *r, float *er, float *eg, float *eb,
+                   const uint8_t *type, int ntypes, const int *flags, const int *ovr_set,
+                   const float *ovr, int norm_int, float *out, float *size) {
+  struct Norm {
+    float minv = 1e37f, maxv = -1e37f;
+    void collect(float v) { minv = std::min(minv, v); maxv = std::max(maxv, v); }
+    void clamp(float &v) const {
+      if (maxv == minv) v = 1.f;
+      else v = (std::max(minv, std::min(maxv, v)) - minv) / (maxv - minv);
+    }
+  };
+  std::vector<Norm> in(ntypes), cn(ntypes), rn(ntypes);
+  for (int64_t m = 0; m < n; ++m) {
+    const int t = type ? type[m] : 0;
+    const int *f = flags + 4 * t;
+    rn[t].collect(r[m]);
+    if (f[0]) {
+      if (I[m] > 0) {
+        I[m] = std::log10(I[m]);
+        in[t].collect(I[m]);
+      } else {
+        I[m] = -38;
+      }
+    } else {
+      in[t].collect(I[m]);
+    }
+    if (f[1]) {
+      if (er[m] > 0) {
+        er[m] = std::log10(er[m]);
+        cn[t].collect(er[m]);
+      } else {
+        er[m] = -38;
+      }
+    } else {
+      if (f[2]) er[m] = float(my_asinh(er[m]));
+      cn[t].collect(er[m]);
+    }
+    if (f[3]) {
+      if (f[1]) {
+        eg[m] = std::log10(eg[m]);
+        eb[m] = std::log10(eb[m]);
+      }
+      if (f[2]) {
+        eg[m] = float(my_asinh(eg[m]));
+        eb[m] = float(my_asinh(eb[m]));
+      }
+      cn[t].collect(eg[m]);
+      cn[t].collect(eb[m]);
+    }
+  }
+  for (int t = 0; t < ntypes; ++t) {
+    out[6 * t + 0] = cn[t].minv;
+    out[6 * t + 1] = cn[t].maxv;
+    if (ovr_set[4 * t + 0]) in[t].minv = ovr[4 * t + 0];
+    if (ovr_set[4 * t + 1]) in[t].maxv = ovr[4 * t + 1];
+    if (ovr_set[4 * t + 2]) cn[t].minv = ovr[4 * t + 2];
+    if (ovr_set[4 * t + 3]) cn[t].maxv = ovr[4 * t + 3];
+    out[6 * t + 2] = cn[t].minv;
+    out[6 * t + 3] = cn[t].maxv;
+    out[6 * t + 4] = in[t].minv;
+    out[6 * t + 5] = in[t].maxv;
+    size[2 * t] = rn[t].minv;
+    size[2 * t + 1] = rn[t].maxv;
+  }
+  for (int64_t m = 0; m < n; ++m) {
+    const int t = type ? type[m] : 0;
+    if (norm_int) in[t].clamp(I[m]);
+    cn[t].clamp(er[m]);
+    if (flags[4 * t + 3]) {
+      cn[t].clamp(eg[m]);
+      cn[t].clamp(eb[m]);
+    }
+  }
 }
 
 /* ---------------------------------------------------------------------------

@@ -52,6 +52,24 @@ class Post(C.Structure):
                 ("brightness", C.c_double), ("contrast", C.c_double)]
 
 
+class Norm(C.Structure):
+    """splhip_norm: particle_normalize switches (scenemaker.cc:40-296)."""
+    _fields_ = [("ptypes", C.c_int32)] + [
+        (k, C.c_int32 * MAX_TYPES) for k in (
+            "intensity_log", "color_log", "color_asinh", "col_vector", "has_intensity_min",
+            "has_intensity_max", "has_color_min", "has_color_max")] + [
+        (k, C.c_float * MAX_TYPES) for k in (
+            "intensity_min", "intensity_max", "color_min", "color_max")] + [
+        ("norm_intensity", C.c_int32)]
+
+
+class NormRanges(C.Structure):
+    """splhip_norm_ranges: per-type ranges found and applied."""
+    _fields_ = [(k, C.c_float) for k in ("range_min", "range_max", "clamp_min", "clamp_max",
+                                         "intensity_min", "intensity_max", "size_min",
+                                         "size_max")]
+
+
 class Particles(C.Structure):
     _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("hsml", C.c_void_p),
                 ("inten", C.c_void_p), ("col", C.c_void_p), ("c1", C.c_void_p),
@@ -81,7 +99,7 @@ EXPORTS = ["splhip_create", "splhip_destroy", "splhip_last_error", "splhip_camer
            "splhip_exptable", "splhip_set_profiling", "splhip_stage_times", "splhip_set_trace",
            "splhip_trace_items", "splhip_render_batched", "splhip_render_backward",
            "splhip_last_tuning", "splhip_last_binning", "splhip_post_from_par",
-           "splhip_postprocess"]
+           "splhip_postprocess", "splhip_norm_from_par", "splhip_normalize"]
 
 STAGES = ("project", "ordered_sort", "ordered_bin", "ordered_splat", "additive_compact",
           "additive_bin", "additive_splat", "ordered_k_splat8", "ordered_k_splat_heavy",
@@ -222,6 +240,14 @@ def post_from_par(text: str) -> Post:
     return q
 
 
+def norm_from_par(text: str) -> Norm:
+    """The particle_normalize keys of a .par text (splhip_norm_from_par)."""
+    q = Norm()
+    if lib().splhip_norm_from_par(text.encode(), C.byref(q)) != 0:
+        raise SplhipError("bad parameter text")
+    return q
+
+
 def tga_header(xres: int, yres: int) -> bytes:
     """The 18-byte header LS_Image::write_TGA writes (ls_image.cc:183-186)."""
     return bytes([0, 0, 2, 0, 0, 0, 0, 0, 0, 0, 0, 0, xres % 256, xres // 256, yres % 256,
@@ -345,6 +371,21 @@ class Renderer:
             C.c_void_p(rgb8.data_ptr() if rgb8 is not None else None), C.c_int32(layout),
             self._stream(stream)), "splhip_postprocess")
         return out, rgb8
+
+    def normalize(self, dev: dict, q: Norm, stream=None):
+        """particle_normalize in place on device fields (dev: inten, col, c1,
+        c2, hsml, type as cuda tensors).  Returns the per-type ranges (list of
+        dicts)."""
+        n = int(dev["x"].numel()) if "x" in dev else int(dev["inten"].numel())
+
+        def ptr(k):
+            t = dev.get(k)
+            return C.c_void_p(t.data_ptr() if t is not None else None)
+        rr = (NormRanges * MAX_TYPES)()
+        self._check(lib().splhip_normalize(
+            self._c, C.c_int64(n), ptr("inten"), ptr("col"), ptr("c1"), ptr("c2"), ptr("hsml"),
+            ptr("type"), C.byref(q), rr, self._stream(stream)), "splhip_normalize")
+        return [{k: getattr(rr[t], k) for k, _ in NormRanges._fields_} for t in range(q.ptypes)]
 
     def render_batched(self, views, stream=None):
         """views: list of (dev, cam, prm, pic) as for render(); all images in

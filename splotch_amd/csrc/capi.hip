@@ -1487,6 +1487,108 @@ int splhip_postprocess(splhip_ctx *c, const float *pic, int32_t xres, int32_t yr
   return 0;
 }
 
+int splhip_norm_from_par(const char *par_text, splhip_norm *out) {
+  if (!out) return SPLHIP_E_ARG;
+  try {
+    const auto d = parse_par(par_text);
+    std::memset(out, 0, sizeof(*out));
+    const int nt = par_num<int>(d, "ptypes", 1);
+    if (nt < 1 || nt > kMaxTypes) return SPLHIP_E_ARG;
+    out->ptypes = nt;
+    out->norm_intensity = 1;
+    for (int t = 0; t < nt; ++t) {  // scenemaker.cc:51-55, 223-233
+      const std::string ts = std::to_string(t);
+      out->intensity_log[t] = par_bool(d, "intensity_log" + ts, false) ? 1 : 0;
+      out->color_log[t] = par_bool(d, "color_log" + ts, false) ? 1 : 0;
+      out->color_asinh[t] = par_bool(d, "color_asinh" + ts, false) ? 1 : 0;
+      out->col_vector[t] = par_bool(d, "color_is_vector" + ts, false) ? 1 : 0;
+      struct { const char *key; int32_t *has; float *val; } ov[4] = {
+          {"intensity_min", out->has_intensity_min, out->intensity_min},
+          {"intensity_max", out->has_intensity_max, out->intensity_max},
+          {"color_min", out->has_color_min, out->color_min},
+          {"color_max", out->has_color_max, out->color_max}};
+      for (auto &o : ov)
+        if (d.count(o.key + ts)) {
+          o.has[t] = 1;
+          o.val[t] = par_num<float>(d, o.key + ts, 0.f);
+        }
+    }
+  } catch (const ParError &) {
+    return SPLHIP_E_ARG;
+  }
+  return 0;
+}
+
+int splhip_normalize(splhip_ctx *c, int64_t n, float *inten, float *col, float *c1, float *c2,
+                     const float *hsml, const uint8_t *type, const splhip_norm *norm,
+                     splhip_norm_ranges *ranges, void *stream) {
+  if (!c || !norm) return fail(c, SPLHIP_E_ARG, "null argument");
+  if (n < 0) return fail(c, SPLHIP_E_ARG, "negative particle count");
+  const int nt = norm->ptypes;
+  if (nt < 1 || nt > kMaxTypes) return fail(c, SPLHIP_E_ARG, "bad ptypes");
+  if (n > 0 && (!inten || !col || !hsml)) return fail(c, SPLHIP_E_ARG, "null field");
+  bool vec = false;
+  for (int t = 0; t < nt; ++t) vec = vec || norm->col_vector[t];
+  if (n > 0 && vec && (!c1 || !c2))
+    return fail(c, SPLHIP_E_ARG, "vector colour without c1/c2");
+  try {
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    NormArgs a{};
+    a.n = n;
+    a.I = inten;
+    a.er = col;
+    a.eg = c1;
+    a.eb = c2;
+    a.r = hsml;
+    a.type = type;
+    a.ntypes = nt;
+    a.norm_int = norm->norm_intensity ? 1 : 0;
+    for (int t = 0; t < nt; ++t) {
+      a.flags[t][0] = norm->intensity_log[t] ? 1 : 0;
+      a.flags[t][1] = norm->color_log[t] ? 1 : 0;
+      a.flags[t][2] = norm->color_asinh[t] ? 1 : 0;
+      a.flags[t][3] = norm->col_vector[t] ? 1 : 0;
+      a.ovr_set[t][0] = norm->has_intensity_min[t] ? 1 : 0;
+      a.ovr_set[t][1] = norm->has_intensity_max[t] ? 1 : 0;
+      a.ovr_set[t][2] = norm->has_color_min[t] ? 1 : 0;
+      a.ovr_set[t][3] = norm->has_color_max[t] ? 1 : 0;
+      a.ovr[t][0] = norm->intensity_min[t];
+      a.ovr[t][1] = norm->intensity_max[t];
+      a.ovr[t][2] = norm->color_min[t];
+      a.ovr[t][3] = norm->color_max[t];
+    }
+    uint32_t *keys = c->ws.get_as<uint32_t>(S_NORM, 6 * kMaxTypes);
+    launch_normalize(a, keys, s);
+    if (ranges) {
+      uint32_t h[6 * kMaxTypes];
+      hip_check(hipMemcpyAsync(h, keys, sizeof(uint32_t) * 6 * nt, hipMemcpyDeviceToHost, s),
+                "read ranges");
+      hip_check(hipStreamSynchronize(s), "sync ranges");
+      auto dek = [](uint32_t k) {
+        const uint32_t b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+        float f;
+        std::memcpy(&f, &b, 4);
+        return f;
+      };
+      for (int t = 0; t < nt; ++t) {
+        splhip_norm_ranges &R = ranges[t];
+        R.range_min = dek(h[6 * t + 0]);
+        R.range_max = dek(h[6 * t + 1]);
+        R.clamp_min = norm->has_color_min[t] ? norm->color_min[t] : R.range_min;
+        R.clamp_max = norm->has_color_max[t] ? norm->color_max[t] : R.range_max;
+        R.intensity_min = norm->has_intensity_min[t] ? norm->intensity_min[t] : dek(h[6 * t + 2]);
+        R.intensity_max = norm->has_intensity_max[t] ? norm->intensity_max[t] : dek(h[6 * t + 3]);
+        R.size_min = dek(h[6 * t + 4]);
+        R.size_max = dek(h[6 * t + 5]);
+      }
+    }
+  } catch (const HipError &e) {
+    return hip_fail(c, e);
+  }
+  return 0;
+}
+
 int splhip_set_palette(splhip_ctx *c, int32_t type, int32_t n, const double *x,
                        const float *rgb) {
   if (!c) return SPLHIP_E_ARG;

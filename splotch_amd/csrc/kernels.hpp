@@ -172,4 +172,21 @@ struct PostArgs {
 // (rows iy ascending, B G R).
 void launch_post(const PostArgs &a, int layout, hipStream_t s);
 
+// particle_normalize over SoA fields (normalize.hip): flags[t] = intensity_log,
+// color_log, color_asinh, color_is_vector; ovr_set / ovr[t] = intensity_min,
+// intensity_max, color_min, color_max given as parameters.
+struct NormArgs {
+  int64_t n;
+  float *I, *er, *eg, *eb;
+  const float *r;
+  const uint8_t *type;
+  int32_t ntypes, norm_int;
+  int32_t flags[kMaxTypes][4];
+  int32_t ovr_set[kMaxTypes][4];
+  float ovr[kMaxTypes][4];
+};
+// keys: 6 ntypes u32 of scratch; after the call they hold the per-type
+// extremes as order-preserving keys (colour, intensity, size; min then max).
+void launch_normalize(const NormArgs &a, uint32_t *keys, hipStream_t s);
+
 }  // namespace splhip

@@ -12,11 +12,14 @@ returns:
   * render_new raw images (splotch_host.cc:330-697) in both modes, several tiles;
   * one full host_rendering() call (splotch_host.cc:711-834);
   * the master's post-processing and TGA output (splotch.cc:272-310):
-    -expm1, colour_adjust, add_colorbar, Colour8 quantization, write_TGA.
+    -expm1, colour_adjust, add_colorbar, Colour8 quantization, write_TGA;
+  * sceneMaker::particle_normalize (splotch/scenemaker.cc:40-296): logs,
+    asinh, per-type ranges, overrides and clamping.
 Fixtures are inputs + outputs only (numpy .npz, no pickles).
 
     python tests/golden/make_golden.py                 # every fixture
     python tests/golden/make_golden.py postprocess     # only postprocess.npz
+    python tests/golden/make_golden.py normalize       # only normalize.npz
 """
 from __future__ import annotations
 
@@ -185,6 +188,7 @@ def main():
     manifest["host_rendering"] = "host_rendering(): 10k, 120x100, ordered, T=32, g=0.3"
 
     postprocess_fixture(manifest)
+    normalize_fixture(manifest)
 
     # palette used by every fixture (this project's own ramp)
     np.savez_compressed(os.path.join(HERE, "palette.npz"), x=pal[0], rgb=pal[1])
@@ -235,11 +239,54 @@ def postprocess_fixture(manifest):
                                "gamma/brightness/contrast, colorbars, edge values)" % len(cases))
 
 
+def normalize_fixture(manifest):
+    """Seeded particle fields (three types, lognormal intensities and colours,
+    non-positive values, a NaN) through the genuine particle_normalize: case
+    k stores nm{k}_kv, nm{k}_{I,r,er,eg,eb,type} (inputs), nm{k}_out_{I,er,eg,eb}
+    and nm{k}_ranges (range_min, range_max, clamp_min, clamp_max parameters)."""
+    rng = np.random.default_rng(41)
+    n = 6000
+    cases = [
+        dict(),
+        dict(intensity_log0="true", color_log0="true", color_asinh1="true",
+             color_is_vector1="true"),
+        dict(intensity_log1="T", color_log2="true", color_is_vector2="true", color_min0=0.2,
+             color_max0=3.5, intensity_max2=5.0),
+        dict(color_asinh0="true", color_is_vector0="true", intensity_min1=-1.0,
+             intensity_max1=-1.0, color_log2="true"),
+    ]
+    out = dict(ncases=np.array([len(cases)]), ntypes=np.array([3]))
+    for k, kv in enumerate(cases):
+        t = rng.integers(0, 3, n).astype(np.uint8)
+        d = dict(I=rng.lognormal(0, 2, n).astype(np.float32),
+                 r=rng.lognormal(-1, 0.5, n).astype(np.float32),
+                 er=rng.lognormal(0, 1.5, n).astype(np.float32),
+                 eg=rng.lognormal(0, 1, n).astype(np.float32),
+                 eb=rng.uniform(-1, 3, n).astype(np.float32), type=t)
+        d["I"][:40] = -rng.uniform(0, 1, 40).astype(np.float32)
+        d["I"][40:45] = 0.0
+        d["er"][50:90] = 0.0
+        d["er"][90:100] = -2.0
+        d["eg"][100] = np.nan
+        g, rr = O.ref_normalize(d, kv, 3)
+        out["nm%d_kv" % k] = np.array(O.kv_text(kv))
+        for q in ("I", "r", "er", "eg", "eb", "type"):
+            out["nm%d_%s" % (k, q)] = d[q]
+        for q in ("I", "er", "eg", "eb"):
+            out["nm%d_out_%s" % (k, q)] = g[q]
+        out["nm%d_ranges" % k] = rr
+    save("normalize", **out)
+    manifest["normalize"] = ("particle_normalize of %d fields of %d particles, 3 types (logs, "
+                             "asinh, vector colours, overrides)" % (len(cases), n))
+
+
 if __name__ == "__main__":
-    if sys.argv[1:] == ["postprocess"]:
+    parts = {"postprocess": postprocess_fixture, "normalize": normalize_fixture}
+    if sys.argv[1:] and all(a in parts for a in sys.argv[1:]):
         with open(os.path.join(HERE, "MANIFEST.json")) as fh:
             man = json.load(fh)
-        postprocess_fixture(man)
+        for a in sys.argv[1:]:
+            parts[a](man)
         write_manifest(man)
     else:
         main()

@@ -71,3 +71,12 @@ def pp_case(f, pal, k):
     pal3 = (f["pal3_x"], f["pal3_rgb"])
     pals = [pal, pal3, pal3][:int(kv.get("ptypes", 1))]
     return f["pp%d_in" % k], kv, pals, f["pp%d_out" % k], f["pp%d_tga" % k]
+
+
+def nm_case(f, k):
+    """Case k of the normalize fixture: (kv dict, input fields, expected
+    output fields, range parameters float64[ntypes, 4])."""
+    kv = parse_kv(f["nm%d_kv" % k])
+    fields = {q: f["nm%d_%s" % (k, q)] for q in ("I", "r", "er", "eg", "eb", "type")}
+    want = {q: f["nm%d_out_%s" % (k, q)] for q in ("I", "er", "eg", "eb")}
+    return kv, fields, want, f["nm%d_ranges" % k]

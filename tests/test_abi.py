@@ -113,3 +113,18 @@ def test_post_from_par_defaults_and_keys(golden):
         assert list(q.col_vector[:q.ptypes]) == want["col_vector"]
     with pytest.raises(S.SplhipError):
         S.post_from_par("colorbar=2")
+
+
+def test_norm_from_par_keys():
+    """The particle_normalize keys (scenemaker.cc:51-55, 223-233)."""
+    q = S.norm_from_par("")
+    assert q.ptypes == 1 and q.norm_intensity == 1
+    assert not q.intensity_log[0] and not q.color_log[0] and not q.has_color_min[0]
+    q = S.norm_from_par("ptypes=3\nintensity_log1=T\ncolor_log2=true\ncolor_asinh0=.true.\n"
+                        "color_is_vector2=t\ncolor_min0=0.25\nintensity_max2=5\n")
+    assert q.ptypes == 3 and q.intensity_log[1] and q.color_log[2] and q.color_asinh[0]
+    assert q.col_vector[2] and not q.col_vector[0]
+    assert q.has_color_min[0] and q.color_min[0] == 0.25 and not q.has_color_max[0]
+    assert q.has_intensity_max[2] and q.intensity_max[2] == 5.0
+    with pytest.raises(S.SplhipError):
+        S.norm_from_par("intensity_log0=maybe")

@@ -165,3 +165,39 @@ def test_postprocess_closed_forms(golden):
     codes = tga[18::3]
     want = list(range(256)) + [255, 255, 0, 0, 0, 0, 0, 0]
     assert codes.tolist() == want
+
+
+def test_normalize_fixture(golden):
+    """particle_normalize: the oracle against the genuine code, bit for bit
+    (logs through glibc's log10f, asinh in double, NaN never in a range), and
+    the ranges it records (the parameters hold them with 8 digits)."""
+    from helpers import nm_case
+    f = golden("normalize")
+    nt = int(f["ntypes"][0])
+    for k in range(int(f["ncases"][0])):
+        kv, fields, want, ranges = nm_case(f, k)
+        got, out, _ = O.normalize(fields, kv, nt)
+        for q in ("I", "er", "eg", "eb"):
+            assert_bitexact(got[q], want[q], "case %d field %s" % (k, q))
+        assert np.allclose(out[:, :4], ranges, rtol=1e-7, atol=0), (k, out, ranges)
+
+
+def test_normalize_closed_forms():
+    """Per type the smallest value maps to 0 and the largest to 1; a type with
+    one value maps to 1; log10 of powers of ten is exact; non-positive
+    values under a log become -38 and stay out of the range."""
+    I = np.array([2.0, 5.0, 3.0, 7.0, 7.0, 1e-3, 1.0, 1e3, -4.0, 0.0], np.float32)
+    er = np.array([0.5, 0.25, 1.0, 3.0, 3.0, 1.0, 10.0, 100.0, 0.0, 5.0], np.float32)
+    typ = np.array([0, 0, 0, 1, 1, 2, 2, 2, 2, 2], np.uint8)
+    z = np.zeros_like(I)
+    f = dict(I=I, r=np.ones_like(I), er=er, eg=z, eb=z, type=typ)
+    kv = dict(intensity_log2="true", color_log2="true")
+    got, out, _ = O.normalize(f, kv, 3)
+    assert got["I"][0] == 0 and got["I"][3 - 2] == 1 and got["I"][2] == np.float32(1 / 3)
+    assert got["er"][1] == 0 and got["er"][2] == 1
+    assert got["I"][3] == 1 and got["I"][4] == 1 and got["er"][3] == 1
+    assert out[2, 4] == -3 and out[2, 5] == 3          # log10 of 1e-3 and 1e3
+    assert out[2, 0] == 0 and out[2, 1] == 2           # log10 of 1 and 100 (0 excluded)
+    assert got["I"][5] == 0 and got["I"][7] == 1 and got["I"][6] == 0.5
+    assert got["I"][8] == 0 and got["I"][9] == 0       # -38 clamps to the range minimum
+    assert got["er"][8] == 0
