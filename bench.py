@@ -32,6 +32,10 @@ The JSON line also carries:
                 TGA byte order; splhip_postprocess) as one kernel: algorithmic
                 bytes 12 B read + 3 B written per pixel over its event time,
                 and the host copy of the 8-bit frame vs the float frame;
+  normalize     particle_normalize of the frame's 16M particles (two passes:
+                transform + per-type ranges, then clamp), bytes 13 B read in
+                the first and 9 B read + 8 B written in the second per
+                particle, over its event time;
   cpu_baseline  the genuine reference host renderer (oracle/_ref, built from
                 the Splotch sources) — or the oracle restatement if that
                 library is absent — timed on this host's cores on a bounded
@@ -268,6 +272,26 @@ def bench_epilogue(S, R, torch, pic, W, H, reps=20):
             "achieved_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1), "peak_GBs": HBM_PEAK_GBS,
             "frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "d2h_ms_rgb8": round(d2h(rgb8), 4), "d2h_ms_float": round(d2h(pic), 4)}
+
+
+def bench_normalize(S, R, torch, dev, reps=10):
+    d = {k: dev[k].clone() for k in ("inten", "col", "hsml", "type")}
+    n = int(d["inten"].numel())
+    q = S.norm_from_par("ptypes=2\n")
+    R.normalize(d, q)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        R.normalize(d, q, ranges=False)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nbytes = 30 * n
+    return {"kernels": "k_norm_reduce + k_norm_clamp", "particles": n, "ms": round(ms, 4),
+            "bytes": nbytes, "achieved_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+            "peak_GBs": HBM_PEAK_GBS, "frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
 
 def bench_large_share(S, torch, args):
@@ -528,6 +552,7 @@ def main():
         "valu": valu,
     }
     out["epilogue"] = bench_epilogue(S, R, torch, pic, W, H)
+    out["normalize"] = bench_normalize(S, R, torch, dev)
     if not args.no_batched:
         out["batched"] = bench_batched(S, R, torch, max(2, args.steps // 2), 1)
     if not args.no_backward:

@@ -372,19 +372,22 @@ class Renderer:
             self._stream(stream)), "splhip_postprocess")
         return out, rgb8
 
-    def normalize(self, dev: dict, q: Norm, stream=None):
+    def normalize(self, dev: dict, q: Norm, stream=None, ranges: bool = True):
         """particle_normalize in place on device fields (dev: inten, col, c1,
         c2, hsml, type as cuda tensors).  Returns the per-type ranges (list of
-        dicts)."""
+        dicts; the call then waits for the stream), or None with
+        ranges=False."""
         n = int(dev["x"].numel()) if "x" in dev else int(dev["inten"].numel())
 
         def ptr(k):
             t = dev.get(k)
             return C.c_void_p(t.data_ptr() if t is not None else None)
-        rr = (NormRanges * MAX_TYPES)()
+        rr = (NormRanges * MAX_TYPES)() if ranges else None
         self._check(lib().splhip_normalize(
             self._c, C.c_int64(n), ptr("inten"), ptr("col"), ptr("c1"), ptr("c2"), ptr("hsml"),
             ptr("type"), C.byref(q), rr, self._stream(stream)), "splhip_normalize")
+        if not ranges:
+            return None
         return [{k: getattr(rr[t], k) for k, _ in NormRanges._fields_} for t in range(q.ptypes)]
 
     def render_batched(self, views, stream=None):
