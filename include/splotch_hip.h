@@ -75,7 +75,7 @@ typedef struct {
   int32_t hip_chunk_entries;               /* hip_chunk_entries: a single-view pass with more
                                               (block, particle) entries is rendered in chunks
                                               of consecutive render order, each continuing
-                                              the pixels' chains (0 = auto: 2^30) */
+                                              the pixels' chains (0 = auto: 2^31 - 2^20) */
   int32_t hip_autotune;                    /* hip_autotune: 1 = time the splat under a set of
                                               (hip_variant, hip_split, hip_heavy_max) choices
                                               on the first render of a batch shape and reuse

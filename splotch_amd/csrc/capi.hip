@@ -708,8 +708,10 @@ Tuning tuning_of(const splhip_params *prm) {
   // a producer/consumer workgroup occupies a CU: leave the splat8 items
   // about a quarter of the CUs while the longest lists run
   t.heavy_max = prm->hip_heavy_max > 0 ? static_cast<uint32_t>(prm->hip_heavy_max) : 192u;
+  // the largest int32 key value in whole MiB: fewer chunks, each a full
+  // pipeline with a host sync and a splat tail; offsets stay below 2^31
   t.chunk = prm->hip_chunk_entries > 0 ? static_cast<uint64_t>(prm->hip_chunk_entries)
-                                       : (uint64_t(1) << 30);
+                                       : (uint64_t(1) << 31) - (uint64_t(1) << 20);
   return t;
 }
 
