@@ -36,6 +36,10 @@ The JSON line also carries:
                 transform + per-type ranges, then clamp), bytes 13 B read in
                 the first and 9 B read + 8 B written in the second per
                 particle, over its event time;
+  ingest        the frame's 16M particles written as a block binary file
+                (bin_reader_block layout: six big-endian float columns) and
+                read into device SoA arrays by splhip_read_bin_block (file in
+                the page cache): bytes of the file per second;
   cpu_baseline  the genuine reference host renderer (oracle/_ref, built from
                 the Splotch sources) — or the oracle restatement if that
                 library is absent — timed on this host's cores on a bounded
@@ -292,6 +296,29 @@ def bench_normalize(S, R, torch, dev, reps=10):
     return {"kernels": "k_norm_reduce + k_norm_clamp", "particles": n, "ms": round(ms, 4),
             "bytes": nbytes, "achieved_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1),
             "peak_GBs": HBM_PEAK_GBS, "frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
+def bench_ingest(S, R, torch, field, reps=3):
+    import tempfile
+    import numpy as np
+    cols = [field.x, field.y, field.z, field.hsml, field.inten, field.col]
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "particles.bin")
+        with open(path, "wb") as fh:
+            for c in cols:
+                np.ascontiguousarray(c, ">f4").tofile(fh)
+        text = "infile=%s\nnum_blocks=6\nx=1\ny=2\nz=3\nr=4\nI=5\nC1=6\n" % path
+        R.read_bin_block(text)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            R.read_bin_block(text)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        nbytes = os.path.getsize(path)
+    return {"particles": field.n, "file_bytes": nbytes, "ms": round(ms, 2),
+            "GBs": round(nbytes / (ms * 1e-3) / 1e9, 2),
+            "note": "host file read (page cache) + PCIe copy + byte swap on the GPU"}
 
 
 def bench_large_share(S, torch, args):
@@ -553,6 +580,7 @@ def main():
     }
     out["epilogue"] = bench_epilogue(S, R, torch, pic, W, H)
     out["normalize"] = bench_normalize(S, R, torch, dev)
+    out["ingest"] = bench_ingest(S, R, torch, field)
     if not args.no_batched:
         out["batched"] = bench_batched(S, R, torch, max(2, args.steps // 2), 1)
     if not args.no_backward:

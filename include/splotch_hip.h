@@ -357,6 +357,25 @@ int splhip_normalize(splhip_ctx *ctx, int64_t n, float *inten, float *col, float
                      const float *hsml, const uint8_t *type, const splhip_norm *norm,
                      splhip_norm_ranges *ranges, void *stream);
 
+/* --- SoA ingest of block binary files ------------------------------------------ */
+
+/* bin_reader_block (reader/bin_reader.cc:34-88,147-190): the file `infile`
+   holds num_blocks float columns of npart values each (npart = file size /
+   (4 num_blocks)); the .par keys x, y, z, r, I, C1, C2, C3 give 1-based
+   column numbers (r, I, C2, C3 optional), swap_endian (default T) swaps each
+   float's bytes, smooth_factor (default 1) scales r.  Missing columns take the
+   reader's defaults: r = smooth_factor, I = 0.5, C1 = 1, C2 = C3 = 0; type 0.
+   splhip_bin_block_count returns npart; splhip_read_bin_block reads the
+   columns straight into the caller's device arrays (any may be NULL to skip
+   it; n must equal npart), through pinned staging, and fixes byte order and
+   scale on the GPU.  The whole file is one rank's share (the reference's MPI
+   split is calcShare over ranks).  Stream-ordered; the call returns when the
+   file has been read and the copies queued. */
+int splhip_bin_block_count(const char *par_text, int64_t *npart);
+int splhip_read_bin_block(splhip_ctx *ctx, const char *par_text, int64_t n, float *x, float *y,
+                          float *z, float *hsml, float *inten, float *col, float *c1, float *c2,
+                          uint8_t *type, void *stream);
+
 /* Host-built tables (device copies are used by the kernels): tab[0..1023] =
    tab1, tab[1024..2047] = tab2; consts = {expfac, taylorlimit, h2sigma, bfak,
    rfac}.  Host only. */

@@ -341,6 +341,58 @@ def ref_postprocess(pic, palettes, params: dict, tga_path):
     return out, tga
 
 
+BIN_FIELDS = ("x", "y", "z", "r", "I", "er", "eg", "eb")
+
+
+def read_bin_block(kv: dict):
+    """Restatement of bin_reader_block (reader/bin_reader.cc:34-88,147-190) in
+    numpy: column q of the file (1-based numbers in kv) holds npart floats,
+    npart = file size // (4 num_blocks); swap_endian (default true) reads them
+    big-endian; r = smooth_factor * value (float product; a NaN comes out
+    quiet with its payload) or smooth_factor; I = value or 0.5; C1 = value or
+    1; C2, C3 = value or 0 (bit copies: 1.0 * value folds to a copy); type 0.
+    Returns a dict of float32 arrays (+ type uint8)."""
+    truth = ("t", "y", "true", ".true.")
+    nfields = int(kv["num_blocks"])
+    swap = str(kv.get("swap_endian", "true")).lower() in truth
+    smooth = np.float32(float(kv.get("smooth_factor", 1.0)))
+    raw = np.fromfile(kv["infile"], dtype=">u4" if swap else "<u4")
+    npart = raw.size // nfields
+    keys = ("x", "y", "z", "r", "I", "C1", "C2", "C3")
+    noval = (0.0, 0.0, 0.0, smooth, 0.5, 1.0, 0.0, 0.0)
+    out = {}
+    for q, (key, name) in enumerate(zip(keys, BIN_FIELDS)):
+        if key in kv:
+            c = int(kv[key]) - 1
+            b = raw[c * npart:(c + 1) * npart].astype(np.uint32)
+            v = b.view(np.float32)
+            if q == 3:
+                nan = np.isnan(v)
+                quiet = (b | np.uint32(0x00400000)).view(np.float32)
+                with np.errstate(all="ignore"):
+                    v = np.where(nan, quiet, (smooth * v).astype(np.float32))
+            out[name] = np.ascontiguousarray(v, np.float32)
+        else:
+            out[name] = np.full(npart, noval[q], np.float32)
+    out["type"] = np.zeros(npart, np.uint8)
+    return out
+
+
+def ref_read_bin_block(kv: dict, n_cap: int):
+    """The genuine bin_reader_block (ref_bin_block).  Returns a dict as
+    read_bin_block."""
+    out = {k: np.zeros(n_cap, np.float32) for k in BIN_FIELDS}
+    out["type"] = np.zeros(n_cap, np.uint8)
+    n = C.c_int64(0)
+    with open(os.devnull, "w"):
+        rc = ref().ref_bin_block(kv_text(kv).encode(), C.c_int64(n_cap),
+                                 *[_p(out[k]) for k in BIN_FIELDS], _p(out["type"]),
+                                 C.byref(n))
+    if rc != 0:
+        raise RuntimeError("reference bin_reader_block failed")
+    return {k: v[:n.value] for k, v in out.items()}
+
+
 NORM_FLAGS = ("intensity_log", "color_log", "color_asinh", "color_is_vector")
 NORM_OVR = ("intensity_min", "intensity_max", "color_min", "color_max")
 

@@ -13,6 +13,7 @@
  *   exptable<float32>              splotch/splotchutils.h:174-215
  *   colour_adjust, add_colorbar    splotch/splotchutils.cc:35-56,230-250
  *   sceneMaker::particle_normalize splotch/scenemaker.cc:40-296
+ *   bin_reader_block               reader/bin_reader.cc:147-190
  *   LS_Image::put_pixel/write_TGA  cxxsupport/ls_image.h:65-90,134-139, ls_image.cc:178-202
  * It is used to write the golden fixtures under tests/golden/ (in the build
  * container) and as the "reference" CPU baseline in bench.py.
@@ -47,6 +48,7 @@
 #define private public
 #include "splotch/scenemaker.h"
 #undef private
+#include "reader/reader.h"
 
 namespace host_funct {
 // Definitions in splotch_host.cc (the header declares different signatures).
@@ -309,6 +311,28 @@ int ref_postprocess(const char *kv, int xres, int yres, float *pic, int ntypes,
     if (bytes.size() != 18 + size_t(3) * xres * yres) return -1;
     std::memcpy(tga_out, bytes.data(), bytes.size());
     std::remove(tga_path);
+  } catch (...) {
+    return -1;
+  }
+  return 0;
+}
+
+/* The genuine bin_reader_block over the .par keys kv (infile, num_blocks,
+   x ... C3, swap_endian, smooth_factor): the particles it returns, as SoA
+   (n_cap entries at most; *n = the count).  Returns 0/-1. */
+int ref_bin_block(const char *kv, int64_t n_cap, float *x, float *y, float *z, float *r, float *I,
+                  float *er, float *eg, float *eb, uint8_t *type, int64_t *n) {
+  try {
+    paramfile params(parse_kv(kv), false);
+    std::vector<particle_sim> p;
+    bin_reader_block(params, p);
+    if ((int64_t)p.size() > n_cap) return -1;
+    *n = (int64_t)p.size();
+    for (size_t i = 0; i < p.size(); ++i) {
+      x[i] = p[i].x; y[i] = p[i].y; z[i] = p[i].z; r[i] = p[i].r; I[i] = p[i].I;
+      er[i] = p[i].e.r; eg[i] = p[i].e.g; eb[i] = p[i].e.b;
+      type[i] = (uint8_t)p[i].type;
+    }
   } catch (...) {
     return -1;
   }

@@ -99,7 +99,8 @@ EXPORTS = ["splhip_create", "splhip_destroy", "splhip_last_error", "splhip_camer
            "splhip_exptable", "splhip_set_profiling", "splhip_stage_times", "splhip_set_trace",
            "splhip_trace_items", "splhip_render_batched", "splhip_render_backward",
            "splhip_last_tuning", "splhip_last_binning", "splhip_post_from_par",
-           "splhip_postprocess", "splhip_norm_from_par", "splhip_normalize"]
+           "splhip_postprocess", "splhip_norm_from_par", "splhip_normalize",
+           "splhip_bin_block_count", "splhip_read_bin_block"]
 
 STAGES = ("project", "ordered_sort", "ordered_bin", "ordered_splat", "additive_compact",
           "additive_bin", "additive_splat", "ordered_k_splat8", "ordered_k_splat_heavy",
@@ -248,6 +249,14 @@ def norm_from_par(text: str) -> Norm:
     return q
 
 
+def bin_block_count(text: str) -> int:
+    """Particles of the block file a .par text names (splhip_bin_block_count)."""
+    n = C.c_int64(0)
+    if lib().splhip_bin_block_count(text.encode(), C.byref(n)) != 0:
+        raise SplhipError("bad block-file keys or unreadable file")
+    return n.value
+
+
 def tga_header(xres: int, yres: int) -> bytes:
     """The 18-byte header LS_Image::write_TGA writes (ls_image.cc:183-186)."""
     return bytes([0, 0, 2, 0, 0, 0, 0, 0, 0, 0, 0, 0, xres % 256, xres // 256, yres % 256,
@@ -389,6 +398,22 @@ class Renderer:
         if not ranges:
             return None
         return [{k: getattr(rr[t], k) for k, _ in NormRanges._fields_} for t in range(q.ptypes)]
+
+    def read_bin_block(self, text: str, stream=None):
+        """bin_reader_block into new device tensors: dict x, y, z, hsml,
+        inten, col, c1, c2 (float32) and type (uint8), the SoA the render
+        takes."""
+        import torch
+        n = bin_block_count(text)
+        d = {k: torch.empty(n, dtype=torch.float32, device="cuda")
+             for k in ("x", "y", "z", "hsml", "inten", "col", "c1", "c2")}
+        d["type"] = torch.empty(n, dtype=torch.uint8, device="cuda")
+        self._check(lib().splhip_read_bin_block(
+            self._c, text.encode(), C.c_int64(n),
+            *[C.c_void_p(d[k].data_ptr()) for k in ("x", "y", "z", "hsml", "inten", "col", "c1",
+                                                    "c2", "type")],
+            self._stream(stream)), "splhip_read_bin_block")
+        return d
 
     def render_batched(self, views, stream=None):
         """views: list of (dev, cam, prm, pic) as for render(); all images in

@@ -71,6 +71,9 @@ struct splhip_ctx {
   hipStream_t aux2 = nullptr;
   hipEvent_t ev_fork2 = nullptr, ev_join2 = nullptr;
   Workspace ws2;
+  // block-file ingest: two pinned buffers, each freed for reuse by its event
+  char *h_ing[2] = {nullptr, nullptr};
+  hipEvent_t ev_ing[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -1280,6 +1283,10 @@ void splhip_destroy(splhip_ctx *c) {
   for (auto &e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->ev_stage) (void)hipEventDestroy(c->ev_stage);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
+  for (int k = 0; k < 2; ++k) {
+    if (c->h_ing[k]) (void)hipHostFree(c->h_ing[k]);
+    if (c->ev_ing[k]) (void)hipEventDestroy(c->ev_ing[k]);
+  }
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   if (c->aux) (void)hipStreamDestroy(c->aux);
@@ -1588,6 +1595,126 @@ int splhip_normalize(splhip_ctx *c, int64_t n, float *inten, float *col, float *
   } catch (const HipError &e) {
     return hip_fail(c, e);
   }
+  return 0;
+}
+
+namespace {
+
+// bin_reader_prep (reader/bin_reader.cc:34-88): keys and column numbers.
+struct BinBlock {
+  std::string file;
+  int nfields = 0;
+  int col[8];  // 0-based column of x, y, z, r, I, C1, C2, C3, or -1
+  bool swap = true;
+  float smooth = 1.f;
+  int64_t npart = 0;
+};
+
+int bin_block_keys(const char *par_text, BinBlock &b, std::string &err) {
+  try {
+    const auto d = parse_par(par_text);
+    if (!d.count("infile") || !d.count("num_blocks") || !d.count("x") || !d.count("y") ||
+        !d.count("z") || !d.count("C1")) {
+      err = "block file keys infile, num_blocks, x, y, z and C1 are required";
+      return SPLHIP_E_ARG;
+    }
+    b.file = d.at("infile");
+    b.nfields = par_num<int>(d, "num_blocks", 0);
+    b.swap = par_bool(d, "swap_endian", true);
+    b.smooth = par_num<float>(d, "smooth_factor", 1.f);
+    const char *keys[8] = {"x", "y", "z", "r", "I", "C1", "C2", "C3"};
+    if (b.nfields < 1) {
+      err = "num_blocks must be positive";
+      return SPLHIP_E_ARG;
+    }
+    for (int q = 0; q < 8; ++q) {
+      b.col[q] = d.count(keys[q]) ? par_num<int>(d, keys[q], 0) - 1 : -1;
+      if (d.count(keys[q]) && (b.col[q] < 0 || b.col[q] >= b.nfields)) {
+        err = std::string("column of ") + keys[q] + " outside 1..num_blocks";
+        return SPLHIP_E_ARG;
+      }
+    }
+  } catch (const ParError &e) {
+    err = e.msg;
+    return SPLHIP_E_ARG;
+  }
+  FILE *f = std::fopen(b.file.c_str(), "rb");
+  if (!f) {
+    err = "could not open input file '" + b.file + "'";
+    return SPLHIP_E_ARG;
+  }
+  std::fseek(f, 0, SEEK_END);
+  const int64_t size = static_cast<int64_t>(std::ftell(f));
+  std::fclose(f);
+  b.npart = size / (4 * static_cast<int64_t>(b.nfields));
+  return 0;
+}
+
+}  // namespace
+
+int splhip_bin_block_count(const char *par_text, int64_t *npart) {
+  if (!npart) return SPLHIP_E_ARG;
+  BinBlock b;
+  std::string err;
+  if (int rc = bin_block_keys(par_text, b, err)) return rc;
+  *npart = b.npart;
+  return 0;
+}
+
+int splhip_read_bin_block(splhip_ctx *c, const char *par_text, int64_t n, float *x, float *y,
+                          float *z, float *hsml, float *inten, float *col, float *c1, float *c2,
+                          uint8_t *type, void *stream) {
+  if (!c) return SPLHIP_E_ARG;
+  BinBlock b;
+  std::string err;
+  if (int rc = bin_block_keys(par_text, b, err)) return fail(c, rc, err.c_str());
+  if (n != b.npart) return fail(c, SPLHIP_E_ARG, "n differs from the file's particle count");
+  FILE *f = std::fopen(b.file.c_str(), "rb");
+  if (!f) return fail(c, SPLHIP_E_ARG, "could not open input file");
+  try {
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    constexpr int64_t kChunk = int64_t(1) << 22;  // floats per staging buffer (16 MiB)
+    for (int k = 0; k < 2; ++k)
+      if (!c->h_ing[k]) {
+        hip_check(hipHostMalloc(reinterpret_cast<void **>(&c->h_ing[k]), kChunk * 4,
+                                hipHostMallocDefault),
+                  "hipHostMalloc ingest");
+        hip_check(hipEventCreateWithFlags(&c->ev_ing[k], hipEventDisableTiming),
+                  "hipEventCreate");
+      }
+    float *dst[8] = {x, y, z, hsml, inten, col, c1, c2};
+    // the reader's values for a missing column (CASEMACRO__ noval)
+    const float noval[8] = {0.f, 0.f, 0.f, b.smooth, 0.5f, 1.f, 0.f, 0.f};
+    int k = 0;
+    for (int q = 0; q < 8; ++q) {
+      if (!dst[q] || n == 0) continue;
+      if (b.col[q] < 0) {
+        launch_fill(dst[q], noval[q], n, s);
+        continue;
+      }
+      const int64_t base = static_cast<int64_t>(b.col[q]) * b.npart;
+      for (int64_t i0 = 0; i0 < n; i0 += kChunk) {
+        const int64_t cnt = std::min<int64_t>(kChunk, n - i0);
+        hip_check(hipEventSynchronize(c->ev_ing[k]), "wait ingest buffer");
+        if (std::fseek(f, static_cast<long>(4 * (base + i0)), SEEK_SET) != 0 ||
+            std::fread(c->h_ing[k], 4, static_cast<size_t>(cnt), f) != static_cast<size_t>(cnt)) {
+          std::fclose(f);
+          return fail(c, SPLHIP_E_ARG, "short read of the input file");
+        }
+        hip_check(hipMemcpyAsync(dst[q] + i0, c->h_ing[k], cnt * 4, hipMemcpyHostToDevice, s),
+                  "H2D column");
+        hip_check(hipEventRecord(c->ev_ing[k], s), "hipEventRecord");
+        k ^= 1;
+      }
+      launch_ingest_fix(dst[q], n, b.swap ? 1 : 0, q == 3 ? 1 : 0, b.smooth, s);
+    }
+    if (type && n > 0) launch_fill_u8(type, 0, n, s);
+  } catch (const HipError &e) {
+    std::fclose(f);
+    return hip_fail(c, e);
+  }
+  std::fclose(f);
   return 0;
 }
 

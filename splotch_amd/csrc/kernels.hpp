@@ -185,6 +185,11 @@ struct NormArgs {
   int32_t ovr_set[kMaxTypes][4];
   float ovr[kMaxTypes][4];
 };
+// Block-file ingest (ingest.hip): in place, byte swap when swap, then
+// scale * v when scaled (else a multiply by one: only NaNs are quieted).
+void launch_ingest_fix(float *d, int64_t n, int swap, int scaled, float scale, hipStream_t s);
+void launch_fill_u8(uint8_t *p, uint8_t v, int64_t n, hipStream_t s);
+
 // keys: 6 ntypes u32 of scratch; after the call they hold the per-type
 // extremes as order-preserving keys (colour, intensity, size; min then max).
 void launch_normalize(const NormArgs &a, uint32_t *keys, hipStream_t s);

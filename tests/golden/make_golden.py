@@ -14,12 +14,15 @@ returns:
   * the master's post-processing and TGA output (splotch.cc:272-310):
     -expm1, colour_adjust, add_colorbar, Colour8 quantization, write_TGA;
   * sceneMaker::particle_normalize (splotch/scenemaker.cc:40-296): logs,
-    asinh, per-type ranges, overrides and clamping.
+    asinh, per-type ranges, overrides and clamping;
+  * bin_reader_block (reader/bin_reader.cc:147-190): block binary files,
+    both byte orders, missing columns, smooth_factor, NaN payloads.
 Fixtures are inputs + outputs only (numpy .npz, no pickles).
 
     python tests/golden/make_golden.py                 # every fixture
     python tests/golden/make_golden.py postprocess     # only postprocess.npz
     python tests/golden/make_golden.py normalize       # only normalize.npz
+    python tests/golden/make_golden.py ingest          # only ingest.npz
 """
 from __future__ import annotations
 
@@ -189,6 +192,7 @@ def main():
 
     postprocess_fixture(manifest)
     normalize_fixture(manifest)
+    ingest_fixture(manifest)
 
     # palette used by every fixture (this project's own ramp)
     np.savez_compressed(os.path.join(HERE, "palette.npz"), x=pal[0], rgb=pal[1])
@@ -280,8 +284,44 @@ def normalize_fixture(manifest):
                              "asinh, vector colours, overrides)" % (len(cases), n))
 
 
+def ingest_fixture(manifest):
+    """Block binary files (9 columns of 2000 floats with NaN / inf / denormal /
+    signalling-NaN values) in both byte orders through the genuine
+    bin_reader_block: case k stores bb{k}_file (the file's bytes), bb{k}_kv
+    (.par keys; infile is replaced by the test's path) and bb{k}_{x,...,type}."""
+    import tempfile
+    rng = np.random.default_rng(43)
+    n, nf = 2000, 9
+    bits = rng.normal(0, 10, (nf, n)).astype(np.float32).view(np.uint32).copy()
+    bits[0, :5] = np.array([np.nan, np.inf, -np.inf, 1e-40, -0.0], np.float32).view(np.uint32)
+    bits[1, 7] = 0x7f800001   # signalling NaN
+    bits[3, 9] = 0xff812345   # signalling NaN with a payload, negative
+    vals = bits.view(np.float32)
+    cases = [
+        (True, dict(x=1, y=2, z=3, r=4, I=5, C1=6, C2=7, C3=8, smooth_factor=1.7)),
+        (True, dict(x=9, y=2, z=1, C1=4)),
+        (False, dict(x=2, y=3, z=4, r=4, C1=1, C3=5, smooth_factor=0.3)),
+    ]
+    out = dict(ncases=np.array([len(cases)]))
+    with tempfile.TemporaryDirectory() as d:
+        for k, (swap, kv) in enumerate(cases):
+            path = os.path.join(d, "block%d.bin" % k)
+            vals.astype(">f4" if swap else "<f4").tofile(path)
+            kv = dict(kv, num_blocks=nf, swap_endian="T" if swap else "F")
+            g = O.ref_read_bin_block(dict(kv, infile=path), n + 1)
+            out["bb%d_file" % k] = np.fromfile(path, np.uint8)
+            out["bb%d_kv" % k] = np.array(O.kv_text(kv))
+            for q, v in g.items():
+                out["bb%d_%s" % (k, q)] = v
+    save("ingest", **out)
+    manifest["ingest"] = ("bin_reader_block of %d block files (%d particles, %d columns; both "
+                          "byte orders, missing columns, smooth_factor, NaN payloads)"
+                          % (len(cases), n, nf))
+
+
 if __name__ == "__main__":
-    parts = {"postprocess": postprocess_fixture, "normalize": normalize_fixture}
+    parts = {"postprocess": postprocess_fixture, "normalize": normalize_fixture,
+             "ingest": ingest_fixture}
     if sys.argv[1:] and all(a in parts for a in sys.argv[1:]):
         with open(os.path.join(HERE, "MANIFEST.json")) as fh:
             man = json.load(fh)

@@ -80,3 +80,12 @@ def nm_case(f, k):
     fields = {q: f["nm%d_%s" % (k, q)] for q in ("I", "r", "er", "eg", "eb", "type")}
     want = {q: f["nm%d_out_%s" % (k, q)] for q in ("I", "er", "eg", "eb")}
     return kv, fields, want, f["nm%d_ranges" % k]
+
+
+def bb_case(f, k, path):
+    """Case k of the ingest fixture, its file written to path: (kv dict with
+    infile = path, expected fields)."""
+    f["bb%d_file" % k].tofile(path)
+    kv = dict(parse_kv(f["bb%d_kv" % k]), infile=str(path))
+    want = {q: f["bb%d_%s" % (k, q)] for q in ("x", "y", "z", "r", "I", "er", "eg", "eb", "type")}
+    return kv, want

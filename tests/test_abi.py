@@ -128,3 +128,16 @@ def test_norm_from_par_keys():
     assert q.has_intensity_max[2] and q.intensity_max[2] == 5.0
     with pytest.raises(S.SplhipError):
         S.norm_from_par("intensity_log0=maybe")
+
+
+def test_bin_block_count(golden, tmp_path):
+    """The particle count of a block file and the key checks (host only)."""
+    from helpers import bb_case
+    f = golden("ingest")
+    kv, want = bb_case(f, 0, tmp_path / "b.bin")
+    import pyoracle as O
+    assert S.bin_block_count(O.kv_text(kv)) == want["x"].size
+    with pytest.raises(S.SplhipError):   # a column past num_blocks
+        S.bin_block_count(O.kv_text(dict(kv, C3=99)))
+    with pytest.raises(S.SplhipError):   # no such file
+        S.bin_block_count(O.kv_text(dict(kv, infile=str(tmp_path / "missing.bin"))))

@@ -201,3 +201,16 @@ def test_normalize_closed_forms():
     assert got["I"][5] == 0 and got["I"][7] == 1 and got["I"][6] == 0.5
     assert got["I"][8] == 0 and got["I"][9] == 0       # -38 clamps to the range minimum
     assert got["er"][8] == 0
+
+
+def test_bin_block_fixture(golden, tmp_path):
+    """bin_reader_block: the numpy restatement against the genuine reader, bit
+    for bit (byte orders, defaults of missing columns, smooth_factor, the
+    NaN payloads of the r column quieted, the others copied)."""
+    from helpers import bb_case
+    f = golden("ingest")
+    for k in range(int(f["ncases"][0])):
+        kv, want = bb_case(f, k, tmp_path / ("b%d.bin" % k))
+        got = O.read_bin_block(kv)
+        for q, w in want.items():
+            assert_bitexact(got[q], w, "case %d %s" % (k, q))
