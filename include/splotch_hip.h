@@ -376,6 +376,32 @@ int splhip_read_bin_block(splhip_ctx *ctx, const char *par_text, int64_t n, floa
                           float *z, float *hsml, float *inten, float *col, float *c1, float *c2,
                           uint8_t *type, void *stream);
 
+/* --- scene files: the frames of an animation ----------------------------------- */
+
+/* One scene of a scene file: the camera getNextScene takes from the
+   parameters after applying the scene's keys (scenemaker.cc:1140-1166:
+   camera_*, lookat_*, sky_* (default 0, 0, 1), center_* (default the camera
+   position)), its fidx (default the .par fidx, else 0), keep_particles (the
+   next scene has the same fidx within 1e-5 relative, so its particles are
+   reused: scenemaker.cc:656-663) and line, the 0-based line of scene_text the
+   scene came from (for callers that apply its other keys). */
+typedef struct {
+  double campos[3], lookat[3], sky[3], center[3];
+  double fidx;
+  int32_t keep_particles;
+  int32_t line;
+} splhip_scene;
+
+/* The scene list sceneMaker builds from scene_text (scenemaker.cc:599-673):
+   a header line of parameter names (or, when the first line holds ten
+   numbers, the old format: camera_x camera_y camera_z lookat_x lookat_y
+   lookat_z sky_x sky_y sky_z fidx, with that line as data), then one line of
+   values per scene, from line scene_start (par key, default 0) in steps of
+   scene_incr (1) while the scene number is below last_scene (1000000).
+   Scenes after the capacity are counted in *n but not written.  Host only. */
+int splhip_scene_file_parse(const char *par_text, const char *scene_text, splhip_scene *out,
+                            int32_t cap, int32_t *n);
+
 /* Host-built tables (device copies are used by the kernels): tab[0..1023] =
    tab1, tab[1024..2047] = tab2; consts = {expfac, taylorlimit, h2sigma, bfak,
    rfac}.  Host only. */

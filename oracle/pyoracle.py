@@ -393,6 +393,21 @@ def ref_read_bin_block(kv: dict, n_cap: int):
     return {k: v[:n.value] for k, v in out.items()}
 
 
+def ref_scene_file(kv: dict, cap: int = 4096):
+    """The genuine sceneMaker scene list (ref_scene_file): per scene a dict
+    campos, lookat, sky, center (3-tuples), fidx, keep_particles."""
+    out = np.zeros((cap, 16), np.float64)
+    n = C.c_int(0)
+    rc = ref().ref_scene_file(kv_text(kv).encode(), _p(out), C.c_int(cap), C.byref(n))
+    if rc != 0:
+        raise RuntimeError("reference sceneMaker failed")
+    res = []
+    for o in out[:min(n.value, cap)]:
+        res.append(dict(campos=tuple(o[0:3]), lookat=tuple(o[3:6]), sky=tuple(o[6:9]),
+                        center=tuple(o[9:12]), fidx=float(o[12]), keep_particles=bool(o[13])))
+    return res
+
+
 NORM_FLAGS = ("intensity_log", "color_log", "color_asinh", "color_is_vector")
 NORM_OVR = ("intensity_min", "intensity_max", "color_min", "color_max")
 

@@ -14,6 +14,7 @@
  *   colour_adjust, add_colorbar    splotch/splotchutils.cc:35-56,230-250
  *   sceneMaker::particle_normalize splotch/scenemaker.cc:40-296
  *   bin_reader_block               reader/bin_reader.cc:147-190
+ *   sceneMaker scene list          splotch/scenemaker.cc:586-673
  *   LS_Image::put_pixel/write_TGA  cxxsupport/ls_image.h:65-90,134-139, ls_image.cc:178-202
  * It is used to write the golden fixtures under tests/golden/ (in the build
  * container) and as the "reference" CPU baseline in bench.py.
@@ -311,6 +312,44 @@ int ref_postprocess(const char *kv, int xres, int yres, float *pic, int ntypes,
     if (bytes.size() != 18 + size_t(3) * xres * yres) return -1;
     std::memcpy(tga_out, bytes.data(), bytes.size());
     std::remove(tga_path);
+  } catch (...) {
+    return -1;
+  }
+  return 0;
+}
+
+/* The scene list of the genuine sceneMaker constructor for the .par keys kv
+   (scene_file, scene_start ...), with each scene's camera taken as
+   getNextScene takes it (scenemaker.cc:1140-1166: the scene's keys applied to
+   the parameters, which keep them for later scenes).  Per scene, out[16 i ..]
+   = campos, lookat, sky, center, fidx, keep_particles.  *n = the number of
+   scenes (at most cap are written).  Returns 0/-1. */
+int ref_scene_file(const char *kv, double *out, int cap, int *n) {
+  try {
+    auto m = parse_kv(kv);
+    if (!m.count("outfile")) m["outfile"] = "scene_probe";
+    paramfile params(m, false);
+    sceneMaker sm(params);
+    *n = (int)sm.scenes.size();
+    for (int i = 0; i < *n && i < cap; ++i) {
+      const auto sp = sm.scenes[i].sceneParameters.getParams();
+      for (auto it = sp.begin(); it != sp.end(); ++it) params.setParam(it->first, it->second);
+      double *o = out + 16 * i;
+      o[0] = params.find<double>("camera_x"); o[1] = params.find<double>("camera_y");
+      o[2] = params.find<double>("camera_z");
+      o[3] = params.find<double>("lookat_x"); o[4] = params.find<double>("lookat_y");
+      o[5] = params.find<double>("lookat_z");
+      o[6] = params.find<double>("sky_x", 0); o[7] = params.find<double>("sky_y", 0);
+      o[8] = params.find<double>("sky_z", 1);
+      if (params.param_present("center_x")) {
+        o[9] = params.find<double>("center_x"); o[10] = params.find<double>("center_y");
+        o[11] = params.find<double>("center_z");
+      } else {
+        o[9] = o[0]; o[10] = o[1]; o[11] = o[2];
+      }
+      o[12] = params.find<double>("fidx", 0);
+      o[13] = sm.scenes[i].keep_particles ? 1 : 0;
+    }
   } catch (...) {
     return -1;
   }

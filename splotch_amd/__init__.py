@@ -10,8 +10,10 @@ workloads (synth.py).
 from . import synth  # noqa: F401
 from .capi import (Camera, Norm, Params, Post, Renderer, SplhipError,  # noqa: F401
                    bin_block_count, camera_setup, exptable, norm_from_par, params,
-                   params_from_par, post, post_from_par, tga_header, to_device, to_particle_sim)
+                   params_from_par, post, post_from_par, scene_file_parse, scene_views,
+                   tga_header, to_device, to_particle_sim)
 
 __all__ = ["Camera", "Norm", "Params", "Post", "Renderer", "SplhipError", "bin_block_count",
            "camera_setup", "exptable", "norm_from_par", "params", "params_from_par", "post",
-           "post_from_par", "synth", "tga_header", "to_device", "to_particle_sim"]
+           "post_from_par", "scene_file_parse", "scene_views", "synth", "tga_header",
+           "to_device", "to_particle_sim"]

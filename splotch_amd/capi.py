@@ -70,6 +70,13 @@ class NormRanges(C.Structure):
                                          "size_max")]
 
 
+class Scene(C.Structure):
+    """splhip_scene: one scene of a scene file."""
+    _fields_ = [("campos", C.c_double * 3), ("lookat", C.c_double * 3), ("sky", C.c_double * 3),
+                ("center", C.c_double * 3), ("fidx", C.c_double), ("keep_particles", C.c_int32),
+                ("line", C.c_int32)]
+
+
 class Particles(C.Structure):
     _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("hsml", C.c_void_p),
                 ("inten", C.c_void_p), ("col", C.c_void_p), ("c1", C.c_void_p),
@@ -100,7 +107,7 @@ EXPORTS = ["splhip_create", "splhip_destroy", "splhip_last_error", "splhip_camer
            "splhip_trace_items", "splhip_render_batched", "splhip_render_backward",
            "splhip_last_tuning", "splhip_last_binning", "splhip_post_from_par",
            "splhip_postprocess", "splhip_norm_from_par", "splhip_normalize",
-           "splhip_bin_block_count", "splhip_read_bin_block"]
+           "splhip_bin_block_count", "splhip_read_bin_block", "splhip_scene_file_parse"]
 
 STAGES = ("project", "ordered_sort", "ordered_bin", "ordered_splat", "additive_compact",
           "additive_bin", "additive_splat", "ordered_k_splat8", "ordered_k_splat_heavy",
@@ -255,6 +262,50 @@ def bin_block_count(text: str) -> int:
     if lib().splhip_bin_block_count(text.encode(), C.byref(n)) != 0:
         raise SplhipError("bad block-file keys or unreadable file")
     return n.value
+
+
+def scene_file_parse(par_text: str, scene_text: str):
+    """The scenes of a scene file (splhip_scene_file_parse): list of dicts
+    campos, lookat, sky, center, fidx, keep_particles, line."""
+    n = C.c_int32(0)
+    if lib().splhip_scene_file_parse(par_text.encode(), scene_text.encode(), None, 0,
+                                     C.byref(n)) != 0:
+        raise SplhipError("bad scene file or parameters")
+    arr = (Scene * max(n.value, 1))()
+    if lib().splhip_scene_file_parse(par_text.encode(), scene_text.encode(), arr, n.value,
+                                     C.byref(n)) != 0:
+        raise SplhipError("bad scene file or parameters")
+    return [dict(campos=tuple(s.campos), lookat=tuple(s.lookat), sky=tuple(s.sky),
+                 center=tuple(s.center), fidx=s.fidx, keep_particles=bool(s.keep_particles),
+                 line=s.line) for s in arr[:n.value]]
+
+
+def scene_views(par_text: str, scene_text: str, b_brightness: float = 1.0):
+    """Per scene of a scene file: (Camera, Params, fidx, keep_particles), the
+    scene's keys applied over par_text as getNextScene applies them."""
+    import types
+    scenes = scene_file_parse(par_text, scene_text)
+    lines = scene_text.splitlines()
+    first = lines[0].split() if lines else []
+    try:
+        [float(w) for w in first]
+        keys = ("camera_x camera_y camera_z lookat_x lookat_y lookat_z sky_x sky_y sky_z "
+                "fidx").split() if len(first) == 10 else first
+    except ValueError:
+        keys = first
+    text = par_text
+    out = []
+    for s in scenes:
+        vals = lines[s["line"]].split()
+        text = text + "\n" + "\n".join("%s=%s" % kv for kv in zip(keys, vals))
+        prm, sc = params_from_par(text, b_brightness)
+        v = types.SimpleNamespace(campos=s["campos"], lookat=s["lookat"], sky=s["sky"],
+                                  center=s["center"], xres=sc["xres"], yres=sc["yres"],
+                                  fov=sc["fov"])
+        cam = camera_setup(v, zmin=sc["zmin"], zmax=sc["zmax"], projection=sc["projection"],
+                           minrad_pix=sc["minrad_pix"])
+        out.append((cam, prm, s["fidx"], s["keep_particles"]))
+    return out
 
 
 def tga_header(xres: int, yres: int) -> bytes:
@@ -414,6 +465,34 @@ class Renderer:
                                                     "c2", "type")],
             self._stream(stream)), "splhip_read_bin_block")
         return d
+
+    def render_scene_file(self, particles_of, par_text: str, scene_text: str, stream=None):
+        """The frames of a scene file: consecutive scenes that share their
+        particles (keep_particles) go through one splhip_render_batched
+        call.  particles_of(fidx) returns the device fields of a fidx.
+        Returns the images (cuda float32 tensors, x-major RGB)."""
+        import torch
+        frames = scene_views(par_text, scene_text)
+        images, group = [], []
+
+        def flush():
+            if not group:
+                return
+            dev = particles_of(group[0][2])
+            views = []
+            for cam, prm, _, _ in group:
+                pic = torch.empty(cam.xres * cam.yres * 3, dtype=torch.float32, device="cuda")
+                views.append((dev, cam, prm, pic))
+            self.render_batched(views, stream)
+            images.extend(v[3] for v in views)
+            group.clear()
+
+        for f in frames:
+            group.append(f)
+            if not f[3]:  # the next scene loads other particles
+                flush()
+        flush()
+        return images
 
     def render_batched(self, views, stream=None):
         """views: list of (dev, cam, prm, pic) as for render(); all images in

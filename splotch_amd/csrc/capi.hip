@@ -1718,6 +1718,93 @@ int splhip_read_bin_block(splhip_ctx *c, const char *par_text, int64_t n, float 
   return 0;
 }
 
+int splhip_scene_file_parse(const char *par_text, const char *scene_text, splhip_scene *out,
+                            int32_t cap, int32_t *n) {
+  if (!n || (!out && cap > 0)) return SPLHIP_E_ARG;
+  try {
+    auto params = parse_par(par_text);
+    const int start = par_num<int>(params, "scene_start", 0);
+    const int incr = par_num<int>(params, "scene_incr", 1);
+    const int last = par_num<int>(params, "last_scene", 1000000);
+    if (incr < 1) return SPLHIP_E_ARG;
+    std::vector<std::string> lines;
+    {
+      std::istringstream in(scene_text ? scene_text : "");
+      std::string l;
+      while (std::getline(in, l)) lines.push_back(l);
+    }
+    auto split = [](const std::string &l) {
+      std::istringstream s(l);
+      std::vector<std::string> w;
+      std::string x;
+      while (s >> x) w.push_back(x);
+      return w;
+    };
+    size_t pos = 0;
+    std::vector<std::string> keys;
+    if (!lines.empty()) {
+      double t[10];
+      if (std::sscanf(lines[0].c_str(), "%lf %lf %lf %lf %lf %lf %lf %lf %lf %lf", &t[0], &t[1],
+                      &t[2], &t[3], &t[4], &t[5], &t[6], &t[7], &t[8], &t[9]) == 10) {
+        keys = {"camera_x", "camera_y", "camera_z", "lookat_x", "lookat_y", "lookat_z",
+                "sky_x", "sky_y", "sky_z", "fidx"};  // old format: the line is data
+      } else {
+        keys = split(lines[0]);
+        pos = 1;
+      }
+    }
+    pos += static_cast<size_t>(std::max(start, 0));
+    int scene = start, count = 0;
+    double fidx_prev = 0.;
+    const double fidx_par = par_num<double>(params, "fidx", 0.);
+    for (; pos < lines.size() && scene < last; pos += static_cast<size_t>(incr), scene += incr) {
+      const std::vector<std::string> vals = split(lines[pos]);
+      if (vals.size() != keys.size()) return SPLHIP_E_ARG;  // "ERROR in scene file detected"
+      // getNextScene applies the scene's keys to the parameters (they stay
+      // set for later scenes; every scene line sets the same keys)
+      for (size_t i = 0; i < keys.size(); ++i) params[keys[i]] = vals[i];
+      auto num = [&](const char *k, const double *deflt) {
+        if (!params.count(k)) {
+          if (deflt) return *deflt;
+          throw ParError{std::string("missing key ") + k};
+        }
+        return par_num<double>(params, k, 0.);
+      };
+      const double zero = 0., one = 1.;
+      splhip_scene S{};
+      S.campos[0] = num("camera_x", nullptr);
+      S.campos[1] = num("camera_y", nullptr);
+      S.campos[2] = num("camera_z", nullptr);
+      S.lookat[0] = num("lookat_x", nullptr);
+      S.lookat[1] = num("lookat_y", nullptr);
+      S.lookat[2] = num("lookat_z", nullptr);
+      S.sky[0] = num("sky_x", &zero);
+      S.sky[1] = num("sky_y", &zero);
+      S.sky[2] = num("sky_z", &one);
+      if (params.count("center_x")) {
+        S.center[0] = num("center_x", nullptr);
+        S.center[1] = num("center_y", nullptr);
+        S.center[2] = num("center_z", nullptr);
+      } else {
+        std::memcpy(S.center, S.campos, sizeof(S.center));
+      }
+      S.fidx = params.count("fidx") ? par_num<double>(params, "fidx", 0.) : fidx_par;
+      S.line = static_cast<int32_t>(pos);
+      // approx(fidx, fidxold): |a - b| <= 1e-5 |b| (math_utils.h:43-47)
+      if (count > 0 && std::fabs(S.fidx - fidx_prev) <= 1e-5 * std::fabs(fidx_prev) &&
+          count - 1 < cap)
+        out[count - 1].keep_particles = 1;
+      if (count < cap) out[count] = S;
+      fidx_prev = S.fidx;
+      ++count;
+    }
+    *n = count;
+  } catch (const ParError &) {
+    return SPLHIP_E_ARG;
+  }
+  return 0;
+}
+
 int splhip_set_palette(splhip_ctx *c, int32_t type, int32_t n, const double *x,
                        const float *rgb) {
   if (!c) return SPLHIP_E_ARG;

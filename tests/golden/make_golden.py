@@ -16,13 +16,16 @@ returns:
   * sceneMaker::particle_normalize (splotch/scenemaker.cc:40-296): logs,
     asinh, per-type ranges, overrides and clamping;
   * bin_reader_block (reader/bin_reader.cc:147-190): block binary files,
-    both byte orders, missing columns, smooth_factor, NaN payloads.
+    both byte orders, missing columns, smooth_factor, NaN payloads;
+  * the sceneMaker scene list and per-scene cameras of scene files
+    (splotch/scenemaker.cc:586-673, 1140-1166).
 Fixtures are inputs + outputs only (numpy .npz, no pickles).
 
     python tests/golden/make_golden.py                 # every fixture
     python tests/golden/make_golden.py postprocess     # only postprocess.npz
     python tests/golden/make_golden.py normalize       # only normalize.npz
     python tests/golden/make_golden.py ingest          # only ingest.npz
+    python tests/golden/make_golden.py scenes          # only scenes.npz
 """
 from __future__ import annotations
 
@@ -193,6 +196,7 @@ def main():
     postprocess_fixture(manifest)
     normalize_fixture(manifest)
     ingest_fixture(manifest)
+    scenes_fixture(manifest)
 
     # palette used by every fixture (this project's own ramp)
     np.savez_compressed(os.path.join(HERE, "palette.npz"), x=pal[0], rgb=pal[1])
@@ -319,9 +323,56 @@ def ingest_fixture(manifest):
                           % (len(cases), n, nf))
 
 
+def scenes_fixture(manifest):
+    """Scene files through the genuine sceneMaker constructor: case k stores
+    sc{k}_par (.par keys, scene_file added by the test), sc{k}_text (the scene
+    file) and sc{k}_out (per scene: campos, lookat, sky, center, fidx,
+    keep_particles)."""
+    import tempfile
+    base = "camera_x=50\ncamera_y=50\ncamera_z=-150\nlookat_x=50\nlookat_y=50\nlookat_z=50\n"
+    rng = np.random.default_rng(47)
+    rows = []
+    for k in range(12):
+        a = 0.4 * k
+        rows.append("%.6f %.6f %.6f 50 50 50 0 0 1 %d" % (50 + 150 * math.cos(a),
+                                                          50 + 150 * math.sin(a),
+                                                          50 + 10 * rng.normal(), k // 3))
+    old = "\n".join(rows) + "\n"
+    new = ("camera_x camera_y camera_z lookat_z fidx brightness0\n" +
+           "".join("%.5f %.5f %.5f %.3f %s %.2f\n" % (10 * k, 20 - k, -150 + k, 50 + k,
+                                                      ("1.0", "1.000001", "2")[k % 3],
+                                                      0.5 + 0.1 * k) for k in range(9)))
+    cases = [
+        (dict(), old),
+        (dict(scene_start=2, scene_incr=3, last_scene=11), old),
+        (dict(center_x=40, center_y=45, center_z=50, sky_x=0, sky_y=1, sky_z=0, fidx=3), new),
+    ]
+    out = dict(ncases=np.array([len(cases)]))
+    with tempfile.TemporaryDirectory() as d:
+        for k, (kv, text) in enumerate(cases):
+            path = os.path.join(d, "scenes%d.txt" % k)
+            with open(path, "w") as fh:
+                fh.write(text)
+            par = base + O.kv_text(kv)
+            res = O.ref_scene_file(dict(helpers_kv(par), scene_file=path))
+            out["sc%d_par" % k] = np.array(par)
+            out["sc%d_text" % k] = np.array(text)
+            out["sc%d_out" % k] = np.array([list(r["campos"]) + list(r["lookat"]) +
+                                            list(r["sky"]) + list(r["center"]) +
+                                            [r["fidx"], float(r["keep_particles"])]
+                                            for r in res], np.float64)
+    save("scenes", **out)
+    manifest["scenes"] = ("sceneMaker scene lists of %d scene files (old and new formats, "
+                          "scene_start / scene_incr / last_scene, fidx reuse)" % len(cases))
+
+
+def helpers_kv(text):
+    return dict(line.split("=", 1) for line in text.splitlines() if "=" in line)
+
+
 if __name__ == "__main__":
     parts = {"postprocess": postprocess_fixture, "normalize": normalize_fixture,
-             "ingest": ingest_fixture}
+             "ingest": ingest_fixture, "scenes": scenes_fixture}
     if sys.argv[1:] and all(a in parts for a in sys.argv[1:]):
         with open(os.path.join(HERE, "MANIFEST.json")) as fh:
             man = json.load(fh)

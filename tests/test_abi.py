@@ -141,3 +141,20 @@ def test_bin_block_count(golden, tmp_path):
         S.bin_block_count(O.kv_text(dict(kv, C3=99)))
     with pytest.raises(S.SplhipError):   # no such file
         S.bin_block_count(O.kv_text(dict(kv, infile=str(tmp_path / "missing.bin"))))
+
+
+def test_scene_file_parse_fixture(golden):
+    """The scene list and per-scene cameras against the genuine sceneMaker
+    (old and new scene-file formats, scene_start / scene_incr / last_scene,
+    center and sky from the parameters, fidx and its 1e-5 reuse test)."""
+    f = golden("scenes")
+    for k in range(int(f["ncases"][0])):
+        got = S.scene_file_parse(str(f["sc%d_par" % k]), str(f["sc%d_text" % k]))
+        want = f["sc%d_out" % k]
+        assert len(got) == len(want), k
+        for g, w in zip(got, want):
+            row = list(g["campos"]) + list(g["lookat"]) + list(g["sky"]) + list(g["center"]) + \
+                [g["fidx"], float(g["keep_particles"])]
+            assert np.array_equal(np.array(row), w), (k, row, w)
+    with pytest.raises(S.SplhipError):   # a scene line with a missing value
+        S.scene_file_parse(str(f["sc0_par"]), "camera_x camera_y\n1 2\n3\n")

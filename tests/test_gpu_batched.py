@@ -200,3 +200,28 @@ def test_batched_autotune(R):
         torch.cuda.synchronize()
         for j, (a, b) in enumerate(zip(views, ref)):
             assert_bitexact(a[3].cpu().numpy(), b[3].cpu().numpy(), "autotuned job %d" % j)
+
+
+def test_scene_file_frames(R):
+    """The frames of a scene file (12 cameras on an orbit, particles changing
+    every third frame by fidx) rendered group by group through
+    splhip_render_batched: each image equals the plain render of its camera."""
+    import splotch_amd as S
+    rows = []
+    for k in range(12):
+        a = 0.5 * k
+        rows.append("%.5f %.5f %.5f 50 50 50 0 0 1 %d" % (50 + 150 * math.cos(a),
+                                                         50 + 150 * math.sin(a), 60.0, k // 3))
+    text = "\n".join(rows) + "\n"
+    par = "xres=160\nyres=128\nfov=30\na_eq_e=false\n"
+    fields = {f: S.to_device(synth.plummer_field(20_000, 2800 + f, hsml_mu=math.log(0.5)))
+              for f in range(4)}
+    imgs = R.render_scene_file(lambda fidx: fields[int(round(fidx))], par, text)
+    torch.cuda.synchronize()
+    frames = S.scene_views(par, text)
+    assert len(imgs) == 12 and [f[3] for f in frames][:3] == [True, True, False]
+    for k, ((cam, prm, fidx, _), img) in enumerate(zip(frames, imgs)):
+        p = torch.empty_like(img)
+        R.render(fields[int(round(fidx))], cam, prm, p)
+        torch.cuda.synchronize()
+        assert_bitexact(img.cpu().numpy(), p.cpu().numpy(), "scene frame %d" % k)
