@@ -1063,7 +1063,54 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
     }
   }
   bool tuned = false;
-  if ((pl.m[0] || pl.m[1]) && !chunked[0] && !chunked[1]) {
+  // A single view with both passes (no autotuning): the additive pass is
+  // binned first and splatted on a second stream, into a cleared scratch
+  // image, while the ordered pass bins (memory-bound) and splats; the
+  // scratch image is then added to the image: pic + l per pixel, the
+  // additive pass's own update (pixels it does not touch add +0, which
+  // leaves any value but -0 as it is; the ordered pass never writes -0).
+  // Its longest lists stay in the 8x8 kernel: producer/consumer workgroups
+  // would hold whole CUs while the ordered pass bins.
+  const bool early = !chunked[0] && !chunked[1] && nj == 1 && G == 8 && pl.m[0] && pl.m[1] &&
+                     pl.js[0].accumulate[1] == 1 && !jobs[0].prm->hip_autotune;
+  if (early) {
+    Tuning ta = tn;
+    ta.heavy = 0xffffffffu;
+    ta.heavy_max = 1;
+    stage_bins(c, pl, 1, s);
+    stage_items_count(c, pl, 1, ta, tot, s);
+    read_totals(c, tot + T_ITEMS, 4, s);
+    pl.nitems[1] = c->h_tot[1];
+    pl.nheavy[1] = 0;
+    const JobSplat &J0 = pl.js[0];
+    const int64_t npix = static_cast<int64_t>(J0.xres) * J0.yres * 3;
+    float *scratch = ws.get_as<float>(S_SCRATCH, static_cast<size_t>(npix));
+    JobSplat *d_alt = ws.get_as<JobSplat>(S_JOBS2, 1);
+    // same flags as the items were counted with, but written to the scratch
+    // image (blocks without items stay zero)
+    JobSplat alt = J0;
+    alt.pic = scratch;
+    alt.accumulate[1] = 0;
+    char *h = staging(c, sizeof(JobSplat));
+    std::memcpy(h, &alt, sizeof(JobSplat));
+    hip_check(hipMemcpyAsync(d_alt, h, sizeof(JobSplat), hipMemcpyHostToDevice, s),
+              "upload scratch job");
+    staged(c, s);
+    ensure_aux2(c);
+    hip_check(hipEventRecord(c->ev_fork2, s), "record fork");
+    hip_check(hipStreamWaitEvent(c->aux2, c->ev_fork2, 0), "wait fork");
+    launch_fill(scratch, 0.f, npix, c->aux2);
+    stage_splat(c, pl, 1, ta, c->aux2, &c->ws2, d_alt, false);
+    hip_check(hipEventRecord(c->ev_join2, c->aux2), "record join");
+    stage_bins(c, pl, 0, s);
+    stage_items_count(c, pl, 0, tn, tot, s);
+    read_totals(c, tot + T_ITEMS, 4, s);  // waits for stream s only
+    pl.nitems[0] = c->h_tot[0];
+    pl.nheavy[0] = std::min(c->h_tot[2], tn.heavy_max);
+    stage_splat(c, pl, 0, tn, s);
+    hip_check(hipStreamWaitEvent(s, c->ev_join2, 0), "wait join");
+    launch_add_images(J0.pic, scratch, npix, s);
+  } else if ((pl.m[0] || pl.m[1]) && !chunked[0] && !chunked[1]) {
     for (int p = 0; p < 2; ++p)
       if (pl.m[p]) stage_bins(c, pl, p, s);
     if (G == 8 && jobs[0].prm->hip_autotune) {
