@@ -330,6 +330,30 @@ def test_region_binning_vs_oracle(R, T):
     assert_bitexact(pic, (pg + ps).astype(np.float32), "region binning T=%d" % T)
 
 
+def test_chunked_region_binning(R):
+    """Chunks of a pass binned through 16x16 regions (large footprints): the
+    image equals the unchunked render bit for bit."""
+    import splotch_amd as S
+    gas = synth.plummer_field(20_000, 1560, ptype=0, halo_frac=1.0, n_halos=2,
+                              hsml_mu=math.log(3.0))
+    stars = synth.plummer_field(20_000, 1561, ptype=1, hsml_mu=math.log(2.0))
+    fld = synth.concat(gas, stars)
+    v = synth.default_view(176, 160)
+    pal = synth.default_palette()
+    for aeq in ([0, 1], [0, 0], [1, 1]):
+        want = render(R, fld, v, [pal, pal], S.params(ptypes=2, a_eq_e=aeq,
+                                                      brightness=[0.5, 2.0]))
+        got = render(R, fld, v, [pal, pal], S.params(ptypes=2, a_eq_e=aeq, brightness=[0.5, 2.0],
+                                                     hip_chunk_entries=60_000))
+        regions = R.binning()
+        _, counts = R.stage_times()
+        log("chunked regions", aeq, regions, counts["ordered_entries"],
+            counts["additive_entries"])
+        assert max(counts["ordered_entries"], counts["additive_entries"]) > 60_000
+        assert regions[0] > 0 or regions[1] > 0
+        assert_bitexact(got, want, "chunked region binning %s" % (aeq,))
+
+
 @pytest.mark.parametrize("aeq", [False, True])
 def test_heavy_blocks_on_fixtures(R, golden, aeq):
     """Every non-empty block through the producer/consumer kernel."""
