@@ -40,6 +40,10 @@ The JSON line also carries:
                 (bin_reader_block layout: six big-endian float columns) and
                 read into device SoA arrays by splhip_read_bin_block (file in
                 the page cache): bytes of the file per second;
+  fragments     the frame's per-pixel updates (render_new's fragments, counted
+                by the oracle: `bench.py --count-fragments`, committed summary)
+                over the frame time and over the ordered splat stage;
+  hbm_copy      a measured device copy (1 GiB) beside the 8 TB/s datasheet peak;
   cpu_baseline  the genuine reference host renderer (oracle/_ref, built from
                 the Splotch sources) — or the oracle restatement if that
                 library is absent — timed on this host's cores on a bounded
@@ -413,6 +417,41 @@ def bench_large_share(S, torch, args):
     print(json.dumps(out), flush=True)
 
 
+def count_fragments(args):
+    """Fragments of the default frame per pass, from the oracle (CPU)."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    from splotch_amd import synth
+    field = synth.gas_and_stars(args.n_gas, args.n_stars, 1100, 1101)
+    view = synth.default_view(args.res)
+    pal = synth.default_palette()
+    p = O.project_colorize(field, view, [pal, pal], brightness=[0.5, 2.0])
+    out = {"workload": "gas_stars_%dM_%d" % (field.n // 1000000, args.res)}
+    for name, t in (("ordered", 0), ("additive", 1)):
+        q = p.take(np.arange(p.n))
+        q.active = (p.active * (p.type == t)).astype(np.uint8)
+        out["fragments_" + name] = O.fragments(q, view.xres, view.yres, 50)
+    print(json.dumps(out), flush=True)
+
+
+def hbm_copy(torch, nbytes=1 << 30, reps=10):
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    return {"bytes_moved": 2 * nbytes, "ms": round(ms, 4),
+            "GBs": round(2 * nbytes / (ms * 1e-3) / 1e9, 1), "datasheet_GBs": HBM_PEAK_GBS}
+
+
 def copy_calibration(torch, steps=20):
     """A plain device copy of 1 GiB, for scaling rocprofv3 FETCH_SIZE /
     WRITE_SIZE: each dispatch reads and writes exactly 1 GiB."""
@@ -441,6 +480,8 @@ def main():
     ap.add_argument("--workload", default="gas_stars",
                     choices=["gas_stars", "large_share"],
                     help="gas_stars: 16M at 2048^2 (default); large_share: 200M at 4096^2")
+    ap.add_argument("--count-fragments", action="store_true",
+                    help="print the default frame's fragments per pass (oracle, CPU) and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (profiler counter calibration)")
     ap.add_argument("--traffic-kib", type=float, nargs=2, default=None,
@@ -454,6 +495,10 @@ def main():
               "(the image-sum sharding is described in DESIGN.md); run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.count_fragments:
+        count_fragments(args)
+        return
 
     import numpy as np
     import torch
@@ -578,6 +623,22 @@ def main():
                              "limit"},
         "valu": valu,
     }
+    fprof = os.path.join(ROOT, "profiles", "fragments_gas_stars_16M_2048.md")
+    if (args.n_gas, args.n_stars, args.res) == (8_000_000, 8_000_000, 2048) and \
+            os.path.exists(fprof):
+        fr = {}
+        for line in open(fprof):
+            if line.startswith("fragments_"):
+                k, v = line.split(":")
+                fr[k.strip()] = int(v)
+        tot = fr["fragments_ordered"] + fr["fragments_additive"]
+        out["fragments"] = {
+            "ordered": fr["fragments_ordered"], "additive": fr["fragments_additive"],
+            "per_s_frame": round(tot / (ms_frame * 1e-3), 1),
+            "ordered_per_s_splat_stage": round(fr["fragments_ordered"] /
+                                               (stages["ordered_splat"] * 1e-3), 1),
+            "source": "profiles/fragments_gas_stars_16M_2048.md"}
+    out["hbm_copy"] = hbm_copy(torch)
     out["epilogue"] = bench_epilogue(S, R, torch, pic, W, H)
     out["normalize"] = bench_normalize(S, R, torch, dev)
     out["ingest"] = bench_ingest(S, R, torch, field)

@@ -254,6 +254,13 @@ def render(p: Projected, xres, yres, a_eq_e=True, grayabsorb=0.2, tile=50):
     return pic
 
 
+def fragments(p: Projected, xres, yres, tile=50):
+    """The number of per-pixel updates render_new makes (orc_fragments)."""
+    lib().orc_fragments.restype = C.c_int64
+    return int(lib().orc_fragments(C.c_int64(p.n), _p(p.x), _p(p.y), _p(p.r), _p(p.active),
+                                   C.c_int(xres), C.c_int(yres), C.c_int(tile)))
+
+
 def render_bruteforce(p: Projected, xres, yres, tile=50):
     pic = np.zeros((xres, yres, 3), np.float32)
     lib().orc_render_bruteforce(C.c_int64(p.n), _p(p.x), _p(p.y), _p(p.r), _p(p.e0),

@@ -23,6 +23,10 @@
  *   orc_bins          brute force over every (tile, particle) + render parity
  *   orc_render        genuine-build fixtures (raw pic) bit-exact, both modes;
  *                     single-particle and two-particle closed forms
+ *   orc_fragments     a count of orc_render's per-pixel updates (measurement
+ *                     only, not a parity function): the same footprint loops
+ *                     without the arithmetic; checked against a per-pixel
+ *                     brute-force count of the same rule on small fields
  *   orc_render64      FP64 twin, checked against orc_render within the exptable
  *                     bound and by additivity
  *   orc_postprocess   genuine-build fixtures (exptable -expm1, colour_adjust,
@@ -485,6 +489,42 @@ void orc_render(int64_t n, const float *px, const float *py, const float *pr, co
         }
     }
   }
+}
+
+/* The number of per-pixel updates render_new makes (its fragments): the
+   footprint loops of splotch_host.cc:461-609 per (tile, particle) of the
+   tile lists, counting the rows of each column. */
+int64_t orc_fragments(int64_t n, const float *px, const float *py, const float *pr,
+                      const uint8_t *active, int xres, int yres, int tile) {
+  const int chunkdim = tile;
+  const int ncx = (xres + chunkdim - 1) / chunkdim, ncy = (yres + chunkdim - 1) / chunkdim;
+  int64_t total = orc_bins(n, px, py, pr, active, xres, yres, tile, nullptr, nullptr, nullptr);
+  std::vector<int64_t> off(size_t(ncx) * ncy + 1);
+  std::vector<uint32_t> idx(total);
+  orc_bins(n, px, py, pr, active, xres, yres, tile, nullptr, off.data(), idx.data());
+  const int nchunks = ncx * ncy;
+  int64_t frags = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : frags)
+  for (int chunk = 0; chunk < nchunks; ++chunk) {
+    const int cix = chunk % ncx, ciy = chunk / ncx;
+    const int x0s = cix * chunkdim, y0s = ciy * chunkdim;
+    const int x1 = std::min(x0s + chunkdim, xres) - x0s, y1 = std::min(y0s + chunkdim, yres) - y0s;
+    for (int64_t m = off[chunk]; m < off[chunk + 1]; ++m) {
+      const uint32_t i = idx[m];
+      const float rfacr = pr[i] * kRfac;
+      const float posx = px[i] - x0s, posy = py[i] - y0s;
+      const int minx = std::max(int(posx - rfacr + 1), 0), maxx = std::min(int(posx + rfacr + 1), x1);
+      const int miny = std::max(int(posy - rfacr + 1), 0), maxy = std::min(int(posy + rfacr + 1), y1);
+      const float radsq = rfacr * rfacr;
+      for (int xx = minx; xx < maxx; ++xx) {
+        const float dxsq = (xx - posx) * (xx - posx);
+        const float dy = std::sqrt(radsq - dxsq);
+        const int lo = std::max(miny, int(posy - dy + 1)), hi = std::min(maxy, int(posy + dy + 1));
+        if (hi > lo) frags += hi - lo;
+      }
+    }
+  }
+  return frags;
 }
 
 /* Brute force (no binning): every pixel loops over ALL active particles in

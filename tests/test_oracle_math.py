@@ -164,3 +164,35 @@ def test_additivity_and_fp64_twin():
     rel = np.abs(full - p64) / np.maximum(np.abs(p64), 1e-6 * np.abs(p64).max())
     # exptable: <= 2 truncation steps per contribution, plus float accumulation
     assert rel.max() < 5e-5
+
+
+def test_fragments_count():
+    """orc_fragments (the count of render_new's per-pixel updates, for the
+    fragment rate) equals a numpy count of the same footprint rule over the
+    tile lists."""
+    f = synth.plummer_field(1500, 24, hsml_mu=math.log(0.6))
+    v = synth.default_view(96, 80)
+    p = O.project_colorize(f, v, [synth.default_palette()])
+    for T in (50, 16):
+        off, idx = O.bins(p, v.xres, v.yres, T)
+        ncx = (v.xres + T - 1) // T
+        want = 0
+        for t in range(len(off) - 1):
+            x0s, y0s = (t % ncx) * T, (t // ncx) * T
+            x1 = min(x0s + T, v.xres) - x0s
+            y1 = min(y0s + T, v.yres) - y0s
+            for i in idx[off[t]:off[t + 1]]:
+                rf = np.float32(p.r[i]) * np.float32(0.75)
+                px = np.float32(p.x[i]) - np.float32(x0s)
+                py = np.float32(p.y[i]) - np.float32(y0s)
+                minx = max(int(np.float32(px - rf + np.float32(1))), 0)
+                maxx = min(int(np.float32(px + rf + np.float32(1))), x1)
+                miny = max(int(np.float32(py - rf + np.float32(1))), 0)
+                maxy = min(int(np.float32(py + rf + np.float32(1))), y1)
+                for xx in range(minx, maxx):
+                    dxsq = np.float32(np.float32(xx) - px) * np.float32(np.float32(xx) - px)
+                    dy = np.sqrt(np.float32(rf * rf - dxsq), dtype=np.float32)
+                    lo = max(miny, int(np.float32(py - dy + np.float32(1))))
+                    hi = min(maxy, int(np.float32(py + dy + np.float32(1))))
+                    want += max(hi - lo, 0)
+        assert O.fragments(p, v.xres, v.yres, T) == want, T
