@@ -491,8 +491,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the image-sum sharding is described in DESIGN.md); run with --gpus 1.",
+        print("bench.py measures one GPU only: the library renders on one device per "
+              "context and has no multi-GPU path (an additive frame would shard by "
+              "particles and sum the images); run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 
