@@ -12,6 +12,7 @@
 #include <new>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "common.hpp"
@@ -74,6 +75,9 @@ struct splhip_ctx {
   // block-file ingest: two pinned buffers, each freed for reuse by its event
   char *h_ing[2] = {nullptr, nullptr};
   hipEvent_t ev_ing[2] = {nullptr, nullptr};
+  // host-record upload: per worker thread two pinned buffers and their events
+  std::vector<char *> h_rec;
+  std::vector<hipEvent_t> ev_rec;
 };
 
 namespace {
@@ -1878,6 +1882,80 @@ int splhip_render(splhip_ctx *c, const splhip_particles *P, const splhip_camera 
   }
 }
 
+namespace {
+
+constexpr int64_t kRecChunk = int64_t(1) << 17;  // records per pinned buffer
+constexpr int kRecWorkers = 8;
+
+// Host particle_sim records into the device SoA (soa: 8 columns of nn floats;
+// typ: u8): worker w gathers the fields of its chunks into its two pinned
+// buffers in turn (each reused once its copy is done), a chunk's columns
+// packed back to back, and queues one copy per chunk into the device staging
+// area (chunk-major); one kernel then unpacks the columns.  The colour's
+// second and third components travel only when a type has a vector colour.
+// Types above 255 become 255 (no rendered type).
+void upload_records(splhip_ctx *c, const splhip_particle_sim *hp, int64_t n, float *soa,
+                    int64_t nn, uint8_t *typ, bool colour3, hipStream_t s) {
+  constexpr size_t kBuf = kRecChunk * (8 * sizeof(float) + 1);
+  if (c->h_rec.empty()) {
+    c->h_rec.assign(2 * kRecWorkers, nullptr);
+    c->ev_rec.assign(2 * kRecWorkers, nullptr);
+    for (int k = 0; k < 2 * kRecWorkers; ++k) {
+      hip_check(hipHostMalloc(reinterpret_cast<void **>(&c->h_rec[k]), kBuf, hipHostMallocDefault),
+                "hipHostMalloc records");
+      hip_check(hipEventCreateWithFlags(&c->ev_rec[k], hipEventDisableTiming), "hipEventCreate");
+    }
+  }
+  const int ncol = colour3 ? 8 : 6;
+  const int64_t rb = 4 * ncol + 1;  // staged bytes per record
+  char *stage = c->ws.get_as<char>(S_RECSTAGE, static_cast<size_t>(n * rb));
+  const int64_t nchunk = (n + kRecChunk - 1) / kRecChunk;
+  std::vector<std::string> errs(kRecWorkers);
+  auto work = [&](int w) {
+    try {
+      hip_check(hipSetDevice(c->device), "hipSetDevice");
+      int k = 0;
+      for (int64_t ch = w; ch < nchunk; ch += kRecWorkers) {
+        const int64_t i0 = ch * kRecChunk, cnt = std::min(kRecChunk, n - i0);
+        const int slot = 2 * w + k;
+        hip_check(hipEventSynchronize(c->ev_rec[slot]), "wait record buffer");
+        float *col = reinterpret_cast<float *>(c->h_rec[slot]);
+        uint8_t *tb = reinterpret_cast<uint8_t *>(col + ncol * cnt);
+        const splhip_particle_sim *p = hp + i0;
+        for (int64_t i = 0; i < cnt; ++i) {
+          col[0 * cnt + i] = p[i].x;
+          col[1 * cnt + i] = p[i].y;
+          col[2 * cnt + i] = p[i].z;
+          col[3 * cnt + i] = p[i].r;
+          col[4 * cnt + i] = p[i].I;
+          col[5 * cnt + i] = p[i].er;
+          if (colour3) {
+            col[6 * cnt + i] = p[i].eg;
+            col[7 * cnt + i] = p[i].eb;
+          }
+          tb[i] = static_cast<uint8_t>(std::min<uint32_t>(p[i].type, 255u));
+        }
+        hip_check(hipMemcpyAsync(stage + i0 * rb, col, static_cast<size_t>(cnt * rb),
+                                 hipMemcpyHostToDevice, s),
+                  "H2D records");
+        hip_check(hipEventRecord(c->ev_rec[slot], s), "hipEventRecord");
+        k ^= 1;
+      }
+    } catch (const HipError &e) {
+      errs[w] = e.what;
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int w = 1; w < kRecWorkers; ++w) pool.emplace_back(work, w);
+  work(0);
+  for (auto &t : pool) t.join();
+  for (const auto &e : errs)
+    if (!e.empty()) throw HipError{hipErrorUnknown, "record upload failed"};
+  launch_unpack_records(stage, n, kRecChunk, ncol, soa, nn, typ, s);
+}
+
+}  // namespace
+
 int splhip_render_host(splhip_ctx *c, const splhip_particle_sim *hp, int64_t n,
                        const splhip_camera *cam, const splhip_params *prm, float *host_pic,
                        void *stream) {
@@ -1886,19 +1964,18 @@ int splhip_render_host(splhip_ctx *c, const splhip_particle_sim *hp, int64_t n,
     hip_check(hipSetDevice(c->device), "hipSetDevice");
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int64_t nn = std::max<int64_t>(n, 1);
-    auto *aos = c->ws.get_as<splhip_particle_sim>(S_AOS, static_cast<size_t>(nn));
     float *soa = c->ws.get_as<float>(S_SOA, static_cast<size_t>(nn) * 8);
     uint8_t *typ = c->ws.get_as<uint8_t>(S_AUX0, static_cast<size_t>(nn));
     const size_t npix = static_cast<size_t>(cam->xres) * cam->yres * 3;
     float *pic = c->ws.get_as<float>(S_PIC, npix);
-    if (n > 0) {
-      hip_check(hipMemcpyAsync(aos, hp, n * sizeof(splhip_particle_sim), hipMemcpyHostToDevice, s),
-                "H2D particles");
-      launch_aos_to_soa(aos, n, soa, typ, s);
-    }
+    bool colour3 = false;
+    for (int t = 0; t < prm->ptypes && t < kMaxTypes; ++t) colour3 = colour3 || prm->col_vector[t];
+    if (n > 0) upload_records(c, hp, n, soa, nn, typ, colour3, s);
     splhip_particles P{};
     P.x = soa; P.y = soa + nn; P.z = soa + 2 * nn; P.hsml = soa + 3 * nn;
-    P.inten = soa + 4 * nn; P.col = soa + 5 * nn; P.c1 = soa + 6 * nn; P.c2 = soa + 7 * nn;
+    P.inten = soa + 4 * nn; P.col = soa + 5 * nn;
+    P.c1 = colour3 ? soa + 6 * nn : nullptr;
+    P.c2 = colour3 ? soa + 7 * nn : nullptr;
     P.type = typ;
     P.n = n;
     if (int rc = check_inputs(c, &P, cam, prm)) return rc;

@@ -581,19 +581,16 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t *p, uint32_t v, int64
   for (int64_t i = gtid(); i < n; i += gstride()) p[i] = v;
 }
 
-__global__ __launch_bounds__(256) void k_aos_to_soa(const splhip_particle_sim *aos, int64_t n,
-                                                    float *soa, uint8_t *type) {
+__global__ __launch_bounds__(256) void k_unpack_records(const char *stage, int64_t n, int64_t C,
+                                                        int ncol, float *soa, int64_t nn,
+                                                        uint8_t *type) {
+  const int64_t rb = 4 * ncol + 1;
   for (int64_t i = gtid(); i < n; i += gstride()) {
-    const splhip_particle_sim p = aos[i];
-    soa[0 * n + i] = p.x;
-    soa[1 * n + i] = p.y;
-    soa[2 * n + i] = p.z;
-    soa[3 * n + i] = p.r;
-    soa[4 * n + i] = p.I;
-    soa[5 * n + i] = p.er;
-    soa[6 * n + i] = p.eg;
-    soa[7 * n + i] = p.eb;
-    type[i] = static_cast<uint8_t>(p.type);
+    const int64_t k = i / C, j = i - k * C, cnt = min(C, n - k * C);
+    const char *base = stage + k * C * rb;
+    const float *col = reinterpret_cast<const float *>(base);
+    for (int q = 0; q < ncol; ++q) soa[q * nn + i] = col[q * cnt + j];
+    type[i] = static_cast<uint8_t>(base[4 * ncol * cnt + j]);
   }
 }
 
@@ -892,10 +889,12 @@ void launch_fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t s) {
   hip_check(hipGetLastError(), "k_fill_u32 launch");
 }
 
-void launch_aos_to_soa(const splhip_particle_sim *aos, int64_t n, float *soa, uint8_t *type,
-                       hipStream_t s) {
-  k_aos_to_soa<<<grid_for(n), 256, 0, s>>>(aos, n, soa, type);
-  hip_check(hipGetLastError(), "k_aos_to_soa launch");
+
+void launch_unpack_records(const char *stage, int64_t n, int64_t C, int ncol, float *soa,
+                           int64_t nn, uint8_t *type, hipStream_t s) {
+  if (n <= 0) return;
+  k_unpack_records<<<grid_for(n), 256, 0, s>>>(stage, n, C, ncol, soa, nn, type);
+  hip_check(hipGetLastError(), "k_unpack_records launch");
 }
 
 void launch_splat(int G, bool additive, uint32_t nbins, const SplatArgs &a, hipStream_t s) {

@@ -113,8 +113,12 @@ void launch_sum_u64(const uint32_t *v, int64_t n, uint64_t *out, hipStream_t s);
 void launch_add_images(float *pic, const float *add, int64_t n, hipStream_t s);
 // A one-wave kernel that returns after `us` microseconds (bounded wait).
 void launch_delay(uint32_t us, hipStream_t s);
-void launch_aos_to_soa(const splhip_particle_sim *aos, int64_t n, float *soa /*8 planes*/,
-                       uint8_t *type, hipStream_t s);
+// Host records staged chunk-major (chunk k: records [k C, k C + cnt_k), its
+// ncol float columns of cnt_k values back to back, then cnt_k type bytes)
+// into the SoA columns soa[q nn + i] and type[i].
+void launch_unpack_records(const char *stage, int64_t n, int64_t C, int ncol, float *soa,
+                           int64_t nn, uint8_t *type, hipStream_t s);
+
 // Wave-per-8x8-block splat (splat8.hip): work items (unit << 2 | channel,
 // channel 3 = all), heaviest first.  heavy_cnt (device, optional): incremented
 // once per unit whose list reaches a.heavy_len.
