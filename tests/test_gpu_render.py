@@ -204,6 +204,31 @@ def test_render_host_records(R, golden):
     assert err < 1e-3
 
 
+def test_render_host_records_vector_colour(R, golden):
+    """Host records with a vector-coloured type (all three colour components
+    uploaded) and more than one upload chunk: the genuine two-type fixture;
+    and a 600k-record field equal to its device-resident render."""
+    import splotch_amd as S
+    f = golden("two_types")
+    pals = [palette(golden), (f["pal1_x"], f["pal1_rgb"])]
+    for t, (x, rgb) in enumerate(pals):
+        R.set_palette(t, x, rgb)
+    kw = dict(ptypes=2, smooth_fac=[1.3, 0.8], brightness=[0.7, 2.0], col_vector=[0, 1],
+              b_brightness=float(f["b_brightness"][0]))
+    fld = field_from(f)
+    pic = R.render_host(S.to_particle_sim(fld), S.camera_setup(view_from(f), minrad_pix=1.5),
+                        S.params(tile_size=50, **kw))
+    assert_bitexact(pic, f["pic_T50"], "host records, vector colour")
+    big = synth.concat(synth.plummer_field(300_000, 1580, ptype=0),
+                       synth.plummer_field(300_000, 1581, ptype=1, hsml_fixed=0.05))
+    v = synth.default_view(320, 256)
+    pal = synth.default_palette()
+    prm = S.params(ptypes=2, a_eq_e=[0, 1], brightness=[0.5, 2.0])
+    want = render(R, big, v, [pal, pal], prm)
+    got = R.render_host(S.to_particle_sim(big), S.camera_setup(v), prm)
+    assert_bitexact(got.reshape(want.shape), want, "host records, 600k, 5 chunks")
+
+
 @pytest.mark.parametrize("seed,n,res,aeq,T,G", [
     (101, 200_000, (512, 384), True, 50, 32),
     (102, 200_000, (384, 512), False, 50, 32),
