@@ -1338,6 +1338,10 @@ void splhip_destroy(splhip_ctx *c) {
     if (c->h_ing[k]) (void)hipHostFree(c->h_ing[k]);
     if (c->ev_ing[k]) (void)hipEventDestroy(c->ev_ing[k]);
   }
+  for (char *h : c->h_rec)
+    if (h) (void)hipHostFree(h);
+  for (hipEvent_t e : c->ev_rec)
+    if (e) (void)hipEventDestroy(e);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   if (c->aux) (void)hipStreamDestroy(c->aux);
