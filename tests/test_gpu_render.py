@@ -355,6 +355,35 @@ def test_region_binning_vs_oracle(R, T):
     assert_bitexact(pic, (pg + ps).astype(np.float32), "region binning T=%d" % T)
 
 
+def test_profiling_and_trace_both_passes(R):
+    """Stage timing and the work-item trace on a frame with both passes (the
+    additive splat on the second stream): the image is unchanged, every
+    stage of both passes has a time, and each pass's trace has one record per
+    work item with end >= start."""
+    import splotch_amd as S
+    fld = synth.gas_and_stars(30_000, 30_000, 1590, 1591)
+    v = synth.default_view(192, 160)
+    pal = synth.default_palette()
+    prm = S.params(ptypes=2, a_eq_e=[0, 1], brightness=[0.5, 2.0])
+    want = render(R, fld, v, [pal, pal], prm)
+    R.set_profiling(True)
+    R.set_trace(True)
+    try:
+        got = render(R, fld, v, [pal, pal], prm)
+        st, counts = R.stage_times()
+        tr = {o: R.trace_items(o) for o in (True, False)}
+    finally:
+        R.set_trace(False)
+        R.set_profiling(False)
+    assert_bitexact(got, want, "profiled and traced frame")
+    log("stages", st)
+    for k in ("project", "ordered_bin", "ordered_splat", "additive_bin", "additive_splat"):
+        assert st[k] > 0, k
+    for o, t in tr.items():
+        assert len(t) > 0, o
+        assert np.all(t[:, 1] >= t[:, 0]), o
+
+
 def test_chunked_region_binning(R):
     """Chunks of a pass binned through 16x16 regions (large footprints): the
     image equals the unchunked render bit for bit."""
