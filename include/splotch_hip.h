@@ -80,6 +80,11 @@ typedef struct {
                                               (hip_variant, hip_split, hip_heavy_max) choices
                                               on the first render of a batch shape and reuse
                                               the fastest for that shape (0 = off) */
+  int32_t hip_bin_group;                   /* hip_bin_group: 8 px units are binned through
+                                              regions of g x g units, then split: 4, 2, or
+                                              1 = sort the unit entries directly (0 = auto: the
+                                              largest of 4, 2 whose region ids and unit masks
+                                              fit 32-bit keys, when it merges entries) */
 } splhip_params;
 
 /* SoA particle fields (device pointers), post particle_normalize:

@@ -372,7 +372,8 @@ struct Plan {
   int nj = 0, G = 8;
   int64_t ntot = 0;
   uint32_t nblocks = 0, nunits = 0;
-  uint32_t nregions = 0;             // G == 8: 16x16 regions (2x2 units)
+  uint32_t nregions = 0;             // G == 8: regions of 2^rsh x 2^rsh units
+  int rsh = 0, rbits = 0;            // rsh 0: no regions; rbits: bits of a region id
   bool present[2] = {false, false};  // some job has types of the pass
   bool sort = false;                 // some job depth-sorts its ordered types
   std::vector<ProjectArgs> pa;
@@ -432,7 +433,8 @@ void make_plan(splhip_ctx *c, int nj, const JobIn *jobs, int G, Plan &pl, hipStr
   pl.pbase.assign(nj + 1, 0);
   pl.ubase.assign(nj + 1, 0);
   pl.rbase.assign(nj + 1, 0);
-  uint64_t units = 0, blocks = 0, regions = 0;
+  std::vector<uint32_t> rbase4(nj + 1, 0);
+  uint64_t units = 0, blocks = 0, regions = 0, regions4 = 0;
   int64_t ntot = 0;
   for (int j = 0; j < nj; ++j) {
     const splhip_particles *P = jobs[j].P;
@@ -464,14 +466,17 @@ void make_plan(splhip_ctx *c, int nj, const JobIn *jobs, int G, Plan &pl, hipStr
     pl.blk[j] = static_cast<uint32_t>(blocks);
     pl.ubase[j] = static_cast<uint32_t>(units);
     pl.rbase[j] = static_cast<uint32_t>(regions);
+    rbase4[j] = static_cast<uint32_t>(regions4);
     ntot += P->n;
     blocks += (static_cast<uint64_t>(P->n) + kPerBlock - 1) / kPerBlock;
     const BinGeom g = make_geom(cam, prm, G, c->consts);
     pl.geom[j] = g;
     units += static_cast<uint64_t>(g.ncx) * g.ncy * (G > 0 ? g.ngm * g.ngm : 1);
     if (G == 8) {
-      const uint64_t n16 = (static_cast<uint64_t>(g.ngm) + 1) / 2;
-      regions += static_cast<uint64_t>(g.ncx) * g.ncy * n16 * n16;
+      const uint64_t n2 = (static_cast<uint64_t>(g.ngm) + 1) / 2;
+      const uint64_t n4 = (static_cast<uint64_t>(g.ngm) + 3) / 4;
+      regions += static_cast<uint64_t>(g.ncx) * g.ncy * n2 * n2;
+      regions4 += static_cast<uint64_t>(g.ncx) * g.ncy * n4 * n4;
     }
     JobSplat &J = pl.js[j];
     J = JobSplat{};
@@ -491,8 +496,27 @@ void make_plan(splhip_ctx *c, int nj, const JobIn *jobs, int G, Plan &pl, hipStr
   pl.blk[nj] = pl.nblocks;
   pl.pbase[nj] = static_cast<uint32_t>(ntot);
   pl.ubase[nj] = pl.nunits;
-  pl.nregions = static_cast<uint32_t>(regions);
-  pl.rbase[nj] = pl.nregions;
+  pl.rbase[nj] = static_cast<uint32_t>(regions);
+  rbase4[nj] = static_cast<uint32_t>(regions4);
+  // Binning through regions: the largest group (4 x 4, then 2 x 2 units)
+  // whose region ids and unit masks fit a 32-bit key (hip_bin_group forces
+  // one, 1 = none).
+  pl.rsh = 0;
+  pl.nregions = 0;
+  if (G == 8) {
+    const int bg = jobs[0].prm->hip_bin_group;
+    for (int S : {4, 2}) {
+      if (bg == 1 || (bg > 0 && bg != S)) continue;
+      const uint64_t nr = S == 4 ? regions4 : regions;
+      const int rbits = ceil_log2(nr);
+      if (nr == 0 || nr >= (uint64_t(1) << 31) || rbits + S * S > 32) continue;
+      pl.rsh = S == 4 ? 2 : 1;
+      pl.rbits = rbits;
+      pl.nregions = static_cast<uint32_t>(nr);
+      if (S == 4) pl.rbase = rbase4;
+      break;
+    }
+  }
   // one upload: [ProjectArgs][blk][pbase][ubase][rbase][BinGeom][JobSplat]
   const size_t o_pa = 0;
   const size_t o_blk = align256(o_pa + nj * sizeof(ProjectArgs));
@@ -626,12 +650,13 @@ void stage_count(splhip_ctx *c, Plan &pl, int p, const Rec *rec, uint32_t *tot, 
   Workspace &ws = c->ws;
   const uint32_t m = pl.m[p];
   mark(c, p == 0 ? 2 : 5, true, s);
-  pl.r16[p] = pl.G == 8 && m < (1u << 28);
+  pl.r16[p] = pl.rsh > 0;
   Rec *recs = ws.get_as<Rec>(pslot(p, P_RECS), m);
   uint32_t *cnt = ws.get_as<uint32_t>(pslot(p, P_CNT), m);
   uint32_t *off = ws.get_as<uint32_t>(pslot(p, P_OFF), m);
   uint32_t *cnt16 = pl.r16[p] ? ws.get_as<uint32_t>(pslot(p, P_CNT16), m) : nullptr;
-  launch_gather_count(pl.order[p], m, rec, recs, cnt, cnt16, pl.d_geom, pl.d_mo[p], pl.nj, s);
+  launch_gather_count(pl.order[p], m, rec, recs, cnt, cnt16, pl.rsh, pl.d_geom, pl.d_mo[p], pl.nj,
+                      s);
   scan_u32(cnt, off, m, tot + T_E + p, ws, s);
   launch_sum_u64(cnt, m, reinterpret_cast<uint64_t *>(tot + T_E64 + 2 * p), s);
   if (cnt16)
@@ -658,26 +683,29 @@ void stage_bins(splhip_ctx *c, Plan &pl, int p, hipStream_t s) {
     uint32_t *rv0 = ws.get_as<uint32_t>(pslot(p, P_EV0), e16);
     uint32_t *rk1 = ws.get_as<uint32_t>(pslot(p, P_EK1), e16);
     uint32_t *rv1 = ws.get_as<uint32_t>(pslot(p, P_EV1), e16);
-    launch_emit(1, pl.recs[p], m, off16, e16, pl.d_geom, pl.d_rbase, pl.d_mo[p], pl.nj, rk0, rv0,
-                s);
-    const int r = radix_sort_pairs(rk0, rv0, rk1, rv1, e16, 0, ceil_log2(nr), ws, s);
-    const uint32_t *rv = r ? rv1 : rv0;
+    const int S = 1 << pl.rsh, rbits = pl.rbits;
+    launch_emit(pl.rsh, rbits, pl.recs[p], m, off16, e16, pl.d_geom, pl.d_rbase, pl.d_mo[p],
+                pl.nj, rk0, rv0, s);
+    // the sort orders the region bits only: the unit masks above them ride
+    // along, and each region's list keeps render order
+    const int r = radix_sort_pairs(rk0, rv0, rk1, rv1, e16, 0, rbits, ws, s);
+    const uint32_t *rk = r ? rk1 : rk0, *rv = r ? rv1 : rv0;
     uint32_t *roff = ws.get_as<uint32_t>(pslot(p, P_ROFF), static_cast<size_t>(nr) + 1);
-    launch_bin_offsets(r ? rk1 : rk0, e16, nr, roff, s);
-    const size_t nc4 = 4 * static_cast<size_t>(split_chunks(e16));
-    uint32_t *ccnt = ws.get_as<uint32_t>(pslot(p, P_CCNT), nc4 + 1);
-    uint32_t *cpre = ws.get_as<uint32_t>(pslot(p, P_CPRE), nc4 + 1);
-    uint32_t *sx = ws.get_as<uint32_t>(pslot(p, P_SX), 4 * static_cast<size_t>(nr) + 4);
+    launch_bin_offsets(rk, e16, nr, roff, s, (1u << rbits) - 1u);
+    const size_t ncb = static_cast<size_t>(S * S) * split_chunks(e16);
+    uint32_t *ccnt = ws.get_as<uint32_t>(pslot(p, P_CCNT), ncb + 1);
+    uint32_t *cpre = ws.get_as<uint32_t>(pslot(p, P_CPRE), ncb + 1);
+    uint32_t *sx = ws.get_as<uint32_t>(pslot(p, P_SX), static_cast<size_t>(S * S) * (nr + 1));
     uint32_t *ucnt = ws.get_as<uint32_t>(pslot(p, P_UCNT), nu);
     uint32_t *bin_off = ws.get_as<uint32_t>(pslot(p, P_BINOFF), static_cast<size_t>(nu) + 1);
     uint32_t *ent = ws.get_as<uint32_t>(pslot(p, P_ENT), e);
-    launch_split_chunks(rv, e16, ccnt, s);
-    scan_u32(ccnt, cpre, static_cast<int64_t>(nc4), cpre + nc4, ws, s);
-    launch_split_bounds(roff, rv, nr, cpre, e16, pl.d_rbase, pl.d_ubase, pl.d_geom, pl.nj, sx,
-                        ucnt, s);
+    launch_split_chunks(S, rk, e16, rbits, ccnt, s);
+    scan_u32(ccnt, cpre, static_cast<int64_t>(ncb), cpre + ncb, ws, s);
+    launch_split_bounds(S, roff, rk, nr, cpre, e16, rbits, pl.d_rbase, pl.d_ubase, pl.d_geom,
+                        pl.nj, sx, ucnt, s);
     scan_u32(ucnt, bin_off, nu, bin_off + nu, ws, s);
-    launch_split_base(nr, pl.d_rbase, pl.d_ubase, pl.d_geom, pl.nj, sx, bin_off, sx, s);
-    launch_split_scatter(r ? rk1 : rk0, rv, e16, cpre, sx, ent, s);
+    launch_split_base(S, nr, pl.d_rbase, pl.d_ubase, pl.d_geom, pl.nj, sx, bin_off, sx, s);
+    launch_split_scatter(S, rk, rv, e16, rbits, cpre, sx, ent, s);
     pl.ent[p] = ent;
     pl.bin_off[p] = bin_off;
     pl.used16[p] += e16;
@@ -689,7 +717,7 @@ void stage_bins(splhip_ctx *c, Plan &pl, int p, hipStream_t s) {
   uint32_t *ev0 = ws.get_as<uint32_t>(pslot(p, P_EV0), e);
   uint32_t *ek1 = ws.get_as<uint32_t>(pslot(p, P_EK1), e);
   uint32_t *ev1 = ws.get_as<uint32_t>(pslot(p, P_EV1), e);
-  launch_emit(0, pl.recs[p], m, off, e, pl.d_geom, pl.d_ubase, pl.d_mo[p], pl.nj, ek0, ev0, s);
+  launch_emit(0, 0, pl.recs[p], m, off, e, pl.d_geom, pl.d_ubase, pl.d_mo[p], pl.nj, ek0, ev0, s);
   const int r = radix_sort_pairs(ek0, ev0, ek1, ev1, e, 0, ceil_log2(pl.nunits), ws, s);
   uint32_t *bin_off = ws.get_as<uint32_t>(pslot(p, P_BINOFF), static_cast<size_t>(pl.nunits) + 1);
   launch_bin_offsets(r ? ek1 : ek0, e, pl.nunits, bin_off, s);
@@ -1438,6 +1466,7 @@ int splhip_params_default(splhip_params *p) {
   p->hip_heavy_max = 0;
   p->hip_chunk_entries = 0;
   p->hip_autotune = 0;
+  p->hip_bin_group = 0;
   return 0;
 }
 
@@ -1478,6 +1507,7 @@ int splhip_params_from_par(const char *par_text, float b_brightness, splhip_para
     p->hip_heavy_max = par_num<int>(d, "hip_heavy_max", 0);
     p->hip_chunk_entries = par_num<int>(d, "hip_chunk_entries", 0);
     p->hip_autotune = par_num<int>(d, "hip_autotune", 0);
+    p->hip_bin_group = par_num<int>(d, "hip_bin_group", 0);
   } catch (const ParError &) {
     return SPLHIP_E_ARG;
   }

@@ -186,21 +186,22 @@ __global__ __launch_bounds__(256) void k_job_of_index(const uint32_t *vals, uint
 
 __global__ __launch_bounds__(256) void k_gather_count(const uint32_t *order, uint32_t m,
                                                       const Rec *rec, Rec *recs, uint32_t *cnt,
-                                                      uint32_t *cnt16, const BinGeom *geoms,
-                                                      const uint32_t *mo, int nj) {
+                                                      uint32_t *cnt16, int rsh,
+                                                      const BinGeom *geoms, const uint32_t *mo,
+                                                      int nj) {
   for (int64_t s = gtid(); s < m; s += gstride()) {
     const BinGeom g = geoms[find_job(mo, nj, static_cast<uint32_t>(s))];
     const Rec R = rec[order[s]];
     recs[s] = R;
     const BinRange b = bin_range(R, g);
-    uint32_t c = 0, c16 = 0;  // units of the binned tiles; their 2x2 groups
+    uint32_t c = 0, c16 = 0;  // units of the binned tiles; their regions (2^rsh units square)
     for (int ix = b.minx; ix <= b.maxx; ++ix)
       for (int iy = b.miny; iy <= b.maxy; ++iy) {
         BinRect rc;
         if (bin_tile(R, g, b, ix, iy, rc)) {
           c += static_cast<uint32_t>((rc.gx1 - rc.gx0 + 1) * (rc.gy1 - rc.gy0 + 1));
-          c16 += static_cast<uint32_t>(((rc.gx1 >> 1) - (rc.gx0 >> 1) + 1) *
-                                       ((rc.gy1 >> 1) - (rc.gy0 >> 1) + 1));
+          c16 += static_cast<uint32_t>(((rc.gx1 >> rsh) - (rc.gx0 >> rsh) + 1) *
+                                       ((rc.gy1 >> rsh) - (rc.gy0 >> rsh) + 1));
         }
       }
     cnt[s] = c;
@@ -232,22 +233,27 @@ struct EmitQueue {
   uint32_t rect[kEmitQ];  // R == 2: the tile's 8x8-block rectangle gx0 | gx1 | gy0 | gy1 bytes
 };
 
-// 4-bit mask of the 8x8 blocks (2GX + sx, 2GY + sy), bit 2 sy + sx, that lie in
-// the rectangle rc = gx0 | gx1 << 8 | gy0 << 16 | gy1 << 24.
-__device__ __forceinline__ uint32_t quad_mask(uint32_t rc, int GX, int GY) {
+// Mask of the units (S GX + sx, S GY + sy) of region (GX, GY), S = 2^rsh, bit
+// S sy + sx, that lie in the rectangle rc = gx0 | gx1 << 8 | gy0 << 16 | gy1 << 24
+// (the region lies in the rectangle's region range, so each axis has one).
+__device__ __forceinline__ uint32_t region_mask(uint32_t rc, int GX, int GY, int rsh) {
   const int gx0 = rc & 0xff, gx1 = (rc >> 8) & 0xff, gy0 = (rc >> 16) & 0xff, gy1 = rc >> 24;
-  const int x0 = 2 * GX, y0 = 2 * GY;
-  const uint32_t inx0 = x0 >= gx0 && x0 <= gx1, inx1 = x0 + 1 >= gx0 && x0 + 1 <= gx1;
-  const uint32_t iny0 = y0 >= gy0 && y0 <= gy1, iny1 = y0 + 1 >= gy0 && y0 + 1 <= gy1;
-  return (inx0 & iny0) | ((inx1 & iny0) << 1) | ((inx0 & iny1) << 2) | ((inx1 & iny1) << 3);
+  const int S = 1 << rsh;
+  const int lx = max(gx0 - S * GX, 0), hx = min(gx1 - S * GX, S - 1);
+  const int ly = max(gy0 - S * GY, 0), hy = min(gy1 - S * GY, S - 1);
+  const uint32_t xm = ((2u << hx) - 1u) & ~((1u << lx) - 1u);
+  uint32_t m = 0;
+  for (int sy = ly; sy <= hy; ++sy) m |= xm << (sy * S);
+  return m;
 }
 
 // Write the entries of items [0, qn) of the queue: item k covers output
 // [pos, pos + n), entry i = (gy, gx) = (i / nx, i % nx) is unit key + gy*ngm + gx
-// (R == 1: 8x8 blocks; R == 2: 16x16 regions, value | block mask << 28).
+// (R == 1: 8x8 blocks; R == 2: regions of 2^rsh x 2^rsh blocks, key | unit
+// mask << rbits).
 template <int R>
 __device__ __forceinline__ void emit_drain(EmitQueue &Q, int qn, int lane, uint32_t *keys,
-                                           uint32_t *vals) {
+                                           uint32_t *vals, int rsh, int rbits) {
   const uint32_t c0 = lane < qn ? Q.n[lane] : 0u, c1 = lane + 64 < qn ? Q.n[lane + 64] : 0u;
   const uint32_t i0 = wave_incl_scan(c0, lane), i1 = wave_incl_scan(c1, lane);
   const uint32_t t0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(i0), 63));
@@ -275,28 +281,30 @@ __device__ __forceinline__ void emit_drain(EmitQueue &Q, int qn, int lane, uint3
         gx -= nx;
       }
       const uint32_t o = Q.pos[k] + i;
-      keys[o] = Q.key[k] + static_cast<uint32_t>(gy * ngm + gx);
-      if (R == 1) {
-        vals[o] = Q.val[k];
-      } else {
+      uint32_t key = Q.key[k] + static_cast<uint32_t>(gy * ngm + gx);
+      if (R == 2) {
         const uint32_t rc = Q.rect[k];
-        const int GX = static_cast<int>(rc & 0xff) / 2 + gx, GY = static_cast<int>((rc >> 16) & 0xff) / 2 + gy;
-        vals[o] = Q.val[k] | (quad_mask(rc, GX, GY) << 28);
+        const int GX = (static_cast<int>(rc & 0xff) >> rsh) + gx;
+        const int GY = (static_cast<int>((rc >> 16) & 0xff) >> rsh) + gy;
+        key |= region_mask(rc, GX, GY, rsh) << rbits;
       }
+      keys[o] = key;
+      vals[o] = Q.val[k];
     }
   }
   wave_sync_();
 }
 
-// R == 1: (8x8 block, particle) entries.  R == 2: (16x16 region, particle |
-// mask << 28) entries, the mask naming the region's 8x8 blocks the particle is
-// binned to; regions of job j are numbered from ubase[j], ngm16 = (ngm + 1) / 2
-// per tile row.
+// R == 1: (8x8 block, particle) entries.  R == 2: (region | unit mask << rbits,
+// particle) entries for regions of S x S blocks (S = 2^rsh), the mask naming
+// the region's blocks the particle is binned to; regions of job j are numbered
+// from ubase[j], ceil(ngm / S) per tile row.
 template <int R>
 __global__ __launch_bounds__(256) void k_emit(const Rec *recs, uint32_t m, const uint32_t *off,
                                               uint32_t e, const BinGeom *geoms,
                                               const uint32_t *ubase, const uint32_t *mo, int nj,
-                                              uint32_t *keys, uint32_t *vals) {
+                                              uint32_t *keys, uint32_t *vals, int rsh,
+                                              int rbits) {
   __shared__ EmitQueue s_q[4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   EmitQueue &Q = s_q[wave];
@@ -333,13 +341,13 @@ __global__ __launch_bounds__(256) void k_emit(const Rec *recs, uint32_t m, const
             ikey = ub + rc.base + static_cast<uint32_t>(rc.gy0 * g.ngm + rc.gx0);
             inn = nx | (static_cast<uint32_t>(g.ngm) << 16);
           } else {
-            const int n16 = (g.ngm + 1) >> 1;
+            const int nr = (g.ngm + (1 << rsh) - 1) >> rsh;
             const uint32_t tile = static_cast<uint32_t>(iy * g.ncx + ix);
-            nx = static_cast<uint32_t>((rc.gx1 >> 1) - (rc.gx0 >> 1) + 1);
-            cnt = nx * static_cast<uint32_t>((rc.gy1 >> 1) - (rc.gy0 >> 1) + 1);
-            ikey = ub + tile * static_cast<uint32_t>(n16 * n16) +
-                   static_cast<uint32_t>((rc.gy0 >> 1) * n16 + (rc.gx0 >> 1));
-            inn = nx | (static_cast<uint32_t>(n16) << 16);
+            nx = static_cast<uint32_t>((rc.gx1 >> rsh) - (rc.gx0 >> rsh) + 1);
+            cnt = nx * static_cast<uint32_t>((rc.gy1 >> rsh) - (rc.gy0 >> rsh) + 1);
+            ikey = ub + tile * static_cast<uint32_t>(nr * nr) +
+                   static_cast<uint32_t>((rc.gy0 >> rsh) * nr + (rc.gx0 >> rsh));
+            inn = nx | (static_cast<uint32_t>(nr) << 16);
             irect = static_cast<uint32_t>(rc.gx0) | (static_cast<uint32_t>(rc.gx1) << 8) |
                     (static_cast<uint32_t>(rc.gy0) << 16) | (static_cast<uint32_t>(rc.gy1) << 24);
           }
@@ -368,32 +376,34 @@ __global__ __launch_bounds__(256) void k_emit(const Rec *recs, uint32_t m, const
       qn += __builtin_popcountll(hb);
       if (qn >= 64) {
         wave_sync_();
-        emit_drain<R>(Q, qn, lane, keys, vals);
+        emit_drain<R>(Q, qn, lane, keys, vals, rsh, rbits);
         qn = 0;
       }
     }
     if (qn) {
       wave_sync_();
-      emit_drain<R>(Q, qn, lane, keys, vals);
+      emit_drain<R>(Q, qn, lane, keys, vals, rsh, rbits);
     }
   }
 }
 
-// A 16x16 region (id r, job j = the one whose region range holds r) and its
-// four 8x8 blocks: unit ids (or ~0u where the block lies past the tile's ngm).
+// A region (id r of S x S blocks, job j = the one whose region range holds r)
+// and its blocks: unit ids (or ~0u where the block lies past the tile's ngm),
+// u[sy S + sx].
+template <int S>
 __device__ __forceinline__ void region_units(uint32_t r, const uint32_t *rbase,
                                              const uint32_t *ubase, const BinGeom *geoms, int nj,
-                                             uint32_t u[4]) {
+                                             uint32_t (&u)[S * S]) {
   const int j = find_job(rbase, nj, r);
-  const int ngm = geoms[j].ngm, n16 = (ngm + 1) >> 1;
+  const int ngm = geoms[j].ngm, nr = (ngm + S - 1) / S;
   const uint32_t local = r - rbase[j];
-  const uint32_t per = static_cast<uint32_t>(n16 * n16);
+  const uint32_t per = static_cast<uint32_t>(nr * nr);
   const uint32_t tile = local / per;
   const int rem = static_cast<int>(local - tile * per);
-  const int GY = rem / n16, GX = rem - GY * n16;
+  const int GY = rem / nr, GX = rem - GY * nr;
 #pragma unroll
-  for (int b = 0; b < 4; ++b) {
-    const int gx = 2 * GX + (b & 1), gy = 2 * GY + (b >> 1);
+  for (int b = 0; b < S * S; ++b) {
+    const int gx = S * GX + (b % S), gy = S * GY + (b / S);
     u[b] = (gx < ngm && gy < ngm)
                ? ubase[j] + tile * static_cast<uint32_t>(ngm * ngm) +
                      static_cast<uint32_t>(gy * ngm + gx)
@@ -401,9 +411,10 @@ __device__ __forceinline__ void region_units(uint32_t r, const uint32_t *rbase,
   }
 }
 
-// Splitting the sorted region lists into 8x8 block lists.  S_b[i] is the
-// number of entries before position i of the sorted region array whose mask
-// has bit b; an entry i of region r with bit b goes to position
+// Splitting the sorted region lists into block lists.  An entry's key is
+// region | mask << rbits.  S_b[i] is the number of entries before position i
+// of the sorted array whose mask has bit b; an entry i of region r with bit b
+// goes to position
 //   boff[unit(r, b)] + S_b[i] - S_b[roff[r]]
 // of its block's list, which keeps list order.  S_b comes from per-chunk bit
 // counts (chunks of kSplitChunk entries, one wave each, so a long region list
@@ -416,127 +427,151 @@ __device__ __forceinline__ uint32_t mbcnt64(uint64_t bal) {
                                    __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(bal), 0u));
 }
 
-__global__ __launch_bounds__(256) void k_split_chunks(const uint32_t *vals, uint32_t e16,
-                                                      uint32_t nchunks, uint32_t *ccnt) {
+// The value of v[b] on lane b (b < NB), as a lane select (no private-array
+// indexing by lane).
+template <int NB>
+__device__ __forceinline__ uint32_t lane_pick(const uint32_t (&v)[NB], int lane) {
+  uint32_t x = 0;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) x = lane == b ? v[b] : x;
+  return x;
+}
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_split_chunks(const uint32_t *keys, uint32_t e16,
+                                                      uint32_t nchunks, int rbits,
+                                                      uint32_t *ccnt) {
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = static_cast<int64_t>(gridDim.x) * 4;
   for (int64_t c = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6); c < nchunks;
        c += nwaves) {
     const uint32_t i0 = static_cast<uint32_t>(c) * kSplitChunk;
     const uint32_t i1 = min(i0 + kSplitChunk, e16);
-    uint32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
+    uint32_t n[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) n[b] = 0;
     for (uint32_t i = i0; i < i1; i += 64) {
-      const uint32_t mk = (i + lane < i1) ? (vals[i + lane] >> 28) : 0u;
-      n0 += static_cast<uint32_t>(__popcll(__ballot(mk & 1u)));
-      n1 += static_cast<uint32_t>(__popcll(__ballot(mk & 2u)));
-      n2 += static_cast<uint32_t>(__popcll(__ballot(mk & 4u)));
-      n3 += static_cast<uint32_t>(__popcll(__ballot(mk & 8u)));
+      const uint32_t mk = (i + lane < i1) ? (keys[i + lane] >> rbits) : 0u;
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+        n[b] += static_cast<uint32_t>(__popcll(__ballot((mk >> b) & 1u)));
     }
-    const uint32_t v = lane == 0 ? n0 : lane == 1 ? n1 : lane == 2 ? n2 : n3;
-    if (lane < 4) ccnt[static_cast<int64_t>(lane) * nchunks + c] = v;
+    const uint32_t v = lane_pick<NB>(n, lane);
+    if (lane < NB) ccnt[static_cast<int64_t>(lane) * nchunks + c] = v;
   }
 }
 
-// S_b at position x (x <= e16) from the scanned chunk counts cpre.
-__device__ __forceinline__ uint32_t split_prefix(const uint32_t *vals, const uint32_t *cpre,
-                                                 uint32_t nchunks, uint32_t x, int b, int lane) {
+// S_b at position x (x <= e16) for every bit b, from the scanned chunk counts.
+template <int NB>
+__device__ __forceinline__ void split_prefix(const uint32_t *keys, const uint32_t *cpre,
+                                             uint32_t nchunks, uint32_t x, int rbits, int lane,
+                                             uint32_t (&n)[NB]) {
   const uint32_t c = x / kSplitChunk;
-  uint32_t n = cpre[static_cast<int64_t>(b) * nchunks + c];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) n[b] = cpre[static_cast<int64_t>(b) * nchunks + c];
   for (uint32_t i = c * kSplitChunk; i < x; i += 64) {
-    const uint32_t mk = (i + lane < x) ? (vals[i + lane] >> 28) : 0u;
-    n += static_cast<uint32_t>(__popcll(__ballot((mk >> b) & 1u)));
+    const uint32_t mk = (i + lane < x) ? (keys[i + lane] >> rbits) : 0u;
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+      n[b] += static_cast<uint32_t>(__popcll(__ballot((mk >> b) & 1u)));
   }
-  return n;
 }
 
-// One wave per region: S_b at the region's start (sx[4r + b]) and its blocks'
-// list lengths (ucnt; every unit is written once).
-__global__ __launch_bounds__(256) void k_split_bounds(const uint32_t *roff, const uint32_t *vals,
+// One wave per region: S_b at the region's start (sx[NB r + b]) and its
+// blocks' list lengths (ucnt; every unit is written once).
+template <int S>
+__global__ __launch_bounds__(256) void k_split_bounds(const uint32_t *roff, const uint32_t *keys,
                                                       uint32_t nregions, const uint32_t *cpre,
-                                                      uint32_t nchunks, const uint32_t *rbase,
+                                                      uint32_t nchunks, int rbits,
+                                                      const uint32_t *rbase,
                                                       const uint32_t *ubase,
                                                       const BinGeom *geoms, int nj,
                                                       uint32_t *sx, uint32_t *ucnt) {
+  constexpr int NB = S * S;
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = static_cast<int64_t>(gridDim.x) * 4;
   for (int64_t r = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6); r < nregions;
        r += nwaves) {
     const uint32_t x0 = roff[r], x1 = roff[r + 1];
-    uint32_t s0[4], s1[4], u[4];
+    uint32_t s0[NB], s1[NB], u[NB];
+    split_prefix<NB>(keys, cpre, nchunks, x0, rbits, lane, s0);
+    if (x1 == x0) {
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      s0[b] = split_prefix(vals, cpre, nchunks, x0, b, lane);
-      s1[b] = x1 == x0 ? s0[b] : split_prefix(vals, cpre, nchunks, x1, b, lane);
+      for (int b = 0; b < NB; ++b) s1[b] = s0[b];
+    } else {
+      split_prefix<NB>(keys, cpre, nchunks, x1, rbits, lane, s1);
     }
-    region_units(static_cast<uint32_t>(r), rbase, ubase, geoms, nj, u);
-    const uint32_t uu = lane == 0 ? u[0] : lane == 1 ? u[1] : lane == 2 ? u[2] : u[3];
-    const uint32_t a0 = lane == 0 ? s0[0] : lane == 1 ? s0[1] : lane == 2 ? s0[2] : s0[3];
-    const uint32_t a1 = lane == 0 ? s1[0] : lane == 1 ? s1[1] : lane == 2 ? s1[2] : s1[3];
-    if (lane < 4) {
-      sx[4 * r + lane] = a0;
+    region_units<S>(static_cast<uint32_t>(r), rbase, ubase, geoms, nj, u);
+    const uint32_t uu = lane_pick<NB>(u, lane);
+    const uint32_t a0 = lane_pick<NB>(s0, lane), a1 = lane_pick<NB>(s1, lane);
+    if (lane < NB) {
+      sx[NB * r + lane] = a0;
       if (uu != ~0u) ucnt[uu] = a1 - a0;
     }
   }
 }
 
-// Per region and bit: boff[unit] - S_b[region start] (0 for blocks past ngm,
-// which no entry names).
+// Per region and block: boff[unit] - S_b[region start] (0 for blocks past
+// ngm, which no entry names).
+template <int S>
 __global__ __launch_bounds__(256) void k_split_base(uint32_t nregions, const uint32_t *rbase,
                                                     const uint32_t *ubase, const BinGeom *geoms,
                                                     int nj, const uint32_t *sx,
-                                                    const uint32_t *boff, uint4 *base) {
+                                                    const uint32_t *boff, uint32_t *base) {
+  constexpr int NB = S * S;
   for (int64_t r = gtid(); r < nregions; r += gstride()) {
-    uint32_t u[4];
-    region_units(static_cast<uint32_t>(r), rbase, ubase, geoms, nj, u);
-    uint4 o;
-    o.x = u[0] != ~0u ? boff[u[0]] - sx[4 * r] : 0u;
-    o.y = u[1] != ~0u ? boff[u[1]] - sx[4 * r + 1] : 0u;
-    o.z = u[2] != ~0u ? boff[u[2]] - sx[4 * r + 2] : 0u;
-    o.w = u[3] != ~0u ? boff[u[3]] - sx[4 * r + 3] : 0u;
-    base[r] = o;
+    uint32_t u[NB];
+    region_units<S>(static_cast<uint32_t>(r), rbase, ubase, geoms, nj, u);
+    uint32_t o[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) o[b] = u[b] != ~0u ? boff[u[b]] - sx[NB * r + b] : 0u;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) base[NB * r + b] = o[b];
   }
 }
 
-// One wave per chunk: every entry to its blocks' lists (rkeys: the sorted
-// region ids).
-__global__ __launch_bounds__(256) void k_split_scatter(const uint32_t *rkeys, const uint32_t *vals,
-                                                       uint32_t e16, uint32_t nchunks,
-                                                       const uint32_t *cpre, const uint4 *base,
-                                                       uint32_t *ent) {
+// One wave per chunk: every entry to its blocks' lists.
+template <int S>
+__global__ __launch_bounds__(256) void k_split_scatter(const uint32_t *keys, const uint32_t *vals,
+                                                       uint32_t e16, uint32_t nchunks, int rbits,
+                                                       const uint32_t *cpre,
+                                                       const uint32_t *base, uint32_t *ent) {
+  constexpr int NB = S * S;
   const int lane = threadIdx.x & 63;
+  const uint32_t rmask = (1u << rbits) - 1u;
   const int64_t nwaves = static_cast<int64_t>(gridDim.x) * 4;
   for (int64_t c = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6); c < nchunks;
        c += nwaves) {
     const uint32_t i0 = static_cast<uint32_t>(c) * kSplitChunk;
     const uint32_t i1 = min(i0 + kSplitChunk, e16);
-    uint32_t n0 = cpre[c], n1 = cpre[nchunks + c], n2 = cpre[2 * static_cast<int64_t>(nchunks) + c],
-             n3 = cpre[3 * static_cast<int64_t>(nchunks) + c];
+    uint32_t n[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) n[b] = cpre[static_cast<int64_t>(b) * nchunks + c];
     for (uint32_t i = i0; i < i1; i += 64) {
       const bool in = i + lane < i1;
-      const uint32_t v = in ? vals[i + lane] : 0u;
-      const uint32_t mk = v >> 28, idx = v & 0x0fffffffu;
-      const uint4 bs = in ? base[rkeys[i + lane]] : uint4{0u, 0u, 0u, 0u};
-      const uint64_t m0 = __ballot(mk & 1u), m1 = __ballot(mk & 2u), m2 = __ballot(mk & 4u),
-                     m3 = __ballot(mk & 8u);
-      if (mk & 1u) ent[bs.x + n0 + mbcnt64(m0)] = idx;
-      if (mk & 2u) ent[bs.y + n1 + mbcnt64(m1)] = idx;
-      if (mk & 4u) ent[bs.z + n2 + mbcnt64(m2)] = idx;
-      if (mk & 8u) ent[bs.w + n3 + mbcnt64(m3)] = idx;
-      n0 += static_cast<uint32_t>(__popcll(m0));
-      n1 += static_cast<uint32_t>(__popcll(m1));
-      n2 += static_cast<uint32_t>(__popcll(m2));
-      n3 += static_cast<uint32_t>(__popcll(m3));
+      const uint32_t k = in ? keys[i + lane] : 0u;
+      const uint32_t idx = in ? vals[i + lane] : 0u;
+      const uint32_t mk = k >> rbits;
+      const uint32_t *bs = base + static_cast<int64_t>(NB) * (k & rmask);
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const bool on = (mk >> b) & 1u;
+        const uint64_t bal = __ballot(on);
+        if (on) ent[bs[b] + n[b] + mbcnt64(bal)] = idx;
+        n[b] += static_cast<uint32_t>(__popcll(bal));
+      }
     }
   }
 }
 
 __global__ __launch_bounds__(256) void k_bin_offsets(const uint32_t *keys, uint32_t e,
-                                                     uint32_t nbins, uint32_t *bin_off) {
+                                                     uint32_t nbins, uint32_t kmask,
+                                                     uint32_t *bin_off) {
   for (int64_t r = gtid(); r <= nbins; r += gstride()) {
     uint32_t lo = 0, hi = e;
     while (lo < hi) {
       const uint32_t mid = lo + ((hi - lo) >> 1);
-      if (keys[mid] < r) lo = mid + 1;
+      if ((keys[mid] & kmask) < r) lo = mid + 1;
       else hi = mid;
     }
     bin_off[r] = lo;
@@ -790,21 +825,22 @@ void launch_job_of_index(const uint32_t *vals, uint32_t m, const uint32_t *pbase
 }
 
 void launch_gather_count(const uint32_t *order, uint32_t m, const Rec *rec, Rec *recs,
-                         uint32_t *cnt, uint32_t *cnt16, const BinGeom *geoms,
+                         uint32_t *cnt, uint32_t *cnt16, int rsh, const BinGeom *geoms,
                          const uint32_t *mo, int nj, hipStream_t s) {
-  k_gather_count<<<grid_for(m), 256, 0, s>>>(order, m, rec, recs, cnt, cnt16, geoms, mo, nj);
+  k_gather_count<<<grid_for(m), 256, 0, s>>>(order, m, rec, recs, cnt, cnt16, rsh, geoms, mo,
+                                             nj);
   hip_check(hipGetLastError(), "k_gather_count launch");
 }
 
-void launch_emit(int regions16, const Rec *recs, uint32_t m, const uint32_t *off, uint32_t e,
-                 const BinGeom *geoms, const uint32_t *ubase, const uint32_t *mo, int nj,
-                 uint32_t *keys, uint32_t *vals, hipStream_t s) {
-  if (regions16)
+void launch_emit(int rsh, int rbits, const Rec *recs, uint32_t m, const uint32_t *off,
+                 uint32_t e, const BinGeom *geoms, const uint32_t *ubase, const uint32_t *mo,
+                 int nj, uint32_t *keys, uint32_t *vals, hipStream_t s) {
+  if (rsh > 0)
     k_emit<2><<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(recs, m, off, e, geoms, ubase, mo, nj,
-                                                        keys, vals);
+                                                        keys, vals, rsh, rbits);
   else
     k_emit<1><<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(recs, m, off, e, geoms, ubase, mo, nj,
-                                                        keys, vals);
+                                                        keys, vals, 0, 0);
   hip_check(hipGetLastError(), "k_emit launch");
 }
 
@@ -812,46 +848,62 @@ uint32_t split_chunks(uint32_t e16) {
   return (e16 + kSplitChunk - 1) / kSplitChunk;
 }
 
-void launch_split_chunks(const uint32_t *vals, uint32_t e16, uint32_t *ccnt, hipStream_t s) {
+void launch_split_chunks(int S, const uint32_t *keys, uint32_t e16, int rbits, uint32_t *ccnt,
+                         hipStream_t s) {
   const uint32_t nc = split_chunks(e16);
   if (nc == 0) return;
-  k_split_chunks<<<grid_for(nc, 4, 1 << 16), 256, 0, s>>>(vals, e16, nc, ccnt);
+  const unsigned g = grid_for(nc, 4, 1 << 16);
+  if (S == 4) k_split_chunks<16><<<g, 256, 0, s>>>(keys, e16, nc, rbits, ccnt);
+  else k_split_chunks<4><<<g, 256, 0, s>>>(keys, e16, nc, rbits, ccnt);
   hip_check(hipGetLastError(), "k_split_chunks launch");
 }
 
-void launch_split_bounds(const uint32_t *roff, const uint32_t *vals, uint32_t nregions,
-                         const uint32_t *cpre, uint32_t e16, const uint32_t *rbase,
+void launch_split_bounds(int S, const uint32_t *roff, const uint32_t *keys, uint32_t nregions,
+                         const uint32_t *cpre, uint32_t e16, int rbits, const uint32_t *rbase,
                          const uint32_t *ubase, const BinGeom *geoms, int nj, uint32_t *sx,
                          uint32_t *ucnt, hipStream_t s) {
   if (nregions == 0) return;
-  k_split_bounds<<<grid_for(nregions, 4, 1 << 16), 256, 0, s>>>(
-      roff, vals, nregions, cpre, split_chunks(e16), rbase, ubase, geoms, nj, sx, ucnt);
+  const unsigned g = grid_for(nregions, 4, 1 << 16);
+  const uint32_t nc = split_chunks(e16);
+  if (S == 4)
+    k_split_bounds<4><<<g, 256, 0, s>>>(roff, keys, nregions, cpre, nc, rbits, rbase, ubase,
+                                         geoms, nj, sx, ucnt);
+  else
+    k_split_bounds<2><<<g, 256, 0, s>>>(roff, keys, nregions, cpre, nc, rbits, rbase, ubase,
+                                         geoms, nj, sx, ucnt);
   hip_check(hipGetLastError(), "k_split_bounds launch");
 }
 
-void launch_split_base(uint32_t nregions, const uint32_t *rbase, const uint32_t *ubase,
+void launch_split_base(int S, uint32_t nregions, const uint32_t *rbase, const uint32_t *ubase,
                        const BinGeom *geoms, int nj, const uint32_t *sx, const uint32_t *boff,
                        uint32_t *base, hipStream_t s) {
   if (nregions == 0) return;
-  k_split_base<<<grid_for(nregions), 256, 0, s>>>(nregions, rbase, ubase, geoms, nj, sx, boff,
-                                                  reinterpret_cast<uint4 *>(base));
+  if (S == 4)
+    k_split_base<4><<<grid_for(nregions), 256, 0, s>>>(nregions, rbase, ubase, geoms, nj, sx,
+                                                        boff, base);
+  else
+    k_split_base<2><<<grid_for(nregions), 256, 0, s>>>(nregions, rbase, ubase, geoms, nj, sx,
+                                                        boff, base);
   hip_check(hipGetLastError(), "k_split_base launch");
 }
 
-void launch_split_scatter(const uint32_t *rkeys, const uint32_t *vals, uint32_t e16,
-                          const uint32_t *cpre, const uint32_t *base, uint32_t *ent,
+void launch_split_scatter(int S, const uint32_t *keys, const uint32_t *vals, uint32_t e16,
+                          int rbits, const uint32_t *cpre, const uint32_t *base, uint32_t *ent,
                           hipStream_t s) {
   const uint32_t nc = split_chunks(e16);
   if (nc == 0) return;
-  k_split_scatter<<<grid_for(nc, 4, 1 << 16), 256, 0, s>>>(
-      rkeys, vals, e16, nc, cpre, reinterpret_cast<const uint4 *>(base), ent);
+  const unsigned g = grid_for(nc, 4, 1 << 16);
+  if (S == 4)
+    k_split_scatter<4><<<g, 256, 0, s>>>(keys, vals, e16, nc, rbits, cpre, base, ent);
+  else
+    k_split_scatter<2><<<g, 256, 0, s>>>(keys, vals, e16, nc, rbits, cpre, base, ent);
   hip_check(hipGetLastError(), "k_split_scatter launch");
 }
 
 void launch_bin_offsets(const uint32_t *sorted_keys, uint32_t e, uint32_t nbins,
-                        uint32_t *bin_off, hipStream_t s) {
+                        uint32_t *bin_off, hipStream_t s, uint32_t kmask) {
   k_bin_offsets<<<grid_for(static_cast<int64_t>(nbins) + 1), 256, 0, s>>>(sorted_keys, e, nbins,
-                                                                           bin_off);
+                                                                           kmask, bin_off);
   hip_check(hipGetLastError(), "k_bin_offsets launch");
 }
 

@@ -71,38 +71,41 @@ void launch_pass_offsets(const uint32_t *pos, const uint32_t *pbase, int nj, int
 void launch_job_of_index(const uint32_t *vals, uint32_t m, const uint32_t *pbase, int nj,
                          uint32_t *keys, hipStream_t s);
 // cnt: entries per particle (units of the binned tiles); cnt16 (optional):
-// the 2x2 groups of those 8x8 blocks (16x16 regions).
+// the regions of 2^rsh x 2^rsh of those 8x8 blocks they cover.
 void launch_gather_count(const uint32_t *order, uint32_t m, const Rec *rec, Rec *recs,
-                         uint32_t *cnt, uint32_t *cnt16, const BinGeom *geoms,
+                         uint32_t *cnt, uint32_t *cnt16, int rsh, const BinGeom *geoms,
                          const uint32_t *mo, int nj, hipStream_t s);
-// regions16 == 0: (unit, particle) entries; 1: (16x16 region, particle |
-// mask << 28) entries, regions of job j numbered from ubase[j].
-void launch_emit(int regions16, const Rec *recs, uint32_t m, const uint32_t *off, uint32_t e,
-                 const BinGeom *geoms, const uint32_t *ubase, const uint32_t *mo, int nj,
-                 uint32_t *keys, uint32_t *vals, hipStream_t s);
-// 8x8 block lists from the sorted 16x16 region lists (rkeys / vals: e16
-// sorted entries; roff: region CSR; rbase / ubase: first region / unit of
-// each job):
-//   launch_split_chunks   per-chunk mask-bit counts ccnt[4 * split_chunks(e16)]
-//   scan_u32(ccnt -> cpre, 4 * nchunks, total at cpre[4 * nchunks])
-//   launch_split_bounds   sx[4 * nregions] and the block list lengths ucnt
+// rsh == 0: (unit, particle) entries; rsh > 0: (region | unit mask << rbits,
+// particle) entries for regions of 2^rsh x 2^rsh blocks, regions of job j
+// numbered from ubase[j] (then ubase holds the region bases).
+void launch_emit(int rsh, int rbits, const Rec *recs, uint32_t m, const uint32_t *off,
+                 uint32_t e, const BinGeom *geoms, const uint32_t *ubase, const uint32_t *mo,
+                 int nj, uint32_t *keys, uint32_t *vals, hipStream_t s);
+// Block lists from the sorted region lists (keys: region | mask << rbits,
+// vals: particles; S x S blocks per region, S = 2 or 4; roff: region CSR;
+// rbase / ubase: first region / unit of each job):
+//   launch_split_chunks   per-chunk mask-bit counts ccnt[S^2 * split_chunks(e16)]
+//   scan_u32(ccnt -> cpre, S^2 * nchunks, total at cpre[S^2 * nchunks])
+//   launch_split_bounds   sx[S^2 * nregions] and the block list lengths ucnt
 //   scan_u32(ucnt -> boff, nunits, total at boff[nunits])
-//   launch_split_base     base[4 * nregions] from boff and sx
+//   launch_split_base     base[S^2 * nregions] from boff and sx (may be sx)
 //   launch_split_scatter  the block lists ent, in list order
 uint32_t split_chunks(uint32_t e16);
-void launch_split_chunks(const uint32_t *vals, uint32_t e16, uint32_t *ccnt, hipStream_t s);
-void launch_split_bounds(const uint32_t *roff, const uint32_t *vals, uint32_t nregions,
-                         const uint32_t *cpre, uint32_t e16, const uint32_t *rbase,
+void launch_split_chunks(int S, const uint32_t *keys, uint32_t e16, int rbits, uint32_t *ccnt,
+                         hipStream_t s);
+void launch_split_bounds(int S, const uint32_t *roff, const uint32_t *keys, uint32_t nregions,
+                         const uint32_t *cpre, uint32_t e16, int rbits, const uint32_t *rbase,
                          const uint32_t *ubase, const BinGeom *geoms, int nj, uint32_t *sx,
                          uint32_t *ucnt, hipStream_t s);
-void launch_split_base(uint32_t nregions, const uint32_t *rbase, const uint32_t *ubase,
+void launch_split_base(int S, uint32_t nregions, const uint32_t *rbase, const uint32_t *ubase,
                        const BinGeom *geoms, int nj, const uint32_t *sx, const uint32_t *boff,
                        uint32_t *base, hipStream_t s);
-void launch_split_scatter(const uint32_t *rkeys, const uint32_t *vals, uint32_t e16,
-                          const uint32_t *cpre, const uint32_t *base, uint32_t *ent,
+void launch_split_scatter(int S, const uint32_t *keys, const uint32_t *vals, uint32_t e16,
+                          int rbits, const uint32_t *cpre, const uint32_t *base, uint32_t *ent,
                           hipStream_t s);
+// kmask: the bits of a key that hold the bin (the rest ride along unsorted).
 void launch_bin_offsets(const uint32_t *sorted_keys, uint32_t e, uint32_t nbins,
-                        uint32_t *bin_off, hipStream_t s);
+                        uint32_t *bin_off, hipStream_t s, uint32_t kmask = ~0u);
 void launch_map(const uint32_t *idx, const uint32_t *table, uint32_t *out, int64_t n,
                 hipStream_t s);
 void launch_fill(float *p, float v, int64_t n, hipStream_t s);

@@ -325,21 +325,23 @@ def test_dense_cluster_vs_oracle(R, aeq):
                         % (G, var, split, heavy, hmax))
 
 
+@pytest.mark.parametrize("grp", [4, 2])
 @pytest.mark.parametrize("T", [50, 37, 24, 16])
-def test_region_binning_vs_oracle(R, T):
-    """Large footprints are binned through 16x16 regions (2x2 blocks, split
-    into the blocks' lists afterwards), point-like ones per block; both give
-    the oracle's image bit for bit, with tile sizes whose 8x8 grid is odd (a
-    region with blocks past the tile edge) or even."""
+def test_region_binning_vs_oracle(R, T, grp):
+    """Large footprints are binned through regions of grp x grp 8x8 blocks
+    (split into the blocks' lists afterwards), point-like ones per block; both
+    give the oracle's image bit for bit, with tile sizes whose 8x8 grid is not
+    a multiple of the region (regions with blocks past the tile edge) or is."""
     import splotch_amd as S
-    log("case region binning T", T)
+    log("case region binning T", T, "group", grp)
     gas = synth.plummer_field(30_000, 1520 + T, ptype=0, halo_frac=1.0, n_halos=2,
                               hsml_mu=math.log(3.0))  # radii of ~10-30 px
     stars = synth.plummer_field(20_000, 1540 + T, ptype=1, hsml_mu=math.log(0.01))
     fld = synth.concat(gas, stars)
     v = synth.default_view(200, 168)
     pal = synth.default_palette()
-    prm = S.params(ptypes=2, a_eq_e=[0, 1], brightness=[0.5, 2.0], tile_size=T)
+    prm = S.params(ptypes=2, a_eq_e=[0, 1], brightness=[0.5, 2.0], tile_size=T,
+                   hip_bin_group=grp)
     pic = render(R, fld, v, [pal, pal], prm)
     regions = R.binning()
     _, counts = R.stage_times()
