@@ -231,6 +231,7 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
 struct EmitQueue {
   uint32_t pos[kEmitQ], key[kEmitQ], val[kEmitQ], nx_ngm[kEmitQ], n[kEmitQ], pre[kEmitQ];
   uint32_t rect[kEmitQ];  // R == 2: the tile's 8x8-block rectangle gx0 | gx1 | gy0 | gy1 bytes
+  float px[kEmitQ], py[kEmitQ], rf[kEmitQ];  // R == 2: tile-frame centre, footprint radius
 };
 
 // Mask of the units (S GX + sx, S GY + sy) of region (GX, GY), S = 2^rsh, bit
@@ -247,10 +248,47 @@ __device__ __forceinline__ uint32_t region_mask(uint32_t rc, int GX, int GY, int
   return m;
 }
 
+// Blocks (bit S sy + sx) of region (GX, GY), S = 2^rsh, that the footprint of
+// radius rf about tile-frame point (px, py) can reach.  render_new updates
+// pixel column x only for x in (posx - rfacr, posx + rfacr] and, within it,
+// rows y with |y - posy| <= sqrtf(radsq - (x - posx)^2) (splotch_host.cc:
+// 591-607), so every updated pixel lies within rfacr of the centre up to the
+// float rounding of those bounds, which stays far below the 2 px and 2^-16
+// relative margins used here.  A block none of whose pixels [8 b, 8 b + 7]
+// per axis is that close gets no update from the particle: its entry would
+// add +0 to every pixel, and leaving it out of the block's list keeps the
+// image bit for bit.  NaN or infinite operands keep every block.
+__device__ __forceinline__ uint32_t region_reach(float px, float py, float rf, int GX, int GY,
+                                                 int rsh) {
+  const int S = 1 << rsh;
+  const float lim = rf + 2.f;
+  const float thr = (lim * lim) * 1.0000153f;
+  float dy2[4];
+#pragma unroll
+  for (int sy = 0; sy < 4; ++sy) {
+    const float y0 = static_cast<float>(8 * (S * GY + sy));
+    const float d = fmaxf(fmaxf(y0 - py, py - (y0 + 7.f)), 0.f);
+    dy2[sy] = d * d;
+  }
+  uint32_t m = 0;
+#pragma unroll
+  for (int sx = 0; sx < 4; ++sx) {
+    if (sx >= S) break;
+    const float x0 = static_cast<float>(8 * (S * GX + sx));
+    const float d = fmaxf(fmaxf(x0 - px, px - (x0 + 7.f)), 0.f);
+    const float dx2 = d * d;
+#pragma unroll
+    for (int sy = 0; sy < 4; ++sy)
+      if (sy < S && !(dx2 + dy2[sy] > thr)) m |= 1u << (sy * S + sx);
+  }
+  return m;
+}
+
 // Write the entries of items [0, qn) of the queue: item k covers output
 // [pos, pos + n), entry i = (gy, gx) = (i / nx, i % nx) is unit key + gy*ngm + gx
-// (R == 1: 8x8 blocks; R == 2: regions of 2^rsh x 2^rsh blocks, key | unit
-// mask << rbits).
+// (R == 1: 8x8 blocks; R == 2: regions of 2^rsh x 2^rsh blocks, key | mask
+// << rbits of the region's blocks in the rectangle that the footprint
+// reaches; a region whose mask comes out empty adds to no block list).
 template <int R>
 __device__ __forceinline__ void emit_drain(EmitQueue &Q, int qn, int lane, uint32_t *keys,
                                            uint32_t *vals, int rsh, int rbits) {
@@ -286,7 +324,9 @@ __device__ __forceinline__ void emit_drain(EmitQueue &Q, int qn, int lane, uint3
         const uint32_t rc = Q.rect[k];
         const int GX = (static_cast<int>(rc & 0xff) >> rsh) + gx;
         const int GY = (static_cast<int>((rc >> 16) & 0xff) >> rsh) + gy;
-        key |= region_mask(rc, GX, GY, rsh) << rbits;
+        key |= (region_mask(rc, GX, GY, rsh) & region_reach(Q.px[k], Q.py[k], Q.rf[k], GX, GY,
+                                                            rsh))
+               << rbits;
       }
       keys[o] = key;
       vals[o] = Q.val[k];
@@ -331,6 +371,7 @@ __global__ __launch_bounds__(256) void k_emit(const Rec *recs, uint32_t m, const
     while (__ballot(more)) {
       bool have = false;
       uint32_t ipos = 0, ikey = 0, inn = 0, in = 0, irect = 0;
+      float ipx = 0.f, ipy = 0.f, irf = 0.f;
       if (more) {
         BinRect rc;
         if (bin_tile(P, g, br, ix, iy, rc)) {
@@ -350,6 +391,9 @@ __global__ __launch_bounds__(256) void k_emit(const Rec *recs, uint32_t m, const
             inn = nx | (static_cast<uint32_t>(nr) << 16);
             irect = static_cast<uint32_t>(rc.gx0) | (static_cast<uint32_t>(rc.gx1) << 8) |
                     (static_cast<uint32_t>(rc.gy0) << 16) | (static_cast<uint32_t>(rc.gy1) << 24);
+            ipx = P.x - static_cast<float>(ix * g.T);  // as bin_tile's posx, posy, rfacr
+            ipy = P.y - static_cast<float>(iy * g.T);
+            irf = P.r * g.rfac;
           }
           in = min(cnt, end - q);
           have = in > 0;
@@ -372,6 +416,11 @@ __global__ __launch_bounds__(256) void k_emit(const Rec *recs, uint32_t m, const
         Q.nx_ngm[slot] = inn;
         Q.n[slot] = in;
         Q.rect[slot] = irect;
+        if (R == 2) {
+          Q.px[slot] = ipx;
+          Q.py[slot] = ipy;
+          Q.rf[slot] = irf;
+        }
       }
       qn += __builtin_popcountll(hb);
       if (qn >= 64) {
