@@ -610,16 +610,31 @@ __device__ __forceinline__ void lds_barrier() {
 template <int ADD>
 __global__ __launch_bounds__(kPThreads) void k_splat_pc(SplatArgs a, const uint32_t *items,
                                                         uint32_t npc) {
-  __shared__ float s_tab[2048];
-  __shared__ Rec s_rec[2][kSRP];        // staged records, two super-batches
-  __shared__ PcPart s_part[2][kB];
-  __shared__ float4 s_ca[4][kB];        // (a0, a1, a2, max |a|)
-  __shared__ float4 s_cq[4][kB];        // (q0, q1, q2, 0)
-  __shared__ uint64_t s_fast[4];        // bit k: particle k has every a_c in [kFastA, 0]
-  __shared__ uint32_t s_cm[2][kB][8];   // column row-masks
-  __shared__ float s_p2[2][kB][8];      // column factors
-  __shared__ float s_p1[2][kB][8];      // row factors
-  __shared__ float4 s_op[2][kB][64];    // consumer operands (m1_0, m1_1, m1_2, m2) / (t0, t1, t2, 0)
+  // One LDS block with the exp table at address 0, so that a table entry's
+  // address is its byte offset (no base to add per lookup).
+  struct Lds {
+    float tab[2048];
+    float4 op[2][kB][64];   // consumer operands (m1_0, m1_1, m1_2, m2) / (t0, t1, t2, 0)
+    float4 ca[4][kB];       // (a0, a1, a2, max |a|)
+    float4 cq[4][kB];       // (q0, q1, q2, 0)
+    Rec rec[2][kSRP];       // staged records, two super-batches
+    PcPart part[2][kB];
+    uint64_t fast[4];       // bit k: particle k has every a_c in [kFastA, 0]
+    uint32_t cm[2][kB][8];  // column row-masks
+    float p2[2][kB][8];     // column factors
+    float p1[2][kB][8];     // row factors
+  };
+  __shared__ Lds L;
+  float(&s_tab)[2048] = L.tab;
+  Rec(&s_rec)[2][kSRP] = L.rec;
+  PcPart(&s_part)[2][kB] = L.part;
+  float4(&s_ca)[4][kB] = L.ca;
+  float4(&s_cq)[4][kB] = L.cq;
+  uint64_t(&s_fast)[4] = L.fast;
+  uint32_t(&s_cm)[2][kB][8] = L.cm;
+  float(&s_p2)[2][kB][8] = L.p2;
+  float(&s_p1)[2][kB][8] = L.p1;
+  float4(&s_op)[2][kB][64] = L.op;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = static_cast<int>(uniform(static_cast<uint32_t>(tid >> 6)));
   const bool producer = wave >= 1;
