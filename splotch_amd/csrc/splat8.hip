@@ -65,8 +65,16 @@ __device__ __forceinline__ float xexpm1_bf(const float *tab, float arg, float ex
 }
 
 // Absorption range of the fast table path: att * a * expfac stays in [0, 2^20)
-// for att in [0, 1] (expfac = 2^20 / -20).
+// for att in [0, 1] (expfac = 2^20 / -20).  A fast particle also has finite
+// q_c (e_c / (e_c + gray_absorption) is not when gray_absorption <= 0 meets
+// e_c = -gray_absorption): the fast paths apply it outside the footprint as
+// l + ((l - q) * m1) * 0, which must be l.
 constexpr float kFastA = -19.5f;
+
+__device__ __forceinline__ bool finite3(float x, float y, float z) {
+  const float inf = __builtin_inff();
+  return fabsf(x) < inf && fabsf(y) < inf && fabsf(z) < inf;
+}
 
 // exptable::expm1 (splotchutils.h:203-214) split in two for pipelining:
 // xexp_index gives the table index for p = arg*expfac (0 where the table does
@@ -423,7 +431,7 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
         vq2 = R.e2 / (R.e2 + g);
         vmax = fmaxf(fabsf(va0), fmaxf(fabsf(va1), fabsf(va2)));
         vfast = va0 >= kFastA && va0 <= 0.f && va1 >= kFastA && va1 <= 0.f && va2 >= kFastA &&
-                va2 <= 0.f;
+                va2 <= 0.f && finite3(vq0, vq1, vq2);
         wcst[lane][0] = make_float4(va0, va1, va2, vmax);
         wcst[lane][1] = make_float4(vq0, vq1, vq2, 0.f);
       }
@@ -681,7 +689,8 @@ __global__ __launch_bounds__(kPThreads) void k_splat_pc(SplatArgs a, const uint3
       q1 = R.e1 / (R.e1 + g);
       q2 = R.e2 / (R.e2 + g);
       mx = fmaxf(fabsf(a0), fmaxf(fabsf(a1), fabsf(a2)));
-      fast = a0 >= kFastA && a0 <= 0.f && a1 >= kFastA && a1 <= 0.f && a2 >= kFastA && a2 <= 0.f;
+      fast = a0 >= kFastA && a0 <= 0.f && a1 >= kFastA && a1 <= 0.f && a2 >= kFastA &&
+             a2 <= 0.f && finite3(q0, q1, q2);
     }
     const uint64_t fb = __ballot(fast);
     if (lane < kB) {
@@ -703,11 +712,14 @@ __global__ __launch_bounds__(kPThreads) void k_splat_pc(SplatArgs a, const uint3
     bool vt[NT];
 #pragma unroll
     for (int i = 0; i < NT; ++i) {
+      // tasks [0, 8 kB) are columns, [8 kB, 16 kB) rows, so a wave runs
+      // one kind (the column path is the longer one)
       const int t = ptid + i * kPProd;
-      vt[i] = t < cnt * 16;
-      const int tc = vt[i] ? t : 0;
-      kt[i] = tc >> 4;
-      jt[i] = tc & 15;
+      const bool row = t >= kB * 8;
+      const int tr = row ? t - kB * 8 : t;
+      vt[i] = t < kB * 16 && (tr >> 3) < cnt;
+      kt[i] = vt[i] ? tr >> 3 : 0;
+      jt[i] = vt[i] ? (tr & 7) + (row ? 8 : 0) : 0;
       P[i] = part[kt[i]];
     }
     float arg[NT], dd[NT];
