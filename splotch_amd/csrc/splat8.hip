@@ -432,8 +432,12 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
         vmax = fmaxf(fabsf(va0), fmaxf(fabsf(va1), fabsf(va2)));
         vfast = va0 >= kFastA && va0 <= 0.f && va1 >= kFastA && va1 <= 0.f && va2 >= kFastA &&
                 va2 <= 0.f && finite3(vq0, vq1, vq2);
-        wcst[lane][0] = make_float4(va0, va1, va2, vmax);
-        wcst[lane][1] = make_float4(vq0, vq1, vq2, 0.f);
+        // only the fast chunks read these, also for masked slots past the
+        // batch, so a particle that is not fast (possibly a non-finite a or
+        // q) leaves zeros, which a masked slot applies as l + 0
+        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        wcst[lane][0] = vfast ? make_float4(va0, va1, va2, vmax) : z4;
+        wcst[lane][1] = vfast ? make_float4(vq0, vq1, vq2, 0.f) : z4;
       }
     }
     wave_sync();

@@ -325,6 +325,32 @@ def test_dense_cluster_vs_oracle(R, aeq):
                         % (G, var, split, heavy, hmax))
 
 
+@pytest.mark.parametrize("var,heavy,hmax", [(4, -1, 0), (3, -1, 0)])
+def test_zero_gray_absorption_nan_footprints(R, var, heavy, hmax):
+    """gray_absorption = 0 makes q = e / (e + 0) = 0/0 for colour channels
+    with e = 0 (the hot palette's green and blue below a third): the
+    reference writes NaN into those channels of the particle's footprint
+    pixels only.  The 8x8-block splat (producer/consumer workgroups off: they
+    also write NaN into the rest of such a block) gives NaN in exactly the
+    oracle's pixels and every other value bit for bit: its fast chunks apply a
+    particle to the whole block with zero operands outside the footprint, so
+    such a particle must neither take the fast table path nor leave its q
+    where a masked slot reads it."""
+    import splotch_amd as S
+    log("case gray_absorption 0 variant", var, "heavy", heavy, hmax)
+    fld = synth.plummer_field(20_000, 1620, halo_frac=1.0, n_halos=3, hsml_mu=math.log(0.6))
+    v = synth.default_view(176, 144)
+    pal = synth.default_palette()
+    opic, _ = O.host_rendering(fld, v, [pal], a_eq_e=False, grayabsorb=0.0, stable=True)
+    pic = render(R, fld, v, [pal], S.params(a_eq_e=False, gray_absorption=0.0, hip_heavy=heavy,
+                                            hip_heavy_max=hmax, hip_variant=var))
+    on = np.isnan(opic)
+    log("NaN channels: oracle %d of %d" % (int(on.sum()), on.size))
+    assert on.any() and not on.all()
+    assert np.array_equal(np.isnan(pic), on), "NaN pixels differ"
+    assert_bitexact(pic[~on], opic[~on], "finite pixels, gray_absorption 0")
+
+
 @pytest.mark.parametrize("grp", [4, 2])
 @pytest.mark.parametrize("T", [50, 37, 24, 16])
 def test_region_binning_vs_oracle(R, T, grp):
