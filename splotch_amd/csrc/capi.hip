@@ -387,7 +387,7 @@ struct Plan {
   JobSplat *d_js = nullptr;
   // per pass
   uint32_t m[2] = {0, 0}, e[2] = {0, 0}, nitems[2] = {0, 0}, nheavy[2] = {0, 0};
-  uint32_t e16[2] = {0, 0};         // (16x16 region, particle) entries
+  uint32_t e16[2] = {0, 0};         // (region, particle) entries
   bool r16[2] = {false, false};     // the pass's region counts are scanned (e16 valid)
   uint64_t used16[2] = {0, 0};      // region entries of the bins built through regions
   std::vector<uint32_t> mo[2];  // host copies of the job offsets in pass order
@@ -548,7 +548,7 @@ void make_plan(splhip_ctx *c, int nj, const JobIn *jobs, int G, Plan &pl, hipStr
 
 // Totals slot layout (device u32): [0..1] pass sizes, [2..3] bin entries,
 // [4..5] work items, [6..7] heavy items, [8..11] bin entries as u64,
-// [12..13] 16x16-region entries.
+// [12..13] region entries.
 enum { T_M = 0, T_E = 2, T_ITEMS = 4, T_HEAVY = 6, T_E64 = 8, T_E16 = 12, T_N = 14 };
 
 // Project all jobs, then the compaction scan of each present pass and its job
@@ -642,10 +642,10 @@ void stage_order(splhip_ctx *c, Plan &pl, int p, const Rec *rec, const uint8_t *
 }
 
 // Gather into render order and count each particle's units; the entry total
-// lands in tot[T_E + p].  With 8 px units and fewer than 2^28 particles
-// (entry values carry a 4-bit block mask above the particle index) the
-// entries can be binned through 16x16 regions: their counts are scanned too
-// (total in tot[T_E16 + p]).
+// lands in tot[T_E + p].  With 8 px units and a region size chosen for the
+// frame (region ids and block masks fit one 32-bit key) the entries can be
+// binned through regions: their counts are scanned too (total in
+// tot[T_E16 + p]).
 void stage_count(splhip_ctx *c, Plan &pl, int p, const Rec *rec, uint32_t *tot, hipStream_t s) {
   Workspace &ws = c->ws;
   const uint32_t m = pl.m[p];
@@ -666,10 +666,11 @@ void stage_count(splhip_ctx *c, Plan &pl, int p, const Rec *rec, uint32_t *tot, 
 }
 
 // Emit the (unit, particle) entries, sort them by unit (stable, so each list
-// keeps render order) and build the unit offsets.  Through 16x16 regions:
-// emit (region, particle | block mask) entries, sort them by region, and
-// split each region's list into its 8x8 blocks' lists (same lists, same
-// order: the stable sort keeps render order in both forms).
+// keeps render order) and build the unit offsets.  Through regions of S x S
+// blocks: emit (region | block mask, particle) entries, the mask naming the
+// blocks the footprint can reach, sort them by region, and split each
+// region's list into its 8x8 blocks' lists (render order in both forms; the
+// region lists leave out entries that would add +0 to every pixel).
 void stage_bins(splhip_ctx *c, Plan &pl, int p, hipStream_t s) {
   Workspace &ws = c->ws;
   const uint32_t m = pl.m[p], e = pl.e[p];
