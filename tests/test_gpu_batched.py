@@ -131,7 +131,7 @@ def test_batched_mixed_edge_views(R):
 
 
 def test_batched_region_binning(R):
-    """Views of large-footprint particles bin through 16x16 regions, numbered
+    """Views of large-footprint particles bin through regions of 8x8 blocks, numbered
     per view from the view's first region: every image equals its plain
     render and, for the first view, the oracle."""
     tiles, res = [50, 37, 24, 50], [160, 144, 176, 120]

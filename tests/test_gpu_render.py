@@ -413,7 +413,7 @@ def test_profiling_and_trace_both_passes(R):
 
 
 def test_chunked_region_binning(R):
-    """Chunks of a pass binned through 16x16 regions (large footprints): the
+    """Chunks of a pass binned through regions of 8x8 blocks (large footprints): the
     image equals the unchunked render bit for bit."""
     import splotch_amd as S
     gas = synth.plummer_field(20_000, 1560, ptype=0, halo_frac=1.0, n_halos=2,
