@@ -650,7 +650,10 @@ __global__ __launch_bounds__(kPThreads) void k_splat_pc(SplatArgs a, const uint3
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = static_cast<int>(uniform(static_cast<uint32_t>(tid >> 6)));
   const bool producer = wave >= 1;
-  const int ptid = tid - 64;
+  // table task rank: waves 4 and 8 share SIMD 0 with the consumer (waves go
+  // to the SIMDs in turn) and take the last, cheaper row tasks
+  const int trank = (wave & 3) == 0 ? 8 + (wave >> 2) : wave - 1 - (wave >> 2);
+  const int ptid = trank * 64 + lane;
   for (int i = tid; i < 2048; i += kPThreads) s_tab[i] = a.exptab[i];
   const uint32_t it = blockIdx.x;
   const uint64_t t_start = a.trace ? __builtin_amdgcn_s_memrealtime() : 0;
