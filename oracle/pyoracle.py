@@ -358,7 +358,7 @@ def read_bin_block(kv: dict):
     big-endian; r = smooth_factor * value (float product; a NaN comes out
     quiet with its payload) or smooth_factor; I = value or 0.5; C1 = value or
     1; C2, C3 = value or 0 (bit copies: 1.0 * value folds to a copy); type 0.
-    Returns a dict of float32 arrays (+ type uint8)."""
+    A column number <= 0 counts as absent.  Returns a dict of float32 arrays (+ type uint8)."""
     truth = ("t", "y", "true", ".true.")
     nfields = int(kv["num_blocks"])
     swap = str(kv.get("swap_endian", "true")).lower() in truth
@@ -369,8 +369,9 @@ def read_bin_block(kv: dict):
     noval = (0.0, 0.0, 0.0, smooth, 0.5, 1.0, 0.0, 0.0)
     out = {}
     for q, (key, name) in enumerate(zip(keys, BIN_FIELDS)):
-        if key in kv:
-            c = int(kv[key]) - 1
+        # a column number <= 0 names no column (qty_idx < 0, bin_reader.cc:60-67)
+        c = int(kv[key]) - 1 if key in kv else -1
+        if c >= 0:
             b = raw[c * npart:(c + 1) * npart].astype(np.uint32)
             v = b.view(np.float32)
             if q == 3:

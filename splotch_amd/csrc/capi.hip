@@ -348,6 +348,7 @@ BinGeom make_geom(const splhip_camera *cam, const splhip_params *prm, int G, con
   g.rcell = std::sqrt(2.f) * (chunkdim * 0.5f - 0.5f);
   g.cmid0 = 0.5f * (chunkdim - 1);
   g.rfac = k.rfac;
+  g.grayabsorb = prm->gray_absorption;
   return g;
 }
 
@@ -397,6 +398,7 @@ struct Plan {
   const uint32_t *bin_off[2] = {nullptr, nullptr};
   const uint32_t *d_mo[2] = {nullptr, nullptr};
   const uint32_t *items[2] = {nullptr, nullptr};  // sorted work items (8 px regions)
+  const uint32_t *qbad = nullptr;  // device flag: the ordered pass has a non-finite q
 };
 
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
@@ -548,8 +550,8 @@ void make_plan(splhip_ctx *c, int nj, const JobIn *jobs, int G, Plan &pl, hipStr
 
 // Totals slot layout (device u32): [0..1] pass sizes, [2..3] bin entries,
 // [4..5] work items, [6..7] heavy items, [8..11] bin entries as u64,
-// [12..13] region entries.
-enum { T_M = 0, T_E = 2, T_ITEMS = 4, T_HEAVY = 6, T_E64 = 8, T_E16 = 12, T_N = 14 };
+// [12..13] region entries, [14] the ordered pass has a non-finite q.
+enum { T_M = 0, T_E = 2, T_ITEMS = 4, T_HEAVY = 6, T_E64 = 8, T_E16 = 12, T_QBAD = 14, T_N = 15 };
 
 // Project all jobs, then the compaction scan of each present pass and its job
 // offsets in pass order; one host sync reads the pass sizes and offsets.
@@ -656,7 +658,8 @@ void stage_count(splhip_ctx *c, Plan &pl, int p, const Rec *rec, uint32_t *tot, 
   uint32_t *off = ws.get_as<uint32_t>(pslot(p, P_OFF), m);
   uint32_t *cnt16 = pl.r16[p] ? ws.get_as<uint32_t>(pslot(p, P_CNT16), m) : nullptr;
   launch_gather_count(pl.order[p], m, rec, recs, cnt, cnt16, pl.rsh, pl.d_geom, pl.d_mo[p], pl.nj,
-                      s);
+                      p == 0 ? tot + T_QBAD : nullptr, s);
+  if (p == 0) pl.qbad = tot + T_QBAD;
   scan_u32(cnt, off, m, tot + T_E + p, ws, s);
   launch_sum_u64(cnt, m, reinterpret_cast<uint64_t *>(tot + T_E64 + 2 * p), s);
   if (cnt16)
@@ -776,6 +779,7 @@ SplatArgs splat_args(splhip_ctx *c, const Plan &pl, int p, const Tuning &tn) {
   sa.taylorlimit = c->consts.taylorlimit;
   sa.prio_len = tn.split;
   sa.heavy_len = tn.heavy;
+  sa.qbad = p == 0 ? pl.qbad : nullptr;
   return sa;
 }
 
@@ -1713,10 +1717,13 @@ int bin_block_keys(const char *par_text, BinBlock &b, std::string &err) {
       err = "num_blocks must be positive";
       return SPLHIP_E_ARG;
     }
+    // a column number <= 0 names no column: the quantity takes its default
+    // value, as qty_idx < 0 does (bin_reader.cc:60-67,166-178)
     for (int q = 0; q < 8; ++q) {
       b.col[q] = d.count(keys[q]) ? par_num<int>(d, keys[q], 0) - 1 : -1;
-      if (d.count(keys[q]) && (b.col[q] < 0 || b.col[q] >= b.nfields)) {
-        err = std::string("column of ") + keys[q] + " outside 1..num_blocks";
+      if (b.col[q] < 0) b.col[q] = -1;
+      if (b.col[q] >= b.nfields) {
+        err = std::string("column of ") + keys[q] + " past num_blocks";
         return SPLHIP_E_ARG;
       }
     }
@@ -1995,6 +2002,11 @@ int splhip_render_host(splhip_ctx *c, const splhip_particle_sim *hp, int64_t n,
                        const splhip_camera *cam, const splhip_params *prm, float *host_pic,
                        void *stream) {
   if (!c || !cam || !prm || !host_pic || (n > 0 && !hp)) return fail(c, SPLHIP_E_ARG, "null argument");
+  // the image and parameter checks before anything is allocated or uploaded
+  if (n < 0) return fail(c, SPLHIP_E_ARG, "negative particle count");
+  if (n >= (int64_t(1) << 32) - 1) return fail(c, SPLHIP_E_CAPACITY, "too many particles");
+  if (cam->xres <= 0 || cam->yres <= 0) return fail(c, SPLHIP_E_ARG, "bad image size");
+  if (prm->ptypes < 1 || prm->ptypes > kMaxTypes) return fail(c, SPLHIP_E_ARG, "bad ptypes");
   try {
     hip_check(hipSetDevice(c->device), "hipSetDevice");
     hipStream_t s = static_cast<hipStream_t>(stream);

@@ -92,6 +92,7 @@ struct BinGeom {
   int T, xres, yres, ncx, ncy;
   int G, ngm;
   float rcell, cmid0, rfac;
+  float grayabsorb;  // gray_absorption of the job (ordered pass: q = e / (e + grayabsorb))
 };
 
 // Per-job splat description: one entry per view of a batch (a plain render is

@@ -188,11 +188,17 @@ __global__ __launch_bounds__(256) void k_gather_count(const uint32_t *order, uin
                                                       const Rec *rec, Rec *recs, uint32_t *cnt,
                                                       uint32_t *cnt16, int rsh,
                                                       const BinGeom *geoms, const uint32_t *mo,
-                                                      int nj) {
+                                                      int nj, uint32_t *qbad) {
   for (int64_t s = gtid(); s < m; s += gstride()) {
     const BinGeom g = geoms[find_job(mo, nj, static_cast<uint32_t>(s))];
     const Rec R = rec[order[s]];
     recs[s] = R;
+    if (qbad) {  // ordered pass: a non-finite q_c = e_c / (e_c + gray_absorption)
+      const float q0 = R.e0 / (R.e0 + g.grayabsorb), q1 = R.e1 / (R.e1 + g.grayabsorb),
+                  q2 = R.e2 / (R.e2 + g.grayabsorb);
+      const float inf = __builtin_inff();
+      if (!(fabsf(q0) < inf && fabsf(q1) < inf && fabsf(q2) < inf)) atomicOr(qbad, 1u);
+    }
     const BinRange b = bin_range(R, g);
     uint32_t c = 0, c16 = 0;  // units of the binned tiles; their regions (2^rsh units square)
     for (int ix = b.minx; ix <= b.maxx; ++ix)
@@ -875,9 +881,9 @@ void launch_job_of_index(const uint32_t *vals, uint32_t m, const uint32_t *pbase
 
 void launch_gather_count(const uint32_t *order, uint32_t m, const Rec *rec, Rec *recs,
                          uint32_t *cnt, uint32_t *cnt16, int rsh, const BinGeom *geoms,
-                         const uint32_t *mo, int nj, hipStream_t s) {
+                         const uint32_t *mo, int nj, uint32_t *qbad, hipStream_t s) {
   k_gather_count<<<grid_for(m), 256, 0, s>>>(order, m, rec, recs, cnt, cnt16, rsh, geoms, mo,
-                                             nj);
+                                             nj, qbad);
   hip_check(hipGetLastError(), "k_gather_count launch");
 }
 

@@ -55,6 +55,10 @@ struct SplatArgs {
   uint32_t prio_len;  // splat8: lists at least this long run at raised wave priority
   uint64_t *trace;    // splat8: optional per-item {start, end, len | chan << 32, hw id}
   uint32_t heavy_len; // splat8: lists at least this long go to the producer/consumer kernel
+  // (device, optional) nonzero: the pass has a particle with a non-finite q_c,
+  // so no list goes to the producer/consumer kernel (it applies every particle
+  // to the whole block with zero operands, and (l - q) * 0 is NaN there)
+  const uint32_t *qbad;
 };
 
 void launch_project(const ProjectArgs *jobs, const uint32_t *blk, int nj, uint32_t nblocks,
@@ -71,10 +75,12 @@ void launch_pass_offsets(const uint32_t *pos, const uint32_t *pbase, int nj, int
 void launch_job_of_index(const uint32_t *vals, uint32_t m, const uint32_t *pbase, int nj,
                          uint32_t *keys, hipStream_t s);
 // cnt: entries per particle (units of the binned tiles); cnt16 (optional):
-// the regions of 2^rsh x 2^rsh of those 8x8 blocks they cover.
+// the regions of 2^rsh x 2^rsh of those 8x8 blocks they cover.  qbad
+// (optional, ordered pass): set to 1 when a particle's q_c = e_c / (e_c +
+// gray_absorption) is not finite.
 void launch_gather_count(const uint32_t *order, uint32_t m, const Rec *rec, Rec *recs,
                          uint32_t *cnt, uint32_t *cnt16, int rsh, const BinGeom *geoms,
-                         const uint32_t *mo, int nj, hipStream_t s);
+                         const uint32_t *mo, int nj, uint32_t *qbad, hipStream_t s);
 // rsh == 0: (unit, particle) entries; rsh > 0: (region | unit mask << rbits,
 // particle) entries for regions of 2^rsh x 2^rsh blocks, regions of job j
 // numbered from ubase[j] (then ubase holds the region bases).

@@ -972,13 +972,19 @@ __global__ __launch_bounds__(kPThreads) void k_splat_pc(SplatArgs a, const uint3
 // heavy_len; otherwise three (one per channel) for long ordered lists, else one.
 // (need_empty: the job's pass writes its whole image, so empty blocks are
 // written too.)
+// The list length from which a unit goes to the producer/consumer kernel:
+// none when the pass has a particle with a non-finite q_c.
+__device__ __forceinline__ uint32_t heavy_len_of(const SplatArgs &a) {
+  return (a.qbad && *a.qbad) ? 0xffffffffu : a.heavy_len;
+}
+
 __device__ __forceinline__ uint32_t unit_items(const SplatArgs &a, uint32_t u, int ordered,
                                                uint32_t split, uint32_t &len) {
   len = a.bin_off[u + 1] - a.bin_off[u];
   const UnitGeom g = unit_geom(a, u);
   if (g.gx * 8 >= g.w || g.gy * 8 >= g.h) return 0;
   if (len == 0) return g.need_empty ? 1u : 0u;
-  if (len >= a.heavy_len) return 1u;  // one producer/consumer workgroup
+  if (len >= heavy_len_of(a)) return 1u;  // one producer/consumer workgroup
   return (len >= (ordered ? split : 2 * split)) ? 3u : 1u;
 }
 
@@ -989,7 +995,7 @@ __global__ __launch_bounds__(256) void k_items_count(SplatArgs a, uint32_t nunit
     uint32_t len;
     const uint32_t k = unit_items(a, static_cast<uint32_t>(u), ordered, split, len);
     cnt[u] = k;
-    if (heavy_cnt && k > 0 && len >= a.heavy_len) atomicAdd(heavy_cnt, k);
+    if (heavy_cnt && k > 0 && len >= heavy_len_of(a)) atomicAdd(heavy_cnt, k);
   }
 }
 

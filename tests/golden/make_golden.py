@@ -305,6 +305,8 @@ def ingest_fixture(manifest):
         (True, dict(x=1, y=2, z=3, r=4, I=5, C1=6, C2=7, C3=8, smooth_factor=1.7)),
         (True, dict(x=9, y=2, z=1, C1=4)),
         (False, dict(x=2, y=3, z=4, r=4, C1=1, C3=5, smooth_factor=0.3)),
+        # column numbers <= 0 name no column (the .par files' "C2 = -1")
+        (True, dict(x=1, y=0, z=3, r=-1, I=-1, C1=6, C2=-1, C3=0, smooth_factor=0.9)),
     ]
     out = dict(ncases=np.array([len(cases)]))
     with tempfile.TemporaryDirectory() as d:
@@ -319,8 +321,8 @@ def ingest_fixture(manifest):
                 out["bb%d_%s" % (k, q)] = v
     save("ingest", **out)
     manifest["ingest"] = ("bin_reader_block of %d block files (%d particles, %d columns; both "
-                          "byte orders, missing columns, smooth_factor, NaN payloads)"
-                          % (len(cases), n, nf))
+                          "byte orders, missing columns, column numbers <= 0, smooth_factor, NaN "
+                          "payloads)" % (len(cases), n, nf))
 
 
 def scenes_fixture(manifest):

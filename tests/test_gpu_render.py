@@ -325,17 +325,19 @@ def test_dense_cluster_vs_oracle(R, aeq):
                         % (G, var, split, heavy, hmax))
 
 
-@pytest.mark.parametrize("var,heavy,hmax", [(4, -1, 0), (3, -1, 0)])
+@pytest.mark.parametrize("var,heavy,hmax", [(4, -1, 0), (3, -1, 0), (4, 0, 0), (4, 1, 1 << 30)])
 def test_zero_gray_absorption_nan_footprints(R, var, heavy, hmax):
     """gray_absorption = 0 makes q = e / (e + 0) = 0/0 for colour channels
     with e = 0 (the hot palette's green and blue below a third): the
     reference writes NaN into those channels of the particle's footprint
-    pixels only.  The 8x8-block splat (producer/consumer workgroups off: they
-    also write NaN into the rest of such a block) gives NaN in exactly the
-    oracle's pixels and every other value bit for bit: its fast chunks apply a
-    particle to the whole block with zero operands outside the footprint, so
-    such a particle must neither take the fast table path nor leave its q
-    where a masked slot reads it."""
+    pixels only.  The 8x8-block splat gives NaN in exactly the oracle's pixels
+    and every other value bit for bit: its fast chunks apply a particle to the
+    whole block with zero operands outside the footprint, so such a particle
+    must neither take the fast table path nor leave its q where a masked slot
+    reads it.  The producer/consumer kernel would spread the NaN over the
+    rest of the block, so a pass with a non-finite q sends no list there, with
+    the default long-list threshold (heavy 0) and with every list eligible
+    (heavy 1)."""
     import splotch_amd as S
     log("case gray_absorption 0 variant", var, "heavy", heavy, hmax)
     fld = synth.plummer_field(20_000, 1620, halo_frac=1.0, n_halos=3, hsml_mu=math.log(0.6))
