@@ -12,8 +12,13 @@
  *  - Plain pointers and sizes only; every function returns 0 on success and a
  *    negative SPLHIP_E* code on error (message via splhip_last_error).
  *  - Device pointers unless stated otherwise ("host" in the name / comment).
- *  - Calls are ordered on the given hipStream_t (passed as void*; NULL = the
- *    null stream).  A context is used by one host thread at a time.
+ *  - Work is queued on the given hipStream_t (passed as void*; NULL = the
+ *    null stream), in call order.  A render call returns once its work is
+ *    queued; to size its launches it waits on the host for the stream up to
+ *    three times (pass sizes, bin entries, work items), except under stream
+ *    capture of a reserved batch (splhip_capture_reserve), where it queues
+ *    everything without waiting.  A context is used by one host thread at a
+ *    time.
  *  - Images are x-major RGB float32: pic[(x*yres + y)*3 + c], the layout of
  *    arr2<COLOUR> pic (cxxsupport/arr.h:474-488) that Splotch composites and
  *    writes.  The raw image is what render_new leaves in pic
@@ -56,7 +61,8 @@ typedef struct {
   float smooth_fac[SPLHIP_MAX_TYPES];      /* smooth_factor{t} (1) */
   float brightness[SPLHIP_MAX_TYPES];      /* brightness{t} * b_brightness (1) */
   int32_t col_vector[SPLHIP_MAX_TYPES];    /* color_is_vector{t} (0) */
-  int32_t a_eq_e[SPLHIP_MAX_TYPES];        /* a_eq_e (1); per type (see splhip_render) */
+  int32_t a_eq_e[SPLHIP_MAX_TYPES];        /* a_eq_e (1); per type (see splhip_render):
+                                              1 additive, 0 ordered, 2 both */
   int32_t sort_type;                       /* sort_type (1) */
   float gray_absorption;                   /* gray_absorption (0.2) */
   int32_t tile_size;                       /* tile_size_cpu (50): reference tile */
@@ -76,15 +82,23 @@ typedef struct {
                                               (block, particle) entries is rendered in chunks
                                               of consecutive render order, each continuing
                                               the pixels' chains (0 = auto: 2^31 - 2^20) */
-  int32_t hip_autotune;                    /* hip_autotune: 1 = time the splat under a set of
-                                              (hip_variant, hip_split, hip_heavy_max) choices
-                                              on the first render of a batch shape and reuse
-                                              the fastest for that shape (0 = off) */
+  int32_t hip_autotune;                    /* hip_autotune: 1 = on the first render of a
+                                              batch bucket (particle count, pixel count and
+                                              pass mix, each by factors of sqrt(2)), time the
+                                              work region hip_region (8, 16, 32, 64 px) and,
+                                              for 8 px, the 8x8 splat's decomposition
+                                              (hip_heavy_max, hip_split, hip_variant and
+                                              hip_ppw), and reuse the fastest for the bucket
+                                              (0 = off) */
   int32_t hip_bin_group;                   /* hip_bin_group: 8 px units are binned through
                                               regions of g x g units, then split: 4, 2, or
                                               1 = sort the unit entries directly (0 = auto: the
                                               largest of 4, 2 whose region ids and unit masks
                                               fit 32-bit keys, when it merges entries) */
+  int32_t hip_ppw;                         /* hip_ppw: particles per chunk of the 8x8 splat's
+                                              walk (hip_variant 4): 1, 2, 4 or 8 (0 = auto:
+                                              8 for one channel, 8 additive / 4 ordered for
+                                              three) */
 } splhip_params;
 
 /* SoA particle fields (device pointers), post particle_normalize:
@@ -148,9 +162,10 @@ int splhip_set_palette(splhip_ctx *ctx, int32_t type, int32_t n, const double *x
    binning and splat (render_new, :330-697).  Writes the raw image `pic`
    (device, xres*yres*3 floats).  Particles are not modified.
    Types with a_eq_e[t]=1 are splatted additively; types with a_eq_e[t]=0 are
-   composited in sort order.  When both kinds are present the image is
-   pic = P_ordered + P_additive, each rendered from zero as render_new does
-   for its mode (the reference's a_eq_e is one global flag). */
+   composited in sort order; types with a_eq_e[t]=2 take part in both passes.
+   When both kinds are present the image is pic = P_ordered + P_additive, each
+   rendered from zero as render_new does for its mode (the reference's a_eq_e
+   is one global flag). */
 int splhip_render(splhip_ctx *ctx, const splhip_particles *parts, const splhip_camera *cam,
                   const splhip_params *prm, float *pic, void *stream);
 
@@ -178,8 +193,38 @@ typedef struct {
    context's palettes; the hip_* tuning keys are taken from jobs[0].prm, and
    batches use 8 px work regions.  Stages run once over all jobs (one launch
    per stage, heaviest work items of all views first); the host waits three
-   times per batch for sizes. */
+   times per pipeline for sizes.  Consecutive views go through one pipeline
+   up to 2^28 particles (a larger view alone); a pipeline with a pass past
+   hip_chunk_entries bin entries is split into smaller groups of views, down
+   to single views rendered in chunks.
+   Under stream capture (hipStreamBeginCapture on `stream`) the call queues
+   the reserved batch (splhip_capture_reserve) with device-side counts: no
+   host wait, allocation, host copy or memset, so the captured graph can be
+   replayed; each replay renders the reserved views from the current values
+   of their particle arrays. */
 int splhip_render_batched(splhip_ctx *ctx, int32_t njobs, const splhip_job *jobs, void *stream);
+
+/* --- capture (no reference counterpart) ---------------------------------------- */
+
+/* Reserve a batch for rendering under stream capture: renders it once as
+   splhip_render_batched does (images written), sizes the per-pass bin-entry
+   capacities from its counts (x headroom >= 1, + 4096), fixes its binning
+   form and tuning, then renders it once more with device-side counts into a
+   workspace of its own (images written again), which is then frozen.  A
+   later splhip_render_batched (or splhip_render, njobs = 1) of the same views
+   (same particle arrays, counts, cameras, params and images) on a capturing
+   stream queues that device-count pipeline.  One reservation per context: a
+   new one invalidates graphs captured under the previous one, as does
+   destroying the context or giving a palette more entries.  Needs 8 px work
+   regions and passes within hip_chunk_entries. */
+int splhip_capture_reserve(splhip_ctx *ctx, int32_t njobs, const splhip_job *jobs,
+                           float headroom, void *stream);
+
+/* After a replay (waits for the device): *overflow bit p (0 ordered, 1
+   additive) is set when a pass had more bin entries than reserved; that pass
+   was left out of the images (nothing is written past a buffer).  Reserve
+   again with more headroom. */
+int splhip_capture_status(splhip_ctx *ctx, int32_t *overflow);
 
 /* --- differentiable form ------------------------------------------------------ */
 
@@ -214,7 +259,8 @@ int splhip_render_backward(splhip_ctx *ctx, const float *dL_dpic, const splhip_g
 
 /* particle_project + particle_colorize only.  rec_out: device, n*8 floats per
    particle {x, y, r, e.r, e.g, e.b, z, I} (state after the reference loops);
-   active_out: device n bytes, 0 inactive, 1 active. */
+   active_out: device n bytes, 0 inactive, else the passes of the particle
+   (bits: 1 ordered, 2 additive). */
 int splhip_project(splhip_ctx *ctx, const splhip_particles *parts, const splhip_camera *cam,
                    const splhip_params *prm, float *rec_out, uint8_t *active_out,
                    void *stream);
@@ -262,9 +308,9 @@ int splhip_set_trace(splhip_ctx *ctx, int32_t enable);
 int splhip_trace_items(splhip_ctx *ctx, int32_t ordered, uint64_t *out, int64_t cap, int64_t *n);
 
 /* The splat tuning of the last render: {hip_variant, hip_split, heavy
-   workgroup cap (0 = none), 1 if the autotuner chose it}.  No reference
-   counterpart. */
-int splhip_last_tuning(splhip_ctx *ctx, int64_t out[4]);
+   workgroup cap (0 = none), 1 if the autotuner chose it, work region in px
+   (hip_region), hip_ppw (0 = auto)}.  No reference counterpart. */
+int splhip_last_tuning(splhip_ctx *ctx, int64_t out[6]);
 
 /* How the last render binned its passes (no reference counterpart): out[p] =
    the (16x16 region, particle) entries pass p (0 ordered, 1 additive) was

@@ -1,4 +1,5 @@
-"""Build the gfx950 HIP library in-tree: splotch_amd/libsplotch_hip.so.
+"""Build the gfx950 HIP library in-tree: splotch_amd/libsplotch_hip.so, and the
+LDS bandwidth microbenchmark used by the bench: splotch_amd/liblds_peak.so.
 
     python -m splotch_amd.build        (or __graft_entry__.build())
 
@@ -17,6 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsplotch_hip.so")
+LDS_LIB = os.path.join(HERE, "liblds_peak.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 SOURCES = ["scan_sort.hip", "kernels.hip", "splat8.hip", "adjoint.hip", "post.hip",
            "normalize.hip", "ingest.hip", "capi.hip"]
@@ -34,16 +36,22 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(verbose: bool = True) -> str:
-    os.makedirs(BUILD, exist_ok=True)
-    with cf.ThreadPoolExecutor(len(SOURCES)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
-    tmp = LIB + ".tmp"
+def _link(objs, lib):
+    tmp = lib + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr))
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
+
+
+def build(verbose: bool = True) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    with cf.ThreadPoolExecutor(len(SOURCES) + 1) as ex:
+        objs = list(ex.map(_compile, SOURCES + ["lds_peak.hip"]))
+    _link(objs[-1:], LDS_LIB)
+    objs = objs[:-1]
+    _link(objs, LIB)
     if verbose:
         print("built", LIB)
     return LIB

@@ -42,7 +42,8 @@ class Params(C.Structure):
                 ("hip_region", C.c_int32), ("hip_split", C.c_int32),
                 ("hip_variant", C.c_int32), ("hip_heavy", C.c_int32),
                 ("hip_heavy_max", C.c_int32), ("hip_chunk_entries", C.c_int32),
-                ("hip_autotune", C.c_int32), ("hip_bin_group", C.c_int32)]
+                ("hip_autotune", C.c_int32), ("hip_bin_group", C.c_int32),
+                ("hip_ppw", C.c_int32)]
 
 
 class Post(C.Structure):
@@ -107,7 +108,8 @@ EXPORTS = ["splhip_create", "splhip_destroy", "splhip_last_error", "splhip_camer
            "splhip_trace_items", "splhip_render_batched", "splhip_render_backward",
            "splhip_last_tuning", "splhip_last_binning", "splhip_post_from_par",
            "splhip_postprocess", "splhip_norm_from_par", "splhip_normalize",
-           "splhip_bin_block_count", "splhip_read_bin_block", "splhip_scene_file_parse"]
+           "splhip_bin_block_count", "splhip_read_bin_block", "splhip_scene_file_parse",
+           "splhip_capture_reserve", "splhip_capture_status"]
 
 STAGES = ("project", "ordered_sort", "ordered_bin", "ordered_splat", "additive_compact",
           "additive_bin", "additive_splat", "ordered_k_splat8", "ordered_k_splat_heavy",
@@ -137,6 +139,9 @@ def lib():
                                   C.c_int64, C.POINTER(C.c_int64), C.c_void_p,
                                   C.POINTER(C.c_int64), C.c_void_p]
         L.splhip_render_batched.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
+        L.splhip_capture_reserve.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_float,
+                                             C.c_void_p]
+        L.splhip_capture_status.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
         L.splhip_render_backward.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(Grads),
                                              C.c_void_p]
         L.splhip_set_profiling.argtypes = [C.c_void_p, C.c_int32]
@@ -181,9 +186,11 @@ def camera_setup(view, zmin=0.0, zmax=1e23, projection=True, minrad_pix=1.0) -> 
 def params(ptypes=1, smooth_fac=None, brightness=None, col_vector=None, a_eq_e=True,
            sort_type=1, gray_absorption=0.2, tile_size=50, hip_region=0, hip_split=0, hip_variant=0,
            hip_heavy=0, hip_heavy_max=0, hip_chunk_entries=0, hip_autotune=0, hip_bin_group=0,
+           hip_ppw=0,
            b_brightness=1.0) -> Params:
     """Render parameters; brightness is brightness{t} (times b_brightness, as
-    splotch_host.cc:262-263 does, in float32)."""
+    splotch_host.cc:262-263 does, in float32).  a_eq_e per type: True / 1
+    additive, False / 0 ordered, 2 both passes."""
     p = Params()
     lib().splhip_params_default(C.byref(p))
     p.ptypes = ptypes
@@ -194,10 +201,12 @@ def params(ptypes=1, smooth_fac=None, brightness=None, col_vector=None, a_eq_e=T
         p.brightness[t] = float(np.float32(b) * np.float32(b_brightness))
         if col_vector is not None and t < len(col_vector):
             p.col_vector[t] = int(bool(col_vector[t]))
+        # 1 additive, 0 ordered, 2 both passes
+        mode = lambda v: 2 if (not isinstance(v, bool) and v == 2) else int(bool(v))
         if isinstance(a_eq_e, (list, tuple)):
-            p.a_eq_e[t] = int(bool(a_eq_e[t])) if t < len(a_eq_e) else 1
+            p.a_eq_e[t] = mode(a_eq_e[t]) if t < len(a_eq_e) else 1
         else:
-            p.a_eq_e[t] = int(bool(a_eq_e))
+            p.a_eq_e[t] = mode(a_eq_e)
     p.sort_type = sort_type
     p.gray_absorption = gray_absorption
     p.tile_size = tile_size
@@ -209,6 +218,7 @@ def params(ptypes=1, smooth_fac=None, brightness=None, col_vector=None, a_eq_e=T
     p.hip_chunk_entries = hip_chunk_entries
     p.hip_autotune = hip_autotune
     p.hip_bin_group = hip_bin_group
+    p.hip_ppw = hip_ppw
     return p
 
 
@@ -495,9 +505,7 @@ class Renderer:
         flush()
         return images
 
-    def render_batched(self, views, stream=None):
-        """views: list of (dev, cam, prm, pic) as for render(); all images in
-        one pipeline.  Returns the list of images."""
+    def _jobs(self, views):
         jobs = (Job * max(len(views), 1))()
         for j, (dev, cam, prm, pic) in enumerate(views):
             assert pic.is_contiguous() and pic.numel() == cam.xres * cam.yres * 3
@@ -505,10 +513,34 @@ class Renderer:
             jobs[j].cam = cam
             jobs[j].prm = prm
             jobs[j].pic = pic.data_ptr()
+        return jobs
+
+    def render_batched(self, views, stream=None):
+        """views: list of (dev, cam, prm, pic) as for render(); all images in
+        one pipeline (on a capturing stream: the reserved batch with device
+        counts).  Returns the list of images."""
+        jobs = self._jobs(views)
         self._check(lib().splhip_render_batched(self._c, C.c_int32(len(views)),
                                                 C.cast(jobs, C.c_void_p),
                                                 self._stream(stream)), "splhip_render_batched")
         return [v[3] for v in views]
+
+    def capture_reserve(self, views, headroom: float = 1.25, stream=None):
+        """Reserve the batch `views` for rendering under stream capture
+        (splhip_capture_reserve); renders it (twice).  Returns the images."""
+        jobs = self._jobs(views)
+        self._check(lib().splhip_capture_reserve(self._c, C.c_int32(len(views)),
+                                                 C.cast(jobs, C.c_void_p), C.c_float(headroom),
+                                                 self._stream(stream)),
+                    "splhip_capture_reserve")
+        return [v[3] for v in views]
+
+    def capture_status(self) -> int:
+        """Overflow bits of the last device-count render (bit 0 ordered pass,
+        bit 1 additive pass); waits for the device."""
+        v = C.c_int32(0)
+        self._check(lib().splhip_capture_status(self._c, C.byref(v)), "splhip_capture_status")
+        return int(v.value)
 
     def backward(self, grad_pic, n: int, vector: bool = False, screen: bool = False,
                  stream=None):

@@ -23,9 +23,15 @@
 using namespace splhip;
 
 struct FwdState;  // the last single-view render, kept for splhip_render_backward
+struct Reservation;  // capacities and workspace of a batch rendered under stream capture
 
 struct splhip_ctx {
   FwdState *fwd = nullptr;
+  Reservation *rv = nullptr;
+  bool split_hint = false;  // the last capacity error can be avoided by smaller groups
+  // the 8x8 splat decomposition found by the last inner autotuner sweep:
+  // variant, ppw, split, heavy, heavy_max
+  uint32_t inner_choice[5] = {4, 0, 32768, 32768, 192};
   int device = 0;
   std::string err;
   Workspace ws;
@@ -60,9 +66,10 @@ struct splhip_ctx {
   int trace = 0;
   std::vector<uint64_t> trace_items[2];
   std::vector<uint32_t> h_mo;
-  // autotuner: fastest splat tuning per batch shape; the last render's choice
-  std::map<uint64_t, std::array<uint32_t, 3>> tuned;
-  uint32_t last_tuning[4] = {0, 0, 0, 0};
+  // autotuner: the choice per batch bucket (opaque, see render_tuned)
+  std::map<uint64_t, std::vector<uint32_t>> tuned;
+  // variant, split, heavy cap (0: none), tuned, work region, ppw
+  uint32_t last_tuning[6] = {0, 0, 0, 0, 0, 0};
   int64_t last_regions[2] = {0, 0};  // region entries binned per pass (0: per block)
   // auxiliary stream (default priority) for the heavy-block kernel
   hipStream_t aux = nullptr;
@@ -319,6 +326,9 @@ int check_inputs(splhip_ctx *c, const splhip_particles *P, const splhip_camera *
   if (prm->tile_size < 1 || prm->tile_size > 4096) return fail(c, SPLHIP_E_ARG, "bad tile_size");
   if (prm->sort_type < 0 || prm->sort_type > 4)
     return fail(c, SPLHIP_E_ARG, "unknown sorting chosen ...");
+  if (prm->hip_ppw != 0 && prm->hip_ppw != 1 && prm->hip_ppw != 2 && prm->hip_ppw != 4 &&
+      prm->hip_ppw != 8)
+    return fail(c, SPLHIP_E_ARG, "hip_ppw must be 1, 2, 4 or 8 (0 = auto)");
   for (int t = 0; t < prm->ptypes; ++t) {
     if (!prm->col_vector[t] && c->pal_x[t].empty())
       return fail(c, SPLHIP_E_ARG, "trying to access an empty map (palette " + std::to_string(t) + ")");
@@ -399,6 +409,15 @@ struct Plan {
   const uint32_t *d_mo[2] = {nullptr, nullptr};
   const uint32_t *items[2] = {nullptr, nullptr};  // sorted work items (8 px regions)
   const uint32_t *qbad = nullptr;  // device flag: the ordered pass has a non-finite q
+  // workspace of the buffers (the context's, or a capture reservation's)
+  Workspace *ws = nullptr;
+  // device counts: sizes stay on the device (m, e, e16, items and heavy items
+  // are capacities on the host; kernels read the counts from tot), so no
+  // stage waits on the host
+  bool dc = false;
+  uint32_t *tot = nullptr;
+  bool use16[2] = {false, false};  // dc: the pass bins through regions
+  uint32_t ovf_bit = 0;
 };
 
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
@@ -424,8 +443,10 @@ void staged(splhip_ctx *c, hipStream_t s) {
   hip_check(hipEventRecord(c->ev_stage, s), "hipEventRecord");
 }
 
-void make_plan(splhip_ctx *c, int nj, const JobIn *jobs, int G, Plan &pl, hipStream_t s) {
+// Host tables of a batch (no device work besides a palette upload).
+void build_plan(splhip_ctx *c, int nj, const JobIn *jobs, int G, Plan &pl) {
   upload_palettes(c);
+  pl.ws = &c->ws;
   pl.nj = nj;
   pl.G = G;
   pl.pa.resize(nj);
@@ -452,10 +473,14 @@ void make_plan(splhip_ctx *c, int nj, const JobIn *jobs, int G, Plan &pl, hipStr
       a.smooth_fac[t] = prm->smooth_fac[t];
       a.brightness[t] = prm->brightness[t];
       a.col_vector[t] = prm->col_vector[t];
-      a.pass_code[t] = prm->a_eq_e[t] ? 2 : 1;
+      // pass bits: 1 ordered, 2 additive, 3 both (a_eq_e 2)
+      a.pass_code[t] = prm->a_eq_e[t] == 2 ? 3 : (prm->a_eq_e[t] ? 2 : 1);
       a.pal_n[t] = static_cast<int32_t>(c->pal_x[t].size());
     }
-    for (int t = 0; t < prm->ptypes; ++t) pl.present[prm->a_eq_e[t] ? 1 : 0] = true;
+    for (int t = 0; t < prm->ptypes; ++t) {
+      if (prm->a_eq_e[t] != 1) pl.present[0] = true;
+      if (prm->a_eq_e[t] != 0) pl.present[1] = true;
+    }
     a.pal_stride = c->pal_stride;
     a.ntypes = prm->ptypes;
     a.sort_type = prm->sort_type;
@@ -519,7 +544,12 @@ void make_plan(splhip_ctx *c, int nj, const JobIn *jobs, int G, Plan &pl, hipStr
       break;
     }
   }
-  // one upload: [ProjectArgs][blk][pbase][ubase][rbase][BinGeom][JobSplat]
+}
+
+// The plan's device tables as one block: [ProjectArgs][blk][pbase][ubase]
+// [rbase][BinGeom][JobSplat]; offsets in o[0..6], total size returned.
+size_t plan_tables(const Plan &pl, std::vector<char> *out, size_t o[7]) {
+  const int nj = pl.nj;
   const size_t o_pa = 0;
   const size_t o_blk = align256(o_pa + nj * sizeof(ProjectArgs));
   const size_t o_pb = align256(o_blk + (nj + 1) * 4);
@@ -528,36 +558,62 @@ void make_plan(splhip_ctx *c, int nj, const JobIn *jobs, int G, Plan &pl, hipStr
   const size_t o_g = align256(o_rb + (nj + 1) * 4);
   const size_t o_js = align256(o_g + nj * sizeof(BinGeom));
   const size_t total = align256(o_js + nj * sizeof(JobSplat));
+  o[0] = o_pa; o[1] = o_blk; o[2] = o_pb; o[3] = o_ub; o[4] = o_rb; o[5] = o_g; o[6] = o_js;
+  if (out) {
+    out->assign(total, 0);
+    char *h = out->data();
+    std::memcpy(h + o_pa, pl.pa.data(), nj * sizeof(ProjectArgs));
+    std::memcpy(h + o_blk, pl.blk.data(), (nj + 1) * 4);
+    std::memcpy(h + o_pb, pl.pbase.data(), (nj + 1) * 4);
+    std::memcpy(h + o_ub, pl.ubase.data(), (nj + 1) * 4);
+    std::memcpy(h + o_rb, pl.rbase.data(), (nj + 1) * 4);
+    std::memcpy(h + o_g, pl.geom.data(), nj * sizeof(BinGeom));
+    std::memcpy(h + o_js, pl.js.data(), nj * sizeof(JobSplat));
+  }
+  return total;
+}
+
+// Point the plan at device tables laid out by plan_tables.
+void plan_bind(Plan &pl, char *d, const size_t o[7]) {
+  pl.d_pa = reinterpret_cast<const ProjectArgs *>(d + o[0]);
+  pl.d_blk = reinterpret_cast<const uint32_t *>(d + o[1]);
+  pl.d_pbase = reinterpret_cast<const uint32_t *>(d + o[2]);
+  pl.d_ubase = reinterpret_cast<const uint32_t *>(d + o[3]);
+  pl.d_rbase = reinterpret_cast<const uint32_t *>(d + o[4]);
+  pl.d_geom = reinterpret_cast<const BinGeom *>(d + o[5]);
+  pl.d_js = reinterpret_cast<JobSplat *>(d + o[6]);
+}
+
+// Host plan of a batch and the upload of its tables into the context's
+// workspace (through the pinned staging buffer).
+void make_plan(splhip_ctx *c, int nj, const JobIn *jobs, int G, Plan &pl, hipStream_t s) {
+  build_plan(c, nj, jobs, G, pl);
+  size_t o[7];
+  const size_t total = plan_tables(pl, nullptr, o);
   char *h = staging(c, total);
-  std::memcpy(h + o_pa, pl.pa.data(), nj * sizeof(ProjectArgs));
-  std::memcpy(h + o_blk, pl.blk.data(), (nj + 1) * 4);
-  std::memcpy(h + o_pb, pl.pbase.data(), (nj + 1) * 4);
-  std::memcpy(h + o_ub, pl.ubase.data(), (nj + 1) * 4);
-  std::memcpy(h + o_rb, pl.rbase.data(), (nj + 1) * 4);
-  std::memcpy(h + o_g, pl.geom.data(), nj * sizeof(BinGeom));
-  std::memcpy(h + o_js, pl.js.data(), nj * sizeof(JobSplat));
+  std::vector<char> tab;
+  plan_tables(pl, &tab, o);
+  std::memcpy(h, tab.data(), total);
   char *d = c->ws.get_as<char>(S_JOBS, total);
   hip_check(hipMemcpyAsync(d, h, total, hipMemcpyHostToDevice, s), "upload job tables");
   staged(c, s);
-  pl.d_pa = reinterpret_cast<const ProjectArgs *>(d + o_pa);
-  pl.d_blk = reinterpret_cast<const uint32_t *>(d + o_blk);
-  pl.d_pbase = reinterpret_cast<const uint32_t *>(d + o_pb);
-  pl.d_ubase = reinterpret_cast<const uint32_t *>(d + o_ub);
-  pl.d_rbase = reinterpret_cast<const uint32_t *>(d + o_rb);
-  pl.d_geom = reinterpret_cast<const BinGeom *>(d + o_g);
-  pl.d_js = reinterpret_cast<JobSplat *>(d + o_js);
+  plan_bind(pl, d, o);
 }
 
 // Totals slot layout (device u32): [0..1] pass sizes, [2..3] bin entries,
 // [4..5] work items, [6..7] heavy items, [8..11] bin entries as u64,
-// [12..13] region entries, [14] the ordered pass has a non-finite q.
-enum { T_M = 0, T_E = 2, T_ITEMS = 4, T_HEAVY = 6, T_E64 = 8, T_E16 = 12, T_QBAD = 14, T_N = 15 };
+// [12..13] region entries, [14] the ordered pass has a non-finite q,
+// [15] capacity overflow bits of a device-count render (bit p: pass p).
+enum {
+  T_M = 0, T_E = 2, T_ITEMS = 4, T_HEAVY = 6, T_E64 = 8, T_E16 = 12, T_QBAD = 14, T_OVF = 15,
+  T_N = 16
+};
 
 // Project all jobs, then the compaction scan of each present pass and its job
 // offsets in pass order; one host sync reads the pass sizes and offsets.
 void stage_project_compact(splhip_ctx *c, Plan &pl, Rec *rec, uint8_t *act, uint32_t *tot,
                            hipStream_t s) {
-  Workspace &ws = c->ws;
+  Workspace &ws = *pl.ws;
   mark(c, 0, true, s);
   launch_project(pl.d_pa, pl.d_blk, pl.nj, pl.nblocks, rec, act, s);
   mark(c, 0, false, s);
@@ -567,10 +623,17 @@ void stage_project_compact(splhip_ctx *c, Plan &pl, Rec *rec, uint8_t *act, uint
     if (!pl.present[p] || pl.ntot == 0) continue;
     mark(c, p == 0 ? 1 : 4, true, s);
     uint32_t *pos = ws.get_as<uint32_t>(pslot(p, P_POS), static_cast<size_t>(pl.ntot));
-    scan_u8eq(act, static_cast<uint8_t>(p + 1), pos, pl.ntot, tot + T_M + p, ws, s);
+    scan_u8bit(act, static_cast<uint8_t>(p + 1), pos, pl.ntot, tot + T_M + p, ws, s);
     mo[p] = ws.get_as<uint32_t>(pslot(p, P_MO), pl.nj + 1);
     launch_pass_offsets(pos, pl.d_pbase, pl.nj, pl.ntot, tot + T_M + p, mo[p], s);
     mark(c, p == 0 ? 1 : 4, false, s);
+  }
+  if (pl.dc) {  // the pass sizes stay on the device; capacities: every particle
+    for (int p = 0; p < 2; ++p) {
+      pl.m[p] = mo[p] ? static_cast<uint32_t>(pl.ntot) : 0u;
+      pl.d_mo[p] = mo[p];
+    }
+    return;
   }
   // sync 1: pass sizes and per-job offsets
   uint32_t *h = c->h_tot;
@@ -616,10 +679,11 @@ void upload_job_flags(splhip_ctx *c, Plan &pl, hipStream_t s) {
 // ordered pass the depth sort, stable, within each job.
 void stage_order(splhip_ctx *c, Plan &pl, int p, const Rec *rec, const uint8_t *act,
                  hipStream_t s) {
-  Workspace &ws = c->ws;
+  Workspace &ws = *pl.ws;
   const uint32_t m = pl.m[p];
   const bool do_sort = (p == 0) && pl.sort;
   mark(c, p == 0 ? 1 : 4, true, s);
+  const uint32_t *dm = pl.dc ? pl.tot + T_M + p : nullptr;  // device count (m: capacity)
   uint32_t *k0 = do_sort ? ws.get_as<uint32_t>(pslot(p, P_K0), m) : nullptr;
   uint32_t *v0 = ws.get_as<uint32_t>(pslot(p, P_V0), m);
   const uint32_t *pos = ws.get_as<uint32_t>(pslot(p, P_POS), static_cast<size_t>(pl.ntot));
@@ -629,13 +693,13 @@ void stage_order(splhip_ctx *c, Plan &pl, int p, const Rec *rec, const uint8_t *
   if (do_sort) {
     uint32_t *k1 = ws.get_as<uint32_t>(pslot(p, P_K1), m);
     uint32_t *v1 = ws.get_as<uint32_t>(pslot(p, P_V1), m);
-    const int r = radix_sort_pairs(k0, v0, k1, v1, m, 0, 32, ws, s);
+    const int r = radix_sort_pairs(k0, v0, k1, v1, m, 0, 32, ws, s, dm);
     uint32_t *kk = r ? k1 : k0, *vv = r ? v1 : v0;
     uint32_t *ko = r ? k0 : k1, *vo = r ? v0 : v1;
     order = vv;
     if (pl.nj > 1) {  // regroup by job, stable: (job, key, index) order
-      launch_job_of_index(vv, m, pl.d_pbase, pl.nj, kk, s);
-      const int r2 = radix_sort_pairs(kk, vv, ko, vo, m, 0, ceil_log2(pl.nj), ws, s);
+      launch_job_of_index(vv, m, pl.d_pbase, pl.nj, kk, s, dm);
+      const int r2 = radix_sort_pairs(kk, vv, ko, vo, m, 0, ceil_log2(pl.nj), ws, s, dm);
       order = r2 ? vo : vv;
     }
   }
@@ -649,7 +713,7 @@ void stage_order(splhip_ctx *c, Plan &pl, int p, const Rec *rec, const uint8_t *
 // binned through regions: their counts are scanned too (total in
 // tot[T_E16 + p]).
 void stage_count(splhip_ctx *c, Plan &pl, int p, const Rec *rec, uint32_t *tot, hipStream_t s) {
-  Workspace &ws = c->ws;
+  Workspace &ws = *pl.ws;
   const uint32_t m = pl.m[p];
   mark(c, p == 0 ? 2 : 5, true, s);
   pl.r16[p] = pl.rsh > 0;
@@ -657,13 +721,20 @@ void stage_count(splhip_ctx *c, Plan &pl, int p, const Rec *rec, uint32_t *tot, 
   uint32_t *cnt = ws.get_as<uint32_t>(pslot(p, P_CNT), m);
   uint32_t *off = ws.get_as<uint32_t>(pslot(p, P_OFF), m);
   uint32_t *cnt16 = pl.r16[p] ? ws.get_as<uint32_t>(pslot(p, P_CNT16), m) : nullptr;
+  const uint32_t *dm = pl.dc ? tot + T_M + p : nullptr;  // device count (m: capacity)
   launch_gather_count(pl.order[p], m, rec, recs, cnt, cnt16, pl.rsh, pl.d_geom, pl.d_mo[p], pl.nj,
-                      p == 0 ? tot + T_QBAD : nullptr, s);
+                      p == 0 ? tot + T_QBAD : nullptr, s, dm);
   if (p == 0) pl.qbad = tot + T_QBAD;
-  scan_u32(cnt, off, m, tot + T_E + p, ws, s);
-  launch_sum_u64(cnt, m, reinterpret_cast<uint64_t *>(tot + T_E64 + 2 * p), s);
+  scan_u32(cnt, off, m, tot + T_E + p, ws, s, 0, dm);
+  launch_sum_u64(cnt, m, reinterpret_cast<uint64_t *>(tot + T_E64 + 2 * p), s, dm);
   if (cnt16)
-    scan_u32(cnt16, ws.get_as<uint32_t>(pslot(p, P_OFF16), m), m, tot + T_E16 + p, ws, s);
+    scan_u32(cnt16, ws.get_as<uint32_t>(pslot(p, P_OFF16), m), m, tot + T_E16 + p, ws, s, 0,
+             dm);
+  if (pl.dc)  // entries past the reserved capacity: the pass is dropped and flagged
+    launch_cap_check(tot + T_M + p, tot + T_E + p,
+                     reinterpret_cast<const uint64_t *>(tot + T_E64 + 2 * p),
+                     pl.use16[p] ? tot + T_E16 + p : nullptr, pl.e[p], pl.e16[p], 1u << p,
+                     tot + T_OVF, s);
   pl.recs[p] = recs;
   mark(c, p == 0 ? 2 : 5, false, s);
 }
@@ -675,12 +746,16 @@ void stage_count(splhip_ctx *c, Plan &pl, int p, const Rec *rec, uint32_t *tot, 
 // region's list into its 8x8 blocks' lists (render order in both forms; the
 // region lists leave out entries that would add +0 to every pixel).
 void stage_bins(splhip_ctx *c, Plan &pl, int p, hipStream_t s) {
-  Workspace &ws = c->ws;
+  Workspace &ws = *pl.ws;
   const uint32_t m = pl.m[p], e = pl.e[p];
   mark(c, p == 0 ? 2 : 5, true, s);
+  // device counts (m, e, e16 are capacities then)
+  const uint32_t *dm = pl.dc ? pl.tot + T_M + p : nullptr;
+  const uint32_t *de = pl.dc ? pl.tot + T_E + p : nullptr;
+  const uint32_t *de16 = pl.dc ? pl.tot + T_E16 + p : nullptr;
   // regions pay off where they merge entries (large footprints); small
   // footprints keep one entry per block either way
-  if (pl.r16[p] && 3ull * pl.e16[p] <= 2ull * e) {
+  if (pl.dc ? pl.use16[p] : (pl.r16[p] && 3ull * pl.e16[p] <= 2ull * e)) {
     const uint32_t e16 = pl.e16[p], nr = pl.nregions, nu = pl.nunits;
     const uint32_t *off16 = ws.get_as<uint32_t>(pslot(p, P_OFF16), m);
     uint32_t *rk0 = ws.get_as<uint32_t>(pslot(p, P_EK0), e16);
@@ -689,13 +764,13 @@ void stage_bins(splhip_ctx *c, Plan &pl, int p, hipStream_t s) {
     uint32_t *rv1 = ws.get_as<uint32_t>(pslot(p, P_EV1), e16);
     const int S = 1 << pl.rsh, rbits = pl.rbits;
     launch_emit(pl.rsh, rbits, pl.recs[p], m, off16, e16, pl.d_geom, pl.d_rbase, pl.d_mo[p],
-                pl.nj, rk0, rv0, s);
+                pl.nj, rk0, rv0, s, dm, de16);
     // the sort orders the region bits only: the unit masks above them ride
     // along, and each region's list keeps render order
-    const int r = radix_sort_pairs(rk0, rv0, rk1, rv1, e16, 0, rbits, ws, s);
+    const int r = radix_sort_pairs(rk0, rv0, rk1, rv1, e16, 0, rbits, ws, s, de16);
     const uint32_t *rk = r ? rk1 : rk0, *rv = r ? rv1 : rv0;
     uint32_t *roff = ws.get_as<uint32_t>(pslot(p, P_ROFF), static_cast<size_t>(nr) + 1);
-    launch_bin_offsets(rk, e16, nr, roff, s, (1u << rbits) - 1u);
+    launch_bin_offsets(rk, e16, nr, roff, s, (1u << rbits) - 1u, de16);
     const size_t ncb = static_cast<size_t>(S * S) * split_chunks(e16);
     uint32_t *ccnt = ws.get_as<uint32_t>(pslot(p, P_CCNT), ncb + 1);
     uint32_t *cpre = ws.get_as<uint32_t>(pslot(p, P_CPRE), ncb + 1);
@@ -703,13 +778,13 @@ void stage_bins(splhip_ctx *c, Plan &pl, int p, hipStream_t s) {
     uint32_t *ucnt = ws.get_as<uint32_t>(pslot(p, P_UCNT), nu);
     uint32_t *bin_off = ws.get_as<uint32_t>(pslot(p, P_BINOFF), static_cast<size_t>(nu) + 1);
     uint32_t *ent = ws.get_as<uint32_t>(pslot(p, P_ENT), e);
-    launch_split_chunks(S, rk, e16, rbits, ccnt, s);
+    launch_split_chunks(S, rk, e16, rbits, ccnt, s, de16);
     scan_u32(ccnt, cpre, static_cast<int64_t>(ncb), cpre + ncb, ws, s);
     launch_split_bounds(S, roff, rk, nr, cpre, e16, rbits, pl.d_rbase, pl.d_ubase, pl.d_geom,
                         pl.nj, sx, ucnt, s);
     scan_u32(ucnt, bin_off, nu, bin_off + nu, ws, s);
     launch_split_base(S, nr, pl.d_rbase, pl.d_ubase, pl.d_geom, pl.nj, sx, bin_off, sx, s);
-    launch_split_scatter(S, rk, rv, e16, rbits, cpre, sx, ent, s);
+    launch_split_scatter(S, rk, rv, e16, rbits, cpre, sx, ent, s, de16);
     pl.ent[p] = ent;
     pl.bin_off[p] = bin_off;
     pl.used16[p] += e16;
@@ -721,17 +796,18 @@ void stage_bins(splhip_ctx *c, Plan &pl, int p, hipStream_t s) {
   uint32_t *ev0 = ws.get_as<uint32_t>(pslot(p, P_EV0), e);
   uint32_t *ek1 = ws.get_as<uint32_t>(pslot(p, P_EK1), e);
   uint32_t *ev1 = ws.get_as<uint32_t>(pslot(p, P_EV1), e);
-  launch_emit(0, 0, pl.recs[p], m, off, e, pl.d_geom, pl.d_ubase, pl.d_mo[p], pl.nj, ek0, ev0, s);
-  const int r = radix_sort_pairs(ek0, ev0, ek1, ev1, e, 0, ceil_log2(pl.nunits), ws, s);
+  launch_emit(0, 0, pl.recs[p], m, off, e, pl.d_geom, pl.d_ubase, pl.d_mo[p], pl.nj, ek0, ev0, s,
+              dm, de);
+  const int r = radix_sort_pairs(ek0, ev0, ek1, ev1, e, 0, ceil_log2(pl.nunits), ws, s, de);
   uint32_t *bin_off = ws.get_as<uint32_t>(pslot(p, P_BINOFF), static_cast<size_t>(pl.nunits) + 1);
-  launch_bin_offsets(r ? ek1 : ek0, e, pl.nunits, bin_off, s);
+  launch_bin_offsets(r ? ek1 : ek0, e, pl.nunits, bin_off, s, ~0u, de);
   pl.ent[p] = r ? ev1 : ev0;
   pl.bin_off[p] = bin_off;
   mark(c, p == 0 ? 2 : 5, false, s);
 }
 
 struct Tuning {
-  int variant;
+  int variant, ppw;
   uint32_t split, heavy, heavy_max;
   uint64_t chunk;  // most (block, particle) entries of one pass chunk
 };
@@ -739,6 +815,7 @@ struct Tuning {
 Tuning tuning_of(const splhip_params *prm) {
   Tuning t;
   t.variant = prm->hip_variant > 0 ? prm->hip_variant : 4;
+  t.ppw = prm->hip_ppw;
   // lists from hip_heavy on go to producer/consumer workgroups (up to the cap)
   // or stay whole: splitting by channel pays only without those workgroups
   t.split = prm->hip_split > 0 ? static_cast<uint32_t>(prm->hip_split) : 32768u;
@@ -764,15 +841,6 @@ SplatArgs splat_args(splhip_ctx *c, const Plan &pl, int p, const Tuning &tn) {
   sa.ubase = pl.d_ubase;
   sa.nj = pl.nj;
   sa.pass = p;
-  const JobSplat &J = pl.js[0];
-  sa.pic = J.pic;
-  sa.T = J.T;
-  sa.xres = J.xres;
-  sa.yres = J.yres;
-  sa.ncx = J.ncx;
-  sa.ngm = J.ngm;
-  sa.accumulate = J.accumulate[p];
-  sa.grayabsorb = J.grayabsorb;
   sa.rfac = c->consts.rfac;
   sa.h2sigma = c->consts.h2sigma;
   sa.expfac = c->consts.expfac;
@@ -786,7 +854,7 @@ SplatArgs splat_args(splhip_ctx *c, const Plan &pl, int p, const Tuning &tn) {
 // Work items of pass p: count per unit (heavy units counted too), scan.
 void stage_items_count(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, uint32_t *tot,
                        hipStream_t s) {
-  Workspace &ws = c->ws;
+  Workspace &ws = *pl.ws;
   mark(c, p == 0 ? 2 : 5, true, s);
   const SplatArgs sa = splat_args(c, pl, p, tn);
   uint32_t *icnt = ws.get_as<uint32_t>(pslot(p, P_ICNT), pl.nunits);
@@ -806,7 +874,7 @@ void stage_items_count(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, uint32_
 // stream beside splat8 (else on s, first).
 void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s,
                  Workspace *wt = nullptr, JobSplat *jobs = nullptr, bool fork = true) {
-  Workspace &ws = c->ws;
+  Workspace &ws = *pl.ws;
   SplatArgs sa0 = splat_args(c, pl, p, tn);
   if (jobs) sa0.jobs = jobs;
   mark(c, p == 0 ? 3 : 6, true, s);
@@ -822,20 +890,31 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
   uint32_t *ik1 = ws.get_as<uint32_t>(pslot(p, P_IK1), nitems);
   uint32_t *iv1 = ws.get_as<uint32_t>(pslot(p, P_IV1), nitems);
   launch_items_emit(pl.bin_off[p], pl.nunits, sa0, p == 0 ? 1 : 0, tn.split, ioff, ik0, iv0, s);
-  const int rs = radix_sort_pairs(ik0, iv0, ik1, iv1, nitems, 0, 32, wt ? *wt : ws, s);
+  // device counts: nitems and nheavy are capacities; the kernels read the
+  // pass's item and long-list totals and skip past the long lists themselves
+  ItemCounts dc;
+  if (pl.dc) {
+    dc.items = pl.tot + T_ITEMS + p;
+    dc.heavy = pl.tot + T_HEAVY + p;
+    dc.heavy_cap = nheavy;
+  }
+  const int rs = radix_sort_pairs(ik0, iv0, ik1, iv1, nitems, 0, 32, wt ? *wt : ws, s, dc.items);
   const uint32_t *items = rs ? iv1 : iv0;
   pl.items[p] = items;
   SplatArgs sa = sa0;
-  if (c->trace)
+  if (c->trace && !pl.dc)
     sa.trace = ws.get_as<uint64_t>(pslot(p, P_TRACE), static_cast<size_t>(nitems) * 4);
   SplatArgs sn = sa;  // normal items follow the heavy ones
   if (sn.trace) sn.trace += 4 * static_cast<size_t>(nheavy);
+  // normal items: after the long lists (with device counts the kernel skips them)
+  const uint32_t *nit = pl.dc ? items : items + nheavy;
+  const uint32_t nn = pl.dc ? nitems : nitems - nheavy;
   if (nheavy > 0 && !fork) {
     mark(c, p == 0 ? 8 : 10, true, s);
-    launch_splat_pc(p == 1, items, nheavy, sa, s);
+    launch_splat_pc(p == 1, items, nheavy, sa, s, dc);
     mark(c, p == 0 ? 8 : 10, false, s);
     mark(c, p == 0 ? 7 : 9, true, s);
-    launch_splat8(p == 1, tn.variant, items + nheavy, nitems - nheavy, sn, s);
+    launch_splat8(p == 1, tn.variant, nit, nn, sn, s, dc, tn.ppw);
     mark(c, p == 0 ? 7 : 9, false, s);
   } else if (nheavy > 0) {
     // heavy blocks run on a second stream, beside the normal items
@@ -843,23 +922,23 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
     hip_check(hipEventRecord(c->ev_fork, s), "record fork");
     hip_check(hipStreamWaitEvent(c->aux, c->ev_fork, 0), "wait fork");
     mark(c, p == 0 ? 8 : 10, true, c->aux);
-    launch_splat_pc(p == 1, items, nheavy, sa, c->aux);
+    launch_splat_pc(p == 1, items, nheavy, sa, c->aux, dc);
     mark(c, p == 0 ? 8 : 10, false, c->aux);
     hip_check(hipEventRecord(c->ev_join, c->aux), "record join");
     // The normal items fill every CU; the heavy workgroups (the critical
     // path) must get their slots first, so the normal launch waits briefly.
     launch_delay(40, s);
     mark(c, p == 0 ? 7 : 9, true, s);
-    launch_splat8(p == 1, tn.variant, items + nheavy, nitems - nheavy, sn, s);
+    launch_splat8(p == 1, tn.variant, nit, nn, sn, s, dc, tn.ppw);
     mark(c, p == 0 ? 7 : 9, false, s);
     hip_check(hipStreamWaitEvent(s, c->ev_join, 0), "wait join");
   } else {
     mark(c, p == 0 ? 7 : 9, true, s);
-    launch_splat8(p == 1, tn.variant, items, nitems, sn, s);
+    launch_splat8(p == 1, tn.variant, items, nitems, sn, s, dc, tn.ppw);
     mark(c, p == 0 ? 7 : 9, false, s);
   }
   mark(c, p == 0 ? 3 : 6, false, s);
-  if (c->trace) {
+  if (c->trace && !pl.dc) {
     std::vector<uint64_t> &tv = c->trace_items[p];
     tv.resize(static_cast<size_t>(nitems) * 4);
     if (nitems)
@@ -966,34 +1045,13 @@ void render_chunked_pass(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, uint3
   pl.recs[p] = recs;
 }
 
-// Batch shape for the autotuner's cache: views, their particle counts, image
-// sizes, region size and pass mix (FNV-1a over the values).
-uint64_t batch_shape(const Plan &pl) {
-  uint64_t h = 1469598103934665603ull;
-  auto mix = [&](uint64_t v) {
-    for (int i = 0; i < 8; ++i) {
-      h ^= (v >> (8 * i)) & 0xff;
-      h *= 1099511628211ull;
-    }
-  };
-  mix(static_cast<uint64_t>(pl.nj));
-  mix(static_cast<uint64_t>(pl.G));
-  for (int j = 0; j < pl.nj; ++j) {
-    mix(static_cast<uint64_t>(pl.pa[j].n));
-    mix((static_cast<uint64_t>(pl.js[j].xres) << 32) | static_cast<uint32_t>(pl.js[j].yres));
-    mix(static_cast<uint64_t>(pl.js[j].T));
-  }
-  mix((static_cast<uint64_t>(pl.m[0]) << 32) | pl.m[1]);
-  return h;
-}
-
-// Autotuner: times the splat of every pass that can be repeated (no view adds
-// to an earlier pass) under each candidate tuning and returns the fastest.
-// All candidates produce the same images; they differ only in how the GPU
-// work is decomposed: the 8x8 splat's inner loop (hip_variant 3: software
-// pipelined, 4: chunks of 4-8 particles), the list length from which a block
-// is split per channel, and how many heaviest blocks get producer/consumer
-// workgroups.
+// Autotuner, inner part (8 px work regions): times the splat of every pass
+// that can be repeated (no view adds to an earlier pass) on the frame's own
+// bins under candidate decompositions, one knob at a time: the cap on
+// producer/consumer workgroups (hip_heavy_max, 0 = none), then the list
+// length from which a block is split per channel (hip_split), then the 8x8
+// splat's inner loop (hip_variant 3: software pipelined; 4: chunks of hip_ppw
+// particles).  All candidates give the same images.
 Tuning autotune(splhip_ctx *c, Plan &pl, const Tuning &base, uint32_t *tot, hipStream_t s) {
   bool repeatable[2];
   for (int p = 0; p < 2; ++p) {
@@ -1005,36 +1063,52 @@ Tuning autotune(splhip_ctx *c, Plan &pl, const Tuning &base, uint32_t *tot, hipS
   hipEvent_t e0, e1;
   hip_check(hipEventCreate(&e0), "hipEventCreate");
   hip_check(hipEventCreate(&e1), "hipEventCreate");
+  auto time_of = [&](const Tuning &t) {
+    float ms = 0.f;
+    for (int p = 0; p < 2; ++p) {
+      if (!repeatable[p]) continue;
+      stage_items_count(c, pl, p, t, tot, s);
+      read_totals(c, tot + T_ITEMS, 4, s);
+      pl.nitems[p] = c->h_tot[p];
+      pl.nheavy[p] = std::min(c->h_tot[2 + p], t.heavy_max);
+      hip_check(hipEventRecord(e0, s), "hipEventRecord");
+      stage_splat(c, pl, p, t, s);
+      hip_check(hipEventRecord(e1, s), "hipEventRecord");
+      hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+      float x = 0.f;
+      hip_check(hipEventElapsedTime(&x, e0, e1), "hipEventElapsedTime");
+      ms += x;
+    }
+    return ms;
+  };
   Tuning best = base;
+  best.variant = 4;
+  best.ppw = 0;
   float best_ms = 1e30f;
-  for (int variant : {4, 3})
-    for (uint32_t split : {4096u, 8192u, 32768u})
-      for (uint32_t hmax : {0u, 96u, 192u, 256u}) {
-        Tuning t = base;
-        t.variant = variant;
-        t.split = split;
-        t.heavy_max = hmax == 0 ? 1u : hmax;
-        if (hmax == 0) t.heavy = 0xffffffffu;
-        float ms = 0.f;
-        for (int p = 0; p < 2; ++p) {
-          if (!repeatable[p]) continue;
-          stage_items_count(c, pl, p, t, tot, s);
-          read_totals(c, tot + T_ITEMS, 4, s);
-          pl.nitems[p] = c->h_tot[p];
-          pl.nheavy[p] = std::min(c->h_tot[2 + p], t.heavy_max);
-          hip_check(hipEventRecord(e0, s), "hipEventRecord");
-          stage_splat(c, pl, p, t, s);
-          hip_check(hipEventRecord(e1, s), "hipEventRecord");
-          hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
-          float x = 0.f;
-          hip_check(hipEventElapsedTime(&x, e0, e1), "hipEventElapsedTime");
-          ms += x;
-        }
-        if (ms < best_ms) {
-          best_ms = ms;
-          best = t;
-        }
-      }
+  for (uint32_t hmax : {0u, 96u, 192u, 256u}) {
+    Tuning t = best;
+    t.heavy_max = hmax == 0 ? 1u : hmax;
+    t.heavy = hmax == 0 ? 0xffffffffu : base.heavy;
+    if (hmax != 0 && t.heavy == 0xffffffffu) t.heavy = 32768u;
+    const float ms = time_of(t);
+    if (ms < best_ms) { best_ms = ms; best = t; }
+  }
+  const Tuning b1 = best;
+  for (uint32_t split : {4096u, 8192u, 32768u}) {
+    Tuning t = b1;
+    t.split = split;
+    const float ms = time_of(t);
+    if (ms < best_ms) { best_ms = ms; best = t; }
+  }
+  const Tuning b2 = best;
+  const int cand[6][2] = {{4, 0}, {4, 1}, {4, 2}, {4, 4}, {4, 8}, {3, 0}};
+  for (const auto &vp : cand) {
+    Tuning t = b2;
+    t.variant = vp[0];
+    t.ppw = vp[1];
+    const float ms = time_of(t);
+    if (ms < best_ms) { best_ms = ms; best = t; }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return best;
@@ -1042,19 +1116,27 @@ Tuning autotune(splhip_ctx *c, Plan &pl, const Tuning &base, uint32_t *tot, hipS
 
 // The frame pipeline over a batch of jobs; three host syncs per batch (pass
 // sizes, bin entries, work items).
-int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool keep_fwd) {
-  const int G = nj == 1 ? region_size(jobs[0].prm) : 8;
+// A choice of the autotuner: work region and the 8x8 splat's decomposition.
+struct Choice {
+  int G;
+  Tuning tn;
+};
+
+// force: a tuning chosen by the autotuner (null: the params' keys; with
+// hip_autotune and 8 px regions the splat's decomposition is then swept on
+// this frame, and the choice left in c->inner_choice).
+int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool keep_fwd,
+                const Choice *force) {
+  const int G = force ? force->G : region_size(jobs[0].prm);
   if (G != 8 && G != 16 && G != 32 && G != 64)
     return fail(c, SPLHIP_E_ARG, "hip_region must be 8, 16, 32 or 64");
-  if (nj > 1 && jobs[0].prm->hip_region != 0 && jobs[0].prm->hip_region != 8)
-    return fail(c, SPLHIP_E_ARG, "batched renders use 8 px work regions (hip_region 0 or 8)");
   drop_fwd(c);
   for (int k = 0; k < SPLHIP_NSTAGES; ++k) c->used[k] = false;
   c->nmarks = 0;
   c->last_entries[0] = c->last_entries[1] = c->last_active[0] = c->last_active[1] = 0;
   c->trace_items[0].clear();
   c->trace_items[1].clear();
-  Tuning tn = tuning_of(jobs[0].prm);
+  Tuning tn = force ? force->tn : tuning_of(jobs[0].prm);
   Plan pl;
   make_plan(c, nj, jobs, G, pl, s);
   Workspace &ws = c->ws;
@@ -1079,6 +1161,7 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
       chunked[p] = pl.m[p] && e64[p] > tn.chunk;
     }
     if (chunked[0] || chunked[1]) {
+      c->split_hint = nj > 1;  // the caller may render the views in smaller groups
       if (nj != 1 || G != 8)
         return fail(c, SPLHIP_E_CAPACITY,
                     "a batched (or 16-64 px region) pass exceeds hip_chunk_entries bin entries; "
@@ -1108,8 +1191,9 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
   // leaves any value but -0 as it is; the ordered pass never writes -0).
   // Its longest lists stay in the 8x8 kernel: producer/consumer workgroups
   // would hold whole CUs while the ordered pass bins.
+  const bool sweep = !force && G == 8 && jobs[0].prm->hip_autotune;
   const bool early = !chunked[0] && !chunked[1] && nj == 1 && G == 8 && pl.m[0] && pl.m[1] &&
-                     pl.js[0].accumulate[1] == 1 && !jobs[0].prm->hip_autotune;
+                     pl.js[0].accumulate[1] == 1 && !sweep;
   if (early) {
     Tuning ta = tn;
     ta.heavy = 0xffffffffu;
@@ -1150,19 +1234,11 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
   } else if ((pl.m[0] || pl.m[1]) && !chunked[0] && !chunked[1]) {
     for (int p = 0; p < 2; ++p)
       if (pl.m[p]) stage_bins(c, pl, p, s);
-    if (G == 8 && jobs[0].prm->hip_autotune) {
-      const uint64_t key = batch_shape(pl);
-      auto it = c->tuned.find(key);
-      if (it == c->tuned.end()) {
-        const Tuning t = autotune(c, pl, tn, tot, s);
-        it = c->tuned.emplace(key, std::array<uint32_t, 3>{static_cast<uint32_t>(t.variant),
-                                                           t.split, t.heavy_max}).first;
-        if (t.heavy == 0xffffffffu) it->second[2] = 0;
-      }
-      tn.variant = static_cast<int>(it->second[0]);
-      tn.split = it->second[1];
-      tn.heavy_max = it->second[2] == 0 ? 1u : it->second[2];
-      if (it->second[2] == 0) tn.heavy = 0xffffffffu;
+    if (sweep) {
+      tn = autotune(c, pl, tn, tot, s);
+      const uint32_t ic[5] = {static_cast<uint32_t>(tn.variant), static_cast<uint32_t>(tn.ppw),
+                              tn.split, tn.heavy, tn.heavy_max};
+      std::memcpy(c->inner_choice, ic, sizeof(ic));
       tuned = true;
     }
     for (int p = 0; p < 2; ++p)
@@ -1226,11 +1302,286 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
   c->last_tuning[0] = static_cast<uint32_t>(tn.variant);
   c->last_tuning[1] = tn.split;
   c->last_tuning[2] = tn.heavy == 0xffffffffu ? 0u : tn.heavy_max;
-  c->last_tuning[3] = tuned ? 1u : 0u;
+  c->last_tuning[3] = (tuned || force) ? 1u : 0u;
+  c->last_tuning[4] = static_cast<uint32_t>(G);
+  c->last_tuning[5] = static_cast<uint32_t>(tn.ppw);
   c->last_regions[0] = static_cast<int64_t>(pl.used16[0]);
   c->last_regions[1] = static_cast<int64_t>(pl.used16[1]);
   if (keep_fwd && nj == 1 && G == 8 && !chunked[0] && !chunked[1]) {
     c->fwd = new FwdState{*jobs[0].P, *jobs[0].cam, *jobs[0].prm, pl, tn};
+  }
+  return 0;
+}
+
+// Autotuner, outer part: on the first render of a batch bucket (views,
+// particles and pixels each by factors of sqrt(2) or 2, the pass mix and the
+// reference tile size), the 8 px work regions with the inner sweep of the
+// splat's decomposition, then 16, 32 and 64 px regions, each timed over one
+// whole render; the fastest is reused for the bucket.  Every candidate gives
+// the same images.  Renders that keep the backward's state (keep_fwd) and
+// renders with hip_region set keep their work region.
+uint64_t bucket_of(int nj, const JobIn *jobs) {
+  int64_t n = 0;
+  double px = 0.0;
+  bool mode[2] = {false, false};
+  for (int j = 0; j < nj; ++j) {
+    n += jobs[j].P->n;
+    px += static_cast<double>(jobs[j].cam->xres) * jobs[j].cam->yres;
+    for (int t = 0; t < jobs[j].prm->ptypes; ++t) {
+      if (jobs[j].prm->a_eq_e[t] != 1) mode[0] = true;
+      if (jobs[j].prm->a_eq_e[t] != 0) mode[1] = true;
+    }
+  }
+  const uint64_t bn = static_cast<uint64_t>(std::floor(2.0 * std::log2(std::max<double>(n, 1))));
+  const uint64_t bp = static_cast<uint64_t>(std::floor(2.0 * std::log2(std::max(px, 1.0))));
+  const uint64_t bj = static_cast<uint64_t>(std::floor(std::log2(static_cast<double>(nj))));
+  const uint64_t T = static_cast<uint64_t>(jobs[0].prm->tile_size) & 0xfff;
+  return bn | (bp << 8) | (bj << 16) | (static_cast<uint64_t>(mode[0]) << 24) |
+         (static_cast<uint64_t>(mode[1]) << 25) | (T << 26);
+}
+
+int render_tuned(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool keep_fwd,
+                 bool region8 = false) {
+  const splhip_params *prm = jobs[0].prm;
+  if (!prm->hip_autotune) return render_jobs(c, nj, jobs, s, keep_fwd, nullptr);
+  const uint64_t key = bucket_of(nj, jobs) | ((keep_fwd || region8) ? uint64_t(1) << 40 : 0) |
+                       (static_cast<uint64_t>(prm->hip_region & 0x7f) << 41);
+  auto decode = [](const std::vector<uint32_t> &v, Choice &ch) {
+    ch.G = static_cast<int>(v[0]);
+    ch.tn.variant = static_cast<int>(v[1]);
+    ch.tn.ppw = static_cast<int>(v[2]);
+    ch.tn.split = v[3];
+    ch.tn.heavy = v[4];
+    ch.tn.heavy_max = v[5];
+  };
+  auto it = c->tuned.find(key);
+  if (it != c->tuned.end()) {
+    Choice ch{8, tuning_of(prm)};
+    decode(it->second, ch);
+    return render_jobs(c, nj, jobs, s, keep_fwd, &ch);
+  }
+  hipEvent_t e0, e1;
+  hip_check(hipEventCreate(&e0), "hipEventCreate");
+  hip_check(hipEventCreate(&e1), "hipEventCreate");
+  auto timed = [&](const Choice *ch, float &ms) {
+    hip_check(hipEventRecord(e0, s), "hipEventRecord");
+    const int rc = render_jobs(c, nj, jobs, s, false, ch);
+    hip_check(hipEventRecord(e1, s), "hipEventRecord");
+    hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    return rc;
+  };
+  const Tuning base = tuning_of(prm);
+  const int G0 = region_size(prm);
+  Choice best{G0, base};
+  float best_ms = 1e30f, ms = 0.f;
+  int rc = 0;
+  if (G0 == 8) {  // the inner sweep on this frame, then its choice timed over a render
+    const uint32_t ic[5] = {static_cast<uint32_t>(base.variant), static_cast<uint32_t>(base.ppw),
+                            base.split, base.heavy, base.heavy_max};
+    std::memcpy(c->inner_choice, ic, sizeof(ic));
+    rc = render_jobs(c, nj, jobs, s, false, nullptr);
+    best.tn.variant = static_cast<int>(c->inner_choice[0]);
+    best.tn.ppw = static_cast<int>(c->inner_choice[1]);
+    best.tn.split = c->inner_choice[2];
+    best.tn.heavy = c->inner_choice[3];
+    best.tn.heavy_max = c->inner_choice[4];
+  }
+  if (rc == 0) rc = timed(&best, best_ms);
+  if (rc == 0 && !keep_fwd && !region8 && prm->hip_region == 0) {
+    for (int G : {16, 32, 64}) {
+      const Choice ch{G, base};
+      if (timed(&ch, ms) == 0 && ms < best_ms) {
+        best_ms = ms;
+        best = ch;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  c->tuned[key] = {static_cast<uint32_t>(best.G), static_cast<uint32_t>(best.tn.variant),
+                   static_cast<uint32_t>(best.tn.ppw), best.tn.split, best.tn.heavy,
+                   best.tn.heavy_max};
+  return render_jobs(c, nj, jobs, s, keep_fwd, &best);
+}
+
+// A batch in groups of consecutive views of at most kGroupParticles
+// particles (a larger view alone), each group one pipeline; a group with a
+// pass past the chunk size (hip_chunk_entries) is halved until its views
+// fit or render alone, chunked.  Every image equals its view's plain render.
+constexpr int64_t kGroupParticles = int64_t(1) << 28;
+
+int render_groups(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s) {
+  int j0 = 0;
+  while (j0 < nj) {
+    int j1 = j0 + 1;
+    int64_t n = jobs[j0].P->n;
+    while (j1 < nj && n + jobs[j1].P->n <= kGroupParticles) n += jobs[j1++].P->n;
+    std::vector<std::pair<int, int>> todo{{j0, j1}};
+    while (!todo.empty()) {
+      const auto g = todo.back();
+      todo.pop_back();
+      c->split_hint = false;
+      const int rc = render_tuned(c, g.second - g.first, jobs + g.first, s, false);
+      if (rc == SPLHIP_E_CAPACITY && c->split_hint && g.second - g.first > 1) {
+        const int mid = (g.first + g.second) / 2;
+        todo.push_back({mid, g.second});
+        todo.push_back({g.first, mid});
+        continue;
+      }
+      if (rc) return rc;
+    }
+    j0 = j1;
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------ capture
+
+}  // namespace
+
+// A batch reserved for rendering with device counts (no host waits, no
+// allocation), e.g. under stream capture: its device tables, the binning
+// form and entry capacities of each pass, the tuning, and a workspace that
+// only this batch uses and that is frozen once sized.
+struct Reservation {
+  std::vector<char> tables;  // host copy of the tables (a captured batch must match)
+  size_t o[7] = {};
+  char *d_tables = nullptr;  // in ws (S_JOBS)
+  int nj = 0;
+  bool use16[2] = {false, false};
+  uint32_t ecap[2] = {0, 0}, e16cap[2] = {0, 0};
+  Tuning tn{};
+  Workspace ws;
+};
+
+namespace {
+
+// The frame pipeline with device counts over a reserved batch: every launch
+// is sized from the reservation's capacities and reads the counts from the
+// device, the job flags and empty images are set by kernels, and nothing
+// waits on the host, allocates or copies from the host.  A pass whose
+// entries exceed its capacity is dropped and flagged (splhip_capture_status).
+int render_jobs_dc(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s) {
+  Reservation &rv = *c->rv;
+  if (region_size(jobs[0].prm) != 8)
+    return fail(c, SPLHIP_E_ARG, "device-count renders use 8 px work regions");
+  if (c->pal_dirty)
+    return fail(c, SPLHIP_E_STATE, "palettes changed since the batch was reserved");
+  Plan pl;
+  build_plan(c, nj, jobs, 8, pl);
+  std::vector<char> tab;
+  size_t o[7];
+  plan_tables(pl, &tab, o);
+  if (nj != rv.nj || tab != rv.tables)
+    return fail(c, SPLHIP_E_STATE,
+                "the batch differs from the one reserved by splhip_capture_reserve");
+  plan_bind(pl, rv.d_tables, o);
+  pl.ws = &rv.ws;
+  pl.dc = true;
+  Workspace &ws = rv.ws;
+  const Tuning tn = rv.tn;
+  const int prof = c->prof;
+  c->prof = 0;  // no stage events (they would need host waits)
+  Rec *rec = ws.get_as<Rec>(S_REC, static_cast<size_t>(std::max<int64_t>(pl.ntot, 1)));
+  uint8_t *act = ws.get_as<uint8_t>(S_ACT, static_cast<size_t>(std::max<int64_t>(pl.ntot, 1)));
+  uint32_t *tot = ws.get_as<uint32_t>(S_TOTALS, 64);
+  pl.tot = tot;
+  for (int p = 0; p < 2; ++p) {
+    pl.use16[p] = rv.use16[p];
+    pl.e[p] = rv.ecap[p];
+    pl.e16[p] = rv.e16cap[p];
+  }
+  stage_project_compact(c, pl, rec, act, tot, s);
+  launch_job_flags(pl.d_mo[0], pl.d_mo[1], nj, pl.d_js, s);
+  for (int p = 0; p < 2; ++p)
+    if (pl.m[p]) {
+      stage_order(c, pl, p, rec, act, s);
+      stage_count(c, pl, p, rec, tot, s);
+    }
+  for (int p = 0; p < 2; ++p)
+    if (pl.m[p]) stage_bins(c, pl, p, s);
+  for (int p = 0; p < 2; ++p)
+    if (pl.m[p]) {
+      stage_items_count(c, pl, p, tn, tot, s);
+      pl.nitems[p] = 3u * pl.nunits;  // at most three items per unit
+      pl.nheavy[p] = tn.heavy == 0xffffffffu ? 0u : tn.heavy_max;
+    }
+  for (int p = 0; p < 2; ++p)
+    if (pl.m[p]) stage_splat(c, pl, p, tn, s);
+  launch_fill_empty_jobs(pl.d_mo[0], pl.d_mo[1], nj, pl.d_js, s);
+  c->prof = prof;
+  return 0;
+}
+
+bool stream_capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &st) != hipSuccess) return false;
+  return st == hipStreamCaptureStatusActive;
+}
+
+// Render the batch once with host waits, size the capacities from its counts
+// (entries x headroom), then render it once with device counts into a fresh
+// workspace, which is frozen afterwards.
+int capture_reserve(splhip_ctx *c, int nj, const JobIn *jobs, float headroom, hipStream_t s) {
+  if (region_size(jobs[0].prm) != 8)
+    return fail(c, SPLHIP_E_ARG, "device-count renders use 8 px work regions");
+  if (nj > 65535) return fail(c, SPLHIP_E_CAPACITY, "at most 65535 views per reserved batch");
+  if (!(headroom >= 1.f)) return fail(c, SPLHIP_E_ARG, "headroom must be at least 1");
+  delete c->rv;
+  c->rv = nullptr;
+  if (int rc = render_tuned(c, nj, jobs, s, false, true)) return rc;
+  Tuning tn = tuning_of(jobs[0].prm);
+  if (c->last_entries[0] > static_cast<int64_t>(tn.chunk) ||
+      c->last_entries[1] > static_cast<int64_t>(tn.chunk))
+    return fail(c, SPLHIP_E_CAPACITY,
+                "a pass exceeds hip_chunk_entries bin entries: not renderable without host waits");
+  tn.variant = static_cast<int>(c->last_tuning[0]);
+  tn.ppw = static_cast<int>(c->last_tuning[5]);
+  tn.split = c->last_tuning[1];
+  if (c->last_tuning[2] == 0) {
+    tn.heavy = 0xffffffffu;
+    tn.heavy_max = 1;
+  } else {
+    tn.heavy_max = c->last_tuning[2];
+  }
+  Reservation *rv = new Reservation;
+  rv->tn = tn;
+  rv->nj = nj;
+  auto cap = [&](int64_t v) {
+    const double x = static_cast<double>(v) * headroom + 4096.0;
+    return static_cast<uint32_t>(std::min(x, 4294967295.0 - 65536.0));
+  };
+  for (int p = 0; p < 2; ++p) {
+    rv->use16[p] = c->last_regions[p] > 0;
+    rv->ecap[p] = cap(c->last_entries[p]);
+    rv->e16cap[p] = cap(c->last_regions[p]);
+  }
+  Plan pl;
+  build_plan(c, nj, jobs, 8, pl);
+  const size_t total = plan_tables(pl, &rv->tables, rv->o);
+  rv->d_tables = rv->ws.get_as<char>(S_JOBS, total);
+  hip_check(hipStreamSynchronize(s), "sync before the table upload");
+  hip_check(hipMemcpy(rv->d_tables, rv->tables.data(), total, hipMemcpyHostToDevice),
+            "upload reserved tables");
+  ensure_aux(c);
+  c->rv = rv;
+  if (int rc = render_jobs_dc(c, nj, jobs, s)) {
+    delete c->rv;
+    c->rv = nullptr;
+    return rc;
+  }
+  uint32_t ovf = 0;
+  hip_check(hipMemcpyAsync(c->h_tot, rv->ws.get_as<uint32_t>(S_TOTALS, 64) + T_OVF, 4,
+                           hipMemcpyDeviceToHost, s), "read overflow flags");
+  hip_check(hipStreamSynchronize(s), "sync reserve");
+  ovf = c->h_tot[0];
+  rv->ws.freeze(true);
+  if (ovf != 0) {
+    delete c->rv;
+    c->rv = nullptr;
+    return fail(c, SPLHIP_E_CAPACITY, "reserved capacities overflowed on the sizing render");
   }
   return 0;
 }
@@ -1320,9 +1671,9 @@ int splhip_set_profiling(splhip_ctx *c, int32_t enable) {
   return 0;
 }
 
-int splhip_last_tuning(splhip_ctx *c, int64_t out[4]) {
+int splhip_last_tuning(splhip_ctx *c, int64_t out[6]) {
   if (!c || !out) return SPLHIP_E_ARG;
-  for (int k = 0; k < 4; ++k) out[k] = c->last_tuning[k];
+  for (int k = 0; k < 6; ++k) out[k] = c->last_tuning[k];
   return 0;
 }
 
@@ -1364,6 +1715,7 @@ void splhip_destroy(splhip_ctx *c) {
   delete c->fwd;
   (void)hipSetDevice(c->device);
   (void)hipDeviceSynchronize();
+  delete c->rv;
   for (auto &e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->ev_stage) (void)hipEventDestroy(c->ev_stage);
   if (c->h_stage) (void)hipHostFree(c->h_stage);
@@ -1472,6 +1824,7 @@ int splhip_params_default(splhip_params *p) {
   p->hip_chunk_entries = 0;
   p->hip_autotune = 0;
   p->hip_bin_group = 0;
+  p->hip_ppw = 0;
   return 0;
 }
 
@@ -1513,6 +1866,7 @@ int splhip_params_from_par(const char *par_text, float b_brightness, splhip_para
     p->hip_chunk_entries = par_num<int>(d, "hip_chunk_entries", 0);
     p->hip_autotune = par_num<int>(d, "hip_autotune", 0);
     p->hip_bin_group = par_num<int>(d, "hip_bin_group", 0);
+    p->hip_ppw = par_num<int>(d, "hip_ppw", 0);
   } catch (const ParError &) {
     return SPLHIP_E_ARG;
   }
@@ -1918,7 +2272,12 @@ int splhip_render(splhip_ctx *c, const splhip_particles *P, const splhip_camera 
   try {
     hip_check(hipSetDevice(c->device), "hipSetDevice");
     const JobIn job{P, cam, prm, pic};
-    return render_jobs(c, 1, &job, static_cast<hipStream_t>(stream), true);
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    if (stream_capturing(s)) {
+      if (!c->rv) return fail(c, SPLHIP_E_STATE, "capture needs splhip_capture_reserve first");
+      return render_jobs_dc(c, 1, &job, s);
+    }
+    return render_tuned(c, 1, &job, s, true);
   } catch (const HipError &e) {
     return hip_fail(c, e);
   }
@@ -2027,7 +2386,7 @@ int splhip_render_host(splhip_ctx *c, const splhip_particle_sim *hp, int64_t n,
     P.n = n;
     if (int rc = check_inputs(c, &P, cam, prm)) return rc;
     const JobIn job{&P, cam, prm, pic};
-    if (int rc = render_jobs(c, 1, &job, s, false)) return rc;
+    if (int rc = render_tuned(c, 1, &job, s, false)) return rc;
     hip_check(hipMemcpyAsync(host_pic, pic, npix * sizeof(float), hipMemcpyDeviceToHost, s),
               "D2H image");
     hip_check(hipStreamSynchronize(s), "sync");
@@ -2140,10 +2499,49 @@ int splhip_render_batched(splhip_ctx *c, int32_t njobs, const splhip_job *jobs, 
   }
   try {
     hip_check(hipSetDevice(c->device), "hipSetDevice");
-    return render_jobs(c, njobs, in.data(), static_cast<hipStream_t>(stream), false);
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    if (stream_capturing(s)) {
+      if (!c->rv) return fail(c, SPLHIP_E_STATE, "capture needs splhip_capture_reserve first");
+      return render_jobs_dc(c, njobs, in.data(), s);
+    }
+    return render_groups(c, njobs, in.data(), s);
   } catch (const HipError &e) {
     return hip_fail(c, e);
   }
+}
+
+int splhip_capture_reserve(splhip_ctx *c, int32_t njobs, const splhip_job *jobs, float headroom,
+                           void *stream) {
+  if (!c || njobs < 1 || !jobs) return fail(c, SPLHIP_E_ARG, "null argument");
+  std::vector<JobIn> in(njobs);
+  for (int j = 0; j < njobs; ++j) {
+    if (int rc = check_inputs(c, &jobs[j].parts, &jobs[j].cam, &jobs[j].prm)) return rc;
+    if (!jobs[j].pic) return fail(c, SPLHIP_E_ARG, "null image");
+    in[j] = JobIn{&jobs[j].parts, &jobs[j].cam, &jobs[j].prm, jobs[j].pic};
+  }
+  try {
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    if (stream_capturing(s)) return fail(c, SPLHIP_E_STATE, "reserve outside stream capture");
+    return capture_reserve(c, njobs, in.data(), headroom, s);
+  } catch (const HipError &e) {
+    return hip_fail(c, e);
+  }
+}
+
+int splhip_capture_status(splhip_ctx *c, int32_t *overflow) {
+  if (!c || !overflow) return fail(c, SPLHIP_E_ARG, "null argument");
+  if (!c->rv) return fail(c, SPLHIP_E_STATE, "no reserved batch");
+  try {
+    hip_check(hipSetDevice(c->device), "hipSetDevice");
+    uint32_t *tot = c->rv->ws.get_as<uint32_t>(S_TOTALS, 64);
+    hip_check(hipDeviceSynchronize(), "sync status");
+    hip_check(hipMemcpy(c->h_tot, tot + T_OVF, 4, hipMemcpyDeviceToHost), "read overflow flags");
+    *overflow = static_cast<int32_t>(c->h_tot[0]);
+  } catch (const HipError &e) {
+    return hip_fail(c, e);
+  }
+  return 0;
 }
 
 }  // extern "C"

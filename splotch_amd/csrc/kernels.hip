@@ -163,7 +163,7 @@ __global__ __launch_bounds__(256) void k_compact(const ProjectArgs *__restrict__
   const int64_t iend = min(i0 + kPerBlock, n);
   for (int64_t k = i0 + threadIdx.x; k < iend; k += blockDim.x) {
     const int64_t i = poff + k;
-    if (act[i] != code) continue;
+    if (!(act[i] & code)) continue;
     const uint32_t p = pos[i];
     vals[p] = static_cast<uint32_t>(i);
     if (keys) keys[p] = sort_key(rec[i], sort_type);
@@ -177,18 +177,27 @@ __global__ __launch_bounds__(256) void k_pass_offsets(const uint32_t *pos, const
     mo[j] = (j < nj && pbase[j] < n) ? pos[pbase[j]] : *total;
 }
 
-__global__ __launch_bounds__(256) void k_job_of_index(const uint32_t *vals, uint32_t m,
-                                                      const uint32_t *pbase, int nj,
-                                                      uint32_t *keys) {
+// The count of a launch sized for `cap` items: cap, or the device count *dn
+// when it is given (at most cap).
+__device__ __forceinline__ uint32_t dcount(uint32_t cap, const uint32_t *dn) {
+  return dn ? min(cap, *dn) : cap;
+}
+
+__global__ __launch_bounds__(256) void k_job_of_index(const uint32_t *vals, uint32_t cap,
+                                                      const uint32_t *dm, const uint32_t *pbase,
+                                                      int nj, uint32_t *keys) {
+  const uint32_t m = dcount(cap, dm);
   for (int64_t i = gtid(); i < m; i += gstride())
     keys[i] = static_cast<uint32_t>(find_job(pbase, nj, vals[i]));
 }
 
-__global__ __launch_bounds__(256) void k_gather_count(const uint32_t *order, uint32_t m,
-                                                      const Rec *rec, Rec *recs, uint32_t *cnt,
+__global__ __launch_bounds__(256) void k_gather_count(const uint32_t *order, uint32_t cap,
+                                                      const uint32_t *dm, const Rec *rec,
+                                                      Rec *recs, uint32_t *cnt,
                                                       uint32_t *cnt16, int rsh,
                                                       const BinGeom *geoms, const uint32_t *mo,
                                                       int nj, uint32_t *qbad) {
+  const uint32_t m = dcount(cap, dm);
   for (int64_t s = gtid(); s < m; s += gstride()) {
     const BinGeom g = geoms[find_job(mo, nj, static_cast<uint32_t>(s))];
     const Rec R = rec[order[s]];
@@ -346,12 +355,14 @@ __device__ __forceinline__ void emit_drain(EmitQueue &Q, int qn, int lane, uint3
 // the region's blocks the particle is binned to; regions of job j are numbered
 // from ubase[j], ceil(ngm / S) per tile row.
 template <int R>
-__global__ __launch_bounds__(256) void k_emit(const Rec *recs, uint32_t m, const uint32_t *off,
-                                              uint32_t e, const BinGeom *geoms,
+__global__ __launch_bounds__(256) void k_emit(const Rec *recs, uint32_t mcap, const uint32_t *dm,
+                                              const uint32_t *off, uint32_t ecap,
+                                              const uint32_t *de, const BinGeom *geoms,
                                               const uint32_t *ubase, const uint32_t *mo, int nj,
                                               uint32_t *keys, uint32_t *vals, int rsh,
                                               int rbits) {
   __shared__ EmitQueue s_q[4];
+  const uint32_t m = dcount(mcap, dm), e = dcount(ecap, de);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   EmitQueue &Q = s_q[wave];
   const int64_t nwaves = static_cast<int64_t>(gridDim.x) * 4;
@@ -493,9 +504,10 @@ __device__ __forceinline__ uint32_t lane_pick(const uint32_t (&v)[NB], int lane)
 }
 
 template <int NB>
-__global__ __launch_bounds__(256) void k_split_chunks(const uint32_t *keys, uint32_t e16,
-                                                      uint32_t nchunks, int rbits,
-                                                      uint32_t *ccnt) {
+__global__ __launch_bounds__(256) void k_split_chunks(const uint32_t *keys, uint32_t ecap,
+                                                      const uint32_t *de, uint32_t nchunks,
+                                                      int rbits, uint32_t *ccnt) {
+  const uint32_t e16 = dcount(ecap, de);
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = static_cast<int64_t>(gridDim.x) * 4;
   for (int64_t c = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6); c < nchunks;
@@ -588,9 +600,11 @@ __global__ __launch_bounds__(256) void k_split_base(uint32_t nregions, const uin
 // One wave per chunk: every entry to its blocks' lists.
 template <int S>
 __global__ __launch_bounds__(256) void k_split_scatter(const uint32_t *keys, const uint32_t *vals,
-                                                       uint32_t e16, uint32_t nchunks, int rbits,
+                                                       uint32_t ecap, const uint32_t *de,
+                                                       uint32_t nchunks, int rbits,
                                                        const uint32_t *cpre,
                                                        const uint32_t *base, uint32_t *ent) {
+  const uint32_t e16 = dcount(ecap, de);
   constexpr int NB = S * S;
   const int lane = threadIdx.x & 63;
   const uint32_t rmask = (1u << rbits) - 1u;
@@ -619,9 +633,10 @@ __global__ __launch_bounds__(256) void k_split_scatter(const uint32_t *keys, con
   }
 }
 
-__global__ __launch_bounds__(256) void k_bin_offsets(const uint32_t *keys, uint32_t e,
-                                                     uint32_t nbins, uint32_t kmask,
-                                                     uint32_t *bin_off) {
+__global__ __launch_bounds__(256) void k_bin_offsets(const uint32_t *keys, uint32_t ecap,
+                                                     const uint32_t *de, uint32_t nbins,
+                                                     uint32_t kmask, uint32_t *bin_off) {
+  const uint32_t e = dcount(ecap, de);
   for (int64_t r = gtid(); r <= nbins; r += gstride()) {
     uint32_t lo = 0, hi = e;
     while (lo < hi) {
@@ -643,8 +658,9 @@ __global__ __launch_bounds__(256) void k_fill(float *p, float v, int64_t n) {
 }
 
 // 64-bit sum of n u32 values (block partial sums added atomically).
-__global__ __launch_bounds__(256) void k_sum_u64(const uint32_t *v, int64_t n,
-                                                 unsigned long long *out) {
+__global__ __launch_bounds__(256) void k_sum_u64(const uint32_t *v, int64_t cap,
+                                                 const uint32_t *dn, unsigned long long *out) {
+  const int64_t n = dn ? min(cap, static_cast<int64_t>(*dn)) : cap;
   __shared__ unsigned long long part[256];
   unsigned long long acc = 0;
   for (int64_t i = gtid(); i < n; i += gstride()) acc += v[i];
@@ -669,6 +685,50 @@ __global__ __launch_bounds__(64) void k_delay(uint32_t ticks) {
 
 __global__ __launch_bounds__(256) void k_fill_u32(uint32_t *p, uint32_t v, int64_t n) {
   for (int64_t i = gtid(); i < n; i += gstride()) p[i] = v;
+}
+
+// Per-job pass flags from the pass offsets mo0 / mo1 (null: pass absent):
+// the ordered pass writes the image of a job that has ordered particles; the
+// additive pass adds to it there and writes the image elsewhere.
+__global__ __launch_bounds__(64) void k_job_flags(const uint32_t *mo0, const uint32_t *mo1,
+                                                  int nj, JobSplat *js) {
+  for (int64_t j = gtid(); j < nj; j += gstride()) {
+    const bool w0 = mo0 && mo0[j + 1] > mo0[j];
+    const bool w1 = mo1 && mo1[j + 1] > mo1[j];
+    JobSplat &J = js[j];
+    J.accumulate[0] = 0;
+    J.accumulate[1] = w0 ? 1 : 0;
+    J.need_empty[0] = w0 ? 1 : 0;
+    J.need_empty[1] = (w1 && !w0) ? 1 : 0;
+  }
+}
+
+// Zeros into the image of every job with no particle in either pass
+// (grid.y = job).
+__global__ __launch_bounds__(256) void k_fill_empty_jobs(const uint32_t *mo0, const uint32_t *mo1,
+                                                         const JobSplat *js) {
+  const int j = blockIdx.y;
+  if ((mo0 && mo0[j + 1] > mo0[j]) || (mo1 && mo1[j + 1] > mo1[j])) return;
+  const JobSplat &J = js[j];
+  const int64_t n = static_cast<int64_t>(J.xres) * J.yres * 3;
+  for (int64_t i = gtid(); i < n; i += gstride()) J.pic[i] = 0.f;
+}
+
+// Capacity check of a pass binned with device counts: if its entries (u64
+// total e64) exceed ecap, or its region entries (e16) exceed e16cap, the
+// pass's particle and entry counts are set to zero (nothing is emitted or
+// splatted, so nothing is written past a buffer) and bit `bit` of *ovf is set.
+__global__ void k_cap_check(uint32_t *m, uint32_t *e, const unsigned long long *e64,
+                            uint32_t *e16, uint64_t ecap, uint32_t e16cap, uint32_t bit,
+                            uint32_t *ovf) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const bool over = *e64 > ecap || (e16 && *e16 > e16cap);
+  if (over) {
+    *m = 0;
+    *e = 0;
+    if (e16) *e16 = 0;
+    *ovf |= bit;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_unpack_records(const char *stage, int64_t n, int64_t C,
@@ -704,14 +764,21 @@ __global__ __launch_bounds__(256) void k_splat(SplatArgs a) {
   __shared__ float s_pre1[B][G];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const uint32_t rid = blockIdx.x;
-  const int per_tile = a.ngm * a.ngm;
-  const int tile = static_cast<int>(rid / per_tile);
-  const int rem = static_cast<int>(rid - static_cast<uint32_t>(tile) * per_tile);
-  const int gy = rem / a.ngm, gx = rem - gy * a.ngm;
-  const int cix = tile % a.ncx, ciy = tile / a.ncx;
-  const int x0s = cix * a.T, y0s = ciy * a.T;
-  const int w = min(x0s + a.T, a.xres) - x0s, h = min(y0s + a.T, a.yres) - y0s;
+  const uint32_t rid = blockIdx.x;  // unit (G x G region) of job j
+  const int j = find_job(a.ubase, a.nj, rid);
+  const JobSplat &J = a.jobs[j];
+  const uint32_t local = rid - a.ubase[j];
+  const int ngm = J.ngm, T = J.T, yres = J.yres;
+  const int accumulate = J.accumulate[a.pass];
+  const float grayabsorb = J.grayabsorb;
+  float *const pic = J.pic;
+  const int per_tile = ngm * ngm;
+  const int tile = static_cast<int>(local / per_tile);
+  const int rem = static_cast<int>(local - static_cast<uint32_t>(tile) * per_tile);
+  const int gy = rem / ngm, gx = rem - gy * ngm;
+  const int cix = tile % J.ncx, ciy = tile / J.ncx;
+  const int x0s = cix * T, y0s = ciy * T;
+  const int w = min(x0s + T, J.xres) - x0s, h = min(y0s + T, yres) - y0s;
   const int rx0 = gx * G, ry0 = gy * G;
   if (rx0 >= w || ry0 >= h) return;
   const int rw = min(G, w - rx0), rh = min(G, h - ry0);
@@ -748,7 +815,7 @@ __global__ __launch_bounds__(256) void k_splat(SplatArgs a) {
       s_a[1][tid] = -R.e1;
       s_a[2][tid] = -R.e2;
       if (!ADD) {
-        const float g = a.grayabsorb;
+        const float g = grayabsorb;
         s_q[0][tid] = R.e0 / (R.e0 + g);
         s_q[1][tid] = R.e1 / (R.e1 + g);
         s_q[2][tid] = R.e2 / (R.e2 + g);
@@ -836,8 +903,8 @@ __global__ __launch_bounds__(256) void k_splat(SplatArgs a) {
     const int lx = bxo + (lane >> 3), ly = byo + (lane & 7);
     if (lx < rx0 + rw && ly < ry0 + rh) {
       const int64_t X = x0s + lx, Y = y0s + ly;
-      float *d = a.pic + (X * a.yres + Y) * 3;
-      if (a.accumulate) {
+      float *d = pic + (X * yres + Y) * 3;
+      if (accumulate) {
         d[0] = d[0] + acc[j][0];
         d[1] = d[1] + acc[j][1];
         d[2] = d[2] + acc[j][2];
@@ -874,28 +941,30 @@ void launch_pass_offsets(const uint32_t *pos, const uint32_t *pbase, int nj, int
 }
 
 void launch_job_of_index(const uint32_t *vals, uint32_t m, const uint32_t *pbase, int nj,
-                         uint32_t *keys, hipStream_t s) {
-  k_job_of_index<<<grid_for(m), 256, 0, s>>>(vals, m, pbase, nj, keys);
+                         uint32_t *keys, hipStream_t s, const uint32_t *dm) {
+  k_job_of_index<<<grid_for(m), 256, 0, s>>>(vals, m, dm, pbase, nj, keys);
   hip_check(hipGetLastError(), "k_job_of_index launch");
 }
 
 void launch_gather_count(const uint32_t *order, uint32_t m, const Rec *rec, Rec *recs,
                          uint32_t *cnt, uint32_t *cnt16, int rsh, const BinGeom *geoms,
-                         const uint32_t *mo, int nj, uint32_t *qbad, hipStream_t s) {
-  k_gather_count<<<grid_for(m), 256, 0, s>>>(order, m, rec, recs, cnt, cnt16, rsh, geoms, mo,
-                                             nj, qbad);
+                         const uint32_t *mo, int nj, uint32_t *qbad, hipStream_t s,
+                         const uint32_t *dm) {
+  k_gather_count<<<grid_for(m), 256, 0, s>>>(order, m, dm, rec, recs, cnt, cnt16, rsh, geoms,
+                                             mo, nj, qbad);
   hip_check(hipGetLastError(), "k_gather_count launch");
 }
 
 void launch_emit(int rsh, int rbits, const Rec *recs, uint32_t m, const uint32_t *off,
                  uint32_t e, const BinGeom *geoms, const uint32_t *ubase, const uint32_t *mo,
-                 int nj, uint32_t *keys, uint32_t *vals, hipStream_t s) {
+                 int nj, uint32_t *keys, uint32_t *vals, hipStream_t s, const uint32_t *dm,
+                 const uint32_t *de) {
   if (rsh > 0)
-    k_emit<2><<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(recs, m, off, e, geoms, ubase, mo, nj,
-                                                        keys, vals, rsh, rbits);
+    k_emit<2><<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(recs, m, dm, off, e, de, geoms, ubase,
+                                                        mo, nj, keys, vals, rsh, rbits);
   else
-    k_emit<1><<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(recs, m, off, e, geoms, ubase, mo, nj,
-                                                        keys, vals, 0, 0);
+    k_emit<1><<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(recs, m, dm, off, e, de, geoms, ubase,
+                                                        mo, nj, keys, vals, 0, 0);
   hip_check(hipGetLastError(), "k_emit launch");
 }
 
@@ -904,12 +973,12 @@ uint32_t split_chunks(uint32_t e16) {
 }
 
 void launch_split_chunks(int S, const uint32_t *keys, uint32_t e16, int rbits, uint32_t *ccnt,
-                         hipStream_t s) {
+                         hipStream_t s, const uint32_t *de16) {
   const uint32_t nc = split_chunks(e16);
   if (nc == 0) return;
   const unsigned g = grid_for(nc, 4, 1 << 16);
-  if (S == 4) k_split_chunks<16><<<g, 256, 0, s>>>(keys, e16, nc, rbits, ccnt);
-  else k_split_chunks<4><<<g, 256, 0, s>>>(keys, e16, nc, rbits, ccnt);
+  if (S == 4) k_split_chunks<16><<<g, 256, 0, s>>>(keys, e16, de16, nc, rbits, ccnt);
+  else k_split_chunks<4><<<g, 256, 0, s>>>(keys, e16, de16, nc, rbits, ccnt);
   hip_check(hipGetLastError(), "k_split_chunks launch");
 }
 
@@ -944,21 +1013,21 @@ void launch_split_base(int S, uint32_t nregions, const uint32_t *rbase, const ui
 
 void launch_split_scatter(int S, const uint32_t *keys, const uint32_t *vals, uint32_t e16,
                           int rbits, const uint32_t *cpre, const uint32_t *base, uint32_t *ent,
-                          hipStream_t s) {
+                          hipStream_t s, const uint32_t *de16) {
   const uint32_t nc = split_chunks(e16);
   if (nc == 0) return;
   const unsigned g = grid_for(nc, 4, 1 << 16);
   if (S == 4)
-    k_split_scatter<4><<<g, 256, 0, s>>>(keys, vals, e16, nc, rbits, cpre, base, ent);
+    k_split_scatter<4><<<g, 256, 0, s>>>(keys, vals, e16, de16, nc, rbits, cpre, base, ent);
   else
-    k_split_scatter<2><<<g, 256, 0, s>>>(keys, vals, e16, nc, rbits, cpre, base, ent);
+    k_split_scatter<2><<<g, 256, 0, s>>>(keys, vals, e16, de16, nc, rbits, cpre, base, ent);
   hip_check(hipGetLastError(), "k_split_scatter launch");
 }
 
 void launch_bin_offsets(const uint32_t *sorted_keys, uint32_t e, uint32_t nbins,
-                        uint32_t *bin_off, hipStream_t s, uint32_t kmask) {
-  k_bin_offsets<<<grid_for(static_cast<int64_t>(nbins) + 1), 256, 0, s>>>(sorted_keys, e, nbins,
-                                                                           kmask, bin_off);
+                        uint32_t *bin_off, hipStream_t s, uint32_t kmask, const uint32_t *de) {
+  k_bin_offsets<<<grid_for(static_cast<int64_t>(nbins) + 1), 256, 0, s>>>(sorted_keys, e, de,
+                                                                           nbins, kmask, bin_off);
   hip_check(hipGetLastError(), "k_bin_offsets launch");
 }
 
@@ -973,10 +1042,11 @@ void launch_fill(float *p, float v, int64_t n, hipStream_t s) {
   hip_check(hipGetLastError(), "k_fill launch");
 }
 
-void launch_sum_u64(const uint32_t *v, int64_t n, uint64_t *out, hipStream_t s) {
+void launch_sum_u64(const uint32_t *v, int64_t n, uint64_t *out, hipStream_t s,
+                    const uint32_t *dn) {
   k_fill_u32<<<1, 64, 0, s>>>(reinterpret_cast<uint32_t *>(out), 0u, 2);
   if (n > 0)
-    k_sum_u64<<<grid_for(n, 256, 2048), 256, 0, s>>>(v, n,
+    k_sum_u64<<<grid_for(n, 256, 2048), 256, 0, s>>>(v, n, dn,
                                                      reinterpret_cast<unsigned long long *>(out));
   hip_check(hipGetLastError(), "k_sum_u64 launch");
 }
@@ -994,6 +1064,27 @@ void launch_delay(uint32_t us, hipStream_t s) {
 void launch_fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t s) {
   k_fill_u32<<<grid_for(n), 256, 0, s>>>(p, v, n);
   hip_check(hipGetLastError(), "k_fill_u32 launch");
+}
+
+void launch_job_flags(const uint32_t *mo0, const uint32_t *mo1, int nj, JobSplat *js,
+                      hipStream_t s) {
+  k_job_flags<<<(nj + 63) / 64, 64, 0, s>>>(mo0, mo1, nj, js);
+  hip_check(hipGetLastError(), "k_job_flags launch");
+}
+
+void launch_fill_empty_jobs(const uint32_t *mo0, const uint32_t *mo1, int nj, const JobSplat *js,
+                            hipStream_t s) {
+  if (nj == 0) return;
+  k_fill_empty_jobs<<<dim3(64, nj), 256, 0, s>>>(mo0, mo1, js);
+  hip_check(hipGetLastError(), "k_fill_empty_jobs launch");
+}
+
+void launch_cap_check(uint32_t *m, uint32_t *e, const uint64_t *e64, uint32_t *e16,
+                      uint64_t ecap, uint32_t e16cap, uint32_t bit, uint32_t *ovf,
+                      hipStream_t s) {
+  k_cap_check<<<1, 64, 0, s>>>(m, e, reinterpret_cast<const unsigned long long *>(e64), e16,
+                               ecap, e16cap, bit, ovf);
+  hip_check(hipGetLastError(), "k_cap_check launch");
 }
 
 

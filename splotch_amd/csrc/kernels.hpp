@@ -47,11 +47,7 @@ struct SplatArgs {
   const uint32_t *ubase;    // [nj+1] first unit of each job
   int nj;
   int pass;                 // 0 ordered, 1 additive
-  // single-job scalars (the G > 8 region kernel)
-  float *pic;
-  int T, xres, yres, ncx, ngm;
-  int accumulate;  // 0: write the chain's value, 1: add it, 2: continue from the image
-  float rfac, h2sigma, expfac, taylorlimit, grayabsorb;
+  float rfac, h2sigma, expfac, taylorlimit;
   uint32_t prio_len;  // splat8: lists at least this long run at raised wave priority
   uint64_t *trace;    // splat8: optional per-item {start, end, len | chan << 32, hw id}
   uint32_t heavy_len; // splat8: lists at least this long go to the producer/consumer kernel
@@ -63,30 +59,37 @@ struct SplatArgs {
 
 void launch_project(const ProjectArgs *jobs, const uint32_t *blk, int nj, uint32_t nblocks,
                     Rec *rec, uint8_t *act, hipStream_t s);
-// Compaction of one pass (act == code) into pass order; keys (optional): the
+// Compaction of one pass (act & code) into pass order; keys (optional): the
 // depth-sort key of each job's sort_type.
 void launch_compact(const ProjectArgs *jobs, const uint32_t *blk, int nj, uint32_t nblocks,
                     const uint8_t *act, uint8_t code, const uint32_t *pos, const Rec *rec,
                     uint32_t *keys, uint32_t *vals, hipStream_t s);
+// Device counts: a launcher given an optional `const uint32_t *d*` pointer
+// sizes its grid for the host count (then a capacity) and reads the actual
+// count from that device word (at most the capacity), so that no host wait
+// is needed between the stage that counts and the stage that consumes.
+//
 // mo[j] = pos[pbase[j]] (pass particles before job j), mo[nj] = *total.
 void launch_pass_offsets(const uint32_t *pos, const uint32_t *pbase, int nj, int64_t n,
                          const uint32_t *total, uint32_t *mo, hipStream_t s);
 // keys[i] = job of global particle index vals[i].
 void launch_job_of_index(const uint32_t *vals, uint32_t m, const uint32_t *pbase, int nj,
-                         uint32_t *keys, hipStream_t s);
+                         uint32_t *keys, hipStream_t s, const uint32_t *dm = nullptr);
 // cnt: entries per particle (units of the binned tiles); cnt16 (optional):
 // the regions of 2^rsh x 2^rsh of those 8x8 blocks they cover.  qbad
 // (optional, ordered pass): set to 1 when a particle's q_c = e_c / (e_c +
 // gray_absorption) is not finite.
 void launch_gather_count(const uint32_t *order, uint32_t m, const Rec *rec, Rec *recs,
                          uint32_t *cnt, uint32_t *cnt16, int rsh, const BinGeom *geoms,
-                         const uint32_t *mo, int nj, uint32_t *qbad, hipStream_t s);
+                         const uint32_t *mo, int nj, uint32_t *qbad, hipStream_t s,
+                         const uint32_t *dm = nullptr);
 // rsh == 0: (unit, particle) entries; rsh > 0: (region | unit mask << rbits,
 // particle) entries for regions of 2^rsh x 2^rsh blocks, regions of job j
 // numbered from ubase[j] (then ubase holds the region bases).
 void launch_emit(int rsh, int rbits, const Rec *recs, uint32_t m, const uint32_t *off,
                  uint32_t e, const BinGeom *geoms, const uint32_t *ubase, const uint32_t *mo,
-                 int nj, uint32_t *keys, uint32_t *vals, hipStream_t s);
+                 int nj, uint32_t *keys, uint32_t *vals, hipStream_t s,
+                 const uint32_t *dm = nullptr, const uint32_t *de = nullptr);
 // Block lists from the sorted region lists (keys: region | mask << rbits,
 // vals: particles; S x S blocks per region, S = 2 or 4; roff: region CSR;
 // rbase / ubase: first region / unit of each job):
@@ -98,7 +101,7 @@ void launch_emit(int rsh, int rbits, const Rec *recs, uint32_t m, const uint32_t
 //   launch_split_scatter  the block lists ent, in list order
 uint32_t split_chunks(uint32_t e16);
 void launch_split_chunks(int S, const uint32_t *keys, uint32_t e16, int rbits, uint32_t *ccnt,
-                         hipStream_t s);
+                         hipStream_t s, const uint32_t *de16 = nullptr);
 void launch_split_bounds(int S, const uint32_t *roff, const uint32_t *keys, uint32_t nregions,
                          const uint32_t *cpre, uint32_t e16, int rbits, const uint32_t *rbase,
                          const uint32_t *ubase, const BinGeom *geoms, int nj, uint32_t *sx,
@@ -108,16 +111,29 @@ void launch_split_base(int S, uint32_t nregions, const uint32_t *rbase, const ui
                        uint32_t *base, hipStream_t s);
 void launch_split_scatter(int S, const uint32_t *keys, const uint32_t *vals, uint32_t e16,
                           int rbits, const uint32_t *cpre, const uint32_t *base, uint32_t *ent,
-                          hipStream_t s);
+                          hipStream_t s, const uint32_t *de16 = nullptr);
 // kmask: the bits of a key that hold the bin (the rest ride along unsorted).
 void launch_bin_offsets(const uint32_t *sorted_keys, uint32_t e, uint32_t nbins,
-                        uint32_t *bin_off, hipStream_t s, uint32_t kmask = ~0u);
+                        uint32_t *bin_off, hipStream_t s, uint32_t kmask = ~0u,
+                        const uint32_t *de = nullptr);
 void launch_map(const uint32_t *idx, const uint32_t *table, uint32_t *out, int64_t n,
                 hipStream_t s);
 void launch_fill(float *p, float v, int64_t n, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t v, int64_t n, hipStream_t s);
 // *out (device, 8 bytes) = sum of v[0..n) in 64 bits.
-void launch_sum_u64(const uint32_t *v, int64_t n, uint64_t *out, hipStream_t s);
+void launch_sum_u64(const uint32_t *v, int64_t n, uint64_t *out, hipStream_t s,
+                    const uint32_t *dn = nullptr);
+// Device-count pipeline helpers: per-job pass flags from the pass offsets
+// (null: pass absent), zeros into the images of jobs with no particle in
+// either pass, and the capacity check of a pass (counts zeroed, bit set in
+// *ovf when e64 > ecap or *e16 > e16cap).
+void launch_job_flags(const uint32_t *mo0, const uint32_t *mo1, int nj, JobSplat *js,
+                      hipStream_t s);
+void launch_fill_empty_jobs(const uint32_t *mo0, const uint32_t *mo1, int nj, const JobSplat *js,
+                            hipStream_t s);
+void launch_cap_check(uint32_t *m, uint32_t *e, const uint64_t *e64, uint32_t *e16,
+                      uint64_t ecap, uint32_t e16cap, uint32_t bit, uint32_t *ovf,
+                      hipStream_t s);
 // pic[i] += add[i]
 void launch_add_images(float *pic, const float *add, int64_t n, hipStream_t s);
 // A one-wave kernel that returns after `us` microseconds (bounded wait).
@@ -137,12 +153,24 @@ void launch_items_count(const uint32_t *bin_off, uint32_t nunits, const SplatArg
 void launch_items_emit(const uint32_t *bin_off, uint32_t nunits, const SplatArgs &a, int ordered,
                        uint32_t split, const uint32_t *off, uint32_t *keys, uint32_t *vals,
                        hipStream_t s);
+// Device item counts of a pass (all null: the host counts are exact).
+struct ItemCounts {
+  const uint32_t *items = nullptr;  // work items of the pass
+  const uint32_t *heavy = nullptr;  // of which lists that reach a.heavy_len (sorted first)
+  uint32_t heavy_cap = 0;           // at most this many go to the producer/consumer kernel
+};
+// With dc set, launch_splat8 is given the pass's whole item array and a grid
+// for nitems (a capacity) and skips the first min(*dc.heavy, heavy_cap) items;
+// launch_splat_pc is given npc = heavy_cap.  ppw (variant 4): particles per
+// chunk of the walk, 1, 2, 4 or 8 (0: 8 for one channel, 8 additive / 4
+// ordered for all three).
 void launch_splat8(bool additive, int variant, const uint32_t *items, uint32_t nitems,
-                   const SplatArgs &a, hipStream_t s);
+                   const SplatArgs &a, hipStream_t s, const ItemCounts &dc = ItemCounts{},
+                   int ppw = 0);
 // Producer/consumer workgroups (splat8.hip), one per item: the first npc
 // items, the units whose lists reach a.heavy_len.
 void launch_splat_pc(bool additive, const uint32_t *items, uint32_t npc, const SplatArgs &a,
-                     hipStream_t s);
+                     hipStream_t s, const ItemCounts &dc = ItemCounts{});
 // Differentiable form (adjoint.hip): screen-space gradient accumulation over
 // the forward's work items (gscr: n x 6 doubles, zeroed by the caller), the
 // chain to the particle fields of job 0, and an FP64 fill.

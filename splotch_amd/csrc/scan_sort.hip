@@ -28,10 +28,10 @@ struct LoadU32 {
   __device__ __forceinline__ uint32_t operator()(int64_t i) const { return p[i]; }
 };
 
-struct LoadU8Eq {
+struct LoadU8Bit {
   const uint8_t *p;
   uint8_t v;
-  __device__ __forceinline__ uint32_t operator()(int64_t i) const { return p[i] == v ? 1u : 0u; }
+  __device__ __forceinline__ uint32_t operator()(int64_t i) const { return (p[i] & v) ? 1u : 0u; }
 };
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
@@ -63,9 +63,17 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *s_w) {
   return woff + inc - v;
 }
 
+// The element count of a launch sized for `cap` elements: cap, or the device
+// count *dn when it is given (at most cap).
+__device__ __forceinline__ int64_t count_of(int64_t cap, const uint32_t *dn) {
+  return dn ? min(cap, static_cast<int64_t>(*dn)) : cap;
+}
+
 template <class In>
-__global__ __launch_bounds__(256) void k_scan_reduce(In in, int64_t n, uint32_t *partial) {
+__global__ __launch_bounds__(256) void k_scan_reduce(In in, int64_t cap, const uint32_t *dn,
+                                                     uint32_t *partial) {
   __shared__ uint32_t s_w[4];
+  const int64_t n = count_of(cap, dn);
   const int64_t base = static_cast<int64_t>(blockIdx.x) * kTile;
   uint32_t s = 0;
 #pragma unroll
@@ -80,9 +88,11 @@ __global__ __launch_bounds__(256) void k_scan_reduce(In in, int64_t n, uint32_t 
 }
 
 template <class In>
-__global__ __launch_bounds__(256) void k_scan_apply(In in, uint32_t *out, int64_t n,
-                                                    const uint32_t *partial, uint32_t *total) {
+__global__ __launch_bounds__(256) void k_scan_apply(In in, uint32_t *out, int64_t cap,
+                                                    const uint32_t *dn, const uint32_t *partial,
+                                                    uint32_t *total) {
   __shared__ uint32_t s_w[4];
+  const int64_t n = count_of(cap, dn);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t base = static_cast<int64_t>(blockIdx.x) * kTile + threadIdx.x * kItems;
   uint32_t v[kItems];
@@ -118,28 +128,32 @@ __global__ __launch_bounds__(256) void k_scan_apply(In in, uint32_t *out, int64_
 
 __global__ void k_set_u32(uint32_t *p, uint32_t v) { *p = v; }
 
+// n: the element count, or with dn the capacity the launches are sized for
+// (the count itself then read from *dn on the device).
 template <class In>
 void scan_impl(In in, uint32_t *out, int64_t n, uint32_t *total, Workspace &ws, hipStream_t s,
-               int level) {
+               int level, const uint32_t *dn) {
   if (n <= 0) {
     if (total) k_set_u32<<<1, 1, 0, s>>>(total, 0u);
     return;
   }
   const int64_t nb = (n + kTile - 1) / kTile;
   if (nb == 1) {
-    k_scan_apply<In><<<1, 256, 0, s>>>(in, out, n, nullptr, total);
+    k_scan_apply<In><<<1, 256, 0, s>>>(in, out, n, dn, nullptr, total);
     return;
   }
   if (level >= 4) throw HipError{hipErrorInvalidValue, "scan too deep"};
   uint32_t *partial = ws.get_as<uint32_t>(S_SCAN0 + level, static_cast<size_t>(nb));
-  k_scan_reduce<In><<<static_cast<unsigned>(nb), 256, 0, s>>>(in, n, partial);
-  scan_impl(LoadU32{partial}, partial, nb, nullptr, ws, s, level + 1);
-  k_scan_apply<In><<<static_cast<unsigned>(nb), 256, 0, s>>>(in, out, n, partial, total);
+  k_scan_reduce<In><<<static_cast<unsigned>(nb), 256, 0, s>>>(in, n, dn, partial);
+  scan_impl(LoadU32{partial}, partial, nb, nullptr, ws, s, level + 1, nullptr);
+  k_scan_apply<In><<<static_cast<unsigned>(nb), 256, 0, s>>>(in, out, n, dn, partial, total);
 }
 
 template <int BITS>
-__global__ __launch_bounds__(256) void k_rs_hist(const uint32_t *keys, int64_t n, int shift,
-                                                 uint32_t *hist, int64_t nblocks) {
+__global__ __launch_bounds__(256) void k_rs_hist(const uint32_t *keys, int64_t cap,
+                                                 const uint32_t *dn, int shift, uint32_t *hist,
+                                                 int64_t nblocks) {
+  const int64_t n = count_of(cap, dn);
   constexpr int R = 1 << BITS;
   constexpr uint32_t M = R - 1;
   __shared__ uint32_t h[4][R];
@@ -159,9 +173,10 @@ __global__ __launch_bounds__(256) void k_rs_hist(const uint32_t *keys, int64_t n
 
 template <int BITS>
 __global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t *kin, const uint32_t *vin,
-                                                    uint32_t *kout, uint32_t *vout, int64_t n,
-                                                    int shift, const uint32_t *hist,
-                                                    int64_t nblocks) {
+                                                    uint32_t *kout, uint32_t *vout, int64_t cap,
+                                                    const uint32_t *dn, int shift,
+                                                    const uint32_t *hist, int64_t nblocks) {
+  const int64_t n = count_of(cap, dn);
   constexpr int R = 1 << BITS;
   constexpr uint32_t M = R - 1;
   constexpr int DPT = (R + 255) / 256;  // digits per thread in the block scan
@@ -253,30 +268,32 @@ __global__ __launch_bounds__(256) void k_rs_scatter(const uint32_t *kin, const u
 
 template <int BITS>
 void rs_pass(const uint32_t *sk, const uint32_t *sv, uint32_t *dk, uint32_t *dv, int64_t n,
-             int shift, int64_t nb, uint32_t *hist, Workspace &ws, hipStream_t s) {
-  k_rs_hist<BITS><<<static_cast<unsigned>(nb), 256, 0, s>>>(sk, n, shift, hist, nb);
+             const uint32_t *dn, int shift, int64_t nb, uint32_t *hist, Workspace &ws,
+             hipStream_t s) {
+  k_rs_hist<BITS><<<static_cast<unsigned>(nb), 256, 0, s>>>(sk, n, dn, shift, hist, nb);
   scan_u32(hist, hist, (int64_t(1) << BITS) * nb, nullptr, ws, s, 0);
-  k_rs_scatter<BITS><<<static_cast<unsigned>(nb), 256, 0, s>>>(sk, sv, dk, dv, n, shift, hist,
-                                                               nb);
+  k_rs_scatter<BITS><<<static_cast<unsigned>(nb), 256, 0, s>>>(sk, sv, dk, dv, n, dn, shift,
+                                                               hist, nb);
   hip_check(hipGetLastError(), "radix sort launch");
 }
 
 }  // namespace
 
 void scan_u32(const uint32_t *in, uint32_t *out, int64_t n, uint32_t *total, Workspace &ws,
-              hipStream_t s, int level) {
-  scan_impl(LoadU32{in}, out, n, total, ws, s, level);
+              hipStream_t s, int level, const uint32_t *dn) {
+  scan_impl(LoadU32{in}, out, n, total, ws, s, level, dn);
   hip_check(hipGetLastError(), "scan_u32 launch");
 }
 
-void scan_u8eq(const uint8_t *flags, uint8_t value, uint32_t *out, int64_t n, uint32_t *total,
-               Workspace &ws, hipStream_t s) {
-  scan_impl(LoadU8Eq{flags, value}, out, n, total, ws, s, 0);
-  hip_check(hipGetLastError(), "scan_u8eq launch");
+void scan_u8bit(const uint8_t *flags, uint8_t bits, uint32_t *out, int64_t n, uint32_t *total,
+                Workspace &ws, hipStream_t s) {
+  scan_impl(LoadU8Bit{flags, bits}, out, n, total, ws, s, 0, nullptr);
+  hip_check(hipGetLastError(), "scan_u8bit launch");
 }
 
 int radix_sort_pairs(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, int64_t n,
-                     int begin_bit, int end_bit, Workspace &ws, hipStream_t s) {
+                     int begin_bit, int end_bit, Workspace &ws, hipStream_t s,
+                     const uint32_t *dn) {
   if (n <= 1 || end_bit <= begin_bit) return 0;
   const int64_t nb = (n + kTile - 1) / kTile;
   const int bits = end_bit - begin_bit;
@@ -291,15 +308,15 @@ int radix_sort_pairs(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, int
     uint32_t *dk = cur ? k0 : k1;
     uint32_t *dv = cur ? v0 : v1;
     switch (w) {
-      case 1: rs_pass<1>(sk, sv, dk, dv, n, shift, nb, hist, ws, s); break;
-      case 2: rs_pass<2>(sk, sv, dk, dv, n, shift, nb, hist, ws, s); break;
-      case 3: rs_pass<3>(sk, sv, dk, dv, n, shift, nb, hist, ws, s); break;
-      case 4: rs_pass<4>(sk, sv, dk, dv, n, shift, nb, hist, ws, s); break;
-      case 5: rs_pass<5>(sk, sv, dk, dv, n, shift, nb, hist, ws, s); break;
-      case 6: rs_pass<6>(sk, sv, dk, dv, n, shift, nb, hist, ws, s); break;
-      case 7: rs_pass<7>(sk, sv, dk, dv, n, shift, nb, hist, ws, s); break;
-      case 8: rs_pass<8>(sk, sv, dk, dv, n, shift, nb, hist, ws, s); break;
-      default: rs_pass<9>(sk, sv, dk, dv, n, shift, nb, hist, ws, s); break;
+      case 1: rs_pass<1>(sk, sv, dk, dv, n, dn, shift, nb, hist, ws, s); break;
+      case 2: rs_pass<2>(sk, sv, dk, dv, n, dn, shift, nb, hist, ws, s); break;
+      case 3: rs_pass<3>(sk, sv, dk, dv, n, dn, shift, nb, hist, ws, s); break;
+      case 4: rs_pass<4>(sk, sv, dk, dv, n, dn, shift, nb, hist, ws, s); break;
+      case 5: rs_pass<5>(sk, sv, dk, dv, n, dn, shift, nb, hist, ws, s); break;
+      case 6: rs_pass<6>(sk, sv, dk, dv, n, dn, shift, nb, hist, ws, s); break;
+      case 7: rs_pass<7>(sk, sv, dk, dv, n, dn, shift, nb, hist, ws, s); break;
+      case 8: rs_pass<8>(sk, sv, dk, dv, n, dn, shift, nb, hist, ws, s); break;
+      default: rs_pass<9>(sk, sv, dk, dv, n, dn, shift, nb, hist, ws, s); break;
     }
     cur ^= 1;
   }

@@ -321,15 +321,32 @@ __device__ __forceinline__ void walk_batch(int nb, int cx, int cy, const uint32_
   }
 }
 
-template <int ADD, int VAR>
+// Work items of a launch: items[0, n), or with device counts (dc: the pass's
+// item total and long-list total) the items after the first min(dc[1], cap)
+// (those go to the producer/consumer kernel).
+__device__ __forceinline__ uint32_t splat_items(const uint32_t *&items, uint32_t n,
+                                                const ItemCounts &dc) {
+  if (!dc.items) return n;
+  const uint32_t tot = *dc.items, nh = min(min(*dc.heavy, dc.heavy_cap), tot);
+  items += nh;
+  return tot - nh;
+}
+
+// PPW (variant 4): particles per chunk of the walk (1, 2, 4 or 8; 0: 8 for
+// one channel, 8 additive / 4 ordered for all three).
+template <int ADD, int VAR, int PPW>
 __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *items,
-                                                uint32_t nitems) {
+                                                uint32_t nitems_, ItemCounts dc) {
+  constexpr int J1 = PPW ? PPW : 8, J3 = PPW ? PPW : (ADD ? 8 : 4);
+  const uint32_t nitems = splat_items(items, nitems_, dc);
   __shared__ float s_tab[2048];
   __shared__ uint32_t s_col[4][8][kPad];
   __shared__ float s_pre2[4][8][kPad];
   __shared__ float s_pre1[4][8][kPad];
   __shared__ float4 s_cst[ADD ? 1 : 4][ADD ? 1 : 64][2];  // ordered: per-particle constants
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // a workgroup past the items (a grid sized for a capacity) leaves at once
+  if (blockIdx.x * 4u >= nitems) return;
   for (int i = tid; i < 2048; i += 256) s_tab[i] = a.exptab[i];
   __syncthreads();
   const uint32_t it = blockIdx.x * 4u + wave;
@@ -445,10 +462,10 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
     // Phase B: lane = pixel, particles in list order.
     if (VAR == 4) {
       switch (chan) {
-        case 0: walk_chunks<ADD, 0, 8>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, wcst, fastmask, expfac, tl, l0, l1, l2); break;
-        case 1: walk_chunks<ADD, 1, 8>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, wcst, fastmask, expfac, tl, l0, l1, l2); break;
-        case 2: walk_chunks<ADD, 2, 8>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, wcst, fastmask, expfac, tl, l0, l1, l2); break;
-        default: walk_chunks<ADD, 3, (ADD ? 8 : 4)>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, wcst, fastmask, expfac, tl, l0, l1, l2); break;
+        case 0: walk_chunks<ADD, 0, J1>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, wcst, fastmask, expfac, tl, l0, l1, l2); break;
+        case 1: walk_chunks<ADD, 1, J1>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, wcst, fastmask, expfac, tl, l0, l1, l2); break;
+        case 2: walk_chunks<ADD, 2, J1>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, wcst, fastmask, expfac, tl, l0, l1, l2); break;
+        default: walk_chunks<ADD, 3, J3>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, wcst, fastmask, expfac, tl, l0, l1, l2); break;
       }
       wave_sync();
       continue;
@@ -621,7 +638,11 @@ __device__ __forceinline__ void lds_barrier() {
 
 template <int ADD>
 __global__ __launch_bounds__(kPThreads) void k_splat_pc(SplatArgs a, const uint32_t *items,
-                                                        uint32_t npc) {
+                                                        uint32_t npc, ItemCounts dc) {
+  // device counts: the first min(long-list total, cap) items; the rest of the
+  // grid (sized for the cap) has no work (uniform per workgroup, before any
+  // barrier)
+  if (dc.items && blockIdx.x >= min(min(*dc.heavy, dc.heavy_cap), *dc.items)) return;
   // One LDS block with the exp table at address 0, so that a table entry's
   // address is its byte offset (no base to add per lookup).
   struct Lds {
@@ -1043,30 +1064,34 @@ void launch_items_emit(const uint32_t *bin_off, uint32_t nunits, const SplatArgs
 }
 
 void launch_splat_pc(bool additive, const uint32_t *items, uint32_t npc, const SplatArgs &a,
-                     hipStream_t s) {
+                     hipStream_t s, const ItemCounts &dc) {
   if (npc == 0) return;
-  if (additive) k_splat_pc<1><<<npc, kPThreads, 0, s>>>(a, items, npc);
-  else k_splat_pc<0><<<npc, kPThreads, 0, s>>>(a, items, npc);
+  if (additive) k_splat_pc<1><<<npc, kPThreads, 0, s>>>(a, items, npc, dc);
+  else k_splat_pc<0><<<npc, kPThreads, 0, s>>>(a, items, npc, dc);
   hip_check(hipGetLastError(), "k_splat_pc launch");
 }
 
 void launch_splat8(bool additive, int variant, const uint32_t *items, uint32_t nitems,
-                   const SplatArgs &a, hipStream_t s) {
+                   const SplatArgs &a, hipStream_t s, const ItemCounts &dc, int ppw) {
   if (nitems == 0) return;
   const unsigned grid = (nitems + 3) / 4;
+#define SPLAT8(ADDV, VARV, PPWV) k_splat8<ADDV, VARV, PPWV><<<grid, 256, 0, s>>>(a, items, nitems, dc)
   if (variant == 4) {
-    if (additive) k_splat8<1, 4><<<grid, 256, 0, s>>>(a, items, nitems);
-    else k_splat8<0, 4><<<grid, 256, 0, s>>>(a, items, nitems);
+    switch (ppw) {
+      case 1: if (additive) SPLAT8(1, 4, 1); else SPLAT8(0, 4, 1); break;
+      case 2: if (additive) SPLAT8(1, 4, 2); else SPLAT8(0, 4, 2); break;
+      case 4: if (additive) SPLAT8(1, 4, 4); else SPLAT8(0, 4, 4); break;
+      case 8: if (additive) SPLAT8(1, 4, 8); else SPLAT8(0, 4, 8); break;
+      default: if (additive) SPLAT8(1, 4, 0); else SPLAT8(0, 4, 0); break;
+    }
   } else if (variant == 3) {
-    if (additive) k_splat8<1, 3><<<grid, 256, 0, s>>>(a, items, nitems);
-    else k_splat8<0, 3><<<grid, 256, 0, s>>>(a, items, nitems);
+    if (additive) SPLAT8(1, 3, 0); else SPLAT8(0, 3, 0);
   } else if (variant == 2) {
-    if (additive) k_splat8<1, 2><<<grid, 256, 0, s>>>(a, items, nitems);
-    else k_splat8<0, 2><<<grid, 256, 0, s>>>(a, items, nitems);
+    if (additive) SPLAT8(1, 2, 0); else SPLAT8(0, 2, 0);
   } else {
-    if (additive) k_splat8<1, 1><<<grid, 256, 0, s>>>(a, items, nitems);
-    else k_splat8<0, 1><<<grid, 256, 0, s>>>(a, items, nitems);
+    if (additive) SPLAT8(1, 1, 0); else SPLAT8(0, 1, 0);
   }
+#undef SPLAT8
   hip_check(hipGetLastError(), "k_splat8 launch");
 }
 

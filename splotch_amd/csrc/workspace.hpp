@@ -41,8 +41,13 @@ inline void hip_check(hipError_t e, const char *what) {
 class Workspace {
  public:
   ~Workspace() { release(); }
+  // A frozen workspace never grows: a request past a slot's capacity throws
+  // (no allocation or device synchronisation, e.g. under stream capture).
+  void freeze(bool f) { frozen_ = f; }
   void *get(int slot, size_t bytes) {
     if (bytes == 0) bytes = 256;
+    if (cap_[slot] < bytes && frozen_)
+      throw HipError{hipErrorOutOfMemory, "request past the reserved workspace"};
     if (cap_[slot] < bytes) {
       if (ptr_[slot]) {
         hip_check(hipDeviceSynchronize(), "sync before workspace growth");
@@ -76,6 +81,7 @@ class Workspace {
  private:
   void *ptr_[S_NSLOTS] = {};
   size_t cap_[S_NSLOTS] = {};
+  bool frozen_ = false;
 };
 
 }  // namespace splhip

@@ -210,8 +210,10 @@ def batch_jobs(n_jobs: int = 64, seed: int = 1200, n_scale: float = 1.0,
 def large_share(n: int = 200_000_000, seed: int = 1400) -> Field:
     """One GPU's share of a large simulation: n particles, half in 5000 Plummer
     halos and half uniform background, hsml log-normal (mu = ln 0.2, sigma =
-    1.2).  The halo half is the gas (type 0, depth-sorted emission-absorption)
-    and the background half the additive type 1."""
+    1.2).  The halo half is the gas (type 0) and the background half type 1.
+    The frame (LARGE_SHARE_MODES) is an additive pass over all particles plus
+    a depth-sorted emission-absorption pass over the gas: type 0 takes part in
+    both passes (a_eq_e 2), type 1 in the additive one."""
     f = plummer_field(n, seed, halo_frac=0.5, n_halos=5000, hsml_mu=math.log(0.2),
                       hsml_sigma=1.2)
     n_halo = int(round(n * 0.5))
@@ -219,3 +221,19 @@ def large_share(n: int = 200_000_000, seed: int = 1400) -> Field:
     typ[:n_halo] = 0
     f.type = typ
     return f
+
+
+# a_eq_e per type of the large-share frame: gas in both passes, the rest additive
+LARGE_SHARE_MODES = [2, 1]
+
+
+def perturbed(field: Field, seed: int = 1302, sigma: float = 0.05) -> Field:
+    """The differentiable workload's target field: positions + N(0, sigma) per
+    axis, intensities * U(0.8, 1.2); hsml and colour unchanged."""
+    rng = np.random.default_rng(seed)
+    n = field.n
+    x, y, z = (getattr(field, k) + rng.normal(0, sigma, n).astype(np.float32)
+               for k in ("x", "y", "z"))
+    inten = (field.inten * rng.uniform(0.8, 1.2, n)).astype(np.float32)
+    return Field(np.ascontiguousarray(x, np.float32), np.ascontiguousarray(y, np.float32),
+                 np.ascontiguousarray(z, np.float32), field.hsml, inten, field.col, field.type)

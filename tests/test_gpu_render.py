@@ -492,3 +492,32 @@ def test_chunked_passes_bitexact(R, chunk, heavy, hmax):
         _, counts = R.stage_times()
         assert max(counts["ordered_entries"], counts["additive_entries"]) > chunk
         assert_bitexact(got, want, "chunked %s" % (aeq,))
+
+
+def test_type_in_both_passes(R):
+    """A type with a_eq_e 2 takes part in both passes (the large-share frame:
+    additive over all particles plus the ordered gas pass): the image is the
+    oracle's P_ordered (gas, stable depth order) + P_additive (gas and the
+    rest), bit for bit, through the 8x8 splat and the producer/consumer
+    kernel."""
+    import splotch_amd as S
+    log("case both passes")
+    gas = synth.plummer_field(40_000, 1410, ptype=0, hsml_mu=math.log(0.6))
+    rest = synth.plummer_field(40_000, 1411, ptype=1, hsml_mu=math.log(0.3))
+    fld = synth.concat(gas, rest)
+    v = synth.default_view(224, 192)
+    pal = synth.default_palette()
+    p = O.project_colorize(fld, v, [pal, pal], brightness=[0.5, 2.0])
+    g = p.take(np.arange(p.n))
+    g.active = (p.active * (p.type == 0)).astype(np.uint8)
+    g = g.take(O.sort_perm(g, 1, stable=True))
+    want = (O.render(g, v.xres, v.yres, False, 0.2, 50) +
+            O.render(p, v.xres, v.yres, True, 0.2, 50)).astype(np.float32)
+    for heavy, hmax in ((0, 0), (1, 1 << 30)):
+        pic = render(R, fld, v, [pal, pal], S.params(ptypes=2, a_eq_e=[2, 1],
+                                                      brightness=[0.5, 2.0], hip_heavy=heavy,
+                                                      hip_heavy_max=hmax))
+        assert_bitexact(pic, want, "type 0 in both passes, heavy %d" % heavy)
+    _, counts = R.stage_times()
+    assert counts["ordered_active"] == int(g.active.sum())
+    assert counts["additive_active"] == int(p.active.sum())
