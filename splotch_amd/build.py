@@ -1,5 +1,5 @@
 """Build the gfx950 HIP library in-tree: splotch_amd/libsplotch_hip.so, and the
-LDS bandwidth microbenchmark used by the bench: splotch_amd/liblds_peak.so.
+memory microbenchmarks the bench measures peaks with: splotch_amd/libmembench.so.
 
     python -m splotch_amd.build        (or __graft_entry__.build())
 
@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libsplotch_hip.so")
-LDS_LIB = os.path.join(HERE, "liblds_peak.so")
+BENCH_LIB = os.path.join(HERE, "libmembench.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 SOURCES = ["scan_sort.hip", "kernels.hip", "splat8.hip", "adjoint.hip", "post.hip",
            "normalize.hip", "ingest.hip", "capi.hip"]
@@ -48,8 +48,8 @@ def _link(objs, lib):
 def build(verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
     with cf.ThreadPoolExecutor(len(SOURCES) + 1) as ex:
-        objs = list(ex.map(_compile, SOURCES + ["lds_peak.hip"]))
-    _link(objs[-1:], LDS_LIB)
+        objs = list(ex.map(_compile, SOURCES + ["membench.hip"]))
+    _link(objs[-1:], BENCH_LIB)
     objs = objs[:-1]
     _link(objs, LIB)
     if verbose:

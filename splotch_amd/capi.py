@@ -372,11 +372,12 @@ class Renderer:
 
     def tuning(self):
         """The splat tuning the last render used: dict variant, split,
-        heavy_max, and whether it came from the autotuner."""
-        v = np.zeros(4, np.int64)
+        heavy_max, whether it came from the autotuner, the work region
+        (region, px) and particles per wave (ppw, 0 = auto)."""
+        v = np.zeros(6, np.int64)
         self._check(lib().splhip_last_tuning(self._c, _np(v)), "splhip_last_tuning")
         return {"variant": int(v[0]), "split": int(v[1]), "heavy_max": int(v[2]),
-                "autotuned": bool(v[3])}
+                "autotuned": bool(v[3]), "region": int(v[4]), "ppw": int(v[5])}
 
     def set_trace(self, on: bool = True):
         self._check(lib().splhip_set_trace(self._c, C.c_int32(int(on))), "set_trace")

@@ -99,10 +99,13 @@ def test_batched_vs_oracle(R):
         assert_bitexact(got, opic, "job %d vs oracle" % j)
 
 
-def test_batched_mixed_edge_views(R):
+@pytest.mark.parametrize("region", [0, 16, 64])
+def test_batched_mixed_edge_views(R, region):
     """Tile sizes, both particle kinds in one view, an empty view, a view whose
-    particles are all behind the camera, and a view repeated twice."""
+    particles are all behind the camera, and a view repeated twice, with 8 px
+    work regions and larger ones (the region splat looks up each unit's view)."""
     import splotch_amd as S
+    log("case edge views, hip_region", region)
     base = _small_batch(seed=2400, n_jobs=3)
     gas = synth.plummer_field(8000, 2401, ptype=0)
     stars = synth.plummer_field(8000, 2402, hsml_fixed=0.05, ptype=1)
@@ -114,7 +117,7 @@ def test_batched_mixed_edge_views(R):
                             synth.View((50.0, 50.0, 300.0), (50.0, 50.0, 400.0), (0.0, 1.0, 0.0),
                                        128, 100, 30.0), False)
     jobs = base + [mixed, empty, behind, base[1]]
-    views = _views(jobs)
+    views = _views(jobs, hip_region=region)
     # per-view tile sizes (a render key of each view)
     for (dev, cam, prm, pic), T in zip(views, [50, 16, 37, 64, 50, 8, 50]):
         prm.tile_size = T
@@ -157,15 +160,20 @@ def test_batched_region_binning(R):
     assert_bitexact(got[0].reshape(opic.shape), opic, "region job 0 vs oracle")
 
 
-@pytest.mark.parametrize("variant,split,heavy,hmax", [(4, 0, 0, 0), (1, 0, -1, 0), (3, 64, 200, 0),
-                                                      (4, 64, 1, 0), (4, 0, 300, 1 << 30)])
-def test_batched_tuning_invariant(R, variant, split, heavy, hmax):
-    """The splat tuning keys change the GPU work decomposition only."""
+@pytest.mark.parametrize("variant,split,heavy,hmax,ppw,region",
+                         [(4, 0, 0, 0, 0, 0), (1, 0, -1, 0, 0, 0), (3, 64, 200, 0, 0, 0),
+                          (4, 64, 1, 0, 0, 0), (4, 0, 300, 1 << 30, 0, 0), (4, 64, -1, 0, 1, 0),
+                          (4, 0, -1, 0, 2, 0), (4, 64, 1, 0, 4, 0), (4, 0, -1, 0, 8, 0),
+                          (4, 0, 0, 0, 0, 16), (4, 0, 0, 0, 0, 32), (4, 0, 0, 0, 0, 64)])
+def test_batched_tuning_invariant(R, variant, split, heavy, hmax, ppw, region):
+    """The tuning keys (the 8x8 splat's decomposition, particles per wave,
+    and the work region of 8-64 px) change the GPU work decomposition only."""
+    log("case batched tuning", variant, split, heavy, hmax, ppw, region)
     jobs = _small_batch(seed=2500, n_jobs=6)
     views0 = _views(jobs)
     R.render_batched(views0)
     views1 = _views(jobs, hip_variant=variant, hip_split=split, hip_heavy=heavy,
-                    hip_heavy_max=hmax)
+                    hip_heavy_max=hmax, hip_ppw=ppw, hip_region=region)
     R.render_batched(views1)
     torch.cuda.synchronize()
     for j, (a, b) in enumerate(zip(views0, views1)):
@@ -175,28 +183,38 @@ def test_batched_tuning_invariant(R, variant, split, heavy, hmax):
 def test_batched_errors(R):
     import splotch_amd as S
     jobs = _small_batch(seed=2600, n_jobs=2)
-    views = _views(jobs, hip_region=16)
+    views = _views(jobs, hip_region=24)
     with pytest.raises(S.SplhipError):
         R.render_batched(views)
+    with pytest.raises(S.SplhipError):
+        R.render_batched(_views(jobs, hip_ppw=3))
     R.render_batched([])  # no views: nothing to do
 
 
 def test_batched_autotune(R):
-    """hip_autotune times the candidate splat tunings on the first render of a
-    batch shape, then reuses the choice; images are those of the default."""
+    """hip_autotune times the work regions (8-64 px) and the 8x8 splat's
+    decomposition (producer/consumer cap, channel split, inner loop and
+    particles per wave) on the first render of a batch bucket, then reuses
+    the choice, also for another batch of the same bucket; images are those
+    of the default."""
     jobs = _small_batch(seed=2700, n_jobs=6)
     ref = _views(jobs)
     R.render_batched(ref)
     first = R.tuning()
     assert not first["autotuned"]
+    chosen = None
     for call in range(2):
         views = _views(jobs, hip_autotune=1)
         R.render_batched(views)
         t = R.tuning()
         log("autotune call %d chose" % call, t)
         assert t["autotuned"]
-        assert t["variant"] in (3, 4) and t["split"] in (4096, 8192, 32768)
-        assert t["heavy_max"] in (0, 96, 192, 256)
+        assert t["region"] in (8, 16, 32, 64)
+        if t["region"] == 8:
+            assert t["variant"] in (3, 4) and t["split"] in (4096, 8192, 32768)
+            assert t["heavy_max"] in (0, 96, 192, 256) and t["ppw"] in (0, 1, 2, 4, 8)
+        assert chosen is None or t == chosen
+        chosen = t
         torch.cuda.synchronize()
         for j, (a, b) in enumerate(zip(views, ref)):
             assert_bitexact(a[3].cpu().numpy(), b[3].cpu().numpy(), "autotuned job %d" % j)
