@@ -11,15 +11,19 @@
 // (the limit of both branches of splotch_host.cc:648-658).  Pairs outside the
 // footprint contribute nothing whatever the parameters.
 //
-// k_adjoint8: one wave per work item of the forward render (an 8x8 block of a
-// reference tile, all channels or one), lane = pixel, particles in render
-// order.  Ordered mode sweeps each list twice: the first sweep sums the
-// optical depth tau_c = sum_k e_c,k att_k per pixel, the second carries the
-// composited value P_{k-1} and the prefix depth, from which the transmittance
-// behind particle k, S_k = exp(-(tau - prefix_k)), follows without division.
+// k_adjoint_add: additive mode, one wave per list segment of an 8x8 block,
+// lane = particle: a particle's gradient is a sum over its own footprint
+// pixels.  k_adjoint_ord: ordered mode, one wave per list segment, lane =
+// pixel, particles in render order, in passes: per segment the composite map
+// and optical depth of its particles (A), the segments joined per pixel
+// (k_adjoint_join, B), then the gradients (C), which carry the composited
+// value P_{k-1} and the prefix depth, from which the transmittance behind
+// particle k, S_k = exp(-(tau - prefix_k)), follows without division.
 // Per particle, the six screen-space partials d/d(sx, sy, r, e0, e1, e2) are
-// summed over the block's 64 pixels and added (FP64 atomics) to the
-// particle's accumulator.
+// summed over the block's 64 pixels and stored per list entry (float); the
+// entries of each particle are then gathered in a fixed order (a stable sort
+// of the entries by particle) and summed in FP64, so the gradient is the same
+// bit for bit on every run.
 //
 // k_chain: per particle, d(screen state)/d(x, y, z, hsml, I, colour) of
 // particle_project + particle_colorize (splotch_host.cc:111-178, 252-287),
@@ -66,8 +70,7 @@ __device__ __forceinline__ float wave_sum(float v) {
 // cross-lane reduction.
 __global__ __launch_bounds__(256) void k_adjoint_add(SplatArgs a, const uint32_t *seg_unit,
                                                      const uint32_t *seg_first, uint32_t nseg,
-                                                     const float *grad, const uint32_t *order,
-                                                     double *gscr) {
+                                                     const float *grad, float *part) {
   __shared__ float s_g[4][3][64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t sg = blockIdx.x * 4u + wave;
@@ -134,15 +137,13 @@ __global__ __launch_bounds__(256) void k_adjoint_add(SplatArgs a, const uint32_t
         ge2 -= g2 * att;
       }
     }
-    if (gsx != 0.f || gsy != 0.f || gr != 0.f || ge0 != 0.f || ge1 != 0.f || ge2 != 0.f) {
-      double *g = gscr + 6 * static_cast<size_t>(order[s]);
-      unsafeAtomicAdd(g + 0, static_cast<double>(gsx));
-      unsafeAtomicAdd(g + 1, static_cast<double>(gsy));
-      unsafeAtomicAdd(g + 2, static_cast<double>(gr));
-      if (chan == 3 || chan == 0) unsafeAtomicAdd(g + 3, static_cast<double>(ge0));
-      if (chan == 3 || chan == 1) unsafeAtomicAdd(g + 4, static_cast<double>(ge1));
-      if (chan == 3 || chan == 2) unsafeAtomicAdd(g + 5, static_cast<double>(ge2));
-    }
+    float *pp = part + 6 * static_cast<size_t>(base + lane);  // this entry's partials
+    pp[0] = gsx;
+    pp[1] = gsy;
+    pp[2] = gr;
+    pp[3] = ge0;
+    pp[4] = ge1;
+    pp[5] = ge2;
   }
 }
 
@@ -189,8 +190,7 @@ __global__ __launch_bounds__(256) void k_seg_emit(const uint32_t *bin_off, uint3
 template <int PASS>
 __global__ __launch_bounds__(256) void k_adjoint_ord(SplatArgs a, const uint32_t *seg_unit,
                                                      const uint32_t *seg_first, uint32_t nseg,
-                                                     const float *grad, const uint32_t *order,
-                                                     double *gscr, SegView sv) {
+                                                     const float *grad, float *part, SegView sv) {
   __shared__ uint32_t s_col[4][8][kPadA];
   __shared__ float s_ex[4][8][kPadA];
   __shared__ float s_ey[4][8][kPadA];
@@ -221,17 +221,14 @@ __global__ __launch_bounds__(256) void k_adjoint_ord(SplatArgs a, const uint32_t
 
   // per-lane particle state of the current batch (lane = particle)
   float pe[3], pq[3], ppx, ppy, pstp, pdstp;
-  uint32_t pidx;
   // Phase A (lane = particle): the forward's footprint masks of this block's
   // columns (same float rules as the splat), smooth column / row factors.
   auto phase_a = [&](uint32_t base, int nb) {
     pe[0] = pe[1] = pe[2] = pq[0] = pq[1] = pq[2] = 0.f;
     ppx = ppy = pstp = pdstp = 0.f;
-    pidx = 0;
     if (lane < nb) {
       const uint32_t s = a.ent[base + lane];
       const Rec R = a.recs[s];
-      pidx = order[s];
       const float rfacr = R.r * a.rfac;
       const float posx = R.x - static_cast<float>(x0s), posy = R.y - static_cast<float>(y0s);
       const int bx0 = max(cvt_i32(posx - rfacr + 1), 0), bx1 = min(cvt_i32(posx + rfacr + 1), w);
@@ -310,7 +307,11 @@ __global__ __launch_bounds__(256) void k_adjoint_ord(SplatArgs a, const uint32_t
         continue;
       }
       const uint64_t any = __ballot(in);
-      if (any == 0) continue;  // uniform: the particle misses this block's pixels
+      float *pp = part + 6 * static_cast<size_t>(base + k);  // entry base + k's partials
+      if (any == 0) {  // uniform: the particle misses this block's pixels
+        if (lane < 6) pp[lane] = 0.f;
+        continue;
+      }
       const float att = ey[cy][k] * ex[cx][k];
       const float dx = lxf - lane_f(ppx, k), dy = lyf - lane_f(ppy, k);
       const float stp = lane_f(pstp, k);
@@ -338,15 +339,9 @@ __global__ __launch_bounds__(256) void k_adjoint_ord(SplatArgs a, const uint32_t
       const float gr = gatt * att * (dx * dx + dy * dy) * lane_f(pdstp, k);
       const float s0 = wave_sum(gsx), s1 = wave_sum(gsy), s2 = wave_sum(gr);
       const float s3 = wave_sum(ge[0]), s4 = wave_sum(ge[1]), s5 = wave_sum(ge[2]);
-      if (lane == 0) {
-        double *g = gscr + 6 * static_cast<size_t>(__builtin_amdgcn_readlane(pidx, k));
-        unsafeAtomicAdd(g + 0, static_cast<double>(s0));
-        unsafeAtomicAdd(g + 1, static_cast<double>(s1));
-        unsafeAtomicAdd(g + 2, static_cast<double>(s2));
-        unsafeAtomicAdd(g + 3, static_cast<double>(s3));
-        unsafeAtomicAdd(g + 4, static_cast<double>(s4));
-        unsafeAtomicAdd(g + 5, static_cast<double>(s5));
-      }
+      const float v = lane == 0 ? s0 : lane == 1 ? s1 : lane == 2 ? s2 :
+                      lane == 3 ? s3 : lane == 4 ? s4 : s5;
+      if (lane < 6) pp[lane] = v;
     }
     wave_sync();
   }
@@ -505,6 +500,29 @@ unsigned grid_of(int64_t n) {
   return static_cast<unsigned>(g);
 }
 
+
+__global__ __launch_bounds__(256) void k_iota(uint32_t *v, uint32_t n) {
+  for (int64_t i = gtid_(); i < n; i += gstride_()) v[i] = static_cast<uint32_t>(i);
+}
+
+// Per particle s of a pass (render position): the sum in FP64 of its list
+// entries' partials, the entries in ascending list position (jv[poff[s] ..
+// poff[s+1]), sorted stably by particle), added to the particle's gradient.
+__global__ __launch_bounds__(256) void k_reduce_parts(const uint32_t *poff, const uint32_t *jv,
+                                                      const float *part, const uint32_t *order,
+                                                      uint32_t m, double *gscr) {
+  for (int64_t s = gtid_(); s < m; s += gstride_()) {
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (uint32_t t = poff[s]; t < poff[s + 1]; ++t) {
+      const float *pp = part + 6 * static_cast<size_t>(jv[t]);
+#pragma unroll
+      for (int i = 0; i < 6; ++i) acc[i] += static_cast<double>(pp[i]);
+    }
+    double *g = gscr + 6 * static_cast<size_t>(order[s]);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) g[i] += acc[i];
+  }
+}
 }  // namespace
 
 void launch_seg_count(const uint32_t *bin_off, uint32_t nunits, uint32_t *cnt, hipStream_t s) {
@@ -520,23 +538,35 @@ void launch_seg_emit(const uint32_t *bin_off, uint32_t nunits, const uint32_t *s
 
 void launch_adjoint(bool additive, const uint32_t *seg_unit, const uint32_t *seg_first,
                     uint32_t nseg, const uint32_t *soff, uint32_t nunits, const SplatArgs &a,
-                    const float *grad, const uint32_t *order, double *gscr, double *segstate,
-                    hipStream_t s) {
+                    const float *grad, float *part, double *segstate, hipStream_t s) {
   if (nseg == 0) return;
   const unsigned grid = (nseg + 3) / 4;
   if (additive) {
-    k_adjoint_add<<<grid, 256, 0, s>>>(a, seg_unit, seg_first, nseg, grad, order, gscr);
+    k_adjoint_add<<<grid, 256, 0, s>>>(a, seg_unit, seg_first, nseg, grad, part);
     hip_check(hipGetLastError(), "k_adjoint_add launch");
     return;
   }
   const size_t per = static_cast<size_t>(nseg) * 3 * 64;
   const SegView sv{segstate, segstate + per, segstate + 2 * per};
-  k_adjoint_ord<0><<<grid, 256, 0, s>>>(a, seg_unit, seg_first, nseg, grad, order, gscr, sv);
+  k_adjoint_ord<0><<<grid, 256, 0, s>>>(a, seg_unit, seg_first, nseg, grad, part, sv);
   hip_check(hipGetLastError(), "k_adjoint_ord (segment maps) launch");
   k_adjoint_join<<<nunits, 192, 0, s>>>(soff, nunits, sv);
   hip_check(hipGetLastError(), "k_adjoint_join launch");
-  k_adjoint_ord<2><<<grid, 256, 0, s>>>(a, seg_unit, seg_first, nseg, grad, order, gscr, sv);
+  k_adjoint_ord<2><<<grid, 256, 0, s>>>(a, seg_unit, seg_first, nseg, grad, part, sv);
   hip_check(hipGetLastError(), "k_adjoint_ord (gradients) launch");
+}
+
+void launch_iota(uint32_t *v, uint32_t n, hipStream_t s) {
+  if (n == 0) return;
+  k_iota<<<grid_of(n), 256, 0, s>>>(v, n);
+  hip_check(hipGetLastError(), "k_iota launch");
+}
+
+void launch_reduce_parts(const uint32_t *poff, const uint32_t *jv, const float *part,
+                         const uint32_t *order, uint32_t m, double *gscr, hipStream_t s) {
+  if (m == 0) return;
+  k_reduce_parts<<<grid_of(m), 256, 0, s>>>(poff, jv, part, order, m, gscr);
+  hip_check(hipGetLastError(), "k_reduce_parts launch");
 }
 
 void launch_chain(const ProjectArgs *jobs, int64_t n, const uint8_t *act, const double *gscr,

@@ -1604,15 +1604,29 @@ int backward_impl(splhip_ctx *c, const float *grad, const splhip_grads *out, hip
     launch_seg_count(pl.bin_off[p], nu, cnt, s);
     scan_u32(cnt, soff, nu, soff + nu, ws, s);
     hip_check(hipMemcpyAsync(tot, soff + nu, 4, hipMemcpyDeviceToDevice, s), "segment total");
-    read_totals(c, tot, 1, s);
-    const uint32_t nseg = c->h_tot[0];
+    hip_check(hipMemcpyAsync(tot + 1, pl.bin_off[p] + nu, 4, hipMemcpyDeviceToDevice, s),
+              "entry total");
+    read_totals(c, tot, 2, s);
+    const uint32_t nseg = c->h_tot[0], ne = c->h_tot[1];
     if (nseg == 0) continue;
     uint32_t *seg = ws.get_as<uint32_t>(S_GRAD3, 2 * static_cast<size_t>(nseg));
     launch_seg_emit(pl.bin_off[p], nu, soff, seg, seg + nseg, s);
     double *state = p == 0 ? ws.get_as<double>(S_GRAD4, static_cast<size_t>(nseg) * 3 * 64 * 3)
                            : nullptr;
+    // per-entry partials, then per particle their sum in a fixed order: the
+    // entries sorted stably by particle (render position), summed in FP64
+    float *part = ws.get_as<float>(S_GRAD2, static_cast<size_t>(ne) * 6);
     const SplatArgs sa = splat_args(c, pl, p, F.tn);
-    launch_adjoint(p == 1, seg, seg + nseg, nseg, soff, nu, sa, grad, pl.order[p], gscr, state, s);
+    launch_adjoint(p == 1, seg, seg + nseg, nseg, soff, nu, sa, grad, part, state, s);
+    uint32_t *k0 = ws.get_as<uint32_t>(S_GRAD5, ne), *v0 = ws.get_as<uint32_t>(S_GRAD6, ne);
+    uint32_t *k1 = ws.get_as<uint32_t>(S_GRAD7, ne), *v1 = ws.get_as<uint32_t>(S_GRAD8, ne);
+    uint32_t *poff = ws.get_as<uint32_t>(S_GRAD9, static_cast<size_t>(pl.m[p]) + 1);
+    hip_check(hipMemcpyAsync(k0, pl.ent[p], static_cast<size_t>(ne) * 4, hipMemcpyDeviceToDevice,
+                             s), "entry particles");
+    launch_iota(v0, ne, s);
+    const int r = radix_sort_pairs(k0, v0, k1, v1, ne, 0, ceil_log2(pl.m[p]), ws, s);
+    launch_bin_offsets(r ? k1 : k0, ne, pl.m[p], poff, s);
+    launch_reduce_parts(poff, r ? v1 : v0, part, pl.order[p], pl.m[p], gscr, s);
   }
   const uint8_t *act = ws.get_as<uint8_t>(S_ACT, static_cast<size_t>(std::max<int64_t>(n, 1)));
   launch_chain(pl.d_pa, n, act, gscr, *out, s);

@@ -180,11 +180,17 @@ constexpr uint32_t kAdjSeg = 2048;
 void launch_seg_count(const uint32_t *bin_off, uint32_t nunits, uint32_t *cnt, hipStream_t s);
 void launch_seg_emit(const uint32_t *bin_off, uint32_t nunits, const uint32_t *soff,
                      uint32_t *seg_unit, uint32_t *seg_first, hipStream_t s);
-// segstate: ordered mode only, nseg * 3 * 64 * 3 doubles.
+// part: 6 floats per list entry (the particle's screen-space partials over
+// the entry's block); segstate: ordered mode only, nseg * 3 * 64 * 3 doubles.
 void launch_adjoint(bool additive, const uint32_t *seg_unit, const uint32_t *seg_first,
                     uint32_t nseg, const uint32_t *soff, uint32_t nunits, const SplatArgs &a,
-                    const float *grad, const uint32_t *order, double *gscr, double *segstate,
-                    hipStream_t s);
+                    const float *grad, float *part, double *segstate, hipStream_t s);
+// v[i] = i
+void launch_iota(uint32_t *v, uint32_t n, hipStream_t s);
+// gscr[order[s]] += sum over t in [poff[s], poff[s+1]) of part[jv[t]] (FP64,
+// in that order), for the m particles of a pass.
+void launch_reduce_parts(const uint32_t *poff, const uint32_t *jv, const float *part,
+                         const uint32_t *order, uint32_t m, double *gscr, hipStream_t s);
 void launch_chain(const ProjectArgs *jobs, int64_t n, const uint8_t *act, const double *gscr,
                   const splhip_grads &out, hipStream_t s);
 void launch_zero_f64(double *p, int64_t n, hipStream_t s);

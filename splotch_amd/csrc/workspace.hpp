@@ -12,7 +12,7 @@ namespace splhip {
 enum Slot : int {
   S_REC, S_ACT, S_HIST, S_SCAN0, S_SCAN1, S_SCAN2, S_SCAN3,
   S_TOTALS, S_SOA, S_RECSTAGE, S_PIC, S_MAP, S_AUX0, S_JOBS, S_GRAD0, S_GRAD1, S_GRAD2, S_GRAD3,
-  S_GRAD4, S_SCRATCH, S_JOBS2, S_NORM,
+  S_GRAD4, S_GRAD5, S_GRAD6, S_GRAD7, S_GRAD8, S_GRAD9, S_SCRATCH, S_JOBS2, S_NORM,
   S_PASS  // first per-pass slot
 };
 

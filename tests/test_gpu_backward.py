@@ -232,3 +232,30 @@ def test_backward_state_errors(R):
     R.render_batched([(dev, cam, S.params(), pic)])
     with pytest.raises(S.SplhipError):
         R.backward(g, fld.n)  # batches are not differentiated
+
+
+@pytest.mark.parametrize("a_eq_e", [True, False])
+def test_backward_deterministic(R, a_eq_e):
+    """The per-particle sums are taken in a fixed order (per-entry partials,
+    entries sorted stably by particle, FP64 sums): two backward calls, and
+    two forward + backward rounds, give the same gradients bit for bit."""
+    import splotch_amd as S
+    log("case backward determinism", a_eq_e)
+    fld = synth.plummer_field(80_000, 1310, halo_frac=0.9, n_halos=3, hsml_mu=math.log(0.6))
+    view = synth.default_view(256, 224)
+    G = np.random.default_rng(1311).normal(0.0, 1.0, (256, 224, 3))
+    dev = S.to_device(fld)  # the backward reads the forward's particle arrays
+    cam = S.camera_setup(view)
+    pic = torch.empty(view.xres * view.yres * 3, dtype=torch.float32, device="cuda")
+    g = torch.from_numpy(G.astype(np.float32).reshape(-1)).cuda()
+    runs = []
+    for _ in range(2):
+        R.render(dev, cam, S.params(a_eq_e=a_eq_e), pic)
+        got = {k: v.cpu().numpy() for k, v in R.backward(g, fld.n, screen=True).items()}
+        again = R.backward(g, fld.n, screen=True)
+        torch.cuda.synchronize()
+        for k, v in again.items():
+            assert np.array_equal(v.cpu().numpy().view(np.uint32), got[k].view(np.uint32)), k
+        runs.append(got)
+    for k in runs[0]:
+        assert np.array_equal(runs[0][k].view(np.uint32), runs[1][k].view(np.uint32)), k
