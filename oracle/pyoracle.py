@@ -638,3 +638,61 @@ def frozen_model(field, view, palettes, a_eq_e=True, sort_type=1, **kw):
         return p, np.arange(p.n)
     perm = sort_perm(p, sort_type, True)
     return p.take(perm), perm
+
+
+# ------------------------------------------------------------------ Splotch-side binding
+
+SHIM_SO = os.path.join(HERE, "_ref", "libsplshim.so")
+_shim = None
+
+
+def shim_available() -> bool:
+    return os.path.exists(SHIM_SO)
+
+
+def shim():
+    """The Splotch-side binding (splotch_amd/shim/hip_splotch.cc) compiled
+    with the genuine reference objects (shim_driver.cpp)."""
+    global _shim
+    if _shim is None:
+        if not shim_available():
+            raise RuntimeError("binding test library not built (make -C oracle shim)")
+        _shim = C.CDLL(SHIM_SO)
+    return _shim
+
+
+def shim_setup(kv: dict, view, b_brightness=1.0, stride=1024):
+    """hip_setup on a genuine paramfile (kv, with palette{t} file paths) and
+    the colour maps the genuine get_colourmaps loads: (params struct bytes,
+    camera struct bytes, per-type palettes [(x, rgb)])."""
+    from splotch_amd import capi
+    prm, cam = capi.Params(), capi.Camera()
+    nt = int(kv.get("ptypes", 1))
+    pal_n = np.zeros(8, np.int32)
+    pal_x = np.zeros(8 * stride, np.float64)
+    pal_rgb = np.zeros(8 * stride * 3, np.float32)
+    rc = shim().shim_setup(kv_text(kv).encode(), _p(_v3(view.campos)), _p(_v3(view.lookat)),
+                           _p(_v3(view.sky)), _p(_v3(view.center)), C.c_float(b_brightness),
+                           C.byref(prm), C.byref(cam), _p(pal_n), _p(pal_x), _p(pal_rgb),
+                           C.c_int(stride))
+    if rc != 0:
+        raise RuntimeError("shim_setup failed (%d)" % rc)
+    pals = [(pal_x[t * stride:t * stride + pal_n[t]].copy(),
+             pal_rgb[3 * t * stride:3 * (t * stride + pal_n[t])].reshape(-1, 3).copy())
+            for t in range(nt)]
+    return prm, cam, pals
+
+
+def shim_render(kv: dict, records: np.ndarray, view, genuine: bool, b_brightness=1.0):
+    """One frame of particle_sim records through hip_rendering (the GPU
+    renderer) or the genuine host_rendering, both called with genuine
+    Splotch objects.  Returns the x-major RGB image."""
+    pic = np.zeros((view.xres, view.yres, 3), np.float32)
+    rec = np.ascontiguousarray(records)
+    rc = shim().shim_render(kv_text(kv).encode(), C.c_int64(len(rec)), _p(rec),
+                            _p(_v3(view.campos)), _p(_v3(view.lookat)), _p(_v3(view.sky)),
+                            _p(_v3(view.center)), C.c_float(b_brightness), C.c_int(view.xres),
+                            C.c_int(view.yres), C.c_int(int(genuine)), _p(pic))
+    if rc != 0:
+        raise RuntimeError("shim_render failed (%d)" % rc)
+    return pic
