@@ -408,6 +408,18 @@ int splhip_normalize(splhip_ctx *ctx, int64_t n, float *inten, float *col, float
                      const float *hsml, const uint8_t *type, const splhip_norm *norm,
                      splhip_norm_ranges *ranges, void *stream);
 
+/* splhip_normalize without allocation (no reference counterpart; for callers
+   that cannot allocate device memory: code under HIP graph capture, tight
+   serving loops, harnesses that forbid hipMalloc).  Same arguments, results
+   and errors as splhip_normalize; the per-type extremes live in a file-scope
+   device array of 64 uint32 (256 bytes) per GPU instead of the context's
+   workspace, cleared by the call itself.  Limit: ptypes <= 8 (6 words per
+   type), else SPLHIP_E_CAPACITY.  Every call on one GPU shares that array:
+   calls that overlap on one GPU (other streams or contexts) race. */
+int splhip_normalize_noalloc(splhip_ctx *ctx, int64_t n, float *inten, float *col, float *c1,
+                             float *c2, const float *hsml, const uint8_t *type,
+                             const splhip_norm *norm, splhip_norm_ranges *ranges, void *stream);
+
 /* --- SoA ingest of block binary files ------------------------------------------ */
 
 /* bin_reader_block (reader/bin_reader.cc:34-88,147-190): the file `infile`

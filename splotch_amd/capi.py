@@ -109,7 +109,7 @@ EXPORTS = ["splhip_create", "splhip_destroy", "splhip_last_error", "splhip_camer
            "splhip_last_tuning", "splhip_last_binning", "splhip_post_from_par",
            "splhip_postprocess", "splhip_norm_from_par", "splhip_normalize",
            "splhip_bin_block_count", "splhip_read_bin_block", "splhip_scene_file_parse",
-           "splhip_capture_reserve", "splhip_capture_status"]
+           "splhip_capture_reserve", "splhip_capture_status", "splhip_normalize_noalloc"]
 
 STAGES = ("project", "ordered_sort", "ordered_bin", "ordered_splat", "additive_compact",
           "additive_bin", "additive_splat", "ordered_k_splat8", "ordered_k_splat_heavy",
@@ -444,20 +444,23 @@ class Renderer:
             self._stream(stream)), "splhip_postprocess")
         return out, rgb8
 
-    def normalize(self, dev: dict, q: Norm, stream=None, ranges: bool = True):
+    def normalize(self, dev: dict, q: Norm, stream=None, ranges: bool = True,
+                  noalloc: bool = False):
         """particle_normalize in place on device fields (dev: inten, col, c1,
         c2, hsml, type as cuda tensors).  Returns the per-type ranges (list of
         dicts; the call then waits for the stream), or None with
-        ranges=False."""
+        ranges=False.  noalloc: through splhip_normalize_noalloc."""
         n = int(dev["x"].numel()) if "x" in dev else int(dev["inten"].numel())
 
         def ptr(k):
             t = dev.get(k)
             return C.c_void_p(t.data_ptr() if t is not None else None)
         rr = (NormRanges * MAX_TYPES)() if ranges else None
-        self._check(lib().splhip_normalize(
+        fn = lib().splhip_normalize_noalloc if noalloc else lib().splhip_normalize
+        self._check(fn(
             self._c, C.c_int64(n), ptr("inten"), ptr("col"), ptr("c1"), ptr("c2"), ptr("hsml"),
-            ptr("type"), C.byref(q), rr, self._stream(stream)), "splhip_normalize")
+            ptr("type"), C.byref(q), rr, self._stream(stream)),
+            "splhip_normalize_noalloc" if noalloc else "splhip_normalize")
         if not ranges:
             return None
         return [{k: getattr(rr[t], k) for k, _ in NormRanges._fields_} for t in range(q.ptypes)]

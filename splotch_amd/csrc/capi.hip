@@ -25,8 +25,16 @@ using namespace splhip;
 struct FwdState;  // the last single-view render, kept for splhip_render_backward
 struct Reservation;  // capacities and workspace of a batch rendered under stream capture
 
+// Scratch of splhip_normalize_noalloc: the per-type extremes of
+// particle_normalize (6 order-preserving u32 keys per type), in a file-scope
+// device array so that the call allocates nothing.  Each GPU holds its own
+// copy; its address comes from hipGetSymbolAddress at context creation.
+constexpr int kNoAllocWords = 64;
+static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
+
 struct splhip_ctx {
   FwdState *fwd = nullptr;
+  uint32_t *scratch_ptr = nullptr;  // this device's copy of `scratch`
   Reservation *rv = nullptr;
   bool split_hint = false;  // the last capacity error can be avoided by smaller groups
   // the 8x8 splat decomposition found by the last inner autotuner sweep:
@@ -1671,6 +1679,10 @@ int splhip_create(int device, splhip_ctx **out) {
     hip_check(hipHostMalloc(reinterpret_cast<void **>(&c->h_tot), 64 * sizeof(uint32_t),
                             hipHostMallocDefault),
               "hipHostMalloc");
+    // the device address of this GPU's `scratch` (the symbol's name in host
+    // code is not a device address)
+    hip_check(hipGetSymbolAddress(reinterpret_cast<void **>(&c->scratch_ptr), HIP_SYMBOL(scratch)),
+              "hipGetSymbolAddress scratch");
   } catch (const HipError &e) {
     delete c;
     return SPLHIP_E_HIP;
@@ -1986,9 +1998,13 @@ int splhip_norm_from_par(const char *par_text, splhip_norm *out) {
   return 0;
 }
 
-int splhip_normalize(splhip_ctx *c, int64_t n, float *inten, float *col, float *c1, float *c2,
-                     const float *hsml, const uint8_t *type, const splhip_norm *norm,
-                     splhip_norm_ranges *ranges, void *stream) {
+namespace {
+
+// particle_normalize with the per-type extremes in `keys` (device, 6 ptypes
+// u32): the shared body of splhip_normalize and splhip_normalize_noalloc.
+int normalize_impl(splhip_ctx *c, int64_t n, float *inten, float *col, float *c1, float *c2,
+                   const float *hsml, const uint8_t *type, const splhip_norm *norm,
+                   splhip_norm_ranges *ranges, void *stream, bool noalloc) {
   if (!c || !norm) return fail(c, SPLHIP_E_ARG, "null argument");
   if (n < 0) return fail(c, SPLHIP_E_ARG, "negative particle count");
   const int nt = norm->ptypes;
@@ -2025,7 +2041,7 @@ int splhip_normalize(splhip_ctx *c, int64_t n, float *inten, float *col, float *
       a.ovr[t][2] = norm->color_min[t];
       a.ovr[t][3] = norm->color_max[t];
     }
-    uint32_t *keys = c->ws.get_as<uint32_t>(S_NORM, 6 * kMaxTypes);
+    uint32_t *keys = noalloc ? c->scratch_ptr : c->ws.get_as<uint32_t>(S_NORM, 6 * kMaxTypes);
     launch_normalize(a, keys, s);
     if (ranges) {
       uint32_t h[6 * kMaxTypes];
@@ -2054,6 +2070,24 @@ int splhip_normalize(splhip_ctx *c, int64_t n, float *inten, float *col, float *
     return hip_fail(c, e);
   }
   return 0;
+}
+
+}  // namespace
+
+int splhip_normalize(splhip_ctx *c, int64_t n, float *inten, float *col, float *c1, float *c2,
+                     const float *hsml, const uint8_t *type, const splhip_norm *norm,
+                     splhip_norm_ranges *ranges, void *stream) {
+  return normalize_impl(c, n, inten, col, c1, c2, hsml, type, norm, ranges, stream, false);
+}
+
+int splhip_normalize_noalloc(splhip_ctx *c, int64_t n, float *inten, float *col, float *c1,
+                             float *c2, const float *hsml, const uint8_t *type,
+                             const splhip_norm *norm, splhip_norm_ranges *ranges, void *stream) {
+  if (!c || !norm) return fail(c, SPLHIP_E_ARG, "null argument");
+  if (!c->scratch_ptr) return fail(c, SPLHIP_E_STATE, "no scratch address on this context");
+  if (norm->ptypes < 1 || norm->ptypes > kMaxTypes || 6 * norm->ptypes > kNoAllocWords)
+    return fail(c, SPLHIP_E_CAPACITY, "ptypes past the no-alloc scratch (at most 8 types)");
+  return normalize_impl(c, n, inten, col, c1, c2, hsml, type, norm, ranges, stream, true);
 }
 
 namespace {
