@@ -240,7 +240,7 @@ def gpu_time(torch, fn, steps, warmup):
 
 def membench():
     lib = ctypes.CDLL(os.path.join(ROOT, "splotch_amd", "libmembench.so"))
-    lib.lds_peak_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+    lib.lds_peak_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                  ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)]
     lib.hbm_copy_run.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                  ctypes.POINTER(ctypes.c_float)]
@@ -248,12 +248,20 @@ def membench():
 
 
 def lds_peak(grid=2048, iters=12000):
-    ms, nbytes = ctypes.c_float(), ctypes.c_double()
-    rc = membench().lds_peak_run(0, grid, iters, ctypes.byref(ms), ctypes.byref(nbytes))
-    if rc != 0:
-        raise RuntimeError("lds_peak_run failed (%d)" % rc)
-    return {"kernel": "k_lds_read (ds_read_b128, conflict-free)", "ms": round(ms.value, 4),
-            "bytes": nbytes.value, "GBs": round(nbytes.value / (ms.value * 1e-3) / 1e9, 1)}
+    """LDS read bandwidth of conflict-free ds_read_b32 / b64 / b128 streams;
+    the best is the measured peak."""
+    runs = {}
+    for words, name in ((1, "b32"), (2, "b64"), (4, "b128")):
+        ms, nbytes = ctypes.c_float(), ctypes.c_double()
+        it = iters * 4 // words  # about the same time per variant
+        rc = membench().lds_peak_run(0, grid, it, words, ctypes.byref(ms), ctypes.byref(nbytes))
+        if rc != 0:
+            raise RuntimeError("lds_peak_run failed (%d)" % rc)
+        runs[name] = {"ms": round(ms.value, 4), "bytes": nbytes.value,
+                      "GBs": round(nbytes.value / (ms.value * 1e-3) / 1e9, 1)}
+    best = max(runs, key=lambda k: runs[k]["GBs"])
+    return {"kernel": "k_lds_read<words> (conflict-free ds_read_%s)" % best,
+            "GBs": runs[best]["GBs"], "variants": runs}
 
 
 def hbm_copy(nbytes=1 << 30, reps=10):
@@ -287,9 +295,15 @@ def bench_c0(S, R, torch, O, synth, args):
     prm = S.params(a_eq_e=True)
     pic = torch.empty(800 * 800 * 3, dtype=torch.float32, device="cuda")
     ms = gpu_time(torch, lambda: R.render(dev, cam, prm, pic), 20, 2)
+    R.set_profiling(True)
+    R.render(dev, cam, prm, pic)
+    stages, counts = R.stage_times()
+    R.set_profiling(False)
     got = pic.cpu().numpy().reshape(800, 800, 3)
     out = {"workload": "plain_1M_800sq", "particles": fld.n, "image": [800, 800],
-           "ms_per_frame": round(ms, 4), "mparticles_s": round(fld.n / (ms * 1e-3) / 1e6, 2)}
+           "ms_per_frame": round(ms, 4), "mparticles_s": round(fld.n / (ms * 1e-3) / 1e6, 2),
+           "stages_ms": {k: round(v, 4) for k, v in stages.items() if v},
+           "counts": counts}
     want, _ = oracle_images(O, fld, view, [pal], [1], None, False)
     f64, _ = oracle_images(O, fld, view, [pal], [1], None, True)
     par = {"vs_oracle": max_rel(got, want), "vs_fp64": max_rel(got, f64),
@@ -647,8 +661,9 @@ def main():
     ap.add_argument("--no-backward", action="store_true", help="skip the fwd+bwd workload")
     ap.add_argument("--no-c0", action="store_true", help="skip the 1M / 800^2 workload")
     ap.add_argument("--workload", default="gas_stars",
-                    choices=["gas_stars", "large_share"],
-                    help="gas_stars: 16M at 2048^2 (default); large_share: 200M at 4096^2")
+                    choices=["gas_stars", "large_share", "plain_1M"],
+                    help="gas_stars: 16M at 2048^2 (default); large_share: 200M at 4096^2; "
+                         "plain_1M: only the 1M / 800^2 additive workload (its JSON)")
     ap.add_argument("--count-fragments", action="store_true",
                     help="print the default frame's fragments per pass (oracle, CPU) and exit")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -681,6 +696,14 @@ def main():
     if args.workload == "large_share":
         import splotch_amd as S
         bench_large_share(S, torch, args)
+        return
+    if args.workload == "plain_1M":
+        import splotch_amd as S
+        from splotch_amd import synth
+        R = S.Renderer(0)
+        R.set_palette(0, *synth.default_palette())
+        print(json.dumps(bench_c0(S, R, torch, oracle(), synth, args)), flush=True)
+        R.close()
         return
 
     import splotch_amd as S

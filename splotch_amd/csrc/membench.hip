@@ -3,11 +3,12 @@
 // and known-size kernels for scaling profiler counters.
 //
 // LDS read bandwidth: every workgroup fills 32 KiB of LDS, then each lane
-// reads 16-byte vectors (ds_read_b128) at conflict-free addresses: a wave's
-// 64 lanes read 1 KiB of consecutive bytes per instruction.  The reads are
-// folded into an integer checksum (two VALU operations per read, far below
-// the VALU issue limit), which is written once per lane so that no read can
-// be removed.  Bytes per launch = grid * 256 lanes * iters * 8 reads * 16 B.
+// reads 4-, 8- or 16-byte words (ds_read_b32 / b64 / b128) at conflict-free
+// addresses: a wave's 64 lanes read consecutive bytes per instruction.  The
+// reads are folded into an integer checksum (at most two VALU operations per
+// read, below the VALU issue limit), which is written once per lane so that
+// no read can be removed.  Bytes per launch = grid * 256 lanes * iters * 8
+// reads * the word size.
 //
 // HBM copy: dst[i] = src[i] over n 16-byte vectors, grid-stride, each lane
 // moving four vectors per step with all four loads issued before the stores.
@@ -20,6 +21,8 @@ namespace {
 
 constexpr int kWords = 2048;  // uint4 slots: 32 KiB
 
+// V: 32-bit words per lane and read (1: ds_read_b32, 2: b64, 4: b128).
+template <int V>
 __global__ __launch_bounds__(256) void k_lds_read(uint32_t *out, int iters) {
   __shared__ uint4 buf[kWords];
   const int t = threadIdx.x;
@@ -28,13 +31,22 @@ __global__ __launch_bounds__(256) void k_lds_read(uint32_t *out, int iters) {
   __syncthreads();
   uint32_t acc = 0;
   int base = t;
+  constexpr int kSlots = kWords * 4 / V;  // V-word slots in the buffer
   for (int k = 0; k < iters; ++k) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const uint4 v = buf[(base + j * 256) & (kWords - 1)];
-      acc ^= v.x + v.w;
+      const int slot = (base + j * 256) & (kSlots - 1);
+      if (V == 4) {
+        const uint4 v = buf[slot];
+        acc ^= v.x + v.w;
+      } else if (V == 2) {
+        const uint2 v = reinterpret_cast<const uint2 *>(buf)[slot];
+        acc ^= v.x + v.y;
+      } else {
+        acc += reinterpret_cast<const uint32_t *>(buf)[slot];
+      }
     }
-    base = (base + 64) & (kWords - 1);  // the next reads start a wave further on
+    base = (base + 64) & (kSlots - 1);  // the next reads start a wave further on
   }
   out[static_cast<int64_t>(blockIdx.x) * 256 + t] = acc;
 }
@@ -61,12 +73,13 @@ __global__ __launch_bounds__(256) void k_copy(const uint4 *__restrict__ src,
 
 extern "C" {
 
-// Times one launch of `grid` workgroups x `iters` iterations (after one
-// warm-up launch) with HIP events on a stream of its own.  Returns 0 and
-// writes the milliseconds and the bytes read from LDS; a HIP error code
-// otherwise.
-int lds_peak_run(int device, int grid, int iters, float *ms, double *bytes) {
-  if (grid < 1 || iters < 1 || !ms || !bytes) return -1;
+// Times one launch of `grid` workgroups x `iters` iterations of reads of
+// `words` 32-bit words per lane (1, 2 or 4), after one warm-up launch, with
+// HIP events on a stream of its own.  Returns 0 and writes the milliseconds
+// and the bytes read from LDS; a HIP error code otherwise.
+int lds_peak_run(int device, int grid, int iters, int words, float *ms, double *bytes) {
+  if (grid < 1 || iters < 1 || !ms || !bytes || (words != 1 && words != 2 && words != 4))
+    return -1;
   if (hipSetDevice(device) != hipSuccess) return -2;
   uint32_t *out = nullptr;
   hipStream_t s = nullptr;
@@ -77,15 +90,20 @@ int lds_peak_run(int device, int grid, int iters, float *ms, double *bytes) {
       hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
     rc = -2;
   } else {
-    k_lds_read<<<grid, 256, 0, s>>>(out, iters);
+    auto launch = [&]() {
+      if (words == 4) k_lds_read<4><<<grid, 256, 0, s>>>(out, iters);
+      else if (words == 2) k_lds_read<2><<<grid, 256, 0, s>>>(out, iters);
+      else k_lds_read<1><<<grid, 256, 0, s>>>(out, iters);
+    };
+    launch();
     (void)hipEventRecord(e0, s);
-    k_lds_read<<<grid, 256, 0, s>>>(out, iters);
+    launch();
     (void)hipEventRecord(e1, s);
     if (hipEventSynchronize(e1) != hipSuccess || hipGetLastError() != hipSuccess) {
       rc = -2;
     } else {
       (void)hipEventElapsedTime(ms, e0, e1);
-      *bytes = static_cast<double>(grid) * 256.0 * iters * 8.0 * 16.0;
+      *bytes = static_cast<double>(grid) * 256.0 * iters * 8.0 * 4.0 * words;
     }
   }
   if (e0) (void)hipEventDestroy(e0);
