@@ -99,6 +99,14 @@ typedef struct {
                                               walk (hip_variant 4): 1, 2, 4 or 8 (0 = auto:
                                               8 for one channel, 8 additive / 4 ordered for
                                               three) */
+  int32_t hip_stage;                       /* hip_stage: list length from which an 8x8 block
+                                              of an additive pass is staged: its per-pixel
+                                              terms are computed across the GPU first, then
+                                              summed in list order by one wave per channel
+                                              (0 = auto: 8192, but none for a single view
+                                              with both passes, whose additive splat runs
+                                              beside the ordered pass; < 0 = never; no
+                                              staging in device-count renders) */
 } splhip_params;
 
 /* SoA particle fields (device pointers), post particle_normalize:

@@ -87,6 +87,9 @@ struct splhip_ctx {
   hipStream_t aux2 = nullptr;
   hipEvent_t ev_fork2 = nullptr, ev_join2 = nullptr;
   Workspace ws2;
+  // stream of the staged lists' chains (they follow the operand kernel)
+  hipStream_t aux3 = nullptr;
+  hipEvent_t ev_fork3 = nullptr, ev_join3 = nullptr;
   // block-file ingest: two pinned buffers, each freed for reuse by its event
   char *h_ing[2] = {nullptr, nullptr};
   hipEvent_t ev_ing[2] = {nullptr, nullptr};
@@ -294,6 +297,13 @@ void ensure_aux2(splhip_ctx *c) {
   hip_check(hipEventCreateWithFlags(&c->ev_join2, hipEventDisableTiming), "hipEventCreate");
 }
 
+void ensure_aux3(splhip_ctx *c) {
+  if (c->aux3) return;
+  hip_check(hipStreamCreateWithFlags(&c->aux3, hipStreamNonBlocking), "hipStreamCreate");
+  hip_check(hipEventCreateWithFlags(&c->ev_fork3, hipEventDisableTiming), "hipEventCreate");
+  hip_check(hipEventCreateWithFlags(&c->ev_join3, hipEventDisableTiming), "hipEventCreate");
+}
+
 
 void upload_palettes(splhip_ctx *c) {
   if (!c->pal_dirty) return;
@@ -406,6 +416,10 @@ struct Plan {
   JobSplat *d_js = nullptr;
   // per pass
   uint32_t m[2] = {0, 0}, e[2] = {0, 0}, nitems[2] = {0, 0}, nheavy[2] = {0, 0};
+  // staged lists (the first nstage items; nheavy counts them too) and their
+  // groups of 4 entries
+  uint32_t nstage[2] = {0, 0};
+  uint64_t sgroups[2] = {0, 0};
   uint32_t e16[2] = {0, 0};         // (region, particle) entries
   bool r16[2] = {false, false};     // the pass's region counts are scanned (e16 valid)
   uint64_t used16[2] = {0, 0};      // region entries of the bins built through regions
@@ -611,10 +625,11 @@ void make_plan(splhip_ctx *c, int nj, const JobIn *jobs, int G, Plan &pl, hipStr
 // Totals slot layout (device u32): [0..1] pass sizes, [2..3] bin entries,
 // [4..5] work items, [6..7] heavy items, [8..11] bin entries as u64,
 // [12..13] region entries, [14] the ordered pass has a non-finite q,
-// [15] capacity overflow bits of a device-count render (bit p: pass p).
+// [15] capacity overflow bits of a device-count render (bit p: pass p),
+// [16..17] staged lists, [18..19] their groups of 4 entries.
 enum {
   T_M = 0, T_E = 2, T_ITEMS = 4, T_HEAVY = 6, T_E64 = 8, T_E16 = 12, T_QBAD = 14, T_OVF = 15,
-  T_N = 16
+  T_STAGE = 16, T_SGRP = 18, T_N = 20
 };
 
 // Project all jobs, then the compaction scan of each present pass and its job
@@ -818,7 +833,12 @@ struct Tuning {
   int variant, ppw;
   uint32_t split, heavy, heavy_max;
   uint64_t chunk;  // most (block, particle) entries of one pass chunk
+  uint32_t stage;  // additive lists from this length on are staged (0xffffffff: none)
 };
+
+// Staged operands of a pass: 3 KiB per group of 4 entries; a pass whose
+// staged lists need more renders them unstaged.
+constexpr uint64_t kStageBudget = uint64_t(8) << 30;
 
 Tuning tuning_of(const splhip_params *prm) {
   Tuning t;
@@ -836,6 +856,10 @@ Tuning tuning_of(const splhip_params *prm) {
   // pipeline with a host sync and a splat tail; offsets stay below 2^31
   t.chunk = prm->hip_chunk_entries > 0 ? static_cast<uint64_t>(prm->hip_chunk_entries)
                                        : (uint64_t(1) << 31) - (uint64_t(1) << 20);
+  // additive lists from 8192 entries on: a lane-per-pixel wave walks such a
+  // list at about 100 ns per entry
+  t.stage = prm->hip_stage < 0 ? 0xffffffffu
+            : prm->hip_stage > 0 ? static_cast<uint32_t>(prm->hip_stage) : 8192u;
   return t;
 }
 
@@ -855,6 +879,7 @@ SplatArgs splat_args(splhip_ctx *c, const Plan &pl, int p, const Tuning &tn) {
   sa.taylorlimit = c->consts.taylorlimit;
   sa.prio_len = tn.split;
   sa.heavy_len = tn.heavy;
+  sa.stage_len = (p == 1 && !pl.dc) ? tn.stage : 0xffffffffu;
   sa.qbad = p == 0 ? pl.qbad : nullptr;
   return sa;
 }
@@ -870,10 +895,32 @@ void stage_items_count(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, uint32_
   // the count of this call's long-list items only (chunks and autotuning
   // candidates count again)
   launch_fill_u32(tot + T_HEAVY + p, 0u, 1, s);
+  launch_fill_u32(tot + T_STAGE + p, 0u, 1, s);
+  launch_fill_u32(tot + T_SGRP + p, 0u, 1, s);
   launch_items_count(pl.bin_off[p], pl.nunits, sa, p == 0 ? 1 : 0, tn.split, icnt,
-                     tot + T_HEAVY + p, s);
+                     tot + T_HEAVY + p, s, tot + T_STAGE + p, tot + T_SGRP + p);
   scan_u32(icnt, ioff, pl.nunits, tot + T_ITEMS + p, ws, s);
   mark(c, p == 0 ? 2 : 5, false, s);
+}
+
+// Item counts of pass p from c->h_tot read from tot + T_ITEMS on (T_N -
+// T_ITEMS words): work items; long lists = the staged ones (when their
+// operands fit kStageBudget, else none) and then at most heavy_max for the
+// producer/consumer kernel.
+void take_items(splhip_ctx *c, Plan &pl, int p, const Tuning &tn) {
+  const uint32_t *h = c->h_tot;
+  pl.nitems[p] = pl.m[p] ? h[T_ITEMS - T_ITEMS + p] : 0u;
+  const uint32_t nlong = pl.m[p] ? h[T_HEAVY - T_ITEMS + p] : 0u;
+  uint32_t ns = pl.m[p] ? h[T_STAGE - T_ITEMS + p] : 0u;
+  const uint64_t groups = h[T_SGRP - T_ITEMS + p];
+  if (groups * 3 * 1024 > kStageBudget) ns = 0;
+  ns = std::min(ns, nlong);
+  pl.nstage[p] = ns;
+  pl.sgroups[p] = ns ? groups : 0;
+  const uint32_t npc = tn.heavy == 0xffffffffu ? 0u : std::min(nlong - ns, tn.heavy_max);
+  // the heaviest items (sorted first); a unit's channel items are
+  // independent, so the cut may fall between them
+  pl.nheavy[p] = std::min(ns + npc, pl.nitems[p]);
 }
 
 // Splat of pass p: items heaviest first; heavy units on the auxiliary stream.
@@ -917,29 +964,57 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
   // normal items: after the long lists (with device counts the kernel skips them)
   const uint32_t *nit = pl.dc ? items : items + nheavy;
   const uint32_t nn = pl.dc ? nitems : nitems - nheavy;
-  if (nheavy > 0 && !fork) {
-    mark(c, p == 0 ? 8 : 10, true, s);
-    launch_splat_pc(p == 1, items, nheavy, sa, s, dc);
-    mark(c, p == 0 ? 8 : 10, false, s);
-    mark(c, p == 0 ? 7 : 9, true, s);
-    launch_splat8(p == 1, tn.variant, nit, nn, sn, s, dc, tn.ppw);
-    mark(c, p == 0 ? 7 : 9, false, s);
-  } else if (nheavy > 0) {
-    // heavy blocks run on a second stream, beside the normal items
-    ensure_aux(c);
-    hip_check(hipEventRecord(c->ev_fork, s), "record fork");
-    hip_check(hipStreamWaitEvent(c->aux, c->ev_fork, 0), "wait fork");
-    mark(c, p == 0 ? 8 : 10, true, c->aux);
-    launch_splat_pc(p == 1, items, nheavy, sa, c->aux, dc);
-    mark(c, p == 0 ? 8 : 10, false, c->aux);
-    hip_check(hipEventRecord(c->ev_join, c->aux), "record join");
+  // long lists: the staged ones first, then the producer/consumer workgroups'
+  const uint32_t nstage = pl.dc ? 0u : std::min(pl.nstage[p], nheavy);
+  const uint32_t npc = nheavy - nstage;
+  SplatArgs sp = sa;
+  if (sp.trace) sp.trace += 4 * static_cast<size_t>(nstage);
+  const uint32_t *pit = items + nstage;
+  if (nstage > 0 || (npc > 0 && fork)) {
+    // heavy blocks run on second streams, beside the normal items: the
+    // producer/consumer workgroups at once, the staged chains after the
+    // operand kernel (on s, across the GPU)
+    if (npc > 0 && fork) {
+      ensure_aux(c);
+      hip_check(hipEventRecord(c->ev_fork, s), "record fork");
+      hip_check(hipStreamWaitEvent(c->aux, c->ev_fork, 0), "wait fork");
+      mark(c, p == 0 ? 8 : 10, true, c->aux);
+      launch_splat_pc(p == 1, pit, npc, sp, c->aux, dc);
+      mark(c, p == 0 ? 8 : 10, false, c->aux);
+      hip_check(hipEventRecord(c->ev_join, c->aux), "record join");
+    } else if (npc > 0) {
+      mark(c, p == 0 ? 8 : 10, true, s);
+      launch_splat_pc(p == 1, pit, npc, sp, s, dc);
+      mark(c, p == 0 ? 8 : 10, false, s);
+    }
+    if (nstage > 0) {
+      uint32_t *cb = ws.get_as<uint32_t>(pslot(p, P_SBASE), nstage + 1);
+      uint32_t *cbt = ws.get_as<uint32_t>(pslot(p, P_SBAT), nstage + 1);
+      float4 *ops = ws.get_as<float4>(S_STAGE, static_cast<size_t>(pl.sgroups[p]) * 192);
+      mark(c, p == 0 ? 8 : 10, true, s);
+      launch_stage(items, nstage, pl.sgroups[p], cb, cbt, ops, sa, s);
+      ensure_aux3(c);
+      hip_check(hipEventRecord(c->ev_fork3, s), "record fork");
+      hip_check(hipStreamWaitEvent(c->aux3, c->ev_fork3, 0), "wait fork");
+      launch_stage_chain(items, nstage, cb, ops, sa, c->aux3);
+      mark(c, p == 0 ? 8 : 10, false, c->aux3);
+      hip_check(hipEventRecord(c->ev_join3, c->aux3), "record join");
+    }
     // The normal items fill every CU; the heavy workgroups (the critical
     // path) must get their slots first, so the normal launch waits briefly.
     launch_delay(40, s);
     mark(c, p == 0 ? 7 : 9, true, s);
     launch_splat8(p == 1, tn.variant, nit, nn, sn, s, dc, tn.ppw);
     mark(c, p == 0 ? 7 : 9, false, s);
-    hip_check(hipStreamWaitEvent(s, c->ev_join, 0), "wait join");
+    if (npc > 0 && fork) hip_check(hipStreamWaitEvent(s, c->ev_join, 0), "wait join");
+    if (nstage > 0) hip_check(hipStreamWaitEvent(s, c->ev_join3, 0), "wait join");
+  } else if (npc > 0) {
+    mark(c, p == 0 ? 8 : 10, true, s);
+    launch_splat_pc(p == 1, pit, npc, sp, s, dc);
+    mark(c, p == 0 ? 8 : 10, false, s);
+    mark(c, p == 0 ? 7 : 9, true, s);
+    launch_splat8(p == 1, tn.variant, nit, nn, sn, s, dc, tn.ppw);
+    mark(c, p == 0 ? 7 : 9, false, s);
   } else {
     mark(c, p == 0 ? 7 : 9, true, s);
     launch_splat8(p == 1, tn.variant, items, nitems, sn, s, dc, tn.ppw);
@@ -1033,9 +1108,8 @@ void render_chunked_pass(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, uint3
     if (e > 0) {
       stage_bins(c, pl, p, s);
       stage_items_count(c, pl, p, tn, tot, s);
-      read_totals(c, tot + T_ITEMS, 4, s);
-      pl.nitems[p] = c->h_tot[p];
-      pl.nheavy[p] = std::min(c->h_tot[2 + p], tn.heavy_max);
+      read_totals(c, tot + T_ITEMS, T_N - T_ITEMS, s);
+      take_items(c, pl, p, tn);
       stage_splat(c, pl, p, tn, s);
     } else if (chunk == 0 && J.need_empty[p]) {
       launch_fill(target, 0.f, npix, s);  // no entries at all: the pass leaves zeros
@@ -1076,9 +1150,8 @@ Tuning autotune(splhip_ctx *c, Plan &pl, const Tuning &base, uint32_t *tot, hipS
     for (int p = 0; p < 2; ++p) {
       if (!repeatable[p]) continue;
       stage_items_count(c, pl, p, t, tot, s);
-      read_totals(c, tot + T_ITEMS, 4, s);
-      pl.nitems[p] = c->h_tot[p];
-      pl.nheavy[p] = std::min(c->h_tot[2 + p], t.heavy_max);
+      read_totals(c, tot + T_ITEMS, T_N - T_ITEMS, s);
+      take_items(c, pl, p, t);
       hip_check(hipEventRecord(e0, s), "hipEventRecord");
       stage_splat(c, pl, p, t, s);
       hip_check(hipEventRecord(e1, s), "hipEventRecord");
@@ -1153,6 +1226,11 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
   uint32_t *tot = ws.get_as<uint32_t>(S_TOTALS, 64);
   stage_project_compact(c, pl, rec, act, tot, s);
   upload_job_flags(c, pl, s);
+  // A single view with both passes: the additive splat runs beside the
+  // ordered pass on a second stream, where its long lists are off the
+  // critical path and staged operands' HBM traffic would slow the ordered
+  // pass's binning: no staging unless hip_stage asks for it.
+  if (nj == 1 && pl.m[0] && pl.m[1] && jobs[0].prm->hip_stage == 0) tn.stage = 0xffffffffu;
   for (int p = 0; p < 2; ++p)
     if (pl.m[p]) {
       stage_order(c, pl, p, rec, act, s);
@@ -1183,9 +1261,8 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
         }
         stage_bins(c, pl, p, s);
         stage_items_count(c, pl, p, tn, tot, s);
-        read_totals(c, tot + T_ITEMS, 4, s);
-        pl.nitems[p] = c->h_tot[p];
-        pl.nheavy[p] = std::min(c->h_tot[2 + p], tn.heavy_max);
+        read_totals(c, tot + T_ITEMS, T_N - T_ITEMS, s);
+        take_items(c, pl, p, tn);
         stage_splat(c, pl, p, tn, s);
       }
     }
@@ -1208,9 +1285,8 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
     ta.heavy_max = 1;
     stage_bins(c, pl, 1, s);
     stage_items_count(c, pl, 1, ta, tot, s);
-    read_totals(c, tot + T_ITEMS, 4, s);
-    pl.nitems[1] = c->h_tot[1];
-    pl.nheavy[1] = 0;
+    read_totals(c, tot + T_ITEMS, T_N - T_ITEMS, s);
+    take_items(c, pl, 1, ta);
     const JobSplat &J0 = pl.js[0];
     const int64_t npix = static_cast<int64_t>(J0.xres) * J0.yres * 3;
     float *scratch = ws.get_as<float>(S_SCRATCH, static_cast<size_t>(npix));
@@ -1233,9 +1309,8 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
     hip_check(hipEventRecord(c->ev_join2, c->aux2), "record join");
     stage_bins(c, pl, 0, s);
     stage_items_count(c, pl, 0, tn, tot, s);
-    read_totals(c, tot + T_ITEMS, 4, s);  // waits for stream s only
-    pl.nitems[0] = c->h_tot[0];
-    pl.nheavy[0] = std::min(c->h_tot[2], tn.heavy_max);
+    read_totals(c, tot + T_ITEMS, T_N - T_ITEMS, s);  // waits for stream s only
+    take_items(c, pl, 0, tn);
     stage_splat(c, pl, 0, tn, s);
     hip_check(hipStreamWaitEvent(s, c->ev_join2, 0), "wait join");
     launch_add_images(J0.pic, scratch, npix, s);
@@ -1252,13 +1327,8 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
     for (int p = 0; p < 2; ++p)
       if (pl.m[p] && G == 8) stage_items_count(c, pl, p, tn, tot, s);
     if (G == 8) {
-      read_totals(c, tot + T_ITEMS, 4, s);  // sync 3
-      for (int p = 0; p < 2; ++p) {
-        pl.nitems[p] = pl.m[p] ? c->h_tot[p] : 0;
-        // the heaviest items (sorted first) up to the cap; a unit's channel
-        // items are independent, so the cut may fall between them
-        pl.nheavy[p] = pl.m[p] ? std::min(c->h_tot[2 + p], tn.heavy_max) : 0;
-      }
+      read_totals(c, tot + T_ITEMS, T_N - T_ITEMS, s);  // sync 3
+      for (int p = 0; p < 2; ++p) take_items(c, pl, p, tn);
     }
     // A single view with both passes: the additive pass's splat renders into
     // a scratch image on a second stream, beside the ordered splat, and is
@@ -1759,6 +1829,9 @@ void splhip_destroy(splhip_ctx *c) {
   if (c->ev_fork2) (void)hipEventDestroy(c->ev_fork2);
   if (c->ev_join2) (void)hipEventDestroy(c->ev_join2);
   if (c->aux2) (void)hipStreamDestroy(c->aux2);
+  if (c->ev_fork3) (void)hipEventDestroy(c->ev_fork3);
+  if (c->ev_join3) (void)hipEventDestroy(c->ev_join3);
+  if (c->aux3) (void)hipStreamDestroy(c->aux3);
   c->ws.release();
   if (c->d_exptab) (void)hipFree(c->d_exptab);
   if (c->d_palx) (void)hipFree(c->d_palx);
@@ -1851,6 +1924,7 @@ int splhip_params_default(splhip_params *p) {
   p->hip_autotune = 0;
   p->hip_bin_group = 0;
   p->hip_ppw = 0;
+  p->hip_stage = 0;
   return 0;
 }
 
@@ -1893,6 +1967,7 @@ int splhip_params_from_par(const char *par_text, float b_brightness, splhip_para
     p->hip_autotune = par_num<int>(d, "hip_autotune", 0);
     p->hip_bin_group = par_num<int>(d, "hip_bin_group", 0);
     p->hip_ppw = par_num<int>(d, "hip_ppw", 0);
+    p->hip_stage = par_num<int>(d, "hip_stage", 0);
   } catch (const ParError &) {
     return SPLHIP_E_ARG;
   }

@@ -17,6 +17,7 @@
 // independent), which cuts the serial work per lane to about a third.
 #include "kernels.hpp"
 
+#include <algorithm>
 #include <type_traits>
 
 #include "device_util.hpp"
@@ -321,6 +322,62 @@ __device__ __forceinline__ void walk_batch(int nb, int cx, int cy, const uint32_
   }
 }
 
+// Phase A of a batch, lane = particle (splotch_host.cc:462-476 + 596-607, tile
+// frame): the lane's particle (list entry `entry`) tabulates, for the block's
+// 8 columns, its row range (BITS: bit r set when block row byo + r lies in
+// the column's [lo, hi); else lo | hi << 16) and Gaussian column factor, and
+// for the 8 rows its Gaussian row factor, into column `lane` of the wave's
+// LDS slices; returns the particle's record.
+template <bool BITS>
+__device__ __forceinline__ Rec tabulate(const SplatArgs &a, uint32_t entry, int lane, int x0s,
+                                        int y0s, int w, int h, int bxo, int byo,
+                                        const float *s_tab, uint32_t (*col)[kPad],
+                                        float (*pre2)[kPad], float (*pre1)[kPad],
+                                        float expfac) {
+  const Rec R = a.recs[a.ent[entry]];
+  const float rfacr = R.r * a.rfac;
+  const float posx = R.x - static_cast<float>(x0s), posy = R.y - static_cast<float>(y0s);
+  const int bx0 = max(cvt_i32(posx - rfacr + 1), 0), bx1 = min(cvt_i32(posx + rfacr + 1), w);
+  const int by0 = max(cvt_i32(posy - rfacr + 1), 0), by1 = min(cvt_i32(posy + rfacr + 1), h);
+  const float radsq = rfacr * rfacr;
+  const float sigma = a.h2sigma * R.r;
+  const float stp = -1.f / (sigma * sigma);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int x = bxo + c;
+    uint32_t lohi = 0;
+    float p2 = 0.f;
+    if (x >= bx0 && x < bx1) {
+      const float dxsq = (static_cast<float>(x) - posx) * (static_cast<float>(x) - posx);
+      const float dy = sqrtf(radsq - dxsq);
+      const int lo = max(by0, cvt_i32(posy - dy + 1));
+      const int hi = min(by1, cvt_i32(posy + dy + 1));
+      if (hi > lo) {
+        if (BITS) {
+          const int r0 = min(max(lo - byo, 0), 8), r1 = min(max(hi - byo, 0), 8);
+          lohi = (r1 > r0) ? (((1u << r1) - 1u) & ~((1u << r0) - 1u)) : 0u;
+        } else {
+          lohi = static_cast<uint32_t>(lo) | (static_cast<uint32_t>(hi) << 16);
+        }
+      }
+      p2 = xop(s_tab, stp * dxsq, expfac);
+    }
+    col[c][lane] = lohi;
+    pre2[c][lane] = p2;
+  }
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int y = byo + r;
+    float p1 = 0.f;
+    if (y >= by0 && y < by1) {
+      const float dyf = static_cast<float>(y) - posy;
+      p1 = xop(s_tab, stp * dyf * dyf, expfac);
+    }
+    pre1[r][lane] = p1;
+  }
+  return R;
+}
+
 // Work items of a launch: items[0, n), or with device counts (dc: the pass's
 // item total and long-list total) the items after the first min(dc[1], cap)
 // (those go to the producer/consumer kernel).
@@ -397,47 +454,8 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
     float va0 = 0.f, va1 = 0.f, va2 = 0.f, vq0 = 0.f, vq1 = 0.f, vq2 = 0.f, vmax = 0.f;
     bool vfast = false;
     if (lane < nb) {
-      const Rec R = a.recs[a.ent[base + lane]];
-      const float rfacr = R.r * a.rfac;
-      const float posx = R.x - static_cast<float>(x0s), posy = R.y - static_cast<float>(y0s);
-      const int bx0 = max(cvt_i32(posx - rfacr + 1), 0), bx1 = min(cvt_i32(posx + rfacr + 1), w);
-      const int by0 = max(cvt_i32(posy - rfacr + 1), 0), by1 = min(cvt_i32(posy + rfacr + 1), h);
-      const float radsq = rfacr * rfacr;
-      const float sigma = a.h2sigma * R.r;
-      const float stp = -1.f / (sigma * sigma);
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const int x = bxo + c;
-        uint32_t lohi = 0;
-        float p2 = 0.f;
-        if (x >= bx0 && x < bx1) {
-          const float dxsq = (static_cast<float>(x) - posx) * (static_cast<float>(x) - posx);
-          const float dy = sqrtf(radsq - dxsq);
-          const int lo = max(by0, cvt_i32(posy - dy + 1));
-          const int hi = min(by1, cvt_i32(posy + dy + 1));
-          if (hi > lo) {
-            if (VAR >= 3) {  // bit r: block row byo + r lies in [lo, hi)
-              const int r0 = min(max(lo - byo, 0), 8), r1 = min(max(hi - byo, 0), 8);
-              lohi = (r1 > r0) ? (((1u << r1) - 1u) & ~((1u << r0) - 1u)) : 0u;
-            } else {
-              lohi = static_cast<uint32_t>(lo) | (static_cast<uint32_t>(hi) << 16);
-            }
-          }
-          p2 = xop(s_tab, stp * dxsq, expfac);
-        }
-        col[c][lane] = lohi;
-        pre2[c][lane] = p2;
-      }
-#pragma unroll
-      for (int r = 0; r < 8; ++r) {
-        const int y = byo + r;
-        float p1 = 0.f;
-        if (y >= by0 && y < by1) {
-          const float dyf = static_cast<float>(y) - posy;
-          p1 = xop(s_tab, stp * dyf * dyf, expfac);
-        }
-        pre1[r][lane] = p1;
-      }
+      const Rec R = tabulate<VAR >= 3>(a, base + lane, lane, x0s, y0s, w, h, bxo, byo, s_tab,
+                                       col, pre2, pre1, expfac);
       va0 = -R.e0;
       va1 = -R.e1;
       va2 = -R.e2;
@@ -988,6 +1006,201 @@ __global__ __launch_bounds__(kPThreads) void k_splat_pc(SplatArgs a, const uint3
   }
 }
 
+// ---------------------------------------------------------------------------
+// Staged long lists (additive pass).  An additive pixel's chain is one add per
+// particle, l_c = l_c + t_c with t_c = a_c * att (splotch_host.cc:615), so a
+// long list is latency-bound in a lane-per-pixel wave: the per-particle table
+// work, not the add, sets its pace.  The staged path moves that work off the
+// chain: k_stage_ops (every CU) writes each staged list's operands t_c per
+// pixel to HBM, and k_stage_chain runs one wave per (list, channel) that only
+// streams its plane and adds, in list order.  A pixel outside a particle's
+// footprint gets t = +0 (l + 0 == l: l starts at +0 and a float sum is -0
+// only if both terms are -0), so every pixel sees the reference's sums and
+// rounding.
+//
+// Operand layout: the lists' entries in groups of 4 (a list's last group
+// padded with zero operands); list i owns groups [cbase[i], cbase[i+1]), and
+// group g of channel c is 64 float4 (lane = pixel cx * 8 + cy, components =
+// the group's 4 entries) at ops + (g * 3 + c) * 64.
+
+// List offsets of the staged items (one workgroup): cbase (groups of 4
+// entries) and cbat (batches of 64 entries), exclusive, totals at [n].
+__global__ __launch_bounds__(256) void k_stage_base(SplatArgs a, const uint32_t *items,
+                                                    uint32_t n, uint32_t *cbase, uint32_t *cbat) {
+  __shared__ uint32_t s_g[256], s_b[256];
+  __shared__ uint32_t s_carry[2];
+  const int tid = threadIdx.x;
+  if (tid == 0) s_carry[0] = s_carry[1] = 0;
+  __syncthreads();
+  for (uint32_t i0 = 0; i0 < n; i0 += 256) {
+    const uint32_t i = i0 + tid;
+    uint32_t len = 0;
+    if (i < n) {
+      const uint32_t u = items[i] >> 2;
+      len = a.bin_off[u + 1] - a.bin_off[u];
+    }
+    const uint32_t g = (len + 3) / 4, b = (len + 63) / 64;
+    s_g[tid] = g;
+    s_b[tid] = b;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {  // inclusive scan (Hillis-Steele)
+      const uint32_t vg = tid >= d ? s_g[tid - d] : 0u, vb = tid >= d ? s_b[tid - d] : 0u;
+      __syncthreads();
+      s_g[tid] += vg;
+      s_b[tid] += vb;
+      __syncthreads();
+    }
+    if (i < n) {
+      cbase[i] = s_carry[0] + s_g[tid] - g;
+      cbat[i] = s_carry[1] + s_b[tid] - b;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      s_carry[0] += s_g[255];
+      s_carry[1] += s_b[255];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    cbase[n] = s_carry[0];
+    cbat[n] = s_carry[1];
+  }
+}
+
+// Operands of the staged lists: the batches of 64 entries (cbat[n] of them)
+// split evenly over the launch's waves, each wave walking a contiguous run
+// (phase A as the 8x8 splat's, then lane = pixel for the batch's groups).
+__global__ __launch_bounds__(256) void k_stage_ops(SplatArgs a, const uint32_t *items, uint32_t n,
+                                                   const uint32_t *cbase, const uint32_t *cbat,
+                                                   float4 *ops) {
+  __shared__ float s_tab[2048];
+  __shared__ uint32_t s_col[4][8][kPad];
+  __shared__ float s_pre2[4][8][kPad];
+  __shared__ float s_pre1[4][8][kPad];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int i = tid; i < 2048; i += 256) s_tab[i] = a.exptab[i];
+  __syncthreads();
+  const uint32_t total = uniform(cbat[n]);
+  const uint32_t nw = gridDim.x * 4u, w = blockIdx.x * 4u + static_cast<uint32_t>(wave);
+  const uint32_t span = (total + nw - 1) / nw;
+  uint32_t gb = w * span;
+  const uint32_t gend = min(total, gb + span);
+  if (gb >= gend) return;
+  // the staged item holding batch gb: cbat[lo] <= gb < cbat[lo + 1]
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (uniform(cbat[mid]) <= gb) lo = mid;
+    else hi = mid;
+  }
+  uint32_t (*col)[kPad] = s_col[wave];
+  float(*pre2)[kPad] = s_pre2[wave];
+  float(*pre1)[kPad] = s_pre1[wave];
+  const float expfac = a.expfac;
+  const int cx = lane >> 3, cy = lane & 7;
+  uint32_t i = lo, b = gb - uniform(cbat[lo]);
+  for (; gb < gend; ++gb, ++b) {
+    while (gb >= uniform(cbat[i + 1])) {  // next list (uniform)
+      ++i;
+      b = 0;
+    }
+    const uint32_t u = uniform(items[i]) >> 2;
+    const UnitGeom ug = unit_geom_uniform(a, u);
+    const uint32_t e0 = uniform(a.bin_off[u]), e1 = uniform(a.bin_off[u + 1]);
+    const uint32_t base = e0 + 64u * b;
+    const int nb = static_cast<int>(uniform(min<uint32_t>(64u, e1 - base)));
+    float va0 = 0.f, va1 = 0.f, va2 = 0.f;
+    if (lane < nb) {
+      const Rec R = tabulate<true>(a, base + lane, lane, ug.x0s, ug.y0s, ug.w, ug.h, ug.gx * 8,
+                                   ug.gy * 8, s_tab, col, pre2, pre1, expfac);
+      va0 = -R.e0;
+      va1 = -R.e1;
+      va2 = -R.e2;
+    }
+    wave_sync();
+    float4 *dst = ops + (static_cast<size_t>(uniform(cbase[i])) + 16u * b) * 3u * 64u + lane;
+    const int ng = (nb + 3) / 4;
+    for (int q = 0; q < ng; ++q) {
+      float t0[4], t1[4], t2[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int k0 = 4 * q + j, k = min(k0, nb - 1);  // uniform; slots past the list are 0
+        const bool in = k0 < nb && ((col[cx][k] >> cy) & 1u);
+        const float at = pre1[cy][k] * pre2[cx][k];
+        const float x0 = lane_f(va0, k) * at, x1 = lane_f(va1, k) * at, x2 = lane_f(va2, k) * at;
+        t0[j] = in ? x0 : 0.f;
+        t1[j] = in ? x1 : 0.f;
+        t2[j] = in ? x2 : 0.f;
+      }
+      float4 *g = dst + static_cast<size_t>(q) * 3u * 64u;
+      g[0] = make_float4(t0[0], t0[1], t0[2], t0[3]);
+      g[64] = make_float4(t1[0], t1[1], t1[2], t1[3]);
+      g[128] = make_float4(t2[0], t2[1], t2[2], t2[3]);
+    }
+    wave_sync();
+  }
+}
+
+// The chains of the staged lists: workgroup 3 i + c applies list i's channel c
+// to its block's 64 pixels (lane = pixel), kR groups of loads in flight.
+constexpr int kR = 48;
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_stage_chain(
+    SplatArgs a, const uint32_t *items,
+                                                    const uint32_t *cbase, const float4 *ops) {
+  const uint32_t i = blockIdx.x / 3u;
+  const int c = static_cast<int>(blockIdx.x - 3u * i);
+  const int lane = threadIdx.x;
+  const uint64_t t_start = a.trace ? __builtin_amdgcn_s_memrealtime() : 0;
+  __builtin_amdgcn_s_setprio(3);
+  const uint32_t u = uniform(items[i]) >> 2;
+  const UnitGeom ug = unit_geom_uniform(a, u);
+  const uint32_t len = uniform(a.bin_off[u + 1]) - uniform(a.bin_off[u]);
+  const uint32_t G = (len + 3) / 4;  // a staged list is not empty
+  const int cx = lane >> 3, cy = lane & 7;
+  const int lx = ug.gx * 8 + cx, ly = ug.gy * 8 + cy;
+  const bool pix_in = lx < min(ug.gx * 8 + 8, ug.w) && ly < min(ug.gy * 8 + 8, ug.h);
+  float *d = ug.pic + (static_cast<int64_t>(ug.x0s + lx) * ug.yres + (ug.y0s + ly)) * 3 + c;
+  float l = 0.f;
+  if (ug.accumulate == 2 && pix_in) l = *d;  // a later chunk: continue the chain
+  // l is in its register before the operand loads are issued: a load still
+  // in flight at the loop would make the compiler wait for every load there
+  asm volatile("" : "+v"(l));
+  const float4 *src = ops + static_cast<size_t>(uniform(cbase[i])) * 3u * 64u +
+                      static_cast<size_t>(c) * 64u + lane;
+  float4 buf[kR];
+#pragma unroll
+  for (int r = 0; r < kR; ++r) {  // issued in the order they are consumed
+    buf[r] = src[static_cast<size_t>(min<uint32_t>(r, G - 1)) * 192u];
+    asm volatile("" ::: "memory");
+  }
+  for (uint32_t q = 0; q < G; q += kR) {
+#pragma unroll
+    for (int r = 0; r < kR; ++r) {
+      const float4 v = buf[r];
+      if (q + r < G) {
+        l = l + v.x;
+        l = l + v.y;
+        l = l + v.z;
+        l = l + v.w;
+      }
+      // then the next round's group into the same registers (re-reads the
+      // last group past the list end); the barrier orders the sums before
+      // the load and keeps the load from sinking to its use a round later,
+      // so kR groups stay in flight
+      asm volatile("" : "+v"(l) : : "memory");
+      buf[r] = src[static_cast<size_t>(min<uint32_t>(q + kR + r, G - 1)) * 192u];
+    }
+  }
+  if (a.trace && c == 0 && lane == 0) {
+    uint64_t *tr = a.trace + 4 * static_cast<uint64_t>(i);
+    tr[0] = t_start;
+    tr[1] = __builtin_amdgcn_s_memrealtime();
+    tr[2] = static_cast<uint64_t>(len) | (uint64_t(3) << 32);
+    tr[3] = static_cast<uint64_t>(__smid());
+  }
+  if (pix_in) *d = ug.accumulate == 1 ? *d + l : l;
+}
+
 // Work items per unit: none for units outside their tile; none for empty units
 // when accumulating; one producer/consumer item for lists of at least
 // heavy_len; otherwise three (one per channel) for long ordered lists, else one.
@@ -999,24 +1212,35 @@ __device__ __forceinline__ uint32_t heavy_len_of(const SplatArgs &a) {
   return (a.qbad && *a.qbad) ? 0xffffffffu : a.heavy_len;
 }
 
+// Long lists (one item each, sorted first): staged lists, then the
+// producer/consumer kernel's.
+__device__ __forceinline__ uint32_t long_len_of(const SplatArgs &a) {
+  return min(heavy_len_of(a), a.stage_len);
+}
+
 __device__ __forceinline__ uint32_t unit_items(const SplatArgs &a, uint32_t u, int ordered,
                                                uint32_t split, uint32_t &len) {
   len = a.bin_off[u + 1] - a.bin_off[u];
   const UnitGeom g = unit_geom(a, u);
   if (g.gx * 8 >= g.w || g.gy * 8 >= g.h) return 0;
   if (len == 0) return g.need_empty ? 1u : 0u;
-  if (len >= heavy_len_of(a)) return 1u;  // one producer/consumer workgroup
+  if (len >= long_len_of(a)) return 1u;  // staged, or one producer/consumer workgroup
   return (len >= (ordered ? split : 2 * split)) ? 3u : 1u;
 }
 
 __global__ __launch_bounds__(256) void k_items_count(SplatArgs a, uint32_t nunits, int ordered,
                                                      uint32_t split, uint32_t *cnt,
-                                                     uint32_t *heavy_cnt) {
+                                                     uint32_t *heavy_cnt, uint32_t *stage_cnt,
+                                                     uint32_t *stage_grp) {
   for (int64_t u = gtid(); u < nunits; u += gstride()) {
     uint32_t len;
     const uint32_t k = unit_items(a, static_cast<uint32_t>(u), ordered, split, len);
     cnt[u] = k;
-    if (heavy_cnt && k > 0 && len >= heavy_len_of(a)) atomicAdd(heavy_cnt, k);
+    if (heavy_cnt && k > 0 && len >= long_len_of(a)) atomicAdd(heavy_cnt, k);
+    if (stage_cnt && k > 0 && len >= a.stage_len) {
+      atomicAdd(stage_cnt, 1u);
+      atomicAdd(stage_grp, (len + 3u) / 4u);
+    }
   }
 }
 
@@ -1045,11 +1269,12 @@ unsigned grid_for(int64_t n) {
 
 void launch_items_count(const uint32_t *bin_off, uint32_t nunits, const SplatArgs &a,
                         int ordered, uint32_t split, uint32_t *cnt, uint32_t *heavy_cnt,
-                        hipStream_t s) {
+                        hipStream_t s, uint32_t *stage_cnt, uint32_t *stage_grp) {
   if (nunits == 0) return;
   SplatArgs b = a;
   b.bin_off = bin_off;
-  k_items_count<<<grid_for(nunits), 256, 0, s>>>(b, nunits, ordered, split, cnt, heavy_cnt);
+  k_items_count<<<grid_for(nunits), 256, 0, s>>>(b, nunits, ordered, split, cnt, heavy_cnt,
+                                                 stage_cnt, stage_grp);
   hip_check(hipGetLastError(), "k_items_count launch");
 }
 
@@ -1061,6 +1286,25 @@ void launch_items_emit(const uint32_t *bin_off, uint32_t nunits, const SplatArgs
   b.bin_off = bin_off;
   k_items_emit<<<grid_for(nunits), 256, 0, s>>>(b, nunits, ordered, split, off, keys, vals);
   hip_check(hipGetLastError(), "k_items_emit launch");
+}
+
+void launch_stage(const uint32_t *items, uint32_t n, uint64_t groups, uint32_t *cbase,
+                  uint32_t *cbat, float4 *ops, const SplatArgs &a, hipStream_t s) {
+  if (n == 0) return;
+  k_stage_base<<<1, 256, 0, s>>>(a, items, n, cbase, cbat);
+  hip_check(hipGetLastError(), "k_stage_base launch");
+  // batches: at most groups / 16 + n; about four per wave when there are many
+  const uint64_t nbat = groups / 16 + n;
+  const unsigned grid = static_cast<unsigned>(std::min<uint64_t>(2048, (nbat + 15) / 16));
+  k_stage_ops<<<std::max(grid, 1u), 256, 0, s>>>(a, items, n, cbase, cbat, ops);
+  hip_check(hipGetLastError(), "k_stage_ops launch");
+}
+
+void launch_stage_chain(const uint32_t *items, uint32_t n, const uint32_t *cbase,
+                        const float4 *ops, const SplatArgs &a, hipStream_t s) {
+  if (n == 0) return;
+  k_stage_chain<<<3 * n, 64, 0, s>>>(a, items, cbase, ops);
+  hip_check(hipGetLastError(), "k_stage_chain launch");
 }
 
 void launch_splat_pc(bool additive, const uint32_t *items, uint32_t npc, const SplatArgs &a,

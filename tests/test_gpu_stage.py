@@ -1,0 +1,202 @@
+"""GPU parity of the staged long lists (additive pass), through the C-ABI.
+
+A list from hip_stage entries on has its per-pixel terms computed across the
+GPU first (k_stage_ops), then summed in list order by one wave per channel
+(k_stage_chain).  Bar: bit-exact against the genuine fixtures and the
+oracle, and against the unstaged render, with every list staged, with staged
+lists beside producer/consumer workgroups and normal items, in chunked
+passes (chains continued from the image), on top of an ordered pass (scratch
+image, second stream), in batches of views, and with non-finite particle
+intensities (NaN in exactly the oracle's pixels).
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from helpers import assert_bitexact, field_from, view_from
+from splotch_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def log(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def R():
+    import splotch_amd as S
+    assert torch.cuda.is_available(), "GPU test without a GPU"
+    r = S.Renderer(0)
+    yield r
+    torch.cuda.synchronize()
+    r.close()
+
+
+def render(R, field, view, pals, prm):
+    import splotch_amd as S
+    for t, (x, rgb) in enumerate(pals):
+        R.set_palette(t, x, rgb)
+    cam = S.camera_setup(view)
+    dev = S.to_device(field)
+    pic = torch.empty(view.xres * view.yres * 3, dtype=torch.float32, device="cuda")
+    R.render(dev, cam, prm, pic)
+    torch.cuda.synchronize()
+    return pic.cpu().numpy().reshape(view.xres, view.yres, 3)
+
+
+@pytest.mark.parametrize("T", [50, 16, 8, 64])
+def test_staged_fixtures(R, golden, T):
+    """Every non-empty block staged (hip_stage 1), against the genuine build's
+    raw image, for each reference tile size (blocks cut at tile edges)."""
+    import splotch_amd as S
+    log("case staged fixture T", T)
+    f = golden("additive_small")
+    p = golden("palette")
+    pic = render(R, field_from(f), view_from(f), [(p["x"], p["rgb"])],
+                 S.params(a_eq_e=True, tile_size=T, hip_stage=1))
+    assert_bitexact(pic, f["pic_T%d" % T], "staged fixture T=%d" % T)
+
+
+def test_staged_dense_cluster_vs_oracle(R):
+    """A dense cluster (lists of thousands of particles per 8x8 block): staged
+    at several thresholds, beside producer/consumer workgroups (hip_heavy
+    below hip_stage) and per-channel items, and unstaged."""
+    import splotch_amd as S
+    fld = synth.plummer_field(60_000, 113, halo_frac=1.0, n_halos=1, hsml_mu=math.log(0.8))
+    v = synth.default_view(160, 144)
+    pal = synth.default_palette()
+    opic, _ = O.host_rendering(fld, v, [pal], a_eq_e=True)
+    for stage, heavy, hmax, split, var in ((1, 0, 0, 0, 0), (64, 0, 0, 0, 0),
+                                           (256, 64, 0, 0, 0), (1024, 16, 3, 32, 0),
+                                           (4096, -1, 0, 16, 3), (256, -1, 0, 0, 1),
+                                           (-1, 0, 0, 0, 0), (0, 0, 0, 0, 0)):
+        log("case dense cluster staged", stage, heavy, hmax, split, var)
+        pic = render(R, fld, v, [pal], S.params(a_eq_e=True, hip_stage=stage, hip_heavy=heavy,
+                                                hip_heavy_max=hmax, hip_split=split,
+                                                hip_variant=var))
+        assert_bitexact(pic, opic, "dense cluster stage=%d heavy=%d max=%d split=%d var=%d"
+                        % (stage, heavy, hmax, split, var))
+
+
+def test_staged_trace_and_profile(R):
+    """The work-item trace has a record per staged list (all three channels,
+    its length) and the additive heavy stage a time; the image is unchanged."""
+    import splotch_amd as S
+    fld = synth.plummer_field(60_000, 114, halo_frac=1.0, n_halos=1, hsml_mu=math.log(0.8))
+    v = synth.default_view(160, 144)
+    pal = synth.default_palette()
+    prm = S.params(a_eq_e=True, hip_stage=512)
+    want = render(R, fld, v, [pal], S.params(a_eq_e=True, hip_stage=-1))
+    R.set_profiling(True)
+    R.set_trace(True)
+    try:
+        got = render(R, fld, v, [pal], prm)
+        st, _ = R.stage_times()
+        tr = R.trace_items(False)
+    finally:
+        R.set_trace(False)
+        R.set_profiling(False)
+    assert_bitexact(got, want, "traced staged frame")
+    staged = tr[(tr[:, 2] & 0xffffffff) >= 512]
+    log("staged lists traced: %d, stages %s" % (len(staged), st))
+    assert len(staged) > 0
+    assert np.all((staged[:, 2] >> 32) == 3)
+    assert np.all(tr[:, 1] >= tr[:, 0])
+    assert st["additive_k_splat_heavy"] > 0
+
+
+@pytest.mark.parametrize("chunk", [20_000, 3_000])
+def test_staged_chunked_passes(R, chunk):
+    """Chunked passes continue each staged chain from the image (and, for the
+    additive pass on top of an ordered one, from a scratch image): equal to
+    the unchunked, unstaged render."""
+    import splotch_amd as S
+    log("case staged chunked", chunk)
+    gas = synth.plummer_field(30_000, 1500, ptype=0, hsml_mu=math.log(0.8))
+    stars = synth.plummer_field(30_000, 1501, ptype=1, hsml_mu=math.log(0.8))
+    fld = synth.concat(gas, stars)
+    v = synth.default_view(192, 160)
+    pal = synth.default_palette()
+    for aeq in ([0, 1], [1, 1], [2, 1]):
+        want = render(R, fld, v, [pal, pal], S.params(ptypes=2, a_eq_e=aeq,
+                                                      brightness=[0.5, 2.0], hip_stage=-1))
+        for stage in (32, 300):
+            got = render(R, fld, v, [pal, pal],
+                         S.params(ptypes=2, a_eq_e=aeq, brightness=[0.5, 2.0], hip_stage=stage,
+                                  hip_chunk_entries=chunk))
+            _, counts = R.stage_times()
+            assert counts["additive_entries"] > chunk
+            assert_bitexact(got, want, "staged chunked %s stage %d" % (aeq, stage))
+            got = render(R, fld, v, [pal, pal],
+                         S.params(ptypes=2, a_eq_e=aeq, brightness=[0.5, 2.0], hip_stage=stage))
+            assert_bitexact(got, want, "staged %s stage %d" % (aeq, stage))
+
+
+def test_staged_batched_views(R):
+    """A batch of additive and ordered views of several image sizes and tile
+    sizes: with staged lists every image equals its unstaged plain render."""
+    import splotch_amd as S
+    pal = synth.default_palette()
+    for t in range(2):
+        R.set_palette(t, *pal)
+    jobs = synth.batch_jobs(8, seed=2900, n_min=2e4, n_max=8e4)
+    for bj in jobs:
+        bj.view.xres = bj.view.yres = bj.view.xres // 4
+    tiles = [50, 37, 16, 64, 50, 24, 50, 8]
+    views, plain = [], []
+    for bj, T in zip(jobs, tiles):
+        dev = S.to_device(bj.field)
+        cam = S.camera_setup(bj.view)
+        n = bj.view.xres * bj.view.yres * 3
+        p = torch.empty(n, dtype=torch.float32, device="cuda")
+        R.render(dev, cam, S.params(a_eq_e=bj.additive, tile_size=T, hip_stage=-1), p)
+        plain.append(p)
+        views.append((dev, cam, S.params(a_eq_e=bj.additive, tile_size=T, hip_stage=64),
+                      torch.full((n,), float("nan"), dtype=torch.float32, device="cuda")))
+    log("staged batch:", [(bj.field.n, bj.view.xres, bj.additive) for bj in jobs])
+    assert any(bj.additive for bj in jobs)
+    R.render_batched(views)
+    torch.cuda.synchronize()
+    for j, (v, p) in enumerate(zip(views, plain)):
+        assert_bitexact(v[3].cpu().numpy(), p.cpu().numpy(), "staged batch view %d" % j)
+
+
+def test_staged_non_finite(R):
+    """Particles with infinite or NaN intensity: an infinite term meets a
+    zero Gaussian factor inside the footprint (NaN) and nothing outside it;
+    the staged sums have NaN in exactly the oracle's pixels and every other
+    value bit for bit."""
+    import splotch_amd as S
+    fld = synth.plummer_field(20_000, 1630, halo_frac=1.0, n_halos=1, hsml_mu=math.log(0.8))
+    rng = np.random.default_rng(1631)
+    bad = rng.choice(fld.n, size=40, replace=False)
+    fld.inten[bad[:20]] = np.inf
+    fld.inten[bad[20:]] = np.nan
+    v = synth.default_view(160, 144)
+    pal = synth.default_palette()
+    opic, _ = O.host_rendering(fld, v, [pal], a_eq_e=True)
+    pic = render(R, fld, v, [pal], S.params(a_eq_e=True, hip_stage=16))
+    on = np.isnan(opic)
+    log("NaN channels: oracle %d of %d" % (int(on.sum()), on.size))
+    assert on.any()
+    assert np.array_equal(np.isnan(pic), on), "NaN pixels differ"
+    assert_bitexact(pic[~on], opic[~on], "finite pixels, staged non-finite intensities")
+
+
+def test_staged_deterministic(R):
+    import splotch_amd as S
+    fld = synth.plummer_field(150_000, 1640, halo_frac=0.9, n_halos=2)
+    v = synth.default_view(256, 224)
+    pal = synth.default_palette()
+    a = render(R, fld, v, [pal], S.params(a_eq_e=True, hip_stage=128))
+    b = render(R, fld, v, [pal], S.params(a_eq_e=True, hip_stage=128))
+    c = render(R, fld, v, [pal], S.params(a_eq_e=True, hip_stage=-1))
+    assert_bitexact(a, b, "staged repeat")
+    assert_bitexact(a, c, "staged vs unstaged")
