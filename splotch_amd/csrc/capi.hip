@@ -417,7 +417,7 @@ struct Plan {
   // per pass
   uint32_t m[2] = {0, 0}, e[2] = {0, 0}, nitems[2] = {0, 0}, nheavy[2] = {0, 0};
   // staged lists (the first nstage items; nheavy counts them too) and their
-  // groups of 4 entries
+  // groups of 4 entries (padded to whole chain loads)
   uint32_t nstage[2] = {0, 0};
   uint64_t sgroups[2] = {0, 0};
   uint32_t e16[2] = {0, 0};         // (region, particle) entries
@@ -626,7 +626,7 @@ void make_plan(splhip_ctx *c, int nj, const JobIn *jobs, int G, Plan &pl, hipStr
 // [4..5] work items, [6..7] heavy items, [8..11] bin entries as u64,
 // [12..13] region entries, [14] the ordered pass has a non-finite q,
 // [15] capacity overflow bits of a device-count render (bit p: pass p),
-// [16..17] staged lists, [18..19] their groups of 4 entries.
+// [16..17] staged lists, [18..19] their groups of 4 entries (padded).
 enum {
   T_M = 0, T_E = 2, T_ITEMS = 4, T_HEAVY = 6, T_E64 = 8, T_E16 = 12, T_QBAD = 14, T_OVF = 15,
   T_STAGE = 16, T_SGRP = 18, T_N = 20

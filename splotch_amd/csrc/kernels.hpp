@@ -149,7 +149,7 @@ void launch_unpack_records(const char *stage, int64_t n, int64_t C, int ncol, fl
 // channel 3 = all), heaviest first.  heavy_cnt (device, optional): incremented
 // once per long unit (list at least min(a.heavy_len, a.stage_len)); stage_cnt /
 // stage_grp (optional): once per unit whose list reaches a.stage_len, and by
-// its groups of 4 entries.
+// its groups of 4 entries (padded to whole chain loads).
 void launch_items_count(const uint32_t *bin_off, uint32_t nunits, const SplatArgs &a,
                         int ordered, uint32_t split, uint32_t *cnt, uint32_t *heavy_cnt,
                         hipStream_t s, uint32_t *stage_cnt = nullptr,
@@ -172,7 +172,7 @@ void launch_splat8(bool additive, int variant, const uint32_t *items, uint32_t n
                    const SplatArgs &a, hipStream_t s, const ItemCounts &dc = ItemCounts{},
                    int ppw = 0);
 // Staged lists (splat8.hip), the first n items of an additive pass (groups:
-// their groups of 4 entries): list offsets into cbase / cbat ([n + 1] each)
+// their groups of 4 entries, padded): list offsets into cbase / cbat ([n + 1] each)
 // and the per-pixel terms into ops (groups * 3 * 64 float4); then the chains,
 // one wave per list and channel, which write the lists' blocks.
 void launch_stage(const uint32_t *items, uint32_t n, uint64_t groups, uint32_t *cbase,

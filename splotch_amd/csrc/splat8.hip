@@ -1018,10 +1018,15 @@ __global__ __launch_bounds__(kPThreads) void k_splat_pc(SplatArgs a, const uint3
 // only if both terms are -0), so every pixel sees the reference's sums and
 // rounding.
 //
-// Operand layout: the lists' entries in groups of 4 (a list's last group
-// padded with zero operands); list i owns groups [cbase[i], cbase[i+1]), and
-// group g of channel c is 64 float4 (lane = pixel cx * 8 + cy, components =
-// the group's 4 entries) at ops + (g * 3 + c) * 64.
+// Operand layout: the lists' entries in groups of 4, a list's groups padded
+// with zero operands to a multiple of kStageLoad (the groups of one load of
+// the chain kernel); list i owns groups [cbase[i], cbase[i+1]), and group g
+// of channel c is 64 float4 (lane = pixel cx * 8 + cy, components = the
+// group's 4 entries) at ops + (g * 3 + c) * 64.
+constexpr int kStageLoad = 4;
+__host__ __device__ constexpr uint32_t stage_groups(uint32_t len) {
+  return (len + 4 * kStageLoad - 1) / (4 * kStageLoad) * kStageLoad;
+}
 
 // List offsets of the staged items (one workgroup): cbase (groups of 4
 // entries) and cbat (batches of 64 entries), exclusive, totals at [n].
@@ -1039,7 +1044,7 @@ __global__ __launch_bounds__(256) void k_stage_base(SplatArgs a, const uint32_t 
       const uint32_t u = items[i] >> 2;
       len = a.bin_off[u + 1] - a.bin_off[u];
     }
-    const uint32_t g = (len + 3) / 4, b = (len + 63) / 64;
+    const uint32_t g = stage_groups(len), b = (len + 63) / 64;
     s_g[tid] = g;
     s_b[tid] = b;
     __syncthreads();
@@ -1119,7 +1124,7 @@ __global__ __launch_bounds__(256) void k_stage_ops(SplatArgs a, const uint32_t *
     }
     wave_sync();
     float4 *dst = ops + (static_cast<size_t>(uniform(cbase[i])) + 16u * b) * 3u * 64u + lane;
-    const int ng = (nb + 3) / 4;
+    const int ng = static_cast<int>(stage_groups(nb));  // 64 entries: whole loads
     for (int q = 0; q < ng; ++q) {
       float t0[4], t1[4], t2[4];
 #pragma unroll
@@ -1141,57 +1146,101 @@ __global__ __launch_bounds__(256) void k_stage_ops(SplatArgs a, const uint32_t *
   }
 }
 
-// The chains of the staged lists: workgroup 3 i + c applies list i's channel c
-// to its block's 64 pixels (lane = pixel), kR groups of loads in flight.
-constexpr int kR = 48;
+// The chains of the staged lists.  A wave applies channel c of list i to a
+// slice of kPix pixels of the block: workgroup (3 i + c) kS + h takes pixels
+// [h kPix, h kPix + kPix).  One load instruction brings kS consecutive groups
+// (lane quarter q: group g + q of the slice's pixels, 16 bytes per lane), so
+// the kR loads in flight per wave cover kR kS groups; lanes [0, kPix) run the
+// chains and read the other quarters' groups back through LDS.  A chain's
+// wave is bound by the loads it has in flight, not by its adds, so narrower
+// slices (more waves per list) move more bytes per list at once.  Lists are
+// padded with zero groups to a multiple of kS groups.
+constexpr int kS = kStageLoad, kPix = 64 / kS, kR = 40;
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_stage_chain(
-    SplatArgs a, const uint32_t *items,
-                                                    const uint32_t *cbase, const float4 *ops) {
-  const uint32_t i = blockIdx.x / 3u;
-  const int c = static_cast<int>(blockIdx.x - 3u * i);
-  const int lane = threadIdx.x;
+    SplatArgs a, const uint32_t *items, const uint32_t *cbase, const float4 *ops) {
+  __shared__ float4 xs[64];
+  const uint32_t i = blockIdx.x / (3u * kS);
+  const uint32_t r3 = blockIdx.x - i * (3u * kS);
+  const int c = static_cast<int>(r3 / kS), h = static_cast<int>(r3 % kS);
+  const int lane = threadIdx.x, qo = lane / kPix, pl = lane % kPix;
+  const int px = h * kPix + pl;  // pixel cx * 8 + cy
   const uint64_t t_start = a.trace ? __builtin_amdgcn_s_memrealtime() : 0;
   __builtin_amdgcn_s_setprio(3);
   const uint32_t u = uniform(items[i]) >> 2;
   const UnitGeom ug = unit_geom_uniform(a, u);
   const uint32_t len = uniform(a.bin_off[u + 1]) - uniform(a.bin_off[u]);
-  const uint32_t G = (len + 3) / 4;  // a staged list is not empty
-  const int cx = lane >> 3, cy = lane & 7;
+  const uint32_t G = stage_groups(len);  // the list is not empty
+  const int cx = px >> 3, cy = px & 7;
   const int lx = ug.gx * 8 + cx, ly = ug.gy * 8 + cy;
-  const bool pix_in = lx < min(ug.gx * 8 + 8, ug.w) && ly < min(ug.gy * 8 + 8, ug.h);
+  const bool pix_in = lane < kPix && lx < min(ug.gx * 8 + 8, ug.w) &&
+                      ly < min(ug.gy * 8 + 8, ug.h);
   float *d = ug.pic + (static_cast<int64_t>(ug.x0s + lx) * ug.yres + (ug.y0s + ly)) * 3 + c;
   float l = 0.f;
   if (ug.accumulate == 2 && pix_in) l = *d;  // a later chunk: continue the chain
   // l is in its register before the operand loads are issued: a load still
   // in flight at the loop would make the compiler wait for every load there
   asm volatile("" : "+v"(l));
+  // group g of this channel and pixel: src[g * 192]; lane quarter qo reads g + qo
   const float4 *src = ops + static_cast<size_t>(uniform(cbase[i])) * 3u * 64u +
-                      static_cast<size_t>(c) * 64u + lane;
+                      static_cast<size_t>(c) * 64u + px;
+  auto group = [&](uint32_t g0) {  // groups [g0, g0 + kS), the last ones again past the list
+    return src[static_cast<size_t>(min<uint32_t>(g0, G - kS) + qo) * 192u];
+  };
+  // one load's 16 entries for lanes [0, kPix): its own quarter, then the
+  // other three read back through LDS
+  struct Q {
+    float4 a, b, c, d;
+  };
+  auto exchange = [&](const float4 &v) {
+    xs[lane] = v;
+    wave_sync();
+    return Q{v, xs[pl + kPix], xs[pl + 2 * kPix], xs[pl + 3 * kPix]};
+  };
+  auto add16 = [&](const Q &q) {
+    l = l + q.a.x;
+    l = l + q.a.y;
+    l = l + q.a.z;
+    l = l + q.a.w;
+    l = l + q.b.x;
+    l = l + q.b.y;
+    l = l + q.b.z;
+    l = l + q.b.w;
+    l = l + q.c.x;
+    l = l + q.c.y;
+    l = l + q.c.z;
+    l = l + q.c.w;
+    l = l + q.d.x;
+    l = l + q.d.y;
+    l = l + q.d.z;
+    l = l + q.d.w;
+  };
   float4 buf[kR];
 #pragma unroll
   for (int r = 0; r < kR; ++r) {  // issued in the order they are consumed
-    buf[r] = src[static_cast<size_t>(min<uint32_t>(r, G - 1)) * 192u];
+    buf[r] = group(r * kS);
     asm volatile("" ::: "memory");
   }
-  for (uint32_t q = 0; q < G; q += kR) {
+  // Software pipeline over the loads: step r of a round adds the entries of
+  // load r (exchanged a step earlier), exchanges load r + 1 meanwhile, and
+  // refills that load's registers with the groups kR loads further on.
+  Q cur = exchange(buf[0]);
+  asm volatile("" ::: "memory");
+  buf[0] = group(kR * kS);
+  for (uint32_t g = 0; g < G; g += kR * kS) {
 #pragma unroll
     for (int r = 0; r < kR; ++r) {
-      const float4 v = buf[r];
-      if (q + r < G) {
-        l = l + v.x;
-        l = l + v.y;
-        l = l + v.z;
-        l = l + v.w;
-      }
-      // then the next round's group into the same registers (re-reads the
-      // last group past the list end); the barrier orders the sums before
-      // the load and keeps the load from sinking to its use a round later,
-      // so kR groups stay in flight
+      const int rn = (r + 1) % kR;
+      const Q nxt = exchange(buf[rn]);
+      if (g + r * kS < G) add16(cur);
+      cur = nxt;
+      // the barrier orders the exchange and sums before the refill (and the
+      // next exchange's LDS write after these reads) and keeps the load from
+      // sinking to its use a round later, so kR loads stay in flight
       asm volatile("" : "+v"(l) : : "memory");
-      buf[r] = src[static_cast<size_t>(min<uint32_t>(q + kR + r, G - 1)) * 192u];
+      buf[rn] = group(g + (r + 1) * kS + kR * kS);
     }
   }
-  if (a.trace && c == 0 && lane == 0) {
+  if (a.trace && c == 0 && h == 0 && lane == 0) {
     uint64_t *tr = a.trace + 4 * static_cast<uint64_t>(i);
     tr[0] = t_start;
     tr[1] = __builtin_amdgcn_s_memrealtime();
@@ -1239,7 +1288,7 @@ __global__ __launch_bounds__(256) void k_items_count(SplatArgs a, uint32_t nunit
     if (heavy_cnt && k > 0 && len >= long_len_of(a)) atomicAdd(heavy_cnt, k);
     if (stage_cnt && k > 0 && len >= a.stage_len) {
       atomicAdd(stage_cnt, 1u);
-      atomicAdd(stage_grp, (len + 3u) / 4u);
+      atomicAdd(stage_grp, stage_groups(len));
     }
   }
 }
@@ -1303,7 +1352,7 @@ void launch_stage(const uint32_t *items, uint32_t n, uint64_t groups, uint32_t *
 void launch_stage_chain(const uint32_t *items, uint32_t n, const uint32_t *cbase,
                         const float4 *ops, const SplatArgs &a, hipStream_t s) {
   if (n == 0) return;
-  k_stage_chain<<<3 * n, 64, 0, s>>>(a, items, cbase, ops);
+  k_stage_chain<<<3 * kS * n, 64, 0, s>>>(a, items, cbase, ops);
   hip_check(hipGetLastError(), "k_stage_chain launch");
 }
 
