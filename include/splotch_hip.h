@@ -104,9 +104,15 @@ typedef struct {
                                               terms are computed across the GPU first, then
                                               summed in list order by one wave per channel
                                               (0 = auto: 8192, but none for a single view
-                                              with both passes, whose additive splat runs
-                                              beside the ordered pass; < 0 = never; no
-                                              staging in device-count renders) */
+                                              with both passes in one chunk, whose additive
+                                              splat runs beside the ordered pass; < 0 =
+                                              never; no staging in device-count renders) */
+  int32_t hip_stage_ordered;               /* hip_stage_ordered: the same for an ordered pass
+                                              (per-pixel operands of the compositing update
+                                              computed first; not when a particle's q_c is
+                                              non-finite) (0 = auto: 65536 for a pass of at
+                                              most 2^26 bin entries, else none; < 0 =
+                                              never) */
 } splhip_params;
 
 /* SoA particle fields (device pointers), post particle_normalize:

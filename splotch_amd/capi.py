@@ -43,7 +43,8 @@ class Params(C.Structure):
                 ("hip_variant", C.c_int32), ("hip_heavy", C.c_int32),
                 ("hip_heavy_max", C.c_int32), ("hip_chunk_entries", C.c_int32),
                 ("hip_autotune", C.c_int32), ("hip_bin_group", C.c_int32),
-                ("hip_ppw", C.c_int32), ("hip_stage", C.c_int32)]
+                ("hip_ppw", C.c_int32), ("hip_stage", C.c_int32),
+                ("hip_stage_ordered", C.c_int32)]
 
 
 class Post(C.Structure):
@@ -186,7 +187,7 @@ def camera_setup(view, zmin=0.0, zmax=1e23, projection=True, minrad_pix=1.0) -> 
 def params(ptypes=1, smooth_fac=None, brightness=None, col_vector=None, a_eq_e=True,
            sort_type=1, gray_absorption=0.2, tile_size=50, hip_region=0, hip_split=0, hip_variant=0,
            hip_heavy=0, hip_heavy_max=0, hip_chunk_entries=0, hip_autotune=0, hip_bin_group=0,
-           hip_ppw=0, hip_stage=0,
+           hip_ppw=0, hip_stage=0, hip_stage_ordered=0,
            b_brightness=1.0) -> Params:
     """Render parameters; brightness is brightness{t} (times b_brightness, as
     splotch_host.cc:262-263 does, in float32).  a_eq_e per type: True / 1
@@ -220,6 +221,7 @@ def params(ptypes=1, smooth_fac=None, brightness=None, col_vector=None, a_eq_e=T
     p.hip_bin_group = hip_bin_group
     p.hip_ppw = hip_ppw
     p.hip_stage = hip_stage
+    p.hip_stage_ordered = hip_stage_ordered
     return p
 
 

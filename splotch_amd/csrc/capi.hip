@@ -420,6 +420,7 @@ struct Plan {
   // groups of 4 entries (padded to whole chain loads)
   uint32_t nstage[2] = {0, 0};
   uint64_t sgroups[2] = {0, 0};
+  uint64_t etot[2] = {0, 0};  // (block, particle) entries of each pass (or chunk)
   uint32_t e16[2] = {0, 0};         // (region, particle) entries
   bool r16[2] = {false, false};     // the pass's region counts are scanned (e16 valid)
   uint64_t used16[2] = {0, 0};      // region entries of the bins built through regions
@@ -626,10 +627,11 @@ void make_plan(splhip_ctx *c, int nj, const JobIn *jobs, int G, Plan &pl, hipStr
 // [4..5] work items, [6..7] heavy items, [8..11] bin entries as u64,
 // [12..13] region entries, [14] the ordered pass has a non-finite q,
 // [15] capacity overflow bits of a device-count render (bit p: pass p),
-// [16..17] staged lists, [18..19] their groups of 4 entries (padded).
+// [16..17] staged lists, [18..19] their groups of 4 entries (padded),
+// [20..21] the longest list.
 enum {
   T_M = 0, T_E = 2, T_ITEMS = 4, T_HEAVY = 6, T_E64 = 8, T_E16 = 12, T_QBAD = 14, T_OVF = 15,
-  T_STAGE = 16, T_SGRP = 18, T_N = 20
+  T_STAGE = 16, T_SGRP = 18, T_LMAX = 20, T_N = 22
 };
 
 // Project all jobs, then the compaction scan of each present pass and its job
@@ -833,12 +835,25 @@ struct Tuning {
   int variant, ppw;
   uint32_t split, heavy, heavy_max;
   uint64_t chunk;  // most (block, particle) entries of one pass chunk
-  uint32_t stage;  // additive lists from this length on are staged (0xffffffff: none)
+  uint32_t stage, stage_ord;  // lists from this length on are staged, additive / ordered
+                              // pass (0xffffffff: none)
+  bool stage_ord_auto;        // ordered staging only for a pass bound by its longest list
 };
 
-// Staged operands of a pass: 3 KiB per group of 4 entries; a pass whose
-// staged lists need more renders them unstaged.
-constexpr uint64_t kStageBudget = uint64_t(8) << 30;
+// Staged operands of a pass (3 KiB per group of 4 entries additive, 4 KiB +
+// 48 B ordered); a pass whose staged lists need more renders them unstaged.
+constexpr uint64_t kStageBudget = uint64_t(32) << 30;
+// Default ordered staging only for a pass bound by its longest list: one
+// whose longest list, at about 108 clocks per entry in a producer/consumer
+// workgroup, takes at least twice as long as the pass's entries at about 0.25
+// clocks each over all CUs (the 8x8 splat's VALU work), i.e. longest list x
+// kStageOrderedRatio >= entries.  A pass below keeps every CU's VALU busy for
+// about as long as its longest lists take, and the operand kernel's extra
+// VALU and HBM work would only slow it (16M particles at 2048^2: 837k-entry
+// list, 301M entries, 43.3-53.7 ms staged from 786432-393216 entries on
+// against 43.7 ms unstaged; 64 views: 65.6M ordered entries, 68.6 -> 48.1 ms;
+// 500k at 512^2: 121k-entry list, 1.3M entries, 6.21 -> 3.58 ms).
+constexpr uint64_t kStageOrderedRatio = 216;
 
 Tuning tuning_of(const splhip_params *prm) {
   Tuning t;
@@ -860,6 +875,12 @@ Tuning tuning_of(const splhip_params *prm) {
   // list at about 100 ns per entry
   t.stage = prm->hip_stage < 0 ? 0xffffffffu
             : prm->hip_stage > 0 ? static_cast<uint32_t>(prm->hip_stage) : 8192u;
+  // ordered lists from 65536 entries on (for a pass bound by its longest
+  // list, see kStageOrderedRatio)
+  t.stage_ord = prm->hip_stage_ordered < 0 ? 0xffffffffu
+                : prm->hip_stage_ordered > 0 ? static_cast<uint32_t>(prm->hip_stage_ordered)
+                                             : 65536u;
+  t.stage_ord_auto = prm->hip_stage_ordered == 0;
   return t;
 }
 
@@ -879,7 +900,7 @@ SplatArgs splat_args(splhip_ctx *c, const Plan &pl, int p, const Tuning &tn) {
   sa.taylorlimit = c->consts.taylorlimit;
   sa.prio_len = tn.split;
   sa.heavy_len = tn.heavy;
-  sa.stage_len = (p == 1 && !pl.dc) ? tn.stage : 0xffffffffu;
+  sa.stage_len = pl.dc ? 0xffffffffu : p == 1 ? tn.stage : tn.stage_ord;
   sa.qbad = p == 0 ? pl.qbad : nullptr;
   return sa;
 }
@@ -897,8 +918,10 @@ void stage_items_count(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, uint32_
   launch_fill_u32(tot + T_HEAVY + p, 0u, 1, s);
   launch_fill_u32(tot + T_STAGE + p, 0u, 1, s);
   launch_fill_u32(tot + T_SGRP + p, 0u, 1, s);
+  launch_fill_u32(tot + T_LMAX + p, 0u, 1, s);
   launch_items_count(pl.bin_off[p], pl.nunits, sa, p == 0 ? 1 : 0, tn.split, icnt,
-                     tot + T_HEAVY + p, s, tot + T_STAGE + p, tot + T_SGRP + p);
+                     tot + T_HEAVY + p, s, tot + T_STAGE + p, tot + T_SGRP + p,
+                     tot + T_LMAX + p);
   scan_u32(icnt, ioff, pl.nunits, tot + T_ITEMS + p, ws, s);
   mark(c, p == 0 ? 2 : 5, false, s);
 }
@@ -913,7 +936,10 @@ void take_items(splhip_ctx *c, Plan &pl, int p, const Tuning &tn) {
   const uint32_t nlong = pl.m[p] ? h[T_HEAVY - T_ITEMS + p] : 0u;
   uint32_t ns = pl.m[p] ? h[T_STAGE - T_ITEMS + p] : 0u;
   const uint64_t groups = h[T_SGRP - T_ITEMS + p];
-  if (groups * 3 * 1024 > kStageBudget) ns = 0;
+  const uint64_t bytes = 4 * (stage_floats(p == 1, groups) + stage_q_floats(p == 1, groups));
+  if (bytes > kStageBudget) ns = 0;
+  const uint64_t lmax = h[T_LMAX - T_ITEMS + p];
+  if (p == 0 && tn.stage_ord_auto && lmax * kStageOrderedRatio < pl.etot[p]) ns = 0;
   ns = std::min(ns, nlong);
   pl.nstage[p] = ns;
   pl.sgroups[p] = ns ? groups : 0;
@@ -988,15 +1014,20 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
       mark(c, p == 0 ? 8 : 10, false, s);
     }
     if (nstage > 0) {
+      // the operand kernel and then the chains on a third stream, beside the
+      // normal items
+      const bool add = p == 1;
       uint32_t *cb = ws.get_as<uint32_t>(pslot(p, P_SBASE), nstage + 1);
       uint32_t *cbt = ws.get_as<uint32_t>(pslot(p, P_SBAT), nstage + 1);
-      float4 *ops = ws.get_as<float4>(S_STAGE, static_cast<size_t>(pl.sgroups[p]) * 192);
-      mark(c, p == 0 ? 8 : 10, true, s);
-      launch_stage(items, nstage, pl.sgroups[p], cb, cbt, ops, sa, s);
+      float4 *ops = ws.get_as<float4>(pslot(p, P_SOPS), stage_floats(add, pl.sgroups[p]) / 4);
+      float *qv = add ? nullptr
+                      : ws.get_as<float>(pslot(p, P_SQ), stage_q_floats(add, pl.sgroups[p]));
       ensure_aux3(c);
       hip_check(hipEventRecord(c->ev_fork3, s), "record fork");
       hip_check(hipStreamWaitEvent(c->aux3, c->ev_fork3, 0), "wait fork");
-      launch_stage_chain(items, nstage, cb, ops, sa, c->aux3);
+      mark(c, p == 0 ? 8 : 10, true, c->aux3);
+      launch_stage(add, items, nstage, pl.sgroups[p], cb, cbt, ops, qv, sa, c->aux3);
+      launch_stage_chain(add, items, nstage, cb, ops, qv, sa, c->aux3);
       mark(c, p == 0 ? 8 : 10, false, c->aux3);
       hip_check(hipEventRecord(c->ev_join3, c->aux3), "record join");
     }
@@ -1080,6 +1111,7 @@ void render_chunked_pass(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, uint3
     uint32_t n = m - s0;
     uint32_t e = 0;
     uint32_t e16 = 0;
+    uint64_t ec_last = 0;
     for (;;) {  // shrink the range until its entries fit a chunk
       scan_u32(cnt + s0, off, n, tot + T_E + p, ws, s);
       launch_sum_u64(cnt + s0, n, e64, s);
@@ -1087,6 +1119,7 @@ void render_chunked_pass(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, uint3
       read_totals(c, tot, T_N, s);
       uint64_t ec;
       std::memcpy(&ec, c->h_tot + T_E64 + 2 * p, 8);
+      ec_last = ec;
       e = c->h_tot[T_E + p];
       e16 = c->h_tot[T_E16 + p];
       if (ec <= tn.chunk || n == 1) break;
@@ -1103,6 +1136,7 @@ void render_chunked_pass(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, uint3
     // this range as the pass: render positions are local to the range
     pl.m[p] = n;
     pl.e[p] = e;
+    pl.etot[p] = ec_last;
     pl.e16[p] = e16;
     pl.recs[p] = recs + s0;
     if (e > 0) {
@@ -1226,11 +1260,6 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
   uint32_t *tot = ws.get_as<uint32_t>(S_TOTALS, 64);
   stage_project_compact(c, pl, rec, act, tot, s);
   upload_job_flags(c, pl, s);
-  // A single view with both passes: the additive splat runs beside the
-  // ordered pass on a second stream, where its long lists are off the
-  // critical path and staged operands' HBM traffic would slow the ordered
-  // pass's binning: no staging unless hip_stage asks for it.
-  if (nj == 1 && pl.m[0] && pl.m[1] && jobs[0].prm->hip_stage == 0) tn.stage = 0xffffffffu;
   for (int p = 0; p < 2; ++p)
     if (pl.m[p]) {
       stage_order(c, pl, p, rec, act, s);
@@ -1246,6 +1275,14 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
       pl.e16[p] = pl.m[p] ? c->h_tot[(T_E16 - T_E) + p] : 0;
       chunked[p] = pl.m[p] && e64[p] > tn.chunk;
     }
+    // A single unchunked view with both passes: the additive splat runs
+    // beside the ordered pass on a second stream, where its long lists are
+    // off the critical path and staged operands' HBM traffic would slow the
+    // ordered pass's binning: no staging unless hip_stage asks for it.
+    if (nj == 1 && pl.m[0] && pl.m[1] && !chunked[0] && !chunked[1] &&
+        jobs[0].prm->hip_stage == 0)
+      tn.stage = 0xffffffffu;
+    for (int p = 0; p < 2; ++p) pl.etot[p] = e64[p];
     if (chunked[0] || chunked[1]) {
       c->split_hint = nj > 1;  // the caller may render the views in smaller groups
       if (nj != 1 || G != 8)
@@ -1925,6 +1962,7 @@ int splhip_params_default(splhip_params *p) {
   p->hip_bin_group = 0;
   p->hip_ppw = 0;
   p->hip_stage = 0;
+  p->hip_stage_ordered = 0;
   return 0;
 }
 
@@ -1968,6 +2006,7 @@ int splhip_params_from_par(const char *par_text, float b_brightness, splhip_para
     p->hip_bin_group = par_num<int>(d, "hip_bin_group", 0);
     p->hip_ppw = par_num<int>(d, "hip_ppw", 0);
     p->hip_stage = par_num<int>(d, "hip_stage", 0);
+    p->hip_stage_ordered = par_num<int>(d, "hip_stage_ordered", 0);
   } catch (const ParError &) {
     return SPLHIP_E_ARG;
   }

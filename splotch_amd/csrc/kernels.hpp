@@ -51,7 +51,7 @@ struct SplatArgs {
   uint32_t prio_len;  // splat8: lists at least this long run at raised wave priority
   uint64_t *trace;    // splat8: optional per-item {start, end, len | chan << 32, hw id}
   uint32_t heavy_len; // splat8: lists at least this long go to the producer/consumer kernel
-  uint32_t stage_len; // additive lists at least this long are staged (0xffffffff: none)
+  uint32_t stage_len; // lists at least this long are staged (0xffffffff: none)
   // (device, optional) nonzero: the pass has a particle with a non-finite q_c,
   // so no list goes to the producer/consumer kernel (it applies every particle
   // to the whole block with zero operands, and (l - q) * 0 is NaN there)
@@ -149,11 +149,12 @@ void launch_unpack_records(const char *stage, int64_t n, int64_t C, int ncol, fl
 // channel 3 = all), heaviest first.  heavy_cnt (device, optional): incremented
 // once per long unit (list at least min(a.heavy_len, a.stage_len)); stage_cnt /
 // stage_grp (optional): once per unit whose list reaches a.stage_len, and by
-// its groups of 4 entries (padded to whole chain loads).
+// its groups of 4 entries (padded to whole chain loads); lmax (optional): the
+// longest list of a unit with items.
 void launch_items_count(const uint32_t *bin_off, uint32_t nunits, const SplatArgs &a,
                         int ordered, uint32_t split, uint32_t *cnt, uint32_t *heavy_cnt,
                         hipStream_t s, uint32_t *stage_cnt = nullptr,
-                        uint32_t *stage_grp = nullptr);
+                        uint32_t *stage_grp = nullptr, uint32_t *lmax = nullptr);
 void launch_items_emit(const uint32_t *bin_off, uint32_t nunits, const SplatArgs &a, int ordered,
                        uint32_t split, const uint32_t *off, uint32_t *keys, uint32_t *vals,
                        hipStream_t s);
@@ -171,14 +172,18 @@ struct ItemCounts {
 void launch_splat8(bool additive, int variant, const uint32_t *items, uint32_t nitems,
                    const SplatArgs &a, hipStream_t s, const ItemCounts &dc = ItemCounts{},
                    int ppw = 0);
-// Staged lists (splat8.hip), the first n items of an additive pass (groups:
-// their groups of 4 entries, padded): list offsets into cbase / cbat ([n + 1] each)
-// and the per-pixel terms into ops (groups * 3 * 64 float4); then the chains,
-// one wave per list and channel, which write the lists' blocks.
-void launch_stage(const uint32_t *items, uint32_t n, uint64_t groups, uint32_t *cbase,
-                  uint32_t *cbat, float4 *ops, const SplatArgs &a, hipStream_t s);
-void launch_stage_chain(const uint32_t *items, uint32_t n, const uint32_t *cbase,
-                        const float4 *ops, const SplatArgs &a, hipStream_t s);
+// Staged lists (splat8.hip), the first n items of a pass (groups: their
+// groups of 4 entries, padded): list offsets into cbase / cbat ([n + 1] each)
+// and the per-pixel operands into ops (stage_floats floats) and, ordered, the
+// entries' q_c into qv (stage_q_floats floats); then the chains, one wave per
+// list, channel and 16-pixel slice, which write the lists' blocks.
+uint64_t stage_floats(bool additive, uint64_t groups);
+uint64_t stage_q_floats(bool additive, uint64_t groups);
+void launch_stage(bool additive, const uint32_t *items, uint32_t n, uint64_t groups,
+                  uint32_t *cbase, uint32_t *cbat, float4 *ops, float *qv, const SplatArgs &a,
+                  hipStream_t s);
+void launch_stage_chain(bool additive, const uint32_t *items, uint32_t n, const uint32_t *cbase,
+                        const float4 *ops, const float *qv, const SplatArgs &a, hipStream_t s);
 // Producer/consumer workgroups (splat8.hip), one per item: the first npc
 // items, the units whose lists reach a.heavy_len.
 void launch_splat_pc(bool additive, const uint32_t *items, uint32_t npc, const SplatArgs &a,

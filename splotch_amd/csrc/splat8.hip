@@ -1007,25 +1007,37 @@ __global__ __launch_bounds__(kPThreads) void k_splat_pc(SplatArgs a, const uint3
 }
 
 // ---------------------------------------------------------------------------
-// Staged long lists (additive pass).  An additive pixel's chain is one add per
-// particle, l_c = l_c + t_c with t_c = a_c * att (splotch_host.cc:615), so a
-// long list is latency-bound in a lane-per-pixel wave: the per-particle table
-// work, not the add, sets its pace.  The staged path moves that work off the
-// chain: k_stage_ops (every CU) writes each staged list's operands t_c per
-// pixel to HBM, and k_stage_chain runs one wave per (list, channel) that only
-// streams its plane and adds, in list order.  A pixel outside a particle's
-// footprint gets t = +0 (l + 0 == l: l starts at +0 and a float sum is -0
-// only if both terms are -0), so every pixel sees the reference's sums and
-// rounding.
+// Staged long lists.  A long list's pixel chains are serial, and in a
+// lane-per-pixel wave (or a producer/consumer workgroup) the per-particle
+// table work, not the chain, sets their pace.  The staged path moves that
+// work off the chain: k_stage_ops (every CU) writes each staged list's
+// per-pixel operands to HBM, and k_stage_chain streams them back in list
+// order, one wave per list, channel and 16-pixel slice, which applies only
+// the chain's own operations (splotch_host.cc:615, 648-658):
+//   additive  l_c = l_c + t_c                 t_c = a_c * att
+//   ordered   l_c = l_c + ((l_c - q_c) * m1_c) * m2
+//             Taylor branch m1 = a, m2 = att; expm1 branch m1 = expm1(att a),
+//             m2 = 1 (x * 1 == x)
+// A pixel outside a particle's footprint gets t = +0, or m1 = m2 = 0 (l + +-0
+// == l: l starts at +0 and a float sum is -0 only if both terms are -0; for
+// finite records l - q stays finite, so (l - q) * 0 is a zero, and ordered
+// lists are staged only when every q_c of the pass is finite), so every
+// pixel sees the reference's sums and compositing, rounded as it rounds them.
 //
 // Operand layout: the lists' entries in groups of 4, a list's groups padded
 // with zero operands to a multiple of kStageLoad (the groups of one load of
-// the chain kernel); list i owns groups [cbase[i], cbase[i+1]), and group g
-// of channel c is 64 float4 (lane = pixel cx * 8 + cy, components = the
-// group's 4 entries) at ops + (g * 3 + c) * 64.
+// the chain kernel); list i owns groups [cbase[i], cbase[i+1]).  Group g of
+// plane c is 64 float4 (lane = pixel cx * 8 + cy, components = the group's 4
+// entries) at ops + (g * P + c) * 64, with P = 3 planes t_c (additive) or 4
+// (m1_0, m1_1, m1_2, m2; ordered).  Ordered lists also have q_c per entry:
+// qv[(g / kStageLoad) * 48 + c * 16 + e] for entry e of the load holding g.
 constexpr int kStageLoad = 4;
 __host__ __device__ constexpr uint32_t stage_groups(uint32_t len) {
   return (len + 4 * kStageLoad - 1) / (4 * kStageLoad) * kStageLoad;
+}
+template <int ADD>
+constexpr uint32_t stage_planes() {
+  return ADD ? 3u : 4u;
 }
 
 // List offsets of the staged items (one workgroup): cbase (groups of 4
@@ -1075,9 +1087,11 @@ __global__ __launch_bounds__(256) void k_stage_base(SplatArgs a, const uint32_t 
 // Operands of the staged lists: the batches of 64 entries (cbat[n] of them)
 // split evenly over the launch's waves, each wave walking a contiguous run
 // (phase A as the 8x8 splat's, then lane = pixel for the batch's groups).
+template <int ADD>
 __global__ __launch_bounds__(256) void k_stage_ops(SplatArgs a, const uint32_t *items, uint32_t n,
                                                    const uint32_t *cbase, const uint32_t *cbat,
-                                                   float4 *ops) {
+                                                   float4 *ops, float *qv) {
+  constexpr uint32_t P = stage_planes<ADD>();
   __shared__ float s_tab[2048];
   __shared__ uint32_t s_col[4][8][kPad];
   __shared__ float s_pre2[4][8][kPad];
@@ -1101,7 +1115,7 @@ __global__ __launch_bounds__(256) void k_stage_ops(SplatArgs a, const uint32_t *
   uint32_t (*col)[kPad] = s_col[wave];
   float(*pre2)[kPad] = s_pre2[wave];
   float(*pre1)[kPad] = s_pre1[wave];
-  const float expfac = a.expfac;
+  const float expfac = a.expfac, tl = a.taylorlimit;
   const int cx = lane >> 3, cy = lane & 7;
   uint32_t i = lo, b = gb - uniform(cbat[lo]);
   for (; gb < gend; ++gb, ++b) {
@@ -1114,33 +1128,61 @@ __global__ __launch_bounds__(256) void k_stage_ops(SplatArgs a, const uint32_t *
     const uint32_t e0 = uniform(a.bin_off[u]), e1 = uniform(a.bin_off[u + 1]);
     const uint32_t base = e0 + 64u * b;
     const int nb = static_cast<int>(uniform(min<uint32_t>(64u, e1 - base)));
-    float va0 = 0.f, va1 = 0.f, va2 = 0.f;
+    const uint32_t g0 = uniform(cbase[i]) + 16u * b;  // the batch's first group
+    const int ng = static_cast<int>(stage_groups(nb));  // 64 entries: whole loads
+    float va0 = 0.f, va1 = 0.f, va2 = 0.f, vmx = 0.f;
+    float vq[3] = {0.f, 0.f, 0.f};
     if (lane < nb) {
       const Rec R = tabulate<true>(a, base + lane, lane, ug.x0s, ug.y0s, ug.w, ug.h, ug.gx * 8,
                                    ug.gy * 8, s_tab, col, pre2, pre1, expfac);
       va0 = -R.e0;
       va1 = -R.e1;
       va2 = -R.e2;
+      if (!ADD) {
+        const float gray = ug.grayabsorb;
+        vq[0] = R.e0 / (R.e0 + gray);
+        vq[1] = R.e1 / (R.e1 + gray);
+        vq[2] = R.e2 / (R.e2 + gray);
+        vmx = fmaxf(fabsf(va0), fmaxf(fabsf(va1), fabsf(va2)));
+      }
+    }
+    if (!ADD && lane < ng * 4) {  // q_c per entry (0 past the list), lane = entry
+      float *qd = qv + (g0 / kStageLoad + static_cast<uint32_t>(lane) / 16u) * 48u + lane % 16;
+      qd[0] = vq[0];
+      qd[16] = vq[1];
+      qd[32] = vq[2];
     }
     wave_sync();
-    float4 *dst = ops + (static_cast<size_t>(uniform(cbase[i])) + 16u * b) * 3u * 64u + lane;
-    const int ng = static_cast<int>(stage_groups(nb));  // 64 entries: whole loads
+    float4 *dst = ops + static_cast<size_t>(g0) * P * 64u + lane;
     for (int q = 0; q < ng; ++q) {
-      float t0[4], t1[4], t2[4];
+      float t0[4], t1[4], t2[4], t3[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int k0 = 4 * q + j, k = min(k0, nb - 1);  // uniform; slots past the list are 0
         const bool in = k0 < nb && ((col[cx][k] >> cy) & 1u);
         const float at = pre1[cy][k] * pre2[cx][k];
-        const float x0 = lane_f(va0, k) * at, x1 = lane_f(va1, k) * at, x2 = lane_f(va2, k) * at;
-        t0[j] = in ? x0 : 0.f;
-        t1[j] = in ? x1 : 0.f;
-        t2[j] = in ? x2 : 0.f;
+        const float a0 = lane_f(va0, k), a1 = lane_f(va1, k), a2 = lane_f(va2, k);
+        if (ADD) {
+          const float x0 = a0 * at, x1 = a1 * at, x2 = a2 * at;
+          t0[j] = in ? x0 : 0.f;
+          t1[j] = in ? x1 : 0.f;
+          t2[j] = in ? x2 : 0.f;
+        } else {
+          const bool tay = (lane_f(vmx, k) * at) < tl;
+          const float em0 = xexpm1_bf(s_tab, at * a0, expfac, tl);
+          const float em1 = xexpm1_bf(s_tab, at * a1, expfac, tl);
+          const float em2 = xexpm1_bf(s_tab, at * a2, expfac, tl);
+          t0[j] = in ? (tay ? a0 : em0) : 0.f;
+          t1[j] = in ? (tay ? a1 : em1) : 0.f;
+          t2[j] = in ? (tay ? a2 : em2) : 0.f;
+          t3[j] = in ? (tay ? at : 1.f) : 0.f;
+        }
       }
-      float4 *g = dst + static_cast<size_t>(q) * 3u * 64u;
+      float4 *g = dst + static_cast<size_t>(q) * P * 64u;
       g[0] = make_float4(t0[0], t0[1], t0[2], t0[3]);
       g[64] = make_float4(t1[0], t1[1], t1[2], t1[3]);
       g[128] = make_float4(t2[0], t2[1], t2[2], t2[3]);
+      if (!ADD) g[192] = make_float4(t3[0], t3[1], t3[2], t3[3]);
     }
     wave_sync();
   }
@@ -1149,16 +1191,20 @@ __global__ __launch_bounds__(256) void k_stage_ops(SplatArgs a, const uint32_t *
 // The chains of the staged lists.  A wave applies channel c of list i to a
 // slice of kPix pixels of the block: workgroup (3 i + c) kS + h takes pixels
 // [h kPix, h kPix + kPix).  One load instruction brings kS consecutive groups
-// (lane quarter q: group g + q of the slice's pixels, 16 bytes per lane), so
-// the kR loads in flight per wave cover kR kS groups; lanes [0, kPix) run the
-// chains and read the other quarters' groups back through LDS.  A chain's
-// wave is bound by the loads it has in flight, not by its adds, so narrower
-// slices (more waves per list) move more bytes per list at once.  Lists are
-// padded with zero groups to a multiple of kS groups.
-constexpr int kS = kStageLoad, kPix = 64 / kS, kR = 40;
+// of a plane (lane quarter q: group g + q of the slice's pixels, 16 bytes per
+// lane), so the loads in flight per wave cover kS groups each; lanes
+// [0, kPix) run the chains and read the other quarters' groups back through
+// LDS one step ahead.  A chain's wave is bound by its loads in flight and its
+// serial operations, not by issue, so narrower slices (more waves per list)
+// run a list faster.
+constexpr int kS = kStageLoad, kPix = 64 / kS;
+template <int ADD>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_stage_chain(
-    SplatArgs a, const uint32_t *items, const uint32_t *cbase, const float4 *ops) {
-  __shared__ float4 xs[64];
+    SplatArgs a, const uint32_t *items, const uint32_t *cbase, const float4 *__restrict__ ops,
+    const float *__restrict__ qv) {
+  constexpr uint32_t P = stage_planes<ADD>();
+  constexpr int kR = ADD ? 40 : 20;  // loads in flight (ordered: two planes per step)
+  __shared__ float4 xs[2][64];
   const uint32_t i = blockIdx.x / (3u * kS);
   const uint32_t r3 = blockIdx.x - i * (3u * kS);
   const int c = static_cast<int>(r3 / kS), h = static_cast<int>(r3 % kS);
@@ -1170,6 +1216,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
   const UnitGeom ug = unit_geom_uniform(a, u);
   const uint32_t len = uniform(a.bin_off[u + 1]) - uniform(a.bin_off[u]);
   const uint32_t G = stage_groups(len);  // the list is not empty
+  const uint32_t gb0 = uniform(cbase[i]);
   const int cx = px >> 3, cy = px & 7;
   const int lx = ug.gx * 8 + cx, ly = ug.gy * 8 + cy;
   const bool pix_in = lane < kPix && lx < min(ug.gx * 8 + 8, ug.w) &&
@@ -1180,64 +1227,94 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
   // l is in its register before the operand loads are issued: a load still
   // in flight at the loop would make the compiler wait for every load there
   asm volatile("" : "+v"(l));
-  // group g of this channel and pixel: src[g * 192]; lane quarter qo reads g + qo
-  const float4 *src = ops + static_cast<size_t>(uniform(cbase[i])) * 3u * 64u +
-                      static_cast<size_t>(c) * 64u + px;
-  auto group = [&](uint32_t g0) {  // groups [g0, g0 + kS), the last ones again past the list
-    return src[static_cast<size_t>(min<uint32_t>(g0, G - kS) + qo) * 192u];
+  // group g of plane c and this pixel: src[g * P * 64]; lane quarter qo reads g + qo
+  const float4 *src = ops + static_cast<size_t>(gb0) * P * 64u + static_cast<size_t>(c) * 64u + px;
+  const float4 *srcw = ops + static_cast<size_t>(gb0) * P * 64u + 3u * 64u + px;  // m2 (ordered)
+  const float *qsrc = qv + static_cast<size_t>(gb0 / kS) * 48u + static_cast<size_t>(c) * 16u;
+  auto at_load = [&](uint32_t g0) {  // groups [g0, g0 + kS), the last ones again past the list
+    return static_cast<size_t>(min<uint32_t>(g0, G - kS) + qo) * P * 64u;
   };
   // one load's 16 entries for lanes [0, kPix): its own quarter, then the
   // other three read back through LDS
   struct Q {
     float4 a, b, c, d;
   };
-  auto exchange = [&](const float4 &v) {
-    xs[lane] = v;
+  auto exchange = [&](int slot, const float4 &v) {
+    xs[slot][lane] = v;
     wave_sync();
-    return Q{v, xs[pl + kPix], xs[pl + 2 * kPix], xs[pl + 3 * kPix]};
+    return Q{v, xs[slot][pl + kPix], xs[slot][pl + 2 * kPix], xs[slot][pl + 3 * kPix]};
   };
-  auto add16 = [&](const Q &q) {
-    l = l + q.a.x;
-    l = l + q.a.y;
-    l = l + q.a.z;
-    l = l + q.a.w;
-    l = l + q.b.x;
-    l = l + q.b.y;
-    l = l + q.b.z;
-    l = l + q.b.w;
-    l = l + q.c.x;
-    l = l + q.c.y;
-    l = l + q.c.z;
-    l = l + q.c.w;
-    l = l + q.d.x;
-    l = l + q.d.y;
-    l = l + q.d.z;
-    l = l + q.d.w;
+  struct Step {
+    Q m, w;
+    float q[16];  // ordered: q_c of the load's entries (uniform)
   };
-  float4 buf[kR];
+  // the exchange of the load at group g0 (and its q_c, read with scalar loads)
+  auto step_of = [&](const float4 &vm, const float4 &vw, uint32_t g0) {
+    Step st;
+    st.m = exchange(0, vm);
+    if (!ADD) {
+      st.w = exchange(1, vw);
+      const float *qp = qsrc + static_cast<size_t>(min<uint32_t>(g0, G - kS) / kS) * 48u;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) st.q[e] = qp[e];
+    }
+    return st;
+  };
+  auto apply = [&](const Step &st) {
+    if (ADD) {
+      const float4 *q4[4] = {&st.m.a, &st.m.b, &st.m.c, &st.m.d};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        l = l + q4[k]->x;
+        l = l + q4[k]->y;
+        l = l + q4[k]->z;
+        l = l + q4[k]->w;
+      }
+    } else {
+      const float4 *m4[4] = {&st.m.a, &st.m.b, &st.m.c, &st.m.d};
+      const float4 *w4[4] = {&st.w.a, &st.w.b, &st.w.c, &st.w.d};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {  // load quarter k: entries 4k .. 4k + 3
+        l = l + ((l - st.q[4 * k]) * m4[k]->x) * w4[k]->x;
+        l = l + ((l - st.q[4 * k + 1]) * m4[k]->y) * w4[k]->y;
+        l = l + ((l - st.q[4 * k + 2]) * m4[k]->z) * w4[k]->z;
+        l = l + ((l - st.q[4 * k + 3]) * m4[k]->w) * w4[k]->w;
+      }
+    }
+  };
+  float4 bm[kR], bw[ADD ? 1 : kR];
 #pragma unroll
   for (int r = 0; r < kR; ++r) {  // issued in the order they are consumed
-    buf[r] = group(r * kS);
+    const size_t o = at_load(r * kS);
+    bm[r] = src[o];
+    if (!ADD) bw[r] = srcw[o];
     asm volatile("" ::: "memory");
   }
-  // Software pipeline over the loads: step r of a round adds the entries of
-  // load r (exchanged a step earlier), exchanges load r + 1 meanwhile, and
+  // Software pipeline over the loads: step r of a round applies the entries
+  // of load r (exchanged a step earlier), exchanges load r + 1 meanwhile, and
   // refills that load's registers with the groups kR loads further on.
-  Q cur = exchange(buf[0]);
+  Step cur = step_of(bm[0], bw[0], 0);
   asm volatile("" ::: "memory");
-  buf[0] = group(kR * kS);
+  {
+    const size_t o = at_load(kR * kS);
+    bm[0] = src[o];
+    if (!ADD) bw[0] = srcw[o];
+  }
   for (uint32_t g = 0; g < G; g += kR * kS) {
 #pragma unroll
     for (int r = 0; r < kR; ++r) {
       const int rn = (r + 1) % kR;
-      const Q nxt = exchange(buf[rn]);
-      if (g + r * kS < G) add16(cur);
+      const uint32_t g0 = g + r * kS;
+      const Step nxt = step_of(bm[rn], bw[ADD ? 0 : rn], g0 + kS);
+      if (g0 < G) apply(cur);
       cur = nxt;
       // the barrier orders the exchange and sums before the refill (and the
-      // next exchange's LDS write after these reads) and keeps the load from
+      // next exchange's LDS writes after these reads) and keeps the load from
       // sinking to its use a round later, so kR loads stay in flight
       asm volatile("" : "+v"(l) : : "memory");
-      buf[rn] = group(g + (r + 1) * kS + kR * kS);
+      const size_t o = at_load(g + (r + 1) * kS + kR * kS);
+      bm[rn] = src[o];
+      if (!ADD) bw[rn] = srcw[o];
     }
   }
   if (a.trace && c == 0 && h == 0 && lane == 0) {
@@ -1261,10 +1338,17 @@ __device__ __forceinline__ uint32_t heavy_len_of(const SplatArgs &a) {
   return (a.qbad && *a.qbad) ? 0xffffffffu : a.heavy_len;
 }
 
+// The list length from which a unit is staged: none when the pass has a
+// particle with a non-finite q_c (a staged ordered list applies zero operands
+// outside the footprints, as the producer/consumer kernel does).
+__device__ __forceinline__ uint32_t stage_len_of(const SplatArgs &a) {
+  return (a.qbad && *a.qbad) ? 0xffffffffu : a.stage_len;
+}
+
 // Long lists (one item each, sorted first): staged lists, then the
 // producer/consumer kernel's.
 __device__ __forceinline__ uint32_t long_len_of(const SplatArgs &a) {
-  return min(heavy_len_of(a), a.stage_len);
+  return min(heavy_len_of(a), stage_len_of(a));
 }
 
 __device__ __forceinline__ uint32_t unit_items(const SplatArgs &a, uint32_t u, int ordered,
@@ -1280,16 +1364,17 @@ __device__ __forceinline__ uint32_t unit_items(const SplatArgs &a, uint32_t u, i
 __global__ __launch_bounds__(256) void k_items_count(SplatArgs a, uint32_t nunits, int ordered,
                                                      uint32_t split, uint32_t *cnt,
                                                      uint32_t *heavy_cnt, uint32_t *stage_cnt,
-                                                     uint32_t *stage_grp) {
+                                                     uint32_t *stage_grp, uint32_t *lmax) {
   for (int64_t u = gtid(); u < nunits; u += gstride()) {
     uint32_t len;
     const uint32_t k = unit_items(a, static_cast<uint32_t>(u), ordered, split, len);
     cnt[u] = k;
     if (heavy_cnt && k > 0 && len >= long_len_of(a)) atomicAdd(heavy_cnt, k);
-    if (stage_cnt && k > 0 && len >= a.stage_len) {
+    if (stage_cnt && k > 0 && len >= stage_len_of(a)) {
       atomicAdd(stage_cnt, 1u);
       atomicAdd(stage_grp, stage_groups(len));
     }
+    if (lmax && k > 0) atomicMax(lmax, len);
   }
 }
 
@@ -1318,12 +1403,13 @@ unsigned grid_for(int64_t n) {
 
 void launch_items_count(const uint32_t *bin_off, uint32_t nunits, const SplatArgs &a,
                         int ordered, uint32_t split, uint32_t *cnt, uint32_t *heavy_cnt,
-                        hipStream_t s, uint32_t *stage_cnt, uint32_t *stage_grp) {
+                        hipStream_t s, uint32_t *stage_cnt, uint32_t *stage_grp,
+                        uint32_t *lmax) {
   if (nunits == 0) return;
   SplatArgs b = a;
   b.bin_off = bin_off;
   k_items_count<<<grid_for(nunits), 256, 0, s>>>(b, nunits, ordered, split, cnt, heavy_cnt,
-                                                 stage_cnt, stage_grp);
+                                                 stage_cnt, stage_grp, lmax);
   hip_check(hipGetLastError(), "k_items_count launch");
 }
 
@@ -1337,22 +1423,33 @@ void launch_items_emit(const uint32_t *bin_off, uint32_t nunits, const SplatArgs
   hip_check(hipGetLastError(), "k_items_emit launch");
 }
 
-void launch_stage(const uint32_t *items, uint32_t n, uint64_t groups, uint32_t *cbase,
-                  uint32_t *cbat, float4 *ops, const SplatArgs &a, hipStream_t s) {
+uint64_t stage_floats(bool additive, uint64_t groups) {
+  return groups * (additive ? 3u : 4u) * 256u;
+}
+
+uint64_t stage_q_floats(bool additive, uint64_t groups) {
+  return additive ? 0 : groups / kStageLoad * 48u;
+}
+
+void launch_stage(bool additive, const uint32_t *items, uint32_t n, uint64_t groups,
+                  uint32_t *cbase, uint32_t *cbat, float4 *ops, float *qv, const SplatArgs &a,
+                  hipStream_t s) {
   if (n == 0) return;
   k_stage_base<<<1, 256, 0, s>>>(a, items, n, cbase, cbat);
   hip_check(hipGetLastError(), "k_stage_base launch");
   // batches: at most groups / 16 + n; about four per wave when there are many
   const uint64_t nbat = groups / 16 + n;
-  const unsigned grid = static_cast<unsigned>(std::min<uint64_t>(2048, (nbat + 15) / 16));
-  k_stage_ops<<<std::max(grid, 1u), 256, 0, s>>>(a, items, n, cbase, cbat, ops);
+  const unsigned grid = std::max(1u, static_cast<unsigned>(std::min<uint64_t>(2048, (nbat + 15) / 16)));
+  if (additive) k_stage_ops<1><<<grid, 256, 0, s>>>(a, items, n, cbase, cbat, ops, qv);
+  else k_stage_ops<0><<<grid, 256, 0, s>>>(a, items, n, cbase, cbat, ops, qv);
   hip_check(hipGetLastError(), "k_stage_ops launch");
 }
 
-void launch_stage_chain(const uint32_t *items, uint32_t n, const uint32_t *cbase,
-                        const float4 *ops, const SplatArgs &a, hipStream_t s) {
+void launch_stage_chain(bool additive, const uint32_t *items, uint32_t n, const uint32_t *cbase,
+                        const float4 *ops, const float *qv, const SplatArgs &a, hipStream_t s) {
   if (n == 0) return;
-  k_stage_chain<<<3 * kS * n, 64, 0, s>>>(a, items, cbase, ops);
+  if (additive) k_stage_chain<1><<<3 * kS * n, 64, 0, s>>>(a, items, cbase, ops, qv);
+  else k_stage_chain<0><<<3 * kS * n, 64, 0, s>>>(a, items, cbase, ops, qv);
   hip_check(hipGetLastError(), "k_stage_chain launch");
 }
 

@@ -12,7 +12,7 @@ namespace splhip {
 enum Slot : int {
   S_REC, S_ACT, S_HIST, S_SCAN0, S_SCAN1, S_SCAN2, S_SCAN3,
   S_TOTALS, S_SOA, S_RECSTAGE, S_PIC, S_MAP, S_AUX0, S_JOBS, S_GRAD0, S_GRAD1, S_GRAD2, S_GRAD3,
-  S_GRAD4, S_GRAD5, S_GRAD6, S_GRAD7, S_GRAD8, S_GRAD9, S_SCRATCH, S_JOBS2, S_NORM, S_STAGE,
+  S_GRAD4, S_GRAD5, S_GRAD6, S_GRAD7, S_GRAD8, S_GRAD9, S_SCRATCH, S_JOBS2, S_NORM,
   S_PASS  // first per-pass slot
 };
 
@@ -20,7 +20,7 @@ enum Slot : int {
 enum PassSlot : int {
   P_POS, P_MO, P_K0, P_V0, P_K1, P_V1, P_RECS, P_CNT, P_OFF, P_EK0, P_EV0, P_EK1, P_EV1,
   P_BINOFF, P_ICNT, P_IOFF, P_IK0, P_IV0, P_IK1, P_IV1, P_TRACE,
-  P_CNT16, P_OFF16, P_ROFF, P_UCNT, P_ENT, P_CCNT, P_CPRE, P_SX, P_SBASE, P_SBAT,
+  P_CNT16, P_OFF16, P_ROFF, P_UCNT, P_ENT, P_CCNT, P_CPRE, P_SX, P_SBASE, P_SBAT, P_SOPS, P_SQ,
   P_NSLOTS
 };
 

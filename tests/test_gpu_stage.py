@@ -1,13 +1,15 @@
-"""GPU parity of the staged long lists (additive pass), through the C-ABI.
+"""GPU parity of the staged long lists, through the C-ABI.
 
-A list from hip_stage entries on has its per-pixel terms computed across the
-GPU first (k_stage_ops), then summed in list order by one wave per channel
-(k_stage_chain).  Bar: bit-exact against the genuine fixtures and the
-oracle, and against the unstaged render, with every list staged, with staged
-lists beside producer/consumer workgroups and normal items, in chunked
-passes (chains continued from the image), on top of an ordered pass (scratch
-image, second stream), in batches of views, and with non-finite particle
-intensities (NaN in exactly the oracle's pixels).
+A list from hip_stage (additive pass) or hip_stage_ordered (ordered pass)
+entries on has its per-pixel operands computed across the GPU first
+(k_stage_ops), then applied in list order by one wave per list, channel and
+16-pixel slice (k_stage_chain).  Bar: bit-exact against the genuine fixtures
+and the oracle, and against the unstaged render, with every list staged,
+with staged lists beside producer/consumer workgroups and normal items, in
+chunked passes (chains continued from the image), for both passes of one
+view, in batches of views, with non-finite particle intensities (NaN in
+exactly the oracle's pixels), and with a non-finite q_c (gray_absorption 0:
+no ordered staging, and the image unchanged).
 """
 import math
 import sys
@@ -62,6 +64,62 @@ def test_staged_fixtures(R, golden, T):
     pic = render(R, field_from(f), view_from(f), [(p["x"], p["rgb"])],
                  S.params(a_eq_e=True, tile_size=T, hip_stage=1))
     assert_bitexact(pic, f["pic_T%d" % T], "staged fixture T=%d" % T)
+
+
+@pytest.mark.parametrize("g05", [False, True])
+def test_staged_ordered_fixture(R, golden, g05):
+    """Every non-empty block of the ordered fixture staged, against the
+    genuine build's raw image in the stable depth order (gray_absorption 0.2
+    and 0.5)."""
+    import splotch_amd as S
+    log("case staged ordered fixture, gray 0.5:", g05)
+    f = golden("ordered_small")
+    p = golden("palette")
+    kw = dict(gray_absorption=0.5) if g05 else {}
+    pic = render(R, field_from(f), view_from(f), [(p["x"], p["rgb"])],
+                 S.params(a_eq_e=False, brightness=[float(f["brightness"][0])],
+                          hip_stage_ordered=1, **kw))
+    assert_bitexact(pic, f["pic_stable_T50_g05" if g05 else "pic_stable_T50"],
+                    "staged ordered fixture")
+
+
+def test_staged_ordered_dense_cluster_vs_oracle(R):
+    """Ordered: staged at several thresholds, beside producer/consumer
+    workgroups and channel items, in 8 px and larger tiles."""
+    import splotch_amd as S
+    fld = synth.plummer_field(60_000, 113, halo_frac=1.0, n_halos=1, hsml_mu=math.log(0.8))
+    v = synth.default_view(160, 144)
+    pal = synth.default_palette()
+    opic, _ = O.host_rendering(fld, v, [pal], a_eq_e=False, stable=True)
+    for stage, heavy, hmax, split, T in ((1, 0, 0, 0, 50), (64, 0, 0, 0, 50),
+                                         (2048, 64, 0, 0, 50), (512, 16, 3, 32, 37),
+                                         (4096, -1, 0, 16, 16), (-1, 0, 0, 0, 50)):
+        log("case ordered dense cluster staged", stage, heavy, hmax, split, T)
+        pic = render(R, fld, v, [pal], S.params(a_eq_e=False, hip_stage_ordered=stage,
+                                                hip_heavy=heavy, hip_heavy_max=hmax,
+                                                hip_split=split, tile_size=T))
+        if T == 50:
+            assert_bitexact(pic, opic, "ordered dense cluster stage=%d heavy=%d max=%d split=%d"
+                            % (stage, heavy, hmax, split))
+        else:
+            ref = render(R, fld, v, [pal], S.params(a_eq_e=False, hip_stage_ordered=-1,
+                                                    tile_size=T))
+            assert_bitexact(pic, ref, "ordered dense cluster T=%d stage=%d" % (T, stage))
+
+
+def test_staged_ordered_gray_zero(R):
+    """gray_absorption 0 gives q_c = 0/0 for channels with e_c = 0: no list is
+    staged (it would spread the NaN over whole blocks), and the image equals
+    the unstaged render, NaN in the same pixels."""
+    import splotch_amd as S
+    fld = synth.plummer_field(20_000, 1620, halo_frac=1.0, n_halos=3, hsml_mu=math.log(0.6))
+    v = synth.default_view(176, 144)
+    pal = synth.default_palette()
+    a = render(R, fld, v, [pal], S.params(a_eq_e=False, gray_absorption=0.0, hip_stage_ordered=1))
+    b = render(R, fld, v, [pal], S.params(a_eq_e=False, gray_absorption=0.0,
+                                          hip_stage_ordered=-1))
+    assert np.isnan(a).any()
+    assert_bitexact(a, b, "gray_absorption 0, staging requested")
 
 
 def test_staged_dense_cluster_vs_oracle(R):
@@ -126,16 +184,18 @@ def test_staged_chunked_passes(R, chunk):
     pal = synth.default_palette()
     for aeq in ([0, 1], [1, 1], [2, 1]):
         want = render(R, fld, v, [pal, pal], S.params(ptypes=2, a_eq_e=aeq,
-                                                      brightness=[0.5, 2.0], hip_stage=-1))
+                                                      brightness=[0.5, 2.0], hip_stage=-1,
+                                                      hip_stage_ordered=-1))
         for stage in (32, 300):
             got = render(R, fld, v, [pal, pal],
                          S.params(ptypes=2, a_eq_e=aeq, brightness=[0.5, 2.0], hip_stage=stage,
-                                  hip_chunk_entries=chunk))
+                                  hip_stage_ordered=stage, hip_chunk_entries=chunk))
             _, counts = R.stage_times()
             assert counts["additive_entries"] > chunk
             assert_bitexact(got, want, "staged chunked %s stage %d" % (aeq, stage))
             got = render(R, fld, v, [pal, pal],
-                         S.params(ptypes=2, a_eq_e=aeq, brightness=[0.5, 2.0], hip_stage=stage))
+                         S.params(ptypes=2, a_eq_e=aeq, brightness=[0.5, 2.0], hip_stage=stage,
+                                  hip_stage_ordered=stage))
             assert_bitexact(got, want, "staged %s stage %d" % (aeq, stage))
 
 
@@ -156,12 +216,14 @@ def test_staged_batched_views(R):
         cam = S.camera_setup(bj.view)
         n = bj.view.xres * bj.view.yres * 3
         p = torch.empty(n, dtype=torch.float32, device="cuda")
-        R.render(dev, cam, S.params(a_eq_e=bj.additive, tile_size=T, hip_stage=-1), p)
+        R.render(dev, cam, S.params(a_eq_e=bj.additive, tile_size=T, hip_stage=-1,
+                                    hip_stage_ordered=-1), p)
         plain.append(p)
-        views.append((dev, cam, S.params(a_eq_e=bj.additive, tile_size=T, hip_stage=64),
+        views.append((dev, cam, S.params(a_eq_e=bj.additive, tile_size=T, hip_stage=64,
+                                         hip_stage_ordered=64),
                       torch.full((n,), float("nan"), dtype=torch.float32, device="cuda")))
     log("staged batch:", [(bj.field.n, bj.view.xres, bj.additive) for bj in jobs])
-    assert any(bj.additive for bj in jobs)
+    assert any(bj.additive for bj in jobs) and not all(bj.additive for bj in jobs)
     R.render_batched(views)
     torch.cuda.synchronize()
     for j, (v, p) in enumerate(zip(views, plain)):
