@@ -87,9 +87,10 @@ struct splhip_ctx {
   hipStream_t aux2 = nullptr;
   hipEvent_t ev_fork2 = nullptr, ev_join2 = nullptr;
   Workspace ws2;
-  // stream of the staged lists' chains (they follow the operand kernel)
-  hipStream_t aux3 = nullptr;
-  hipEvent_t ev_fork3 = nullptr, ev_join3 = nullptr;
+  // streams of the staged lists: the longest lists' operands and chains, then
+  // the other lists' (they follow the operand kernel)
+  hipStream_t aux3 = nullptr, aux4 = nullptr;
+  hipEvent_t ev_fork3 = nullptr, ev_join3 = nullptr, ev_fork4 = nullptr, ev_join4 = nullptr;
   // block-file ingest: two pinned buffers, each freed for reuse by its event
   char *h_ing[2] = {nullptr, nullptr};
   hipEvent_t ev_ing[2] = {nullptr, nullptr};
@@ -300,8 +301,11 @@ void ensure_aux2(splhip_ctx *c) {
 void ensure_aux3(splhip_ctx *c) {
   if (c->aux3) return;
   hip_check(hipStreamCreateWithFlags(&c->aux3, hipStreamNonBlocking), "hipStreamCreate");
+  hip_check(hipStreamCreateWithFlags(&c->aux4, hipStreamNonBlocking), "hipStreamCreate");
   hip_check(hipEventCreateWithFlags(&c->ev_fork3, hipEventDisableTiming), "hipEventCreate");
   hip_check(hipEventCreateWithFlags(&c->ev_join3, hipEventDisableTiming), "hipEventCreate");
+  hip_check(hipEventCreateWithFlags(&c->ev_fork4, hipEventDisableTiming), "hipEventCreate");
+  hip_check(hipEventCreateWithFlags(&c->ev_join4, hipEventDisableTiming), "hipEventCreate");
 }
 
 
@@ -627,11 +631,12 @@ void make_plan(splhip_ctx *c, int nj, const JobIn *jobs, int G, Plan &pl, hipStr
 // [4..5] work items, [6..7] heavy items, [8..11] bin entries as u64,
 // [12..13] region entries, [14] the ordered pass has a non-finite q,
 // [15] capacity overflow bits of a device-count render (bit p: pass p),
-// [16..17] staged lists, [18..19] their groups of 4 entries (padded),
-// [20..21] the longest list.
+// [16..31] lists from stage_len << k entries on, [32..47] their groups of 4
+// entries (padded), k = 0..7 ([16 + 8 p + k]: pass p), [48..49] the longest
+// list.
 enum {
   T_M = 0, T_E = 2, T_ITEMS = 4, T_HEAVY = 6, T_E64 = 8, T_E16 = 12, T_QBAD = 14, T_OVF = 15,
-  T_STAGE = 16, T_SGRP = 18, T_LMAX = 20, T_N = 22
+  T_STAGE = 16, T_SGRP = 32, T_LMAX = 48, T_N = 50
 };
 
 // Project all jobs, then the compaction scan of each present pass and its job
@@ -854,6 +859,13 @@ constexpr uint64_t kStageBudget = uint64_t(32) << 30;
 // against 43.7 ms unstaged; 64 views: 65.6M ordered entries, 68.6 -> 48.1 ms;
 // 500k at 512^2: 121k-entry list, 1.3M entries, 6.21 -> 3.58 ms).
 constexpr uint64_t kStageOrderedRatio = 216;
+// Staged lists whose chains start right after their own operands (the
+// longest; the others' operands and chains follow beside them).
+constexpr uint32_t kStagePhase1 = 64;
+// Default ordered staging threshold: about the longest list / 16 (the
+// 64-view batch: 40.4 ms staged from 65536 entries on, 43.9 from 8192; 500k
+// at 512^2, longest list 121k: 2.67 ms from 8192, 3.39 from 65536).
+constexpr uint64_t kStageOrderedSpan = 16;
 
 Tuning tuning_of(const splhip_params *prm) {
   Tuning t;
@@ -875,11 +887,11 @@ Tuning tuning_of(const splhip_params *prm) {
   // list at about 100 ns per entry
   t.stage = prm->hip_stage < 0 ? 0xffffffffu
             : prm->hip_stage > 0 ? static_cast<uint32_t>(prm->hip_stage) : 8192u;
-  // ordered lists from 65536 entries on (for a pass bound by its longest
+  // ordered lists from 8192 entries on (for a pass bound by its longest
   // list, see kStageOrderedRatio)
   t.stage_ord = prm->hip_stage_ordered < 0 ? 0xffffffffu
                 : prm->hip_stage_ordered > 0 ? static_cast<uint32_t>(prm->hip_stage_ordered)
-                                             : 65536u;
+                                             : 8192u;
   t.stage_ord_auto = prm->hip_stage_ordered == 0;
   return t;
 }
@@ -916,34 +928,44 @@ void stage_items_count(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, uint32_
   // the count of this call's long-list items only (chunks and autotuning
   // candidates count again)
   launch_fill_u32(tot + T_HEAVY + p, 0u, 1, s);
-  launch_fill_u32(tot + T_STAGE + p, 0u, 1, s);
-  launch_fill_u32(tot + T_SGRP + p, 0u, 1, s);
+  launch_fill_u32(tot + T_STAGE + 8 * p, 0u, 8, s);
+  launch_fill_u32(tot + T_SGRP + 8 * p, 0u, 8, s);
   launch_fill_u32(tot + T_LMAX + p, 0u, 1, s);
   launch_items_count(pl.bin_off[p], pl.nunits, sa, p == 0 ? 1 : 0, tn.split, icnt,
-                     tot + T_HEAVY + p, s, tot + T_STAGE + p, tot + T_SGRP + p,
+                     tot + T_HEAVY + p, s, tot + T_STAGE + 8 * p, tot + T_SGRP + 8 * p,
                      tot + T_LMAX + p);
   scan_u32(icnt, ioff, pl.nunits, tot + T_ITEMS + p, ws, s);
   mark(c, p == 0 ? 2 : 5, false, s);
 }
 
 // Item counts of pass p from c->h_tot read from tot + T_ITEMS on (T_N -
-// T_ITEMS words): work items; long lists = the staged ones (when their
-// operands fit kStageBudget, else none) and then at most heavy_max for the
-// producer/consumer kernel.
+// T_ITEMS words): work items; long lists (one item each, sorted first) = the
+// staged ones (when their operands fit kStageBudget, else none), then at
+// most heavy_max of the others from heavy_len on for the producer/consumer
+// kernel; the rest go to the 8x8 splat.  Ordered passes by default stage the lists from
+// stage_len << k entries on, k the least with stage_len << k >= the longest
+// list / kStageOrderedSpan: shorter lists are off the critical path, and
+// staging them costs more than the 8x8 splat does.
 void take_items(splhip_ctx *c, Plan &pl, int p, const Tuning &tn) {
   const uint32_t *h = c->h_tot;
   pl.nitems[p] = pl.m[p] ? h[T_ITEMS - T_ITEMS + p] : 0u;
-  const uint32_t nlong = pl.m[p] ? h[T_HEAVY - T_ITEMS + p] : 0u;
-  uint32_t ns = pl.m[p] ? h[T_STAGE - T_ITEMS + p] : 0u;
-  const uint64_t groups = h[T_SGRP - T_ITEMS + p];
+  const uint32_t nheavy_lists = pl.m[p] ? h[T_HEAVY - T_ITEMS + p] : 0u;
+  const uint64_t lmax = h[T_LMAX - T_ITEMS + p];
+  const uint32_t len0 = p == 1 ? tn.stage : tn.stage_ord;
+  int k = 0;
+  if (p == 0 && tn.stage_ord_auto)
+    while (k < 7 && (static_cast<uint64_t>(len0) << k) * kStageOrderedSpan < lmax) ++k;
+  uint32_t ns = pl.m[p] ? h[T_STAGE - T_ITEMS + 8 * p + k] : 0u;
+  const uint64_t groups = h[T_SGRP - T_ITEMS + 8 * p + k];
   const uint64_t bytes = 4 * (stage_floats(p == 1, groups) + stage_q_floats(p == 1, groups));
   if (bytes > kStageBudget) ns = 0;
-  const uint64_t lmax = h[T_LMAX - T_ITEMS + p];
   if (p == 0 && tn.stage_ord_auto && lmax * kStageOrderedRatio < pl.etot[p]) ns = 0;
-  ns = std::min(ns, nlong);
   pl.nstage[p] = ns;
   pl.sgroups[p] = ns ? groups : 0;
-  const uint32_t npc = tn.heavy == 0xffffffffu ? 0u : std::min(nlong - ns, tn.heavy_max);
+  // staged lists from a threshold at least heavy_len are all among those
+  // from heavy_len on; from a lower one they include all of those
+  const uint32_t npc = tn.heavy == 0xffffffffu || nheavy_lists <= ns
+                           ? 0u : std::min(nheavy_lists - ns, tn.heavy_max);
   // the heaviest items (sorted first); a unit's channel items are
   // independent, so the cut may fall between them
   pl.nheavy[p] = std::min(ns + npc, pl.nitems[p]);
@@ -1022,12 +1044,24 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
       float4 *ops = ws.get_as<float4>(pslot(p, P_SOPS), stage_floats(add, pl.sgroups[p]) / 4);
       float *qv = add ? nullptr
                       : ws.get_as<float>(pslot(p, P_SQ), stage_q_floats(add, pl.sgroups[p]));
+      // the kStagePhase1 longest lists first: their chains start after their
+      // own operands; the other lists' operands and chains follow on a fourth
+      // stream, beside those chains
+      const uint32_t n1 = std::min(nstage, kStagePhase1);
       ensure_aux3(c);
       hip_check(hipEventRecord(c->ev_fork3, s), "record fork");
       hip_check(hipStreamWaitEvent(c->aux3, c->ev_fork3, 0), "wait fork");
       mark(c, p == 0 ? 8 : 10, true, c->aux3);
-      launch_stage(add, items, nstage, pl.sgroups[p], cb, cbt, ops, qv, sa, c->aux3);
-      launch_stage_chain(add, items, nstage, cb, ops, qv, sa, c->aux3);
+      launch_stage_base(items, nstage, cb, cbt, sa, c->aux3);
+      launch_stage_ops(add, items, 0, n1, pl.sgroups[p], cb, cbt, ops, qv, sa, c->aux3);
+      if (n1 < nstage) {
+        hip_check(hipEventRecord(c->ev_fork4, c->aux3), "record fork");
+        hip_check(hipStreamWaitEvent(c->aux4, c->ev_fork4, 0), "wait fork");
+        launch_stage_ops(add, items, n1, nstage, pl.sgroups[p], cb, cbt, ops, qv, sa, c->aux4);
+        launch_stage_chain(add, items, n1, nstage, cb, ops, qv, sa, c->aux4);
+        hip_check(hipEventRecord(c->ev_join4, c->aux4), "record join");
+      }
+      launch_stage_chain(add, items, 0, n1, cb, ops, qv, sa, c->aux3);
       mark(c, p == 0 ? 8 : 10, false, c->aux3);
       hip_check(hipEventRecord(c->ev_join3, c->aux3), "record join");
     }
@@ -1039,6 +1073,7 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
     mark(c, p == 0 ? 7 : 9, false, s);
     if (npc > 0 && fork) hip_check(hipStreamWaitEvent(s, c->ev_join, 0), "wait join");
     if (nstage > 0) hip_check(hipStreamWaitEvent(s, c->ev_join3, 0), "wait join");
+    if (nstage > kStagePhase1) hip_check(hipStreamWaitEvent(s, c->ev_join4, 0), "wait join");
   } else if (npc > 0) {
     mark(c, p == 0 ? 8 : 10, true, s);
     launch_splat_pc(p == 1, pit, npc, sp, s, dc);
@@ -1868,7 +1903,10 @@ void splhip_destroy(splhip_ctx *c) {
   if (c->aux2) (void)hipStreamDestroy(c->aux2);
   if (c->ev_fork3) (void)hipEventDestroy(c->ev_fork3);
   if (c->ev_join3) (void)hipEventDestroy(c->ev_join3);
+  if (c->ev_fork4) (void)hipEventDestroy(c->ev_fork4);
+  if (c->ev_join4) (void)hipEventDestroy(c->ev_join4);
   if (c->aux3) (void)hipStreamDestroy(c->aux3);
+  if (c->aux4) (void)hipStreamDestroy(c->aux4);
   c->ws.release();
   if (c->d_exptab) (void)hipFree(c->d_exptab);
   if (c->d_palx) (void)hipFree(c->d_palx);

@@ -147,10 +147,10 @@ void launch_unpack_records(const char *stage, int64_t n, int64_t C, int ncol, fl
 
 // Wave-per-8x8-block splat (splat8.hip): work items (unit << 2 | channel,
 // channel 3 = all), heaviest first.  heavy_cnt (device, optional): incremented
-// once per long unit (list at least min(a.heavy_len, a.stage_len)); stage_cnt /
-// stage_grp (optional): once per unit whose list reaches a.stage_len, and by
-// its groups of 4 entries (padded to whole chain loads); lmax (optional): the
-// longest list of a unit with items.
+// once per unit whose list reaches a.heavy_len (one item); stage_cnt /
+// stage_grp (optional, [8] each): [j] once per unit whose list reaches
+// a.stage_len << j, and by its groups of 4 entries (padded to whole chain
+// loads); lmax (optional): the longest list of a unit with items.
 void launch_items_count(const uint32_t *bin_off, uint32_t nunits, const SplatArgs &a,
                         int ordered, uint32_t split, uint32_t *cnt, uint32_t *heavy_cnt,
                         hipStream_t s, uint32_t *stage_cnt = nullptr,
@@ -173,17 +173,21 @@ void launch_splat8(bool additive, int variant, const uint32_t *items, uint32_t n
                    const SplatArgs &a, hipStream_t s, const ItemCounts &dc = ItemCounts{},
                    int ppw = 0);
 // Staged lists (splat8.hip), the first n items of a pass (groups: their
-// groups of 4 entries, padded): list offsets into cbase / cbat ([n + 1] each)
-// and the per-pixel operands into ops (stage_floats floats) and, ordered, the
-// entries' q_c into qv (stage_q_floats floats); then the chains, one wave per
-// list, channel and 16-pixel slice, which write the lists' blocks.
+// groups of 4 entries, padded): list offsets into cbase / cbat ([n + 1] each);
+// then, for lists [i0, i1), the per-pixel operands into ops (stage_floats
+// floats for all n) and, ordered, the entries' q_c into qv (stage_q_floats
+// floats), and the chains, one wave per list, channel and 16-pixel slice,
+// which write the lists' blocks.
 uint64_t stage_floats(bool additive, uint64_t groups);
 uint64_t stage_q_floats(bool additive, uint64_t groups);
-void launch_stage(bool additive, const uint32_t *items, uint32_t n, uint64_t groups,
-                  uint32_t *cbase, uint32_t *cbat, float4 *ops, float *qv, const SplatArgs &a,
-                  hipStream_t s);
-void launch_stage_chain(bool additive, const uint32_t *items, uint32_t n, const uint32_t *cbase,
-                        const float4 *ops, const float *qv, const SplatArgs &a, hipStream_t s);
+void launch_stage_base(const uint32_t *items, uint32_t n, uint32_t *cbase, uint32_t *cbat,
+                       const SplatArgs &a, hipStream_t s);
+void launch_stage_ops(bool additive, const uint32_t *items, uint32_t i0, uint32_t i1,
+                      uint64_t groups, const uint32_t *cbase, const uint32_t *cbat, float4 *ops,
+                      float *qv, const SplatArgs &a, hipStream_t s);
+void launch_stage_chain(bool additive, const uint32_t *items, uint32_t i0, uint32_t i1,
+                        const uint32_t *cbase, const float4 *ops, const float *qv,
+                        const SplatArgs &a, hipStream_t s);
 // Producer/consumer workgroups (splat8.hip), one per item: the first npc
 // items, the units whose lists reach a.heavy_len.
 void launch_splat_pc(bool additive, const uint32_t *items, uint32_t npc, const SplatArgs &a,

@@ -1084,11 +1084,13 @@ __global__ __launch_bounds__(256) void k_stage_base(SplatArgs a, const uint32_t 
   }
 }
 
-// Operands of the staged lists: the batches of 64 entries (cbat[n] of them)
-// split evenly over the launch's waves, each wave walking a contiguous run
-// (phase A as the 8x8 splat's, then lane = pixel for the batch's groups).
+// Operands of staged lists [i0, i1): their batches of 64 entries (cbat[i0]
+// to cbat[i1]) split evenly over the launch's waves, each wave walking a
+// contiguous run (phase A as the 8x8 splat's, then lane = pixel for the
+// batch's groups).
 template <int ADD>
-__global__ __launch_bounds__(256) void k_stage_ops(SplatArgs a, const uint32_t *items, uint32_t n,
+__global__ __launch_bounds__(256) void k_stage_ops(SplatArgs a, const uint32_t *items,
+                                                   uint32_t i0, uint32_t i1,
                                                    const uint32_t *cbase, const uint32_t *cbat,
                                                    float4 *ops, float *qv) {
   constexpr uint32_t P = stage_planes<ADD>();
@@ -1099,14 +1101,14 @@ __global__ __launch_bounds__(256) void k_stage_ops(SplatArgs a, const uint32_t *
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int i = tid; i < 2048; i += 256) s_tab[i] = a.exptab[i];
   __syncthreads();
-  const uint32_t total = uniform(cbat[n]);
+  const uint32_t first = uniform(cbat[i0]), total = uniform(cbat[i1]) - first;
   const uint32_t nw = gridDim.x * 4u, w = blockIdx.x * 4u + static_cast<uint32_t>(wave);
   const uint32_t span = (total + nw - 1) / nw;
-  uint32_t gb = w * span;
-  const uint32_t gend = min(total, gb + span);
+  uint32_t gb = first + w * span;
+  const uint32_t gend = first + min(total, w * span + span);
   if (gb >= gend) return;
   // the staged item holding batch gb: cbat[lo] <= gb < cbat[lo + 1]
-  uint32_t lo = 0, hi = n;
+  uint32_t lo = i0, hi = i1;
   while (hi - lo > 1) {
     const uint32_t mid = (lo + hi) >> 1;
     if (uniform(cbat[mid]) <= gb) lo = mid;
@@ -1200,13 +1202,13 @@ __global__ __launch_bounds__(256) void k_stage_ops(SplatArgs a, const uint32_t *
 constexpr int kS = kStageLoad, kPix = 64 / kS;
 template <int ADD>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_stage_chain(
-    SplatArgs a, const uint32_t *items, const uint32_t *cbase, const float4 *__restrict__ ops,
-    const float *__restrict__ qv) {
+    SplatArgs a, const uint32_t *items, uint32_t i0, const uint32_t *cbase,
+    const float4 *__restrict__ ops, const float *__restrict__ qv) {
   constexpr uint32_t P = stage_planes<ADD>();
-  constexpr int kR = ADD ? 40 : 20;  // loads in flight (ordered: two planes per step)
-  __shared__ float4 xs[2][64];
-  const uint32_t i = blockIdx.x / (3u * kS);
-  const uint32_t r3 = blockIdx.x - i * (3u * kS);
+  constexpr int kR = ADD ? 40 : 10;  // steps in flight (ordered: three loads per step)
+  __shared__ float4 xs[3][64];
+  const uint32_t i = i0 + blockIdx.x / (3u * kS);
+  const uint32_t r3 = blockIdx.x % (3u * kS);
   const int c = static_cast<int>(r3 / kS), h = static_cast<int>(r3 % kS);
   const int lane = threadIdx.x, qo = lane / kPix, pl = lane % kPix;
   const int px = h * kPix + pl;  // pixel cx * 8 + cy
@@ -1230,9 +1232,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
   // group g of plane c and this pixel: src[g * P * 64]; lane quarter qo reads g + qo
   const float4 *src = ops + static_cast<size_t>(gb0) * P * 64u + static_cast<size_t>(c) * 64u + px;
   const float4 *srcw = ops + static_cast<size_t>(gb0) * P * 64u + 3u * 64u + px;  // m2 (ordered)
-  const float *qsrc = qv + static_cast<size_t>(gb0 / kS) * 48u + static_cast<size_t>(c) * 16u;
+  // q_c of the entries of group g + qo (4 floats; the same for a quarter's lanes)
+  const float4 *qsrc = reinterpret_cast<const float4 *>(qv + static_cast<size_t>(gb0 / kS) * 48u +
+                                                       static_cast<size_t>(c) * 16u) + qo;
   auto at_load = [&](uint32_t g0) {  // groups [g0, g0 + kS), the last ones again past the list
     return static_cast<size_t>(min<uint32_t>(g0, G - kS) + qo) * P * 64u;
+  };
+  auto at_q = [&](uint32_t g0) {  // that load's q_c (48 floats per load of kS groups)
+    return static_cast<size_t>(min<uint32_t>(g0, G - kS) / kS) * 12u;
   };
   // one load's 16 entries for lanes [0, kPix): its own quarter, then the
   // other three read back through LDS
@@ -1245,18 +1252,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     return Q{v, xs[slot][pl + kPix], xs[slot][pl + 2 * kPix], xs[slot][pl + 3 * kPix]};
   };
   struct Step {
-    Q m, w;
-    float q[16];  // ordered: q_c of the load's entries (uniform)
+    Q m, w, q;  // ordered: m1_c, m2 and q_c (the same in every lane) of the load's entries
   };
-  // the exchange of the load at group g0 (and its q_c, read with scalar loads)
-  auto step_of = [&](const float4 &vm, const float4 &vw, uint32_t g0) {
+  auto step_of = [&](const float4 &vm, const float4 &vw, const float4 &vq) {
     Step st;
     st.m = exchange(0, vm);
     if (!ADD) {
       st.w = exchange(1, vw);
-      const float *qp = qsrc + static_cast<size_t>(min<uint32_t>(g0, G - kS) / kS) * 48u;
-#pragma unroll
-      for (int e = 0; e < 16; ++e) st.q[e] = qp[e];
+      st.q = exchange(2, vq);
     }
     return st;
   };
@@ -1273,39 +1276,46 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     } else {
       const float4 *m4[4] = {&st.m.a, &st.m.b, &st.m.c, &st.m.d};
       const float4 *w4[4] = {&st.w.a, &st.w.b, &st.w.c, &st.w.d};
+      const float4 *q4[4] = {&st.q.a, &st.q.b, &st.q.c, &st.q.d};
 #pragma unroll
       for (int k = 0; k < 4; ++k) {  // load quarter k: entries 4k .. 4k + 3
-        l = l + ((l - st.q[4 * k]) * m4[k]->x) * w4[k]->x;
-        l = l + ((l - st.q[4 * k + 1]) * m4[k]->y) * w4[k]->y;
-        l = l + ((l - st.q[4 * k + 2]) * m4[k]->z) * w4[k]->z;
-        l = l + ((l - st.q[4 * k + 3]) * m4[k]->w) * w4[k]->w;
+        l = l + ((l - q4[k]->x) * m4[k]->x) * w4[k]->x;
+        l = l + ((l - q4[k]->y) * m4[k]->y) * w4[k]->y;
+        l = l + ((l - q4[k]->z) * m4[k]->z) * w4[k]->z;
+        l = l + ((l - q4[k]->w) * m4[k]->w) * w4[k]->w;
       }
     }
   };
-  float4 bm[kR], bw[ADD ? 1 : kR];
+  float4 bm[kR], bw[ADD ? 1 : kR], bq[ADD ? 1 : kR];
 #pragma unroll
   for (int r = 0; r < kR; ++r) {  // issued in the order they are consumed
     const size_t o = at_load(r * kS);
     bm[r] = src[o];
-    if (!ADD) bw[r] = srcw[o];
+    if (!ADD) {
+      bw[r] = srcw[o];
+      bq[r] = qsrc[at_q(r * kS)];
+    }
     asm volatile("" ::: "memory");
   }
   // Software pipeline over the loads: step r of a round applies the entries
   // of load r (exchanged a step earlier), exchanges load r + 1 meanwhile, and
   // refills that load's registers with the groups kR loads further on.
-  Step cur = step_of(bm[0], bw[0], 0);
+  Step cur = step_of(bm[0], bw[0], bq[0]);
   asm volatile("" ::: "memory");
   {
     const size_t o = at_load(kR * kS);
     bm[0] = src[o];
-    if (!ADD) bw[0] = srcw[o];
+    if (!ADD) {
+      bw[0] = srcw[o];
+      bq[0] = qsrc[at_q(kR * kS)];
+    }
   }
   for (uint32_t g = 0; g < G; g += kR * kS) {
 #pragma unroll
     for (int r = 0; r < kR; ++r) {
       const int rn = (r + 1) % kR;
       const uint32_t g0 = g + r * kS;
-      const Step nxt = step_of(bm[rn], bw[ADD ? 0 : rn], g0 + kS);
+      const Step nxt = step_of(bm[rn], bw[ADD ? 0 : rn], bq[ADD ? 0 : rn]);
       if (g0 < G) apply(cur);
       cur = nxt;
       // the barrier orders the exchange and sums before the refill (and the
@@ -1314,7 +1324,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
       asm volatile("" : "+v"(l) : : "memory");
       const size_t o = at_load(g + (r + 1) * kS + kR * kS);
       bm[rn] = src[o];
-      if (!ADD) bw[rn] = srcw[o];
+      if (!ADD) {
+        bw[rn] = srcw[o];
+        bq[rn] = qsrc[at_q(g + (r + 1) * kS + kR * kS)];
+      }
     }
   }
   if (a.trace && c == 0 && h == 0 && lane == 0) {
@@ -1369,10 +1382,14 @@ __global__ __launch_bounds__(256) void k_items_count(SplatArgs a, uint32_t nunit
     uint32_t len;
     const uint32_t k = unit_items(a, static_cast<uint32_t>(u), ordered, split, len);
     cnt[u] = k;
-    if (heavy_cnt && k > 0 && len >= long_len_of(a)) atomicAdd(heavy_cnt, k);
-    if (stage_cnt && k > 0 && len >= stage_len_of(a)) {
-      atomicAdd(stage_cnt, 1u);
-      atomicAdd(stage_grp, stage_groups(len));
+    if (heavy_cnt && k > 0 && len >= heavy_len_of(a)) atomicAdd(heavy_cnt, k);
+    const uint32_t sl = stage_len_of(a);
+    if (stage_cnt && k > 0 && len >= sl) {  // counts per threshold sl << j, j = 0..7
+      for (int j = 0; j < 8; ++j) {
+        if (static_cast<uint64_t>(len) < (static_cast<uint64_t>(sl) << j)) break;
+        atomicAdd(stage_cnt + j, 1u);
+        atomicAdd(stage_grp + j, stage_groups(len));
+      }
     }
     if (lmax && k > 0) atomicMax(lmax, len);
   }
@@ -1431,25 +1448,32 @@ uint64_t stage_q_floats(bool additive, uint64_t groups) {
   return additive ? 0 : groups / kStageLoad * 48u;
 }
 
-void launch_stage(bool additive, const uint32_t *items, uint32_t n, uint64_t groups,
-                  uint32_t *cbase, uint32_t *cbat, float4 *ops, float *qv, const SplatArgs &a,
-                  hipStream_t s) {
+void launch_stage_base(const uint32_t *items, uint32_t n, uint32_t *cbase, uint32_t *cbat,
+                       const SplatArgs &a, hipStream_t s) {
   if (n == 0) return;
   k_stage_base<<<1, 256, 0, s>>>(a, items, n, cbase, cbat);
   hip_check(hipGetLastError(), "k_stage_base launch");
-  // batches: at most groups / 16 + n; about four per wave when there are many
-  const uint64_t nbat = groups / 16 + n;
+}
+
+void launch_stage_ops(bool additive, const uint32_t *items, uint32_t i0, uint32_t i1,
+                      uint64_t groups, const uint32_t *cbase, const uint32_t *cbat, float4 *ops,
+                      float *qv, const SplatArgs &a, hipStream_t s) {
+  if (i1 <= i0) return;
+  // batches: at most groups / 16 + lists; about four per wave when there are many
+  const uint64_t nbat = groups / 16 + (i1 - i0);
   const unsigned grid = std::max(1u, static_cast<unsigned>(std::min<uint64_t>(2048, (nbat + 15) / 16)));
-  if (additive) k_stage_ops<1><<<grid, 256, 0, s>>>(a, items, n, cbase, cbat, ops, qv);
-  else k_stage_ops<0><<<grid, 256, 0, s>>>(a, items, n, cbase, cbat, ops, qv);
+  if (additive) k_stage_ops<1><<<grid, 256, 0, s>>>(a, items, i0, i1, cbase, cbat, ops, qv);
+  else k_stage_ops<0><<<grid, 256, 0, s>>>(a, items, i0, i1, cbase, cbat, ops, qv);
   hip_check(hipGetLastError(), "k_stage_ops launch");
 }
 
-void launch_stage_chain(bool additive, const uint32_t *items, uint32_t n, const uint32_t *cbase,
-                        const float4 *ops, const float *qv, const SplatArgs &a, hipStream_t s) {
-  if (n == 0) return;
-  if (additive) k_stage_chain<1><<<3 * kS * n, 64, 0, s>>>(a, items, cbase, ops, qv);
-  else k_stage_chain<0><<<3 * kS * n, 64, 0, s>>>(a, items, cbase, ops, qv);
+void launch_stage_chain(bool additive, const uint32_t *items, uint32_t i0, uint32_t i1,
+                        const uint32_t *cbase, const float4 *ops, const float *qv,
+                        const SplatArgs &a, hipStream_t s) {
+  if (i1 <= i0) return;
+  const unsigned grid = 3u * kS * (i1 - i0);
+  if (additive) k_stage_chain<1><<<grid, 64, 0, s>>>(a, items, i0, cbase, ops, qv);
+  else k_stage_chain<0><<<grid, 64, 0, s>>>(a, items, i0, cbase, ops, qv);
   hip_check(hipGetLastError(), "k_stage_chain launch");
 }
 
