@@ -78,6 +78,9 @@ struct splhip_ctx {
   std::map<uint64_t, std::vector<uint32_t>> tuned;
   // variant, split, heavy cap (0: none), tuned, work region, ppw
   uint32_t last_tuning[6] = {0, 0, 0, 0, 0, 0};
+  // staged lists and their groups of the last render's passes (capture_reserve)
+  uint32_t last_stage_n[2] = {0, 0};
+  uint64_t last_stage_groups[2] = {0, 0};
   int64_t last_regions[2] = {0, 0};  // region entries binned per pass (0: per block)
   // auxiliary stream (default priority) for the heavy-block kernel
   hipStream_t aux = nullptr;
@@ -633,10 +636,11 @@ void make_plan(splhip_ctx *c, int nj, const JobIn *jobs, int G, Plan &pl, hipStr
 // [15] capacity overflow bits of a device-count render (bit p: pass p),
 // [16..31] lists from stage_len << k entries on, [32..47] their groups of 4
 // entries (padded), k = 0..7 ([16 + 8 p + k]: pass p), [48..49] the longest
-// list.
+// list, [50..57] a device-count render's long-list partition (4 per pass:
+// staged, producer/consumer, both).
 enum {
   T_M = 0, T_E = 2, T_ITEMS = 4, T_HEAVY = 6, T_E64 = 8, T_E16 = 12, T_QBAD = 14, T_OVF = 15,
-  T_STAGE = 16, T_SGRP = 32, T_LMAX = 48, T_N = 50
+  T_STAGE = 16, T_SGRP = 32, T_LMAX = 48, T_PART = 50, T_N = 58
 };
 
 // Project all jobs, then the compaction scan of each present pass and its job
@@ -912,7 +916,8 @@ SplatArgs splat_args(splhip_ctx *c, const Plan &pl, int p, const Tuning &tn) {
   sa.taylorlimit = c->consts.taylorlimit;
   sa.prio_len = tn.split;
   sa.heavy_len = tn.heavy;
-  sa.stage_len = pl.dc ? 0xffffffffu : p == 1 ? tn.stage : tn.stage_ord;
+  // device counts: staging only where the reservation holds room for it
+  sa.stage_len = (pl.dc && pl.nstage[p] == 0) ? 0xffffffffu : p == 1 ? tn.stage : tn.stage_ord;
   sa.qbad = p == 0 ? pl.qbad : nullptr;
   return sa;
 }
@@ -962,6 +967,8 @@ void take_items(splhip_ctx *c, Plan &pl, int p, const Tuning &tn) {
   if (p == 0 && tn.stage_ord_auto && lmax * kStageOrderedRatio < pl.etot[p]) ns = 0;
   pl.nstage[p] = ns;
   pl.sgroups[p] = ns ? groups : 0;
+  c->last_stage_n[p] = ns;
+  c->last_stage_groups[p] = pl.sgroups[p];
   // staged lists from a threshold at least heavy_len are all among those
   // from heavy_len on; from a lower one they include all of those
   const uint32_t npc = tn.heavy == 0xffffffffu || nheavy_lists <= ns
@@ -996,10 +1003,29 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
   // device counts: nitems and nheavy are capacities; the kernels read the
   // pass's item and long-list totals and skip past the long lists themselves
   ItemCounts dc;
+  uint32_t *part = nullptr;  // device counts: staged, producer/consumer, both
   if (pl.dc) {
+    // the long-list partition from the device counts, as take_items makes it
+    part = pl.tot + T_PART + 4 * p;
+    PartitionArgs pa;
+    pa.stage_cnt = pl.tot + T_STAGE + 8 * p;
+    pa.stage_grp = pl.tot + T_SGRP + 8 * p;
+    pa.heavy_cnt = pl.tot + T_HEAVY + p;
+    pa.lmax = pl.tot + T_LMAX + p;
+    pa.etot = reinterpret_cast<const uint64_t *>(pl.tot + T_E64 + 2 * p);
+    pa.len0 = pl.nstage[p] ? (p == 1 ? tn.stage : tn.stage_ord) : 0xffffffffu;
+    pa.ordered_auto = p == 0 && tn.stage_ord_auto;
+    pa.ns_cap = pl.nstage[p];
+    pa.groups_cap = pl.sgroups[p];
+    pa.heavy_max = tn.heavy == 0xffffffffu ? 0u : tn.heavy_max;
+    pa.span = static_cast<uint32_t>(kStageOrderedSpan);
+    pa.ratio = static_cast<uint32_t>(kStageOrderedRatio);
+    launch_partition(pa, part, s);
     dc.items = pl.tot + T_ITEMS + p;
-    dc.heavy = pl.tot + T_HEAVY + p;
-    dc.heavy_cap = nheavy;
+    dc.heavy = part + 2;
+    dc.heavy_cap = 0xffffffffu;
+    dc.first = part;
+    dc.npc = part + 1;
   }
   const int rs = radix_sort_pairs(ik0, iv0, ik1, iv1, nitems, 0, 32, wt ? *wt : ws, s, dc.items);
   const uint32_t *items = rs ? iv1 : iv0;
@@ -1013,11 +1039,12 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
   const uint32_t *nit = pl.dc ? items : items + nheavy;
   const uint32_t nn = pl.dc ? nitems : nitems - nheavy;
   // long lists: the staged ones first, then the producer/consumer workgroups'
-  const uint32_t nstage = pl.dc ? 0u : std::min(pl.nstage[p], nheavy);
-  const uint32_t npc = nheavy - nstage;
+  // (device counts: capacities; the kernels take the partition from part)
+  const uint32_t nstage = pl.dc ? pl.nstage[p] : std::min(pl.nstage[p], nheavy);
+  const uint32_t npc = pl.dc ? (tn.heavy == 0xffffffffu ? 0u : tn.heavy_max) : nheavy - nstage;
   SplatArgs sp = sa;
   if (sp.trace) sp.trace += 4 * static_cast<size_t>(nstage);
-  const uint32_t *pit = items + nstage;
+  const uint32_t *pit = pl.dc ? items : items + nstage;
   if (nstage > 0 || (npc > 0 && fork)) {
     // heavy blocks run on second streams, beside the normal items: the
     // producer/consumer workgroups at once, the staged chains after the
@@ -1052,16 +1079,17 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
       hip_check(hipEventRecord(c->ev_fork3, s), "record fork");
       hip_check(hipStreamWaitEvent(c->aux3, c->ev_fork3, 0), "wait fork");
       mark(c, p == 0 ? 8 : 10, true, c->aux3);
-      launch_stage_base(items, nstage, cb, cbt, sa, c->aux3);
-      launch_stage_ops(add, items, 0, n1, pl.sgroups[p], cb, cbt, ops, qv, sa, c->aux3);
+      launch_stage_base(items, nstage, cb, cbt, sa, c->aux3, part);
+      launch_stage_ops(add, items, 0, n1, pl.sgroups[p], cb, cbt, ops, qv, sa, c->aux3, part);
       if (n1 < nstage) {
         hip_check(hipEventRecord(c->ev_fork4, c->aux3), "record fork");
         hip_check(hipStreamWaitEvent(c->aux4, c->ev_fork4, 0), "wait fork");
-        launch_stage_ops(add, items, n1, nstage, pl.sgroups[p], cb, cbt, ops, qv, sa, c->aux4);
-        launch_stage_chain(add, items, n1, nstage, cb, ops, qv, sa, c->aux4);
+        launch_stage_ops(add, items, n1, nstage, pl.sgroups[p], cb, cbt, ops, qv, sa, c->aux4,
+                         part);
+        launch_stage_chain(add, items, n1, nstage, cb, ops, qv, sa, c->aux4, part);
         hip_check(hipEventRecord(c->ev_join4, c->aux4), "record join");
       }
-      launch_stage_chain(add, items, 0, n1, cb, ops, qv, sa, c->aux3);
+      launch_stage_chain(add, items, 0, n1, cb, ops, qv, sa, c->aux3, part);
       mark(c, p == 0 ? 8 : 10, false, c->aux3);
       hip_check(hipEventRecord(c->ev_join3, c->aux3), "record join");
     }
@@ -1602,6 +1630,8 @@ struct Reservation {
   int nj = 0;
   bool use16[2] = {false, false};
   uint32_t ecap[2] = {0, 0}, e16cap[2] = {0, 0};
+  uint32_t ns_cap[2] = {0, 0};        // staged lists (0: no staging)
+  uint64_t groups_cap[2] = {0, 0};    // and their groups of 4 entries
   Tuning tn{};
   Workspace ws;
 };
@@ -1645,6 +1675,10 @@ int render_jobs_dc(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s) {
   }
   stage_project_compact(c, pl, rec, act, tot, s);
   launch_job_flags(pl.d_mo[0], pl.d_mo[1], nj, pl.d_js, s);
+  for (int p = 0; p < 2; ++p) {
+    pl.nstage[p] = rv.ns_cap[p];
+    pl.sgroups[p] = rv.groups_cap[p];
+  }
   for (int p = 0; p < 2; ++p)
     if (pl.m[p]) {
       stage_order(c, pl, p, rec, act, s);
@@ -1656,7 +1690,7 @@ int render_jobs_dc(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s) {
     if (pl.m[p]) {
       stage_items_count(c, pl, p, tn, tot, s);
       pl.nitems[p] = 3u * pl.nunits;  // at most three items per unit
-      pl.nheavy[p] = tn.heavy == 0xffffffffu ? 0u : tn.heavy_max;
+      pl.nheavy[p] = pl.nstage[p] + (tn.heavy == 0xffffffffu ? 0u : tn.heavy_max);
     }
   for (int p = 0; p < 2; ++p)
     if (pl.m[p]) stage_splat(c, pl, p, tn, s);
@@ -1707,6 +1741,18 @@ int capture_reserve(splhip_ctx *c, int nj, const JobIn *jobs, float headroom, hi
     rv->use16[p] = c->last_regions[p] > 0;
     rv->ecap[p] = cap(c->last_entries[p]);
     rv->e16cap[p] = cap(c->last_regions[p]);
+    // staging where the sizing render staged
+    if (c->last_stage_n[p] > 0) {
+      rv->ns_cap[p] = static_cast<uint32_t>(std::min(
+          static_cast<double>(c->last_stage_n[p]) * headroom + 16.0, 1048576.0));
+      const double g = static_cast<double>(c->last_stage_groups[p]) * headroom + 1024.0;
+      const uint64_t gc = static_cast<uint64_t>(g / kStageLoad + 1) * kStageLoad;
+      if (4 * (stage_floats(p == 1, gc) + stage_q_floats(p == 1, gc)) <= kStageBudget) {
+        rv->groups_cap[p] = gc;
+      } else {
+        rv->ns_cap[p] = 0;
+      }
+    }
   }
   Plan pl;
   build_plan(c, nj, jobs, 8, pl);
@@ -1716,6 +1762,7 @@ int capture_reserve(splhip_ctx *c, int nj, const JobIn *jobs, float headroom, hi
   hip_check(hipMemcpy(rv->d_tables, rv->tables.data(), total, hipMemcpyHostToDevice),
             "upload reserved tables");
   ensure_aux(c);
+  ensure_aux3(c);
   c->rv = rv;
   if (int rc = render_jobs_dc(c, nj, jobs, s)) {
     delete c->rv;

@@ -161,14 +161,16 @@ void launch_items_emit(const uint32_t *bin_off, uint32_t nunits, const SplatArgs
 // Device item counts of a pass (all null: the host counts are exact).
 struct ItemCounts {
   const uint32_t *items = nullptr;  // work items of the pass
-  const uint32_t *heavy = nullptr;  // of which lists that reach a.heavy_len (sorted first)
-  uint32_t heavy_cap = 0;           // at most this many go to the producer/consumer kernel
+  const uint32_t *heavy = nullptr;  // of which the long lists (sorted first)
+  uint32_t heavy_cap = 0;           // the 8x8 splat skips min(*heavy, heavy_cap) items
+  const uint32_t *first = nullptr;  // producer/consumer items: from *first on,
+  const uint32_t *npc = nullptr;    // *npc of them (a grid sized for a capacity)
 };
 // With dc set, launch_splat8 is given the pass's whole item array and a grid
 // for nitems (a capacity) and skips the first min(*dc.heavy, heavy_cap) items;
-// launch_splat_pc is given npc = heavy_cap.  ppw (variant 4): particles per
-// chunk of the walk, 1, 2, 4 or 8 (0: 8 for one channel, 8 additive / 4
-// ordered for all three).
+// launch_splat_pc is given the whole item array and npc = its capacity.  ppw
+// (variant 4): particles per chunk of the walk, 1, 2, 4 or 8 (0: 8 for one
+// channel, 8 additive / 4 ordered for all three).
 void launch_splat8(bool additive, int variant, const uint32_t *items, uint32_t nitems,
                    const SplatArgs &a, hipStream_t s, const ItemCounts &dc = ItemCounts{},
                    int ppw = 0);
@@ -177,17 +179,37 @@ void launch_splat8(bool additive, int variant, const uint32_t *items, uint32_t n
 // then, for lists [i0, i1), the per-pixel operands into ops (stage_floats
 // floats for all n) and, ordered, the entries' q_c into qv (stage_q_floats
 // floats), and the chains, one wave per list, channel and 16-pixel slice,
-// which write the lists' blocks.
+// which write the lists' blocks.  dn (device, optional): the staged lists
+// are min(n, *dn) (n, i1 are capacities).
+constexpr int kStageLoad = 4;  // groups of 4 entries per load of a staged chain
 uint64_t stage_floats(bool additive, uint64_t groups);
 uint64_t stage_q_floats(bool additive, uint64_t groups);
 void launch_stage_base(const uint32_t *items, uint32_t n, uint32_t *cbase, uint32_t *cbat,
-                       const SplatArgs &a, hipStream_t s);
+                       const SplatArgs &a, hipStream_t s, const uint32_t *dn = nullptr);
 void launch_stage_ops(bool additive, const uint32_t *items, uint32_t i0, uint32_t i1,
                       uint64_t groups, const uint32_t *cbase, const uint32_t *cbat, float4 *ops,
-                      float *qv, const SplatArgs &a, hipStream_t s);
+                      float *qv, const SplatArgs &a, hipStream_t s,
+                      const uint32_t *dn = nullptr);
 void launch_stage_chain(bool additive, const uint32_t *items, uint32_t i0, uint32_t i1,
                         const uint32_t *cbase, const float4 *ops, const float *qv,
-                        const SplatArgs &a, hipStream_t s);
+                        const SplatArgs &a, hipStream_t s, const uint32_t *dn = nullptr);
+// Device-count partition of a pass's long lists (one thread), as the host
+// does it with exact counts: the threshold level k (0, or for ordered_auto
+// the least with len0 << k >= *lmax / span), the staged lists stage_cnt[k]
+// (none past ns_cap or groups_cap, or for ordered_auto when *lmax * ratio <
+// *etot), then at most heavy_max of the other lists from heavy_len on (the
+// count heavy_cnt).  out[0] staged, out[1] producer/consumer, out[2] both.
+struct PartitionArgs {
+  const uint32_t *stage_cnt, *stage_grp, *heavy_cnt, *lmax;
+  const uint64_t *etot;
+  uint32_t len0;  // 0xffffffff: no staging
+  int ordered_auto;
+  uint32_t ns_cap;
+  uint64_t groups_cap;
+  uint32_t heavy_max;  // 0: no producer/consumer workgroups
+  uint32_t span, ratio;
+};
+void launch_partition(const PartitionArgs &pa, uint32_t *out, hipStream_t s);
 // Producer/consumer workgroups (splat8.hip), one per item: the first npc
 // items, the units whose lists reach a.heavy_len.
 void launch_splat_pc(bool additive, const uint32_t *items, uint32_t npc, const SplatArgs &a,

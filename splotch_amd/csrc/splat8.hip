@@ -657,10 +657,10 @@ __device__ __forceinline__ void lds_barrier() {
 template <int ADD>
 __global__ __launch_bounds__(kPThreads) void k_splat_pc(SplatArgs a, const uint32_t *items,
                                                         uint32_t npc, ItemCounts dc) {
-  // device counts: the first min(long-list total, cap) items; the rest of the
-  // grid (sized for the cap) has no work (uniform per workgroup, before any
+  // device counts: *dc.npc items from *dc.first on; the rest of the grid
+  // (sized for a capacity) has no work (uniform per workgroup, before any
   // barrier)
-  if (dc.items && blockIdx.x >= min(min(*dc.heavy, dc.heavy_cap), *dc.items)) return;
+  if (dc.npc && blockIdx.x >= *dc.npc) return;
   // One LDS block with the exp table at address 0, so that a table entry's
   // address is its byte offset (no base to add per lookup).
   struct Lds {
@@ -694,7 +694,7 @@ __global__ __launch_bounds__(kPThreads) void k_splat_pc(SplatArgs a, const uint3
   const int trank = (wave & 3) == 0 ? 8 + (wave >> 2) : wave - 1 - (wave >> 2);
   const int ptid = trank * 64 + lane;
   for (int i = tid; i < 2048; i += kPThreads) s_tab[i] = a.exptab[i];
-  const uint32_t it = blockIdx.x;
+  const uint32_t it = blockIdx.x + (dc.first ? uniform(*dc.first) : 0u);
   const uint64_t t_start = a.trace ? __builtin_amdgcn_s_memrealtime() : 0;
   const uint32_t u = uniform(items[it]) >> 2;
   const UnitGeom ug = unit_geom_uniform(a, u);
@@ -1031,7 +1031,6 @@ __global__ __launch_bounds__(kPThreads) void k_splat_pc(SplatArgs a, const uint3
 // entries) at ops + (g * P + c) * 64, with P = 3 planes t_c (additive) or 4
 // (m1_0, m1_1, m1_2, m2; ordered).  Ordered lists also have q_c per entry:
 // qv[(g / kStageLoad) * 48 + c * 16 + e] for entry e of the load holding g.
-constexpr int kStageLoad = 4;
 __host__ __device__ constexpr uint32_t stage_groups(uint32_t len) {
   return (len + 4 * kStageLoad - 1) / (4 * kStageLoad) * kStageLoad;
 }
@@ -1043,7 +1042,9 @@ constexpr uint32_t stage_planes() {
 // List offsets of the staged items (one workgroup): cbase (groups of 4
 // entries) and cbat (batches of 64 entries), exclusive, totals at [n].
 __global__ __launch_bounds__(256) void k_stage_base(SplatArgs a, const uint32_t *items,
-                                                    uint32_t n, uint32_t *cbase, uint32_t *cbat) {
+                                                    uint32_t n, uint32_t *cbase, uint32_t *cbat,
+                                                    const uint32_t *dn) {
+  if (dn) n = min(n, *dn);
   __shared__ uint32_t s_g[256], s_b[256];
   __shared__ uint32_t s_carry[2];
   const int tid = threadIdx.x;
@@ -1092,8 +1093,10 @@ template <int ADD>
 __global__ __launch_bounds__(256) void k_stage_ops(SplatArgs a, const uint32_t *items,
                                                    uint32_t i0, uint32_t i1,
                                                    const uint32_t *cbase, const uint32_t *cbat,
-                                                   float4 *ops, float *qv) {
+                                                   float4 *ops, float *qv, const uint32_t *dn) {
   constexpr uint32_t P = stage_planes<ADD>();
+  if (dn) i1 = min(i1, *dn);
+  if (i1 <= i0) return;  // the whole grid (device count)
   __shared__ float s_tab[2048];
   __shared__ uint32_t s_col[4][8][kPad];
   __shared__ float s_pre2[4][8][kPad];
@@ -1203,11 +1206,12 @@ constexpr int kS = kStageLoad, kPix = 64 / kS;
 template <int ADD>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_stage_chain(
     SplatArgs a, const uint32_t *items, uint32_t i0, const uint32_t *cbase,
-    const float4 *__restrict__ ops, const float *__restrict__ qv) {
+    const float4 *__restrict__ ops, const float *__restrict__ qv, const uint32_t *dn) {
   constexpr uint32_t P = stage_planes<ADD>();
   constexpr int kR = ADD ? 40 : 10;  // steps in flight (ordered: three loads per step)
   __shared__ float4 xs[3][64];
   const uint32_t i = i0 + blockIdx.x / (3u * kS);
+  if (dn && i >= *dn) return;  // past the device count (uniform per workgroup)
   const uint32_t r3 = blockIdx.x % (3u * kS);
   const int c = static_cast<int>(r3 / kS), h = static_cast<int>(r3 % kS);
   const int lane = threadIdx.x, qo = lane / kPix, pl = lane % kPix;
@@ -1449,32 +1453,55 @@ uint64_t stage_q_floats(bool additive, uint64_t groups) {
 }
 
 void launch_stage_base(const uint32_t *items, uint32_t n, uint32_t *cbase, uint32_t *cbat,
-                       const SplatArgs &a, hipStream_t s) {
+                       const SplatArgs &a, hipStream_t s, const uint32_t *dn) {
   if (n == 0) return;
-  k_stage_base<<<1, 256, 0, s>>>(a, items, n, cbase, cbat);
+  k_stage_base<<<1, 256, 0, s>>>(a, items, n, cbase, cbat, dn);
   hip_check(hipGetLastError(), "k_stage_base launch");
 }
 
 void launch_stage_ops(bool additive, const uint32_t *items, uint32_t i0, uint32_t i1,
                       uint64_t groups, const uint32_t *cbase, const uint32_t *cbat, float4 *ops,
-                      float *qv, const SplatArgs &a, hipStream_t s) {
+                      float *qv, const SplatArgs &a, hipStream_t s, const uint32_t *dn) {
   if (i1 <= i0) return;
   // batches: at most groups / 16 + lists; about four per wave when there are many
   const uint64_t nbat = groups / 16 + (i1 - i0);
   const unsigned grid = std::max(1u, static_cast<unsigned>(std::min<uint64_t>(2048, (nbat + 15) / 16)));
-  if (additive) k_stage_ops<1><<<grid, 256, 0, s>>>(a, items, i0, i1, cbase, cbat, ops, qv);
-  else k_stage_ops<0><<<grid, 256, 0, s>>>(a, items, i0, i1, cbase, cbat, ops, qv);
+  if (additive) k_stage_ops<1><<<grid, 256, 0, s>>>(a, items, i0, i1, cbase, cbat, ops, qv, dn);
+  else k_stage_ops<0><<<grid, 256, 0, s>>>(a, items, i0, i1, cbase, cbat, ops, qv, dn);
   hip_check(hipGetLastError(), "k_stage_ops launch");
 }
 
 void launch_stage_chain(bool additive, const uint32_t *items, uint32_t i0, uint32_t i1,
                         const uint32_t *cbase, const float4 *ops, const float *qv,
-                        const SplatArgs &a, hipStream_t s) {
+                        const SplatArgs &a, hipStream_t s, const uint32_t *dn) {
   if (i1 <= i0) return;
   const unsigned grid = 3u * kS * (i1 - i0);
-  if (additive) k_stage_chain<1><<<grid, 64, 0, s>>>(a, items, i0, cbase, ops, qv);
-  else k_stage_chain<0><<<grid, 64, 0, s>>>(a, items, i0, cbase, ops, qv);
+  if (additive) k_stage_chain<1><<<grid, 64, 0, s>>>(a, items, i0, cbase, ops, qv, dn);
+  else k_stage_chain<0><<<grid, 64, 0, s>>>(a, items, i0, cbase, ops, qv, dn);
   hip_check(hipGetLastError(), "k_stage_chain launch");
+}
+
+__global__ void k_partition(PartitionArgs a, uint32_t *out) {
+  uint32_t ns = 0;
+  if (a.len0 != 0xffffffffu) {
+    const uint32_t lmax = *a.lmax;
+    int k = 0;
+    if (a.ordered_auto)
+      while (k < 7 && (static_cast<uint64_t>(a.len0) << k) * a.span < lmax) ++k;
+    ns = a.stage_cnt[k];
+    if (ns > a.ns_cap || a.stage_grp[k] > a.groups_cap) ns = 0;
+    if (a.ordered_auto && static_cast<uint64_t>(lmax) * a.ratio < *a.etot) ns = 0;
+  }
+  const uint32_t nh = *a.heavy_cnt;
+  const uint32_t npc = (a.heavy_max == 0 || nh <= ns) ? 0u : min(nh - ns, a.heavy_max);
+  out[0] = ns;
+  out[1] = npc;
+  out[2] = ns + npc;
+}
+
+void launch_partition(const PartitionArgs &pa, uint32_t *out, hipStream_t s) {
+  k_partition<<<1, 1, 0, s>>>(pa, out);
+  hip_check(hipGetLastError(), "k_partition launch");
 }
 
 void launch_splat_pc(bool additive, const uint32_t *items, uint32_t npc, const SplatArgs &a,

@@ -5,7 +5,8 @@ capacities from it and renders it once more with device-side counts (no host
 wait, allocation, host copy or memset).  Under stream capture the same
 splhip_render_batched call queues that pipeline, and the graph is replayed.
 Bar: every image of the reserve and of every replay equals the plain render
-of its view bit for bit; replays after the particle values change (same
+of its view bit for bit (also with staged long lists, whose partition against
+producer/consumer workgroups and the 8x8 splat is then made on the device); replays after the particle values change (same
 arrays, same counts) equal the plain renders of the new values; a replay
 whose entries exceed the reserved capacity reports it (splhip_capture_status)
 and writes nothing past its buffers.  Replays are interleaved with eager work
@@ -78,7 +79,10 @@ def new_values(views, jobs, seed):
             dev[k].copy_(torch.from_numpy(getattr(f, k)))
 
 
-@pytest.mark.parametrize("prm_kw", [dict(), dict(hip_heavy=1, hip_heavy_max=64)])
+@pytest.mark.parametrize("prm_kw", [dict(), dict(hip_heavy=1, hip_heavy_max=64),
+                                    dict(hip_stage=64, hip_stage_ordered=64),
+                                    dict(hip_stage=256, hip_stage_ordered=128, hip_heavy=64,
+                                         hip_heavy_max=4)])
 def test_capture_replay_bitexact(R, prm_kw):
     import splotch_amd as S
     log("case capture: 12 views of batch_jobs(seed 1200, 1/20 scale)", prm_kw)
