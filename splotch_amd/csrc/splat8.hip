@@ -119,7 +119,7 @@ __device__ __forceinline__ void walk_chunks(int nb, int cx, int cy, const uint32
                                             const float (*pre2)[kPad], const float (*pre1)[kPad],
                                             const float *tab, float va0, float va1, float va2,
                                             float vq0, float vq1, float vq2, float vmax,
-                                            const float4 (*cst)[2], uint64_t fastmask,
+                                            const float4 *cst, uint64_t fastmask,
                                             float expfac, float tl, float &l0, float &l1,
                                             float &l2) {
   constexpr bool C0 = (CH == 3 || CH == 0), C1 = (CH == 3 || CH == 1), C2 = (CH == 3 || CH == 2);
@@ -148,8 +148,8 @@ __device__ __forceinline__ void walk_chunks(int nb, int cx, int cy, const uint32
 #pragma unroll
         for (int j = 0; j < J; ++j) {
           const int k = k0 + j;
-          ca[j] = cst[k][0];
-          cq[j] = cst[k][1];
+          ca[j] = cst[2 * k];
+          cq[j] = cst[2 * k + 1];
         }
 #pragma unroll
         for (int j = 0; j < J; ++j) {
@@ -184,6 +184,30 @@ __device__ __forceinline__ void walk_chunks(int nb, int cx, int cy, const uint32
         }
         continue;
       }
+    } else {
+      // Additive: the particles' a_c come from LDS (cst: (a0, a1, a2, 0)) as
+      // vector operands, and slots past the batch (stale or uninitialised
+      // values) are read at their own offsets and masked by the select, so
+      // no address or lane read is computed per particle.
+      float at[J];
+      float4 ca[J];
+      uint32_t mv[J];
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const int k = k0 + j;
+        ca[j] = cst[k];
+        const uint32_t mk = cmask[cx][k];
+        mv[j] = (k < nb) ? mk : 0u;
+        at[j] = pre1[cy][k] * pre2[cx][k];
+      }
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const bool in = (mv[j] >> cy) & 1u;
+        if (C0) { const float t = ca[j].x * at[j]; l0 = l0 + (in ? t : 0.f); }
+        if (C1) { const float t = ca[j].y * at[j]; l1 = l1 + (in ? t : 0.f); }
+        if (C2) { const float t = ca[j].z * at[j]; l2 = l2 + (in ? t : 0.f); }
+      }
+      continue;
     }
     float att[J], g0[J], g1[J], g2[J], p0[J], p1[J], p2[J];
     float u0[J], w0[J], u1[J], w1[J], u2[J], w2[J];
@@ -400,7 +424,9 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
   __shared__ uint32_t s_col[4][8][kPad];
   __shared__ float s_pre2[4][8][kPad];
   __shared__ float s_pre1[4][8][kPad];
-  __shared__ float4 s_cst[ADD ? 1 : 4][ADD ? 1 : 64][2];  // ordered: per-particle constants
+  // per-particle constants: ordered (a0, a1, a2, max |a|), (q0, q1, q2, 0);
+  // additive (a0, a1, a2, 0)
+  __shared__ float4 s_cst[4][64 * (ADD ? 1 : 2)];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   // a workgroup past the items (a grid sized for a capacity) leaves at once
   if (blockIdx.x * 4u >= nitems) return;
@@ -422,7 +448,7 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
   uint32_t(*col)[kPad] = s_col[wave];
   float(*pre2)[kPad] = s_pre2[wave];
   float(*pre1)[kPad] = s_pre1[wave];
-  float4(*wcst)[2] = s_cst[ADD ? 0 : wave];
+  float4 *wcst = s_cst[wave];
   if (!ADD) {
     // the fast chunks read every slot of a chunk, also past a short batch
     // (masked there): start from finite values
@@ -431,8 +457,8 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
       (&pre2[0][0])[i] = 0.f;
       (&pre1[0][0])[i] = 0.f;
     }
-    wcst[lane][0] = make_float4(0.f, 0.f, 0.f, 0.f);
-    wcst[lane][1] = make_float4(0.f, 0.f, 0.f, 0.f);
+    wcst[2 * lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+    wcst[2 * lane + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
   const float expfac = a.expfac, tl = a.taylorlimit;
 
@@ -471,8 +497,10 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
         // batch, so a particle that is not fast (possibly a non-finite a or
         // q) leaves zeros, which a masked slot applies as l + 0
         const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-        wcst[lane][0] = vfast ? make_float4(va0, va1, va2, vmax) : z4;
-        wcst[lane][1] = vfast ? make_float4(vq0, vq1, vq2, 0.f) : z4;
+        wcst[2 * lane] = vfast ? make_float4(va0, va1, va2, vmax) : z4;
+        wcst[2 * lane + 1] = vfast ? make_float4(vq0, vq1, vq2, 0.f) : z4;
+      } else {
+        wcst[lane] = make_float4(va0, va1, va2, 0.f);
       }
     }
     wave_sync();
