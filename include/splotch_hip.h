@@ -106,13 +106,16 @@ typedef struct {
                                               (0 = auto: 8192, but none for a single view
                                               with both passes in one chunk, whose additive
                                               splat runs beside the ordered pass; < 0 =
-                                              never; no staging in device-count renders) */
+                                              never) */
   int32_t hip_stage_ordered;               /* hip_stage_ordered: the same for an ordered pass
                                               (per-pixel operands of the compositing update
                                               computed first; not when a particle's q_c is
-                                              non-finite) (0 = auto: 65536 for a pass of at
-                                              most 2^26 bin entries, else none; < 0 =
-                                              never) */
+                                              non-finite) (0 = auto: when the longest list
+                                              x 216 reaches the pass's bin entries, the
+                                              lists from about the longest / 16 on; else,
+                                              when it x 432 does, the lists from entries /
+                                              432 on; else none; thresholds are levels
+                                              8192 x 2^(j/2); < 0 = never) */
 } splhip_params;
 
 /* SoA particle fields (device pointers), post particle_normalize:
@@ -306,9 +309,11 @@ int splhip_bins(splhip_ctx *ctx, const splhip_particles *parts, const splhip_cam
      [7] ordered k_splat8 launch (normal items)    [9] additive k_splat8 launch
      [8] ordered k_splat_heavy launch (auxiliary   [10] additive k_splat_heavy launch
          stream, concurrent with [7])
+     [11] ordered staged long lists: operands and  [12] additive staged long lists
+          chains (third stream, concurrent with [7])
    counts (optional, 4 values): active particles and bin entries of the
    ordered pass, then of the additive pass. */
-#define SPLHIP_NSTAGES 11
+#define SPLHIP_NSTAGES 13
 int splhip_set_profiling(splhip_ctx *ctx, int32_t enable);
 int splhip_stage_times(splhip_ctx *ctx, float *ms, int64_t *counts);
 

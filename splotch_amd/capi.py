@@ -114,7 +114,7 @@ EXPORTS = ["splhip_create", "splhip_destroy", "splhip_last_error", "splhip_camer
 
 STAGES = ("project", "ordered_sort", "ordered_bin", "ordered_splat", "additive_compact",
           "additive_bin", "additive_splat", "ordered_k_splat8", "ordered_k_splat_heavy",
-          "additive_k_splat8", "additive_k_splat_heavy")
+          "additive_k_splat8", "additive_k_splat_heavy", "ordered_staged", "additive_staged")
 
 
 def lib():

@@ -634,14 +634,17 @@ void make_plan(splhip_ctx *c, int nj, const JobIn *jobs, int G, Plan &pl, hipStr
 // [4..5] work items, [6..7] heavy items, [8..11] bin entries as u64,
 // [12..13] region entries, [14] the ordered pass has a non-finite q,
 // [15] capacity overflow bits of a device-count render (bit p: pass p),
-// [16..31] lists from stage_len << k entries on, [32..47] their groups of 4
-// entries (padded), k = 0..7 ([16 + 8 p + k]: pass p), [48..49] the longest
-// list, [50..57] a device-count render's long-list partition (4 per pass:
-// staged, producer/consumer, both).
+// [16..47] lists from stage_level(stage_len, j) entries on, [48..79] their
+// groups of 4 entries (padded), j = 0..15 ([16 + 16 p + j]: pass p),
+// [80..81] the longest list, [82..89] a device-count render's long-list
+// partition (4 per pass: staged, producer/consumer, both).
 enum {
   T_M = 0, T_E = 2, T_ITEMS = 4, T_HEAVY = 6, T_E64 = 8, T_E16 = 12, T_QBAD = 14, T_OVF = 15,
-  T_STAGE = 16, T_SGRP = 32, T_LMAX = 48, T_PART = 50, T_N = 58
+  T_STAGE = 16, T_SGRP = 48, T_LMAX = 80, T_PART = 82, T_N = 90
 };
+constexpr int kTotalsWords = 128;  // the totals buffer, device and pinned host
+static_assert(T_N <= kTotalsWords, "totals fit their buffers");
+static_assert(T_SGRP - T_STAGE == 2 * kStageLevels, "one level count per pass");
 
 // Project all jobs, then the compaction scan of each present pass and its job
 // offsets in pass order; one host sync reads the pass sizes and offsets.
@@ -863,6 +866,13 @@ constexpr uint64_t kStageBudget = uint64_t(32) << 30;
 // against 43.7 ms unstaged; 64 views: 65.6M ordered entries, 68.6 -> 48.1 ms;
 // 500k at 512^2: 121k-entry list, 1.3M entries, 6.21 -> 3.58 ms).
 constexpr uint64_t kStageOrderedRatio = 216;
+// Below that, a pass whose longest lists would still end after its other
+// entries (longest list x kStageOrderedCrit >= entries, at 108 / 0.25 clocks
+// per entry) stages only those: the lists from entries / kStageOrderedCrit
+// on (rounded down to a level), whose operands then compete least with the
+// rest (16M particles at 2048^2: 813k-entry list, 331M entries; the 4 lists
+// from 741376 on staged, 41.8 ms against 42.8 unstaged).
+constexpr uint64_t kStageOrderedCrit = 432;
 // Staged lists whose chains start right after their own operands (the
 // longest; the others' operands and chains follow beside them).
 constexpr uint32_t kStagePhase1 = 64;
@@ -933,11 +943,12 @@ void stage_items_count(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, uint32_
   // the count of this call's long-list items only (chunks and autotuning
   // candidates count again)
   launch_fill_u32(tot + T_HEAVY + p, 0u, 1, s);
-  launch_fill_u32(tot + T_STAGE + 8 * p, 0u, 8, s);
-  launch_fill_u32(tot + T_SGRP + 8 * p, 0u, 8, s);
+  launch_fill_u32(tot + T_STAGE + kStageLevels * p, 0u, kStageLevels, s);
+  launch_fill_u32(tot + T_SGRP + kStageLevels * p, 0u, kStageLevels, s);
   launch_fill_u32(tot + T_LMAX + p, 0u, 1, s);
   launch_items_count(pl.bin_off[p], pl.nunits, sa, p == 0 ? 1 : 0, tn.split, icnt,
-                     tot + T_HEAVY + p, s, tot + T_STAGE + 8 * p, tot + T_SGRP + 8 * p,
+                     tot + T_HEAVY + p, s, tot + T_STAGE + kStageLevels * p,
+                     tot + T_SGRP + kStageLevels * p,
                      tot + T_LMAX + p);
   scan_u32(icnt, ioff, pl.nunits, tot + T_ITEMS + p, ws, s);
   mark(c, p == 0 ? 2 : 5, false, s);
@@ -957,14 +968,12 @@ void take_items(splhip_ctx *c, Plan &pl, int p, const Tuning &tn) {
   const uint32_t nheavy_lists = pl.m[p] ? h[T_HEAVY - T_ITEMS + p] : 0u;
   const uint64_t lmax = h[T_LMAX - T_ITEMS + p];
   const uint32_t len0 = p == 1 ? tn.stage : tn.stage_ord;
-  int k = 0;
-  if (p == 0 && tn.stage_ord_auto)
-    while (k < 7 && (static_cast<uint64_t>(len0) << k) * kStageOrderedSpan < lmax) ++k;
-  uint32_t ns = pl.m[p] ? h[T_STAGE - T_ITEMS + 8 * p + k] : 0u;
-  const uint64_t groups = h[T_SGRP - T_ITEMS + 8 * p + k];
+  const int j = stage_rule(len0, p == 0 && tn.stage_ord_auto, lmax, pl.etot[p],
+                           kStageOrderedSpan, kStageOrderedRatio, kStageOrderedCrit);
+  uint32_t ns = pl.m[p] && j >= 0 ? h[T_STAGE - T_ITEMS + kStageLevels * p + j] : 0u;
+  const uint64_t groups = j >= 0 ? h[T_SGRP - T_ITEMS + kStageLevels * p + j] : 0u;
   const uint64_t bytes = 4 * (stage_floats(p == 1, groups) + stage_q_floats(p == 1, groups));
   if (bytes > kStageBudget) ns = 0;
-  if (p == 0 && tn.stage_ord_auto && lmax * kStageOrderedRatio < pl.etot[p]) ns = 0;
   pl.nstage[p] = ns;
   pl.sgroups[p] = ns ? groups : 0;
   c->last_stage_n[p] = ns;
@@ -1008,8 +1017,8 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
     // the long-list partition from the device counts, as take_items makes it
     part = pl.tot + T_PART + 4 * p;
     PartitionArgs pa;
-    pa.stage_cnt = pl.tot + T_STAGE + 8 * p;
-    pa.stage_grp = pl.tot + T_SGRP + 8 * p;
+    pa.stage_cnt = pl.tot + T_STAGE + kStageLevels * p;
+    pa.stage_grp = pl.tot + T_SGRP + kStageLevels * p;
     pa.heavy_cnt = pl.tot + T_HEAVY + p;
     pa.lmax = pl.tot + T_LMAX + p;
     pa.etot = reinterpret_cast<const uint64_t *>(pl.tot + T_E64 + 2 * p);
@@ -1020,6 +1029,7 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
     pa.heavy_max = tn.heavy == 0xffffffffu ? 0u : tn.heavy_max;
     pa.span = static_cast<uint32_t>(kStageOrderedSpan);
     pa.ratio = static_cast<uint32_t>(kStageOrderedRatio);
+    pa.crit = static_cast<uint32_t>(kStageOrderedCrit);
     launch_partition(pa, part, s);
     dc.items = pl.tot + T_ITEMS + p;
     dc.heavy = part + 2;
@@ -1078,7 +1088,7 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
       ensure_aux3(c);
       hip_check(hipEventRecord(c->ev_fork3, s), "record fork");
       hip_check(hipStreamWaitEvent(c->aux3, c->ev_fork3, 0), "wait fork");
-      mark(c, p == 0 ? 8 : 10, true, c->aux3);
+      mark(c, p == 0 ? 11 : 12, true, c->aux3);
       launch_stage_base(items, nstage, cb, cbt, sa, c->aux3, part);
       launch_stage_ops(add, items, 0, n1, pl.sgroups[p], cb, cbt, ops, qv, sa, c->aux3, part);
       if (n1 < nstage) {
@@ -1090,7 +1100,7 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
         hip_check(hipEventRecord(c->ev_join4, c->aux4), "record join");
       }
       launch_stage_chain(add, items, 0, n1, cb, ops, qv, sa, c->aux3, part);
-      mark(c, p == 0 ? 8 : 10, false, c->aux3);
+      mark(c, p == 0 ? 11 : 12, false, c->aux3);
       hip_check(hipEventRecord(c->ev_join3, c->aux3), "record join");
     }
     // The normal items fill every CU; the heavy workgroups (the critical
@@ -1320,7 +1330,7 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
   Workspace &ws = c->ws;
   Rec *rec = ws.get_as<Rec>(S_REC, static_cast<size_t>(std::max<int64_t>(pl.ntot, 1)));
   uint8_t *act = ws.get_as<uint8_t>(S_ACT, static_cast<size_t>(std::max<int64_t>(pl.ntot, 1)));
-  uint32_t *tot = ws.get_as<uint32_t>(S_TOTALS, 64);
+  uint32_t *tot = ws.get_as<uint32_t>(S_TOTALS, kTotalsWords);
   stage_project_compact(c, pl, rec, act, tot, s);
   upload_job_flags(c, pl, s);
   for (int p = 0; p < 2; ++p)
@@ -1666,7 +1676,7 @@ int render_jobs_dc(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s) {
   c->prof = 0;  // no stage events (they would need host waits)
   Rec *rec = ws.get_as<Rec>(S_REC, static_cast<size_t>(std::max<int64_t>(pl.ntot, 1)));
   uint8_t *act = ws.get_as<uint8_t>(S_ACT, static_cast<size_t>(std::max<int64_t>(pl.ntot, 1)));
-  uint32_t *tot = ws.get_as<uint32_t>(S_TOTALS, 64);
+  uint32_t *tot = ws.get_as<uint32_t>(S_TOTALS, kTotalsWords);
   pl.tot = tot;
   for (int p = 0; p < 2; ++p) {
     pl.use16[p] = rv.use16[p];
@@ -1770,8 +1780,9 @@ int capture_reserve(splhip_ctx *c, int nj, const JobIn *jobs, float headroom, hi
     return rc;
   }
   uint32_t ovf = 0;
-  hip_check(hipMemcpyAsync(c->h_tot, rv->ws.get_as<uint32_t>(S_TOTALS, 64) + T_OVF, 4,
-                           hipMemcpyDeviceToHost, s), "read overflow flags");
+  const uint32_t *d_tot = rv->ws.get_as<uint32_t>(S_TOTALS, kTotalsWords);
+  hip_check(hipMemcpyAsync(c->h_tot, d_tot + T_OVF, 4, hipMemcpyDeviceToHost, s),
+            "read overflow flags");
   hip_check(hipStreamSynchronize(s), "sync reserve");
   ovf = c->h_tot[0];
   rv->ws.freeze(true);
@@ -1791,7 +1802,7 @@ int backward_impl(splhip_ctx *c, const float *grad, const splhip_grads *out, hip
   Workspace &ws = c->ws;
   double *gscr = ws.get_as<double>(S_GRAD0, static_cast<size_t>(std::max<int64_t>(n, 1)) * 6);
   launch_zero_f64(gscr, n * 6, s);
-  uint32_t *tot = ws.get_as<uint32_t>(S_TOTALS, 64);
+  uint32_t *tot = ws.get_as<uint32_t>(S_TOTALS, kTotalsWords);
   for (int p = 0; p < 2; ++p) {
     if (!pl.m[p]) continue;
     // work items: list segments of at most kAdjSeg entries
@@ -1865,8 +1876,8 @@ int splhip_create(int device, splhip_ctx **out) {
     hip_check(hipMalloc(&c->d_exptab, 2048 * sizeof(float)), "hipMalloc exptable");
     hip_check(hipMemcpy(c->d_exptab, tab.data(), 2048 * sizeof(float), hipMemcpyHostToDevice),
               "upload exptable");
-    hip_check(hipHostMalloc(reinterpret_cast<void **>(&c->h_tot), 64 * sizeof(uint32_t),
-                            hipHostMallocDefault),
+    hip_check(hipHostMalloc(reinterpret_cast<void **>(&c->h_tot),
+                            kTotalsWords * sizeof(uint32_t), hipHostMallocDefault),
               "hipHostMalloc");
     // the device address of this GPU's `scratch` (the symbol's name in host
     // code is not a device address)
@@ -2678,7 +2689,7 @@ int splhip_bins(splhip_ctx *c, const splhip_particles *P, const splhip_camera *c
     Workspace &ws = c->ws;
     Rec *rec = ws.get_as<Rec>(S_REC, static_cast<size_t>(std::max<int64_t>(pl.ntot, 1)));
     uint8_t *act = ws.get_as<uint8_t>(S_ACT, static_cast<size_t>(std::max<int64_t>(pl.ntot, 1)));
-    uint32_t *tot = ws.get_as<uint32_t>(S_TOTALS, 64);
+    uint32_t *tot = ws.get_as<uint32_t>(S_TOTALS, kTotalsWords);
     c->nmarks = 0;
     stage_project_compact(c, pl, rec, act, tot, s);
     if (pl.m[p]) {
@@ -2781,7 +2792,7 @@ int splhip_capture_status(splhip_ctx *c, int32_t *overflow) {
   if (!c->rv) return fail(c, SPLHIP_E_STATE, "no reserved batch");
   try {
     hip_check(hipSetDevice(c->device), "hipSetDevice");
-    uint32_t *tot = c->rv->ws.get_as<uint32_t>(S_TOTALS, 64);
+    uint32_t *tot = c->rv->ws.get_as<uint32_t>(S_TOTALS, kTotalsWords);
     hip_check(hipDeviceSynchronize(), "sync status");
     hip_check(hipMemcpy(c->h_tot, tot + T_OVF, 4, hipMemcpyDeviceToHost), "read overflow flags");
     *overflow = static_cast<int32_t>(c->h_tot[0]);

@@ -145,12 +145,21 @@ void launch_delay(uint32_t us, hipStream_t s);
 void launch_unpack_records(const char *stage, int64_t n, int64_t C, int ncol, float *soa,
                            int64_t nn, uint8_t *type, hipStream_t s);
 
+// Staging thresholds: level j = 0..kStageLevels-1 of a base length sl is
+// sl * 2^(j/2) (odd levels times 181/128, about sqrt 2, rounded down).
+constexpr int kStageLevels = 16;
+__host__ __device__ inline uint64_t stage_level(uint32_t sl, int j) {
+  const uint64_t b = static_cast<uint64_t>(sl) << (j >> 1);
+  return (j & 1) ? (b * 181u) >> 7 : b;
+}
+
 // Wave-per-8x8-block splat (splat8.hip): work items (unit << 2 | channel,
 // channel 3 = all), heaviest first.  heavy_cnt (device, optional): incremented
 // once per unit whose list reaches a.heavy_len (one item); stage_cnt /
-// stage_grp (optional, [8] each): [j] once per unit whose list reaches
-// a.stage_len << j, and by its groups of 4 entries (padded to whole chain
-// loads); lmax (optional): the longest list of a unit with items.
+// stage_grp (optional, [kStageLevels] each): [j] once per unit whose list
+// reaches stage_level(a.stage_len, j), and by its groups of 4 entries (padded
+// to whole chain loads); lmax (optional): the longest list of a unit with
+// items.
 void launch_items_count(const uint32_t *bin_off, uint32_t nunits, const SplatArgs &a,
                         int ordered, uint32_t split, uint32_t *cnt, uint32_t *heavy_cnt,
                         hipStream_t s, uint32_t *stage_cnt = nullptr,
@@ -194,11 +203,10 @@ void launch_stage_chain(bool additive, const uint32_t *items, uint32_t i0, uint3
                         const uint32_t *cbase, const float4 *ops, const float *qv,
                         const SplatArgs &a, hipStream_t s, const uint32_t *dn = nullptr);
 // Device-count partition of a pass's long lists (one thread), as the host
-// does it with exact counts: the threshold level k (0, or for ordered_auto
-// the least with len0 << k >= *lmax / span), the staged lists stage_cnt[k]
-// (none past ns_cap or groups_cap, or for ordered_auto when *lmax * ratio <
-// *etot), then at most heavy_max of the other lists from heavy_len on (the
-// count heavy_cnt).  out[0] staged, out[1] producer/consumer, out[2] both.
+// does it with exact counts (stage_rule): the staged lists stage_cnt[j] at
+// the rule's level j (none past ns_cap or groups_cap), then at most heavy_max
+// of the other lists from heavy_len on (the count heavy_cnt).  out[0] staged,
+// out[1] producer/consumer, out[2] both.
 struct PartitionArgs {
   const uint32_t *stage_cnt, *stage_grp, *heavy_cnt, *lmax;
   const uint64_t *etot;
@@ -207,8 +215,28 @@ struct PartitionArgs {
   uint32_t ns_cap;
   uint64_t groups_cap;
   uint32_t heavy_max;  // 0: no producer/consumer workgroups
-  uint32_t span, ratio;
+  uint32_t span, ratio, crit;
 };
+// The staging level of a pass: 0, or for an ordered pass by default (auto)
+// - when lmax * ratio >= etot (bound by its longest lists): the least level
+//   with level * span >= lmax,
+// - else when lmax * crit >= etot (its longest lists would end after the
+//   rest): the greatest level with level * crit <= etot (none when even the
+//   top level is below etot / crit),
+// - else none (-1).
+__host__ __device__ inline int stage_rule(uint32_t len0, bool ordered_auto, uint64_t lmax,
+                                          uint64_t etot, uint32_t span, uint32_t ratio,
+                                          uint32_t crit) {
+  if (!ordered_auto) return 0;
+  int j = 0;
+  if (lmax * ratio >= etot) {
+    while (j < kStageLevels - 1 && stage_level(len0, j) * span < lmax) ++j;
+    return j;
+  }
+  if (lmax * crit < etot || stage_level(len0, kStageLevels - 1) * crit < etot) return -1;
+  while (j < kStageLevels - 1 && stage_level(len0, j + 1) * crit <= etot) ++j;
+  return j;
+}
 void launch_partition(const PartitionArgs &pa, uint32_t *out, hipStream_t s);
 // Producer/consumer workgroups (splat8.hip), one per item: the first npc
 // items, the units whose lists reach a.heavy_len.

@@ -166,18 +166,24 @@ __device__ __forceinline__ void walk_chunks(int nb, int cx, int cy, const uint32
         }
 #pragma unroll
         for (int j = 0; j < J; ++j) {
+          // the table product is formed before the select: written inside
+          // it, the compiler may move the table reads into a branch that
+          // waits for every LDS read in flight
           if (C0) {
-            const float em = (fabsf(g0[j]) < tl) ? g0[j] : u0[j] * w0[j] - 1.f;
+            const float ex = u0[j] * w0[j] - 1.f;
+            const float em = (fabsf(g0[j]) < tl) ? g0[j] : ex;
             const float m1 = tay[j] ? ca[j].x : em;
             l0 = l0 + ((l0 - cq[j].x) * m1) * m2[j];
           }
           if (C1) {
-            const float em = (fabsf(g1[j]) < tl) ? g1[j] : u1[j] * w1[j] - 1.f;
+            const float ex = u1[j] * w1[j] - 1.f;
+            const float em = (fabsf(g1[j]) < tl) ? g1[j] : ex;
             const float m1 = tay[j] ? ca[j].y : em;
             l1 = l1 + ((l1 - cq[j].y) * m1) * m2[j];
           }
           if (C2) {
-            const float em = (fabsf(g2[j]) < tl) ? g2[j] : u2[j] * w2[j] - 1.f;
+            const float ex = u2[j] * w2[j] - 1.f;
+            const float em = (fabsf(g2[j]) < tl) ? g2[j] : ex;
             const float m1 = tay[j] ? ca[j].z : em;
             l2 = l2 + ((l2 - cq[j].z) * m1) * m2[j];
           }
@@ -347,18 +353,17 @@ __device__ __forceinline__ void walk_batch(int nb, int cx, int cy, const uint32_
 }
 
 // Phase A of a batch, lane = particle (splotch_host.cc:462-476 + 596-607, tile
-// frame): the lane's particle (list entry `entry`) tabulates, for the block's
-// 8 columns, its row range (BITS: bit r set when block row byo + r lies in
-// the column's [lo, hi); else lo | hi << 16) and Gaussian column factor, and
-// for the 8 rows its Gaussian row factor, into column `lane` of the wave's
-// LDS slices; returns the particle's record.
+// frame): the lane's particle (record R) tabulates, for the block's 8
+// columns, its row range (BITS: bit r set when block row byo + r lies in the
+// column's [lo, hi); else lo | hi << 16) and Gaussian column factor, and for
+// the 8 rows its Gaussian row factor, into column `lane` of the wave's LDS
+// slices.
 template <bool BITS>
-__device__ __forceinline__ Rec tabulate(const SplatArgs &a, uint32_t entry, int lane, int x0s,
+__device__ __forceinline__ void tabulate(const SplatArgs &a, const Rec &R, int lane, int x0s,
                                         int y0s, int w, int h, int bxo, int byo,
                                         const float *s_tab, uint32_t (*col)[kPad],
                                         float (*pre2)[kPad], float (*pre1)[kPad],
                                         float expfac) {
-  const Rec R = a.recs[a.ent[entry]];
   const float rfacr = R.r * a.rfac;
   const float posx = R.x - static_cast<float>(x0s), posy = R.y - static_cast<float>(y0s);
   const int bx0 = max(cvt_i32(posx - rfacr + 1), 0), bx1 = min(cvt_i32(posx + rfacr + 1), w);
@@ -399,7 +404,6 @@ __device__ __forceinline__ Rec tabulate(const SplatArgs &a, uint32_t entry, int 
     }
     pre1[r][lane] = p1;
   }
-  return R;
 }
 
 // Work items of a launch: items[0, n), or with device counts (dc: the pass's
@@ -474,14 +478,22 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
   // long lists are near the critical path: let their waves issue ahead of the
   // rest (priority 2; the producer/consumer workgroups, the critical path, run at 3)
   if (e1 - e0 >= a.prio_len) __builtin_amdgcn_s_setprio(2);
+  // The records of a batch are gathered one batch ahead and their slots two
+  // ahead, so that the walk of a batch covers the latency of the next gather.
+  Rec rn{};
+  uint32_t sn = 0;
+  if (e0 + lane < e1) rn = a.recs[a.ent[e0 + lane]];
+  if (e0 + 64u + lane < e1) sn = a.ent[e0 + 64u + lane];
   for (uint32_t base = e0; base < e1; base += 64) {
     const int nb = static_cast<int>(uniform(min<uint32_t>(64u, e1 - base)));
+    const Rec R = rn;
+    if (base + 64u + lane < e1) rn = a.recs[sn];
+    if (base + 128u + lane < e1) sn = a.ent[base + 128u + lane];
     // Phase A: lane = particle (splotch_host.cc:462-476 + 596-607, tile frame)
     float va0 = 0.f, va1 = 0.f, va2 = 0.f, vq0 = 0.f, vq1 = 0.f, vq2 = 0.f, vmax = 0.f;
     bool vfast = false;
     if (lane < nb) {
-      const Rec R = tabulate<VAR >= 3>(a, base + lane, lane, x0s, y0s, w, h, bxo, byo, s_tab,
-                                       col, pre2, pre1, expfac);
+      tabulate<VAR >= 3>(a, R, lane, x0s, y0s, w, h, bxo, byo, s_tab, col, pre2, pre1, expfac);
       va0 = -R.e0;
       va1 = -R.e1;
       va2 = -R.e2;
@@ -1125,6 +1137,8 @@ __global__ __launch_bounds__(256) void k_stage_ops(SplatArgs a, const uint32_t *
   constexpr uint32_t P = stage_planes<ADD>();
   if (dn) i1 = min(i1, *dn);
   if (i1 <= i0) return;  // the whole grid (device count)
+  // the chains wait for these operands: issue ahead of the 8x8 splat's waves
+  __builtin_amdgcn_s_setprio(3);
   __shared__ float s_tab[2048];
   __shared__ uint32_t s_col[4][8][kPad];
   __shared__ float s_pre2[4][8][kPad];
@@ -1166,8 +1180,9 @@ __global__ __launch_bounds__(256) void k_stage_ops(SplatArgs a, const uint32_t *
     float va0 = 0.f, va1 = 0.f, va2 = 0.f, vmx = 0.f;
     float vq[3] = {0.f, 0.f, 0.f};
     if (lane < nb) {
-      const Rec R = tabulate<true>(a, base + lane, lane, ug.x0s, ug.y0s, ug.w, ug.h, ug.gx * 8,
-                                   ug.gy * 8, s_tab, col, pre2, pre1, expfac);
+      const Rec R = a.recs[a.ent[base + lane]];
+      tabulate<true>(a, R, lane, ug.x0s, ug.y0s, ug.w, ug.h, ug.gx * 8, ug.gy * 8, s_tab, col, pre2,
+                     pre1, expfac);
       va0 = -R.e0;
       va1 = -R.e1;
       va2 = -R.e2;
@@ -1416,9 +1431,9 @@ __global__ __launch_bounds__(256) void k_items_count(SplatArgs a, uint32_t nunit
     cnt[u] = k;
     if (heavy_cnt && k > 0 && len >= heavy_len_of(a)) atomicAdd(heavy_cnt, k);
     const uint32_t sl = stage_len_of(a);
-    if (stage_cnt && k > 0 && len >= sl) {  // counts per threshold sl << j, j = 0..7
-      for (int j = 0; j < 8; ++j) {
-        if (static_cast<uint64_t>(len) < (static_cast<uint64_t>(sl) << j)) break;
+    if (stage_cnt && k > 0 && len >= sl) {  // counts per level of sl
+      for (int j = 0; j < kStageLevels; ++j) {
+        if (static_cast<uint64_t>(len) < stage_level(sl, j)) break;
         atomicAdd(stage_cnt + j, 1u);
         atomicAdd(stage_grp + j, stage_groups(len));
       }
@@ -1487,13 +1502,33 @@ void launch_stage_base(const uint32_t *items, uint32_t n, uint32_t *cbase, uint3
   hip_check(hipGetLastError(), "k_stage_base launch");
 }
 
+// Workgroups of kernel k (256 threads) resident at once on the current device.
+unsigned resident_grid(const void *k, int slot) {
+  static unsigned cache[2][16] = {};
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "get device");
+  unsigned &r = cache[slot][dev & 15];
+  if (!r) {
+    int cus = 0, per = 0;
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
+    hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, 256, 0), "occupancy");
+    r = static_cast<unsigned>(std::max(1, cus * per));
+  }
+  return r;
+}
+
 void launch_stage_ops(bool additive, const uint32_t *items, uint32_t i0, uint32_t i1,
                       uint64_t groups, const uint32_t *cbase, const uint32_t *cbat, float4 *ops,
                       float *qv, const SplatArgs &a, hipStream_t s, const uint32_t *dn) {
   if (i1 <= i0) return;
-  // batches: at most groups / 16 + lists; about four per wave when there are many
+  // batches: at most groups / 16 + lists; about four per wave when there are
+  // many, in one round of resident workgroups (a second round would queue
+  // behind the 8x8 splat's workgroups, and the chains wait for all of them)
   const uint64_t nbat = groups / 16 + (i1 - i0);
-  const unsigned grid = std::max(1u, static_cast<unsigned>(std::min<uint64_t>(2048, (nbat + 15) / 16)));
+  const unsigned res = additive ? resident_grid(reinterpret_cast<const void *>(k_stage_ops<1>), 1)
+                                : resident_grid(reinterpret_cast<const void *>(k_stage_ops<0>), 0);
+  const unsigned grid =
+      std::max(1u, static_cast<unsigned>(std::min<uint64_t>(res, (nbat + 15) / 16)));
   if (additive) k_stage_ops<1><<<grid, 256, 0, s>>>(a, items, i0, i1, cbase, cbat, ops, qv, dn);
   else k_stage_ops<0><<<grid, 256, 0, s>>>(a, items, i0, i1, cbase, cbat, ops, qv, dn);
   hip_check(hipGetLastError(), "k_stage_ops launch");
@@ -1512,13 +1547,12 @@ void launch_stage_chain(bool additive, const uint32_t *items, uint32_t i0, uint3
 __global__ void k_partition(PartitionArgs a, uint32_t *out) {
   uint32_t ns = 0;
   if (a.len0 != 0xffffffffu) {
-    const uint32_t lmax = *a.lmax;
-    int k = 0;
-    if (a.ordered_auto)
-      while (k < 7 && (static_cast<uint64_t>(a.len0) << k) * a.span < lmax) ++k;
-    ns = a.stage_cnt[k];
-    if (ns > a.ns_cap || a.stage_grp[k] > a.groups_cap) ns = 0;
-    if (a.ordered_auto && static_cast<uint64_t>(lmax) * a.ratio < *a.etot) ns = 0;
+    const int j = stage_rule(a.len0, a.ordered_auto != 0, *a.lmax, *a.etot, a.span, a.ratio,
+                             a.crit);
+    if (j >= 0) {
+      ns = a.stage_cnt[j];
+      if (ns > a.ns_cap || a.stage_grp[j] > a.groups_cap) ns = 0;
+    }
   }
   const uint32_t nh = *a.heavy_cnt;
   const uint32_t npc = (a.heavy_max == 0 || nh <= ns) ? 0u : min(nh - ns, a.heavy_max);

@@ -167,7 +167,7 @@ def test_staged_trace_and_profile(R):
     assert len(staged) > 0
     assert np.all((staged[:, 2] >> 32) == 3)
     assert np.all(tr[:, 1] >= tr[:, 0])
-    assert st["additive_k_splat_heavy"] > 0
+    assert st["additive_staged"] > 0
 
 
 @pytest.mark.parametrize("chunk", [20_000, 3_000])
