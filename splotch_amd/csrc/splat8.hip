@@ -105,13 +105,20 @@ __device__ __forceinline__ int xexp_index(float p) {
   return static_cast<int>(ok ? p : 0.f);
 }
 
+// x where row cy of the column mask m is set (the pixel is in the particle's
+// footprint), else +0: the row's bit, sign-extended, masks x's bits.
+__device__ __forceinline__ float in_footprint(float x, uint32_t m, int cy) {
+  const uint32_t all = static_cast<uint32_t>(__builtin_amdgcn_sbfe(static_cast<int>(m), cy, 1));
+  return __uint_as_float(__float_as_uint(x) & all);
+}
+
 // Chunked walk of one batch for one channel class CH (0..2 one channel, 3 all):
 // for J particles at a time, phase 1 issues every LDS read of the chunk (column
 // masks, Gaussian factors, then the exp-table entries of each fragment) and
 // phase 2 applies the J updates in list order.  No loaded value is carried
 // across chunks, so the LDS latency is paid twice per chunk, not per particle.
-// The code is branch-free: loads past the batch end read a valid slot and are
-// masked, and a pixel outside a footprint adds +0.  The accumulators start at
+// The code is branch-free: loads past the batch end read slots whose column
+// masks are 0 (phase A clears them), and a pixel outside a footprint adds +0.  The accumulators start at
 // +0 and a sum of floats is -0 only if both terms are -0, so they never hold
 // -0 and l + 0 == l bit for bit: the result equals the reference's skip.
 template <int ADD, int CH, int J>
@@ -154,12 +161,9 @@ __device__ __forceinline__ void walk_chunks(int nb, int cx, int cy, const uint32
 #pragma unroll
         for (int j = 0; j < J; ++j) {
           const int k = k0 + j;
-          const uint32_t mk = cmask[cx][k];
-          const uint32_t mv = (k0 + j < nb) ? mk : 0u;
-          const bool in = (mv >> cy) & 1u;
           const float at = pre1[cy][k] * pre2[cx][k];
           tay[j] = (ca[j].w * at) < tl;
-          m2[j] = in ? (tay[j] ? at : 1.f) : 0.f;
+          m2[j] = in_footprint(tay[j] ? at : 1.f, cmask[cx][k], cy);
           if (C0) { g0[j] = at * ca[j].x; xtab_fast(tab, g0[j], expfac4, u0[j], w0[j]); }
           if (C1) { g1[j] = at * ca[j].y; xtab_fast(tab, g1[j], expfac4, u1[j], w1[j]); }
           if (C2) { g2[j] = at * ca[j].z; xtab_fast(tab, g2[j], expfac4, u2[j], w2[j]); }
@@ -202,16 +206,14 @@ __device__ __forceinline__ void walk_chunks(int nb, int cx, int cy, const uint32
       for (int j = 0; j < J; ++j) {
         const int k = k0 + j;
         ca[j] = cst[k];
-        const uint32_t mk = cmask[cx][k];
-        mv[j] = (k < nb) ? mk : 0u;
+        mv[j] = cmask[cx][k];
         at[j] = pre1[cy][k] * pre2[cx][k];
       }
 #pragma unroll
       for (int j = 0; j < J; ++j) {
-        const bool in = (mv[j] >> cy) & 1u;
-        if (C0) { const float t = ca[j].x * at[j]; l0 = l0 + (in ? t : 0.f); }
-        if (C1) { const float t = ca[j].y * at[j]; l1 = l1 + (in ? t : 0.f); }
-        if (C2) { const float t = ca[j].z * at[j]; l2 = l2 + (in ? t : 0.f); }
+        if (C0) l0 = l0 + in_footprint(ca[j].x * at[j], mv[j], cy);
+        if (C1) l1 = l1 + in_footprint(ca[j].y * at[j], mv[j], cy);
+        if (C2) l2 = l2 + in_footprint(ca[j].z * at[j], mv[j], cy);
       }
       continue;
     }
@@ -514,6 +516,10 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
       } else {
         wcst[lane] = make_float4(va0, va1, va2, 0.f);
       }
+    } else {
+      // a slot past a short batch: no pixel in its footprint
+#pragma unroll
+      for (int c = 0; c < 8; ++c) col[c][lane] = 0u;
     }
     wave_sync();
     const uint64_t fastmask = __ballot(vfast);
