@@ -423,6 +423,9 @@ struct Plan {
   JobSplat *d_js = nullptr;
   // per pass
   uint32_t m[2] = {0, 0}, e[2] = {0, 0}, nitems[2] = {0, 0}, nheavy[2] = {0, 0};
+  // long lists after the first nheavy items (one item each), which the 8x8
+  // splat serves by channel
+  uint32_t nsplit[2] = {0, 0};
   // staged lists (the first nstage items; nheavy counts them too) and their
   // groups of 4 entries (padded to whole chain loads)
   uint32_t nstage[2] = {0, 0};
@@ -873,6 +876,14 @@ constexpr uint64_t kStageOrderedRatio = 216;
 // rest (16M particles at 2048^2: 813k-entry list, 331M entries; the 4 lists
 // from 741376 on staged, 41.8 ms against 42.8 unstaged).
 constexpr uint64_t kStageOrderedCrit = 432;
+// Long lists left to the 8x8 splat that would outlast the pass, walked by
+// one wave at about 230 ns per entry against about 0.1 ns per entry of the
+// pass over all CUs (16M particles at 2048^2), are served by channel, three
+// waves each: those from entries / kSplitRatio on (rounded down to a level).
+// Serving every such list by channel costs more than it saves (41.2 ->
+// 48.2 ms at 16M / 2048^2): three one-channel walks take more VALU work than
+// one three-channel walk.
+constexpr uint32_t kSplitRatio = 2048;
 // Staged lists whose chains start right after their own operands (the
 // longest; the others' operands and chains follow beside them).
 constexpr uint32_t kStagePhase1 = 64;
@@ -985,6 +996,15 @@ void take_items(splhip_ctx *c, Plan &pl, int p, const Tuning &tn) {
   // the heaviest items (sorted first); a unit's channel items are
   // independent, so the cut may fall between them
   pl.nheavy[p] = std::min(ns + npc, pl.nitems[p]);
+  // lists from min(stage length, heavy_len) entries on have one item each
+  const uint32_t len0s = p == 1 ? tn.stage : tn.stage_ord;
+  const uint32_t nlong = !pl.m[p] ? 0u
+                         : len0s <= tn.heavy ? h[T_STAGE - T_ITEMS + kStageLevels * p]
+                                             : nheavy_lists;
+  pl.nsplit[p] = pl.m[p] ? split_lists(len0s, h + (T_STAGE - T_ITEMS + kStageLevels * p),
+                                       std::min(nlong, pl.nitems[p]), pl.nheavy[p], pl.etot[p],
+                                       kSplitRatio)
+                         : 0u;
 }
 
 // Splat of pass p: items heaviest first; heavy units on the auxiliary stream.
@@ -1027,8 +1047,10 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
     pa.ns_cap = pl.nstage[p];
     pa.groups_cap = pl.sgroups[p];
     pa.heavy_max = tn.heavy == 0xffffffffu ? 0u : tn.heavy_max;
+    pa.heavy_len = tn.heavy;
     pa.span = static_cast<uint32_t>(kStageOrderedSpan);
     pa.ratio = static_cast<uint32_t>(kStageOrderedRatio);
+    pa.split = kSplitRatio;
     pa.crit = static_cast<uint32_t>(kStageOrderedCrit);
     launch_partition(pa, part, s);
     dc.items = pl.tot + T_ITEMS + p;
@@ -1036,13 +1058,16 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
     dc.heavy_cap = 0xffffffffu;
     dc.first = part;
     dc.npc = part + 1;
+    dc.nsplit = part + 3;
   }
   const int rs = radix_sort_pairs(ik0, iv0, ik1, iv1, nitems, 0, 32, wt ? *wt : ws, s, dc.items);
   const uint32_t *items = rs ? iv1 : iv0;
   pl.items[p] = items;
   SplatArgs sa = sa0;
+  // the trace holds a record per item, the long lists served by channel three
+  const size_t ntrace = static_cast<size_t>(nitems) + 2 * static_cast<size_t>(pl.nsplit[p]);
   if (c->trace && !pl.dc)
-    sa.trace = ws.get_as<uint64_t>(pslot(p, P_TRACE), static_cast<size_t>(nitems) * 4);
+    sa.trace = ws.get_as<uint64_t>(pslot(p, P_TRACE), ntrace * 4);
   SplatArgs sn = sa;  // normal items follow the heavy ones
   if (sn.trace) sn.trace += 4 * static_cast<size_t>(nheavy);
   // normal items: after the long lists (with device counts the kernel skips them)
@@ -1107,7 +1132,7 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
     // path) must get their slots first, so the normal launch waits briefly.
     launch_delay(40, s);
     mark(c, p == 0 ? 7 : 9, true, s);
-    launch_splat8(p == 1, tn.variant, nit, nn, sn, s, dc, tn.ppw);
+    launch_splat8(p == 1, tn.variant, nit, nn, sn, s, dc, tn.ppw, pl.nsplit[p]);
     mark(c, p == 0 ? 7 : 9, false, s);
     if (npc > 0 && fork) hip_check(hipStreamWaitEvent(s, c->ev_join, 0), "wait join");
     if (nstage > 0) hip_check(hipStreamWaitEvent(s, c->ev_join3, 0), "wait join");
@@ -1117,17 +1142,17 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
     launch_splat_pc(p == 1, pit, npc, sp, s, dc);
     mark(c, p == 0 ? 8 : 10, false, s);
     mark(c, p == 0 ? 7 : 9, true, s);
-    launch_splat8(p == 1, tn.variant, nit, nn, sn, s, dc, tn.ppw);
+    launch_splat8(p == 1, tn.variant, nit, nn, sn, s, dc, tn.ppw, pl.nsplit[p]);
     mark(c, p == 0 ? 7 : 9, false, s);
   } else {
     mark(c, p == 0 ? 7 : 9, true, s);
-    launch_splat8(p == 1, tn.variant, items, nitems, sn, s, dc, tn.ppw);
+    launch_splat8(p == 1, tn.variant, items, nitems, sn, s, dc, tn.ppw, pl.nsplit[p]);
     mark(c, p == 0 ? 7 : 9, false, s);
   }
   mark(c, p == 0 ? 3 : 6, false, s);
   if (c->trace && !pl.dc) {
     std::vector<uint64_t> &tv = c->trace_items[p];
-    tv.resize(static_cast<size_t>(nitems) * 4);
+    tv.resize(ntrace * 4);
     if (nitems)
       hip_check(hipMemcpyAsync(tv.data(), sa.trace, tv.size() * 8, hipMemcpyDeviceToHost, s),
                 "D2H trace");

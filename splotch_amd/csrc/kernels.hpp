@@ -174,15 +174,20 @@ struct ItemCounts {
   uint32_t heavy_cap = 0;           // the 8x8 splat skips min(*heavy, heavy_cap) items
   const uint32_t *first = nullptr;  // producer/consumer items: from *first on,
   const uint32_t *npc = nullptr;    // *npc of them (a grid sized for a capacity)
+  const uint32_t *nsplit = nullptr; // long lists the 8x8 splat serves by channel
 };
 // With dc set, launch_splat8 is given the pass's whole item array and a grid
 // for nitems (a capacity) and skips the first min(*dc.heavy, heavy_cap) items;
 // launch_splat_pc is given the whole item array and npc = its capacity.  ppw
 // (variant 4): particles per chunk of the walk, 1, 2, 4 or 8 (0: 8 for one
-// channel, 8 additive / 4 ordered for all three).
+// channel, 8 additive / 4 ordered for all three).  nsplit (with dc:
+// *dc.nsplit): the first nsplit of the 8x8 splat's items are long lists (one
+// all-channel item each) left over by the staged and producer/consumer
+// kernels; the 8x8 splat serves each as three one-channel items, so that
+// three waves share its chain (a grid for nitems + 2 nsplit items).
 void launch_splat8(bool additive, int variant, const uint32_t *items, uint32_t nitems,
                    const SplatArgs &a, hipStream_t s, const ItemCounts &dc = ItemCounts{},
-                   int ppw = 0);
+                   int ppw = 0, uint32_t nsplit = 0);
 // Staged lists (splat8.hip), the first n items of a pass (groups: their
 // groups of 4 entries, padded): list offsets into cbase / cbat ([n + 1] each);
 // then, for lists [i0, i1), the per-pixel operands into ops (stage_floats
@@ -206,7 +211,9 @@ void launch_stage_chain(bool additive, const uint32_t *items, uint32_t i0, uint3
 // does it with exact counts (stage_rule): the staged lists stage_cnt[j] at
 // the rule's level j (none past ns_cap or groups_cap), then at most heavy_max
 // of the other lists from heavy_len on (the count heavy_cnt).  out[0] staged,
-// out[1] producer/consumer, out[2] both.
+// out[1] producer/consumer, out[2] both, out[3] the first of the other long
+// lists (one item each: from min(len0, heavy_len) entries on), which the 8x8
+// splat serves by channel (split_lists).
 struct PartitionArgs {
   const uint32_t *stage_cnt, *stage_grp, *heavy_cnt, *lmax;
   const uint64_t *etot;
@@ -215,8 +222,22 @@ struct PartitionArgs {
   uint32_t ns_cap;
   uint64_t groups_cap;
   uint32_t heavy_max;  // 0: no producer/consumer workgroups
-  uint32_t span, ratio, crit;
+  uint32_t heavy_len;  // lists from this length on go to producer/consumer workgroups
+  uint32_t span, ratio, crit, split;
 };
+// The long lists after the first nheavy items that the 8x8 splat serves by
+// channel: those from the greatest level of len0 with level * split <= etot
+// on (level 0 when none is; none without a stage length), of the nlong long
+// lists; count[j]: the lists from level j on.
+__host__ __device__ inline uint32_t split_lists(uint32_t len0, const uint32_t *count,
+                                                uint32_t nlong, uint32_t nheavy, uint64_t etot,
+                                                uint32_t split) {
+  if (len0 == 0xffffffffu) return 0u;
+  int j = 0;
+  while (j < kStageLevels - 1 && stage_level(len0, j + 1) * split <= etot) ++j;
+  const uint32_t n = count[j] < nlong ? count[j] : nlong;
+  return n > nheavy ? n - nheavy : 0u;
+}
 // The staging level of a pass: 0, or for an ordered pass by default (auto)
 // - when lmax * ratio >= etot (bound by its longest lists): the least level
 //   with level * span >= lmax,

@@ -112,22 +112,94 @@ __device__ __forceinline__ float in_footprint(float x, uint32_t m, int cy) {
   return __uint_as_float(__float_as_uint(x) & all);
 }
 
+// One fast chunk of the ordered walk (all particles of the chunk have every
+// a_c in [kFastA, 0] and finite q_c) for channel class CH: J particles from
+// slot k0, their constants in cst; bit j of TB set: particle k0 + j takes the
+// Taylor branch at every pixel of its footprint in the block, so its update
+// is l += ((l - q) * a) * att (att replaced by +0 outside the footprint) with
+// no table work.
+template <int CH, int J, uint32_t TB>
+__device__ __forceinline__ void fast_chunk(int k0, int cx, int cy, const uint32_t (*cmask)[kPad],
+                                           const float (*pre2)[kPad],
+                                           const float (*pre1)[kPad], const float *tab,
+                                           const float4 *cst, float expfac4, float tl,
+                                           float &l0, float &l1, float &l2) {
+  constexpr bool C0 = (CH == 3 || CH == 0), C1 = (CH == 3 || CH == 1), C2 = (CH == 3 || CH == 2);
+  float m2[J], g0[J], g1[J], g2[J], u0[J], w0[J], u1[J], w1[J], u2[J], w2[J];
+  float4 ca[J], cq[J];
+  bool tay[J];
+  // slots past the batch hold finite values of earlier batches (or
+  // zeros) and are masked; table offsets are masked into the tables
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int k = k0 + j;
+    ca[j] = cst[2 * k];
+    cq[j] = cst[2 * k + 1];
+  }
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const int k = k0 + j;
+    const float at = pre1[cy][k] * pre2[cx][k];
+    if ((TB >> j) & 1u) {
+      m2[j] = in_footprint(at, cmask[cx][k], cy);
+      continue;
+    }
+    tay[j] = (ca[j].w * at) < tl;
+    m2[j] = in_footprint(tay[j] ? at : 1.f, cmask[cx][k], cy);
+    if (C0) { g0[j] = at * ca[j].x; xtab_fast(tab, g0[j], expfac4, u0[j], w0[j]); }
+    if (C1) { g1[j] = at * ca[j].y; xtab_fast(tab, g1[j], expfac4, u1[j], w1[j]); }
+    if (C2) { g2[j] = at * ca[j].z; xtab_fast(tab, g2[j], expfac4, u2[j], w2[j]); }
+  }
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    if ((TB >> j) & 1u) {
+      if (C0) l0 = l0 + ((l0 - cq[j].x) * ca[j].x) * m2[j];
+      if (C1) l1 = l1 + ((l1 - cq[j].y) * ca[j].y) * m2[j];
+      if (C2) l2 = l2 + ((l2 - cq[j].z) * ca[j].z) * m2[j];
+      continue;
+    }
+    // the table product is formed before the select: written inside
+    // it, the compiler may move the table reads into a branch that
+    // waits for every LDS read in flight
+    if (C0) {
+      const float ex = u0[j] * w0[j] - 1.f;
+      const float em = (fabsf(g0[j]) < tl) ? g0[j] : ex;
+      const float m1 = tay[j] ? ca[j].x : em;
+      l0 = l0 + ((l0 - cq[j].x) * m1) * m2[j];
+    }
+    if (C1) {
+      const float ex = u1[j] * w1[j] - 1.f;
+      const float em = (fabsf(g1[j]) < tl) ? g1[j] : ex;
+      const float m1 = tay[j] ? ca[j].y : em;
+      l1 = l1 + ((l1 - cq[j].y) * m1) * m2[j];
+    }
+    if (C2) {
+      const float ex = u2[j] * w2[j] - 1.f;
+      const float em = (fabsf(g2[j]) < tl) ? g2[j] : ex;
+      const float m1 = tay[j] ? ca[j].z : em;
+      l2 = l2 + ((l2 - cq[j].z) * m1) * m2[j];
+    }
+  }
+}
+
 // Chunked walk of one batch for one channel class CH (0..2 one channel, 3 all):
 // for J particles at a time, phase 1 issues every LDS read of the chunk (column
 // masks, Gaussian factors, then the exp-table entries of each fragment) and
 // phase 2 applies the J updates in list order.  No loaded value is carried
 // across chunks, so the LDS latency is paid twice per chunk, not per particle.
 // The code is branch-free: loads past the batch end read slots whose column
-// masks are 0 (phase A clears them), and a pixel outside a footprint adds +0.  The accumulators start at
-// +0 and a sum of floats is -0 only if both terms are -0, so they never hold
-// -0 and l + 0 == l bit for bit: the result equals the reference's skip.
+// masks are 0 (phase A clears them), and a pixel outside a footprint adds +0.
+// The accumulators start at +0 and a sum of floats is -0 only if both terms
+// are -0, so they never hold -0 and l + 0 == l bit for bit: the result equals
+// the reference's skip.
 template <int ADD, int CH, int J>
 __device__ __forceinline__ void walk_chunks(int nb, int cx, int cy, const uint32_t (*cmask)[kPad],
                                             const float (*pre2)[kPad], const float (*pre1)[kPad],
                                             const float *tab, float va0, float va1, float va2,
                                             float vq0, float vq1, float vq2, float vmax,
                                             const float4 *cst, uint64_t fastmask,
-                                            float expfac, float tl, float &l0, float &l1,
+                                            uint64_t taymask, float expfac, float tl,
+                                            float &l0, float &l1,
                                             float &l2) {
   constexpr bool C0 = (CH == 3 || CH == 0), C1 = (CH == 3 || CH == 1), C2 = (CH == 3 || CH == 2);
   static_assert(64 % J == 0, "chunks tile a batch: slot k0 + j < 64");
@@ -147,50 +219,21 @@ __device__ __forceinline__ void walk_chunks(int nb, int cx, int cy, const uint32
       const int cnt = min(J, nb - k0);
       const uint64_t need = ((1ull << cnt) - 1ull) << k0;
       if ((fastmask & need) == need) {
-        float m2[J], g0[J], g1[J], g2[J], u0[J], w0[J], u1[J], w1[J], u2[J], w2[J];
-        float4 ca[J], cq[J];
-        bool tay[J];
-        // slots past the batch hold finite values of earlier batches (or
-        // zeros) and are masked; table offsets are masked into the tables
-#pragma unroll
-        for (int j = 0; j < J; ++j) {
-          const int k = k0 + j;
-          ca[j] = cst[2 * k];
-          cq[j] = cst[2 * k + 1];
-        }
-#pragma unroll
-        for (int j = 0; j < J; ++j) {
-          const int k = k0 + j;
-          const float at = pre1[cy][k] * pre2[cx][k];
-          tay[j] = (ca[j].w * at) < tl;
-          m2[j] = in_footprint(tay[j] ? at : 1.f, cmask[cx][k], cy);
-          if (C0) { g0[j] = at * ca[j].x; xtab_fast(tab, g0[j], expfac4, u0[j], w0[j]); }
-          if (C1) { g1[j] = at * ca[j].y; xtab_fast(tab, g1[j], expfac4, u1[j], w1[j]); }
-          if (C2) { g2[j] = at * ca[j].z; xtab_fast(tab, g2[j], expfac4, u2[j], w2[j]); }
-        }
-#pragma unroll
-        for (int j = 0; j < J; ++j) {
-          // the table product is formed before the select: written inside
-          // it, the compiler may move the table reads into a branch that
-          // waits for every LDS read in flight
-          if (C0) {
-            const float ex = u0[j] * w0[j] - 1.f;
-            const float em = (fabsf(g0[j]) < tl) ? g0[j] : ex;
-            const float m1 = tay[j] ? ca[j].x : em;
-            l0 = l0 + ((l0 - cq[j].x) * m1) * m2[j];
+        // bit j: particle k0 + j takes the Taylor branch at every pixel of
+        // its footprint in the block (wave-uniform): no table work for it.
+        // Three-channel chunks of 4 have a straight-line body per pattern.
+        const uint32_t tb = static_cast<uint32_t>(taymask >> k0);
+        if (CH == 3 && J == 4) {
+          switch (tb & 15u) {
+#define SPLHIP_FAST(T) case T: fast_chunk<CH, J, T>(k0, cx, cy, cmask, pre2, pre1, tab, cst, expfac4, tl, l0, l1, l2); break;
+            SPLHIP_FAST(0) SPLHIP_FAST(1) SPLHIP_FAST(2) SPLHIP_FAST(3)
+            SPLHIP_FAST(4) SPLHIP_FAST(5) SPLHIP_FAST(6) SPLHIP_FAST(7)
+            SPLHIP_FAST(8) SPLHIP_FAST(9) SPLHIP_FAST(10) SPLHIP_FAST(11)
+            SPLHIP_FAST(12) SPLHIP_FAST(13) SPLHIP_FAST(14) SPLHIP_FAST(15)
+#undef SPLHIP_FAST
           }
-          if (C1) {
-            const float ex = u1[j] * w1[j] - 1.f;
-            const float em = (fabsf(g1[j]) < tl) ? g1[j] : ex;
-            const float m1 = tay[j] ? ca[j].y : em;
-            l1 = l1 + ((l1 - cq[j].y) * m1) * m2[j];
-          }
-          if (C2) {
-            const float ex = u2[j] * w2[j] - 1.f;
-            const float em = (fabsf(g2[j]) < tl) ? g2[j] : ex;
-            const float m1 = tay[j] ? ca[j].z : em;
-            l2 = l2 + ((l2 - cq[j].z) * m1) * m2[j];
-          }
+        } else {
+          fast_chunk<CH, J, 0u>(k0, cx, cy, cmask, pre2, pre1, tab, cst, expfac4, tl, l0, l1, l2);
         }
         continue;
       }
@@ -359,13 +402,15 @@ __device__ __forceinline__ void walk_batch(int nb, int cx, int cy, const uint32_
 // columns, its row range (BITS: bit r set when block row byo + r lies in the
 // column's [lo, hi); else lo | hi << 16) and Gaussian column factor, and for
 // the 8 rows its Gaussian row factor, into column `lane` of the wave's LDS
-// slices.
+// slices.  att_bound: the largest row factor times the largest column factor
+// (+inf if one is NaN); float multiplication being monotone, every pixel's
+// att = pre1 * pre2 is at most it, and vmax * att at most vmax * att_bound.
 template <bool BITS>
 __device__ __forceinline__ void tabulate(const SplatArgs &a, const Rec &R, int lane, int x0s,
                                         int y0s, int w, int h, int bxo, int byo,
                                         const float *s_tab, uint32_t (*col)[kPad],
                                         float (*pre2)[kPad], float (*pre1)[kPad],
-                                        float expfac) {
+                                        float expfac, float &att_bound) {
   const float rfacr = R.r * a.rfac;
   const float posx = R.x - static_cast<float>(x0s), posy = R.y - static_cast<float>(y0s);
   const int bx0 = max(cvt_i32(posx - rfacr + 1), 0), bx1 = min(cvt_i32(posx + rfacr + 1), w);
@@ -373,6 +418,8 @@ __device__ __forceinline__ void tabulate(const SplatArgs &a, const Rec &R, int l
   const float radsq = rfacr * rfacr;
   const float sigma = a.h2sigma * R.r;
   const float stp = -1.f / (sigma * sigma);
+  float b2 = 0.f, b1 = 0.f;
+  bool nan = false;
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     const int x = bxo + c;
@@ -392,6 +439,8 @@ __device__ __forceinline__ void tabulate(const SplatArgs &a, const Rec &R, int l
         }
       }
       p2 = xop(s_tab, stp * dxsq, expfac);
+      b2 = fmaxf(b2, p2);
+      nan = nan || p2 != p2;
     }
     col[c][lane] = lohi;
     pre2[c][lane] = p2;
@@ -403,9 +452,12 @@ __device__ __forceinline__ void tabulate(const SplatArgs &a, const Rec &R, int l
     if (y >= by0 && y < by1) {
       const float dyf = static_cast<float>(y) - posy;
       p1 = xop(s_tab, stp * dyf * dyf, expfac);
+      b1 = fmaxf(b1, p1);
+      nan = nan || p1 != p1;
     }
     pre1[r][lane] = p1;
   }
+  att_bound = nan ? __builtin_inff() : b1 * b2;
 }
 
 // Work items of a launch: items[0, n), or with device counts (dc: the pass's
@@ -423,9 +475,13 @@ __device__ __forceinline__ uint32_t splat_items(const uint32_t *&items, uint32_t
 // one channel, 8 additive / 4 ordered for all three).
 template <int ADD, int VAR, int PPW>
 __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *items,
-                                                uint32_t nitems_, ItemCounts dc) {
+                                                uint32_t nitems_, ItemCounts dc,
+                                                uint32_t nsplit_) {
   constexpr int J1 = PPW ? PPW : 8, J3 = PPW ? PPW : (ADD ? 8 : 4);
-  const uint32_t nitems = splat_items(items, nitems_, dc);
+  const uint32_t nreal = splat_items(items, nitems_, dc);
+  // the first nsplit items are long lists served as three one-channel items
+  const uint32_t nsplit = min(dc.nsplit ? *dc.nsplit : nsplit_, nreal);
+  const uint32_t nitems = nreal + 2u * nsplit;
   __shared__ float s_tab[2048];
   __shared__ uint32_t s_col[4][8][kPad];
   __shared__ float s_pre2[4][8][kPad];
@@ -443,7 +499,8 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
   const uint64_t t_start = a.trace ? __builtin_amdgcn_s_memrealtime() : 0;
   // wave-uniform values are moved to scalar registers so that loop bounds and
   // channel selection compile to scalar branches
-  const uint32_t item = uniform(items[it]);
+  const uint32_t item = it < 3u * nsplit ? (uniform(items[it / 3u]) & ~3u) | (it % 3u)
+                                          : uniform(items[it - 2u * nsplit]);
   const uint32_t u = item >> 2;
   const int chan = static_cast<int>(item & 3u);  // 3: all channels
   const UnitGeom ug = unit_geom_uniform(a, u);
@@ -493,9 +550,11 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
     if (base + 128u + lane < e1) sn = a.ent[base + 128u + lane];
     // Phase A: lane = particle (splotch_host.cc:462-476 + 596-607, tile frame)
     float va0 = 0.f, va1 = 0.f, va2 = 0.f, vq0 = 0.f, vq1 = 0.f, vq2 = 0.f, vmax = 0.f;
-    bool vfast = false;
+    bool vfast = false, vtay = false;
     if (lane < nb) {
-      tabulate<VAR >= 3>(a, R, lane, x0s, y0s, w, h, bxo, byo, s_tab, col, pre2, pre1, expfac);
+      float ab;
+      tabulate<VAR >= 3>(a, R, lane, x0s, y0s, w, h, bxo, byo, s_tab, col, pre2, pre1, expfac,
+                         ab);
       va0 = -R.e0;
       va1 = -R.e1;
       va2 = -R.e2;
@@ -507,6 +566,9 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
         vmax = fmaxf(fabsf(va0), fmaxf(fabsf(va1), fabsf(va2)));
         vfast = va0 >= kFastA && va0 <= 0.f && va1 >= kFastA && va1 <= 0.f && va2 >= kFastA &&
                 va2 <= 0.f && finite3(vq0, vq1, vq2);
+        // Taylor particle: vmax * att < tl at every pixel of the block
+        // (splotch_host.cc:648), shown by the bound on att
+        vtay = vfast && (vmax * ab) < tl;
         // only the fast chunks read these, also for masked slots past the
         // batch, so a particle that is not fast (possibly a non-finite a or
         // q) leaves zeros, which a masked slot applies as l + 0
@@ -522,14 +584,14 @@ __global__ __launch_bounds__(256) void k_splat8(SplatArgs a, const uint32_t *ite
       for (int c = 0; c < 8; ++c) col[c][lane] = 0u;
     }
     wave_sync();
-    const uint64_t fastmask = __ballot(vfast);
+    const uint64_t fastmask = __ballot(vfast), taymask = __ballot(vtay);
     // Phase B: lane = pixel, particles in list order.
     if (VAR == 4) {
       switch (chan) {
-        case 0: walk_chunks<ADD, 0, J1>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, wcst, fastmask, expfac, tl, l0, l1, l2); break;
-        case 1: walk_chunks<ADD, 1, J1>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, wcst, fastmask, expfac, tl, l0, l1, l2); break;
-        case 2: walk_chunks<ADD, 2, J1>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, wcst, fastmask, expfac, tl, l0, l1, l2); break;
-        default: walk_chunks<ADD, 3, J3>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, wcst, fastmask, expfac, tl, l0, l1, l2); break;
+        case 0: walk_chunks<ADD, 0, J1>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, wcst, fastmask, taymask, expfac, tl, l0, l1, l2); break;
+        case 1: walk_chunks<ADD, 1, J1>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, wcst, fastmask, taymask, expfac, tl, l0, l1, l2); break;
+        case 2: walk_chunks<ADD, 2, J1>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, wcst, fastmask, taymask, expfac, tl, l0, l1, l2); break;
+        default: walk_chunks<ADD, 3, J3>(nb, cx, cy, col, pre2, pre1, s_tab, va0, va1, va2, vq0, vq1, vq2, vmax, wcst, fastmask, taymask, expfac, tl, l0, l1, l2); break;
       }
       wave_sync();
       continue;
@@ -1187,8 +1249,9 @@ __global__ __launch_bounds__(256) void k_stage_ops(SplatArgs a, const uint32_t *
     float vq[3] = {0.f, 0.f, 0.f};
     if (lane < nb) {
       const Rec R = a.recs[a.ent[base + lane]];
+      float ab;
       tabulate<true>(a, R, lane, ug.x0s, ug.y0s, ug.w, ug.h, ug.gx * 8, ug.gy * 8, s_tab, col, pre2,
-                     pre1, expfac);
+                     pre1, expfac, ab);
       va0 = -R.e0;
       va1 = -R.e1;
       va2 = -R.e2;
@@ -1562,9 +1625,11 @@ __global__ void k_partition(PartitionArgs a, uint32_t *out) {
   }
   const uint32_t nh = *a.heavy_cnt;
   const uint32_t npc = (a.heavy_max == 0 || nh <= ns) ? 0u : min(nh - ns, a.heavy_max);
+  const uint32_t nlong = a.len0 <= a.heavy_len ? a.stage_cnt[0] : nh;
   out[0] = ns;
   out[1] = npc;
   out[2] = ns + npc;
+  out[3] = split_lists(a.len0, a.stage_cnt, nlong, ns + npc, *a.etot, a.split);
 }
 
 void launch_partition(const PartitionArgs &pa, uint32_t *out, hipStream_t s) {
@@ -1581,10 +1646,14 @@ void launch_splat_pc(bool additive, const uint32_t *items, uint32_t npc, const S
 }
 
 void launch_splat8(bool additive, int variant, const uint32_t *items, uint32_t nitems,
-                   const SplatArgs &a, hipStream_t s, const ItemCounts &dc, int ppw) {
+                   const SplatArgs &a, hipStream_t s, const ItemCounts &dc, int ppw,
+                   uint32_t nsplit) {
   if (nitems == 0) return;
-  const unsigned grid = (nitems + 3) / 4;
-#define SPLAT8(ADDV, VARV, PPWV) k_splat8<ADDV, VARV, PPWV><<<grid, 256, 0, s>>>(a, items, nitems, dc)
+  // device counts: nitems, a capacity of three items per unit, also bounds
+  // the items with the long lists served by channel
+  if (dc.items) nsplit = 0;
+  const unsigned grid = static_cast<unsigned>((uint64_t(nitems) + 2u * uint64_t(nsplit) + 3) / 4);
+#define SPLAT8(ADDV, VARV, PPWV) k_splat8<ADDV, VARV, PPWV><<<grid, 256, 0, s>>>(a, items, nitems, dc, nsplit)
   if (variant == 4) {
     switch (ppw) {
       case 1: if (additive) SPLAT8(1, 4, 1); else SPLAT8(0, 4, 1); break;
