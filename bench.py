@@ -772,7 +772,8 @@ def main():
                           "valu_instructions_per_frame ordered_splat") if default else None
     if ninst is not None:
         rate = ninst / (stages["ordered_splat"] * 1e-3)
-        valu = {"kernels": "k_splat8<ordered> + k_splat_pc<ordered> (concurrent)",
+        valu = {"kernels": "k_splat8<ordered> + k_splat_pc<ordered> + k_stage_ops<ordered> + "
+                           "k_stage_chain<ordered> (concurrent)",
                 "instructions": int(ninst), "stage_ms": stages["ordered_splat"],
                 "achieved": round(rate / 1e9, 2), "peak": VALU_PEAK / 1e9,
                 "unit": "G wave64 instr/s", "frac": round(rate / VALU_PEAK, 4),
