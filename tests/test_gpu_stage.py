@@ -8,8 +8,10 @@ and the oracle, and against the unstaged render, with every list staged,
 with staged lists beside producer/consumer workgroups and normal items, in
 chunked passes (chains continued from the image), for both passes of one
 view, in batches of views, with non-finite particle intensities (NaN in
-exactly the oracle's pixels), and with a non-finite q_c (gray_absorption 0:
-no ordered staging, and the image unchanged).
+exactly the oracle's pixels), with a non-finite q_c (gray_absorption 0:
+no ordered staging, and the image unchanged), and with long lists that
+neither the staged nor the producer/consumer kernels take, walked by channel
+(plain and replayed from a captured device-count render).
 """
 import math
 import sys
@@ -262,3 +264,59 @@ def test_staged_deterministic(R):
     c = render(R, fld, v, [pal], S.params(a_eq_e=True, hip_stage=-1))
     assert_bitexact(a, b, "staged repeat")
     assert_bitexact(a, c, "staged vs unstaged")
+
+
+def test_leftover_long_lists_by_channel(R):
+    """Long lists that neither the staged nor the producer/consumer kernels
+    take are walked by channel, three waves each: 1M gas particles at 512^2
+    stage their lists from 11584 entries on (the longest / 16, rounded up to a
+    level), 4 producer/consumer workgroups take the next 4, and the other
+    lists from 8192 entries on (the split level for 3.9M entries) are left.
+    The trace holds three one-channel records for each of them; the image
+    equals the oracle's bit for bit, plain and replayed from a captured
+    device-count render."""
+    import splotch_amd as S
+    fld = synth.plummer_field(1_000_000, 1100, hsml_mu=math.log(0.5), ptype=0)
+    v = synth.default_view(512)
+    pal = synth.default_palette()
+    log("case leftover long lists by channel: 1M gas, 512^2, hip_heavy_max 4")
+    want, _ = O.host_rendering(fld, v, [pal], a_eq_e=False, stable=True, brightness=[0.5])
+    prm = S.params(a_eq_e=False, brightness=[0.5], hip_heavy_max=4)
+    R.set_trace(True)
+    try:
+        got = render(R, fld, v, [pal], prm)
+        tr = R.trace_items(True)
+    finally:
+        R.set_trace(False)
+    assert_bitexact(got, want, "leftover long lists by channel")
+    ln = (tr[:, 2] & 0xffffffff).astype(np.int64)
+    ch = tr[:, 2] >> 32
+    split = (ch != 3) & (ln >= 8192)
+    log("one-channel records of long lists: %d" % int(split.sum()))
+    assert split.sum() >= 3 and split.sum() % 3 == 0
+    for c in range(3):
+        assert ((ch == c) & (ln >= 8192)).sum() == split.sum() // 3
+    # the same view through a captured device-count render
+    cam = S.camera_setup(v)
+    views = [(S.to_device(fld), cam, prm,
+              torch.empty(v.xres * v.yres * 3, dtype=torch.float32, device="cuda"))]
+    R.capture_reserve(views, 1.5)
+    torch.cuda.synchronize()
+    assert_bitexact(views[0][3].cpu().numpy().reshape(v.xres, v.yres, 3), want,
+                    "leftover long lists, device counts")
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(g, stream=side):
+            R.render_batched(views)
+    torch.cuda.synchronize()
+    for rep in range(2):
+        views[0][3].fill_(7.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert_bitexact(views[0][3].cpu().numpy().reshape(v.xres, v.yres, 3), want,
+                        "leftover long lists, replay %d" % rep)
+        assert R.capture_status() == 0
+    del g
