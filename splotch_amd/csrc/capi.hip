@@ -785,8 +785,10 @@ void stage_count(splhip_ctx *c, Plan &pl, int p, const Rec *rec, uint32_t *tot, 
 // blocks the footprint can reach, sort them by region, and split each
 // region's list into its 8x8 blocks' lists (render order in both forms; the
 // region lists leave out entries that would add +0 to every pixel).
-void stage_bins(splhip_ctx *c, Plan &pl, int p, hipStream_t s) {
+// wt: workspace of the temporaries (sorts and scans; default the plan's).
+void stage_bins(splhip_ctx *c, Plan &pl, int p, hipStream_t s, Workspace *wt = nullptr) {
   Workspace &ws = *pl.ws;
+  Workspace &tmp = wt ? *wt : ws;
   const uint32_t m = pl.m[p], e = pl.e[p];
   mark(c, p == 0 ? 2 : 5, true, s);
   // device counts (m, e, e16 are capacities then)
@@ -807,7 +809,7 @@ void stage_bins(splhip_ctx *c, Plan &pl, int p, hipStream_t s) {
                 pl.nj, rk0, rv0, s, dm, de16);
     // the sort orders the region bits only: the unit masks above them ride
     // along, and each region's list keeps render order
-    const int r = radix_sort_pairs(rk0, rv0, rk1, rv1, e16, 0, rbits, ws, s, de16);
+    const int r = radix_sort_pairs(rk0, rv0, rk1, rv1, e16, 0, rbits, tmp, s, de16);
     const uint32_t *rk = r ? rk1 : rk0, *rv = r ? rv1 : rv0;
     uint32_t *roff = ws.get_as<uint32_t>(pslot(p, P_ROFF), static_cast<size_t>(nr) + 1);
     launch_bin_offsets(rk, e16, nr, roff, s, (1u << rbits) - 1u, de16);
@@ -819,10 +821,10 @@ void stage_bins(splhip_ctx *c, Plan &pl, int p, hipStream_t s) {
     uint32_t *bin_off = ws.get_as<uint32_t>(pslot(p, P_BINOFF), static_cast<size_t>(nu) + 1);
     uint32_t *ent = ws.get_as<uint32_t>(pslot(p, P_ENT), e);
     launch_split_chunks(S, rk, e16, rbits, ccnt, s, de16);
-    scan_u32(ccnt, cpre, static_cast<int64_t>(ncb), cpre + ncb, ws, s);
+    scan_u32(ccnt, cpre, static_cast<int64_t>(ncb), cpre + ncb, tmp, s);
     launch_split_bounds(S, roff, rk, nr, cpre, e16, rbits, pl.d_rbase, pl.d_ubase, pl.d_geom,
                         pl.nj, sx, ucnt, s);
-    scan_u32(ucnt, bin_off, nu, bin_off + nu, ws, s);
+    scan_u32(ucnt, bin_off, nu, bin_off + nu, tmp, s);
     launch_split_base(S, nr, pl.d_rbase, pl.d_ubase, pl.d_geom, pl.nj, sx, bin_off, sx, s);
     launch_split_scatter(S, rk, rv, e16, rbits, cpre, sx, ent, s, de16);
     pl.ent[p] = ent;
@@ -838,7 +840,7 @@ void stage_bins(splhip_ctx *c, Plan &pl, int p, hipStream_t s) {
   uint32_t *ev1 = ws.get_as<uint32_t>(pslot(p, P_EV1), e);
   launch_emit(0, 0, pl.recs[p], m, off, e, pl.d_geom, pl.d_ubase, pl.d_mo[p], pl.nj, ek0, ev0, s,
               dm, de);
-  const int r = radix_sort_pairs(ek0, ev0, ek1, ev1, e, 0, ceil_log2(pl.nunits), ws, s, de);
+  const int r = radix_sort_pairs(ek0, ev0, ek1, ev1, e, 0, ceil_log2(pl.nunits), tmp, s, de);
   uint32_t *bin_off = ws.get_as<uint32_t>(pslot(p, P_BINOFF), static_cast<size_t>(pl.nunits) + 1);
   launch_bin_offsets(r ? ek1 : ek0, e, pl.nunits, bin_off, s, ~0u, de);
   pl.ent[p] = r ? ev1 : ev0;
@@ -945,7 +947,7 @@ SplatArgs splat_args(splhip_ctx *c, const Plan &pl, int p, const Tuning &tn) {
 
 // Work items of pass p: count per unit (heavy units counted too), scan.
 void stage_items_count(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, uint32_t *tot,
-                       hipStream_t s) {
+                       hipStream_t s, Workspace *wt = nullptr) {
   Workspace &ws = *pl.ws;
   mark(c, p == 0 ? 2 : 5, true, s);
   const SplatArgs sa = splat_args(c, pl, p, tn);
@@ -961,7 +963,7 @@ void stage_items_count(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, uint32_
                      tot + T_HEAVY + p, s, tot + T_STAGE + kStageLevels * p,
                      tot + T_SGRP + kStageLevels * p,
                      tot + T_LMAX + p);
-  scan_u32(icnt, ioff, pl.nunits, tot + T_ITEMS + p, ws, s);
+  scan_u32(icnt, ioff, pl.nunits, tot + T_ITEMS + p, wt ? *wt : ws, s);
   mark(c, p == 0 ? 2 : 5, false, s);
 }
 
@@ -1404,8 +1406,8 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
   }
   bool tuned = false;
   // A single view with both passes (no autotuning): the additive pass is
-  // binned first and splatted on a second stream, into a cleared scratch
-  // image, while the ordered pass bins (memory-bound) and splats; the
+  // binned and splatted on a second stream, into a cleared scratch image,
+  // while the ordered pass bins (memory-bound) and splats; the
   // scratch image is then added to the image: pic + l per pixel, the
   // additive pass's own update (pixels it does not touch add +0, which
   // leaves any value but -0 as it is; the ordered pass never writes -0).
@@ -1418,14 +1420,22 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
     Tuning ta = tn;
     ta.heavy = 0xffffffffu;
     ta.heavy_max = 1;
-    stage_bins(c, pl, 1, s);
-    stage_items_count(c, pl, 1, ta, tot, s);
-    read_totals(c, tot + T_ITEMS, T_N - T_ITEMS, s);
-    take_items(c, pl, 1, ta);
     const JobSplat &J0 = pl.js[0];
     const int64_t npix = static_cast<int64_t>(J0.xres) * J0.yres * 3;
     float *scratch = ws.get_as<float>(S_SCRATCH, static_cast<size_t>(npix));
     JobSplat *d_alt = ws.get_as<JobSplat>(S_JOBS2, 1);
+    // The additive pass bins on the second stream, beside the ordered pass's
+    // binning (its sort and scan temporaries in the second workspace), and
+    // its counts are read as soon as that stream has them.
+    ensure_aux2(c);
+    hip_check(hipEventRecord(c->ev_fork2, s), "record fork");
+    hip_check(hipStreamWaitEvent(c->aux2, c->ev_fork2, 0), "wait fork");
+    stage_bins(c, pl, 1, c->aux2, &c->ws2);
+    stage_items_count(c, pl, 1, ta, tot, c->aux2, &c->ws2);
+    stage_bins(c, pl, 0, s);
+    stage_items_count(c, pl, 0, tn, tot, s);
+    read_totals(c, tot + T_ITEMS, T_N - T_ITEMS, c->aux2);  // waits for the second stream only
+    take_items(c, pl, 1, ta);
     // same flags as the items were counted with, but written to the scratch
     // image (blocks without items stay zero)
     JobSplat alt = J0;
@@ -1433,17 +1443,12 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
     alt.accumulate[1] = 0;
     char *h = staging(c, sizeof(JobSplat));
     std::memcpy(h, &alt, sizeof(JobSplat));
-    hip_check(hipMemcpyAsync(d_alt, h, sizeof(JobSplat), hipMemcpyHostToDevice, s),
+    hip_check(hipMemcpyAsync(d_alt, h, sizeof(JobSplat), hipMemcpyHostToDevice, c->aux2),
               "upload scratch job");
-    staged(c, s);
-    ensure_aux2(c);
-    hip_check(hipEventRecord(c->ev_fork2, s), "record fork");
-    hip_check(hipStreamWaitEvent(c->aux2, c->ev_fork2, 0), "wait fork");
+    staged(c, c->aux2);
     launch_fill(scratch, 0.f, npix, c->aux2);
     stage_splat(c, pl, 1, ta, c->aux2, &c->ws2, d_alt, false);
     hip_check(hipEventRecord(c->ev_join2, c->aux2), "record join");
-    stage_bins(c, pl, 0, s);
-    stage_items_count(c, pl, 0, tn, tot, s);
     read_totals(c, tot + T_ITEMS, T_N - T_ITEMS, s);  // waits for stream s only
     take_items(c, pl, 0, tn);
     stage_splat(c, pl, 0, tn, s);
