@@ -48,6 +48,21 @@ __device__ __forceinline__ float xop(const float *tab, float arg, float expfac) 
   return (p < 2147483648.f) ? 0.f : 1.f;
 }
 
+// xop in two steps, so that several lookups can be in flight: xop_reads
+// reads the two table entries for p = arg * expfac (at an in-range index in
+// every case), xop_value gives xop's result from them.
+__device__ __forceinline__ void xop_reads(const float *tab, float p, float &t1, float &t2) {
+  const bool ok = (p < 1048576.f) && (p > -1.f);
+  const int i = static_cast<int>(ok ? p : 0.f);
+  t1 = tab[i >> 10];
+  t2 = tab[1024 + (i & 1023)];
+}
+__device__ __forceinline__ float xop_value(float p, float t1, float t2) {
+  const bool hi_ok = p < 1048576.f;
+  const bool ok = hi_ok && (p > -1.f);
+  return ok ? t1 * t2 : ((hi_ok || !(p < 2147483648.f)) ? 1.f : 0.f);
+}
+
 __device__ __forceinline__ float xexpm1(const float *tab, float arg, float expfac, float tl) {
   if (fabsf(arg) < tl) return arg;
   return xop(tab, arg, expfac) - 1.f;
@@ -418,43 +433,59 @@ __device__ __forceinline__ void tabulate(const SplatArgs &a, const Rec &R, int l
   const float radsq = rfacr * rfacr;
   const float sigma = a.h2sigma * R.r;
   const float stp = -1.f / (sigma * sigma);
-  float b2 = 0.f, b1 = 0.f;
-  bool nan = false;
+  // branch-free: every column and row is computed (outside the box the
+  // values are discarded, e.g. the square root of a negative number), the
+  // table reads of all 16 lookups are issued before any of them is used,
+  // and the LDS stores follow
+  uint32_t lohi[8];
+  bool cin[8], rin[8];
+  float pc[8], pr[8], t1[16], t2[16];
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     const int x = bxo + c;
-    uint32_t lohi = 0;
-    float p2 = 0.f;
-    if (x >= bx0 && x < bx1) {
-      const float dxsq = (static_cast<float>(x) - posx) * (static_cast<float>(x) - posx);
-      const float dy = sqrtf(radsq - dxsq);
-      const int lo = max(by0, cvt_i32(posy - dy + 1));
-      const int hi = min(by1, cvt_i32(posy + dy + 1));
-      if (hi > lo) {
-        if (BITS) {
-          const int r0 = min(max(lo - byo, 0), 8), r1 = min(max(hi - byo, 0), 8);
-          lohi = (r1 > r0) ? (((1u << r1) - 1u) & ~((1u << r0) - 1u)) : 0u;
-        } else {
-          lohi = static_cast<uint32_t>(lo) | (static_cast<uint32_t>(hi) << 16);
-        }
-      }
-      p2 = xop(s_tab, stp * dxsq, expfac);
-      b2 = fmaxf(b2, p2);
-      nan = nan || p2 != p2;
+    cin[c] = x >= bx0 && x < bx1;
+    const float dxsq = (static_cast<float>(x) - posx) * (static_cast<float>(x) - posx);
+    const float dy = sqrtf(radsq - dxsq);
+    const int lo = max(by0, cvt_i32(posy - dy + 1));
+    const int hi = min(by1, cvt_i32(posy + dy + 1));
+    uint32_t lh;
+    if (BITS) {
+      const int r0 = min(max(lo - byo, 0), 8), r1 = min(max(hi - byo, 0), 8);
+      lh = (r1 > r0) ? (((1u << r1) - 1u) & ~((1u << r0) - 1u)) : 0u;
+    } else {
+      lh = static_cast<uint32_t>(lo) | (static_cast<uint32_t>(hi) << 16);
     }
-    col[c][lane] = lohi;
-    pre2[c][lane] = p2;
+    lohi[c] = (cin[c] && hi > lo) ? lh : 0u;
+    pc[c] = (stp * dxsq) * expfac;
+    xop_reads(s_tab, pc[c], t1[c], t2[c]);
   }
 #pragma unroll
   for (int r = 0; r < 8; ++r) {
     const int y = byo + r;
-    float p1 = 0.f;
-    if (y >= by0 && y < by1) {
-      const float dyf = static_cast<float>(y) - posy;
-      p1 = xop(s_tab, stp * dyf * dyf, expfac);
-      b1 = fmaxf(b1, p1);
-      nan = nan || p1 != p1;
-    }
+    rin[r] = y >= by0 && y < by1;
+    const float dyf = static_cast<float>(y) - posy;
+    pr[r] = (stp * dyf * dyf) * expfac;
+    xop_reads(s_tab, pr[r], t1[8 + r], t2[8 + r]);
+  }
+  // every read is used here, so that none is moved into a branch of the
+  // selects below
+#pragma unroll
+  for (int k = 0; k < 16; ++k) asm volatile("" : "+v"(t1[k]), "+v"(t2[k]));
+  float b2 = 0.f, b1 = 0.f;
+  bool nan = false;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const float p2 = cin[c] ? xop_value(pc[c], t1[c], t2[c]) : 0.f;
+    b2 = fmaxf(b2, p2);
+    nan |= p2 != p2;
+    col[c][lane] = lohi[c];
+    pre2[c][lane] = p2;
+  }
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const float p1 = rin[r] ? xop_value(pr[r], t1[8 + r], t2[8 + r]) : 0.f;
+    b1 = fmaxf(b1, p1);
+    nan |= p1 != p1;
     pre1[r][lane] = p1;
   }
   att_bound = nan ? __builtin_inff() : b1 * b2;
