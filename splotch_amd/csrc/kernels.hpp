@@ -192,10 +192,11 @@ void launch_splat8(bool additive, int variant, const uint32_t *items, uint32_t n
 // groups of 4 entries, padded): list offsets into cbase / cbat ([n + 1] each);
 // then, for lists [i0, i1), the per-pixel operands into ops (stage_floats
 // floats for all n) and, ordered, the entries' q_c into qv (stage_q_floats
-// floats), and the chains, one wave per list, channel and 16-pixel slice,
-// which write the lists' blocks.  dn (device, optional): the staged lists
-// are min(n, *dn) (n, i1 are capacities).
-constexpr int kStageLoad = 4;  // groups of 4 entries per load of a staged chain
+// floats), and the chains, one wave per list, channel and pixel slice (8
+// pixels additive, 16 ordered), which write the lists' blocks.  dn (device,
+// optional): the staged lists are min(n, *dn) (n, i1 are capacities).
+constexpr int kStageLoad = 8;  // a list's groups of 4 entries: a multiple of kStageLoad
+constexpr int kStageQ = 4;     // groups per block of q_c (48 floats: 3 channels x 16)
 uint64_t stage_floats(bool additive, uint64_t groups);
 uint64_t stage_q_floats(bool additive, uint64_t groups);
 void launch_stage_base(const uint32_t *items, uint32_t n, uint32_t *cbase, uint32_t *cbat,

@@ -1169,7 +1169,8 @@ __global__ __launch_bounds__(kPThreads) void k_splat_pc(SplatArgs a, const uint3
 // plane c is 64 float4 (lane = pixel cx * 8 + cy, components = the group's 4
 // entries) at ops + (g * P + c) * 64, with P = 3 planes t_c (additive) or 4
 // (m1_0, m1_1, m1_2, m2; ordered).  Ordered lists also have q_c per entry:
-// qv[(g / kStageLoad) * 48 + c * 16 + e] for entry e of the load holding g.
+// qv[(g / kStageQ) * 48 + c * 16 + e] for entry e of the kStageQ groups
+// holding g.  A list's groups are padded to a multiple of kStageLoad.
 __host__ __device__ constexpr uint32_t stage_groups(uint32_t len) {
   return (len + 4 * kStageLoad - 1) / (4 * kStageLoad) * kStageLoad;
 }
@@ -1295,7 +1296,7 @@ __global__ __launch_bounds__(256) void k_stage_ops(SplatArgs a, const uint32_t *
       }
     }
     if (!ADD && lane < ng * 4) {  // q_c per entry (0 past the list), lane = entry
-      float *qd = qv + (g0 / kStageLoad + static_cast<uint32_t>(lane) / 16u) * 48u + lane % 16;
+      float *qd = qv + (g0 / kStageQ + static_cast<uint32_t>(lane) / 16u) * 48u + lane % 16;
       qd[0] = vq[0];
       qd[16] = vq[1];
       qd[32] = vq[2];
@@ -1339,19 +1340,23 @@ __global__ __launch_bounds__(256) void k_stage_ops(SplatArgs a, const uint32_t *
 // The chains of the staged lists.  A wave applies channel c of list i to a
 // slice of kPix pixels of the block: workgroup (3 i + c) kS + h takes pixels
 // [h kPix, h kPix + kPix).  One load instruction brings kS consecutive groups
-// of a plane (lane quarter q: group g + q of the slice's pixels, 16 bytes per
+// of a plane (lane part q: group g + q of the slice's pixels, 16 bytes per
 // lane), so the loads in flight per wave cover kS groups each; lanes
-// [0, kPix) run the chains and read the other quarters' groups back through
-// LDS one step ahead.  A chain's wave is bound by its loads in flight and its
-// serial operations, not by issue, so narrower slices (more waves per list)
-// run a list faster.
-constexpr int kS = kStageLoad, kPix = 64 / kS;
+// [0, kPix) run the chains and read the other parts' groups back through
+// LDS one step ahead.  A chain's wave is bound by its bytes in flight and
+// its serial operations, not by issue, so narrower slices (more waves per
+// list, fewer bytes per entry and wave) run a list faster: 8 pixels for
+// additive chains, 16 for ordered ones (whose three operand planes would not
+// fit the registers at 8).
+template <int ADD> constexpr int chain_groups() { return ADD ? kStageLoad : kStageQ; }
+static_assert(kStageLoad % kStageQ == 0, "chain loads tile the padded groups");
 template <int ADD>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_stage_chain(
     SplatArgs a, const uint32_t *items, uint32_t i0, const uint32_t *cbase,
     const float4 *__restrict__ ops, const float *__restrict__ qv, const uint32_t *dn) {
+  constexpr int kS = chain_groups<ADD>(), kPix = 64 / kS;
   constexpr uint32_t P = stage_planes<ADD>();
-  constexpr int kR = ADD ? 40 : 10;  // steps in flight (ordered: three loads per step)
+  constexpr int kR = ADD ? 32 : 10;  // steps in flight (ordered: three loads per step)
   __shared__ float4 xs[3][64];
   const uint32_t i = i0 + blockIdx.x / (3u * kS);
   if (dn && i >= *dn) return;  // past the device count (uniform per workgroup)
@@ -1379,24 +1384,29 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
   // group g of plane c and this pixel: src[g * P * 64]; lane quarter qo reads g + qo
   const float4 *src = ops + static_cast<size_t>(gb0) * P * 64u + static_cast<size_t>(c) * 64u + px;
   const float4 *srcw = ops + static_cast<size_t>(gb0) * P * 64u + 3u * 64u + px;  // m2 (ordered)
-  // q_c of the entries of group g + qo (4 floats; the same for a quarter's lanes)
-  const float4 *qsrc = reinterpret_cast<const float4 *>(qv + static_cast<size_t>(gb0 / kS) * 48u +
+  // q_c of the entries of group g + qo (4 floats; the same for a part's lanes;
+  // ordered chains load kS = kStageQ groups, one block of q_c)
+  const float4 *qsrc = reinterpret_cast<const float4 *>(qv + static_cast<size_t>(gb0 / kStageQ) * 48u +
                                                        static_cast<size_t>(c) * 16u) + qo;
   auto at_load = [&](uint32_t g0) {  // groups [g0, g0 + kS), the last ones again past the list
     return static_cast<size_t>(min<uint32_t>(g0, G - kS) + qo) * P * 64u;
   };
-  auto at_q = [&](uint32_t g0) {  // that load's q_c (48 floats per load of kS groups)
-    return static_cast<size_t>(min<uint32_t>(g0, G - kS) / kS) * 12u;
+  auto at_q = [&](uint32_t g0) {  // that load's q_c (48 floats per kStageQ groups)
+    return static_cast<size_t>(min<uint32_t>(g0, G - kS) / kStageQ) * 12u;
   };
-  // one load's 16 entries for lanes [0, kPix): its own quarter, then the
-  // other three read back through LDS
+  // one load's 4 kS entries for lanes [0, kPix): its own part, then the
+  // others read back through LDS
   struct Q {
-    float4 a, b, c, d;
+    float4 v[kS];
   };
   auto exchange = [&](int slot, const float4 &v) {
     xs[slot][lane] = v;
     wave_sync();
-    return Q{v, xs[slot][pl + kPix], xs[slot][pl + 2 * kPix], xs[slot][pl + 3 * kPix]};
+    Q r;
+    r.v[0] = v;
+#pragma unroll
+    for (int k = 1; k < kS; ++k) r.v[k] = xs[slot][pl + k * kPix];
+    return r;
   };
   struct Step {
     Q m, w, q;  // ordered: m1_c, m2 and q_c (the same in every lane) of the load's entries
@@ -1412,24 +1422,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
   };
   auto apply = [&](const Step &st) {
     if (ADD) {
-      const float4 *q4[4] = {&st.m.a, &st.m.b, &st.m.c, &st.m.d};
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        l = l + q4[k]->x;
-        l = l + q4[k]->y;
-        l = l + q4[k]->z;
-        l = l + q4[k]->w;
+      for (int k = 0; k < kS; ++k) {  // load part k: entries 4k .. 4k + 3
+        l = l + st.m.v[k].x;
+        l = l + st.m.v[k].y;
+        l = l + st.m.v[k].z;
+        l = l + st.m.v[k].w;
       }
     } else {
-      const float4 *m4[4] = {&st.m.a, &st.m.b, &st.m.c, &st.m.d};
-      const float4 *w4[4] = {&st.w.a, &st.w.b, &st.w.c, &st.w.d};
-      const float4 *q4[4] = {&st.q.a, &st.q.b, &st.q.c, &st.q.d};
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {  // load quarter k: entries 4k .. 4k + 3
-        l = l + ((l - q4[k]->x) * m4[k]->x) * w4[k]->x;
-        l = l + ((l - q4[k]->y) * m4[k]->y) * w4[k]->y;
-        l = l + ((l - q4[k]->z) * m4[k]->z) * w4[k]->z;
-        l = l + ((l - q4[k]->w) * m4[k]->w) * w4[k]->w;
+      for (int k = 0; k < kS; ++k) {
+        const float4 &m4 = st.m.v[k], &w4 = st.w.v[k], &q4 = st.q.v[k];
+        l = l + ((l - q4.x) * m4.x) * w4.x;
+        l = l + ((l - q4.y) * m4.y) * w4.y;
+        l = l + ((l - q4.z) * m4.z) * w4.z;
+        l = l + ((l - q4.w) * m4.w) * w4.w;
       }
     }
   };
@@ -1592,7 +1599,7 @@ uint64_t stage_floats(bool additive, uint64_t groups) {
 }
 
 uint64_t stage_q_floats(bool additive, uint64_t groups) {
-  return additive ? 0 : groups / kStageLoad * 48u;
+  return additive ? 0 : groups / kStageQ * 48u;
 }
 
 void launch_stage_base(const uint32_t *items, uint32_t n, uint32_t *cbase, uint32_t *cbat,
@@ -1638,7 +1645,8 @@ void launch_stage_chain(bool additive, const uint32_t *items, uint32_t i0, uint3
                         const uint32_t *cbase, const float4 *ops, const float *qv,
                         const SplatArgs &a, hipStream_t s, const uint32_t *dn) {
   if (i1 <= i0) return;
-  const unsigned grid = 3u * kS * (i1 - i0);
+  const unsigned grid = 3u * static_cast<unsigned>(additive ? chain_groups<1>() : chain_groups<0>()) *
+                        (i1 - i0);
   if (additive) k_stage_chain<1><<<grid, 64, 0, s>>>(a, items, i0, cbase, ops, qv, dn);
   else k_stage_chain<0><<<grid, 64, 0, s>>>(a, items, i0, cbase, ops, qv, dn);
   hip_check(hipGetLastError(), "k_stage_chain launch");
