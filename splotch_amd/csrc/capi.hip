@@ -83,17 +83,21 @@ struct splhip_ctx {
   uint64_t last_stage_groups[2] = {0, 0};
   int64_t last_regions[2] = {0, 0};  // region entries binned per pass (0: per block)
   // auxiliary stream (default priority) for the heavy-block kernel
-  hipStream_t aux = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // auxiliary streams of a pass's splat (set 1: the additive pass's when it
+  // runs beside the ordered pass's): producer/consumer workgroups (aux), the
+  // staged lists (aux3: the longest lists' operands and chains; aux4: the
+  // other lists'), and their fork and join events
+  struct AuxSet {
+    hipStream_t aux = nullptr, aux3 = nullptr, aux4 = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr, fork3 = nullptr, join3 = nullptr;
+    hipEvent_t fork4 = nullptr, join4 = nullptr;
+  };
+  AuxSet ax[2];
   // second stream for the additive pass's splat, which runs beside the
   // ordered pass's, with its own temporaries
   hipStream_t aux2 = nullptr;
   hipEvent_t ev_fork2 = nullptr, ev_join2 = nullptr;
   Workspace ws2;
-  // streams of the staged lists: the longest lists' operands and chains, then
-  // the other lists' (they follow the operand kernel)
-  hipStream_t aux3 = nullptr, aux4 = nullptr;
-  hipEvent_t ev_fork3 = nullptr, ev_join3 = nullptr, ev_fork4 = nullptr, ev_join4 = nullptr;
   // block-file ingest: two pinned buffers, each freed for reuse by its event
   char *h_ing[2] = {nullptr, nullptr};
   hipEvent_t ev_ing[2] = {nullptr, nullptr};
@@ -287,11 +291,12 @@ void collect_marks(splhip_ctx *c) {
   }
 }
 
-void ensure_aux(splhip_ctx *c) {
-  if (c->aux) return;
-  hip_check(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking), "hipStreamCreate");
-  hip_check(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming), "hipEventCreate");
-  hip_check(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming), "hipEventCreate");
+void ensure_aux(splhip_ctx *c, int k = 0) {
+  splhip_ctx::AuxSet &X = c->ax[k];
+  if (X.aux) return;
+  hip_check(hipStreamCreateWithFlags(&X.aux, hipStreamNonBlocking), "hipStreamCreate");
+  hip_check(hipEventCreateWithFlags(&X.fork, hipEventDisableTiming), "hipEventCreate");
+  hip_check(hipEventCreateWithFlags(&X.join, hipEventDisableTiming), "hipEventCreate");
 }
 
 void ensure_aux2(splhip_ctx *c) {
@@ -301,14 +306,15 @@ void ensure_aux2(splhip_ctx *c) {
   hip_check(hipEventCreateWithFlags(&c->ev_join2, hipEventDisableTiming), "hipEventCreate");
 }
 
-void ensure_aux3(splhip_ctx *c) {
-  if (c->aux3) return;
-  hip_check(hipStreamCreateWithFlags(&c->aux3, hipStreamNonBlocking), "hipStreamCreate");
-  hip_check(hipStreamCreateWithFlags(&c->aux4, hipStreamNonBlocking), "hipStreamCreate");
-  hip_check(hipEventCreateWithFlags(&c->ev_fork3, hipEventDisableTiming), "hipEventCreate");
-  hip_check(hipEventCreateWithFlags(&c->ev_join3, hipEventDisableTiming), "hipEventCreate");
-  hip_check(hipEventCreateWithFlags(&c->ev_fork4, hipEventDisableTiming), "hipEventCreate");
-  hip_check(hipEventCreateWithFlags(&c->ev_join4, hipEventDisableTiming), "hipEventCreate");
+void ensure_aux3(splhip_ctx *c, int k = 0) {
+  splhip_ctx::AuxSet &X = c->ax[k];
+  if (X.aux3) return;
+  hip_check(hipStreamCreateWithFlags(&X.aux3, hipStreamNonBlocking), "hipStreamCreate");
+  hip_check(hipStreamCreateWithFlags(&X.aux4, hipStreamNonBlocking), "hipStreamCreate");
+  hip_check(hipEventCreateWithFlags(&X.fork3, hipEventDisableTiming), "hipEventCreate");
+  hip_check(hipEventCreateWithFlags(&X.join3, hipEventDisableTiming), "hipEventCreate");
+  hip_check(hipEventCreateWithFlags(&X.fork4, hipEventDisableTiming), "hipEventCreate");
+  hip_check(hipEventCreateWithFlags(&X.join4, hipEventDisableTiming), "hipEventCreate");
 }
 
 
@@ -1014,8 +1020,10 @@ void take_items(splhip_ctx *c, Plan &pl, int p, const Tuning &tn) {
 // place of the plan's; fork: the long-list workgroups run on the auxiliary
 // stream beside splat8 (else on s, first).
 void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s,
-                 Workspace *wt = nullptr, JobSplat *jobs = nullptr, bool fork = true) {
+                 Workspace *wt = nullptr, JobSplat *jobs = nullptr, bool fork = true,
+                 int set = 0) {
   Workspace &ws = *pl.ws;
+  splhip_ctx::AuxSet &X = c->ax[set];
   SplatArgs sa0 = splat_args(c, pl, p, tn);
   if (jobs) sa0.jobs = jobs;
   mark(c, p == 0 ? 3 : 6, true, s);
@@ -1087,13 +1095,13 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
     // producer/consumer workgroups at once, the staged chains after the
     // operand kernel (on s, across the GPU)
     if (npc > 0 && fork) {
-      ensure_aux(c);
-      hip_check(hipEventRecord(c->ev_fork, s), "record fork");
-      hip_check(hipStreamWaitEvent(c->aux, c->ev_fork, 0), "wait fork");
-      mark(c, p == 0 ? 8 : 10, true, c->aux);
-      launch_splat_pc(p == 1, pit, npc, sp, c->aux, dc);
-      mark(c, p == 0 ? 8 : 10, false, c->aux);
-      hip_check(hipEventRecord(c->ev_join, c->aux), "record join");
+      ensure_aux(c, set);
+      hip_check(hipEventRecord(X.fork, s), "record fork");
+      hip_check(hipStreamWaitEvent(X.aux, X.fork, 0), "wait fork");
+      mark(c, p == 0 ? 8 : 10, true, X.aux);
+      launch_splat_pc(p == 1, pit, npc, sp, X.aux, dc);
+      mark(c, p == 0 ? 8 : 10, false, X.aux);
+      hip_check(hipEventRecord(X.join, X.aux), "record join");
     } else if (npc > 0) {
       mark(c, p == 0 ? 8 : 10, true, s);
       launch_splat_pc(p == 1, pit, npc, sp, s, dc);
@@ -1112,23 +1120,23 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
       // own operands; the other lists' operands and chains follow on a fourth
       // stream, beside those chains
       const uint32_t n1 = std::min(nstage, kStagePhase1);
-      ensure_aux3(c);
-      hip_check(hipEventRecord(c->ev_fork3, s), "record fork");
-      hip_check(hipStreamWaitEvent(c->aux3, c->ev_fork3, 0), "wait fork");
-      mark(c, p == 0 ? 11 : 12, true, c->aux3);
-      launch_stage_base(items, nstage, cb, cbt, sa, c->aux3, part);
-      launch_stage_ops(add, items, 0, n1, pl.sgroups[p], cb, cbt, ops, qv, sa, c->aux3, part);
+      ensure_aux3(c, set);
+      hip_check(hipEventRecord(X.fork3, s), "record fork");
+      hip_check(hipStreamWaitEvent(X.aux3, X.fork3, 0), "wait fork");
+      mark(c, p == 0 ? 11 : 12, true, X.aux3);
+      launch_stage_base(items, nstage, cb, cbt, sa, X.aux3, part);
+      launch_stage_ops(add, items, 0, n1, pl.sgroups[p], cb, cbt, ops, qv, sa, X.aux3, part);
       if (n1 < nstage) {
-        hip_check(hipEventRecord(c->ev_fork4, c->aux3), "record fork");
-        hip_check(hipStreamWaitEvent(c->aux4, c->ev_fork4, 0), "wait fork");
-        launch_stage_ops(add, items, n1, nstage, pl.sgroups[p], cb, cbt, ops, qv, sa, c->aux4,
+        hip_check(hipEventRecord(X.fork4, X.aux3), "record fork");
+        hip_check(hipStreamWaitEvent(X.aux4, X.fork4, 0), "wait fork");
+        launch_stage_ops(add, items, n1, nstage, pl.sgroups[p], cb, cbt, ops, qv, sa, X.aux4,
                          part);
-        launch_stage_chain(add, items, n1, nstage, cb, ops, qv, sa, c->aux4, part);
-        hip_check(hipEventRecord(c->ev_join4, c->aux4), "record join");
+        launch_stage_chain(add, items, n1, nstage, cb, ops, qv, sa, X.aux4, part);
+        hip_check(hipEventRecord(X.join4, X.aux4), "record join");
       }
-      launch_stage_chain(add, items, 0, n1, cb, ops, qv, sa, c->aux3, part);
-      mark(c, p == 0 ? 11 : 12, false, c->aux3);
-      hip_check(hipEventRecord(c->ev_join3, c->aux3), "record join");
+      launch_stage_chain(add, items, 0, n1, cb, ops, qv, sa, X.aux3, part);
+      mark(c, p == 0 ? 11 : 12, false, X.aux3);
+      hip_check(hipEventRecord(X.join3, X.aux3), "record join");
     }
     // The normal items fill every CU; the heavy workgroups (the critical
     // path) must get their slots first, so the normal launch waits briefly.
@@ -1136,9 +1144,9 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
     mark(c, p == 0 ? 7 : 9, true, s);
     launch_splat8(p == 1, tn.variant, nit, nn, sn, s, dc, tn.ppw, pl.nsplit[p]);
     mark(c, p == 0 ? 7 : 9, false, s);
-    if (npc > 0 && fork) hip_check(hipStreamWaitEvent(s, c->ev_join, 0), "wait join");
-    if (nstage > 0) hip_check(hipStreamWaitEvent(s, c->ev_join3, 0), "wait join");
-    if (nstage > kStagePhase1) hip_check(hipStreamWaitEvent(s, c->ev_join4, 0), "wait join");
+    if (npc > 0 && fork) hip_check(hipStreamWaitEvent(s, X.join, 0), "wait join");
+    if (nstage > 0) hip_check(hipStreamWaitEvent(s, X.join3, 0), "wait join");
+    if (nstage > kStagePhase1) hip_check(hipStreamWaitEvent(s, X.join4, 0), "wait join");
   } else if (npc > 0) {
     mark(c, p == 0 ? 8 : 10, true, s);
     launch_splat_pc(p == 1, pit, npc, sp, s, dc);
@@ -1160,6 +1168,14 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
                 "D2H trace");
     hip_check(hipStreamSynchronize(s), "sync trace");
   }
+}
+
+// No view of the plan has particles in both passes (the passes write
+// disjoint images).
+bool disjoint_passes(const Plan &pl) {
+  for (int j = 0; j < pl.nj; ++j)
+    if (pl.mo[0][j + 1] > pl.mo[0][j] && pl.mo[1][j + 1] > pl.mo[1][j]) return false;
+  return true;
 }
 
 // Read n consecutive device totals into c->h_tot (one sync).
@@ -1416,7 +1432,30 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
   const bool sweep = !force && G == 8 && jobs[0].prm->hip_autotune;
   const bool early = !chunked[0] && !chunked[1] && nj == 1 && G == 8 && pl.m[0] && pl.m[1] &&
                      pl.js[0].accumulate[1] == 1 && !sweep;
-  if (early) {
+  // A batch in which no view has particles of both modes (the passes write
+  // disjoint images): the additive pass bins and splats on a second stream,
+  // with its own temporaries and auxiliary streams, beside the ordered pass;
+  // the ordered splat is launched first (its long lists are the critical
+  // path).
+  const bool side = !early && !sweep && !chunked[0] && !chunked[1] && G == 8 && pl.m[0] &&
+                    pl.m[1] && disjoint_passes(pl);
+  if (side) {
+    ensure_aux2(c);
+    hip_check(hipEventRecord(c->ev_fork2, s), "record fork");
+    hip_check(hipStreamWaitEvent(c->aux2, c->ev_fork2, 0), "wait fork");
+    stage_bins(c, pl, 1, c->aux2, &c->ws2);
+    stage_items_count(c, pl, 1, tn, tot, c->aux2, &c->ws2);
+    stage_bins(c, pl, 0, s);
+    stage_items_count(c, pl, 0, tn, tot, s);
+    read_totals(c, tot + T_ITEMS, T_N - T_ITEMS, s);  // waits for stream s only
+    take_items(c, pl, 0, tn);
+    stage_splat(c, pl, 0, tn, s);
+    read_totals(c, tot + T_ITEMS, T_N - T_ITEMS, c->aux2);  // the second stream only
+    take_items(c, pl, 1, tn);
+    stage_splat(c, pl, 1, tn, c->aux2, &c->ws2, nullptr, true, 1);
+    hip_check(hipEventRecord(c->ev_join2, c->aux2), "record join");
+    hip_check(hipStreamWaitEvent(s, c->ev_join2, 0), "wait join");
+  } else if (early) {
     Tuning ta = tn;
     ta.heavy = 0xffffffffu;
     ta.heavy_max = 1;
@@ -1501,6 +1540,17 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
       hip_check(hipEventRecord(c->ev_join2, c->aux2), "record join");
       hip_check(hipStreamWaitEvent(s, c->ev_join2, 0), "wait join");
       launch_add_images(J0.pic, scratch, npix, s);
+    } else if (G == 8 && pl.m[0] && pl.m[1] && disjoint_passes(pl)) {
+      // No view has particles of both modes: the passes write disjoint
+      // images, and the additive pass's splat runs on a second stream (its
+      // own auxiliary streams and temporaries) beside the ordered pass's.
+      ensure_aux2(c);
+      hip_check(hipEventRecord(c->ev_fork2, s), "record fork");
+      hip_check(hipStreamWaitEvent(c->aux2, c->ev_fork2, 0), "wait fork");
+      stage_splat(c, pl, 0, tn, s);
+      stage_splat(c, pl, 1, tn, c->aux2, &c->ws2, nullptr, true, 1);
+      hip_check(hipEventRecord(c->ev_join2, c->aux2), "record join");
+      hip_check(hipStreamWaitEvent(s, c->ev_join2, 0), "wait join");
     } else {
       for (int p = 0; p < 2; ++p)
         if (pl.m[p]) stage_splat(c, pl, p, tn, s);
@@ -1983,18 +2033,15 @@ void splhip_destroy(splhip_ctx *c) {
     if (h) (void)hipHostFree(h);
   for (hipEvent_t e : c->ev_rec)
     if (e) (void)hipEventDestroy(e);
-  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
-  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
-  if (c->aux) (void)hipStreamDestroy(c->aux);
+  for (splhip_ctx::AuxSet &X : c->ax) {
+    for (hipEvent_t e : {X.fork, X.join, X.fork3, X.join3, X.fork4, X.join4})
+      if (e) (void)hipEventDestroy(e);
+    for (hipStream_t t : {X.aux, X.aux3, X.aux4})
+      if (t) (void)hipStreamDestroy(t);
+  }
   if (c->ev_fork2) (void)hipEventDestroy(c->ev_fork2);
   if (c->ev_join2) (void)hipEventDestroy(c->ev_join2);
   if (c->aux2) (void)hipStreamDestroy(c->aux2);
-  if (c->ev_fork3) (void)hipEventDestroy(c->ev_fork3);
-  if (c->ev_join3) (void)hipEventDestroy(c->ev_join3);
-  if (c->ev_fork4) (void)hipEventDestroy(c->ev_fork4);
-  if (c->ev_join4) (void)hipEventDestroy(c->ev_join4);
-  if (c->aux3) (void)hipStreamDestroy(c->aux3);
-  if (c->aux4) (void)hipStreamDestroy(c->aux4);
   c->ws.release();
   if (c->d_exptab) (void)hipFree(c->d_exptab);
   if (c->d_palx) (void)hipFree(c->d_palx);
