@@ -1464,18 +1464,23 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
       bq[0] = qsrc[at_q(kR * kS)];
     }
   }
-  for (uint32_t g = 0; g < G; g += kR * kS) {
+  // whole rounds first (every step applies: one straight-line block per
+  // round, a scheduling region per step, in which the exchange and the
+  // refill sit among the dependent additions), then the last, partial round
+  auto round = [&](uint32_t g, bool whole) {
 #pragma unroll
     for (int r = 0; r < kR; ++r) {
       const int rn = (r + 1) % kR;
       const uint32_t g0 = g + r * kS;
       const Step nxt = step_of(bm[rn], bw[ADD ? 0 : rn], bq[ADD ? 0 : rn]);
-      if (g0 < G) apply(cur);
+      if (whole || g0 < G) apply(cur);
       cur = nxt;
-      // the barrier orders the exchange and sums before the refill (and the
-      // next exchange's LDS writes after these reads) and keeps the load from
-      // sinking to its use a round later, so kR loads stay in flight
+      // the barriers order the exchange and sums before the refill (and the
+      // next exchange's LDS writes after these reads), keep the load from
+      // sinking to its use a round later, so kR loads stay in flight, and
+      // keep the scheduler from moving an exchange next to its use
       asm volatile("" : "+v"(l) : : "memory");
+      __builtin_amdgcn_sched_barrier(0);
       const size_t o = at_load(g + (r + 1) * kS + kR * kS);
       bm[rn] = src[o];
       if (!ADD) {
@@ -1483,7 +1488,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         bq[rn] = qsrc[at_q(g + (r + 1) * kS + kR * kS)];
       }
     }
-  }
+  };
+  uint32_t g = 0;
+  for (; g + kR * kS <= G; g += kR * kS) round(g, true);
+  if (g < G) round(g, false);
   if (a.trace && c == 0 && h == 0 && lane == 0) {
     uint64_t *tr = a.trace + 4 * static_cast<uint64_t>(i);
     tr[0] = t_start;
