@@ -973,12 +973,31 @@ __global__ __launch_bounds__(kPThreads) void k_splat_pc(SplatArgs a, const uint3
         op[j] = in ? make_float4(ca[j].x * at, ca[j].y * at, ca[j].z * at, 0.f)
                    : make_float4(0.f, 0.f, 0.f, 0.f);
       }
-    } else {
-      float at[kQ], g0[kQ], g1[kQ], g2[kQ], pg0[kQ], pg1[kQ], pg2[kQ];
+#pragma unroll
+      for (int j = 0; j < kQ; ++j) s_op[ts][kb + j][lane] = op[j];
+      return;
+    }
+    float at[kQ];
+    bool in[kQ], tay[kQ];
+    uint32_t tb = 0;  // bit j: every pixel of particle j takes the Taylor branch or is outside
+#pragma unroll
+    for (int j = 0; j < kQ; ++j) {
+      at[j] = p1[j] * p2[j];
+      in[j] = (kb + j < cnt) && ((mk[j] >> cy) & 1u);
+      tay[j] = (ca[j].w * at[j]) < tl;
+      if (__ballot(tay[j] || !in[j]) == ~0ull) tb |= 1u << j;
+    }
+    // One straight-line body per pattern: the all-Taylor particles skip the
+    // exp-table work, the others' table reads are all issued before their
+    // arithmetic.  op = (a_c, att) for a Taylor pixel, (expm1(att a_c), 1)
+    // otherwise, zero outside.
+    auto body = [&](auto tb_tag) {
+      constexpr uint32_t TB = decltype(tb_tag)::value;
+      float g0[kQ], g1[kQ], g2[kQ], pg0[kQ], pg1[kQ], pg2[kQ];
       float u0[kQ], w0[kQ], u1[kQ], w1[kQ], u2[kQ], w2[kQ];
 #pragma unroll
       for (int j = 0; j < kQ; ++j) {
-        at[j] = p1[j] * p2[j];
+        if ((TB >> j) & 1u) continue;
         g0[j] = at[j] * ca[j].x;
         g1[j] = at[j] * ca[j].y;
         g2[j] = at[j] * ca[j].z;
@@ -1001,8 +1020,11 @@ __global__ __launch_bounds__(kPThreads) void k_splat_pc(SplatArgs a, const uint3
       }
 #pragma unroll
       for (int j = 0; j < kQ; ++j) {
-        const bool in = (kb + j < cnt) && ((mk[j] >> cy) & 1u);
-        const bool tay = (ca[j].w * at[j]) < tl;
+        if ((TB >> j) & 1u) {
+          op[j] = in[j] ? make_float4(ca[j].x, ca[j].y, ca[j].z, at[j])
+                        : make_float4(0.f, 0.f, 0.f, 0.f);
+          continue;
+        }
         float em0, em1, em2;
         if (FAST) {
           em0 = (fabsf(g0[j]) < tl) ? g0[j] : u0[j] * w0[j] - 1.f;
@@ -1013,13 +1035,22 @@ __global__ __launch_bounds__(kPThreads) void k_splat_pc(SplatArgs a, const uint3
           em1 = xexpm1_finish(g1[j], pg1[j], u1[j], w1[j], tl);
           em2 = xexpm1_finish(g2[j], pg2[j], u2[j], w2[j], tl);
         }
-        op[j] = in ? make_float4(tay ? ca[j].x : em0, tay ? ca[j].y : em1, tay ? ca[j].z : em2,
-                                 tay ? at[j] : 1.f)
-                   : make_float4(0.f, 0.f, 0.f, 0.f);
+        op[j] = in[j] ? make_float4(tay[j] ? ca[j].x : em0, tay[j] ? ca[j].y : em1,
+                                    tay[j] ? ca[j].z : em2, tay[j] ? at[j] : 1.f)
+                      : make_float4(0.f, 0.f, 0.f, 0.f);
       }
-    }
 #pragma unroll
-    for (int j = 0; j < kQ; ++j) s_op[ts][kb + j][lane] = op[j];
+      for (int j = 0; j < kQ; ++j) s_op[ts][kb + j][lane] = op[j];
+    };
+    static_assert(kQ == 4, "one body per pattern of four particles");
+    switch (uniform(tb)) {
+#define SPLHIP_PCT(T) case T: body(std::integral_constant<uint32_t, T>{}); break;
+      SPLHIP_PCT(0) SPLHIP_PCT(1) SPLHIP_PCT(2) SPLHIP_PCT(3)
+      SPLHIP_PCT(4) SPLHIP_PCT(5) SPLHIP_PCT(6) SPLHIP_PCT(7)
+      SPLHIP_PCT(8) SPLHIP_PCT(9) SPLHIP_PCT(10) SPLHIP_PCT(11)
+      SPLHIP_PCT(12) SPLHIP_PCT(13) SPLHIP_PCT(14) SPLHIP_PCT(15)
+#undef SPLHIP_PCT
+    }
   };
   auto operands = [&](int b, int pw) {
     const uint64_t f64 = s_fast[b & 3];
