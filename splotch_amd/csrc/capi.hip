@@ -893,8 +893,10 @@ constexpr uint64_t kStageOrderedCrit = 432;
 // one three-channel walk.
 constexpr uint32_t kSplitRatio = 2048;
 // Staged lists whose chains start right after their own operands (the
-// longest; the others' operands and chains follow beside them).
-constexpr uint32_t kStagePhase1 = 64;
+// longest; the others' operands and chains follow beside them).  The 64-view
+// batch: 34.9 ms with 64, 33.7 with 16, 34.1 with 4; the 1M / 800^2 frame and
+// the 500k ordered one within 1 % of each other.
+constexpr uint32_t kStagePhase1 = 16;
 // Default ordered staging threshold: about the longest list / 16 (the
 // 64-view batch: 40.4 ms staged from 65536 entries on, 43.9 from 8192; 500k
 // at 512^2, longest list 121k: 2.67 ms from 8192, 3.39 from 65536).
