@@ -5,7 +5,7 @@ files, a vector<particle_sim> — beside the genuine host_rendering on the same
 objects.  Additive frames are bit-exact; ordered frames are bit-exact against
 the oracle in the stable (z, index) depth order, and their difference from
 the genuine std::sort order (equal depths) is stated.  The records handed to
-hip_rendering come back unmodified (checked by the driver).
+hip_rendering come back unmodified (checked in oracle/shim_driver.cpp).
 """
 import math
 import sys
