@@ -616,18 +616,30 @@ __global__ __launch_bounds__(256) void k_split_scatter(const uint32_t *keys, con
     uint32_t n[NB];
 #pragma unroll
     for (int b = 0; b < NB; ++b) n[b] = cpre[static_cast<int64_t>(b) * nchunks + c];
-    for (uint32_t i = i0; i < i1; i += 64) {
-      const bool in = i + lane < i1;
-      const uint32_t k = in ? keys[i + lane] : 0u;
-      const uint32_t idx = in ? vals[i + lane] : 0u;
-      const uint32_t mk = k >> rbits;
-      const uint32_t *bs = base + static_cast<int64_t>(NB) * (k & rmask);
+    // four batches of 64 entries at a time, block by block: a block's four
+    // stores are one contiguous run of its list, issued back to back, so
+    // they reach L2 as whole lines rather than 16 interleaved partial ones
+    for (uint32_t i = i0; i < i1; i += 4 * 64) {
+      uint32_t mk[4], idx[4];
+      const uint32_t *bs[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t t = i + 64u * j + lane;
+        const bool in = t < i1;
+        const uint32_t k = in ? keys[t] : 0u;
+        idx[j] = in ? vals[t] : 0u;
+        mk[j] = in ? k >> rbits : 0u;  // no block for entries past the chunk
+        bs[j] = base + static_cast<int64_t>(NB) * (k & rmask);
+      }
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
-        const bool on = (mk >> b) & 1u;
-        const uint64_t bal = __ballot(on);
-        if (on) ent[bs[b] + n[b] + mbcnt64(bal)] = idx;
-        n[b] += static_cast<uint32_t>(__popcll(bal));
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const bool on = (mk[j] >> b) & 1u;
+          const uint64_t bal = __ballot(on);
+          if (on) ent[bs[j][b] + n[b] + mbcnt64(bal)] = idx[j];
+          n[b] += static_cast<uint32_t>(__popcll(bal));
+        }
       }
     }
   }
