@@ -97,6 +97,9 @@ struct splhip_ctx {
   // ordered pass's, with its own temporaries
   hipStream_t aux2 = nullptr;
   hipEvent_t ev_fork2 = nullptr, ev_join2 = nullptr;
+  // recorded by the ordered splat once its long-list workgroups have been
+  // launched and given their head start (see stage_splat)
+  hipEvent_t ev_placed = nullptr;
   Workspace ws2;
   // block-file ingest: two pinned buffers, each freed for reuse by its event
   char *h_ing[2] = {nullptr, nullptr};
@@ -304,6 +307,7 @@ void ensure_aux2(splhip_ctx *c) {
   hip_check(hipStreamCreateWithFlags(&c->aux2, hipStreamNonBlocking), "hipStreamCreate");
   hip_check(hipEventCreateWithFlags(&c->ev_fork2, hipEventDisableTiming), "hipEventCreate");
   hip_check(hipEventCreateWithFlags(&c->ev_join2, hipEventDisableTiming), "hipEventCreate");
+  hip_check(hipEventCreateWithFlags(&c->ev_placed, hipEventDisableTiming), "hipEventCreate");
 }
 
 void ensure_aux3(splhip_ctx *c, int k = 0) {
@@ -1020,10 +1024,11 @@ void take_items(splhip_ctx *c, Plan &pl, int p, const Tuning &tn) {
 // Splat of pass p: items heaviest first; heavy units on the auxiliary stream.
 // wt: workspace of the temporaries (the item sort); jobs: a job table in
 // place of the plan's; fork: the long-list workgroups run on the auxiliary
-// stream beside splat8 (else on s, first).
+// stream beside splat8 (else on s, first); placed: recorded on s just before
+// the 8x8 splat's launch, after the long-list workgroups' head start.
 void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s,
                  Workspace *wt = nullptr, JobSplat *jobs = nullptr, bool fork = true,
-                 int set = 0) {
+                 int set = 0, hipEvent_t placed = nullptr) {
   Workspace &ws = *pl.ws;
   splhip_ctx::AuxSet &X = c->ax[set];
   SplatArgs sa0 = splat_args(c, pl, p, tn);
@@ -1143,6 +1148,7 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
     // The normal items fill every CU; the heavy workgroups (the critical
     // path) must get their slots first, so the normal launch waits briefly.
     launch_delay(40, s);
+    if (placed) hip_check(hipEventRecord(placed, s), "record placed");
     mark(c, p == 0 ? 7 : 9, true, s);
     launch_splat8(p == 1, tn.variant, nit, nn, sn, s, dc, tn.ppw, pl.nsplit[p]);
     mark(c, p == 0 ? 7 : 9, false, s);
@@ -1153,10 +1159,12 @@ void stage_splat(splhip_ctx *c, Plan &pl, int p, const Tuning &tn, hipStream_t s
     mark(c, p == 0 ? 8 : 10, true, s);
     launch_splat_pc(p == 1, pit, npc, sp, s, dc);
     mark(c, p == 0 ? 8 : 10, false, s);
+    if (placed) hip_check(hipEventRecord(placed, s), "record placed");
     mark(c, p == 0 ? 7 : 9, true, s);
     launch_splat8(p == 1, tn.variant, nit, nn, sn, s, dc, tn.ppw, pl.nsplit[p]);
     mark(c, p == 0 ? 7 : 9, false, s);
   } else {
+    if (placed) hip_check(hipEventRecord(placed, s), "record placed");
     mark(c, p == 0 ? 7 : 9, true, s);
     launch_splat8(p == 1, tn.variant, items, nitems, sn, s, dc, tn.ppw, pl.nsplit[p]);
     mark(c, p == 0 ? 7 : 9, false, s);
@@ -1424,13 +1432,13 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
   }
   bool tuned = false;
   // A single view with both passes (no autotuning): the additive pass is
-  // binned and splatted on a second stream, into a cleared scratch image,
-  // while the ordered pass bins (memory-bound) and splats; the
-  // scratch image is then added to the image: pic + l per pixel, the
-  // additive pass's own update (pixels it does not touch add +0, which
+  // binned on a second stream while the ordered pass bins (memory-bound),
+  // and splatted there, into a cleared scratch image, while the ordered pass
+  // splats; the scratch image is then added to the image: pic + l per pixel,
+  // the additive pass's own update (pixels it does not touch add +0, which
   // leaves any value but -0 as it is; the ordered pass never writes -0).
   // Its longest lists stay in the 8x8 kernel: producer/consumer workgroups
-  // would hold whole CUs while the ordered pass bins.
+  // would hold whole CUs beside the ordered pass's.
   const bool sweep = !force && G == 8 && jobs[0].prm->hip_autotune;
   const bool early = !chunked[0] && !chunked[1] && nj == 1 && G == 8 && pl.m[0] && pl.m[1] &&
                      pl.js[0].accumulate[1] == 1 && !sweep;
@@ -1488,11 +1496,17 @@ int render_jobs(splhip_ctx *c, int nj, const JobIn *jobs, hipStream_t s, bool ke
               "upload scratch job");
     staged(c, c->aux2);
     launch_fill(scratch, 0.f, npix, c->aux2);
-    stage_splat(c, pl, 1, ta, c->aux2, &c->ws2, d_alt, false);
-    hip_check(hipEventRecord(c->ev_join2, c->aux2), "record join");
     read_totals(c, tot + T_ITEMS, T_N - T_ITEMS, s);  // waits for stream s only
     take_items(c, pl, 0, tn);
-    stage_splat(c, pl, 0, tn, s);
+    // The ordered splat first: a producer/consumer workgroup needs a whole
+    // CU, which it gets at once only before the 8x8 grids fill every CU, so
+    // the additive splat (a full grid) waits for those workgroups' head
+    // start.  Launched before them, it won that race or lost it depending on
+    // launch timing (33 or 55 ms for the 16M frame's longest list).
+    stage_splat(c, pl, 0, tn, s, nullptr, nullptr, true, 0, c->ev_placed);
+    hip_check(hipStreamWaitEvent(c->aux2, c->ev_placed, 0), "wait placed");
+    stage_splat(c, pl, 1, ta, c->aux2, &c->ws2, d_alt, false);
+    hip_check(hipEventRecord(c->ev_join2, c->aux2), "record join");
     hip_check(hipStreamWaitEvent(s, c->ev_join2, 0), "wait join");
     launch_add_images(J0.pic, scratch, npix, s);
   } else if ((pl.m[0] || pl.m[1]) && !chunked[0] && !chunked[1]) {
@@ -2043,6 +2057,7 @@ void splhip_destroy(splhip_ctx *c) {
   }
   if (c->ev_fork2) (void)hipEventDestroy(c->ev_fork2);
   if (c->ev_join2) (void)hipEventDestroy(c->ev_join2);
+  if (c->ev_placed) (void)hipEventDestroy(c->ev_placed);
   if (c->aux2) (void)hipStreamDestroy(c->aux2);
   c->ws.release();
   if (c->d_exptab) (void)hipFree(c->d_exptab);
